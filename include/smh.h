@@ -1,0 +1,129 @@
+/*
+ * smh: MI355X (gfx950) kernels for smuthi's lookup-table particle-coupling
+ * solve and plane-wave-expansion near field.
+ *
+ * Plain C ABI: opaque handles, plain pointers and sizes, no framework types.
+ * Complex numbers cross the boundary as interleaved (re, im) doubles, i.e.
+ * numpy complex128 memory. System vectors use smuthi's layout: particle after
+ * particle, each particle's blocksize(l_max_i, m_max_i) coefficients in the
+ * (tau, l, m) order of smuthi.fields.multi_to_single_index.
+ *
+ * Every call returns 0 on success or one of the SMH_ERR_* codes; the message
+ * of the last failure on a context is available from smh_last_error. Calls on
+ * one context (and the operators created from it) share one HIP stream, are
+ * synchronous and are not reentrant.
+ */
+#ifndef SMH_H
+#define SMH_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SMH_OK 0
+#define SMH_ERR_ARG 1         /* invalid argument or size */
+#define SMH_ERR_HIP 2         /* HIP runtime error */
+#define SMH_ERR_UNSUPPORTED 3 /* configuration the library has no kernel for */
+#define SMH_ERR_STATE 4       /* call out of order (e.g. matvec before a table) */
+#define SMH_ERR_NOCONV 5      /* solver stopped before reaching the tolerance */
+
+#define SMH_INTERP_LINEAR 0      /* linear interpolation (smuthi 'linear', CPU and GPU) */
+#define SMH_INTERP_CATMULL_ROM 1 /* 4-tap cubic Hermite, as smuthi's CUDA 'cubic' kernel */
+#define SMH_INTERP_SPLINE 2      /* not-a-knot cubic spline, as smuthi's CPU 'cubic' (scipy interp1d) */
+
+#define SMH_FP32 0 /* FP32 complex arithmetic, FP32 accumulation */
+#define SMH_FP64 1 /* FP64 complex arithmetic */
+
+#define SMH_SOLVE_FP64 0  /* restarted GMRES, FP64 matvec, FP64 Arnoldi */
+#define SMH_SOLVE_MIXED 1 /* FP64 iterative refinement around FP32-matvec GMRES */
+
+typedef struct smh_ctx smh_ctx;
+typedef struct smh_op smh_op;
+
+/* ---- runtime ----------------------------------------------------------
+ * Replaces smuthi.utility.cuda.enable_gpu's device probe (utility/cuda.py:10-68). */
+int smh_init(int device, smh_ctx **out);
+int smh_destroy(smh_ctx *ctx);
+const char *smh_last_error(const smh_ctx *ctx);
+int smh_device_info(smh_ctx *ctx, char *name, int name_len, int *num_cu);
+
+/* ---- radial lookup coupling operator -----------------------------------
+ * Replaces CouplingMatrixRadialLookupCUDA.__init__ (linearsystem/linear_system.py:826-886):
+ * particle coordinates and multipole cut-offs, table grid
+ * rho_j = rho_min + j * resolution (j < len_rho), interpolation kind.
+ * xy: [n_part][2] doubles (all particles share one z). The call bins every
+ * particle pair by distance for the matvec. */
+int smh_radial_create(smh_ctx *ctx, int64_t n_part, const double *xy, const int *lmax, const int *mmax,
+                      int l_max, int m_max, double rho_min, double resolution, int64_t len_rho, int interp,
+                      smh_op **out);
+
+/* Volume lookup operator (CouplingMatrixVolumeLookupCUDA, linear_system.py:702-780):
+ * xyz: [n_part][3]; tables over (rho, z1+z2) and (rho, z1-z2) with the same
+ * resolution; len_z entries from sz_min / dz_min. interp: LINEAR or CATMULL_ROM. */
+int smh_volume_create(smh_ctx *ctx, int64_t n_part, const double *xyz, const int *lmax, const int *mmax,
+                      int l_max, int m_max, double rho_min, double sz_min, double dz_min, double resolution,
+                      int64_t len_rho, int64_t len_z, int interp, smh_op **out);
+
+int smh_op_destroy(smh_op *op);
+
+/* Upload a host table (prepare_lookup.py:253-464 / :16-250 output):
+ * radial: complex128 [len_rho][B][B]; volume: two complex128 [len_rho][len_z][B][B]
+ * (plus, minus). B = blocksize(l_max, m_max). */
+int smh_op_upload_table(smh_op *op, const double *table, const double *table_minus);
+
+/* Build the radial table on the GPU in FP64 (replaces radial_coupling_lookup_table,
+ * prepare_lookup.py:253-464, and its helper kernel, prepare_lookup_cuda.py:55-95):
+ *   L[j,n1,n2] = c64(direct[j,n1,n2]) + sum_k J_|m1-m2|(kappa_k rho_j) * lm_coef[k,n1,n2]
+ *   direct[j,n1,n2] = sum_ld direct_coef[n1,n2,ld] * h_ld(k_medium rho_j)   if rho_j >= mask_radius, else 0
+ * direct_coef: complex128 [B][B][2 l_max + 1]; kappa: complex128 [n_kappa];
+ * lm_coef: complex128 [n_kappa][B][B] (trapezoid weights, Jacobian and the
+ * layer response already folded in) or NULL for a degenerate layer system. */
+int smh_radial_build_table(smh_op *op, double k_re, double k_im, double mask_radius, const double *direct_coef,
+                           int64_t n_kappa, const double *kappa, const double *lm_coef);
+
+/* Copy the FP64 table back to the host (the .lookup_table attribute). */
+int smh_op_download_table(smh_op *op, double *table, double *table_minus);
+
+/* y = W x for nrhs vectors. x, y: host complex128 [nrhs][n_unk] in smuthi's
+ * system-vector layout (replaces the CUDA matvec closure, linear_system.py:888-898). */
+int smh_op_matvec(smh_op *op, const double *x, double *y, int64_t nrhs, int precision);
+
+/* Time `iters` matvecs on device-resident synthetic vectors; returns the
+ * average time of the coupling kernel (HIP events on the operator's stream)
+ * and of the whole matvec in milliseconds. */
+int smh_op_bench_matvec(smh_op *op, int precision, int iters, double *kernel_ms, double *total_ms);
+
+/* Solve (1 - T W) b = T a (LinearSystem.solve with solver_type='gmres',
+ * linear_system.py:254-344) for nrhs right-hand sides.
+ * t_diag: complex128 [n_unk] diagonal sphere T-matrix; rhs, x: complex128
+ * [nrhs][n_unk]; x0 = rhs as in smuthi. Stops at ||rhs - M x|| <= tol ||rhs||
+ * measured in FP64. stats (out, 8 doubles): iterations, FP64 matvecs, FP32
+ * matvecs, worst relative residual, solve seconds, refinement sweeps, 0, 0. */
+int smh_solve(smh_op *op, const double *t_diag, const double *rhs, double *x, int64_t nrhs, double tol,
+              int restart, int maxiter, int mode, double *stats);
+
+/* ---- plane wave expansion fields ---------------------------------------
+ * Replaces PlaneWaveExpansion.electric_field / magnetic_field on the GPU
+ * (fields/expansions.py:635-782, kernels fields/expansions_cuda.py:5-421).
+ * kind 0: E, 1: H*omega (smuthi multiplies by 1/omega on the host).
+ * kp, kz: complex128 [n_k]; alpha: double [n_a]; g: complex128 [2][n_k][n_a];
+ * xyz: [n_pts][3] positions relative to the expansion's reference point;
+ * out: complex128 [3][n_pts]. Nested trapezoid rule, alpha inner, kappa outer. */
+int smh_pwe_field(smh_ctx *ctx, int kind, int64_t n_k, const double *kp, const double *kz, int64_t n_a,
+                  const double *alpha, const double *g, double k_re, double k_im, int64_t n_pts,
+                  const double *xyz, double *out, int precision, double *kernel_ms);
+
+/* ---- host evaluation of the device special functions -------------------
+ * Runs the same code the table kernels use, on the CPU, so it can be checked
+ * without a device: out[j][n] = J_n(z_j) (complex128 [count][nmax+1]) and
+ * out[j][n] = h_n^(1)(z_j) spherical Hankel. z: complex128 [count]. */
+int smh_host_bessel_j(int nmax, int64_t count, const double *z, double *out);
+int smh_host_spherical_hankel(int nmax, int64_t count, const double *z, double *out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SMH_H */

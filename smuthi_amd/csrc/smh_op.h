@@ -1,0 +1,64 @@
+// Coupling operator state shared by the operator, table and solver sources.
+#pragma once
+
+#include "smh_common.h"
+
+struct smh_op {
+  smh_ctx* ctx = nullptr;
+  int kind = 0;  // 0 radial, 1 volume
+  int64_t n_part = 0, n_unk = 0;
+  int l_max = 0, m_max = 0, B = 0, Bh = 0, MP = 0;
+  int interp = 0, T = 0, s_mode = 1, o_mode = 0;
+  double rho_min = 0, h = 0;
+  int64_t len_rho = 0, n_tt = 0;
+  // volume operators
+  double sz_min = 0, dz_min = 0;
+  int64_t len_z = 0;
+
+  // particles and index maps
+  double2* d_xy = nullptr;    // radial: (x, y)
+  double* d_xyz = nullptr;    // volume: [n_part][3]
+  int* d_pk2pad = nullptr;    // packed system index -> padded index i*B + n_global
+  std::vector<int> pk2pad;
+
+  // tables
+  bool has_table = false;
+  double2* d_table = nullptr;        // FP64 table [len_rho][(len_z)][B][B]
+  double2* d_table_minus = nullptr;  // volume: minus table
+  double* d_tt64 = nullptr;          // tap rows [n_tt][MP][B]
+  float* d_tt32 = nullptr;
+
+  // pair binning
+  int Rc = 128, n_chunks = 0, n_items = 0;
+  int64_t n_pairs = 0;
+  uint2* d_pairs = nullptr;
+  int* d_off = nullptr;
+  int4* d_items = nullptr;
+  int* d_chunk_items = nullptr;
+  int S32 = 1, S64 = 1, rows_cap32 = 0, rows_cap64 = 0;
+  size_t lds32 = 0, lds64 = 0;
+
+  // work buffers
+  void* d_ypart = nullptr;
+  size_t ypart_bytes = 0;
+  double2* d_pk_in = nullptr;
+  double2* d_pk_out = nullptr;
+  int64_t pk_cap = 0;  // capacity in complex entries
+  float2* d_x32 = nullptr;
+  float2* d_y32 = nullptr;
+  double2* d_x64 = nullptr;
+  double2* d_y64 = nullptr;
+  float last_kernel_ms = 0.f;
+};
+
+namespace smh {
+// Launch one coupling matvec on padded device vectors (x -> y, [n_part][B]).
+int op_apply(smh_op* op, int precision, const void* x_pad, void* y_pad, bool time_kernel);
+// Ensure packed host staging buffers for nrhs vectors.
+int op_reserve_packed(smh_op* op, int64_t nrhs);
+// packed (smuthi layout) <-> padded conversions on device
+int op_unpack(smh_op* op, int precision, const double2* pk, void* pad);
+int op_pack(smh_op* op, int precision, const void* pad, double2* pk);
+// derive the tap rows (and spline moments) from the FP64 table
+int op_make_taps(smh_op* op);
+}  // namespace smh

@@ -1,0 +1,670 @@
+// Radial-lookup coupling operator: pair binning, tap tables, matvec driver.
+#include <algorithm>
+#include <cmath>
+
+#include "smh_op.h"
+#include "smh_radial_kernel.h"
+
+using namespace smh;
+
+namespace {
+
+constexpr int kThreads = 512;
+
+__global__ void pair_bin_kernel(const double2* __restrict__ xy, int n, double rho_min, double h, int len_rho,
+                                int s_mode, int o_mode, int T, int64_t n_tt, int Rc, int* cnt, const int* off,
+                                int* cursor, uint2* pairs, int* err) {
+  const int nb2 = (n + 255) / 256;
+  const int64_t b = blockIdx.x;
+  const int i1 = (int)(b / nb2);
+  const int i2 = (int)(b % nb2) * 256 + threadIdx.x;
+  if (i1 >= n || i2 >= n) return;
+  const double dx = xy[i1].x - xy[i2].x, dy = xy[i1].y - xy[i2].y;
+  int r;
+  double w, rho;
+  pair_geometry(dx, dy, rho_min, h, r, w, rho);
+  const int64_t row0 = (int64_t)s_mode * r + o_mode;
+  if (r < 0 || r >= len_rho - 1 || row0 < 0 || row0 + T > n_tt) {
+    atomicOr(err, 1);
+    return;
+  }
+  const int64_t key = (int64_t)(i1 / Rc) * len_rho + r;
+  if (pairs == nullptr) {
+    atomicAdd(&cnt[key], 1);
+  } else {
+    const int pos = off[key] + atomicAdd(&cursor[key], 1);
+    pairs[pos] = make_uint2((unsigned)i1, (unsigned)i2);
+  }
+}
+
+template <typename Real>
+__global__ void scatter_kernel(const double2* __restrict__ pk, const int* __restrict__ pk2pad, int64_t n_unk,
+                               typename Vec2<Real>::t* pad) {
+  const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j < n_unk) {
+    const double2 v = pk[j];
+    pad[pk2pad[j]] = typename Vec2<Real>::t{Real(v.x), Real(v.y)};
+  }
+}
+
+template <typename Real>
+__global__ void pack_kernel(const typename Vec2<Real>::t* __restrict__ pad, const int* __restrict__ pk2pad,
+                            int64_t n_unk, double2* pk) {
+  const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j < n_unk) {
+    const typename Vec2<Real>::t v = pad[pk2pad[j]];
+    pk[j] = make_double2((double)v.x, (double)v.y);
+  }
+}
+
+// Sum the work-item accumulators of each receiver chunk in fixed order.
+template <typename Real>
+__global__ void reduce_kernel(const Real* __restrict__ ypart, const int* __restrict__ chunk_items, int Rc, int Bh,
+                              int B, int64_t n_part, typename Vec2<Real>::t* y) {
+  const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n_part * B) return;
+  const int64_t i1 = j / B;
+  const int n1 = (int)(j - i1 * B);
+  const int c = (int)(i1 / Rc), il = (int)(i1 - (int64_t)c * Rc);
+  Real re = 0, im = 0;
+  for (int w = chunk_items[c]; w < chunk_items[c + 1]; ++w) {
+    const Real* p = ypart + ((size_t)w * Rc + il) * 2 * Bh;
+    re += p[n1];
+    im += p[Bh + n1];
+  }
+  y[j] = typename Vec2<Real>::t{re, im};
+}
+
+// Tap rows [n_tt][MP][B]: Re rows 0..B-1, Im rows Bh..Bh+B-1, zero elsewhere.
+// Spline taps interleave values and moments (row 2j: y_j, row 2j+1: Mh_j).
+__global__ void taps_kernel(const double2* __restrict__ table, const double2* __restrict__ mh, int64_t len, int B,
+                            int Bh, int MP, int interleave, double* tt64, float* tt32) {
+  const int64_t n_tt = interleave ? 2 * len : len;
+  const int64_t total = n_tt * MP * B;
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t row = e / ((int64_t)MP * B);
+    const int rem = (int)(e - row * MP * B);
+    const int m = rem / B, n2 = rem - m * B;
+    const double2* src = table;
+    int64_t j = row;
+    if (interleave) {
+      j = row >> 1;
+      if (row & 1) src = mh;
+    }
+    double v = 0.0;
+    if (m < B)
+      v = src[(j * B + m) * B + n2].x;
+    else if (m >= Bh && m < Bh + B)
+      v = src[(j * B + (m - Bh)) * B + n2].y;
+    tt64[e] = v;
+    tt32[e] = (float)v;
+  }
+}
+
+// Not-a-knot cubic spline moments, scaled: Mh_j = h^2/6 S''(rho_j), one
+// thread per complex column (n1, n2). With uniform spacing the system is
+//   Mh_{j-1} + 4 Mh_j + Mh_{j+1} = y_{j-1} - 2 y_j + y_{j+1},  j = 1..n-2,
+// closed by Mh_0 - 2 Mh_1 + Mh_2 = 0 and Mh_{n-3} - 2 Mh_{n-2} + Mh_{n-1} = 0,
+// which gives Mh_1 = rhs_1 / 6, Mh_{n-2} = rhs_{n-2} / 6 and a tridiagonal
+// system for Mh_2..Mh_{n-3} (Thomas algorithm, c' precomputed on the host).
+__global__ void spline_kernel(const double2* __restrict__ y, const double* __restrict__ cprime, int64_t n, int BB,
+                              double2* mh) {
+  const int col = blockIdx.x * blockDim.x + threadIdx.x;
+  if (col >= BB) return;
+  auto Y = [&](int64_t j) { return y[j * BB + col]; };
+  auto rhs = [&](int64_t j) {
+    const double2 a = Y(j - 1), b = Y(j), c = Y(j + 1);
+    return make_double2(a.x - 2.0 * b.x + c.x, a.y - 2.0 * b.y + c.y);
+  };
+  const double2 r1 = rhs(1), rn2 = rhs(n - 2);
+  const double2 m1 = make_double2(r1.x / 6.0, r1.y / 6.0);
+  const double2 mn2 = make_double2(rn2.x / 6.0, rn2.y / 6.0);
+  // forward sweep over j = 2..n-3, d' stored in mh[j]
+  double2 dprev = make_double2(0, 0);
+  for (int64_t j = 2; j <= n - 3; ++j) {
+    double2 d = rhs(j);
+    if (j == 2) {
+      d.x -= m1.x;
+      d.y -= m1.y;
+    }
+    if (j == n - 3) {
+      d.x -= mn2.x;
+      d.y -= mn2.y;
+    }
+    const double den = (j == 2) ? 4.0 : 4.0 - cprime[j - 1];
+    const double2 dp = (j == 2) ? make_double2(d.x / 4.0, d.y / 4.0)
+                                : make_double2((d.x - dprev.x) / den, (d.y - dprev.y) / den);
+    mh[j * BB + col] = dp;
+    dprev = dp;
+  }
+  double2 next = mh[(n - 3) * BB + col];
+  for (int64_t j = n - 4; j >= 2; --j) {
+    const double2 dp = mh[j * BB + col];
+    const double2 v = make_double2(dp.x - cprime[j] * next.x, dp.y - cprime[j] * next.y);
+    mh[j * BB + col] = v;
+    next = v;
+  }
+  const double2 m2 = mh[2 * BB + col], mn3 = mh[(n - 3) * BB + col];
+  mh[1 * BB + col] = m1;
+  mh[(n - 2) * BB + col] = mn2;
+  mh[0 * BB + col] = make_double2(2.0 * m1.x - m2.x, 2.0 * m1.y - m2.y);
+  mh[(n - 1) * BB + col] = make_double2(2.0 * mn2.x - mn3.x, 2.0 * mn2.y - mn3.y);
+}
+
+template <int L, int Mx, int T, typename Real>
+void* kernel_ptr() {
+  return reinterpret_cast<void*>(&radial_matvec_kernel<Real, L, Mx, T>);
+}
+
+typedef void* (*KernelGetter)();
+
+template <int T, typename Real>
+void* pick(int l, int m) {
+#define SMH_K(L, Mx) \
+  if (l == L && m == Mx) return kernel_ptr<L, Mx, T, Real>();
+  SMH_K(1, 1)
+  SMH_K(2, 1)
+  SMH_K(2, 2)
+  SMH_K(3, 1)
+  SMH_K(3, 2)
+  SMH_K(3, 3)
+  SMH_K(4, 1)
+  SMH_K(4, 2)
+  SMH_K(4, 3)
+  SMH_K(4, 4)
+#undef SMH_K
+  return nullptr;
+}
+
+void* radial_kernel(int precision, int T, int l, int m) {
+  if (precision == SMH_FP32) return T == 4 ? pick<4, float>(l, m) : pick<2, float>(l, m);
+  return T == 4 ? pick<4, double>(l, m) : pick<2, double>(l, m);
+}
+
+int blocks_for(int64_t n, int t = 256) { return (int)((n + t - 1) / t); }
+
+}  // namespace
+
+namespace smh {
+
+int op_reserve_packed(smh_op* op, int64_t nrhs) {
+  const int64_t need = op->n_unk * nrhs;
+  if (need <= op->pk_cap) return SMH_OK;
+  if (op->d_pk_in) hipFree(op->d_pk_in);
+  if (op->d_pk_out) hipFree(op->d_pk_out);
+  op->d_pk_in = op->d_pk_out = nullptr;
+  SMH_HIP(op->ctx, hipMalloc(&op->d_pk_in, need * sizeof(double2)));
+  SMH_HIP(op->ctx, hipMalloc(&op->d_pk_out, need * sizeof(double2)));
+  op->pk_cap = need;
+  return SMH_OK;
+}
+
+int op_unpack(smh_op* op, int precision, const double2* pk, void* pad) {
+  hipStream_t s = op->ctx->stream;
+  const int64_t n_pad = op->n_part * op->B;
+  if (precision == SMH_FP32) {
+    SMH_HIP(op->ctx, hipMemsetAsync(pad, 0, n_pad * sizeof(float2), s));
+    hipLaunchKernelGGL(scatter_kernel<float>, dim3(blocks_for(op->n_unk)), dim3(256), 0, s, pk, op->d_pk2pad,
+                       op->n_unk, (float2*)pad);
+  } else {
+    SMH_HIP(op->ctx, hipMemsetAsync(pad, 0, n_pad * sizeof(double2), s));
+    hipLaunchKernelGGL(scatter_kernel<double>, dim3(blocks_for(op->n_unk)), dim3(256), 0, s, pk, op->d_pk2pad,
+                       op->n_unk, (double2*)pad);
+  }
+  SMH_HIP(op->ctx, hipGetLastError());
+  return SMH_OK;
+}
+
+int op_pack(smh_op* op, int precision, const void* pad, double2* pk) {
+  hipStream_t s = op->ctx->stream;
+  if (precision == SMH_FP32)
+    hipLaunchKernelGGL(pack_kernel<float>, dim3(blocks_for(op->n_unk)), dim3(256), 0, s, (const float2*)pad,
+                       op->d_pk2pad, op->n_unk, pk);
+  else
+    hipLaunchKernelGGL(pack_kernel<double>, dim3(blocks_for(op->n_unk)), dim3(256), 0, s, (const double2*)pad,
+                       op->d_pk2pad, op->n_unk, pk);
+  SMH_HIP(op->ctx, hipGetLastError());
+  return SMH_OK;
+}
+
+int op_make_taps(smh_op* op) {
+  smh_ctx* ctx = op->ctx;
+  hipStream_t s = ctx->stream;
+  const int64_t len = op->len_rho;
+  const int BB = op->B * op->B;
+  double2* d_mh = nullptr;
+  double* d_cp = nullptr;
+  if (op->interp == SMH_INTERP_SPLINE) {
+    SMH_CHECK(ctx, len >= 7, SMH_ERR_ARG, "spline interpolation needs at least 7 table rows (got %lld)",
+              (long long)len);
+    std::vector<double> cp(len, 0.0);
+    for (int64_t j = 2; j <= len - 3; ++j) cp[j] = (j == 2) ? 0.25 : 1.0 / (4.0 - cp[j - 1]);
+    SMH_HIP(ctx, hipMalloc(&d_mh, len * BB * sizeof(double2)));
+    SMH_HIP(ctx, hipMalloc(&d_cp, len * sizeof(double)));
+    SMH_HIP(ctx, hipMemcpyAsync(d_cp, cp.data(), len * sizeof(double), hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL(spline_kernel, dim3(blocks_for(BB, 64)), dim3(64), 0, s, op->d_table, d_cp, len, BB, d_mh);
+    SMH_HIP(ctx, hipGetLastError());
+  }
+  if (!op->d_tt64) {
+    SMH_HIP(ctx, hipMalloc(&op->d_tt64, (size_t)op->n_tt * op->MP * op->B * sizeof(double)));
+    SMH_HIP(ctx, hipMalloc(&op->d_tt32, (size_t)op->n_tt * op->MP * op->B * sizeof(float)));
+  }
+  hipLaunchKernelGGL(taps_kernel, dim3(4096), dim3(256), 0, s, op->d_table, d_mh, len, op->B, op->Bh, op->MP,
+                     op->interp == SMH_INTERP_SPLINE ? 1 : 0, op->d_tt64, op->d_tt32);
+  SMH_HIP(ctx, hipGetLastError());
+  SMH_HIP(ctx, hipStreamSynchronize(s));
+  if (d_mh) hipFree(d_mh);
+  if (d_cp) hipFree(d_cp);
+  op->has_table = true;
+  return SMH_OK;
+}
+
+int op_apply(smh_op* op, int precision, const void* x_pad, void* y_pad, bool time_kernel) {
+  smh_ctx* ctx = op->ctx;
+  SMH_CHECK(ctx, op->has_table, SMH_ERR_STATE, "coupling table not set");
+  SMH_CHECK(ctx, op->kind == 0, SMH_ERR_UNSUPPORTED, "operator kind %d has no MFMA matvec here", op->kind);
+  const int S = precision == SMH_FP32 ? op->S32 : op->S64;
+  SMH_CHECK(ctx, S >= 1, SMH_ERR_UNSUPPORTED,
+            "no %s kernel fits LDS for blocksize %d (l_max %d, m_max %d)", precision == SMH_FP32 ? "FP32" : "FP64",
+            op->B, op->l_max, op->m_max);
+  void* k = radial_kernel(precision, op->T, op->l_max, op->m_max);
+  SMH_CHECK(ctx, k != nullptr, SMH_ERR_UNSUPPORTED, "no matvec kernel for l_max %d m_max %d", op->l_max,
+            op->m_max);
+  RadialKArgs a;
+  a.xy = op->d_xy;
+  a.pairs = op->d_pairs;
+  a.off = op->d_off;
+  a.items = op->d_items;
+  a.tt = precision == SMH_FP32 ? (const void*)op->d_tt32 : (const void*)op->d_tt64;
+  a.x = x_pad;
+  a.ypart = op->d_ypart;
+  a.rho_min = op->rho_min;
+  a.h = op->h;
+  a.len_rho = (int)op->len_rho;
+  a.n_tt = (int)op->n_tt;
+  a.s_mode = op->s_mode;
+  a.o_mode = op->o_mode;
+  a.interp = op->interp;
+  a.S = S;
+  a.Rc = op->Rc;
+  a.rows_cap = precision == SMH_FP32 ? op->rows_cap32 : op->rows_cap64;
+  const size_t lds = precision == SMH_FP32 ? op->lds32 : op->lds64;
+  hipStream_t s = ctx->stream;
+  SMH_HIP(ctx, hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  if (op->n_items > 0) {
+    if (time_kernel) SMH_HIP(ctx, hipEventRecord(ctx->ev0, s));
+    void* args[] = {&a};
+    SMH_HIP(ctx, hipLaunchKernel(k, dim3(op->n_items), dim3(kThreads), args, lds, s));
+    if (time_kernel) SMH_HIP(ctx, hipEventRecord(ctx->ev1, s));
+  }
+  const int64_t n = op->n_part * op->B;
+  if (precision == SMH_FP32)
+    hipLaunchKernelGGL(reduce_kernel<float>, dim3(blocks_for(n)), dim3(256), 0, s, (const float*)op->d_ypart,
+                       op->d_chunk_items, op->Rc, op->Bh, op->B, op->n_part, (float2*)y_pad);
+  else
+    hipLaunchKernelGGL(reduce_kernel<double>, dim3(blocks_for(n)), dim3(256), 0, s, (const double*)op->d_ypart,
+                       op->d_chunk_items, op->Rc, op->Bh, op->B, op->n_part, (double2*)y_pad);
+  SMH_HIP(ctx, hipGetLastError());
+  if (time_kernel && op->n_items > 0) {
+    SMH_HIP(ctx, hipEventSynchronize(ctx->ev1));
+    float ms = 0;
+    SMH_HIP(ctx, hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+    op->last_kernel_ms = ms;
+  }
+  return SMH_OK;
+}
+
+}  // namespace smh
+
+static int lds_plan(smh_op* op, size_t real_size, int& rows_cap, int& S, size_t& lds) {
+  const size_t limit = op->ctx->lds_per_block ? op->ctx->lds_per_block : 65536;
+  const size_t yacc = (size_t)op->Rc * 2 * op->Bh * real_size;
+  const size_t row = (size_t)op->MP * (op->B + 1) * real_size;
+  const size_t misc = 64 * sizeof(int) + 64;
+  rows_cap = 0;
+  S = 0;
+  lds = 0;
+  if (yacc + misc + row * op->T > limit) return SMH_OK;  // unsupported at this precision
+  rows_cap = (int)((limit - yacc - misc) / row);
+  S = (rows_cap - op->T) / op->s_mode + 1;
+  if (S > 48) S = 48;  // tile prefix (S + 1 ints) and the period bounds share 64 ints of LDS
+  rows_cap = op->s_mode * (S - 1) + op->T;
+  lds = rows_cap * row + yacc + misc;
+  return SMH_OK;
+}
+
+static void op_free(smh_op* op) {
+  void* ptrs[] = {op->d_xy, op->d_xyz, op->d_pk2pad, op->d_table, op->d_table_minus, op->d_tt64, op->d_tt32,
+                  op->d_pairs, op->d_off, op->d_items, op->d_chunk_items, op->d_ypart, op->d_pk_in, op->d_pk_out,
+                  op->d_x32, op->d_y32, op->d_x64, op->d_y64};
+  for (void* p : ptrs)
+    if (p) hipFree(p);
+}
+
+extern "C" int smh_radial_create(smh_ctx* ctx, int64_t n_part, const double* xy, const int* lmax, const int* mmax,
+                                 int l_max, int m_max, double rho_min, double resolution, int64_t len_rho,
+                                 int interp, smh_op** out) {
+  if (!ctx || !out) return SMH_ERR_ARG;
+  *out = nullptr;
+  SMH_CHECK(ctx, n_part >= 1 && xy && lmax && mmax, SMH_ERR_ARG, "need at least one particle");
+  SMH_CHECK(ctx, n_part < (1 << 28), SMH_ERR_ARG, "too many particles");
+  SMH_CHECK(ctx, l_max >= 1 && m_max >= 1 && m_max <= l_max, SMH_ERR_ARG, "invalid l_max %d / m_max %d", l_max,
+            m_max);
+  SMH_CHECK(ctx, resolution > 0 && len_rho >= 4 && len_rho < (1 << 30), SMH_ERR_ARG, "invalid table grid");
+  SMH_CHECK(ctx, interp >= 0 && interp <= 2, SMH_ERR_ARG, "invalid interpolation kind %d", interp);
+  SMH_CHECK(ctx, radial_kernel(SMH_FP32, 4, l_max, m_max) != nullptr, SMH_ERR_UNSUPPORTED,
+            "no kernel for l_max %d m_max %d", l_max, m_max);
+  SMH_HIP(ctx, hipSetDevice(ctx->device));
+  smh_op* op = new smh_op();
+  op->ctx = ctx;
+  op->kind = 0;
+  op->n_part = n_part;
+  op->l_max = l_max;
+  op->m_max = m_max;
+  op->B = smh_blocksize(l_max, m_max);
+  op->Bh = (op->B + 15) / 16 * 16;
+  op->MP = 2 * op->Bh;
+  op->interp = interp;
+  op->rho_min = rho_min;
+  op->h = resolution;
+  op->len_rho = len_rho;
+  if (interp == SMH_INTERP_LINEAR) {
+    op->T = 2;
+    op->s_mode = 1;
+    op->o_mode = 0;
+    op->n_tt = len_rho;
+  } else if (interp == SMH_INTERP_CATMULL_ROM) {
+    op->T = 4;
+    op->s_mode = 1;
+    op->o_mode = -1;
+    op->n_tt = len_rho;
+  } else {
+    op->T = 4;
+    op->s_mode = 2;
+    op->o_mode = 0;
+    op->n_tt = 2 * len_rho;
+  }
+  int rc = SMH_OK;
+  auto fail = [&](int code) {
+    op_free(op);
+    delete op;
+    return code;
+  };
+  // packed -> padded index map
+  std::vector<int> tau, ll, mm;
+  smh_index_tables(l_max, m_max, tau, ll, mm);
+  (void)tau;
+  int64_t n_unk = 0;
+  for (int64_t i = 0; i < n_part; ++i) {
+    if (lmax[i] < 1 || mmax[i] < 0 || mmax[i] > lmax[i] || lmax[i] > l_max || mmax[i] > m_max) {
+      smh_fail(ctx, SMH_ERR_ARG, "particle %lld has l_max %d m_max %d outside the table's %d/%d", (long long)i,
+               lmax[i], mmax[i], l_max, m_max);
+      return fail(SMH_ERR_ARG);
+    }
+    const int bs = smh_blocksize(lmax[i], mmax[i]);
+    for (int n = 0; n < bs; ++n) {
+      // (tau, l, m) of local index n
+      int t2 = 0, l2 = 0, m2 = 0, k = 0;
+      for (int t = 0; t < 2 && !l2; ++t)
+        for (int l = 1; l <= lmax[i] && !l2; ++l) {
+          int ml = l < mmax[i] ? l : mmax[i];
+          for (int m = -ml; m <= ml; ++m) {
+            if (k == n) {
+              t2 = t;
+              l2 = l;
+              m2 = m;
+              break;
+            }
+            ++k;
+          }
+        }
+      op->pk2pad.push_back((int)(i * op->B + smh_multi_to_single(t2, l2, m2, l_max, m_max)));
+    }
+    n_unk += bs;
+  }
+  op->n_unk = n_unk;
+  if (hipMalloc(&op->d_xy, n_part * sizeof(double2)) != hipSuccess ||
+      hipMalloc(&op->d_pk2pad, n_unk * sizeof(int)) != hipSuccess) {
+    smh_fail(ctx, SMH_ERR_HIP, "device allocation failed");
+    return fail(SMH_ERR_HIP);
+  }
+  hipStream_t s = ctx->stream;
+  std::vector<double2> hxy(n_part);
+  for (int64_t i = 0; i < n_part; ++i) hxy[i] = make_double2(xy[2 * i], xy[2 * i + 1]);
+  if (hipMemcpyAsync(op->d_xy, hxy.data(), n_part * sizeof(double2), hipMemcpyHostToDevice, s) != hipSuccess ||
+      hipMemcpyAsync(op->d_pk2pad, op->pk2pad.data(), n_unk * sizeof(int), hipMemcpyHostToDevice, s) !=
+          hipSuccess) {
+    smh_fail(ctx, SMH_ERR_HIP, "upload failed");
+    return fail(SMH_ERR_HIP);
+  }
+
+  // ---- pair binning: count, scan, fill -------------------------------------
+  op->Rc = op->Bh <= 32 ? 128 : 64;
+  op->n_chunks = (int)((n_part + op->Rc - 1) / op->Rc);
+  const int64_t nkeys = (int64_t)op->n_chunks * len_rho;
+  int *d_cnt = nullptr, *d_cur = nullptr, *d_err = nullptr;
+  auto cleanup_tmp = [&]() {
+    if (d_cnt) hipFree(d_cnt);
+    if (d_cur) hipFree(d_cur);
+    if (d_err) hipFree(d_err);
+  };
+  if (hipMalloc(&d_cnt, (nkeys + 1) * sizeof(int)) != hipSuccess ||
+      hipMalloc(&d_cur, (nkeys + 1) * sizeof(int)) != hipSuccess || hipMalloc(&d_err, sizeof(int)) != hipSuccess ||
+      hipMalloc(&op->d_off, (nkeys + 1) * sizeof(int)) != hipSuccess) {
+    cleanup_tmp();
+    smh_fail(ctx, SMH_ERR_HIP, "device allocation failed (binning)");
+    return fail(SMH_ERR_HIP);
+  }
+  hipMemsetAsync(d_cnt, 0, (nkeys + 1) * sizeof(int), s);
+  hipMemsetAsync(d_cur, 0, (nkeys + 1) * sizeof(int), s);
+  hipMemsetAsync(d_err, 0, sizeof(int), s);
+  const int nb2 = (int)((n_part + 255) / 256);
+  const int64_t nblocks = n_part * nb2;
+  hipLaunchKernelGGL(pair_bin_kernel, dim3((unsigned)nblocks), dim3(256), 0, s, op->d_xy, (int)n_part, rho_min,
+                     resolution, (int)len_rho, op->s_mode, op->o_mode, op->T, op->n_tt, op->Rc, d_cnt,
+                     (const int*)nullptr, d_cur, (uint2*)nullptr, d_err);
+  std::vector<int> cnt(nkeys);
+  int herr = 0;
+  if (hipMemcpyAsync(cnt.data(), d_cnt, nkeys * sizeof(int), hipMemcpyDeviceToHost, s) != hipSuccess ||
+      hipMemcpyAsync(&herr, d_err, sizeof(int), hipMemcpyDeviceToHost, s) != hipSuccess ||
+      hipStreamSynchronize(s) != hipSuccess) {
+    cleanup_tmp();
+    smh_fail(ctx, SMH_ERR_HIP, "binning count failed: %s", hipGetErrorString(hipGetLastError()));
+    return fail(SMH_ERR_HIP);
+  }
+  if (herr) {
+    cleanup_tmp();
+    smh_fail(ctx, SMH_ERR_ARG, "a particle distance lies outside the lookup table (rho_min %g, h %g, %lld rows)",
+             rho_min, resolution, (long long)len_rho);
+    return fail(SMH_ERR_ARG);
+  }
+  std::vector<int> off(nkeys + 1);
+  int64_t acc = 0;
+  for (int64_t k2 = 0; k2 < nkeys; ++k2) {
+    off[k2] = (int)acc;
+    acc += cnt[k2];
+  }
+  if (acc >= (int64_t)INT32_MAX) {
+    cleanup_tmp();
+    smh_fail(ctx, SMH_ERR_UNSUPPORTED, "too many particle pairs (%lld)", (long long)acc);
+    return fail(SMH_ERR_UNSUPPORTED);
+  }
+  off[nkeys] = (int)acc;
+  op->n_pairs = acc;
+  if (hipMalloc(&op->d_pairs, std::max<int64_t>(acc, 1) * sizeof(uint2)) != hipSuccess ||
+      hipMemcpyAsync(op->d_off, off.data(), (nkeys + 1) * sizeof(int), hipMemcpyHostToDevice, s) != hipSuccess) {
+    cleanup_tmp();
+    smh_fail(ctx, SMH_ERR_HIP, "device allocation failed (pairs)");
+    return fail(SMH_ERR_HIP);
+  }
+  hipLaunchKernelGGL(pair_bin_kernel, dim3((unsigned)nblocks), dim3(256), 0, s, op->d_xy, (int)n_part, rho_min,
+                     resolution, (int)len_rho, op->s_mode, op->o_mode, op->T, op->n_tt, op->Rc, d_cnt,
+                     (const int*)op->d_off, d_cur, op->d_pairs, d_err);
+  if (hipStreamSynchronize(s) != hipSuccess) {
+    cleanup_tmp();
+    smh_fail(ctx, SMH_ERR_HIP, "binning fill failed");
+    return fail(SMH_ERR_HIP);
+  }
+  cleanup_tmp();
+
+  // ---- work items: split each chunk's bins into parts of similar pair count
+  const int64_t target_items = std::max<int64_t>(4 * (int64_t)ctx->num_cu, op->n_chunks);
+  const int64_t budget = std::max<int64_t>(1, (acc + target_items - 1) / target_items);
+  std::vector<int4> items;
+  std::vector<int> chunk_items(op->n_chunks + 1, 0);
+  for (int c = 0; c < op->n_chunks; ++c) {
+    chunk_items[c] = (int)items.size();
+    const int64_t base = (int64_t)c * len_rho;
+    int rb = 0;
+    int64_t run = 0;
+    for (int r = 0; r < len_rho; ++r) {
+      run += cnt[base + r];
+      if (run >= budget && r + 1 < len_rho) {
+        items.push_back(make_int4(c, rb, r + 1, 0));
+        rb = r + 1;
+        run = 0;
+      }
+    }
+    if (run > 0 || rb == 0) items.push_back(make_int4(c, rb, (int)len_rho - 1, 0));
+  }
+  chunk_items[op->n_chunks] = (int)items.size();
+  op->n_items = (int)items.size();
+  if (hipMalloc(&op->d_items, items.size() * sizeof(int4)) != hipSuccess ||
+      hipMalloc(&op->d_chunk_items, chunk_items.size() * sizeof(int)) != hipSuccess ||
+      hipMemcpyAsync(op->d_items, items.data(), items.size() * sizeof(int4), hipMemcpyHostToDevice, s) !=
+          hipSuccess ||
+      hipMemcpyAsync(op->d_chunk_items, chunk_items.data(), chunk_items.size() * sizeof(int), hipMemcpyHostToDevice,
+                     s) != hipSuccess) {
+    smh_fail(ctx, SMH_ERR_HIP, "device allocation failed (items)");
+    return fail(SMH_ERR_HIP);
+  }
+
+  // ---- LDS plan per precision and work buffers
+  lds_plan(op, sizeof(float), op->rows_cap32, op->S32, op->lds32);
+  lds_plan(op, sizeof(double), op->rows_cap64, op->S64, op->lds64);
+  op->ypart_bytes = (size_t)op->n_items * op->Rc * 2 * op->Bh * sizeof(double);
+  const int64_t n_pad = n_part * op->B;
+  if (hipMalloc(&op->d_ypart, op->ypart_bytes) != hipSuccess ||
+      hipMalloc(&op->d_x32, n_pad * sizeof(float2)) != hipSuccess ||
+      hipMalloc(&op->d_y32, n_pad * sizeof(float2)) != hipSuccess ||
+      hipMalloc(&op->d_x64, n_pad * sizeof(double2)) != hipSuccess ||
+      hipMalloc(&op->d_y64, n_pad * sizeof(double2)) != hipSuccess) {
+    smh_fail(ctx, SMH_ERR_HIP, "device allocation failed (vectors)");
+    return fail(SMH_ERR_HIP);
+  }
+  if (hipStreamSynchronize(s) != hipSuccess) {
+    smh_fail(ctx, SMH_ERR_HIP, "setup failed");
+    return fail(SMH_ERR_HIP);
+  }
+  (void)rc;
+  *out = op;
+  return SMH_OK;
+}
+
+extern "C" int smh_op_destroy(smh_op* op) {
+  if (!op) return SMH_OK;
+  hipSetDevice(op->ctx->device);
+  hipStreamSynchronize(op->ctx->stream);
+  op_free(op);
+  delete op;
+  return SMH_OK;
+}
+
+extern "C" int smh_op_upload_table(smh_op* op, const double* table, const double* table_minus) {
+  if (!op) return SMH_ERR_ARG;
+  smh_ctx* ctx = op->ctx;
+  SMH_CHECK(ctx, table != nullptr, SMH_ERR_ARG, "table is NULL");
+  SMH_HIP(ctx, hipSetDevice(ctx->device));
+  const int64_t rows = op->kind == 0 ? op->len_rho : op->len_rho * op->len_z;
+  const size_t bytes = (size_t)rows * op->B * op->B * sizeof(double2);
+  if (!op->d_table) SMH_HIP(ctx, hipMalloc(&op->d_table, bytes));
+  SMH_HIP(ctx, hipMemcpyAsync(op->d_table, table, bytes, hipMemcpyHostToDevice, ctx->stream));
+  if (op->kind == 1) {
+    SMH_CHECK(ctx, table_minus != nullptr, SMH_ERR_ARG, "volume operator needs the minus table");
+    if (!op->d_table_minus) SMH_HIP(ctx, hipMalloc(&op->d_table_minus, bytes));
+    SMH_HIP(ctx, hipMemcpyAsync(op->d_table_minus, table_minus, bytes, hipMemcpyHostToDevice, ctx->stream));
+    SMH_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    op->has_table = true;
+    return SMH_OK;
+  }
+  return op_make_taps(op);
+}
+
+extern "C" int smh_op_download_table(smh_op* op, double* table, double* table_minus) {
+  if (!op) return SMH_ERR_ARG;
+  smh_ctx* ctx = op->ctx;
+  SMH_CHECK(ctx, op->d_table != nullptr, SMH_ERR_STATE, "no table");
+  const int64_t rows = op->kind == 0 ? op->len_rho : op->len_rho * op->len_z;
+  const size_t bytes = (size_t)rows * op->B * op->B * sizeof(double2);
+  SMH_HIP(ctx, hipMemcpyAsync(table, op->d_table, bytes, hipMemcpyDeviceToHost, ctx->stream));
+  if (op->kind == 1 && table_minus)
+    SMH_HIP(ctx, hipMemcpyAsync(table_minus, op->d_table_minus, bytes, hipMemcpyDeviceToHost, ctx->stream));
+  SMH_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return SMH_OK;
+}
+
+extern "C" int smh_op_matvec(smh_op* op, const double* x, double* y, int64_t nrhs, int precision) {
+  if (!op) return SMH_ERR_ARG;
+  smh_ctx* ctx = op->ctx;
+  SMH_CHECK(ctx, x && y && nrhs >= 1, SMH_ERR_ARG, "invalid matvec arguments");
+  SMH_CHECK(ctx, precision == SMH_FP32 || precision == SMH_FP64, SMH_ERR_ARG, "invalid precision %d", precision);
+  SMH_HIP(ctx, hipSetDevice(ctx->device));
+  int rc = op_reserve_packed(op, nrhs);
+  if (rc) return rc;
+  hipStream_t s = ctx->stream;
+  SMH_HIP(ctx, hipMemcpyAsync(op->d_pk_in, x, op->n_unk * nrhs * sizeof(double2), hipMemcpyHostToDevice, s));
+  void* xp = precision == SMH_FP32 ? (void*)op->d_x32 : (void*)op->d_x64;
+  void* yp = precision == SMH_FP32 ? (void*)op->d_y32 : (void*)op->d_y64;
+  for (int64_t r = 0; r < nrhs; ++r) {
+    if ((rc = op_unpack(op, precision, op->d_pk_in + r * op->n_unk, xp))) return rc;
+    if ((rc = op_apply(op, precision, xp, yp, false))) return rc;
+    if ((rc = op_pack(op, precision, yp, op->d_pk_out + r * op->n_unk))) return rc;
+  }
+  SMH_HIP(ctx, hipMemcpyAsync(y, op->d_pk_out, op->n_unk * nrhs * sizeof(double2), hipMemcpyDeviceToHost, s));
+  SMH_HIP(ctx, hipStreamSynchronize(s));
+  return SMH_OK;
+}
+
+namespace {
+__global__ void fill_pattern_kernel(float2* x32, double2* x64, int64_t n) {
+  const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j < n) {
+    const double a = sin(0.37 * (double)j + 0.1), b = cos(0.11 * (double)j);
+    x32[j] = make_float2((float)a, (float)b);
+    x64[j] = make_double2(a, b);
+  }
+}
+}  // namespace
+
+extern "C" int smh_op_bench_matvec(smh_op* op, int precision, int iters, double* kernel_ms, double* total_ms) {
+  if (!op) return SMH_ERR_ARG;
+  smh_ctx* ctx = op->ctx;
+  SMH_CHECK(ctx, iters >= 1, SMH_ERR_ARG, "iters must be >= 1");
+  SMH_HIP(ctx, hipSetDevice(ctx->device));
+  hipStream_t s = ctx->stream;
+  const int64_t n = op->n_part * op->B;
+  hipLaunchKernelGGL(fill_pattern_kernel, dim3(blocks_for(n)), dim3(256), 0, s, op->d_x32, op->d_x64, n);
+  void* xp = precision == SMH_FP32 ? (void*)op->d_x32 : (void*)op->d_x64;
+  void* yp = precision == SMH_FP32 ? (void*)op->d_y32 : (void*)op->d_y64;
+  int rc = op_apply(op, precision, xp, yp, false);  // warm-up
+  if (rc) return rc;
+  SMH_HIP(ctx, hipStreamSynchronize(s));
+  double ksum = 0;
+  hipEvent_t t0, t1;
+  SMH_HIP(ctx, hipEventCreate(&t0));
+  SMH_HIP(ctx, hipEventCreate(&t1));
+  SMH_HIP(ctx, hipEventRecord(t0, s));
+  for (int i = 0; i < iters; ++i) {
+    if ((rc = op_apply(op, precision, xp, yp, true))) return rc;
+    ksum += op->last_kernel_ms;
+  }
+  SMH_HIP(ctx, hipEventRecord(t1, s));
+  SMH_HIP(ctx, hipEventSynchronize(t1));
+  float tot = 0;
+  SMH_HIP(ctx, hipEventElapsedTime(&tot, t0, t1));
+  hipEventDestroy(t0);
+  hipEventDestroy(t1);
+  if (kernel_ms) *kernel_ms = ksum / iters;
+  if (total_ms) *total_ms = tot / iters;
+  return SMH_OK;
+}

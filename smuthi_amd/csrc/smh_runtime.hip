@@ -1,0 +1,96 @@
+// Context lifetime, error reporting and the host check entry points.
+#include <cstdarg>
+
+#include "smh_common.h"
+#include "smh_special.h"
+
+int smh_fail(smh_ctx* ctx, int code, const char* fmt, ...) {
+  char buf[1024];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  if (ctx) ctx->last_error = buf;
+  return code;
+}
+
+extern "C" int smh_init(int device, smh_ctx** out) {
+  if (!out) return SMH_ERR_ARG;
+  *out = nullptr;
+  int count = 0;
+  if (hipGetDeviceCount(&count) != hipSuccess || count <= 0) return SMH_ERR_HIP;
+  if (device < 0 || device >= count) return SMH_ERR_ARG;
+  smh_ctx* ctx = new smh_ctx();
+  ctx->device = device;
+  hipError_t e = hipSetDevice(device);
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipEventCreate(&ctx->ev0);
+  if (e == hipSuccess) e = hipEventCreate(&ctx->ev1);
+  hipDeviceProp_t prop;
+  if (e == hipSuccess) e = hipGetDeviceProperties(&prop, device);
+  if (e != hipSuccess) {
+    delete ctx;
+    return SMH_ERR_HIP;
+  }
+  ctx->num_cu = prop.multiProcessorCount;
+  ctx->lds_per_block = prop.sharedMemPerBlock;
+  snprintf(ctx->arch, sizeof(ctx->arch), "%s", prop.gcnArchName);
+  if (strncmp(ctx->arch, "gfx950", 6) != 0) {
+    fprintf(stderr, "smh: device %d is %s, kernels are built for gfx950\n", device, ctx->arch);
+    hipEventDestroy(ctx->ev0);
+    hipEventDestroy(ctx->ev1);
+    hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return SMH_ERR_UNSUPPORTED;
+  }
+  *out = ctx;
+  return SMH_OK;
+}
+
+extern "C" int smh_destroy(smh_ctx* ctx) {
+  if (!ctx) return SMH_OK;
+  hipSetDevice(ctx->device);
+  hipStreamSynchronize(ctx->stream);
+  hipEventDestroy(ctx->ev0);
+  hipEventDestroy(ctx->ev1);
+  hipStreamDestroy(ctx->stream);
+  delete ctx;
+  return SMH_OK;
+}
+
+extern "C" const char* smh_last_error(const smh_ctx* ctx) { return ctx ? ctx->last_error.c_str() : "no context"; }
+
+extern "C" int smh_device_info(smh_ctx* ctx, char* name, int name_len, int* num_cu) {
+  if (!ctx) return SMH_ERR_ARG;
+  if (name && name_len > 0) snprintf(name, name_len, "%s", ctx->arch);
+  if (num_cu) *num_cu = ctx->num_cu;
+  return SMH_OK;
+}
+
+extern "C" int smh_host_bessel_j(int nmax, int64_t count, const double* z, double* out) {
+  if (nmax < 0 || nmax > smh::kMaxBesselOrder || count < 0 || (!z && count) || (!out && count)) return SMH_ERR_ARG;
+  smh::zc buf[smh::kMaxBesselOrder + 1];
+  for (int64_t j = 0; j < count; ++j) {
+    smh::bessel_j_all(nmax, smh::zc{z[2 * j], z[2 * j + 1]}, buf);
+    for (int n = 0; n <= nmax; ++n) {
+      out[2 * (j * (nmax + 1) + n)] = buf[n].re;
+      out[2 * (j * (nmax + 1) + n) + 1] = buf[n].im;
+    }
+  }
+  return SMH_OK;
+}
+
+extern "C" int smh_host_spherical_hankel(int nmax, int64_t count, const double* z, double* out) {
+  if (nmax < 0 || nmax > smh::kMaxBesselOrder || count < 0 || (!z && count) || (!out && count)) return SMH_ERR_ARG;
+  smh::zc buf[smh::kMaxBesselOrder + 1];
+  for (int64_t j = 0; j < count; ++j) {
+    smh::zc zz{z[2 * j], z[2 * j + 1]};
+    if (zz.re == 0.0 && zz.im == 0.0) return SMH_ERR_ARG;
+    smh::spherical_hankel_all(nmax, zz, buf);
+    for (int n = 0; n <= nmax; ++n) {
+      out[2 * (j * (nmax + 1) + n)] = buf[n].re;
+      out[2 * (j * (nmax + 1) + n) + 1] = buf[n].im;
+    }
+  }
+  return SMH_OK;
+}

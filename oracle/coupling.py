@@ -412,9 +412,15 @@ class RadialLookupCPU:
 
     def dense(self):
         W = np.zeros((self.n_unk, self.n_unk), dtype=complex)
-        for i1 in range(self.n_part):
-            for i2 in range(self.n_part):
-                W[self.offs[i1]:self.offs[i1 + 1], self.offs[i2]:self.offs[i2 + 1]] = self.block(i1, i2)
+        np_ = self.n_part
+        ent = radial_interpolated_entries(self.table, self.rho_grid, self.rho.ravel(), self.kind)
+        ent = ent.reshape(np_, np_, ent.shape[1], ent.shape[2])
+        for i1 in range(np_):
+            s1 = slice(self.offs[i1], self.offs[i1 + 1])
+            for i2 in range(np_):
+                s2 = slice(self.offs[i2], self.offs[i2 + 1])
+                blk = ent[i1, i2][np.ix_(self.nlk[s1], self.nlk[s2])]
+                W[s1, s2] = blk * np.exp(1j * (self.m[s2][None, :] - self.m[s1][:, None]) * self.phi[i1, i2])
         return W
 
 

@@ -19,7 +19,12 @@ struct smh_ctx {
   int num_cu = 0;
   size_t lds_per_block = 0;
   char arch[64] = {0};
+  int live_ops = 0;               // operators created on this context and not yet destroyed
+  bool destroy_requested = false; // smh_destroy called while operators were alive
 };
+
+// Release a context whose destruction was requested once its last operator is gone.
+void smh_ctx_release_if_idle(smh_ctx* ctx);
 
 // Record an error on the context and return its code.
 int smh_fail(smh_ctx* ctx, int code, const char* fmt, ...);

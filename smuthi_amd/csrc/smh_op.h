@@ -31,9 +31,15 @@ struct smh_op {
   // pair binning
   int Rc = 128, n_chunks = 0, n_items = 0;
   int64_t n_pairs = 0;
-  uint2* d_pairs = nullptr;
+  uint2* d_pairs = nullptr;   // (i1, i2) while binning; released once the records exist
+  void* d_rec32 = nullptr;     // PairRec<float> per binned pair
+  void* d_rec64 = nullptr;     // PairRec<double> per binned pair
   int* d_off = nullptr;
-  int4* d_items = nullptr;
+  int4* d_items = nullptr;       // (chunk, bin begin, bin end) per work item
+  int4* d_items32 = nullptr;     // (chunk, first period, end period) for the FP32 kernel
+  int4* d_items64 = nullptr;
+  int4* d_periods32 = nullptr;   // (first bin, bins, last non-empty bin, tiles)
+  int4* d_periods64 = nullptr;
   int* d_chunk_items = nullptr;
   int S32 = 1, S64 = 1, rows_cap32 = 0, rows_cap64 = 0;
   size_t lds32 = 0, lds64 = 0;

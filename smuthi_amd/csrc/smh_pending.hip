@@ -7,11 +7,6 @@ extern "C" int smh_volume_create(smh_ctx* ctx, int64_t, const double*, const int
   return smh_fail(ctx, SMH_ERR_UNSUPPORTED, "volume lookup operator not built");
 }
 
-extern "C" int smh_solve(smh_op* op, const double*, const double*, double*, int64_t, double, int, int, int,
-                         double*) {
-  return smh_fail(op ? op->ctx : nullptr, SMH_ERR_UNSUPPORTED, "GPU solver not built");
-}
-
 extern "C" int smh_pwe_field(smh_ctx* ctx, int, int64_t, const double*, const double*, int64_t, const double*,
                              const double*, double, double, int64_t, const double*, double*, int, double*) {
   return smh_fail(ctx, SMH_ERR_UNSUPPORTED, "PWE field kernels not built");

@@ -9,7 +9,7 @@ using namespace smh;
 
 namespace {
 
-constexpr int kThreads = 512;
+constexpr int kThreads = 256;
 
 __global__ void pair_bin_kernel(const double2* __restrict__ xy, int n, double rho_min, double h, int len_rho,
                                 int s_mode, int o_mode, int T, int64_t n_tt, int Rc, int* cnt, const int* off,
@@ -35,6 +35,26 @@ __global__ void pair_bin_kernel(const double2* __restrict__ xy, int n, double rh
     const int pos = off[key] + atomicAdd(&cursor[key], 1);
     pairs[pos] = make_uint2((unsigned)i1, (unsigned)i2);
   }
+}
+
+// Geometry of every binned pair in FP64, stored per precision.
+__global__ void pair_record_kernel(const uint2* __restrict__ pairs, int64_t n_pairs, const double2* __restrict__ xy,
+                                   double rho_min, double h, int Rc, PairRec<float>* r32, PairRec<double>* r64) {
+  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n_pairs) return;
+  const uint2 pr = pairs[p];
+  const double dx = xy[pr.x].x - xy[pr.y].x, dy = xy[pr.x].y - xy[pr.y].y;
+  int r;
+  double w, rho;
+  pair_geometry(dx, dy, rho_min, h, r, w, rho);
+  double c = 1.0, s = 0.0;
+  if (rho > 0.0) {
+    c = dx / rho;
+    s = dy / rho;
+  }
+  const uint32_t ij = pr.y | ((pr.x % (unsigned)Rc) << 24);
+  r32[p] = PairRec<float>{ij, (float)w, (float)c, (float)s};
+  r64[p] = PairRec<double>{ij, 0u, w, c, s};
 }
 
 template <typename Real>
@@ -271,21 +291,17 @@ int op_apply(smh_op* op, int precision, const void* x_pad, void* y_pad, bool tim
   SMH_CHECK(ctx, k != nullptr, SMH_ERR_UNSUPPORTED, "no matvec kernel for l_max %d m_max %d", op->l_max,
             op->m_max);
   RadialKArgs a;
-  a.xy = op->d_xy;
-  a.pairs = op->d_pairs;
+  a.recs = precision == SMH_FP32 ? op->d_rec32 : op->d_rec64;
   a.off = op->d_off;
-  a.items = op->d_items;
+  a.items = precision == SMH_FP32 ? op->d_items32 : op->d_items64;
+  a.periods = precision == SMH_FP32 ? op->d_periods32 : op->d_periods64;
   a.tt = precision == SMH_FP32 ? (const void*)op->d_tt32 : (const void*)op->d_tt64;
   a.x = x_pad;
   a.ypart = op->d_ypart;
-  a.rho_min = op->rho_min;
-  a.h = op->h;
   a.len_rho = (int)op->len_rho;
-  a.n_tt = (int)op->n_tt;
   a.s_mode = op->s_mode;
   a.o_mode = op->o_mode;
   a.interp = op->interp;
-  a.S = S;
   a.Rc = op->Rc;
   a.rows_cap = precision == SMH_FP32 ? op->rows_cap32 : op->rows_cap64;
   const size_t lds = precision == SMH_FP32 ? op->lds32 : op->lds64;
@@ -318,14 +334,17 @@ int op_apply(smh_op* op, int precision, const void* x_pad, void* y_pad, bool tim
 
 static int lds_plan(smh_op* op, size_t real_size, int& rows_cap, int& S, size_t& lds) {
   const size_t limit = op->ctx->lds_per_block ? op->ctx->lds_per_block : 65536;
-  const size_t yacc = (size_t)op->Rc * 2 * op->Bh * real_size;
+  const size_t yacc = (size_t)op->Rc * 2 * (op->Bh + 1) * real_size;
   const size_t row = (size_t)op->MP * (op->B + 1) * real_size;
-  const size_t misc = 64 * sizeof(int) + 64;
+  const size_t misc = 2 * kMetaInts * sizeof(int) + 64;  // two period descriptors
+  // rows the per-thread staging registers hold: 20 16-byte loads per thread
+  const int stage_rows = (int)((size_t)20 * kThreads * 16 / ((size_t)op->MP * op->B * real_size));
   rows_cap = 0;
   S = 0;
   lds = 0;
-  if (yacc + misc + row * op->T > limit) return SMH_OK;  // unsupported at this precision
+  if (yacc + misc + row * op->T > limit || stage_rows < op->T) return SMH_OK;  // unsupported at this precision
   rows_cap = (int)((limit - yacc - misc) / row);
+  if (rows_cap > stage_rows) rows_cap = stage_rows;
   S = (rows_cap - op->T) / op->s_mode + 1;
   if (S > 48) S = 48;  // tile prefix (S + 1 ints) and the period bounds share 64 ints of LDS
   rows_cap = op->s_mode * (S - 1) + op->T;
@@ -335,7 +354,8 @@ static int lds_plan(smh_op* op, size_t real_size, int& rows_cap, int& S, size_t&
 
 static void op_free(smh_op* op) {
   void* ptrs[] = {op->d_xy, op->d_xyz, op->d_pk2pad, op->d_table, op->d_table_minus, op->d_tt64, op->d_tt32,
-                  op->d_pairs, op->d_off, op->d_items, op->d_chunk_items, op->d_ypart, op->d_pk_in, op->d_pk_out,
+                  op->d_pairs, op->d_rec32, op->d_rec64, op->d_off, op->d_items, op->d_items32, op->d_items64,
+                  op->d_periods32, op->d_periods64, op->d_chunk_items, op->d_ypart, op->d_pk_in, op->d_pk_out,
                   op->d_x32, op->d_y32, op->d_x64, op->d_y64};
   for (void* p : ptrs)
     if (p) hipFree(p);
@@ -347,7 +367,7 @@ extern "C" int smh_radial_create(smh_ctx* ctx, int64_t n_part, const double* xy,
   if (!ctx || !out) return SMH_ERR_ARG;
   *out = nullptr;
   SMH_CHECK(ctx, n_part >= 1 && xy && lmax && mmax, SMH_ERR_ARG, "need at least one particle");
-  SMH_CHECK(ctx, n_part < (1 << 28), SMH_ERR_ARG, "too many particles");
+  SMH_CHECK(ctx, n_part < (1 << 24), SMH_ERR_ARG, "too many particles (pair records index 24 bits)");
   SMH_CHECK(ctx, l_max >= 1 && m_max >= 1 && m_max <= l_max, SMH_ERR_ARG, "invalid l_max %d / m_max %d", l_max,
             m_max);
   SMH_CHECK(ctx, resolution > 0 && len_rho >= 4 && len_rho < (1 << 30), SMH_ERR_ARG, "invalid table grid");
@@ -506,6 +526,21 @@ extern "C" int smh_radial_create(smh_ctx* ctx, int64_t n_part, const double* xy,
     return fail(SMH_ERR_HIP);
   }
   cleanup_tmp();
+  if (hipMalloc(&op->d_rec32, std::max<int64_t>(acc, 1) * sizeof(PairRec<float>)) != hipSuccess ||
+      hipMalloc(&op->d_rec64, std::max<int64_t>(acc, 1) * sizeof(PairRec<double>)) != hipSuccess) {
+    smh_fail(ctx, SMH_ERR_HIP, "device allocation failed (pair records, %lld pairs)", (long long)acc);
+    return fail(SMH_ERR_HIP);
+  }
+  if (acc > 0)
+    hipLaunchKernelGGL(pair_record_kernel, dim3((unsigned)((acc + 255) / 256)), dim3(256), 0, s, op->d_pairs, acc,
+                       op->d_xy, rho_min, resolution, op->Rc, (PairRec<float>*)op->d_rec32,
+                       (PairRec<double>*)op->d_rec64);
+  if (hipStreamSynchronize(s) != hipSuccess) {
+    smh_fail(ctx, SMH_ERR_HIP, "pair records failed");
+    return fail(SMH_ERR_HIP);
+  }
+  hipFree(op->d_pairs);
+  op->d_pairs = nullptr;
 
   // ---- work items: split each chunk's bins into parts of similar pair count
   const int64_t target_items = std::max<int64_t>(4 * (int64_t)ctx->num_cu, op->n_chunks);
@@ -542,6 +577,41 @@ extern "C" int smh_radial_create(smh_ctx* ctx, int64_t n_part, const double* xy,
   // ---- LDS plan per precision and work buffers
   lds_plan(op, sizeof(float), op->rows_cap32, op->S32, op->lds32);
   lds_plan(op, sizeof(double), op->rows_cap64, op->S64, op->lds64);
+  // periods: runs of at most S bins starting at a non-empty bin, with their tile counts
+  for (int prec = 0; prec < 2; ++prec) {
+    const int S = prec == 0 ? op->S32 : op->S64;
+    if (S < 1) continue;
+    std::vector<int4> periods, pitems;
+    for (const int4& it : items) {
+      const int64_t base = (int64_t)it.x * len_rho;
+      const int pstart = (int)periods.size();
+      int r = it.y;
+      while (r < it.z) {
+        while (r < it.z && cnt[base + r] == 0) ++r;
+        if (r >= it.z) break;
+        const int nb = std::min(S, it.z - r);
+        int rl = r, tiles = 0;
+        for (int k = 0; k < nb; ++k) {
+          const int c = cnt[base + r + k];
+          if (c > 0) rl = r + k;
+          tiles += (c + 7) >> 3;
+        }
+        periods.push_back(make_int4(r, nb, rl, tiles));
+        r += nb;
+      }
+      pitems.push_back(make_int4(it.x, pstart, (int)periods.size(), 0));
+    }
+    int4** dp = prec == 0 ? &op->d_periods32 : &op->d_periods64;
+    int4** di = prec == 0 ? &op->d_items32 : &op->d_items64;
+    if (hipMalloc(dp, std::max<size_t>(periods.size(), 1) * sizeof(int4)) != hipSuccess ||
+        hipMalloc(di, pitems.size() * sizeof(int4)) != hipSuccess ||
+        hipMemcpyAsync(*dp, periods.data(), periods.size() * sizeof(int4), hipMemcpyHostToDevice, s) != hipSuccess ||
+        hipMemcpyAsync(*di, pitems.data(), pitems.size() * sizeof(int4), hipMemcpyHostToDevice, s) != hipSuccess ||
+        hipStreamSynchronize(s) != hipSuccess) {
+      smh_fail(ctx, SMH_ERR_HIP, "device allocation failed (periods)");
+      return fail(SMH_ERR_HIP);
+    }
+  }
   op->ypart_bytes = (size_t)op->n_items * op->Rc * 2 * op->Bh * sizeof(double);
   const int64_t n_pad = n_part * op->B;
   if (hipMalloc(&op->d_ypart, op->ypart_bytes) != hipSuccess ||
@@ -557,16 +627,20 @@ extern "C" int smh_radial_create(smh_ctx* ctx, int64_t n_part, const double* xy,
     return fail(SMH_ERR_HIP);
   }
   (void)rc;
+  ctx->live_ops++;
   *out = op;
   return SMH_OK;
 }
 
 extern "C" int smh_op_destroy(smh_op* op) {
   if (!op) return SMH_OK;
-  hipSetDevice(op->ctx->device);
-  hipStreamSynchronize(op->ctx->stream);
+  smh_ctx* ctx = op->ctx;
+  hipSetDevice(ctx->device);
+  hipStreamSynchronize(ctx->stream);
   op_free(op);
   delete op;
+  ctx->live_ops--;
+  smh_ctx_release_if_idle(ctx);
   return SMH_OK;
 }
 

@@ -47,8 +47,22 @@ extern "C" int smh_init(int device, smh_ctx** out) {
   return SMH_OK;
 }
 
+void smh_ctx_release_if_idle(smh_ctx* ctx) {
+  if (!ctx || !ctx->destroy_requested || ctx->live_ops > 0) return;
+  hipSetDevice(ctx->device);
+  hipStreamSynchronize(ctx->stream);
+  hipEventDestroy(ctx->ev0);
+  hipEventDestroy(ctx->ev1);
+  hipStreamDestroy(ctx->stream);
+  delete ctx;
+}
+
 extern "C" int smh_destroy(smh_ctx* ctx) {
   if (!ctx) return SMH_OK;
+  if (ctx->live_ops > 0) {  // freed when the last operator is destroyed
+    ctx->destroy_requested = true;
+    return SMH_OK;
+  }
   hipSetDevice(ctx->device);
   hipStreamSynchronize(ctx->stream);
   hipEventDestroy(ctx->ev0);

@@ -1,0 +1,153 @@
+"""Field expansions with smuthi's interface (fields/expansions.py:242-1155):
+plane wave expansions whose E/H fields are evaluated by the libsmh PWE
+kernels, spherical wave expansion containers, and the layer response of a
+PWE (layers.py:141-191).
+"""
+import ctypes
+
+import numpy as np
+
+from . import _lib
+from . import cuda as cu
+from . import fields as flds
+from . import layers as lay
+
+
+class FieldExpansion:
+    def __init__(self):
+        self.validity_conditions = []
+
+    def valid(self, x, y, z):
+        ret = np.ones(np.shape(x), dtype=bool)
+        for check in self.validity_conditions:
+            ret = np.logical_and(ret, check(x, y, z))
+        return ret
+
+
+class SphericalWaveExpansion(FieldExpansion):
+    """Coefficients a_{tau l m} of a regular or outgoing SWE about reference_point."""
+
+    def __init__(self, k, l_max, m_max=None, kind=None, reference_point=None, lower_z=-np.inf, upper_z=np.inf,
+                 inner_r=0, outer_r=np.inf):
+        FieldExpansion.__init__(self)
+        self.k = k
+        self.l_max = l_max
+        self.m_max = l_max if m_max is None else m_max
+        self.coefficients = np.zeros(flds.blocksize(self.l_max, self.m_max), dtype=complex)
+        self.kind = kind
+        self.reference_point = reference_point
+        self.lower_z, self.upper_z = lower_z, upper_z
+        self.inner_r, self.outer_r = inner_r, outer_r
+
+    def coefficients_tlm(self, tau, l, m):
+        return self.coefficients[flds.multi_to_single_index(tau, l, m, self.l_max, self.m_max)]
+
+
+class PlaneWaveExpansion(FieldExpansion):
+    """g_j(kappa, alpha) on a (k_parallel, azimuthal_angles) grid; a single
+    entry in each means a discrete plane wave."""
+
+    def __init__(self, k, k_parallel, azimuthal_angles, kind=None, reference_point=None, lower_z=-np.inf,
+                 upper_z=np.inf):
+        FieldExpansion.__init__(self)
+        self.k = k
+        self.k_parallel = np.array(k_parallel, ndmin=1)
+        self.azimuthal_angles = np.array(azimuthal_angles, ndmin=1)
+        self.kind = kind
+        self.reference_point = reference_point
+        self.lower_z, self.upper_z = lower_z, upper_z
+        self.coefficients = np.zeros((2, len(self.k_parallel), len(self.azimuthal_angles)), dtype=complex)
+
+    def k_z(self):
+        kz = flds.k_z(k_parallel=self.k_parallel, k=self.k)
+        if self.kind == 'upgoing':
+            return kz
+        if self.kind == 'downgoing':
+            return -kz
+        raise ValueError('pwe kind undefined')
+
+    def k_parallel_grid(self):
+        return np.meshgrid(self.k_parallel, self.azimuthal_angles, indexing='ij')[0]
+
+    def azimuthal_angle_grid(self):
+        return np.meshgrid(self.k_parallel, self.azimuthal_angles, indexing='ij')[1]
+
+    def k_z_grid(self):
+        return np.broadcast_to(self.k_z()[:, None], (len(self.k_parallel), len(self.azimuthal_angles)))
+
+    def valid(self, x, y, z):
+        z = np.asarray(z)
+        if self.upper_z == float('inf'):
+            vld = z >= self.lower_z
+        else:
+            vld = np.logical_and(z >= self.lower_z, z < self.upper_z)
+        return np.logical_and(vld, FieldExpansion.valid(self, x, y, z))
+
+    def compatible(self, other):
+        return (isinstance(other, PlaneWaveExpansion) and np.isclose(self.k, other.k)
+                and len(self.k_parallel) == len(other.k_parallel) and np.allclose(self.k_parallel, other.k_parallel)
+                and len(self.azimuthal_angles) == len(other.azimuthal_angles)
+                and np.allclose(self.azimuthal_angles, other.azimuthal_angles) and self.kind == other.kind
+                and np.array_equal(self.reference_point, other.reference_point))
+
+    def __add__(self, other):
+        if not self.compatible(other):
+            raise ValueError('Plane wave expansion are inconsistent.')
+        out = PlaneWaveExpansion(self.k, self.k_parallel, self.azimuthal_angles, self.kind, self.reference_point,
+                                 max(self.lower_z, other.lower_z), min(self.upper_z, other.upper_z))
+        out.coefficients = self.coefficients + other.coefficients
+        return out
+
+    def _field(self, kind, x, y, z, omega, precision):
+        x, y, z = (np.asarray(v, dtype=float) for v in (x, y, z))
+        out = [np.zeros(x.shape, dtype=complex) for _ in range(3)]
+        vld = self.valid(x, y, z)
+        if not vld.any():
+            return tuple(out)
+        pts = np.ascontiguousarray(np.stack([x[vld] - self.reference_point[0], y[vld] - self.reference_point[1],
+                                             z[vld] - self.reference_point[2]], axis=-1))
+        kp = np.ascontiguousarray(self.k_parallel, dtype=np.complex128)
+        kz = np.ascontiguousarray(self.k_z(), dtype=np.complex128)
+        al = np.ascontiguousarray(self.azimuthal_angles, dtype=np.float64)
+        g = np.ascontiguousarray(self.coefficients, dtype=np.complex128)
+        res = np.zeros((3, len(pts)), dtype=np.complex128)
+        ms = ctypes.c_double()
+        prec = _lib.FP64 if precision in ('double', 'double precision') else _lib.FP32
+        k = complex(self.k)
+        _lib.check(_lib.load().smh_pwe_field(cu.context(), kind, len(kp), _lib.dptr(kp), _lib.dptr(kz), len(al),
+                                              _lib.dptr(al), _lib.dptr(g), k.real, k.imag, len(pts), _lib.dptr(pts),
+                                              _lib.dptr(res), prec, ctypes.byref(ms)), cu.context(), 'smh_pwe_field')
+        self.last_kernel_ms = ms.value
+        for c in range(3):
+            out[c][vld] = res[c] / omega
+        return tuple(out)
+
+    def electric_field(self, x, y, z, max_chunksize=50, cpu_precision='single precision', precision='single'):
+        """(E_x, E_y, E_z) at the points; zero where the expansion is not valid."""
+        return self._field(0, x, y, z, 1.0, precision)
+
+    def magnetic_field(self, x, y, z, vacuum_wavelength, max_chunksize=50, cpu_precision='single precision',
+                       precision='single'):
+        """(H_x, H_y, H_z) at the points; zero where the expansion is not valid."""
+        return self._field(1, x, y, z, flds.angular_frequency(vacuum_wavelength), precision)
+
+
+def layer_response(layer_system, pwe, from_layer, to_layer):
+    """(upgoing, downgoing) PWEs of the layer system response to pwe (or to a (up, down) tuple)."""
+    if isinstance(pwe, (tuple, list)):
+        u0, d0 = layer_response(layer_system, pwe[0], from_layer, to_layer)
+        u1, d1 = layer_response(layer_system, pwe[1], from_layer, to_layer)
+        return u0 + u1, d0 + d1
+    omega = pwe.k / layer_system.refractive_indices[from_layer]
+    k_to = omega * layer_system.refractive_indices[to_layer]
+    ref = [0, 0, layer_system.reference_z(to_layer)]
+    loz, upz = layer_system.lower_zlimit(to_layer), layer_system.upper_zlimit(to_layer)
+    up = PlaneWaveExpansion(k_to, pwe.k_parallel, pwe.azimuthal_angles, 'upgoing', ref, loz, upz)
+    dn = PlaneWaveExpansion(k_to, pwe.k_parallel, pwe.azimuthal_angles, 'downgoing', ref, loz, upz)
+    col = 0 if pwe.kind == 'upgoing' else 1
+    for pol in range(2):
+        L = lay.layersystem_response_matrix(pol, layer_system.thicknesses, layer_system.refractive_indices,
+                                            pwe.k_parallel, omega, from_layer, to_layer)
+        up.coefficients[pol] = L[0, col][:, None] * pwe.coefficients[pol]
+        dn.coefficients[pol] = L[1, col][:, None] * pwe.coefficients[pol]
+    return up, dn

@@ -177,3 +177,159 @@ class CouplingMatrixRadialLookupCUDA(_GpuCouplingOperator):
         self.table_seconds = 0.0
         self._make_linear_operator()
         return self
+
+
+class TMatrix(SystemMatrix):
+    """Block-diagonal T of the particles (linear_system.py:410-438); spheres give a diagonal."""
+
+    def __init__(self, particle_list):
+        SystemMatrix.__init__(self, particle_list)
+        diags = []
+        for p in particle_list:
+            t = np.asarray(p.t_matrix)
+            d = np.diag(t)
+            if np.count_nonzero(t - np.diag(d)):
+                raise NotImplementedError('only diagonal (sphere) T-matrices are supported on the GPU path')
+            diags.append(d)
+        self.diagonal = np.ascontiguousarray(np.concatenate(diags), dtype=np.complex128)
+        self.linear_operator = scipy.sparse.linalg.LinearOperator(
+            shape=self.shape, matvec=lambda v: self.diagonal * v, matmat=lambda v: self.diagonal[:, None] * v,
+            dtype=complex)
+
+    def right_hand_side(self):
+        a = np.concatenate([p.initial_field.coefficients for p in self.particle_list])
+        return self.diagonal * a
+
+
+class MasterMatrix(SystemMatrix):
+    """M = 1 - T W as a LinearOperator (FP64 coupling matvec on the GPU)."""
+
+    def __init__(self, t_matrix, coupling_matrix):
+        SystemMatrix.__init__(self, t_matrix.particle_list)
+        self.t_matrix = t_matrix
+        self.coupling_matrix = coupling_matrix
+
+        def apply(v):
+            return v - self.t_matrix.diagonal * self.coupling_matrix.matvec(v, precision='double')
+
+        self.linear_operator = scipy.sparse.linalg.LinearOperator(shape=self.shape, matvec=apply, dtype=complex)
+
+
+class LinearSystem:
+    """Assemble and solve (1 - T W) b = T a with smuthi's constructor arguments
+    (linear_system.py:42-344). The coupling must use a lookup table; the solve
+    runs on the GPU. Extra arguments: precision ('mixed': FP32 coupling
+    matvecs inside FP64 iterative refinement; 'double': plain FP64 GMRES),
+    cubic_interpolant, restart and maxiter."""
+
+    def __init__(self, particle_list, initial_field, layer_system, k_parallel='default', solver_type='gmres',
+                 solver_tolerance=1e-4, store_coupling_matrix=False, coupling_matrix_lookup_resolution=None,
+                 interpolator_kind='cubic', cuda_blocksize=None, periodicity=None, precision='mixed',
+                 cubic_interpolant='spline', restart=20, maxiter=None, **unused):
+        self.particle_list = particle_list
+        self.initial_field = initial_field
+        self.layer_system = layer_system
+        self.k_parallel = k_parallel
+        self.solver_type = solver_type
+        self.solver_tolerance = solver_tolerance
+        self.store_coupling_matrix = store_coupling_matrix
+        self.coupling_matrix_lookup_resolution = coupling_matrix_lookup_resolution
+        self.interpolator_kind = interpolator_kind
+        self.cuda_blocksize = cu.default_blocksize if cuda_blocksize is None else cuda_blocksize
+        self.precision = precision
+        self.cubic_interpolant = cubic_interpolant
+        self.restart = restart
+        self.maxiter = maxiter
+        if periodicity is not None:
+            raise NotImplementedError('periodic particle arrangements are outside the GPU lookup path')
+        self.shape = SystemMatrix(particle_list).shape
+        sys.stdout.write('Number of unknowns: %i\n' % self.shape[0])
+
+    def prepare(self):
+        self.compute_initial_field_coefficients()
+        self.compute_t_matrix()
+        self.compute_coupling_matrix()
+        self.master_matrix = MasterMatrix(self.t_matrix, self.coupling_matrix)
+
+    def compute_initial_field_coefficients(self):
+        from .expansions import SphericalWaveExpansion
+        a = self.initial_field.swe_coefficients(self.particle_list, self.layer_system)
+        off = 0
+        for p in self.particle_list:
+            n = flds.blocksize(p.l_max, p.m_max)
+            i = self.layer_system.layer_number(p.position[2])
+            k = flds.angular_frequency(self.initial_field.vacuum_wavelength) * self.layer_system.refractive_indices[i]
+            swe = SphericalWaveExpansion(k, p.l_max, p.m_max, 'regular', p.position,
+                                         self.layer_system.lower_zlimit(i), self.layer_system.upper_zlimit(i))
+            swe.coefficients = a[off:off + n]
+            p.initial_field = swe
+            off += n
+
+    def compute_t_matrix(self):
+        for p in self.particle_list:
+            i = self.layer_system.layer_number(p.position[2])
+            p.t_matrix = p.compute_t_matrix(self.initial_field.vacuum_wavelength,
+                                            self.layer_system.refractive_indices[i])
+        self.t_matrix = TMatrix(self.particle_list)
+
+    def compute_coupling_matrix(self):
+        if self.coupling_matrix_lookup_resolution is None:
+            raise NotImplementedError('the GPU path couples particles through a lookup table: '
+                                      'set coupling_matrix_lookup_resolution')
+        if self.solver_type.lower() == 'lu' or self.store_coupling_matrix:
+            warnings.warn('explicit coupling matrices are not built on the GPU path; using the lookup with GMRES')
+            self.solver_type = 'gmres'
+        if not cu.use_gpu:
+            raise RuntimeError('smuthi_amd provides the MI355X operators only: call '
+                               'smuthi_amd.cuda.enable_gpu() first')
+        layers = {self.layer_system.layer_number(p.position[2]) for p in self.particle_list}
+        if len(layers) != 1:
+            raise NotImplementedError('lookup coupling needs all particles in one layer')
+        z = [p.position[2] for p in self.particle_list]
+        kw = dict(vacuum_wavelength=self.initial_field.vacuum_wavelength, particle_list=self.particle_list,
+                  layer_system=self.layer_system, k_parallel=self.k_parallel,
+                  resolution=self.coupling_matrix_lookup_resolution, interpolator_kind=self.interpolator_kind)
+        if z.count(z[0]) == len(z):
+            self.coupling_matrix = CouplingMatrixRadialLookupCUDA(cuda_blocksize=self.cuda_blocksize,
+                                                                  cubic_interpolant=self.cubic_interpolant, **kw)
+        else:
+            from .volume import CouplingMatrixVolumeLookupCUDA
+            self.coupling_matrix = CouplingMatrixVolumeLookupCUDA(cuda_blocksize=self.cuda_blocksize, **kw)
+
+    def solve(self, rhs=None):
+        """Solve for the scattered-field coefficients; rhs may hold several right-hand sides [nrhs, n]."""
+        from .expansions import SphericalWaveExpansion
+        if self.solver_type.lower() not in ('gmres', 'lu'):
+            raise ValueError('This solver type is currently not implemented.')
+        single = rhs is None
+        b = np.ascontiguousarray(self.t_matrix.right_hand_side() if single else rhs, dtype=np.complex128)
+        nrhs = 1 if b.ndim == 1 else b.shape[0]
+        x = np.empty_like(b)
+        stats = np.zeros(8)
+        mode = _lib.SOLVE_MIXED if self.precision == 'mixed' else _lib.SOLVE_FP64
+        maxiter = self.maxiter if self.maxiter is not None else 10 * self.shape[0]
+        maxiter = int(min(maxiter, 2 ** 31 - 1))
+        code = _lib.load().smh_solve(self.coupling_matrix._op, _lib.dptr(self.t_matrix.diagonal), _lib.dptr(b),
+                                     _lib.dptr(x), nrhs, float(self.solver_tolerance), int(self.restart), maxiter,
+                                     mode, _lib.dptr(stats))
+        if code == _lib.SMH_ERR_NOCONV:
+            warnings.warn('GMRES did not reach the tolerance: %s'
+                          % _lib.load().smh_last_error(cu.context()).decode(errors='replace'))
+        else:
+            _lib.check(code, cu.context(), 'smh_solve')
+        self.solve_stats = dict(iterations=int(stats[0]), matvecs_fp64=int(stats[1]), matvecs_fp32=int(stats[2]),
+                                relative_residual=float(stats[3]), seconds=float(stats[4]), sweeps=int(stats[5]))
+        if not single:
+            return x
+        off = 0
+        for p in self.particle_list:
+            n = flds.blocksize(p.l_max, p.m_max)
+            i = self.layer_system.layer_number(p.position[2])
+            k = flds.angular_frequency(self.initial_field.vacuum_wavelength) * self.layer_system.refractive_indices[i]
+            swe = SphericalWaveExpansion(k, p.l_max, p.m_max, 'outgoing', p.position,
+                                         self.layer_system.lower_zlimit(i), self.layer_system.upper_zlimit(i))
+            swe.coefficients = x[off:off + n].copy()
+            swe.validity_conditions.append(p.is_outside)
+            p.scattered_field = swe
+            off += n
+        return x

@@ -1,0 +1,109 @@
+"""GPU GMRES solves (plain FP64 and FP32-in-FP64 refinement) against the
+FP64 oracle: the same lookup operator assembled densely and solved by LU."""
+import numpy as np
+import pytest
+
+from oracle import basics as ob
+from oracle import layers as ol
+from oracle import coupling as oc
+from oracle import system as osys
+
+pytestmark = pytest.mark.gpu
+
+WL = 550
+
+
+@pytest.fixture(scope='module')
+def gpu():
+    from smuthi_amd import cuda
+    assert cuda.enable_gpu(), 'MI355X not available'
+    return cuda
+
+
+def _run_product(plist_o, ls_args, kp, pw_args, res, kind, cubic, tol, precision):
+    from smuthi_amd.layers import LayerSystem
+    from smuthi_amd.particles import Sphere
+    from smuthi_amd.initial_field import PlaneWave
+    from smuthi_amd.simulation import Simulation
+    pparts = [Sphere(p.position, p.refractive_index, p.radius, p.l_max, p.m_max) for p in plist_o]
+    sim = Simulation(layer_system=LayerSystem(*ls_args), particle_list=pparts, initial_field=PlaneWave(*pw_args),
+                     k_parallel=kp, solver_type='gmres', solver_tolerance=tol, store_coupling_matrix=False,
+                     coupling_matrix_lookup_resolution=res, coupling_matrix_interpolator_kind=kind,
+                     cubic_interpolant=cubic, precision=precision)
+    sim.run()
+    b = np.concatenate([p.scattered_field.coefficients for p in pparts])
+    return sim, b
+
+
+def _oracle_lookup_solution(plist_o, ls_args, kp, pw_args, res, okind):
+    ls = ol.LayerSystem(*ls_args)
+    table, rho = oc.radial_coupling_lookup_table(WL, plist_o, ls, kp, res)
+    W = oc.RadialLookupCPU(table, rho, plist_o, okind).dense()
+    lsys = osys.LinearSystemOracle(plist_o, osys.PlaneWave(*pw_args), ls, kp)
+    return lsys, W, lsys.solve_lu(W)
+
+
+@pytest.mark.parametrize('precision', ['double', 'mixed'])
+@pytest.mark.parametrize('kind,cubic,okind', [('linear', 'spline', 'linear'), ('cubic', 'spline', 'cubic'),
+                                              ('cubic', 'catmull-rom', 'catmull-rom')])
+def test_three_spheres_solution_matches_oracle(gpu, precision, kind, cubic, okind):
+    # geometry and tolerances of tests/unit_tests/multiple_scattering_tests/test_radial_lookup.py
+    kp = ob.reasonable_Sommerfeld_kpar_contour(WL, neff_waypoints=[0, 0.5, 0.8 - 0.01j, 2 - 0.01j, 2.5, 5],
+                                              neff_resolution=1e-2)
+    plist = [osys.Sphere([0, 0, 150], 2.4 + 0.1j, 100, 3, 3), osys.Sphere([102, -100, 150], 1.9 + 0.2j, 80, 3, 2),
+             osys.Sphere([202, 100, 150], 1.7, 90, 3, 3)]
+    ls_args = ([0, 400, 0], [1.5, 2 + 0.01j, 1])
+    pw_args = (WL, np.pi * 7 / 8, np.pi / 3, 0)
+    sim, b = _run_product(plist, ls_args, kp, pw_args, 5.0, kind, cubic, 1e-8, precision)
+    lsys, W, b_ref = _oracle_lookup_solution(plist, ls_args, kp, pw_args, 5.0, okind)
+    assert np.linalg.norm(b - b_ref) / np.linalg.norm(b_ref) < 1e-6
+    st = sim.linear_system.solve_stats
+    assert st['relative_residual'] <= 1e-8
+    # the lookup solution against the explicit-coupling solution (test_radial_lookup.py:127-140)
+    b_exp = lsys.solve_lu(lsys.explicit_coupling())
+    n0 = 30
+    assert np.linalg.norm(b[:n0] - b_exp[:n0]) / np.linalg.norm(b_exp[:n0]) < 5e-4
+
+
+@pytest.mark.parametrize('precision', ['double', 'mixed'])
+def test_fifty_spheres_on_substrate(gpu, precision):
+    """The CPU-runnable configuration: 50 spheres on glass, radial linear 5 nm, tol 1e-5."""
+    from smuthi_amd import ensembles
+    pp = ensembles.small_substrate_ensemble(seed=1)
+    plist = [osys.Sphere(p.position, p.refractive_index, p.radius, 3, 3) for p in pp]
+    kp = ob.reasonable_Sommerfeld_kpar_contour(WL, layer_refractive_indices=[1.52, 1], neff_resolution=1e-2)
+    ls_args = ([0, 0], [1.52, 1])
+    pw_args = (WL, np.pi, 0, 0)
+    sim, b = _run_product(plist, ls_args, kp, pw_args, 5.0, 'linear', 'spline', 1e-7, precision)
+    lsys, W, b_ref = _oracle_lookup_solution(plist, ls_args, kp, pw_args, 5.0, 'linear')
+    err = np.linalg.norm(b - b_ref) / np.linalg.norm(b_ref)
+    assert err < 1e-5, err
+    rhs = lsys.right_hand_side()
+    M = lsys.master_dense(W)
+    assert np.linalg.norm(rhs - M @ b) / np.linalg.norm(rhs) < 1e-6
+    # scipy GMRES at smuthi's settings reaches the same coefficients within its tolerance
+    b_gm, info = lsys.solve_gmres(lambda v: W @ v, tol=1e-7)
+    assert np.linalg.norm(b - b_gm) / np.linalg.norm(b_gm) < 1e-5
+
+
+def test_multiple_rhs_match_single(gpu):
+    """Several incidence angles in one smh_solve call equal one-by-one solves."""
+    from smuthi_amd.layers import LayerSystem
+    from smuthi_amd.particles import Sphere
+    from smuthi_amd.initial_field import PlaneWave
+    from smuthi_amd.linear_system import LinearSystem
+    from smuthi_amd import ensembles
+    pparts = ensembles.small_substrate_ensemble(seed=3)[:20]
+    kp = ob.reasonable_Sommerfeld_kpar_contour(WL, layer_refractive_indices=[1.52, 1], neff_resolution=1e-2)
+    ls = LayerSystem([0, 0], [1.52, 1])
+    rhs = []
+    sols = []
+    for j, ang in enumerate((np.pi, np.pi * 0.9, np.pi * 0.8)):
+        lsys = LinearSystem(pparts, PlaneWave(WL, ang, 0.3 * j, j % 2), ls, kp, solver_type='gmres',
+                            solver_tolerance=1e-8, coupling_matrix_lookup_resolution=5.0, interpolator_kind='cubic')
+        lsys.prepare()
+        rhs.append(lsys.t_matrix.right_hand_side())
+        sols.append(lsys.solve())
+    X = lsys.solve(np.stack(rhs))
+    for j in range(3):
+        assert np.linalg.norm(X[j] - sols[j]) / np.linalg.norm(sols[j]) < 1e-7

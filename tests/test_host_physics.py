@@ -97,3 +97,20 @@ def test_radial_table_coefficients_reproduce_oracle_table():
         got = direct + layer
         rel = np.abs(got - table[j]).max() / np.abs(table[j]).max()
         assert rel < 1e-7, (j, rel)
+
+
+def test_plane_wave_rhs_matches_oracle():
+    from oracle import system as osys
+    from smuthi_amd.initial_field import PlaneWave
+    from smuthi_amd.particles import Sphere
+    ols = ol.LayerSystem([0, 400, 0], [1.5, 2 + 0.01j, 1])
+    pls = lay.LayerSystem([0, 400, 0], [1.5, 2 + 0.01j, 1])
+    for polar, pol in ((np.pi * 7 / 8, 0), (np.pi / 5, 1), (np.pi, 0)):
+        opw = osys.PlaneWave(WL, polar, np.pi / 3, pol, 1.3, [10, -20, 30])
+        ppw = PlaneWave(WL, polar, np.pi / 3, pol, 1.3, [10, -20, 30])
+        oparts = [OSphere([0, 0, 150], 2.4, 100, 3, 3), OSphere([102, -100, 350], 1.9, 80, 3, 2),
+                  OSphere([202, 100, 500], 1.7, 90, 2, 2)]
+        pparts = [Sphere(p.position, p.refractive_index, p.radius, p.l_max, p.m_max) for p in oparts]
+        want = np.concatenate([osys.initial_field_coefficients(opw, p, ols) for p in oparts])
+        got = ppw.swe_coefficients(pparts, pls)
+        np.testing.assert_allclose(got, want, rtol=1e-12, atol=1e-14)

@@ -1,0 +1,228 @@
+"""Benchmark: RHS-solves/s of the lookup-coupling solve at 10,000 spheres and
+the coupling-matvec throughput, on one MI355X.
+
+Workload (synthetic, seeded): 10,000 spheres (radius U(80,120) nm, index
+U(1.5,2.0) + 0.01i) at z = 200 nm in a 60 x 60 um square with 20 nm minimum
+gaps, on glass (n = 1.52) in air, lambda = 550 nm, l_max = m_max = 3
+(300,000 unknowns), radial lookup with cubic (not-a-knot spline)
+interpolation at 2 nm, Sommerfeld contour with neff_resolution = neff_imag =
+2e-3. A step is one right-hand-side solve (plane wave, incidence angle and
+polarisation vary per step) to relative residual 1e-6: GMRES whose coupling
+matvecs run in FP32 on MFMA, wrapped in FP64 iterative refinement with FP64
+residuals. The coefficients of step 0 are compared with a plain FP64 GPU
+GMRES solve (tol 1e-9) of the same right-hand side.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--particles N]
+"""
+import argparse
+import json
+import os
+import platform
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+MFMA_FP32_PEAK_TFLOPS = 157.3
+MFMA_FP64_PEAK_TFLOPS = 78.6
+HBM_PEAK_GBS = 8000.0
+WL = 550.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def build_workload(n_particles, seed=2):
+    from smuthi_amd import ensembles, fields as flds
+    from smuthi_amd.layers import LayerSystem
+    side = 60000.0 * np.sqrt(n_particles / 10000.0)
+    plist = ensembles.monolayer(n_particles, side, 200.0, seed=seed)
+    ls = LayerSystem([0, 0], [1.52, 1])
+    kp = flds.reasonable_Sommerfeld_kpar_contour(WL, layer_refractive_indices=[1.52, 1], neff_resolution=2e-3,
+                                                 neff_imag=2e-3)
+    return plist, ls, kp, side
+
+
+def rhs_for_step(j):
+    """Incidence of step j: polar angle pi - 8 deg * (j mod 8), polarisation j mod 2."""
+    return np.pi - np.deg2rad(8.0 * (j % 8)), 0.0, j % 2
+
+
+def cpu_baseline(table, rho, n_full, iters_per_solve, n_sample=400, seed=5, budget_s=20.0):
+    """Time the oracle's restatement of smuthi's CPU radial-lookup matvec
+    (CouplingMatrixRadialLookupCPU.matvec: per-(n1, n2) scipy interp1d 'cubic',
+    gather, phase, dot) on a sample ensemble of the same density, scale the
+    matvec by (N / N_sample)^2 and multiply by the GMRES iteration count."""
+    from oracle import coupling as oc
+    from oracle.system import Sphere
+    from smuthi_amd import ensembles
+    side = 60000.0 * np.sqrt(n_sample / 10000.0)
+    pp = ensembles.monolayer(n_sample, side, 200.0, seed=seed)
+    plist = [Sphere(p.position, p.refractive_index, p.radius, 3, 3) for p in pp]
+    # the table of the full ensemble covers every distance of the sample
+    loop = oc.RadialLookupCPUReferenceLoop(table, rho, plist, 'cubic')
+    n_unk = 30 * n_sample
+    v = np.random.default_rng(1).standard_normal(n_unk) + 0j
+    rows, t_used = [], 0.0
+    t0 = time.time()
+    for n1 in range(30):
+        loop.matvec(v, n1_subset=[n1])
+        rows.append(n1)
+        t_used = time.time() - t0
+        if t_used > budget_s:
+            break
+    t_matvec_sample = t_used * 30 / len(rows)
+    t_matvec_full = t_matvec_sample * (n_full / n_sample) ** 2
+    t_solve = t_matvec_full * max(iters_per_solve, 1)
+    return dict(value=1.0 / t_solve, unit='RHS-solves/s', cores=1, kind='port',
+                sample=('smuthi CPU radial-lookup matvec restated (oracle RadialLookupCPUReferenceLoop, scipy '
+                        'interp1d cubic), %d of 30 row groups of a %d-sphere sample at the same density, %.1f s; '
+                        'scaled by (N/N_sample)^2 to %.1f s per matvec and by %d GMRES iterations per solve'
+                        % (len(rows), n_sample, t_used, t_matvec_full, iters_per_solve)),
+                matvec_seconds_est=t_matvec_full)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument('--gpus', type=int, default=1)
+    ap.add_argument('--steps', type=int, default=3)
+    ap.add_argument('--warmup', type=int, default=1)
+    ap.add_argument('--particles', type=int, default=10000)
+    ap.add_argument('--tol', type=float, default=1e-6)
+    ap.add_argument('--no-cpu-baseline', action='store_true')
+    ap.add_argument('--matvec-iters', type=int, default=5)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print('bench.py measures one GPU: this build has no multi-GPU path (run with --gpus 1)', file=sys.stderr)
+        sys.exit(2)
+
+    from smuthi_amd import cuda
+    from smuthi_amd.initial_field import PlaneWave
+    from smuthi_amd.linear_system import LinearSystem
+    if not cuda.enable_gpu():
+        raise SystemExit('no MI355X device')
+    dev, n_cu = cuda.device_info()
+
+    plist, ls, kp, side = build_workload(args.particles)
+    log('workload: %d spheres, side %.0f nm, N_kappa %d' % (len(plist), side, len(kp)))
+
+    def make_system(j, precision, tol):
+        polar, az, pol = rhs_for_step(j)
+        return LinearSystem(plist, PlaneWave(WL, polar, az, pol), ls, kp, solver_type='gmres', solver_tolerance=tol,
+                            coupling_matrix_lookup_resolution=2.0, interpolator_kind='cubic',
+                            precision=precision)
+
+    t0 = time.time()
+    base = make_system(0, 'mixed', args.tol)
+    base.prepare()
+    prep_s = time.time() - t0
+    op = base.coupling_matrix
+    log('prepare %.2f s (table %.2f s), len_rho %d' % (prep_s, op.table_seconds, len(op.radial_distance_array)))
+
+    # right-hand sides of all steps (setup, outside the timed region)
+    rhs = []
+    for j in range(args.warmup + args.steps):
+        s = make_system(j, 'mixed', args.tol)
+        s.compute_initial_field_coefficients()
+        s.compute_t_matrix()
+        rhs.append(s.t_matrix.right_hand_side())
+    base.compute_initial_field_coefficients()  # restore step-0 fields on the particles
+
+    # coupling matvec kernel timing (HIP events on the operator's stream)
+    k32, tot32 = op.bench_matvec('single', iters=args.matvec_iters)
+    k64, tot64 = op.bench_matvec('double', iters=max(2, args.matvec_iters // 2))
+    n_unk = op.shape[0]
+    E = float(n_unk) ** 2
+    flops_alg = E * (4 * 4 + 8 * 1)  # cubic: 4 taps per complex entry, one RHS
+    log('matvec FP32 %.2f ms, FP64 %.2f ms' % (k32, k64))
+
+    # FP64 reference for step 0 (plain FP64 GMRES) and its throughput
+    ref = make_system(0, 'double', 1e-9)
+    ref.t_matrix = base.t_matrix
+    ref.coupling_matrix = op
+    t0 = time.time()
+    x_ref = ref.solve(rhs[0][None, :])[0]
+    fp64_ref_s = time.time() - t0
+    ref_stats = dict(ref.solve_stats)
+    log('FP64 reference solve (tol 1e-9): %.2f s, %d iterations' % (fp64_ref_s, ref_stats['iterations']))
+    plain = make_system(0, 'double', args.tol)
+    plain.t_matrix = base.t_matrix
+    plain.coupling_matrix = op
+    t0 = time.time()
+    plain.solve(rhs[0][None, :])
+    fp64_plain_s = time.time() - t0
+    plain_stats = dict(plain.solve_stats)
+    log('plain FP64 GMRES (tol %.0e): %.2f s, %d iterations' % (args.tol, fp64_plain_s, plain_stats['iterations']))
+
+    solver = make_system(0, 'mixed', args.tol)
+    solver.t_matrix = base.t_matrix
+    solver.coupling_matrix = op
+    for j in range(args.warmup):
+        solver.solve(rhs[j][None, :])
+    t0 = time.time()
+    sols, stats = [], []
+    for j in range(args.warmup, args.warmup + args.steps):
+        sols.append(solver.solve(rhs[j][None, :])[0])
+        stats.append(dict(solver.solve_stats))
+    elapsed = time.time() - t0
+    # error of the mixed-precision solution vs the FP64 solution (same RHS as step 0)
+    x_mixed0 = solver.solve(rhs[0][None, :])[0]
+    coef_err = float(np.linalg.norm(x_mixed0 - x_ref) / np.linalg.norm(x_ref))
+    worst_res = max(s['relative_residual'] for s in stats)
+    ok = coef_err <= 1e-5 and worst_res <= args.tol
+    value = args.steps / elapsed if ok else 0.0
+    log('mixed solves: %.2f s for %d steps, residual %.2e, coefficient error vs FP64 %.2e'
+        % (elapsed, args.steps, worst_res, coef_err))
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        t0 = time.time()
+        cpu = cpu_baseline(op.lookup_table, op.radial_distance_array, len(plist), plain_stats['iterations'])
+        log('cpu baseline %.1f s' % (time.time() - t0))
+
+    achieved = flops_alg / (k32 * 1e-3) / 1e12
+    out = {
+        'metric': 'RHS-solves/s at 10k spheres, err<=1e-5 vs FP64; coupling matvec GB/s',
+        'value': value,
+        'unit': 'RHS-solves/s',
+        'n_gpus': 1,
+        'steps': args.steps,
+        'warmup': args.warmup,
+        'ms_per_step': elapsed / args.steps * 1e3,
+        'higher_is_better': True,
+        'scaling': 'weak',
+        'vs_baseline': None,
+        'dtype': 'fp32 coupling matvec (MFMA), fp64 refinement/Arnoldi/residual',
+        'data': 'synthetic',
+        'config': {
+            'workload': ('radial-lookup coupling solve, %d spheres monolayer on glass, l_max=3, cubic spline 2 nm, '
+                         'GMRES tol %.0e, FP32-in-FP64 refinement' % (len(plist), args.tol)),
+            'particles': len(plist), 'unknowns': n_unk, 'lookup_rows': int(len(op.radial_distance_array)),
+            'n_kappa': int(len(kp)), 'side_nm': side, 'seed': 2, 'tolerance': args.tol,
+            'interpolation': 'cubic (not-a-knot spline)', 'resolution_nm': 2.0,
+        },
+        'coupling_matvec_gbps_equiv_dense': E * 8 / (k32 * 1e-3) / 1e9,
+        'matvec_ms': {'fp32_kernel': k32, 'fp32_total': tot32, 'fp64_kernel': k64, 'fp64_total': tot64},
+        'coef_err_vs_fp64': coef_err,
+        'worst_relative_residual': worst_res,
+        'solve_stats': stats,
+        'fp64_plain_gmres': {'seconds': fp64_plain_s, 'rhs_solves_per_s': 1.0 / fp64_plain_s, **plain_stats},
+        'speedup_vs_fp64_gmres': (value * fp64_plain_s) if value else None,
+        'prepare_seconds': prep_s,
+        'table_seconds': op.table_seconds,
+        'roofline': {'bound': 'mfma', 'achieved': achieved, 'peak': MFMA_FP32_PEAK_TFLOPS, 'unit': 'TFLOP/s',
+                     'frac': achieved / MFMA_FP32_PEAK_TFLOPS, 'traffic': None,
+                     'kernel': 'radial_matvec_kernel<float,3,3,4>',
+                     'flops_per_launch': flops_alg},
+        'cpu_baseline': cpu,
+        'device': dev, 'compute_units': n_cu, 'host': platform.node(), 'host_cpus': os.cpu_count(),
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == '__main__':
+    main()

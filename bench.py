@@ -15,6 +15,7 @@ GMRES solve (tol 1e-9) of the same right-hand side.
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--particles N]
 """
 import argparse
+import contextlib
 import json
 import os
 import platform
@@ -221,8 +222,13 @@ def main():
         'cpu_baseline': cpu,
         'device': dev, 'compute_units': n_cu, 'host': platform.node(), 'host_cpus': os.cpu_count(),
     }
-    print(json.dumps(out), flush=True)
+    return json.dumps(out)
 
 
 if __name__ == '__main__':
-    main()
+    # the drop-in LinearSystem reports 'Number of unknowns' on stdout like smuthi does;
+    # keep stdout to the one JSON line
+    _stdout = sys.stdout
+    with contextlib.redirect_stdout(sys.stderr):
+        _line = main()
+    print(_line, file=_stdout, flush=True)

@@ -83,6 +83,30 @@ class _GpuCouplingOperator(SystemMatrix):
                    cu.context(), 'smh_op_bench_matvec')
         return k.value, t.value
 
+    def solve_master(self, t_diagonal, rhs, tol, precision='mixed', restart=20, maxiter=None, with_stats=False):
+        """Solve (1 - diag(t) W) x = rhs on the GPU (rhs [n] or [nrhs, n]).
+
+        Raises SmhError when GMRES misses the tolerance unless with_stats is set,
+        in which case (x, stats, converged) is returned."""
+        b = np.ascontiguousarray(rhs, dtype=np.complex128)
+        t = np.ascontiguousarray(t_diagonal, dtype=np.complex128)
+        if t.shape != (self.shape[0],) or b.shape[-1] != self.shape[0]:
+            raise ValueError('t_diagonal and rhs must have the operator dimension %d' % self.shape[0])
+        nrhs = 1 if b.ndim == 1 else b.shape[0]
+        x = np.empty_like(b)
+        stats = np.zeros(8)
+        mode = _lib.SOLVE_MIXED if precision == 'mixed' else _lib.SOLVE_FP64
+        maxiter = 10 * self.shape[0] if maxiter is None else maxiter
+        maxiter = int(min(maxiter, 2 ** 31 - 1))
+        code = _lib.load().smh_solve(self._op, _lib.dptr(t), _lib.dptr(b), _lib.dptr(x), nrhs, float(tol),
+                                     int(restart), maxiter, mode, _lib.dptr(stats))
+        converged = code != _lib.SMH_ERR_NOCONV
+        if not with_stats or converged:
+            _lib.check(code, cu.context(), 'smh_solve')
+        st = dict(iterations=int(stats[0]), matvecs_fp64=int(stats[1]), matvecs_fp32=int(stats[2]),
+                  relative_residual=float(stats[3]), seconds=float(stats[4]), sweeps=int(stats[5]))
+        return (x, st, converged) if with_stats else x
+
     def _make_linear_operator(self):
         self.linear_operator = scipy.sparse.linalg.LinearOperator(shape=self.shape, matvec=self.matvec,
                                                                   dtype=complex)
@@ -302,23 +326,13 @@ class LinearSystem:
         if self.solver_type.lower() not in ('gmres', 'lu'):
             raise ValueError('This solver type is currently not implemented.')
         single = rhs is None
-        b = np.ascontiguousarray(self.t_matrix.right_hand_side() if single else rhs, dtype=np.complex128)
-        nrhs = 1 if b.ndim == 1 else b.shape[0]
-        x = np.empty_like(b)
-        stats = np.zeros(8)
-        mode = _lib.SOLVE_MIXED if self.precision == 'mixed' else _lib.SOLVE_FP64
-        maxiter = self.maxiter if self.maxiter is not None else 10 * self.shape[0]
-        maxiter = int(min(maxiter, 2 ** 31 - 1))
-        code = _lib.load().smh_solve(self.coupling_matrix._op, _lib.dptr(self.t_matrix.diagonal), _lib.dptr(b),
-                                     _lib.dptr(x), nrhs, float(self.solver_tolerance), int(self.restart), maxiter,
-                                     mode, _lib.dptr(stats))
-        if code == _lib.SMH_ERR_NOCONV:
+        b = self.t_matrix.right_hand_side() if single else rhs
+        x, self.solve_stats, converged = self.coupling_matrix.solve_master(
+            self.t_matrix.diagonal, b, self.solver_tolerance, precision=self.precision, restart=self.restart,
+            maxiter=self.maxiter, with_stats=True)
+        if not converged:
             warnings.warn('GMRES did not reach the tolerance: %s'
                           % _lib.load().smh_last_error(cu.context()).decode(errors='replace'))
-        else:
-            _lib.check(code, cu.context(), 'smh_solve')
-        self.solve_stats = dict(iterations=int(stats[0]), matvecs_fp64=int(stats[1]), matvecs_fp32=int(stats[2]),
-                                relative_residual=float(stats[3]), seconds=float(stats[4]), sweeps=int(stats[5]))
         if not single:
             return x
         off = 0

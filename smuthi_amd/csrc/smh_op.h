@@ -25,24 +25,25 @@ struct smh_op {
   bool has_table = false;
   double2* d_table = nullptr;        // FP64 table [len_rho][(len_z)][B][B]
   double2* d_table_minus = nullptr;  // volume: minus table
-  double* d_tt64 = nullptr;          // tap rows [n_tt][MP][B]
+  double* d_tt64 = nullptr;          // tap table [n_tt][FR] in MFMA fragment order
   float* d_tt32 = nullptr;
+  int FR = 0;                        // reals per tap row
+  float* d_rowsum = nullptr;         // per tap row: max_n1 sum_n2 |Re| + |Im|
+  float* d_xmax = nullptr;           // largest |Re| + |Im| of the current FP32 input
 
-  // pair binning
-  int Rc = 128, n_chunks = 0, n_items = 0;
+  // pair binning, one per precision (index 0: FP32, 1: FP64); the receiver
+  // chunk size follows the LDS accumulator of that precision
+  struct Binning {
+    int Rc = 0, n_chunks = 0, n_items = 0, n_entries = 0;
+    void* d_rec = nullptr;           // PairRec<Real> sorted by (chunk, bin)
+    int4* d_entries = nullptr;       // (first tap row, first pair, pairs, -)
+    int2* d_items = nullptr;         // (chunk, first entry)
+    int* d_chunk_items = nullptr;    // items of chunk c: [chunk_items[c], chunk_items[c+1])
+    float* d_item_lsum = nullptr;    // per item: largest tap-row sum over its bins
+    int* d_item_en = nullptr;        // per item: ceil(log2) of the most pairs of one receiver
+    size_t lds = 0;
+  } bin[2];
   int64_t n_pairs = 0;
-  uint2* d_pairs = nullptr;   // (i1, i2) while binning; released once the records exist
-  void* d_rec32 = nullptr;     // PairRec<float> per binned pair
-  void* d_rec64 = nullptr;     // PairRec<double> per binned pair
-  int* d_off = nullptr;
-  int4* d_items = nullptr;       // (chunk, bin begin, bin end) per work item
-  int4* d_items32 = nullptr;     // (chunk, first period, end period) for the FP32 kernel
-  int4* d_items64 = nullptr;
-  int4* d_periods32 = nullptr;   // (first bin, bins, last non-empty bin, tiles)
-  int4* d_periods64 = nullptr;
-  int* d_chunk_items = nullptr;
-  int S32 = 1, S64 = 1, rows_cap32 = 0, rows_cap64 = 0;
-  size_t lds32 = 0, lds64 = 0;
 
   // work buffers
   void* d_ypart = nullptr;

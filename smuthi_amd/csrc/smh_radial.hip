@@ -37,9 +37,10 @@ __global__ void pair_bin_kernel(const double2* __restrict__ xy, int n, double rh
   }
 }
 
-// Geometry of every binned pair in FP64, stored per precision.
+// Geometry of every binned pair in FP64, stored in the kernel's precision.
+template <typename Real>
 __global__ void pair_record_kernel(const uint2* __restrict__ pairs, int64_t n_pairs, const double2* __restrict__ xy,
-                                   double rho_min, double h, int Rc, PairRec<float>* r32, PairRec<double>* r64) {
+                                   double rho_min, double h, int Rc, PairRec<Real>* rec) {
   const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= n_pairs) return;
   const uint2 pr = pairs[p];
@@ -47,14 +48,17 @@ __global__ void pair_record_kernel(const uint2* __restrict__ pairs, int64_t n_pa
   int r;
   double w, rho;
   pair_geometry(dx, dy, rho_min, h, r, w, rho);
-  double c = 1.0, s = 0.0;
+  double c = 1.0, sn = 0.0;
   if (rho > 0.0) {
     c = dx / rho;
-    s = dy / rho;
+    sn = dy / rho;
   }
-  const uint32_t ij = pr.y | ((pr.x % (unsigned)Rc) << 24);
-  r32[p] = PairRec<float>{ij, (float)w, (float)c, (float)s};
-  r64[p] = PairRec<double>{ij, 0u, w, c, s};
+  PairRec<Real> out{};
+  out.ij = pr.y | ((pr.x % (unsigned)Rc) << kPairShift);
+  out.w = (Real)w;
+  out.c = (Real)c;
+  out.s = (Real)sn;
+  rec[p] = out;
 }
 
 template <typename Real>
@@ -79,8 +83,8 @@ __global__ void pack_kernel(const typename Vec2<Real>::t* __restrict__ pad, cons
 
 // Sum the work-item accumulators of each receiver chunk in fixed order.
 template <typename Real>
-__global__ void reduce_kernel(const Real* __restrict__ ypart, const int* __restrict__ chunk_items, int Rc, int Bh,
-                              int B, int64_t n_part, typename Vec2<Real>::t* y) {
+__global__ void reduce_kernel(const Real* __restrict__ ypart, const int* __restrict__ chunk_items, int Rc, int B,
+                              int64_t n_part, typename Vec2<Real>::t* y) {
   const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= n_part * B) return;
   const int64_t i1 = j / B;
@@ -88,23 +92,35 @@ __global__ void reduce_kernel(const Real* __restrict__ ypart, const int* __restr
   const int c = (int)(i1 / Rc), il = (int)(i1 - (int64_t)c * Rc);
   Real re = 0, im = 0;
   for (int w = chunk_items[c]; w < chunk_items[c + 1]; ++w) {
-    const Real* p = ypart + ((size_t)w * Rc + il) * 2 * Bh;
+    const Real* p = ypart + ((size_t)w * Rc + il) * 2 * B;
     re += p[n1];
-    im += p[Bh + n1];
+    im += p[B + n1];
   }
   y[j] = typename Vec2<Real>::t{re, im};
 }
 
-// Tap rows [n_tt][MP][B]: Re rows 0..B-1, Im rows Bh..Bh+B-1, zero elsewhere.
-// Spline taps interleave values and moments (row 2j: y_j, row 2j+1: Mh_j).
+// Tap table in MFMA fragment order: row j holds reals
+//   F[i][h][g][rr][e] = A_j(16 i + rr, n2 = h KS + 4 g + e),  i < MT, h < G, g < NG,
+// where A_j rows [0, Bh) are Re L_j(n1, n2) and rows [Bh, 2 Bh) are Im L_j
+// (zero outside n1 < B, n2 < B, 4 g + e < KS). Spline taps interleave values
+// and moments (tap row 2k: y_k, row 2k+1: Mh_k).
 __global__ void taps_kernel(const double2* __restrict__ table, const double2* __restrict__ mh, int64_t len, int B,
-                            int Bh, int MP, int interleave, double* tt64, float* tt32) {
+                            int Bh, int G, int KS, int NG, int FR, int interleave, double* tt64, float* tt32) {
   const int64_t n_tt = interleave ? 2 * len : len;
-  const int64_t total = n_tt * MP * B;
-  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
-    const int64_t row = e / ((int64_t)MP * B);
-    const int rem = (int)(e - row * MP * B);
-    const int m = rem / B, n2 = rem - m * B;
+  const int64_t total = n_tt * FR;
+  for (int64_t f = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; f < total; f += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t row = f / FR;
+    int rem = (int)(f - row * FR);
+    const int e = rem & 3;
+    const int rr = (rem >> 2) & 15;
+    rem >>= 6;
+    const int g = rem % NG;
+    rem /= NG;
+    const int hh = rem % G;
+    const int i = rem / G;
+    const int m = 16 * i + rr;
+    const int k = 4 * g + e;
+    const int n2 = hh * KS + k;
     const double2* src = table;
     int64_t j = row;
     if (interleave) {
@@ -112,13 +128,105 @@ __global__ void taps_kernel(const double2* __restrict__ table, const double2* __
       if (row & 1) src = mh;
     }
     double v = 0.0;
-    if (m < B)
-      v = src[(j * B + m) * B + n2].x;
-    else if (m >= Bh && m < Bh + B)
-      v = src[(j * B + (m - Bh)) * B + n2].y;
-    tt64[e] = v;
-    tt32[e] = (float)v;
+    if (k < KS && n2 < B) {
+      if (m < B)
+        v = src[(j * B + m) * B + n2].x;
+      else if (m >= Bh && m < Bh + B)
+        v = src[(j * B + (m - Bh)) * B + n2].y;
+    }
+    tt64[f] = v;
+    tt32[f] = (float)v;
   }
+}
+
+// Row sums of the tap rows: rowsum[row] = max_n1 sum_n2 (|Re| + |Im|), with a
+// small margin for the FP32 rounding of the tap table.
+__global__ void tap_rowsum_kernel(const double2* __restrict__ table, const double2* __restrict__ mh, int64_t len,
+                                  int B, int interleave, float* rowsum) {
+  const int64_t n_tt = interleave ? 2 * len : len;
+  for (int64_t row = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; row < n_tt;
+       row += (int64_t)gridDim.x * blockDim.x) {
+    const double2* src = table;
+    int64_t j = row;
+    if (interleave) {
+      j = row >> 1;
+      if (row & 1) src = mh;
+    }
+    double best = 0.0;
+    for (int n1 = 0; n1 < B; ++n1) {
+      double sum = 0.0;
+      for (int n2 = 0; n2 < B; ++n2) {
+        const double2 v = src[(j * B + n1) * B + n2];
+        sum += fabs(v.x) + fabs(v.y);
+      }
+      best = fmax(best, sum);
+    }
+    rowsum[row] = (float)(best * 1.001);
+  }
+}
+
+// Per work item: largest row sum over the tap rows its bin entries use.
+__global__ void item_lsum_kernel(const int4* __restrict__ entries, const int2* __restrict__ items, int n_items,
+                                 int n_entries, int T, const float* __restrict__ rowsum, float* item_lsum) {
+  const int k = blockIdx.x;
+  const int e0 = items[k].y, e1 = k + 1 < n_items ? items[k + 1].y : n_entries;
+  float m = 0.f;
+  for (int e = e0 + (int)threadIdx.x; e < e1; e += blockDim.x)
+    for (int t = 0; t < T; ++t) m = fmaxf(m, rowsum[entries[e].x + t]);
+  __shared__ float red[256];
+  red[threadIdx.x] = m;
+  __syncthreads();
+  for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) red[threadIdx.x] = fmaxf(red[threadIdx.x], red[threadIdx.x + w]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) item_lsum[k] = red[0];
+}
+
+// Per work item: eN = ceil(log2) of the most pairs a single receiver has among
+// the item's pairs (a contiguous range of the records).
+template <typename Real>
+__global__ void item_count_kernel(const PairRec<Real>* __restrict__ recs, const int4* __restrict__ entries,
+                                  const int2* __restrict__ items, int n_items, int n_entries, int* item_en) {
+  __shared__ int hist[512];
+  __shared__ int red[256];
+  const int k = blockIdx.x;
+  const int e0 = items[k].y, e1 = (k + 1 < n_items ? items[k + 1].y : n_entries) - 1;
+  for (int j = threadIdx.x; j < 512; j += blockDim.x) hist[j] = 0;
+  __syncthreads();
+  const int p0 = entries[e0].y, p1 = entries[e1].y + entries[e1].z;
+  for (int p = p0 + (int)threadIdx.x; p < p1; p += blockDim.x) atomicAdd(&hist[recs[p].ij >> kPairShift], 1);
+  __syncthreads();
+  int m = 0;
+  for (int j = threadIdx.x; j < 512; j += blockDim.x) m = max(m, hist[j]);
+  red[threadIdx.x] = m;
+  __syncthreads();
+  for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) red[threadIdx.x] = max(red[threadIdx.x], red[threadIdx.x + w]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    int eN = 1;
+    while ((1 << eN) < red[0]) ++eN;
+    item_en[k] = eN;
+  }
+}
+
+// Largest |Re x| + |Im x| of the input (one workgroup).
+__global__ void xmax_kernel(const float2* __restrict__ x, int64_t n, float* xmax) {
+  __shared__ float red[1024];
+  float m = 0.f;
+  for (int64_t j = threadIdx.x; j < n; j += blockDim.x) {
+    const float2 v = x[j];
+    m = fmaxf(m, fabsf(v.x) + fabsf(v.y));
+  }
+  red[threadIdx.x] = m;
+  __syncthreads();
+  for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) red[threadIdx.x] = fmaxf(red[threadIdx.x], red[threadIdx.x + w]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *xmax = red[0];
 }
 
 // Not-a-knot cubic spline moments, scaled: Mh_j = h^2/6 S''(rho_j), one
@@ -171,17 +279,26 @@ __global__ void spline_kernel(const double2* __restrict__ y, const double* __res
   mh[(n - 1) * BB + col] = make_double2(2.0 * mn2.x - mn3.x, 2.0 * mn2.y - mn3.y);
 }
 
+struct KernelInfo {
+  void* fn = nullptr;
+  int threads = 0;
+  int sy = 0;  // accumulator stride per receiver
+};
+
 template <int L, int Mx, int T, typename Real>
-void* kernel_ptr() {
-  return reinterpret_cast<void*>(&radial_matvec_kernel<Real, L, Mx, T>);
+KernelInfo kernel_info() {
+  using Sh = RadialShape<Real, SvwfIdx<L, Mx>::B, T>;
+  KernelInfo k;
+  k.fn = reinterpret_cast<void*>(&radial_matvec_kernel<Real, L, Mx, T>);
+  k.threads = Sh::kThreads;
+  k.sy = Sh::SY;
+  return k;
 }
 
-typedef void* (*KernelGetter)();
-
 template <int T, typename Real>
-void* pick(int l, int m) {
+KernelInfo pick(int l, int m) {
 #define SMH_K(L, Mx) \
-  if (l == L && m == Mx) return kernel_ptr<L, Mx, T, Real>();
+  if (l == L && m == Mx) return kernel_info<L, Mx, T, Real>();
   SMH_K(1, 1)
   SMH_K(2, 1)
   SMH_K(2, 2)
@@ -193,10 +310,10 @@ void* pick(int l, int m) {
   SMH_K(4, 3)
   SMH_K(4, 4)
 #undef SMH_K
-  return nullptr;
+  return KernelInfo{};
 }
 
-void* radial_kernel(int precision, int T, int l, int m) {
+KernelInfo radial_kernel(int precision, int T, int l, int m) {
   if (precision == SMH_FP32) return T == 4 ? pick<4, float>(l, m) : pick<2, float>(l, m);
   return T == 4 ? pick<4, double>(l, m) : pick<2, double>(l, m);
 }
@@ -265,12 +382,26 @@ int op_make_taps(smh_op* op) {
     hipLaunchKernelGGL(spline_kernel, dim3(blocks_for(BB, 64)), dim3(64), 0, s, op->d_table, d_cp, len, BB, d_mh);
     SMH_HIP(ctx, hipGetLastError());
   }
+  const TapGeom g(op->B, op->T);
+  op->FR = g.FR;
   if (!op->d_tt64) {
-    SMH_HIP(ctx, hipMalloc(&op->d_tt64, (size_t)op->n_tt * op->MP * op->B * sizeof(double)));
-    SMH_HIP(ctx, hipMalloc(&op->d_tt32, (size_t)op->n_tt * op->MP * op->B * sizeof(float)));
+    SMH_HIP(ctx, hipMalloc(&op->d_tt64, (size_t)op->n_tt * g.FR * sizeof(double)));
+    SMH_HIP(ctx, hipMalloc(&op->d_tt32, (size_t)op->n_tt * g.FR * sizeof(float)));
   }
-  hipLaunchKernelGGL(taps_kernel, dim3(4096), dim3(256), 0, s, op->d_table, d_mh, len, op->B, op->Bh, op->MP,
-                     op->interp == SMH_INTERP_SPLINE ? 1 : 0, op->d_tt64, op->d_tt32);
+  hipLaunchKernelGGL(taps_kernel, dim3(4096), dim3(256), 0, s, op->d_table, d_mh, len, op->B, op->Bh, g.G, g.KS, g.NG,
+                     g.FR, op->interp == SMH_INTERP_SPLINE ? 1 : 0, op->d_tt64, op->d_tt32);
+  if (!op->d_rowsum) {
+    SMH_HIP(ctx, hipMalloc(&op->d_rowsum, (size_t)op->n_tt * sizeof(float)));
+    SMH_HIP(ctx, hipMalloc(&op->d_xmax, sizeof(float)));
+  }
+  hipLaunchKernelGGL(tap_rowsum_kernel, dim3(blocks_for(op->n_tt)), dim3(256), 0, s, op->d_table, d_mh, len, op->B,
+                     op->interp == SMH_INTERP_SPLINE ? 1 : 0, op->d_rowsum);
+  {
+    smh_op::Binning& bn = op->bin[0];
+    if (bn.n_items > 0)
+      hipLaunchKernelGGL(item_lsum_kernel, dim3(bn.n_items), dim3(256), 0, s, bn.d_entries, bn.d_items, bn.n_items,
+                         bn.n_entries, op->T, op->d_rowsum, bn.d_item_lsum);
+  }
   SMH_HIP(ctx, hipGetLastError());
   SMH_HIP(ctx, hipStreamSynchronize(s));
   if (d_mh) hipFree(d_mh);
@@ -283,45 +414,46 @@ int op_apply(smh_op* op, int precision, const void* x_pad, void* y_pad, bool tim
   smh_ctx* ctx = op->ctx;
   SMH_CHECK(ctx, op->has_table, SMH_ERR_STATE, "coupling table not set");
   SMH_CHECK(ctx, op->kind == 0, SMH_ERR_UNSUPPORTED, "operator kind %d has no MFMA matvec here", op->kind);
-  const int S = precision == SMH_FP32 ? op->S32 : op->S64;
-  SMH_CHECK(ctx, S >= 1, SMH_ERR_UNSUPPORTED,
-            "no %s kernel fits LDS for blocksize %d (l_max %d, m_max %d)", precision == SMH_FP32 ? "FP32" : "FP64",
-            op->B, op->l_max, op->m_max);
-  void* k = radial_kernel(precision, op->T, op->l_max, op->m_max);
-  SMH_CHECK(ctx, k != nullptr, SMH_ERR_UNSUPPORTED, "no matvec kernel for l_max %d m_max %d", op->l_max,
+  const int pi = precision == SMH_FP32 ? 0 : 1;
+  const smh_op::Binning& bn = op->bin[pi];
+  SMH_CHECK(ctx, bn.Rc > 0, SMH_ERR_UNSUPPORTED, "no %s kernel fits LDS for blocksize %d (l_max %d, m_max %d)",
+            pi == 0 ? "FP32" : "FP64", op->B, op->l_max, op->m_max);
+  const KernelInfo k = radial_kernel(precision, op->T, op->l_max, op->m_max);
+  SMH_CHECK(ctx, k.fn != nullptr, SMH_ERR_UNSUPPORTED, "no matvec kernel for l_max %d m_max %d", op->l_max,
             op->m_max);
   RadialKArgs a;
-  a.recs = precision == SMH_FP32 ? op->d_rec32 : op->d_rec64;
-  a.off = op->d_off;
-  a.items = precision == SMH_FP32 ? op->d_items32 : op->d_items64;
-  a.periods = precision == SMH_FP32 ? op->d_periods32 : op->d_periods64;
-  a.tt = precision == SMH_FP32 ? (const void*)op->d_tt32 : (const void*)op->d_tt64;
+  a.recs = bn.d_rec;
+  a.entries = bn.d_entries;
+  a.items = bn.d_items;
+  a.tt = pi == 0 ? (const void*)op->d_tt32 : (const void*)op->d_tt64;
   a.x = x_pad;
   a.ypart = op->d_ypart;
-  a.len_rho = (int)op->len_rho;
-  a.s_mode = op->s_mode;
-  a.o_mode = op->o_mode;
+  a.xmax = op->d_xmax;
+  a.item_lsum = bn.d_item_lsum;
+  a.item_en = bn.d_item_en;
+  a.n_items = bn.n_items;
+  a.n_entries = bn.n_entries;
   a.interp = op->interp;
-  a.Rc = op->Rc;
-  a.rows_cap = precision == SMH_FP32 ? op->rows_cap32 : op->rows_cap64;
-  const size_t lds = precision == SMH_FP32 ? op->lds32 : op->lds64;
+  a.Rc = bn.Rc;
   hipStream_t s = ctx->stream;
-  SMH_HIP(ctx, hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-  if (op->n_items > 0) {
+  SMH_HIP(ctx, hipFuncSetAttribute(k.fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bn.lds));
+  if (pi == 0)
+    hipLaunchKernelGGL(xmax_kernel, dim3(1), dim3(1024), 0, s, (const float2*)x_pad, op->n_part * op->B, op->d_xmax);
+  if (bn.n_items > 0) {
     if (time_kernel) SMH_HIP(ctx, hipEventRecord(ctx->ev0, s));
     void* args[] = {&a};
-    SMH_HIP(ctx, hipLaunchKernel(k, dim3(op->n_items), dim3(kThreads), args, lds, s));
+    SMH_HIP(ctx, hipLaunchKernel(k.fn, dim3(bn.n_items), dim3(k.threads), args, bn.lds, s));
     if (time_kernel) SMH_HIP(ctx, hipEventRecord(ctx->ev1, s));
   }
   const int64_t n = op->n_part * op->B;
-  if (precision == SMH_FP32)
+  if (pi == 0)
     hipLaunchKernelGGL(reduce_kernel<float>, dim3(blocks_for(n)), dim3(256), 0, s, (const float*)op->d_ypart,
-                       op->d_chunk_items, op->Rc, op->Bh, op->B, op->n_part, (float2*)y_pad);
+                       bn.d_chunk_items, bn.Rc, op->B, op->n_part, (float2*)y_pad);
   else
     hipLaunchKernelGGL(reduce_kernel<double>, dim3(blocks_for(n)), dim3(256), 0, s, (const double*)op->d_ypart,
-                       op->d_chunk_items, op->Rc, op->Bh, op->B, op->n_part, (double2*)y_pad);
+                       bn.d_chunk_items, bn.Rc, op->B, op->n_part, (double2*)y_pad);
   SMH_HIP(ctx, hipGetLastError());
-  if (time_kernel && op->n_items > 0) {
+  if (time_kernel && bn.n_items > 0) {
     SMH_HIP(ctx, hipEventSynchronize(ctx->ev1));
     float ms = 0;
     SMH_HIP(ctx, hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
@@ -332,33 +464,180 @@ int op_apply(smh_op* op, int precision, const void* x_pad, void* y_pad, bool tim
 
 }  // namespace smh
 
-static int lds_plan(smh_op* op, size_t real_size, int& rows_cap, int& S, size_t& lds) {
-  const size_t limit = op->ctx->lds_per_block ? op->ctx->lds_per_block : 65536;
-  const size_t yacc = (size_t)op->Rc * 2 * (op->Bh + 1) * real_size;
-  const size_t row = (size_t)op->MP * (op->B + 1) * real_size;
-  const size_t misc = 2 * kMetaInts * sizeof(int) + 64;  // two period descriptors
-  // rows the per-thread staging registers hold: 20 16-byte loads per thread
-  const int stage_rows = (int)((size_t)20 * kThreads * 16 / ((size_t)op->MP * op->B * real_size));
-  rows_cap = 0;
-  S = 0;
-  lds = 0;
-  if (yacc + misc + row * op->T > limit || stage_rows < op->T) return SMH_OK;  // unsupported at this precision
-  rows_cap = (int)((limit - yacc - misc) / row);
-  if (rows_cap > stage_rows) rows_cap = stage_rows;
-  S = (rows_cap - op->T) / op->s_mode + 1;
-  if (S > 48) S = 48;  // tile prefix (S + 1 ints) and the period bounds share 64 ints of LDS
-  rows_cap = op->s_mode * (S - 1) + op->T;
-  lds = rows_cap * row + yacc + misc;
-  return SMH_OK;
+// Receiver chunk of a precision: as many receivers (at most 512, the pair
+// record's receiver field) as the LDS accumulator [Rc][SY] fits in one
+// workgroup's LDS, less 1 KiB.
+static int chunk_size(const smh_op* op, size_t real_size, int sy, size_t& lds) {
+  const size_t limit = std::min<size_t>(op->ctx->lds_per_block ? op->ctx->lds_per_block : 65536, 163840) - 1024;
+  const size_t per = (size_t)sy * real_size;
+  int rc = (int)std::min<size_t>(512, (limit - 16) / per);
+  if (rc < 16) {
+    lds = 0;
+    return 0;
+  }
+  lds = (size_t)rc * per + 16;
+  return rc;
 }
 
 static void op_free(smh_op* op) {
   void* ptrs[] = {op->d_xy, op->d_xyz, op->d_pk2pad, op->d_table, op->d_table_minus, op->d_tt64, op->d_tt32,
-                  op->d_pairs, op->d_rec32, op->d_rec64, op->d_off, op->d_items, op->d_items32, op->d_items64,
-                  op->d_periods32, op->d_periods64, op->d_chunk_items, op->d_ypart, op->d_pk_in, op->d_pk_out,
-                  op->d_x32, op->d_y32, op->d_x64, op->d_y64};
+                  op->d_ypart, op->d_pk_in, op->d_pk_out, op->d_x32, op->d_y32, op->d_x64, op->d_y64,
+                  op->d_rowsum, op->d_xmax};
   for (void* p : ptrs)
     if (p) hipFree(p);
+  for (auto& b : op->bin) {
+    void* bp[] = {b.d_rec, b.d_entries, b.d_items, b.d_chunk_items, b.d_item_lsum, b.d_item_en};
+    for (void* p : bp)
+      if (p) hipFree(p);
+  }
+}
+
+// Bin all ordered pairs by (receiver chunk, table interval) for one precision,
+// write their records and the bin entries / work items the matvec kernel walks.
+template <typename Real>
+static int bin_pairs(smh_op* op, int pi, int Rc) {
+  smh_ctx* ctx = op->ctx;
+  smh_op::Binning& bn = op->bin[pi];
+  hipStream_t s = ctx->stream;
+  const int64_t n_part = op->n_part, len_rho = op->len_rho;
+  bn.Rc = Rc;
+  bn.n_chunks = (int)((n_part + Rc - 1) / Rc);
+  const int64_t nkeys = (int64_t)bn.n_chunks * len_rho;
+  int *d_cnt = nullptr, *d_cur = nullptr, *d_err = nullptr, *d_off = nullptr;
+  uint2* d_pairs = nullptr;
+  auto cleanup = [&]() {
+    if (d_cnt) hipFree(d_cnt);
+    if (d_cur) hipFree(d_cur);
+    if (d_err) hipFree(d_err);
+    if (d_off) hipFree(d_off);
+    if (d_pairs) hipFree(d_pairs);
+  };
+  if (hipMalloc(&d_cnt, (nkeys + 1) * sizeof(int)) != hipSuccess ||
+      hipMalloc(&d_cur, (nkeys + 1) * sizeof(int)) != hipSuccess || hipMalloc(&d_err, sizeof(int)) != hipSuccess ||
+      hipMalloc(&d_off, (nkeys + 1) * sizeof(int)) != hipSuccess) {
+    cleanup();
+    smh_fail(ctx, SMH_ERR_HIP, "device allocation failed (binning)");
+    return SMH_ERR_HIP;
+  }
+  hipMemsetAsync(d_cnt, 0, (nkeys + 1) * sizeof(int), s);
+  hipMemsetAsync(d_cur, 0, (nkeys + 1) * sizeof(int), s);
+  hipMemsetAsync(d_err, 0, sizeof(int), s);
+  const int nb2 = (int)((n_part + 255) / 256);
+  const int64_t nblocks = n_part * nb2;
+  hipLaunchKernelGGL(pair_bin_kernel, dim3((unsigned)nblocks), dim3(256), 0, s, op->d_xy, (int)n_part, op->rho_min,
+                     op->h, (int)len_rho, op->s_mode, op->o_mode, op->T, op->n_tt, Rc, d_cnt, (const int*)nullptr,
+                     d_cur, (uint2*)nullptr, d_err);
+  std::vector<int> cnt(nkeys);
+  int herr = 0;
+  if (hipMemcpyAsync(cnt.data(), d_cnt, nkeys * sizeof(int), hipMemcpyDeviceToHost, s) != hipSuccess ||
+      hipMemcpyAsync(&herr, d_err, sizeof(int), hipMemcpyDeviceToHost, s) != hipSuccess ||
+      hipStreamSynchronize(s) != hipSuccess) {
+    cleanup();
+    smh_fail(ctx, SMH_ERR_HIP, "binning count failed: %s", hipGetErrorString(hipGetLastError()));
+    return SMH_ERR_HIP;
+  }
+  if (herr) {
+    cleanup();
+    smh_fail(ctx, SMH_ERR_ARG, "a particle distance lies outside the lookup table (rho_min %g, h %g, %lld rows)",
+             op->rho_min, op->h, (long long)len_rho);
+    return SMH_ERR_ARG;
+  }
+  std::vector<int> off(nkeys + 1);
+  int64_t acc = 0;
+  for (int64_t k2 = 0; k2 < nkeys; ++k2) {
+    off[k2] = (int)acc;
+    acc += cnt[k2];
+    if (acc >= (int64_t)INT32_MAX) {
+      cleanup();
+      smh_fail(ctx, SMH_ERR_UNSUPPORTED, "too many particle pairs for 32-bit offsets");
+      return SMH_ERR_UNSUPPORTED;
+    }
+  }
+  off[nkeys] = (int)acc;
+  op->n_pairs = acc;
+  if (hipMalloc(&d_pairs, std::max<int64_t>(acc, 1) * sizeof(uint2)) != hipSuccess ||
+      hipMemcpyAsync(d_off, off.data(), (nkeys + 1) * sizeof(int), hipMemcpyHostToDevice, s) != hipSuccess) {
+    cleanup();
+    smh_fail(ctx, SMH_ERR_HIP, "device allocation failed (pairs)");
+    return SMH_ERR_HIP;
+  }
+  hipLaunchKernelGGL(pair_bin_kernel, dim3((unsigned)nblocks), dim3(256), 0, s, op->d_xy, (int)n_part, op->rho_min,
+                     op->h, (int)len_rho, op->s_mode, op->o_mode, op->T, op->n_tt, Rc, d_cnt, (const int*)d_off,
+                     d_cur, d_pairs, d_err);
+  if (hipMalloc(&bn.d_rec, std::max<int64_t>(acc, 1) * sizeof(PairRec<Real>)) != hipSuccess) {
+    cleanup();
+    smh_fail(ctx, SMH_ERR_HIP, "device allocation failed (pair records, %lld pairs)", (long long)acc);
+    return SMH_ERR_HIP;
+  }
+  if (acc > 0)
+    hipLaunchKernelGGL(pair_record_kernel<Real>, dim3((unsigned)((acc + 255) / 256)), dim3(256), 0, s, d_pairs, acc,
+                       op->d_xy, op->rho_min, op->h, Rc, (PairRec<Real>*)bn.d_rec);
+  if (hipStreamSynchronize(s) != hipSuccess) {
+    cleanup();
+    smh_fail(ctx, SMH_ERR_HIP, "pair records failed");
+    return SMH_ERR_HIP;
+  }
+  cleanup();
+
+  // bin entries (at most kEntryPairs pairs each) grouped into work items of
+  // similar cost; a tile of 16 pairs costs 1, loading a bin's taps 1/4
+  std::vector<int4> entries;
+  std::vector<int> ent_chunk;
+  double total = 0;
+  for (int c = 0; c < bn.n_chunks; ++c)
+    for (int64_t r = 0; r < len_rho; ++r) {
+      const int64_t key = (int64_t)c * len_rho + r;
+      const int n = cnt[key];
+      for (int p0 = 0; p0 < n; p0 += kEntryPairs) {
+        const int np = std::min(kEntryPairs, n - p0);
+        entries.push_back(make_int4((int)(op->s_mode * r + op->o_mode), off[key] + p0, np, 0));
+        ent_chunk.push_back(c);
+        total += (np + 15) / 16 + 0.25;
+      }
+    }
+  const int64_t target_items = std::max<int64_t>(4 * (int64_t)ctx->num_cu, bn.n_chunks);
+  const double budget = std::max(1.0, total / (double)target_items);
+  std::vector<int2> items;
+  std::vector<int> chunk_items(bn.n_chunks + 1, 0);
+  int cur_chunk = -1;
+  double run = 0;
+  for (size_t e = 0; e < entries.size(); ++e) {
+    const int c = ent_chunk[e];
+    if (c != cur_chunk || run >= budget) {
+      while (cur_chunk < c) chunk_items[++cur_chunk] = (int)items.size();
+      items.push_back(make_int2(c, (int)e));
+      run = 0;
+    }
+    run += (entries[e].z + 15) / 16 + 0.25;
+  }
+  while (cur_chunk < bn.n_chunks) chunk_items[++cur_chunk] = (int)items.size();
+  bn.n_entries = (int)entries.size();
+  bn.n_items = (int)items.size();
+  if (hipMalloc(&bn.d_entries, std::max<size_t>(entries.size(), 1) * sizeof(int4)) != hipSuccess ||
+      hipMalloc(&bn.d_items, std::max<size_t>(items.size(), 1) * sizeof(int2)) != hipSuccess ||
+      hipMalloc(&bn.d_chunk_items, chunk_items.size() * sizeof(int)) != hipSuccess ||
+      hipMemcpyAsync(bn.d_entries, entries.data(), entries.size() * sizeof(int4), hipMemcpyHostToDevice, s) !=
+          hipSuccess ||
+      hipMemcpyAsync(bn.d_items, items.data(), items.size() * sizeof(int2), hipMemcpyHostToDevice, s) != hipSuccess ||
+      hipMemcpyAsync(bn.d_chunk_items, chunk_items.data(), chunk_items.size() * sizeof(int), hipMemcpyHostToDevice,
+                     s) != hipSuccess ||
+      hipStreamSynchronize(s) != hipSuccess) {
+    smh_fail(ctx, SMH_ERR_HIP, "device allocation failed (work items)");
+    return SMH_ERR_HIP;
+  }
+  if (hipMalloc(&bn.d_item_lsum, std::max(bn.n_items, 1) * sizeof(float)) != hipSuccess ||
+      hipMalloc(&bn.d_item_en, std::max(bn.n_items, 1) * sizeof(int)) != hipSuccess) {
+    smh_fail(ctx, SMH_ERR_HIP, "device allocation failed (work items)");
+    return SMH_ERR_HIP;
+  }
+  if (bn.n_items > 0)
+    hipLaunchKernelGGL(item_count_kernel<Real>, dim3(bn.n_items), dim3(256), 0, s, (const PairRec<Real>*)bn.d_rec,
+                       bn.d_entries, bn.d_items, bn.n_items, bn.n_entries, bn.d_item_en);
+  if (hipStreamSynchronize(s) != hipSuccess) {
+    smh_fail(ctx, SMH_ERR_HIP, "work item statistics failed");
+    return SMH_ERR_HIP;
+  }
+  return SMH_OK;
 }
 
 extern "C" int smh_radial_create(smh_ctx* ctx, int64_t n_part, const double* xy, const int* lmax, const int* mmax,
@@ -367,12 +646,12 @@ extern "C" int smh_radial_create(smh_ctx* ctx, int64_t n_part, const double* xy,
   if (!ctx || !out) return SMH_ERR_ARG;
   *out = nullptr;
   SMH_CHECK(ctx, n_part >= 1 && xy && lmax && mmax, SMH_ERR_ARG, "need at least one particle");
-  SMH_CHECK(ctx, n_part < (1 << 24), SMH_ERR_ARG, "too many particles (pair records index 24 bits)");
+  SMH_CHECK(ctx, n_part <= (int64_t)kPairMask, SMH_ERR_ARG, "too many particles (pair records index 23 bits)");
   SMH_CHECK(ctx, l_max >= 1 && m_max >= 1 && m_max <= l_max, SMH_ERR_ARG, "invalid l_max %d / m_max %d", l_max,
             m_max);
   SMH_CHECK(ctx, resolution > 0 && len_rho >= 4 && len_rho < (1 << 30), SMH_ERR_ARG, "invalid table grid");
   SMH_CHECK(ctx, interp >= 0 && interp <= 2, SMH_ERR_ARG, "invalid interpolation kind %d", interp);
-  SMH_CHECK(ctx, radial_kernel(SMH_FP32, 4, l_max, m_max) != nullptr, SMH_ERR_UNSUPPORTED,
+  SMH_CHECK(ctx, radial_kernel(SMH_FP32, 4, l_max, m_max).fn != nullptr, SMH_ERR_UNSUPPORTED,
             "no kernel for l_max %d m_max %d", l_max, m_max);
   SMH_HIP(ctx, hipSetDevice(ctx->device));
   smh_op* op = new smh_op();
@@ -458,161 +737,20 @@ extern "C" int smh_radial_create(smh_ctx* ctx, int64_t n_part, const double* xy,
     return fail(SMH_ERR_HIP);
   }
 
-  // ---- pair binning: count, scan, fill -------------------------------------
-  op->Rc = op->Bh <= 32 ? 128 : 64;
-  op->n_chunks = (int)((n_part + op->Rc - 1) / op->Rc);
-  const int64_t nkeys = (int64_t)op->n_chunks * len_rho;
-  int *d_cnt = nullptr, *d_cur = nullptr, *d_err = nullptr;
-  auto cleanup_tmp = [&]() {
-    if (d_cnt) hipFree(d_cnt);
-    if (d_cur) hipFree(d_cur);
-    if (d_err) hipFree(d_err);
-  };
-  if (hipMalloc(&d_cnt, (nkeys + 1) * sizeof(int)) != hipSuccess ||
-      hipMalloc(&d_cur, (nkeys + 1) * sizeof(int)) != hipSuccess || hipMalloc(&d_err, sizeof(int)) != hipSuccess ||
-      hipMalloc(&op->d_off, (nkeys + 1) * sizeof(int)) != hipSuccess) {
-    cleanup_tmp();
-    smh_fail(ctx, SMH_ERR_HIP, "device allocation failed (binning)");
-    return fail(SMH_ERR_HIP);
+  // ---- pair binning per precision
+  size_t ypart = 0;
+  for (int pi = 0; pi < 2; ++pi) {
+    const KernelInfo k = radial_kernel(pi == 0 ? SMH_FP32 : SMH_FP64, op->T, l_max, m_max);
+    const size_t rs = pi == 0 ? sizeof(float) : sizeof(double);
+    size_t lds = 0;
+    const int Rc = chunk_size(op, rs, k.sy, lds);
+    if (Rc == 0) continue;  // no kernel of this precision fits; matvecs in it report UNSUPPORTED
+    op->bin[pi].lds = lds;
+    const int code = pi == 0 ? bin_pairs<float>(op, 0, Rc) : bin_pairs<double>(op, 1, Rc);
+    if (code) return fail(code);
+    ypart = std::max(ypart, (size_t)op->bin[pi].n_items * Rc * 2 * op->B * rs);
   }
-  hipMemsetAsync(d_cnt, 0, (nkeys + 1) * sizeof(int), s);
-  hipMemsetAsync(d_cur, 0, (nkeys + 1) * sizeof(int), s);
-  hipMemsetAsync(d_err, 0, sizeof(int), s);
-  const int nb2 = (int)((n_part + 255) / 256);
-  const int64_t nblocks = n_part * nb2;
-  hipLaunchKernelGGL(pair_bin_kernel, dim3((unsigned)nblocks), dim3(256), 0, s, op->d_xy, (int)n_part, rho_min,
-                     resolution, (int)len_rho, op->s_mode, op->o_mode, op->T, op->n_tt, op->Rc, d_cnt,
-                     (const int*)nullptr, d_cur, (uint2*)nullptr, d_err);
-  std::vector<int> cnt(nkeys);
-  int herr = 0;
-  if (hipMemcpyAsync(cnt.data(), d_cnt, nkeys * sizeof(int), hipMemcpyDeviceToHost, s) != hipSuccess ||
-      hipMemcpyAsync(&herr, d_err, sizeof(int), hipMemcpyDeviceToHost, s) != hipSuccess ||
-      hipStreamSynchronize(s) != hipSuccess) {
-    cleanup_tmp();
-    smh_fail(ctx, SMH_ERR_HIP, "binning count failed: %s", hipGetErrorString(hipGetLastError()));
-    return fail(SMH_ERR_HIP);
-  }
-  if (herr) {
-    cleanup_tmp();
-    smh_fail(ctx, SMH_ERR_ARG, "a particle distance lies outside the lookup table (rho_min %g, h %g, %lld rows)",
-             rho_min, resolution, (long long)len_rho);
-    return fail(SMH_ERR_ARG);
-  }
-  std::vector<int> off(nkeys + 1);
-  int64_t acc = 0;
-  for (int64_t k2 = 0; k2 < nkeys; ++k2) {
-    off[k2] = (int)acc;
-    acc += cnt[k2];
-  }
-  if (acc >= (int64_t)INT32_MAX) {
-    cleanup_tmp();
-    smh_fail(ctx, SMH_ERR_UNSUPPORTED, "too many particle pairs (%lld)", (long long)acc);
-    return fail(SMH_ERR_UNSUPPORTED);
-  }
-  off[nkeys] = (int)acc;
-  op->n_pairs = acc;
-  if (hipMalloc(&op->d_pairs, std::max<int64_t>(acc, 1) * sizeof(uint2)) != hipSuccess ||
-      hipMemcpyAsync(op->d_off, off.data(), (nkeys + 1) * sizeof(int), hipMemcpyHostToDevice, s) != hipSuccess) {
-    cleanup_tmp();
-    smh_fail(ctx, SMH_ERR_HIP, "device allocation failed (pairs)");
-    return fail(SMH_ERR_HIP);
-  }
-  hipLaunchKernelGGL(pair_bin_kernel, dim3((unsigned)nblocks), dim3(256), 0, s, op->d_xy, (int)n_part, rho_min,
-                     resolution, (int)len_rho, op->s_mode, op->o_mode, op->T, op->n_tt, op->Rc, d_cnt,
-                     (const int*)op->d_off, d_cur, op->d_pairs, d_err);
-  if (hipStreamSynchronize(s) != hipSuccess) {
-    cleanup_tmp();
-    smh_fail(ctx, SMH_ERR_HIP, "binning fill failed");
-    return fail(SMH_ERR_HIP);
-  }
-  cleanup_tmp();
-  if (hipMalloc(&op->d_rec32, std::max<int64_t>(acc, 1) * sizeof(PairRec<float>)) != hipSuccess ||
-      hipMalloc(&op->d_rec64, std::max<int64_t>(acc, 1) * sizeof(PairRec<double>)) != hipSuccess) {
-    smh_fail(ctx, SMH_ERR_HIP, "device allocation failed (pair records, %lld pairs)", (long long)acc);
-    return fail(SMH_ERR_HIP);
-  }
-  if (acc > 0)
-    hipLaunchKernelGGL(pair_record_kernel, dim3((unsigned)((acc + 255) / 256)), dim3(256), 0, s, op->d_pairs, acc,
-                       op->d_xy, rho_min, resolution, op->Rc, (PairRec<float>*)op->d_rec32,
-                       (PairRec<double>*)op->d_rec64);
-  if (hipStreamSynchronize(s) != hipSuccess) {
-    smh_fail(ctx, SMH_ERR_HIP, "pair records failed");
-    return fail(SMH_ERR_HIP);
-  }
-  hipFree(op->d_pairs);
-  op->d_pairs = nullptr;
-
-  // ---- work items: split each chunk's bins into parts of similar pair count
-  const int64_t target_items = std::max<int64_t>(4 * (int64_t)ctx->num_cu, op->n_chunks);
-  const int64_t budget = std::max<int64_t>(1, (acc + target_items - 1) / target_items);
-  std::vector<int4> items;
-  std::vector<int> chunk_items(op->n_chunks + 1, 0);
-  for (int c = 0; c < op->n_chunks; ++c) {
-    chunk_items[c] = (int)items.size();
-    const int64_t base = (int64_t)c * len_rho;
-    int rb = 0;
-    int64_t run = 0;
-    for (int r = 0; r < len_rho; ++r) {
-      run += cnt[base + r];
-      if (run >= budget && r + 1 < len_rho) {
-        items.push_back(make_int4(c, rb, r + 1, 0));
-        rb = r + 1;
-        run = 0;
-      }
-    }
-    if (run > 0 || rb == 0) items.push_back(make_int4(c, rb, (int)len_rho - 1, 0));
-  }
-  chunk_items[op->n_chunks] = (int)items.size();
-  op->n_items = (int)items.size();
-  if (hipMalloc(&op->d_items, items.size() * sizeof(int4)) != hipSuccess ||
-      hipMalloc(&op->d_chunk_items, chunk_items.size() * sizeof(int)) != hipSuccess ||
-      hipMemcpyAsync(op->d_items, items.data(), items.size() * sizeof(int4), hipMemcpyHostToDevice, s) !=
-          hipSuccess ||
-      hipMemcpyAsync(op->d_chunk_items, chunk_items.data(), chunk_items.size() * sizeof(int), hipMemcpyHostToDevice,
-                     s) != hipSuccess) {
-    smh_fail(ctx, SMH_ERR_HIP, "device allocation failed (items)");
-    return fail(SMH_ERR_HIP);
-  }
-
-  // ---- LDS plan per precision and work buffers
-  lds_plan(op, sizeof(float), op->rows_cap32, op->S32, op->lds32);
-  lds_plan(op, sizeof(double), op->rows_cap64, op->S64, op->lds64);
-  // periods: runs of at most S bins starting at a non-empty bin, with their tile counts
-  for (int prec = 0; prec < 2; ++prec) {
-    const int S = prec == 0 ? op->S32 : op->S64;
-    if (S < 1) continue;
-    std::vector<int4> periods, pitems;
-    for (const int4& it : items) {
-      const int64_t base = (int64_t)it.x * len_rho;
-      const int pstart = (int)periods.size();
-      int r = it.y;
-      while (r < it.z) {
-        while (r < it.z && cnt[base + r] == 0) ++r;
-        if (r >= it.z) break;
-        const int nb = std::min(S, it.z - r);
-        int rl = r, tiles = 0;
-        for (int k = 0; k < nb; ++k) {
-          const int c = cnt[base + r + k];
-          if (c > 0) rl = r + k;
-          tiles += (c + 7) >> 3;
-        }
-        periods.push_back(make_int4(r, nb, rl, tiles));
-        r += nb;
-      }
-      pitems.push_back(make_int4(it.x, pstart, (int)periods.size(), 0));
-    }
-    int4** dp = prec == 0 ? &op->d_periods32 : &op->d_periods64;
-    int4** di = prec == 0 ? &op->d_items32 : &op->d_items64;
-    if (hipMalloc(dp, std::max<size_t>(periods.size(), 1) * sizeof(int4)) != hipSuccess ||
-        hipMalloc(di, pitems.size() * sizeof(int4)) != hipSuccess ||
-        hipMemcpyAsync(*dp, periods.data(), periods.size() * sizeof(int4), hipMemcpyHostToDevice, s) != hipSuccess ||
-        hipMemcpyAsync(*di, pitems.data(), pitems.size() * sizeof(int4), hipMemcpyHostToDevice, s) != hipSuccess ||
-        hipStreamSynchronize(s) != hipSuccess) {
-      smh_fail(ctx, SMH_ERR_HIP, "device allocation failed (periods)");
-      return fail(SMH_ERR_HIP);
-    }
-  }
-  op->ypart_bytes = (size_t)op->n_items * op->Rc * 2 * op->Bh * sizeof(double);
+  op->ypart_bytes = std::max<size_t>(ypart, 16);
   const int64_t n_pad = n_part * op->B;
   if (hipMalloc(&op->d_ypart, op->ypart_bytes) != hipSuccess ||
       hipMalloc(&op->d_x32, n_pad * sizeof(float2)) != hipSuccess ||

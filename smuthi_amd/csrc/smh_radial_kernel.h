@@ -2,22 +2,29 @@
 //
 // Work decomposition. Particle pairs (i1 receiving, i2 emitting) are binned
 // by the table interval r = floor((rho12 - rho_min) / h) and grouped by
-// receiver chunk (Rc consecutive receivers). A work item is one chunk and a
-// range of bins. For a bin r every pair needs the same T table rows ("taps"),
-// so the bin is a GEMM:
+// receiver chunk (Rc consecutive receivers). For a bin r every pair needs the
+// same T table rows ("taps"), so the bin is a GEMM over its pairs:
 //
-//   Y[(re|im, n1), (pair, re|im)] = A_r[(re|im, n1), (n2, t)] * Z[(n2, t), (pair, re|im)]
-//   A_r = [Re; Im] of the T tap rows,   Z[(n2,t), p] = w_t(rho_p) e^{i m2 phi_p} x_{i2(p)}[n2]
+//   [y_re; y_im](n1, pair) = [Lr -Li; Li Lr](n1, (n2, t)) * [Zr; Zi]((n2, t), pair)
+//   Z((n2, t), pair) = w_t(rho) e^{i m2 phi} x_{i2}[n2]
 //
-// done with v_mfma 16x16x4 (FP32 or FP64). The tap rows of a period of S
-// consecutive bins are staged in LDS once and shared by all pairs of the
-// chunk in those bins. Per pair the result is rotated by e^{-i m1 phi} and
-// accumulated into an LDS accumulator of the chunk's receivers, which is
-// written once per work item (no global atomics; a second kernel sums the
-// work items of a chunk in fixed order).
+// run on v_mfma 16x16x4 (FP32 or FP64) with 16 pairs per tile. A wave owns a
+// bin entry at a time and holds the bin's tap fragments in registers in the
+// MFMA A-operand layout (the tap table is stored in exactly that order, so
+// the loads are contiguous 16-byte vectors); the four blocks Lr, Li of the
+// stacked matrix reuse the same registers, with -Zi supplied as a second B
+// operand. Per pair the result is rotated by e^{-i m1 phi} and accumulated
+// into an LDS accumulator of the chunk's receivers, written once per work
+// item; a second kernel sums the work items of a chunk in fixed order (no
+// global atomics, deterministic).
+//
+// K ordering inside one MFMA step: lane quarter q = lane / 16 takes tap
+// t = q / G and multipole n2 = (q % G) * KS + s, G = 4 / T, at K-step s.
 #pragma once
 
 #include <hip/hip_runtime.h>
+
+#include <type_traits>
 
 #include "smh_common.h"
 
@@ -26,9 +33,7 @@ namespace smh {
 template <int LMAX, int MMAX>
 struct SvwfIdx {
   static constexpr int B = 2 * (MMAX * (MMAX + 2) + (LMAX - MMAX) * (2 * MMAX + 1));
-  static constexpr int Bh = (B + 15) / 16 * 16;  // rows of the Re (or Im) half of A, padded to 16
-  static constexpr int MP = 2 * Bh;              // A rows
-  static constexpr int MT = MP / 16;             // MFMA row tiles
+  static constexpr int Bh = (B + 15) / 16 * 16;  // rows of Lr (or Li), padded to 16
   // order m of multipole index n (tau-major, then l, then m)
   __host__ __device__ static constexpr int m_of(int n) {
     int tbs = B / 2;
@@ -40,6 +45,17 @@ struct SvwfIdx {
     }
     return 0;
   }
+};
+
+// Fragment geometry of the tap table for blocksize B and T taps.
+struct TapGeom {
+  int G, KS, NG, MT, FR;
+  __host__ __device__ constexpr TapGeom(int B, int T)
+      : G(4 / T),
+        KS((B + 4 / T - 1) / (4 / T)),
+        NG(((B + 4 / T - 1) / (4 / T) + 3) / 4),
+        MT(2 * ((B + 15) / 16)),
+        FR(2 * ((B + 15) / 16) * (4 / T) * (((B + 4 / T - 1) / (4 / T) + 3) / 4) * 64) {}
 };
 
 template <typename Real>
@@ -71,8 +87,10 @@ struct Vec2<double> {
 };
 
 // One binned pair: emitting particle i2 and receiving particle i1 - chunk base
-// packed as i2 | (i1l << 24), the interpolation weight inside its table
+// packed as i2 | (i1l << 23), the interpolation weight inside its table
 // interval and the direction (cos phi, sin phi) of r1 - r2.
+constexpr int kPairShift = 23;
+constexpr uint32_t kPairMask = (1u << kPairShift) - 1u;
 template <typename Real>
 struct PairRec;
 template <>
@@ -86,16 +104,51 @@ struct PairRec<double> {
   double w, c, s;
 };
 
+// Largest number of pairs in one bin entry (a bin of a chunk with more pairs
+// is split into several entries so waves stay balanced).
+constexpr int kEntryPairs = 256;
+
 struct RadialKArgs {
-  const void* recs;        // PairRec<Real> sorted by (chunk, bin)
-  const int* off;          // CSR over (chunk, bin): n_chunks*len_rho + 1 entries
-  const int4* items;       // (chunk, first period, end period, -)
-  const int4* periods;     // (first bin rf, bins nb, last non-empty bin rl, tiles)
-  const void* tt;          // tap rows [n_tt][MP][B] of Real
-  const void* x;           // padded input [n_part][B] of Real2
-  void* ypart;             // per work item accumulators [n_items][Rc][2][Bh] of Real
-  int len_rho, s_mode, o_mode, interp, Rc, rows_cap;
+  const void* recs;     // PairRec<Real> sorted by (chunk, bin)
+  const int4* entries;  // (first tap row, first pair, pairs, -) per bin entry, grouped by item
+  const int2* items;    // (chunk, first entry); entries of item k end where item k+1 begins
+  const void* tt;       // tap table [n_tt][FR] of Real in fragment order
+  const void* x;        // padded input [n_part][B] of Real2
+  void* ypart;          // per work item accumulators [n_items][Rc][2][B] of Real
+  const float* xmax;        // FP32: max_j |Re x_j| + |Im x_j| of this matvec's input
+  const float* item_lsum;   // FP32: per item, largest tap-row sum of |Re L| + |Im L| over its bins
+  const int* item_en;       // FP32: per item, ceil(log2) of the most pairs one receiver has in it
+  int n_items, n_entries, interp, Rc;
 };
+
+// FP32 receiver accumulation in LDS is fixed point: a contribution v is split
+// exactly into v = h / S_hi + r with integer h, and r is rounded to l / S_lo,
+// S_hi and S_lo powers of two chosen per work item from a bound on |v| and
+// on the number of contributions a receiver gets, so neither int32 sum can
+// overflow. Both parts are added with integer LDS atomics (an order of
+// magnitude cheaper than float LDS atomics on gfx950, and order independent).
+// Bound: |v| <= V = 8 * lsum * xmax (sum_t |w_t| <= 2, a factor 2 for the
+// rotation and 2 for rounding); with at most 2^eN contributions,
+//   S_hi = 2^floor(30 - eN - log2 V),  S_lo = S_hi * 2^(31 - eN),
+// and the accumulated error is below 2^(2 eN - 62) V.
+__device__ inline void fixed_scales(float lsum, float xmax, int eN, float& sh, float& ish, float& sl, float& isl) {
+  const double V = 8.0 * (double)lsum * (double)xmax;
+  int kh = 0;
+  if (V > 0.0 && V < 1e300) kh = (int)floor(30.0 - eN - log2(V));
+  kh = max(-120, min(kh, 120 - (31 - eN)));
+  const int kl = kh + 31 - eN;
+  sh = ldexpf(1.0f, kh);
+  ish = ldexpf(1.0f, -kh);
+  sl = ldexpf(1.0f, kl);
+  isl = ldexpf(1.0f, -kl);
+}
+__device__ inline void fixed_add(int* p, int lo_off, float v, float sh, float ish, float sl) {
+  const float h = rintf(v * sh);
+  const float r = fmaf(-h, ish, v);  // exact: h / S_hi is within half an ulp of S_hi of v
+  const float l = rintf(r * sl);
+  atomicAdd(p, (int)h);
+  atomicAdd(p + lo_off, (int)l);
+}
 
 // Geometry of a pair, identical in the binning and the matvec kernels.
 __host__ __device__ inline void pair_geometry(double dx, double dy, double rho_min, double h, int& r, double& w,
@@ -110,18 +163,19 @@ __host__ __device__ inline void pair_geometry(double dx, double dy, double rho_m
 // Tap weights: linear (2 taps), Catmull-Rom (4 taps over rows r-1..r+2),
 // not-a-knot spline (4 taps over rows y_r, Mh_r, y_{r+1}, Mh_{r+1} where
 // Mh = h^2/6 S'').
-__host__ __device__ inline double tap_weight(int interp, int t, double w) {
-  if (interp == SMH_INTERP_LINEAR) return t == 0 ? 1.0 - w : w;
+template <typename Real>
+__host__ __device__ inline Real tap_weight(int interp, int t, Real w) {
+  if (interp == SMH_INTERP_LINEAR) return t == 0 ? Real(1) - w : w;
   if (interp == SMH_INTERP_CATMULL_ROM) {
-    double w2 = w * w, w3 = w2 * w;
+    const Real w2 = w * w, w3 = w2 * w;
     switch (t) {
-      case 0: return 0.5 * (-w3 + 2.0 * w2 - w);
-      case 1: return 0.5 * (3.0 * w3 - 5.0 * w2 + 2.0);
-      case 2: return 0.5 * (-3.0 * w3 + 4.0 * w2 + w);
-      default: return 0.5 * (w3 - w2);
+      case 0: return Real(0.5) * (-w3 + Real(2) * w2 - w);
+      case 1: return Real(0.5) * (Real(3) * w3 - Real(5) * w2 + Real(2));
+      case 2: return Real(0.5) * (Real(-3) * w3 + Real(4) * w2 + w);
+      default: return Real(0.5) * (w3 - w2);
     }
   }
-  double v = 1.0 - w;
+  const Real v = Real(1) - w;
   switch (t) {
     case 0: return v;
     case 1: return v * v * v - v;
@@ -130,278 +184,233 @@ __host__ __device__ inline double tap_weight(int interp, int t, double w) {
   }
 }
 
-// Period bookkeeping in LDS: [0..63] tile prefix, [64..127] bin offsets.
-constexpr int kMetaInts = 128;
-// accumulator row stride: Bh + 1 keeps the LDS atomics of one instruction on distinct banks
-template <int Bh>
-struct YStride {
-  static constexpr int v = Bh + 1;
+// Launch shape of a kernel instance: split the n2 range into KSPLIT parts so
+// the tap fragments of one part fit the register budget of the occupancy
+// wanted (FP32: 2 waves per SIMD, FP64: 1).
+#ifndef SMH_FP32_TAP_REGS
+#define SMH_FP32_TAP_REGS 128
+#endif
+#ifndef SMH_FP32_WAVES
+#define SMH_FP32_WAVES 8
+#endif
+#ifndef SMH_X_PREFETCH
+#define SMH_X_PREFETCH 0
+#endif
+template <typename Real, int B, int T>
+struct RadialShape {
+  static constexpr TapGeom g{B, T};
+  static constexpr int kRegBudget = sizeof(Real) == 4 ? SMH_FP32_TAP_REGS : 192;
+  static constexpr int regs_for(int split) { return g.MT * ((g.NG + split - 1) / split) * 4 * (int)(sizeof(Real) / 4); }
+  static constexpr int pick() {
+    int s = 1;
+    while (regs_for(s) > kRegBudget && s < g.NG) ++s;
+    return s;
+  }
+  static constexpr int KSPLIT = pick();
+  static constexpr int NGS = (g.NG + KSPLIT - 1) / KSPLIT;  // n2 groups per part
+  static constexpr int kWaves = (sizeof(Real) == 4 && regs_for(KSPLIT) <= 128) ? SMH_FP32_WAVES : 4;
+  static constexpr int kThreads = 64 * kWaves;
+  // accumulator words per receiver (odd stride spreads LDS banks): FP32 keeps
+  // hi and lo int32 words for 2B values, FP64 keeps 2B doubles
+  static constexpr int SY = sizeof(Real) == 4 ? 4 * B + 1 : 2 * B + 1;
 };
 
 template <typename Real, int LMAX, int MMAX, int T>
-__global__ void __launch_bounds__(256) radial_matvec_kernel(RadialKArgs a) {
+__global__ void __launch_bounds__((RadialShape<Real, SvwfIdx<LMAX, MMAX>::B, T>::kThreads))
+    radial_matvec_kernel(RadialKArgs a) {
   using I = SvwfIdx<LMAX, MMAX>;
   using M = Mfma<Real>;
   using R2 = typename Vec2<Real>::t;
   using Rec = PairRec<Real>;
-  constexpr int B = I::B, Bh = I::Bh, MP = I::MP, MT = I::MT, LDB = B + 1;
-  constexpr int YS = YStride<Bh>::v;
-  constexpr int KS = T * B / 4;  // K steps of 4
-  constexpr int NE = 2 * MMAX + 1;
-  constexpr int VW = 16 / sizeof(Real);  // reals per 16-byte staging load
-  constexpr int MAXU = 20;               // staging loads per thread per period
-  constexpr bool kPrefetchX = sizeof(Real) == 4;
-  static_assert((T * B) % 4 == 0, "K must be a multiple of 4");
-  static_assert((MP * B) % VW == 0, "tap rows must be a whole number of 16-byte vectors");
+  using Sh = RadialShape<Real, I::B, T>;
+  constexpr int B = I::B, Bh = I::Bh, MMX = MMAX;
+  constexpr TapGeom g{B, T};
+  constexpr int G = g.G, KS = g.KS, NG = g.NG, MT = g.MT, MH = MT / 2, FR = g.FR;
+  constexpr int KSPLIT = Sh::KSPLIT, NGS = Sh::NGS, SY = Sh::SY;
+  constexpr int NE = 2 * MMX + 1;
 
   extern __shared__ __align__(16) unsigned char smem_raw[];
-  Real* win = reinterpret_cast<Real*>(smem_raw);
-  Real* yacc = win + (size_t)a.rows_cap * MP * LDB;
-  int* meta_a = reinterpret_cast<int*>(yacc + (size_t)a.Rc * 2 * YS);
-  int* meta_b = meta_a + kMetaInts;
+  constexpr bool kFixed = sizeof(Real) == 4;
+  using AccT = typename std::conditional<kFixed, int, double>::type;
+  AccT* yacc = reinterpret_cast<AccT*>(smem_raw);
+  int* next_entry = reinterpret_cast<int*>(yacc + (size_t)a.Rc * SY);
+  float sh = 0.f, ish = 0.f, sl = 0.f, isl = 0.f;
+  if (kFixed) fixed_scales(a.item_lsum[blockIdx.x], *a.xmax, a.item_en[blockIdx.x], sh, ish, sl, isl);
 
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, nwaves = blockDim.x >> 6;
-  const int4 item = a.items[blockIdx.x];
-  const int chunk = item.x, pbeg = item.y, pend = item.z;
-  const int* off = a.off + (size_t)chunk * a.len_rho;
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int item = blockIdx.x;
+  const int2 it = a.items[item];
+  const int e_end = item + 1 < a.n_items ? a.items[item + 1].y : a.n_entries;
   const Rec* recs = reinterpret_cast<const Rec*>(a.recs);
   const R2* xg = reinterpret_cast<const R2*>(a.x);
   const Real* ttg = reinterpret_cast<const Real*>(a.tt);
-  for (int j = tid; j < a.Rc * 2 * YS; j += blockDim.x) yacc[j] = Real(0);
-
-  const int col = lane & 15, pl = col >> 1, part = col & 1, q = lane >> 4;
-  const int tl = (T == 4) ? q : (q & 1);
-
-  // window rows of a period: [s*rf + o, s*rl + o + T)
-  auto stage_load = [&](const int4 ph, Real (&st)[MAXU][VW]) {
-    const int row0 = a.s_mode * ph.x + a.o_mode;
-    const int nvec = (a.s_mode * (ph.z - ph.x) + T) * (MP * B / VW);
-    const Real* g = ttg + (size_t)row0 * MP * B;
-#pragma unroll
-    for (int u = 0; u < MAXU; ++u) {
-      const int v = tid + u * blockDim.x;
-      if (v < nvec) {
-        const R2* src = reinterpret_cast<const R2*>(g + (size_t)v * VW);
-#pragma unroll
-        for (int e = 0; e < VW / 2; ++e) {
-          const R2 t2 = src[e];
-          st[u][2 * e] = t2.x;
-          st[u][2 * e + 1] = t2.y;
-        }
-      }
-    }
-  };
-  auto stage_store = [&](const int4 ph, const Real (&st)[MAXU][VW]) {
-    const int nvec = (a.s_mode * (ph.z - ph.x) + T) * (MP * B / VW);
-#pragma unroll
-    for (int u = 0; u < MAXU; ++u) {
-      const int v = tid + u * blockDim.x;
-      if (v < nvec) {
-#pragma unroll
-        for (int e = 0; e < VW; ++e) {
-          const int j = v * VW + e;
-          const int row = j / (MP * B);
-          const int rem = j - row * (MP * B);
-          const int m2 = rem / B;
-          const int n2 = rem - m2 * B;
-          win[(row * MP + m2) * LDB + n2] = st[u][e];
-        }
-      }
-    }
-  };
-  // bin offsets of a period live in wave 0, one per lane (nb <= 48 < 64)
-  auto meta_load = [&](const int4 ph) { return (wave == 0 && lane <= ph.y) ? off[ph.x + lane] : 0; };
-  auto meta_store = [&](const int4 ph, int poff, int* m) {
-    if (wave != 0) return;
-    const int nxt = __shfl_down(poff, 1);
-    int tiles = (lane < ph.y) ? ((nxt - poff + 7) >> 3) : 0;
-#pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-      const int o = __shfl_up(tiles, d);
-      if (lane >= d) tiles += o;
-    }
-    if (lane <= ph.y) m[64 + lane] = poff;
-    if (lane < ph.y) m[lane + 1] = tiles;
-    if (lane == 0) m[0] = 0;
-  };
-
-  Real stage[MAXU][VW];
-  int4 ph_cur = pbeg < pend ? a.periods[pbeg] : int4{0, 0, 0, 0};
-  if (pbeg < pend) {
-    const int po = meta_load(ph_cur);
-    stage_load(ph_cur, stage);
-    meta_store(ph_cur, po, meta_a);
-    stage_store(ph_cur, stage);
-  }
+  for (int j = tid; j < a.Rc * SY; j += blockDim.x) yacc[j] = AccT(0);
+  if (tid == 0) *next_entry = it.y;
   __syncthreads();
-  int* cur = meta_a;
-  int* nxt = meta_b;
-  for (int p = pbeg; p < pend; ++p) {
-    // prefetch the next period's tap rows and bin offsets while this one computes
-    const bool has_next = p + 1 < pend;
-    int4 ph_next = has_next ? a.periods[p + 1] : int4{0, 0, 0, 0};
-    int po_next = 0;
-    if (has_next) {
-      po_next = meta_load(ph_next);
-      stage_load(ph_next, stage);
-    }
-    const int ntiles = ph_cur.w;
 
-    int kb = 0;  // bin of the tile being looked up; tiles of a wave come in increasing order
-    auto tile_desc = [&](int t, int& kbin, int& p0, int& nval) {
-      while (cur[kb + 1] <= t) ++kb;
-      kbin = kb;
-      p0 = cur[64 + kb] + (t - cur[kb]) * 8;
-      nval = min(8, cur[64 + kb + 1] - p0);
-    };
+  const int col = lane & 15, q = lane >> 4;
+  const int tap = q / G, hq = q % G;
 
-    int t = wave;
-    Rec rec0{}, rec1{};
-    int k0 = 0, p00 = 0, nv0 = 0, k1 = 0, p01 = 0, nv1 = 0;
-    R2 x0[B];
-    if (t < ntiles) {
-      tile_desc(t, k0, p00, nv0);
-      rec0 = recs[p00 + min(pl, nv0 - 1)];
-      const R2* xp = xg + (size_t)(rec0.ij & 0xFFFFFFu) * B;
-#pragma unroll
-      for (int n2 = 0; n2 < B; ++n2) x0[n2] = xp[n2];
-    }
-    if (t + nwaves < ntiles) {
-      tile_desc(t + nwaves, k1, p01, nv1);
-      rec1 = recs[p01 + min(pl, nv1 - 1)];
-    }
-    for (; t < ntiles; t += nwaves) {
-      R2 x1[B];
-      const bool has1 = t + nwaves < ntiles;
-      if (kPrefetchX && has1) {
-        const R2* xp = xg + (size_t)(rec1.ij & 0xFFFFFFu) * B;
-#pragma unroll
-        for (int n2 = 0; n2 < B; ++n2) x1[n2] = xp[n2];
-      }
-      Rec rec2{};
-      int k2 = 0, p02 = 0, nv2 = 0;
-      if (t + 2 * nwaves < ntiles) {
-        tile_desc(t + 2 * nwaves, k2, p02, nv2);
-        rec2 = recs[p02 + min(pl, nv2 - 1)];
-      }
+  for (;;) {
+    int e = 0;
+    if (lane == 0) e = atomicAdd(next_entry, 1);
+    e = __shfl(e, 0);
+    if (e >= e_end) break;
+    const int4 en = a.entries[e];  // (first tap row, first pair, pairs)
+    const Real* trow = ttg + (size_t)(en.x + tap) * FR;
+    const int ntiles = (en.z + 15) >> 4;
 
-      const bool valid = pl < nv0;
-      const Real w = rec0.w;
-      Real wq;
-      if (T == 2) {
-        wq = tl == 0 ? Real(1) - w : w;
-      } else if (a.interp == SMH_INTERP_CATMULL_ROM) {
-        const Real w2 = w * w, w3 = w2 * w;
-        wq = tl == 0 ? Real(0.5) * (-w3 + Real(2) * w2 - w)
-             : tl == 1 ? Real(0.5) * (Real(3) * w3 - Real(5) * w2 + Real(2))
-             : tl == 2 ? Real(0.5) * (Real(-3) * w3 + Real(4) * w2 + w)
-                       : Real(0.5) * (w3 - w2);
-      } else {
-        const Real v = Real(1) - w;
-        wq = tl == 0 ? v : tl == 1 ? v * v * v - v : tl == 2 ? w : w * w * w - w;
-      }
-      if (!valid) wq = Real(0);
-
-      cplx<Real> E[NE];
-      E[MMAX] = {Real(1), Real(0)};
-      E[MMAX + 1] = {rec0.c, rec0.s};
 #pragma unroll
-      for (int m = 2; m <= MMAX; ++m) E[MMAX + m] = cmul(E[MMAX + m - 1], E[MMAX + 1]);
+    for (int sp = 0; sp < KSPLIT; ++sp) {
+      // tap fragments of this n2 part: A(row 16 i + col, n2 = hq*KS + 4 gg + v)
+      Real af[MT][NGS][4];
 #pragma unroll
-      for (int m = 1; m <= MMAX; ++m) E[MMAX - m] = {E[MMAX + m].re, -E[MMAX + m].im};
-
-      if (!kPrefetchX) {
-        const R2* xp = xg + (size_t)(rec0.ij & 0xFFFFFFu) * B;
+      for (int i = 0; i < MT; ++i)
 #pragma unroll
-        for (int n2 = 0; n2 < B; ++n2) x0[n2] = xp[n2];
-      }
-      Real ur[B];
-#pragma unroll
-      for (int n2 = 0; n2 < B; ++n2) {
-        const cplx<Real> e = E[MMAX + I::m_of(n2)];
-        ur[n2] = part ? (e.re * x0[n2].y + e.im * x0[n2].x) : (e.re * x0[n2].x - e.im * x0[n2].y);
-      }
-
-      typename M::v4 acc[MT];
-#pragma unroll
-      for (int i = 0; i < MT; ++i) acc[i] = typename M::v4{0, 0, 0, 0};
-      // bin ph_cur.x + k0: its first tap row sits s_mode * k0 rows into the period's window
-      const Real* ap = win + ((a.s_mode * k0 + tl) * MP + col) * LDB + ((T == 2) ? (q >> 1) : 0);
-#pragma unroll
-      for (int s = 0; s < KS; ++s) {
-        Real b;
-        if (T == 4)
-          b = wq * ur[s];
-        else
-          b = wq * ((q >> 1) ? ur[2 * s + 1] : ur[2 * s]);
-        const int n2off = (T == 4) ? s : 2 * s;
-#pragma unroll
-        for (int i = 0; i < MT; ++i) acc[i] = M::op(ap[16 * i * LDB + n2off], b, acc[i]);
-      }
-
-      Real* yrow = yacc + ((int)(rec0.ij >> 24) * 2 + part) * YS;
-#pragma unroll
-      for (int i = 0; i < Bh / 16; ++i) {
-#pragma unroll
-        for (int v = 0; v < 4; ++v) {
-          const Real arz = acc[i][v], aiz = acc[i + Bh / 16][v];
-          const Real arzp = __shfl_xor(arz, 1), aizp = __shfl_xor(aiz, 1);
-          const Real yr = part ? (arzp - aiz) : (arz - aizp);
-          const Real yi = part ? (arz + aizp) : (arzp + aiz);
-          int n1;
-          cplx<Real> e;
-          if (M::kRowIs4q) {
-            n1 = 16 * i + 4 * q + v;
-            const int c0 = 16 * i + v, c1 = c0 + 4, c2 = c0 + 8, c3 = c0 + 12;
-            const cplx<Real> e0 = E[MMAX + (c0 < B ? I::m_of(c0) : 0)];
-            const cplx<Real> e1 = E[MMAX + (c1 < B ? I::m_of(c1) : 0)];
-            const cplx<Real> e2 = E[MMAX + (c2 < B ? I::m_of(c2) : 0)];
-            const cplx<Real> e3 = E[MMAX + (c3 < B ? I::m_of(c3) : 0)];
-            e = q == 0 ? e0 : (q == 1 ? e1 : (q == 2 ? e2 : e3));
-          } else {
-            n1 = 16 * i + 4 * v + q;
-            const int c0 = 16 * i + 4 * v, c1 = c0 + 1, c2 = c0 + 2, c3 = c0 + 3;
-            const cplx<Real> e0 = E[MMAX + (c0 < B ? I::m_of(c0) : 0)];
-            const cplx<Real> e1 = E[MMAX + (c1 < B ? I::m_of(c1) : 0)];
-            const cplx<Real> e2 = E[MMAX + (c2 < B ? I::m_of(c2) : 0)];
-            const cplx<Real> e3 = E[MMAX + (c3 < B ? I::m_of(c3) : 0)];
-            e = q == 0 ? e0 : (q == 1 ? e1 : (q == 2 ? e2 : e3));
+        for (int gl = 0; gl < NGS; ++gl) {
+          const int gg = sp * NGS + gl;
+          if (gg < NG) {
+            const Real* src = trow + ((size_t)((i * G + hq) * NG + gg) * 16 + col) * 4;
+            if constexpr (sizeof(Real) == 4) {
+              const float4 v = *reinterpret_cast<const float4*>(src);
+              af[i][gl][0] = v.x;
+              af[i][gl][1] = v.y;
+              af[i][gl][2] = v.z;
+              af[i][gl][3] = v.w;
+            } else {
+              const double2 v0 = reinterpret_cast<const double2*>(src)[0];
+              const double2 v1 = reinterpret_cast<const double2*>(src)[1];
+              af[i][gl][0] = v0.x;
+              af[i][gl][1] = v0.y;
+              af[i][gl][2] = v1.x;
+              af[i][gl][3] = v1.y;
+            }
           }
-          // (yr + i yi) * conj(e)
-          const Real val = part ? (yi * e.re - yr * e.im) : (yr * e.re + yi * e.im);
-          if (valid && n1 < B) atomicAdd(&yrow[n1], val);
+        }
+
+      // software pipeline: pair records two tiles ahead, inputs x one tile ahead
+      constexpr int NX = NGS * 4;
+      auto rec_at = [&](int tl) -> Rec {
+        const int ps = tl * 16 + col;
+        return recs[en.y + (ps < en.z ? ps : en.z - 1)];
+      };
+      auto load_x = [&](const Rec& r, R2(&xs)[NX]) {
+        const R2* xp = xg + (size_t)(r.ij & kPairMask) * B;
+#pragma unroll
+        for (int u = 0; u < NX; ++u) {
+          const int s = sp * NX + u;
+          if (s < KS) {
+            const int n2 = (G == 2 && hq) ? KS + s : s;
+            xs[u] = n2 < B ? xp[n2] : R2{Real(0), Real(0)};
+          }
+        }
+      };
+      Rec rec0 = rec_at(0);
+      Rec rec1 = rec_at(ntiles > 1 ? 1 : 0);
+      R2 x0[NX];
+      load_x(rec0, x0);
+      for (int tl = 0; tl < ntiles; ++tl) {
+        const Rec rec2 = rec_at(tl + 2 < ntiles ? tl + 2 : ntiles - 1);
+        R2 x1[NX];
+        if (SMH_X_PREFETCH) load_x(rec1, x1);
+        const Rec rec = rec0;
+        const bool valid = tl * 16 + col < en.z;
+        const Real wq = valid ? tap_weight<Real>(a.interp, tap, rec.w) : Real(0);
+        // e^{i m phi}, m = -MMAX..MMAX
+        cplx<Real> E[NE];
+        E[MMX] = {Real(1), Real(0)};
+        E[MMX + 1] = {rec.c, rec.s};
+#pragma unroll
+        for (int m = 2; m <= MMX; ++m) E[MMX + m] = cmul(E[MMX + m - 1], E[MMX + 1]);
+#pragma unroll
+        for (int m = 1; m <= MMX; ++m) E[MMX - m] = {E[MMX + m].re, -E[MMX + m].im};
+
+        typename M::v4 acc[MT];
+#pragma unroll
+        for (int i = 0; i < MT; ++i) acc[i] = typename M::v4{0, 0, 0, 0};
+#pragma unroll
+        for (int gl = 0; gl < NGS; ++gl) {
+#pragma unroll
+          for (int v = 0; v < 4; ++v) {
+            const int s = (sp * NGS + gl) * 4 + v;  // K-step
+            if (s < KS) {
+              // n2 of this lane: hq * KS + s; its order m is one of G compile-time values
+              const int n2a = s, n2b = (G == 2) ? KS + s : s;
+              const cplx<Real> ea = E[MMX + I::m_of(n2a < B ? n2a : 0)];
+              const cplx<Real> eb = E[MMX + I::m_of(n2b < B ? n2b : 0)];
+              const cplx<Real> em = hq ? eb : ea;
+              R2 xv = x0[gl * 4 + v];
+              xv.x *= wq;
+              xv.y *= wq;
+              const Real zr = em.re * xv.x - em.im * xv.y;
+              const Real zi = em.re * xv.y + em.im * xv.x;
+#pragma unroll
+              for (int i = 0; i < MT; ++i) acc[i] = M::op(af[i][gl][v], zr, acc[i]);
+#pragma unroll
+              for (int i = 0; i < MH; ++i) acc[i] = M::op(af[i + MH][gl][v], -zi, acc[i]);
+#pragma unroll
+              for (int i = 0; i < MH; ++i) acc[i + MH] = M::op(af[i][gl][v], zi, acc[i + MH]);
+            }
+          }
+        }
+
+        // rotate by conj(e^{i m1 phi}) and accumulate into the receiver's LDS row
+        AccT* yrow = yacc + (size_t)(rec.ij >> kPairShift) * SY;
+#pragma unroll
+        for (int i = 0; i < MH; ++i) {
+#pragma unroll
+          for (int v = 0; v < 4; ++v) {
+            int n1;
+            cplx<Real> e;
+            if (M::kRowIs4q) {
+              n1 = 16 * i + 4 * q + v;
+              const int c0 = 16 * i + v;
+              const cplx<Real> e0 = E[MMX + (c0 < B ? I::m_of(c0) : 0)];
+              const cplx<Real> e1 = E[MMX + (c0 + 4 < B ? I::m_of(c0 + 4) : 0)];
+              const cplx<Real> e2 = E[MMX + (c0 + 8 < B ? I::m_of(c0 + 8) : 0)];
+              const cplx<Real> e3 = E[MMX + (c0 + 12 < B ? I::m_of(c0 + 12) : 0)];
+              e = q == 0 ? e0 : (q == 1 ? e1 : (q == 2 ? e2 : e3));
+            } else {
+              n1 = 16 * i + 4 * v + q;
+              const int c0 = 16 * i + 4 * v;
+              const cplx<Real> e0 = E[MMX + (c0 < B ? I::m_of(c0) : 0)];
+              const cplx<Real> e1 = E[MMX + (c0 + 1 < B ? I::m_of(c0 + 1) : 0)];
+              const cplx<Real> e2 = E[MMX + (c0 + 2 < B ? I::m_of(c0 + 2) : 0)];
+              const cplx<Real> e3 = E[MMX + (c0 + 3 < B ? I::m_of(c0 + 3) : 0)];
+              e = q == 0 ? e0 : (q == 1 ? e1 : (q == 2 ? e2 : e3));
+            }
+            const Real yr = acc[i][v], yi = acc[i + MH][v];
+            if (valid && n1 < B) {
+              const Real v0 = yr * e.re + yi * e.im, v1 = yi * e.re - yr * e.im;
+              if constexpr (kFixed) {
+                fixed_add(reinterpret_cast<int*>(&yrow[n1]), 2 * B, v0, sh, ish, sl);
+                fixed_add(reinterpret_cast<int*>(&yrow[B + n1]), 2 * B, v1, sh, ish, sl);
+              } else {
+                atomicAdd(&yrow[n1], v0);
+                atomicAdd(&yrow[B + n1], v1);
+              }
+            }
+          }
+        }
+        rec0 = rec1;
+        rec1 = rec2;
+        if (SMH_X_PREFETCH) {
+#pragma unroll
+          for (int u = 0; u < NX; ++u) x0[u] = x1[u];
+        } else if (tl + 1 < ntiles) {
+          load_x(rec0, x0);
         }
       }
-
-      rec0 = rec1;
-      k0 = k1;
-      p00 = p01;
-      nv0 = nv1;
-      rec1 = rec2;
-      k1 = k2;
-      p01 = p02;
-      nv1 = nv2;
-      if (kPrefetchX && has1) {
-#pragma unroll
-        for (int n2 = 0; n2 < B; ++n2) x0[n2] = x1[n2];
-      }
     }
-    __syncthreads();  // every tile of this period is done with win and cur
-    if (has_next) {
-      stage_store(ph_next, stage);
-      meta_store(ph_next, po_next, nxt);
-    }
-    __syncthreads();
-    int* tmp = cur;
-    cur = nxt;
-    nxt = tmp;
-    ph_cur = ph_next;
   }
   __syncthreads();
-  Real* yp = reinterpret_cast<Real*>(a.ypart) + (size_t)blockIdx.x * a.Rc * 2 * Bh;
-  for (int j = tid; j < a.Rc * 2 * Bh; j += blockDim.x) {
-    const int rowi = j / Bh, n1 = j - rowi * Bh;
-    yp[j] = yacc[rowi * YS + n1];
+  Real* yp = reinterpret_cast<Real*>(a.ypart) + (size_t)item * a.Rc * 2 * B;
+  for (int j = tid; j < a.Rc * 2 * B; j += blockDim.x) {
+    const int il = j / (2 * B), k = j - il * 2 * B;
+    if constexpr (kFixed)
+      yp[j] = (Real)((double)yacc[il * SY + k] * (double)ish + (double)yacc[il * SY + 2 * B + k] * (double)isl);
+    else
+      yp[j] = (Real)yacc[il * SY + k];
   }
 }
 

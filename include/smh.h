@@ -98,7 +98,8 @@ int smh_op_bench_matvec(smh_op *op, int precision, int iters, double *kernel_ms,
 /* Solve (1 - T W) b = T a (LinearSystem.solve with solver_type='gmres',
  * linear_system.py:254-344) for nrhs right-hand sides.
  * t_diag: complex128 [n_unk] diagonal sphere T-matrix; rhs, x: complex128
- * [nrhs][n_unk]; x0 = rhs as in smuthi. Stops at ||rhs - M x|| <= tol ||rhs||
+ * [nrhs][n_unk]. SMH_SOLVE_FP64 starts from x0 = rhs as smuthi does;
+ * SMH_SOLVE_MIXED refines from x0 = 0. Both stop at ||rhs - M x|| <= tol ||rhs||
  * measured in FP64. stats (out, 8 doubles): iterations, FP64 matvecs, FP32
  * matvecs, worst relative residual, solve seconds, refinement sweeps, 0, 0. */
 int smh_solve(smh_op *op, const double *t_diag, const double *rhs, double *x, int64_t nrhs, double tol,

@@ -194,9 +194,14 @@ struct Solver {
     return rc;
   }
 
-  // Restarted GMRES on M d = r from d = x (in/out), stops at ||r - M d|| <= atol.
-  // Returns the number of Arnoldi steps taken in *steps.
-  int gmres(double2* x, const double2* b, double atol, int maxiter, int precision, double& res_out, int& steps) {
+  // Restarted GMRES on M x = b from x (in/out), stops at ||b - M x|| <= atol.
+  // x_zero: x is zero on entry, so the first residual is b itself.
+  // trust_estimate: return as soon as the Arnoldi residual estimate reaches
+  // atol, without recomputing the true residual (the caller checks it).
+  // On return res_out is the true residual of x (or the estimate, see above)
+  // and steps the number of Arnoldi steps taken.
+  int gmres(double2* x, const double2* b, double atol, int maxiter, int precision, double& res_out, int& steps,
+            bool x_zero = false, bool trust_estimate = false) {
     std::vector<zd> H((size_t)(restart + 1) * restart), cs(restart), sn(restart), g(restart + 1), hcol(restart + 1),
         hcol2(restart + 1), yv(restart);
     int rc;
@@ -204,8 +209,12 @@ struct Solver {
     double beta = 0;
     for (int outer = 0;; ++outer) {
       // r = b - M x -> V_0
-      if ((rc = master(x, w, precision))) return rc;
-      hipLaunchKernelGGL(sub_kernel, dim3(blocks_for(n)), dim3(256), 0, s, b, w, w, n);
+      if (outer == 0 && x_zero) {
+        SMH_HIP(ctx, hipMemcpyAsync(w, b, n * sizeof(double2), hipMemcpyDeviceToDevice, s));
+      } else {
+        if ((rc = master(x, w, precision))) return rc;
+        hipLaunchKernelGGL(sub_kernel, dim3(blocks_for(n)), dim3(256), 0, s, b, w, w, n);
+      }
       if ((rc = norm(w, beta))) return rc;
       res_out = beta;
       if (beta <= atol || steps >= maxiter || beta == 0.0) return SMH_OK;
@@ -213,6 +222,7 @@ struct Solver {
       std::fill(g.begin(), g.end(), zd(0));
       g[0] = beta;
       int k = 0;
+      bool est_converged = false;
       for (; k < restart && steps < maxiter; ++k, ++steps) {
         double2* vk = V + (size_t)k * n;
         if ((rc = master(vk, w, precision))) return rc;
@@ -247,6 +257,7 @@ struct Solver {
         for (int j = 0; j <= k; ++j) H[(size_t)j * restart + k] = hcol[j];
         ++iters;
         if (std::abs(g[k + 1]) <= atol || hn == 0.0) {
+          est_converged = true;
           ++k;
           ++steps;
           break;
@@ -262,7 +273,10 @@ struct Solver {
         SMH_HIP(ctx, hipMemcpyAsync(coef, yv.data(), k * sizeof(double2), hipMemcpyHostToDevice, s));
         hipLaunchKernelGGL(multi_update_kernel, dim3(blocks_for(n)), dim3(256), 0, s, V, n, k, coef, x, n);
       }
-      (void)outer;
+      if (trust_estimate && est_converged) {
+        res_out = std::abs(g[k]);
+        return SMH_OK;
+      }
     }
   }
 };
@@ -323,40 +337,50 @@ extern "C" int smh_solve(smh_op* op, const double* t_diag, const double* rhs, do
     if ((rc = smh::op_unpack(op, SMH_FP64, op->d_pk_in, b_pad))) return done2(rc);
     double bnorm;
     if ((rc = S.norm(b_pad, bnorm))) return done2(rc);
-    // x0 = rhs, as smuthi passes rhs as initial guess
-    if (hipMemcpyAsync(x_pad, b_pad, S.n * sizeof(double2), hipMemcpyDeviceToDevice, s) != hipSuccess)
-      return done2(smh_fail(ctx, SMH_ERR_HIP, "copy failed"));
     const double atol = tol * bnorm;
     double res = 0;
     int steps = 0;
     if (mode == SMH_SOLVE_FP64) {
+      // x0 = rhs, as smuthi passes rhs as initial guess; GMRES returns the true residual of x
+      if (hipMemcpyAsync(x_pad, b_pad, S.n * sizeof(double2), hipMemcpyDeviceToDevice, s) != hipSuccess)
+        return done2(smh_fail(ctx, SMH_ERR_HIP, "copy failed"));
       if ((rc = S.gmres(x_pad, b_pad, atol, maxiter, SMH_FP64, res, steps))) return done2(rc);
       steps_total += steps;
       sweeps_total += 1;
     } else {
-      // iterative refinement: FP64 residual, FP32-matvec inner GMRES
+      // iterative refinement from x0 = 0: FP64 residual, FP32-matvec inner GMRES
+      SMH_HIP(ctx, hipMemsetAsync(x_pad, 0, S.n * sizeof(double2), s));
+      SMH_HIP(ctx, hipMemcpyAsync(r_pad, b_pad, S.n * sizeof(double2), hipMemcpyDeviceToDevice, s));
+      res = bnorm;
+      bool fresh = true;  // res is the FP64 residual of the current x
       int budget = maxiter;
       for (int sweep = 0; sweep < 50; ++sweep) {
-        if ((rc = S.master(x_pad, r_pad, SMH_FP64))) return done2(rc);
-        hipLaunchKernelGGL(sub_kernel, dim3(blocks_for(S.n)), dim3(256), 0, s, b_pad, r_pad, r_pad, S.n);
-        if ((rc = S.norm(r_pad, res))) return done2(rc);
+        if (sweep > 0) {
+          if ((rc = S.master(x_pad, r_pad, SMH_FP64))) return done2(rc);
+          hipLaunchKernelGGL(sub_kernel, dim3(blocks_for(S.n)), dim3(256), 0, s, b_pad, r_pad, r_pad, S.n);
+          if ((rc = S.norm(r_pad, res))) return done2(rc);
+          fresh = true;
+        }
         if (res <= atol || budget <= 0) break;
         SMH_HIP(ctx, hipMemsetAsync(d_pad, 0, S.n * sizeof(double2), s));
-        // the FP32 operator resolves ~1e-6 relative; ask the inner solve for 1e-4 of this residual
+        // the FP32 operator resolves ~1e-7 relative; ask the inner solve for 1e-4 of this residual
         double inner_res = 0;
         int inner_steps = 0;
         const double inner_atol = std::max(atol * 0.5, 1e-4 * res);
-        if ((rc = S.gmres(d_pad, r_pad, inner_atol, budget, SMH_FP32, inner_res, inner_steps))) return done2(rc);
+        if ((rc = S.gmres(d_pad, r_pad, inner_atol, budget, SMH_FP32, inner_res, inner_steps, true, true)))
+          return done2(rc);
         budget -= inner_steps;
         steps_total += inner_steps;
         hipLaunchKernelGGL(add_kernel, dim3(blocks_for(S.n)), dim3(256), 0, s, x_pad, d_pad, S.n);
         sweeps_total += 1;
+        fresh = false;
+      }
+      if (!fresh) {
+        if ((rc = S.master(x_pad, r_pad, SMH_FP64))) return done2(rc);
+        hipLaunchKernelGGL(sub_kernel, dim3(blocks_for(S.n)), dim3(256), 0, s, b_pad, r_pad, r_pad, S.n);
+        if ((rc = S.norm(r_pad, res))) return done2(rc);
       }
     }
-    // true FP64 residual
-    if ((rc = S.master(x_pad, r_pad, SMH_FP64))) return done2(rc);
-    hipLaunchKernelGGL(sub_kernel, dim3(blocks_for(S.n)), dim3(256), 0, s, b_pad, r_pad, r_pad, S.n);
-    if ((rc = S.norm(r_pad, res))) return done2(rc);
     const double rel = bnorm > 0 ? res / bnorm : 0.0;
     worst = std::max(worst, rel);
     if (rel > tol) converged_all = false;

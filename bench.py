@@ -53,6 +53,38 @@ def rhs_for_step(j):
     return np.pi - np.deg2rad(8.0 * (j % 8)), 0.0, j % 2
 
 
+TRAFFIC_FILE = os.path.join(ROOT, 'profiles', 'radial_matvec_10k_cubic_traffic.md')
+
+
+def measured_traffic():
+    """HBM bytes per FP32 matvec launch from the committed counter summary, or None."""
+    try:
+        for line in open(TRAFFIC_FILE):
+            if line.startswith('traffic_bytes_per_launch'):
+                return float(line.split('=')[1])
+    except OSError:
+        pass
+    return None
+
+
+def traffic_probe(n_particles, copy_bytes=512 << 20):
+    """Run a plain copy of known size, then three FP32 matvec launches of the
+    bench operator (one warm-up + two timed): the workload the FETCH_SIZE /
+    WRITE_SIZE counter passes are taken on."""
+    import ctypes
+    from smuthi_amd import _lib, cuda
+    from smuthi_amd.linear_system import CouplingMatrixRadialLookupCUDA
+    if not cuda.enable_gpu():
+        raise SystemExit('no MI355X device')
+    plist, ls, kp, side = build_workload(n_particles)
+    op = CouplingMatrixRadialLookupCUDA(WL, plist, ls, k_parallel=kp, resolution=2.0, interpolator_kind='cubic')
+    ms = ctypes.c_double()
+    _lib.check(_lib.load().smh_bench_copy(cuda.context(), copy_bytes, ctypes.byref(ms)), cuda.context(),
+               'smh_bench_copy')
+    k, _ = op.bench_matvec('single', iters=2)
+    log('copy %d bytes: %.3f ms; FP32 matvec kernel %.2f ms' % (copy_bytes, ms.value, k))
+
+
 def cpu_baseline(table, rho, n_full, iters_per_solve, n_sample=400, seed=5, budget_s=20.0):
     """Time the oracle's restatement of smuthi's CPU radial-lookup matvec
     (CouplingMatrixRadialLookupCPU.matvec: per-(n1, n2) scipy interp1d 'cubic',
@@ -96,10 +128,15 @@ def main():
     ap.add_argument('--tol', type=float, default=1e-6)
     ap.add_argument('--no-cpu-baseline', action='store_true')
     ap.add_argument('--matvec-iters', type=int, default=5)
+    ap.add_argument('--traffic-probe', action='store_true',
+                    help='only run the copy calibration and three FP32 matvecs (for counter passes)')
     args = ap.parse_args()
     if args.gpus != 1:
         print('bench.py measures one GPU: this build has no multi-GPU path (run with --gpus 1)', file=sys.stderr)
         sys.exit(2)
+    if args.traffic_probe:
+        traffic_probe(args.particles)
+        return None
 
     from smuthi_amd import cuda
     from smuthi_amd.initial_field import PlaneWave
@@ -216,7 +253,7 @@ def main():
         'prepare_seconds': prep_s,
         'table_seconds': op.table_seconds,
         'roofline': {'bound': 'mfma', 'achieved': achieved, 'peak': MFMA_FP32_PEAK_TFLOPS, 'unit': 'TFLOP/s',
-                     'frac': achieved / MFMA_FP32_PEAK_TFLOPS, 'traffic': None,
+                     'frac': achieved / MFMA_FP32_PEAK_TFLOPS, 'traffic': measured_traffic(),
                      'kernel': 'radial_matvec_kernel<float,3,3,4>',
                      'flops_per_launch': flops_alg},
         'cpu_baseline': cpu,
@@ -231,4 +268,5 @@ if __name__ == '__main__':
     _stdout = sys.stdout
     with contextlib.redirect_stdout(sys.stderr):
         _line = main()
-    print(_line, file=_stdout, flush=True)
+    if _line is not None:
+        print(_line, file=_stdout, flush=True)

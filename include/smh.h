@@ -116,6 +116,12 @@ int smh_pwe_field(smh_ctx *ctx, int kind, int64_t n_k, const double *kp, const d
                   const double *alpha, const double *g, double k_re, double k_im, int64_t n_pts,
                   const double *xyz, double *out, int precision, double *kernel_ms);
 
+/* ---- measurement helper (no counterpart in smuthi) ----------------------
+ * Copy `bytes` (a multiple of 16) between two fresh device buffers with one
+ * plain streaming kernel and return its time; used to calibrate the HBM
+ * traffic counters against a kernel whose traffic is known (2 * bytes). */
+int smh_bench_copy(smh_ctx *ctx, int64_t bytes, double *ms);
+
 /* ---- host evaluation of the device special functions -------------------
  * Runs the same code the table kernels use, on the CPU, so it can be checked
  * without a device: out[j][n] = J_n(z_j) (complex128 [count][nmax+1]) and

@@ -108,3 +108,36 @@ extern "C" int smh_host_spherical_hankel(int nmax, int64_t count, const double* 
   }
   return SMH_OK;
 }
+
+namespace {
+__global__ void copy16_kernel(const uint4* __restrict__ src, uint4* __restrict__ dst, int64_t n) {
+  for (int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j < n; j += (int64_t)gridDim.x * blockDim.x)
+    dst[j] = src[j];
+}
+}  // namespace
+
+extern "C" int smh_bench_copy(smh_ctx* ctx, int64_t bytes, double* ms) {
+  if (!ctx) return SMH_ERR_ARG;
+  SMH_CHECK(ctx, bytes >= 16 && bytes % 16 == 0, SMH_ERR_ARG, "copy size must be a positive multiple of 16 bytes");
+  SMH_HIP(ctx, hipSetDevice(ctx->device));
+  void *a = nullptr, *b = nullptr;
+  SMH_HIP(ctx, hipMalloc(&a, bytes));
+  if (hipMalloc(&b, bytes) != hipSuccess) {
+    hipFree(a);
+    return smh_fail(ctx, SMH_ERR_HIP, "copy buffer allocation failed");
+  }
+  hipStream_t s = ctx->stream;
+  const int64_t n = bytes / 16;
+  hipMemsetAsync(a, 1, bytes, s);
+  hipEventRecord(ctx->ev0, s);
+  hipLaunchKernelGGL(copy16_kernel, dim3(4096), dim3(256), 0, s, (const uint4*)a, (uint4*)b, n);
+  hipEventRecord(ctx->ev1, s);
+  const hipError_t e = hipEventSynchronize(ctx->ev1);
+  float t = 0;
+  hipEventElapsedTime(&t, ctx->ev0, ctx->ev1);
+  hipFree(a);
+  hipFree(b);
+  SMH_CHECK(ctx, e == hipSuccess, SMH_ERR_HIP, "copy failed: %s", hipGetErrorString(e));
+  if (ms) *ms = t;
+  return SMH_OK;
+}

@@ -6,8 +6,3 @@ extern "C" int smh_volume_create(smh_ctx* ctx, int64_t, const double*, const int
   if (out) *out = nullptr;
   return smh_fail(ctx, SMH_ERR_UNSUPPORTED, "volume lookup operator not built");
 }
-
-extern "C" int smh_pwe_field(smh_ctx* ctx, int, int64_t, const double*, const double*, int64_t, const double*,
-                             const double*, double, double, int64_t, const double*, double*, int, double*) {
-  return smh_fail(ctx, SMH_ERR_UNSUPPORTED, "PWE field kernels not built");
-}

@@ -116,6 +116,20 @@ int smh_pwe_field(smh_ctx *ctx, int kind, int64_t n_k, const double *kp, const d
                   const double *alpha, const double *g, double k_re, double k_im, int64_t n_pts,
                   const double *xyz, double *out, int precision, double *kernel_ms);
 
+/* Plane wave expansion of the scattered field of particles in one layer
+ * (the particle sum of swe_to_pwe_conversion, fields/transformations.py:133-226,
+ * as used by scattered_field_pwe, postprocessing/scattered_field.py:80-150):
+ *   g_dir(pol, kappa_k, alpha_a) = sum_p e^{-i kappa (x_p cos a + y_p sin a)} e^{+-i kz (z_ref - z_p)}
+ *                                  sum_m e^{i m a} sum_{tau,l} coef[p][n(tau,l,m)] btab[k][dir][pol][n]
+ * xyz: [n_part][3]; coef: complex128 [n_part][B] in the (l_max, m_max) layout
+ * (m_max <= 4); kp, kz: complex128 [n_k] (kz upgoing); btab: complex128
+ * [n_k][2 dir (up, down)][2 pol][B], the transformation coefficients
+ * B_{n,pol}(+-kz/k, kappa/k) / (2 pi kz k); alpha: [n_a];
+ * g: complex128 [2 dir][2 pol][n_k][n_a]. */
+int smh_pwe_assemble(smh_ctx *ctx, int64_t n_part, const double *xyz, int l_max, int m_max, const double *coef,
+                     double z_ref, int64_t n_k, const double *kp, const double *kz, const double *btab, int64_t n_a,
+                     const double *alpha, double *g, int precision, double *kernel_ms);
+
 /* ---- measurement helper (no counterpart in smuthi) ----------------------
  * Copy `bytes` (a multiple of 16) between two fresh device buffers with one
  * plain streaming kernel and return its time; used to calibrate the HBM

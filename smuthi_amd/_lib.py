@@ -54,6 +54,8 @@ SIGNATURES = {
     'smh_solve': (_c.c_int, [_p, _dp, _dp, _dp, _i64, _c.c_double, _c.c_int, _c.c_int, _c.c_int, _dp]),
     'smh_pwe_field': (_c.c_int, [_p, _c.c_int, _i64, _dp, _dp, _i64, _dp, _dp, _c.c_double, _c.c_double, _i64, _dp,
                                  _dp, _c.c_int, _dp]),
+    'smh_pwe_assemble': (_c.c_int, [_p, _i64, _dp, _c.c_int, _c.c_int, _dp, _c.c_double, _i64, _dp, _dp, _dp, _i64,
+                                    _dp, _dp, _c.c_int, _dp]),
     'smh_bench_copy': (_c.c_int, [_p, _i64, _dp]),
     'smh_host_bessel_j': (_c.c_int, [_c.c_int, _i64, _dp, _dp]),
     'smh_host_spherical_hankel': (_c.c_int, [_c.c_int, _i64, _dp, _dp]),

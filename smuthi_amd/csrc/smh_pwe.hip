@@ -14,6 +14,7 @@
 #include <algorithm>
 #include <cmath>
 #include <complex>
+#include <cstring>
 #include <numeric>
 #include <vector>
 
@@ -146,9 +147,9 @@ __global__ void __launch_bounds__(kLeaf) pwe_field_kernel(PweArgs a) {
 }
 
 // Recursive median split of idx[b, e) along the widest extent into leaves of
-// at most kLeaf points.
-void split_leaves(const double* xyz, std::vector<int>& idx, int b, int e, std::vector<int>& bounds) {
-  if (e - b <= kLeaf) {
+// at most `leaf` points.
+void split_leaves(const double* xyz, std::vector<int>& idx, int b, int e, std::vector<int>& bounds, int leaf = kLeaf) {
+  if (e - b <= leaf) {
     bounds.push_back(e);
     return;
   }
@@ -161,13 +162,13 @@ void split_leaves(const double* xyz, std::vector<int>& idx, int b, int e, std::v
   int dim = 0;
   for (int d = 1; d < 3; ++d)
     if (hi[d] - lo[d] > hi[dim] - lo[dim]) dim = d;
-  // split at a multiple of kLeaf so leaves stay full
-  const int half = ((e - b + 1) / 2 + kLeaf - 1) / kLeaf * kLeaf;
+  // split at a multiple of the leaf size so leaves stay full
+  const int half = ((e - b + 1) / 2 + leaf - 1) / leaf * leaf;
   const int mid = std::min(e - 1, b + half);
   std::nth_element(idx.begin() + b, idx.begin() + mid, idx.begin() + e,
                    [&](int u, int v) { return xyz[3 * u + dim] < xyz[3 * v + dim]; });
-  split_leaves(xyz, idx, b, mid, bounds);
-  split_leaves(xyz, idx, mid, e, bounds);
+  split_leaves(xyz, idx, b, mid, bounds, leaf);
+  split_leaves(xyz, idx, mid, e, bounds, leaf);
 }
 
 }  // namespace
@@ -316,6 +317,262 @@ extern "C" int smh_pwe_field(smh_ctx* ctx, int kind, int64_t n_k, const double* 
       out[2 * ((size_t)c * n_pts + idx[j])] = v.real();
       out[2 * ((size_t)c * n_pts + idx[j]) + 1] = v.imag();
     }
+  if (kernel_ms) *kernel_ms = ms;
+  return SMH_OK;
+}
+
+// ---- plane wave expansion of the scattered field --------------------------
+// g_dir(pol, kappa, alpha) = sum_p e^{-i kappa (x_p cos a + y_p sin a)} e^{i kz_dir (z_ref - z_p)}
+//                            * sum_m e^{i m a} c_{p,m,pol,dir}(kappa),
+// c_{p,m,pol,dir} = sum_{tau,l} b_{p,tau l m} B_{tau l m,pol}(+-kz/k, kappa/k) / (2 pi kz k)
+// (swe_to_pwe_conversion, fields/transformations.py:133-226, summed over the
+// particles of one layer as scattered_field_pwe, postprocessing/scattered_field.py:80-150).
+// One workgroup per kappa, one thread per alpha; particles are taken leaf by
+// leaf, their c staged in LDS; the leaf centre's phase is applied in FP64.
+namespace {
+
+constexpr int kAsmLeaf = 128;
+
+struct AsmArgs {
+  const double* pts;    // [n_part][3], leaf order
+  const double2* coef;  // [n_part][B], leaf order
+  const int* leaf;      // [n_leaves + 1]
+  const double* ctr;    // [n_leaves][2] leaf centres (x, y)
+  const double2* kp;    // [n_k]
+  const double2* kz;    // [n_k] upgoing kz
+  const double2* btab;  // [n_k][2 dir][2 pol][B], 1 / (2 pi kz k) included
+  const int* nlist;     // [NM][LM] multipole indices n of order m = j - m_max (-1: none)
+  const double* alpha;  // [n_a]
+  double2* g;           // [2 dir][2 pol][n_k][n_a]
+  double z_ref;
+  int B, NM, LM, n_leaves, n_k, n_a;
+};
+
+template <typename Real, int NM>
+__global__ void __launch_bounds__(256) pwe_assemble_kernel(AsmArgs a) {
+  using M = PweMath<Real>;
+  constexpr int Q = 4;  // (dir, pol)
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  Real* cl = reinterpret_cast<Real*>(smem_raw);  // [kAsmLeaf][NM][Q][2]
+  Real* dxy = cl + (size_t)kAsmLeaf * NM * Q * 2;  // [kAsmLeaf][2]
+  const int k = blockIdx.x, tid = threadIdx.x;
+  const int ia = blockIdx.y * blockDim.x + tid;
+  const bool active = ia < a.n_a;
+  const double al = active ? a.alpha[ia] : 0.0;
+  const double cad = cos(al), sad = sin(al);
+  const Real ca = (Real)cad, sa = (Real)sad;
+  const double2 kp = a.kp[k], kzu = a.kz[k];
+  const Real kr = (Real)(-kp.x / kTwoPi);   // e^{-i kappa s}: revolutions per nm
+  const Real ki = (Real)(kp.y * kLog2e);    // |e^{-i kappa s}| = 2^(ki s)
+  const bool lossy = kp.y != 0.0;
+  Real U[NM][Q][2];
+#pragma unroll
+  for (int m = 0; m < NM; ++m)
+#pragma unroll
+    for (int q = 0; q < Q; ++q) U[m][q][0] = U[m][q][1] = Real(0);
+  for (int l = 0; l < a.n_leaves; ++l) {
+    const int p0 = a.leaf[l], np = a.leaf[l + 1] - p0;
+    const double X = a.ctr[2 * l], Y = a.ctr[2 * l + 1];
+    __syncthreads();
+    for (int w = tid; w < np * NM; w += blockDim.x) {
+      const int pl = w / NM, m = w - pl * NM, p = p0 + pl;
+      const double zp = a.pts[3 * p + 2];
+      // e^{i kz (z_ref - z_p)} (up) and e^{-i kz (z_ref - z_p)} (down)
+      const double dz = a.z_ref - zp;
+      double su, cu;
+      sincos(kzu.x * dz, &su, &cu);
+      const double mu = exp(-kzu.y * dz), md = exp(kzu.y * dz);
+      const double2 ph[2] = {make_double2(mu * cu, mu * su), make_double2(md * cu, -md * su)};
+      double cr[Q] = {0, 0, 0, 0}, ci[Q] = {0, 0, 0, 0};
+      for (int j = 0; j < a.LM; ++j) {
+        const int n = a.nlist[m * a.LM + j];
+        if (n < 0) break;
+        const double2 b = a.coef[(size_t)p * a.B + n];
+        for (int q = 0; q < Q; ++q) {
+          const double2 t = a.btab[((size_t)k * Q + q) * a.B + n];
+          cr[q] += b.x * t.x - b.y * t.y;
+          ci[q] += b.x * t.y + b.y * t.x;
+        }
+      }
+      Real* dst = cl + ((size_t)pl * NM + m) * Q * 2;
+      for (int q = 0; q < Q; ++q) {
+        const double2 e = ph[q >> 1];
+        dst[2 * q] = (Real)(cr[q] * e.x - ci[q] * e.y);
+        dst[2 * q + 1] = (Real)(cr[q] * e.y + ci[q] * e.x);
+      }
+      if (m == 0) {
+        dxy[2 * pl] = (Real)(a.pts[3 * p] - X);
+        dxy[2 * pl + 1] = (Real)(a.pts[3 * p + 1] - Y);
+      }
+    }
+    __syncthreads();
+    Real S[NM][Q][2];
+#pragma unroll
+    for (int m = 0; m < NM; ++m)
+#pragma unroll
+      for (int q = 0; q < Q; ++q) S[m][q][0] = S[m][q][1] = Real(0);
+    for (int pl = 0; pl < np; ++pl) {
+      const Real s = dxy[2 * pl] * ca + dxy[2 * pl + 1] * sa;
+      Real pc, ps;
+      M::cis_rev(kr * s, pc, ps);
+      if (lossy) {
+        const Real mg = M::exp2_(ki * s);
+        pc *= mg;
+        ps *= mg;
+      }
+      const Real* c = cl + (size_t)pl * NM * Q * 2;
+#pragma unroll
+      for (int m = 0; m < NM; ++m)
+#pragma unroll
+        for (int q = 0; q < Q; ++q) {
+          const Real xr = c[(m * Q + q) * 2], xi = c[(m * Q + q) * 2 + 1];
+          S[m][q][0] += xr * pc - xi * ps;
+          S[m][q][1] += xr * ps + xi * pc;
+        }
+    }
+    // leaf centre phase e^{-i kappa (X cos a + Y sin a)} in FP64
+    const double sc = X * cad + Y * sad;
+    double sn, cn;
+    sincos(-kp.x * sc, &sn, &cn);
+    const double mg = exp(kp.y * sc);
+    const Real lr = (Real)(mg * cn), li = (Real)(mg * sn);
+#pragma unroll
+    for (int m = 0; m < NM; ++m)
+#pragma unroll
+      for (int q = 0; q < Q; ++q) {
+        U[m][q][0] += S[m][q][0] * lr - S[m][q][1] * li;
+        U[m][q][1] += S[m][q][0] * li + S[m][q][1] * lr;
+      }
+  }
+  if (!active) return;
+  const int mmax = (NM - 1) / 2;
+  for (int q = 0; q < Q; ++q) {
+    double gr = 0, gi = 0;
+#pragma unroll
+    for (int m = 0; m < NM; ++m) {
+      double em_s, em_c;
+      sincos((double)(m - mmax) * al, &em_s, &em_c);
+      gr += (double)U[m][q][0] * em_c - (double)U[m][q][1] * em_s;
+      gi += (double)U[m][q][0] * em_s + (double)U[m][q][1] * em_c;
+    }
+    a.g[((size_t)q * a.n_k + k) * a.n_a + ia] = make_double2(gr, gi);
+  }
+}
+
+template <typename Real>
+void* assemble_kernel_for(int nm) {
+  switch (nm) {
+    case 3: return reinterpret_cast<void*>(&pwe_assemble_kernel<Real, 3>);
+    case 5: return reinterpret_cast<void*>(&pwe_assemble_kernel<Real, 5>);
+    case 7: return reinterpret_cast<void*>(&pwe_assemble_kernel<Real, 7>);
+    case 9: return reinterpret_cast<void*>(&pwe_assemble_kernel<Real, 9>);
+    default: return nullptr;
+  }
+}
+
+}  // namespace
+
+extern "C" int smh_pwe_assemble(smh_ctx* ctx, int64_t n_part, const double* xyz, int l_max, int m_max,
+                                const double* coef, double z_ref, int64_t n_k, const double* kp, const double* kz,
+                                const double* btab, int64_t n_a, const double* alpha, double* g, int precision,
+                                double* kernel_ms) {
+  if (!ctx) return SMH_ERR_ARG;
+  SMH_CHECK(ctx, n_part >= 1 && xyz && coef && kp && kz && btab && alpha && g, SMH_ERR_ARG,
+            "invalid PWE assembly arguments");
+  SMH_CHECK(ctx, l_max >= 1 && m_max >= 1 && m_max <= l_max && m_max <= 4, SMH_ERR_UNSUPPORTED,
+            "l_max %d / m_max %d not supported (m_max <= 4)", l_max, m_max);
+  SMH_CHECK(ctx, n_k >= 1 && n_a >= 1 && n_k < (1 << 30) && n_a < (1 << 30) && n_part < (1 << 30), SMH_ERR_ARG,
+            "invalid sizes");
+  SMH_CHECK(ctx, precision == SMH_FP32 || precision == SMH_FP64, SMH_ERR_ARG, "invalid precision %d", precision);
+  if (kernel_ms) *kernel_ms = 0.0;
+  SMH_HIP(ctx, hipSetDevice(ctx->device));
+  const int B = smh_blocksize(l_max, m_max), NM = 2 * m_max + 1, LM = 2 * l_max;
+  // multipole indices of each order m
+  std::vector<int> nlist((size_t)NM * LM, -1), fill(NM, 0);
+  for (int tau = 0; tau < 2; ++tau)
+    for (int l = 1; l <= l_max; ++l) {
+      const int ml = std::min(l, m_max);
+      for (int m = -ml; m <= ml; ++m) {
+        const int j = m + m_max;
+        nlist[(size_t)j * LM + fill[j]++] = smh_multi_to_single(tau, l, m, l_max, m_max);
+      }
+    }
+  // particle leaves
+  std::vector<int> idx(n_part);
+  std::iota(idx.begin(), idx.end(), 0);
+  std::vector<int> bounds{0};
+  split_leaves(xyz, idx, 0, (int)n_part, bounds, kAsmLeaf);
+  const int n_leaves = (int)bounds.size() - 1;
+  std::vector<double> pts((size_t)3 * n_part), ctr((size_t)2 * n_leaves);
+  std::vector<double> cf((size_t)2 * n_part * B);
+  for (int64_t j = 0; j < n_part; ++j) {
+    for (int d = 0; d < 3; ++d) pts[3 * j + d] = xyz[3 * (size_t)idx[j] + d];
+    std::memcpy(&cf[(size_t)2 * j * B], &coef[(size_t)2 * idx[j] * B], sizeof(double) * 2 * B);
+  }
+  for (int l = 0; l < n_leaves; ++l) {
+    double lo[2] = {1e300, 1e300}, hi[2] = {-1e300, -1e300};
+    for (int j = bounds[l]; j < bounds[l + 1]; ++j)
+      for (int d = 0; d < 2; ++d) {
+        lo[d] = std::min(lo[d], pts[3 * j + d]);
+        hi[d] = std::max(hi[d], pts[3 * j + d]);
+      }
+    ctr[2 * l] = 0.5 * (lo[0] + hi[0]);
+    ctr[2 * l + 1] = 0.5 * (lo[1] + hi[1]);
+  }
+  const size_t rs = precision == SMH_FP32 ? sizeof(float) : sizeof(double);
+  const size_t lds = (size_t)kAsmLeaf * NM * 4 * 2 * rs + (size_t)kAsmLeaf * 2 * rs;
+  void* buf[10] = {nullptr};
+  const size_t sizes[10] = {pts.size() * 8, cf.size() * 8, bounds.size() * 4, ctr.size() * 8,
+                            (size_t)n_k * 16, (size_t)n_k * 16, (size_t)n_k * 4 * B * 16, nlist.size() * 4,
+                            (size_t)n_a * 8, (size_t)4 * n_k * n_a * 16};
+  auto release = [&]() {
+    for (void* b : buf)
+      if (b) hipFree(b);
+  };
+  for (int i = 0; i < 10; ++i)
+    if (hipMalloc(&buf[i], sizes[i]) != hipSuccess) {
+      release();
+      return smh_fail(ctx, SMH_ERR_HIP, "PWE assembly allocation failed");
+    }
+  hipStream_t s = ctx->stream;
+  const void* src[9] = {pts.data(), cf.data(), bounds.data(), ctr.data(), kp, kz, btab, nlist.data(), alpha};
+  for (int i = 0; i < 9; ++i)
+    if (hipMemcpyAsync(buf[i], src[i], sizes[i], hipMemcpyHostToDevice, s) != hipSuccess) {
+      release();
+      return smh_fail(ctx, SMH_ERR_HIP, "PWE assembly upload failed");
+    }
+  AsmArgs a;
+  a.pts = (const double*)buf[0];
+  a.coef = (const double2*)buf[1];
+  a.leaf = (const int*)buf[2];
+  a.ctr = (const double*)buf[3];
+  a.kp = (const double2*)buf[4];
+  a.kz = (const double2*)buf[5];
+  a.btab = (const double2*)buf[6];
+  a.nlist = (const int*)buf[7];
+  a.alpha = (const double*)buf[8];
+  a.g = (double2*)buf[9];
+  a.z_ref = z_ref;
+  a.B = B;
+  a.NM = NM;
+  a.LM = LM;
+  a.n_leaves = n_leaves;
+  a.n_k = (int)n_k;
+  a.n_a = (int)n_a;
+  void* kfn = precision == SMH_FP32 ? assemble_kernel_for<float>(NM) : assemble_kernel_for<double>(NM);
+  const int threads = (int)std::min<int64_t>(256, (n_a + 63) / 64 * 64);
+  const unsigned a_blocks = (unsigned)((n_a + threads - 1) / threads);
+  hipError_t e = hipFuncSetAttribute(kfn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  if (e == hipSuccess) e = hipEventRecord(ctx->ev0, s);
+  void* args[] = {&a};
+  if (e == hipSuccess) e = hipLaunchKernel(kfn, dim3((unsigned)n_k, a_blocks), dim3(threads), args, lds, s);
+  if (e == hipSuccess) e = hipEventRecord(ctx->ev1, s);
+  if (e == hipSuccess) e = hipMemcpyAsync(g, buf[9], sizes[9], hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  float ms = 0;
+  if (e == hipSuccess) e = hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1);
+  release();
+  SMH_CHECK(ctx, e == hipSuccess, SMH_ERR_HIP, "PWE assembly kernel failed: %s", hipGetErrorString(e));
   if (kernel_ms) *kernel_ms = ms;
   return SMH_OK;
 }

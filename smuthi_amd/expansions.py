@@ -24,6 +24,37 @@ class FieldExpansion:
         return ret
 
 
+class PiecewiseFieldExpansion(FieldExpansion):
+    """Sum of field expansions, each contributing where it is valid
+    (fields/expansions.py:102-195)."""
+
+    def __init__(self):
+        FieldExpansion.__init__(self)
+        self.expansion_list = []
+
+    def valid(self, x, y, z):
+        vld = np.zeros(np.shape(x), dtype=bool)
+        for fex in self.expansion_list:
+            vld = np.logical_or(vld, fex.valid(x, y, z))
+        return np.logical_and(vld, FieldExpansion.valid(self, x, y, z))
+
+    def _sum(self, fn, x, y, z):
+        x, y, z = np.array(x, ndmin=1), np.array(y, ndmin=1), np.array(z, ndmin=1)
+        out = [np.zeros(x.shape, dtype=complex) for _ in range(3)]
+        vld = self.valid(x, y, z)
+        for fex in self.expansion_list:
+            d = fn(fex, x, y, z)
+            for c in range(3):
+                out[c][vld] += d[c][vld]
+        return tuple(out)
+
+    def electric_field(self, x, y, z, precision='single'):
+        return self._sum(lambda f, *r: f.electric_field(*r, precision=precision), x, y, z)
+
+    def magnetic_field(self, x, y, z, vacuum_wavelength, precision='single'):
+        return self._sum(lambda f, *r: f.magnetic_field(*r, vacuum_wavelength, precision=precision), x, y, z)
+
+
 class SphericalWaveExpansion(FieldExpansion):
     """Coefficients a_{tau l m} of a regular or outgoing SWE about reference_point."""
 

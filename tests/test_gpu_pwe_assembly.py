@@ -1,0 +1,104 @@
+"""GPU parity of the scattered-field plane wave expansion (the particle sum of
+swe_to_pwe_conversion, fields/transformations.py:133-226, in
+scattered_field_pwe, postprocessing/scattered_field.py:80-150) and of the
+extinction cross section (postprocessing/far_field.py:517-654) against the
+FP64 oracle."""
+import numpy as np
+import pytest
+
+from oracle import basics as ob
+from oracle import layers as ol
+from oracle import system as osys
+
+pytestmark = pytest.mark.gpu
+
+WL = 550.0
+
+
+@pytest.fixture(scope='module')
+def gpu():
+    from smuthi_amd import cuda
+    assert cuda.enable_gpu(), 'MI355X not available'
+    return cuda
+
+
+def _rel(a, b):
+    return np.abs(a - b).max() / np.abs(b).max()
+
+
+def _particles(n, seed):
+    rng = np.random.default_rng(seed)
+    plist = []
+    orders = [(3, 3), (3, 2), (2, 2), (4, 3)]
+    for i in range(n):
+        l, m = orders[i % len(orders)]
+        p = osys.Sphere([rng.uniform(-1500, 1500), rng.uniform(-1500, 1500), rng.uniform(80, 320)], 1.6, 60, l, m)
+        nb = 2 * (m * (m + 2) + (l - m) * (2 * m + 1))
+        p.scattered_field = (rng.standard_normal(nb) + 1j * rng.standard_normal(nb)) * rng.uniform(0.1, 1)
+        plist.append(p)
+    return plist
+
+
+@pytest.mark.parametrize('layer_number,direct,mediated', [(1, True, True), (0, True, True), (2, True, True),
+                                                          (1, True, False), (1, False, True)])
+def test_scattered_field_pwe_matches_oracle(gpu, layer_number, direct, mediated):
+    from smuthi_amd.layers import LayerSystem
+    from smuthi_amd.scattered_field import scattered_field_pwe
+    thick, n = [0, 400, 0], [1.5, 2.0 + 0.01j, 1.0]
+    plist = _particles(150, 7)  # > one leaf of 128 particles
+    kp = ob.reasonable_Sommerfeld_kpar_contour(WL, layer_refractive_indices=n, neff_resolution=5e-2,
+                                               neff_imag=1e-2)
+    alpha = np.linspace(0, 2 * np.pi, 12)
+    ou, od = osys.scattered_field_pwe(WL, plist, ol.LayerSystem(thick, n), layer_number, kp, alpha,
+                                      include_direct=direct, include_layer_response=mediated)
+    for prec, tol in (('double', 1e-10), ('single', 2e-5)):
+        pu, pd = scattered_field_pwe(WL, plist, LayerSystem(thick, n), layer_number, kp, alpha,
+                                     include_direct=direct, include_layer_response=mediated, precision=prec)
+        assert pu.lower_z == ou.lower_z and pu.upper_z == ou.upper_z
+        assert pd.lower_z == od.lower_z and pd.upper_z == od.upper_z
+        # one scale for both directions: in a half-space one of them vanishes up to rounding
+        scale = max(np.abs(ou.coefficients).max(), np.abs(od.coefficients).max())
+        assert np.abs(pu.coefficients - ou.coefficients).max() < tol * scale, prec
+        assert np.abs(pd.coefficients - od.coefficients).max() < tol * scale, prec
+
+
+def test_extinction_cross_section_matches_oracle(gpu):
+    from smuthi_amd.far_field import extinction_cross_section
+    from smuthi_amd.initial_field import PlaneWave
+    from smuthi_amd.layers import LayerSystem
+    kp = ob.reasonable_Sommerfeld_kpar_contour(WL, layer_refractive_indices=[2, 1.4, 2], neff_resolution=1e-2)
+    plist = [osys.Sphere([100, 100, 150], 2.4, 110, 4, 4), osys.Sphere([-100, -100, 250], 1.9, 120, 3, 3),
+             osys.Sphere([-200, 100, 300], 1.7, 90, 3, 3)]
+    ls = ol.LayerSystem([0, 400, 0], [2, 1.4, 2])
+    for polar, pol in ((np.pi * 7 / 8, 0), (np.pi / 6, 1)):
+        pw = osys.PlaneWave(WL, polar, np.pi / 3, pol)
+        lsys = osys.LinearSystemOracle(plist, pw, ls, kp)
+        lsys.scatter_solution(lsys.solve_lu(lsys.explicit_coupling()))
+        want = osys.extinction_cross_section(pw, plist, ls)
+        got = extinction_cross_section(PlaneWave(WL, polar, np.pi / 3, pol), plist, LayerSystem([0, 400, 0], [2, 1.4, 2]))
+        assert abs(got - want) <= 1e-10 * abs(want)
+        got32 = extinction_cross_section(PlaneWave(WL, polar, np.pi / 3, pol), plist,
+                                         LayerSystem([0, 400, 0], [2, 1.4, 2]), precision='single')
+        assert abs(got32 - want) <= 1e-5 * abs(want)
+
+
+def test_piecewise_expansion_sums_valid_parts(gpu):
+    from smuthi_amd.layers import LayerSystem
+    from smuthi_amd.scattered_field import scattered_field_piecewise_expansion
+    thick, n = [0, 0], [1.52, 1.0]
+    plist = _particles(20, 9)
+    for p in plist:
+        p.position[2] = 200.0
+    kp = ob.reasonable_Sommerfeld_kpar_contour(WL, layer_refractive_indices=n, neff_resolution=2e-2)
+    alpha = np.linspace(0, 2 * np.pi, 16)
+    sfld = scattered_field_piecewise_expansion(WL, plist, LayerSystem(thick, n), kp, alpha, precision='double')
+    ls = ol.LayerSystem(thick, n)
+    x = np.linspace(-2000, 2000, 40)
+    y = np.zeros_like(x)
+    for zz, layer in ((-300.0, 0), (900.0, 1)):
+        z = np.full_like(x, zz)
+        got = sfld.electric_field(x, y, z, precision='double')
+        up, dn = osys.scattered_field_pwe(WL, plist, ls, layer, kp, alpha)
+        want = [a + b for a, b in zip(up.electric_field(x, y, z), dn.electric_field(x, y, z))]
+        scale = max(np.abs(w).max() for w in want)
+        assert max(np.abs(g - w).max() for g, w in zip(got, want)) < 1e-10 * scale

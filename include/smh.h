@@ -59,18 +59,36 @@ int smh_radial_create(smh_ctx *ctx, int64_t n_part, const double *xy, const int 
                       int l_max, int m_max, double rho_min, double resolution, int64_t len_rho, int interp,
                       smh_op **out);
 
-/* Volume lookup operator (CouplingMatrixVolumeLookupCUDA, linear_system.py:702-780):
- * xyz: [n_part][3]; tables over (rho, z1+z2) and (rho, z1-z2) with the same
- * resolution; len_z entries from sz_min / dz_min. interp: LINEAR or CATMULL_ROM. */
+/* Volume lookup operator (CouplingMatrixVolumeLookupCUDA, linear_system.py:688-783):
+ * xyz: [n_part][3] (all particles in one layer); complex64 tables over
+ * (rho, z1+z2) ("plus") and (rho, z1-z2) ("minus") on the grids
+ * rho_min + i h (i < len_rho), sz_min + j h and dz_min + j h (j < len_z),
+ * interpolated bilinearly (interp must be SMH_INTERP_LINEAR, which is the
+ * reference's GPU 'linear' and CPU RectBivariateSpline kx=ky=1). The call
+ * bins every particle pair by table cell for the matvec. */
 int smh_volume_create(smh_ctx *ctx, int64_t n_part, const double *xyz, const int *lmax, const int *mmax,
                       int l_max, int m_max, double rho_min, double sz_min, double dz_min, double resolution,
                       int64_t len_rho, int64_t len_z, int interp, smh_op **out);
+
+/* Build both volume tables on the GPU (volumetric_coupling_lookup_table,
+ * prepare_lookup.py:16-250, and its helper kernel prepare_lookup_cuda.py:6-52),
+ * FP64 arithmetic, stored complex64:
+ *   plus[r][j]  = c64( sum_k J_|dm|(kappa_k rho_r) (C[k][0] e^{i kz_k (sz_j - 2 z_s)} + C[k][1] e^{-i kz_k (sz_j - 2 z_s)}) )
+ *   minus[r][j] = c64(direct) + c64( sum_k J_|dm|(kappa_k rho_r) (C[k][2] e^{i kz_k dz_j} + C[k][3] e^{-i kz_k dz_j}) )
+ *   direct = sum_ld direct_coef[n1][n2][ld] h_ld(k d) P_ld^|dm|(dz/d), d = sqrt(rho^2 + dz^2),
+ *            zero where rho < mask_radius.
+ * direct_coef: complex128 [B][B][2 l_max + 1]; kappa, kz: complex128 [n_kappa];
+ * layer_coef (C): complex128 [n_kappa][4][B][B] with contour weights and
+ * layer response folded in, or n_kappa = 0 for a degenerate layer system. */
+int smh_volume_build_table(smh_op *op, double k_re, double k_im, double mask_radius, double z_s,
+                           const double *direct_coef, int64_t n_kappa, const double *kappa, const double *kz,
+                           const double *layer_coef);
 
 int smh_op_destroy(smh_op *op);
 
 /* Upload a host table (prepare_lookup.py:253-464 / :16-250 output):
  * radial: complex128 [len_rho][B][B]; volume: two complex128 [len_rho][len_z][B][B]
- * (plus, minus). B = blocksize(l_max, m_max). */
+ * (plus, minus), stored complex64. B = blocksize(l_max, m_max). */
 int smh_op_upload_table(smh_op *op, const double *table, const double *table_minus);
 
 /* Build the radial table on the GPU in FP64 (replaces radial_coupling_lookup_table,

@@ -45,6 +45,8 @@ SIGNATURES = {
                                      _c.c_int, _c.POINTER(_p)]),
     'smh_volume_create': (_c.c_int, [_p, _i64, _dp, _ip, _ip, _c.c_int, _c.c_int, _c.c_double, _c.c_double,
                                      _c.c_double, _c.c_double, _i64, _i64, _c.c_int, _c.POINTER(_p)]),
+    'smh_volume_build_table': (_c.c_int, [_p, _c.c_double, _c.c_double, _c.c_double, _c.c_double, _dp, _i64, _dp,
+                                          _dp, _dp]),
     'smh_op_destroy': (_c.c_int, [_p]),
     'smh_op_upload_table': (_c.c_int, [_p, _dp, _dp]),
     'smh_radial_build_table': (_c.c_int, [_p, _c.c_double, _c.c_double, _c.c_double, _dp, _i64, _dp, _dp]),

@@ -1,0 +1,221 @@
+// Pieces shared by the radial and the volume lookup operators: work-item
+// construction over bin entries, per-item fixed-point statistics, the chunk
+// reduction and small helper kernels. Included by one translation unit per
+// operator kind (everything here has internal linkage).
+#pragma once
+
+#include <algorithm>
+#include <vector>
+
+#include "smh_op.h"
+#include "smh_radial_kernel.h"
+
+namespace {
+
+using namespace smh;
+
+inline int blocks_for(int64_t n, int t = 256) { return (int)((n + t - 1) / t); }
+
+struct KernelInfo {
+  void* fn = nullptr;
+  int threads = 0;
+  int sy = 0;  // accumulator stride per receiver
+};
+
+// Receiver chunk of a precision: as many receivers (at most 512, the pair
+// record's receiver field) as the LDS accumulator [Rc][SY] fits in one
+// workgroup's LDS, less 1 KiB.
+inline int chunk_size(const smh_op* op, size_t real_size, int sy, size_t& lds) {
+  const size_t limit = std::min<size_t>(op->ctx->lds_per_block ? op->ctx->lds_per_block : 65536, 163840) - 1024;
+  const size_t per = (size_t)sy * real_size;
+  int rc = (int)std::min<size_t>(512, (limit - 16) / per);
+  if (rc < 16) {
+    lds = 0;
+    return 0;
+  }
+  lds = (size_t)rc * per + 16;
+  return rc;
+}
+
+// Sum the work-item accumulators of each receiver chunk in fixed order
+// (y = sum, or y += sum when accumulate is set).
+template <typename Real>
+__global__ void reduce_kernel(const Real* __restrict__ ypart, const int* __restrict__ chunk_items, int Rc, int B,
+                              int64_t n_part, typename Vec2<Real>::t* y, int accumulate) {
+  const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n_part * B) return;
+  const int64_t i1 = j / B;
+  const int n1 = (int)(j - i1 * B);
+  const int c = (int)(i1 / Rc), il = (int)(i1 - (int64_t)c * Rc);
+  Real re = 0, im = 0;
+  for (int w = chunk_items[c]; w < chunk_items[c + 1]; ++w) {
+    const Real* p = ypart + ((size_t)w * Rc + il) * 2 * B;
+    re += p[n1];
+    im += p[B + n1];
+  }
+  if (accumulate) {
+    re += y[j].x;
+    im += y[j].y;
+  }
+  y[j] = typename Vec2<Real>::t{re, im};
+}
+
+// Row sums of tap rows: rowsum[row] = max_n1 sum_n2 (|Re| + |Im|) of row
+// `row` of a [rows][B][B] table (with `interleave`, even rows come from
+// table and odd rows from mh, both indexed row / 2), with a small margin for
+// the FP32 rounding of the tap table.
+template <typename Src>
+__global__ void tap_rowsum_kernel(const Src* __restrict__ table, const Src* __restrict__ mh, int64_t rows, int B,
+                                  int interleave, float* rowsum) {
+  for (int64_t row = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; row < rows;
+       row += (int64_t)gridDim.x * blockDim.x) {
+    const Src* src = table;
+    int64_t j = row;
+    if (interleave) {
+      j = row >> 1;
+      if (row & 1) src = mh;
+    }
+    double best = 0.0;
+    for (int n1 = 0; n1 < B; ++n1) {
+      double sum = 0.0;
+      for (int n2 = 0; n2 < B; ++n2) {
+        const Src v = src[(j * B + n1) * B + n2];
+        sum += fabs((double)v.x) + fabs((double)v.y);
+      }
+      best = fmax(best, sum);
+    }
+    rowsum[row] = (float)(best * 1.001);
+  }
+}
+
+// Per work item: largest row sum over the tap rows its bin entries use (tap t
+// of an entry at row x sits at x + t, or at x + (t/2) len_z + t%2 for 2-D lookups).
+__global__ void item_lsum_kernel(const int4* __restrict__ entries, const int2* __restrict__ items, int n_items,
+                                 int n_entries, int T, int len_z, const float* __restrict__ rowsum, float* item_lsum) {
+  const int k = blockIdx.x;
+  const int e0 = items[k].y, e1 = k + 1 < n_items ? items[k + 1].y : n_entries;
+  float m = 0.f;
+  for (int e = e0 + (int)threadIdx.x; e < e1; e += blockDim.x)
+    for (int t = 0; t < T; ++t) {
+      const int off = len_z > 0 ? (t >> 1) * len_z + (t & 1) : t;
+      m = fmaxf(m, rowsum[entries[e].x + off]);
+    }
+  __shared__ float red[256];
+  red[threadIdx.x] = m;
+  __syncthreads();
+  for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) red[threadIdx.x] = fmaxf(red[threadIdx.x], red[threadIdx.x + w]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) item_lsum[k] = red[0];
+}
+
+// Per work item: eN = ceil(log2) of the most pairs a single receiver has among
+// the item's pairs (a contiguous range of the records).
+template <typename Rec>
+__global__ void item_count_kernel(const Rec* __restrict__ recs, const int4* __restrict__ entries,
+                                  const int2* __restrict__ items, int n_items, int n_entries, int* item_en) {
+  __shared__ int hist[512];
+  __shared__ int red[256];
+  const int k = blockIdx.x;
+  const int e0 = items[k].y, e1 = (k + 1 < n_items ? items[k + 1].y : n_entries) - 1;
+  for (int j = threadIdx.x; j < 512; j += blockDim.x) hist[j] = 0;
+  __syncthreads();
+  const int p0 = entries[e0].y, p1 = entries[e1].y + entries[e1].z;
+  for (int p = p0 + (int)threadIdx.x; p < p1; p += blockDim.x) atomicAdd(&hist[recs[p].ij >> kPairShift], 1);
+  __syncthreads();
+  int m = 0;
+  for (int j = threadIdx.x; j < 512; j += blockDim.x) m = max(m, hist[j]);
+  red[threadIdx.x] = m;
+  __syncthreads();
+  for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) red[threadIdx.x] = max(red[threadIdx.x], red[threadIdx.x + w]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    int eN = 1;
+    while ((1 << eN) < red[0]) ++eN;
+    item_en[k] = eN;
+  }
+}
+
+// Largest |Re x| + |Im x| of the input (one workgroup).
+__global__ void xmax_kernel(const float2* __restrict__ x, int64_t n, float* xmax) {
+  __shared__ float red[1024];
+  float m = 0.f;
+  for (int64_t j = threadIdx.x; j < n; j += blockDim.x) {
+    const float2 v = x[j];
+    m = fmaxf(m, fabsf(v.x) + fabsf(v.y));
+  }
+  red[threadIdx.x] = m;
+  __syncthreads();
+  for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) red[threadIdx.x] = fmaxf(red[threadIdx.x], red[threadIdx.x + w]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *xmax = red[0];
+}
+
+// Host: bin entries (at most kEntryPairs pairs each) of every non-empty
+// (chunk, bin) key, grouped into work items of similar cost (a tile of 16
+// pairs costs 1, loading a bin's taps 1/4), then uploaded with the per-item
+// pair statistics. cnt / off: pair count and first pair of each key
+// (chunk-major, bins_per_chunk keys per chunk); row_of(bin) gives the bin's
+// first tap row.
+template <typename Rec, typename RowOf>
+int build_work_items(smh_op* op, smh_op::Binning& bn, const std::vector<int>& cnt, const std::vector<int>& off,
+                     int64_t bins_per_chunk, RowOf row_of) {
+  smh_ctx* ctx = op->ctx;
+  hipStream_t s = ctx->stream;
+  std::vector<int4> entries;
+  std::vector<int> ent_chunk;
+  double total = 0;
+  for (int c = 0; c < bn.n_chunks; ++c)
+    for (int64_t r = 0; r < bins_per_chunk; ++r) {
+      const int64_t key = (int64_t)c * bins_per_chunk + r;
+      const int n = cnt[key];
+      for (int p0 = 0; p0 < n; p0 += kEntryPairs) {
+        const int np = std::min(kEntryPairs, n - p0);
+        entries.push_back(make_int4(row_of(r), off[key] + p0, np, 0));
+        ent_chunk.push_back(c);
+        total += (np + 15) / 16 + 0.25;
+      }
+    }
+  const int64_t target_items = std::max<int64_t>(4 * (int64_t)ctx->num_cu, bn.n_chunks);
+  const double budget = std::max(1.0, total / (double)target_items);
+  std::vector<int2> items;
+  std::vector<int> chunk_items(bn.n_chunks + 1, 0);
+  int cur_chunk = -1;
+  double run = 0;
+  for (size_t e = 0; e < entries.size(); ++e) {
+    const int c = ent_chunk[e];
+    if (c != cur_chunk || run >= budget) {
+      while (cur_chunk < c) chunk_items[++cur_chunk] = (int)items.size();
+      items.push_back(make_int2(c, (int)e));
+      run = 0;
+    }
+    run += (entries[e].z + 15) / 16 + 0.25;
+  }
+  while (cur_chunk < bn.n_chunks) chunk_items[++cur_chunk] = (int)items.size();
+  bn.n_entries = (int)entries.size();
+  bn.n_items = (int)items.size();
+  if (hipMalloc(&bn.d_entries, std::max<size_t>(entries.size(), 1) * sizeof(int4)) != hipSuccess ||
+      hipMalloc(&bn.d_items, std::max<size_t>(items.size(), 1) * sizeof(int2)) != hipSuccess ||
+      hipMalloc(&bn.d_chunk_items, chunk_items.size() * sizeof(int)) != hipSuccess ||
+      hipMalloc(&bn.d_item_lsum, std::max(bn.n_items, 1) * sizeof(float)) != hipSuccess ||
+      hipMalloc(&bn.d_item_en, std::max(bn.n_items, 1) * sizeof(int)) != hipSuccess ||
+      hipMemcpyAsync(bn.d_entries, entries.data(), entries.size() * sizeof(int4), hipMemcpyHostToDevice, s) !=
+          hipSuccess ||
+      hipMemcpyAsync(bn.d_items, items.data(), items.size() * sizeof(int2), hipMemcpyHostToDevice, s) != hipSuccess ||
+      hipMemcpyAsync(bn.d_chunk_items, chunk_items.data(), chunk_items.size() * sizeof(int), hipMemcpyHostToDevice,
+                     s) != hipSuccess) {
+    return smh_fail(ctx, SMH_ERR_HIP, "device allocation failed (work items)");
+  }
+  if (bn.n_items > 0)
+    hipLaunchKernelGGL(item_count_kernel<Rec>, dim3(bn.n_items), dim3(256), 0, s, (const Rec*)bn.d_rec,
+                       bn.d_entries, bn.d_items, bn.n_items, bn.n_entries, bn.d_item_en);
+  if (hipStreamSynchronize(s) != hipSuccess) return smh_fail(ctx, SMH_ERR_HIP, "work item statistics failed");
+  return SMH_OK;
+}
+
+}  // namespace

@@ -45,6 +45,14 @@ struct smh_op {
   } bin[2];
   int64_t n_pairs = 0;
 
+  // volume lookup: complex64 tables [len_rho][len_z][B][B] over (rho, z1+z2)
+  // (index 0, "plus") and (rho, z1-z2) (index 1, "minus"), their FP32 tap
+  // tables in fragment order, row sums, and a binning per table and precision
+  float2* d_vtab[2] = {nullptr, nullptr};
+  float* d_vtt[2] = {nullptr, nullptr};
+  float* d_vrowsum[2] = {nullptr, nullptr};
+  Binning vbin[2][2];
+
   // work buffers
   void* d_ypart = nullptr;
   size_t ypart_bytes = 0;
@@ -58,7 +66,15 @@ struct smh_op {
   float last_kernel_ms = 0.f;
 };
 
+void smh_op_free(smh_op* op);
+
 namespace smh {
+// packed -> padded index map of the particles' multipole blocks
+int op_index_map(smh_op* op, const int* lmax, const int* mmax);
+// volume operator: matvec, table upload / download (smh_volume.hip)
+int vol_apply(smh_op* op, int precision, const void* x_pad, void* y_pad, bool time_kernel);
+int vol_upload(smh_op* op, const double* plus, const double* minus);
+int vol_download(smh_op* op, double* plus, double* minus);
 // Launch one coupling matvec on padded device vectors (x -> y, [n_part][B]).
 int op_apply(smh_op* op, int precision, const void* x_pad, void* y_pad, bool time_kernel);
 // Ensure packed host staging buffers for nrhs vectors.

@@ -3,7 +3,7 @@
 #include <cmath>
 
 #include "smh_op.h"
-#include "smh_radial_kernel.h"
+#include "smh_lookup.h"
 
 using namespace smh;
 
@@ -81,24 +81,6 @@ __global__ void pack_kernel(const typename Vec2<Real>::t* __restrict__ pad, cons
   }
 }
 
-// Sum the work-item accumulators of each receiver chunk in fixed order.
-template <typename Real>
-__global__ void reduce_kernel(const Real* __restrict__ ypart, const int* __restrict__ chunk_items, int Rc, int B,
-                              int64_t n_part, typename Vec2<Real>::t* y) {
-  const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= n_part * B) return;
-  const int64_t i1 = j / B;
-  const int n1 = (int)(j - i1 * B);
-  const int c = (int)(i1 / Rc), il = (int)(i1 - (int64_t)c * Rc);
-  Real re = 0, im = 0;
-  for (int w = chunk_items[c]; w < chunk_items[c + 1]; ++w) {
-    const Real* p = ypart + ((size_t)w * Rc + il) * 2 * B;
-    re += p[n1];
-    im += p[B + n1];
-  }
-  y[j] = typename Vec2<Real>::t{re, im};
-}
-
 // Tap table in MFMA fragment order: row j holds reals
 //   F[i][h][g][rr][e] = A_j(16 i + rr, n2 = h KS + 4 g + e),  i < MT, h < G, g < NG,
 // where A_j rows [0, Bh) are Re L_j(n1, n2) and rows [Bh, 2 Bh) are Im L_j
@@ -137,96 +119,6 @@ __global__ void taps_kernel(const double2* __restrict__ table, const double2* __
     tt64[f] = v;
     tt32[f] = (float)v;
   }
-}
-
-// Row sums of the tap rows: rowsum[row] = max_n1 sum_n2 (|Re| + |Im|), with a
-// small margin for the FP32 rounding of the tap table.
-__global__ void tap_rowsum_kernel(const double2* __restrict__ table, const double2* __restrict__ mh, int64_t len,
-                                  int B, int interleave, float* rowsum) {
-  const int64_t n_tt = interleave ? 2 * len : len;
-  for (int64_t row = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; row < n_tt;
-       row += (int64_t)gridDim.x * blockDim.x) {
-    const double2* src = table;
-    int64_t j = row;
-    if (interleave) {
-      j = row >> 1;
-      if (row & 1) src = mh;
-    }
-    double best = 0.0;
-    for (int n1 = 0; n1 < B; ++n1) {
-      double sum = 0.0;
-      for (int n2 = 0; n2 < B; ++n2) {
-        const double2 v = src[(j * B + n1) * B + n2];
-        sum += fabs(v.x) + fabs(v.y);
-      }
-      best = fmax(best, sum);
-    }
-    rowsum[row] = (float)(best * 1.001);
-  }
-}
-
-// Per work item: largest row sum over the tap rows its bin entries use.
-__global__ void item_lsum_kernel(const int4* __restrict__ entries, const int2* __restrict__ items, int n_items,
-                                 int n_entries, int T, const float* __restrict__ rowsum, float* item_lsum) {
-  const int k = blockIdx.x;
-  const int e0 = items[k].y, e1 = k + 1 < n_items ? items[k + 1].y : n_entries;
-  float m = 0.f;
-  for (int e = e0 + (int)threadIdx.x; e < e1; e += blockDim.x)
-    for (int t = 0; t < T; ++t) m = fmaxf(m, rowsum[entries[e].x + t]);
-  __shared__ float red[256];
-  red[threadIdx.x] = m;
-  __syncthreads();
-  for (int w = blockDim.x / 2; w > 0; w >>= 1) {
-    if ((int)threadIdx.x < w) red[threadIdx.x] = fmaxf(red[threadIdx.x], red[threadIdx.x + w]);
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) item_lsum[k] = red[0];
-}
-
-// Per work item: eN = ceil(log2) of the most pairs a single receiver has among
-// the item's pairs (a contiguous range of the records).
-template <typename Real>
-__global__ void item_count_kernel(const PairRec<Real>* __restrict__ recs, const int4* __restrict__ entries,
-                                  const int2* __restrict__ items, int n_items, int n_entries, int* item_en) {
-  __shared__ int hist[512];
-  __shared__ int red[256];
-  const int k = blockIdx.x;
-  const int e0 = items[k].y, e1 = (k + 1 < n_items ? items[k + 1].y : n_entries) - 1;
-  for (int j = threadIdx.x; j < 512; j += blockDim.x) hist[j] = 0;
-  __syncthreads();
-  const int p0 = entries[e0].y, p1 = entries[e1].y + entries[e1].z;
-  for (int p = p0 + (int)threadIdx.x; p < p1; p += blockDim.x) atomicAdd(&hist[recs[p].ij >> kPairShift], 1);
-  __syncthreads();
-  int m = 0;
-  for (int j = threadIdx.x; j < 512; j += blockDim.x) m = max(m, hist[j]);
-  red[threadIdx.x] = m;
-  __syncthreads();
-  for (int w = blockDim.x / 2; w > 0; w >>= 1) {
-    if ((int)threadIdx.x < w) red[threadIdx.x] = max(red[threadIdx.x], red[threadIdx.x + w]);
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) {
-    int eN = 1;
-    while ((1 << eN) < red[0]) ++eN;
-    item_en[k] = eN;
-  }
-}
-
-// Largest |Re x| + |Im x| of the input (one workgroup).
-__global__ void xmax_kernel(const float2* __restrict__ x, int64_t n, float* xmax) {
-  __shared__ float red[1024];
-  float m = 0.f;
-  for (int64_t j = threadIdx.x; j < n; j += blockDim.x) {
-    const float2 v = x[j];
-    m = fmaxf(m, fabsf(v.x) + fabsf(v.y));
-  }
-  red[threadIdx.x] = m;
-  __syncthreads();
-  for (int w = blockDim.x / 2; w > 0; w >>= 1) {
-    if ((int)threadIdx.x < w) red[threadIdx.x] = fmaxf(red[threadIdx.x], red[threadIdx.x + w]);
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) *xmax = red[0];
 }
 
 // Not-a-knot cubic spline moments, scaled: Mh_j = h^2/6 S''(rho_j), one
@@ -279,17 +171,11 @@ __global__ void spline_kernel(const double2* __restrict__ y, const double* __res
   mh[(n - 1) * BB + col] = make_double2(2.0 * mn2.x - mn3.x, 2.0 * mn2.y - mn3.y);
 }
 
-struct KernelInfo {
-  void* fn = nullptr;
-  int threads = 0;
-  int sy = 0;  // accumulator stride per receiver
-};
-
 template <int L, int Mx, int T, typename Real>
 KernelInfo kernel_info() {
   using Sh = RadialShape<Real, SvwfIdx<L, Mx>::B, T>;
   KernelInfo k;
-  k.fn = reinterpret_cast<void*>(&radial_matvec_kernel<Real, L, Mx, T>);
+  k.fn = reinterpret_cast<void*>(&lookup_matvec_kernel<Real, Real, L, Mx, T, false>);
   k.threads = Sh::kThreads;
   k.sy = Sh::SY;
   return k;
@@ -318,7 +204,6 @@ KernelInfo radial_kernel(int precision, int T, int l, int m) {
   return T == 4 ? pick<4, double>(l, m) : pick<2, double>(l, m);
 }
 
-int blocks_for(int64_t n, int t = 256) { return (int)((n + t - 1) / t); }
 
 }  // namespace
 
@@ -394,13 +279,13 @@ int op_make_taps(smh_op* op) {
     SMH_HIP(ctx, hipMalloc(&op->d_rowsum, (size_t)op->n_tt * sizeof(float)));
     SMH_HIP(ctx, hipMalloc(&op->d_xmax, sizeof(float)));
   }
-  hipLaunchKernelGGL(tap_rowsum_kernel, dim3(blocks_for(op->n_tt)), dim3(256), 0, s, op->d_table, d_mh, len, op->B,
-                     op->interp == SMH_INTERP_SPLINE ? 1 : 0, op->d_rowsum);
+  hipLaunchKernelGGL(tap_rowsum_kernel<double2>, dim3(blocks_for(op->n_tt)), dim3(256), 0, s, op->d_table, d_mh,
+                     op->n_tt, op->B, op->interp == SMH_INTERP_SPLINE ? 1 : 0, op->d_rowsum);
   {
     smh_op::Binning& bn = op->bin[0];
     if (bn.n_items > 0)
       hipLaunchKernelGGL(item_lsum_kernel, dim3(bn.n_items), dim3(256), 0, s, bn.d_entries, bn.d_items, bn.n_items,
-                         bn.n_entries, op->T, op->d_rowsum, bn.d_item_lsum);
+                         bn.n_entries, op->T, 0, op->d_rowsum, bn.d_item_lsum);
   }
   SMH_HIP(ctx, hipGetLastError());
   SMH_HIP(ctx, hipStreamSynchronize(s));
@@ -413,7 +298,7 @@ int op_make_taps(smh_op* op) {
 int op_apply(smh_op* op, int precision, const void* x_pad, void* y_pad, bool time_kernel) {
   smh_ctx* ctx = op->ctx;
   SMH_CHECK(ctx, op->has_table, SMH_ERR_STATE, "coupling table not set");
-  SMH_CHECK(ctx, op->kind == 0, SMH_ERR_UNSUPPORTED, "operator kind %d has no MFMA matvec here", op->kind);
+  if (op->kind == 1) return vol_apply(op, precision, x_pad, y_pad, time_kernel);
   const int pi = precision == SMH_FP32 ? 0 : 1;
   const smh_op::Binning& bn = op->bin[pi];
   SMH_CHECK(ctx, bn.Rc > 0, SMH_ERR_UNSUPPORTED, "no %s kernel fits LDS for blocksize %d (l_max %d, m_max %d)",
@@ -448,10 +333,10 @@ int op_apply(smh_op* op, int precision, const void* x_pad, void* y_pad, bool tim
   const int64_t n = op->n_part * op->B;
   if (pi == 0)
     hipLaunchKernelGGL(reduce_kernel<float>, dim3(blocks_for(n)), dim3(256), 0, s, (const float*)op->d_ypart,
-                       bn.d_chunk_items, bn.Rc, op->B, op->n_part, (float2*)y_pad);
+                       bn.d_chunk_items, bn.Rc, op->B, op->n_part, (float2*)y_pad, 0);
   else
     hipLaunchKernelGGL(reduce_kernel<double>, dim3(blocks_for(n)), dim3(256), 0, s, (const double*)op->d_ypart,
-                       bn.d_chunk_items, bn.Rc, op->B, op->n_part, (double2*)y_pad);
+                       bn.d_chunk_items, bn.Rc, op->B, op->n_part, (double2*)y_pad, 0);
   SMH_HIP(ctx, hipGetLastError());
   if (time_kernel && bn.n_items > 0) {
     SMH_HIP(ctx, hipEventSynchronize(ctx->ev1));
@@ -464,30 +349,54 @@ int op_apply(smh_op* op, int precision, const void* x_pad, void* y_pad, bool tim
 
 }  // namespace smh
 
-// Receiver chunk of a precision: as many receivers (at most 512, the pair
-// record's receiver field) as the LDS accumulator [Rc][SY] fits in one
-// workgroup's LDS, less 1 KiB.
-static int chunk_size(const smh_op* op, size_t real_size, int sy, size_t& lds) {
-  const size_t limit = std::min<size_t>(op->ctx->lds_per_block ? op->ctx->lds_per_block : 65536, 163840) - 1024;
-  const size_t per = (size_t)sy * real_size;
-  int rc = (int)std::min<size_t>(512, (limit - 16) / per);
-  if (rc < 16) {
-    lds = 0;
-    return 0;
+namespace smh {
+
+// System-vector layout: packed index (smuthi's particle-major blocks of
+// blocksize(l_max_i, m_max_i)) -> padded index i * B + n of the operator's
+// (l_max, m_max) order; uploaded to the device.
+int op_index_map(smh_op* op, const int* lmax, const int* mmax) {
+  smh_ctx* ctx = op->ctx;
+  op->pk2pad.clear();
+  int64_t n_unk = 0;
+  for (int64_t i = 0; i < op->n_part; ++i) {
+    if (lmax[i] < 1 || mmax[i] < 0 || mmax[i] > lmax[i] || lmax[i] > op->l_max || mmax[i] > op->m_max)
+      return smh_fail(ctx, SMH_ERR_ARG, "particle %lld has l_max %d m_max %d outside the table's %d/%d",
+                      (long long)i, lmax[i], mmax[i], op->l_max, op->m_max);
+    for (int t = 0; t < 2; ++t)
+      for (int l = 1; l <= lmax[i]; ++l) {
+        const int ml = l < mmax[i] ? l : mmax[i];
+        for (int m = -ml; m <= ml; ++m)
+          op->pk2pad.push_back((int)(i * op->B + smh_multi_to_single(t, l, m, op->l_max, op->m_max)));
+      }
+    n_unk += smh_blocksize(lmax[i], mmax[i]);
   }
-  lds = (size_t)rc * per + 16;
-  return rc;
+  op->n_unk = n_unk;
+  SMH_HIP(ctx, hipMalloc(&op->d_pk2pad, n_unk * sizeof(int)));
+  SMH_HIP(ctx, hipMemcpyAsync(op->d_pk2pad, op->pk2pad.data(), n_unk * sizeof(int), hipMemcpyHostToDevice,
+                              ctx->stream));
+  SMH_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return SMH_OK;
 }
 
-static void op_free(smh_op* op) {
+}  // namespace smh
+
+void smh_op_free(smh_op* op) {
   void* ptrs[] = {op->d_xy, op->d_xyz, op->d_pk2pad, op->d_table, op->d_table_minus, op->d_tt64, op->d_tt32,
                   op->d_ypart, op->d_pk_in, op->d_pk_out, op->d_x32, op->d_y32, op->d_x64, op->d_y64,
                   op->d_rowsum, op->d_xmax};
   for (void* p : ptrs)
     if (p) hipFree(p);
-  for (auto& b : op->bin) {
+  auto free_bin = [](smh_op::Binning& b) {
     void* bp[] = {b.d_rec, b.d_entries, b.d_items, b.d_chunk_items, b.d_item_lsum, b.d_item_en};
     for (void* p : bp)
+      if (p) hipFree(p);
+  };
+  for (auto& b : op->bin) free_bin(b);
+  for (auto& row : op->vbin)
+    for (auto& b : row) free_bin(b);
+  for (int t = 0; t < 2; ++t) {
+    void* vp[] = {op->d_vtab[t], op->d_vtt[t], op->d_vrowsum[t]};
+    for (void* p : vp)
       if (p) hipFree(p);
   }
 }
@@ -579,65 +488,8 @@ static int bin_pairs(smh_op* op, int pi, int Rc) {
   }
   cleanup();
 
-  // bin entries (at most kEntryPairs pairs each) grouped into work items of
-  // similar cost; a tile of 16 pairs costs 1, loading a bin's taps 1/4
-  std::vector<int4> entries;
-  std::vector<int> ent_chunk;
-  double total = 0;
-  for (int c = 0; c < bn.n_chunks; ++c)
-    for (int64_t r = 0; r < len_rho; ++r) {
-      const int64_t key = (int64_t)c * len_rho + r;
-      const int n = cnt[key];
-      for (int p0 = 0; p0 < n; p0 += kEntryPairs) {
-        const int np = std::min(kEntryPairs, n - p0);
-        entries.push_back(make_int4((int)(op->s_mode * r + op->o_mode), off[key] + p0, np, 0));
-        ent_chunk.push_back(c);
-        total += (np + 15) / 16 + 0.25;
-      }
-    }
-  const int64_t target_items = std::max<int64_t>(4 * (int64_t)ctx->num_cu, bn.n_chunks);
-  const double budget = std::max(1.0, total / (double)target_items);
-  std::vector<int2> items;
-  std::vector<int> chunk_items(bn.n_chunks + 1, 0);
-  int cur_chunk = -1;
-  double run = 0;
-  for (size_t e = 0; e < entries.size(); ++e) {
-    const int c = ent_chunk[e];
-    if (c != cur_chunk || run >= budget) {
-      while (cur_chunk < c) chunk_items[++cur_chunk] = (int)items.size();
-      items.push_back(make_int2(c, (int)e));
-      run = 0;
-    }
-    run += (entries[e].z + 15) / 16 + 0.25;
-  }
-  while (cur_chunk < bn.n_chunks) chunk_items[++cur_chunk] = (int)items.size();
-  bn.n_entries = (int)entries.size();
-  bn.n_items = (int)items.size();
-  if (hipMalloc(&bn.d_entries, std::max<size_t>(entries.size(), 1) * sizeof(int4)) != hipSuccess ||
-      hipMalloc(&bn.d_items, std::max<size_t>(items.size(), 1) * sizeof(int2)) != hipSuccess ||
-      hipMalloc(&bn.d_chunk_items, chunk_items.size() * sizeof(int)) != hipSuccess ||
-      hipMemcpyAsync(bn.d_entries, entries.data(), entries.size() * sizeof(int4), hipMemcpyHostToDevice, s) !=
-          hipSuccess ||
-      hipMemcpyAsync(bn.d_items, items.data(), items.size() * sizeof(int2), hipMemcpyHostToDevice, s) != hipSuccess ||
-      hipMemcpyAsync(bn.d_chunk_items, chunk_items.data(), chunk_items.size() * sizeof(int), hipMemcpyHostToDevice,
-                     s) != hipSuccess ||
-      hipStreamSynchronize(s) != hipSuccess) {
-    smh_fail(ctx, SMH_ERR_HIP, "device allocation failed (work items)");
-    return SMH_ERR_HIP;
-  }
-  if (hipMalloc(&bn.d_item_lsum, std::max(bn.n_items, 1) * sizeof(float)) != hipSuccess ||
-      hipMalloc(&bn.d_item_en, std::max(bn.n_items, 1) * sizeof(int)) != hipSuccess) {
-    smh_fail(ctx, SMH_ERR_HIP, "device allocation failed (work items)");
-    return SMH_ERR_HIP;
-  }
-  if (bn.n_items > 0)
-    hipLaunchKernelGGL(item_count_kernel<Real>, dim3(bn.n_items), dim3(256), 0, s, (const PairRec<Real>*)bn.d_rec,
-                       bn.d_entries, bn.d_items, bn.n_items, bn.n_entries, bn.d_item_en);
-  if (hipStreamSynchronize(s) != hipSuccess) {
-    smh_fail(ctx, SMH_ERR_HIP, "work item statistics failed");
-    return SMH_ERR_HIP;
-  }
-  return SMH_OK;
+  return build_work_items<PairRec<Real>>(op, bn, cnt, off, len_rho,
+                                         [&](int64_t r) { return (int)(op->s_mode * r + op->o_mode); });
 }
 
 extern "C" int smh_radial_create(smh_ctx* ctx, int64_t n_part, const double* xy, const int* lmax, const int* mmax,
@@ -685,54 +537,19 @@ extern "C" int smh_radial_create(smh_ctx* ctx, int64_t n_part, const double* xy,
   }
   int rc = SMH_OK;
   auto fail = [&](int code) {
-    op_free(op);
+    smh_op_free(op);
     delete op;
     return code;
   };
-  // packed -> padded index map
-  std::vector<int> tau, ll, mm;
-  smh_index_tables(l_max, m_max, tau, ll, mm);
-  (void)tau;
-  int64_t n_unk = 0;
-  for (int64_t i = 0; i < n_part; ++i) {
-    if (lmax[i] < 1 || mmax[i] < 0 || mmax[i] > lmax[i] || lmax[i] > l_max || mmax[i] > m_max) {
-      smh_fail(ctx, SMH_ERR_ARG, "particle %lld has l_max %d m_max %d outside the table's %d/%d", (long long)i,
-               lmax[i], mmax[i], l_max, m_max);
-      return fail(SMH_ERR_ARG);
-    }
-    const int bs = smh_blocksize(lmax[i], mmax[i]);
-    for (int n = 0; n < bs; ++n) {
-      // (tau, l, m) of local index n
-      int t2 = 0, l2 = 0, m2 = 0, k = 0;
-      for (int t = 0; t < 2 && !l2; ++t)
-        for (int l = 1; l <= lmax[i] && !l2; ++l) {
-          int ml = l < mmax[i] ? l : mmax[i];
-          for (int m = -ml; m <= ml; ++m) {
-            if (k == n) {
-              t2 = t;
-              l2 = l;
-              m2 = m;
-              break;
-            }
-            ++k;
-          }
-        }
-      op->pk2pad.push_back((int)(i * op->B + smh_multi_to_single(t2, l2, m2, l_max, m_max)));
-    }
-    n_unk += bs;
-  }
-  op->n_unk = n_unk;
-  if (hipMalloc(&op->d_xy, n_part * sizeof(double2)) != hipSuccess ||
-      hipMalloc(&op->d_pk2pad, n_unk * sizeof(int)) != hipSuccess) {
+  if (int code = op_index_map(op, lmax, mmax)) return fail(code);
+  if (hipMalloc(&op->d_xy, n_part * sizeof(double2)) != hipSuccess) {
     smh_fail(ctx, SMH_ERR_HIP, "device allocation failed");
     return fail(SMH_ERR_HIP);
   }
   hipStream_t s = ctx->stream;
   std::vector<double2> hxy(n_part);
   for (int64_t i = 0; i < n_part; ++i) hxy[i] = make_double2(xy[2 * i], xy[2 * i + 1]);
-  if (hipMemcpyAsync(op->d_xy, hxy.data(), n_part * sizeof(double2), hipMemcpyHostToDevice, s) != hipSuccess ||
-      hipMemcpyAsync(op->d_pk2pad, op->pk2pad.data(), n_unk * sizeof(int), hipMemcpyHostToDevice, s) !=
-          hipSuccess) {
+  if (hipMemcpyAsync(op->d_xy, hxy.data(), n_part * sizeof(double2), hipMemcpyHostToDevice, s) != hipSuccess) {
     smh_fail(ctx, SMH_ERR_HIP, "upload failed");
     return fail(SMH_ERR_HIP);
   }
@@ -775,7 +592,7 @@ extern "C" int smh_op_destroy(smh_op* op) {
   smh_ctx* ctx = op->ctx;
   hipSetDevice(ctx->device);
   hipStreamSynchronize(ctx->stream);
-  op_free(op);
+  smh_op_free(op);
   delete op;
   ctx->live_ops--;
   smh_ctx_release_if_idle(ctx);
@@ -787,30 +604,24 @@ extern "C" int smh_op_upload_table(smh_op* op, const double* table, const double
   smh_ctx* ctx = op->ctx;
   SMH_CHECK(ctx, table != nullptr, SMH_ERR_ARG, "table is NULL");
   SMH_HIP(ctx, hipSetDevice(ctx->device));
-  const int64_t rows = op->kind == 0 ? op->len_rho : op->len_rho * op->len_z;
-  const size_t bytes = (size_t)rows * op->B * op->B * sizeof(double2);
-  if (!op->d_table) SMH_HIP(ctx, hipMalloc(&op->d_table, bytes));
-  SMH_HIP(ctx, hipMemcpyAsync(op->d_table, table, bytes, hipMemcpyHostToDevice, ctx->stream));
   if (op->kind == 1) {
     SMH_CHECK(ctx, table_minus != nullptr, SMH_ERR_ARG, "volume operator needs the minus table");
-    if (!op->d_table_minus) SMH_HIP(ctx, hipMalloc(&op->d_table_minus, bytes));
-    SMH_HIP(ctx, hipMemcpyAsync(op->d_table_minus, table_minus, bytes, hipMemcpyHostToDevice, ctx->stream));
-    SMH_HIP(ctx, hipStreamSynchronize(ctx->stream));
-    op->has_table = true;
-    return SMH_OK;
+    return vol_upload(op, table, table_minus);
   }
+  const size_t bytes = (size_t)op->len_rho * op->B * op->B * sizeof(double2);
+  if (!op->d_table) SMH_HIP(ctx, hipMalloc(&op->d_table, bytes));
+  SMH_HIP(ctx, hipMemcpyAsync(op->d_table, table, bytes, hipMemcpyHostToDevice, ctx->stream));
   return op_make_taps(op);
 }
 
 extern "C" int smh_op_download_table(smh_op* op, double* table, double* table_minus) {
   if (!op) return SMH_ERR_ARG;
   smh_ctx* ctx = op->ctx;
+  SMH_HIP(ctx, hipSetDevice(ctx->device));
+  if (op->kind == 1) return vol_download(op, table, table_minus);
   SMH_CHECK(ctx, op->d_table != nullptr, SMH_ERR_STATE, "no table");
-  const int64_t rows = op->kind == 0 ? op->len_rho : op->len_rho * op->len_z;
-  const size_t bytes = (size_t)rows * op->B * op->B * sizeof(double2);
+  const size_t bytes = (size_t)op->len_rho * op->B * op->B * sizeof(double2);
   SMH_HIP(ctx, hipMemcpyAsync(table, op->d_table, bytes, hipMemcpyDeviceToHost, ctx->stream));
-  if (op->kind == 1 && table_minus)
-    SMH_HIP(ctx, hipMemcpyAsync(table_minus, op->d_table_minus, bytes, hipMemcpyDeviceToHost, ctx->stream));
   SMH_HIP(ctx, hipStreamSynchronize(ctx->stream));
   return SMH_OK;
 }

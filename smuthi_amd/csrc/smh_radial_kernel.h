@@ -104,6 +104,21 @@ struct PairRec<double> {
   double w, c, s;
 };
 
+// A pair of the volume lookup: two interpolation weights (distance and z
+// coordinate of the node cell).
+template <typename Real>
+struct PairRec2;
+template <>
+struct PairRec2<float> {
+  uint32_t ij;
+  float w, w2, c, s;
+};
+template <>
+struct PairRec2<double> {
+  uint32_t ij, pad;
+  double w, w2, c, s;
+};
+
 // Largest number of pairs in one bin entry (a bin of a chunk with more pairs
 // is split into several entries so waves stay balanced).
 constexpr int kEntryPairs = 256;
@@ -119,6 +134,7 @@ struct RadialKArgs {
   const float* item_lsum;   // FP32: per item, largest tap-row sum of |Re L| + |Im L| over its bins
   const int* item_en;       // FP32: per item, ceil(log2) of the most pairs one receiver has in it
   int n_items, n_entries, interp, Rc;
+  int len_z;                // 2-D lookups: table rows per distance node (tap t sits at (t/2) len_z + t%2)
 };
 
 // FP32 receiver accumulation in LDS is fixed point: a contribution v is split
@@ -215,13 +231,18 @@ struct RadialShape {
   static constexpr int SY = sizeof(Real) == 4 ? 4 * B + 1 : 2 * B + 1;
 };
 
-template <typename Real, int LMAX, int MMAX, int T>
+// Lookup matvec. VOL = false: radial lookup, taps are consecutive tap-table
+// rows, weights from the 1-D interpolant. VOL = true: bilinear volume lookup
+// over nodes (distance r, z index j) laid out row r * len_z + j, taps
+// (r, j), (r, j+1), (r+1, j), (r+1, j+1). TapT: storage type of the tap table
+// (float for FP32, and for FP64 over complex64 tables, which float holds exactly).
+template <typename Real, typename TapT, int LMAX, int MMAX, int T, bool VOL>
 __global__ void __launch_bounds__((RadialShape<Real, SvwfIdx<LMAX, MMAX>::B, T>::kThreads))
-    radial_matvec_kernel(RadialKArgs a) {
+    lookup_matvec_kernel(RadialKArgs a) {
   using I = SvwfIdx<LMAX, MMAX>;
   using M = Mfma<Real>;
   using R2 = typename Vec2<Real>::t;
-  using Rec = PairRec<Real>;
+  using Rec = typename std::conditional<VOL, PairRec2<Real>, PairRec<Real>>::type;
   using Sh = RadialShape<Real, I::B, T>;
   constexpr int B = I::B, Bh = I::Bh, MMX = MMAX;
   constexpr TapGeom g{B, T};
@@ -243,7 +264,7 @@ __global__ void __launch_bounds__((RadialShape<Real, SvwfIdx<LMAX, MMAX>::B, T>:
   const int e_end = item + 1 < a.n_items ? a.items[item + 1].y : a.n_entries;
   const Rec* recs = reinterpret_cast<const Rec*>(a.recs);
   const R2* xg = reinterpret_cast<const R2*>(a.x);
-  const Real* ttg = reinterpret_cast<const Real*>(a.tt);
+  const TapT* ttg = reinterpret_cast<const TapT*>(a.tt);
   for (int j = tid; j < a.Rc * SY; j += blockDim.x) yacc[j] = AccT(0);
   if (tid == 0) *next_entry = it.y;
   __syncthreads();
@@ -257,7 +278,8 @@ __global__ void __launch_bounds__((RadialShape<Real, SvwfIdx<LMAX, MMAX>::B, T>:
     e = __shfl(e, 0);
     if (e >= e_end) break;
     const int4 en = a.entries[e];  // (first tap row, first pair, pairs)
-    const Real* trow = ttg + (size_t)(en.x + tap) * FR;
+    const int toff = VOL ? (tap >> 1) * a.len_z + (tap & 1) : tap;
+    const TapT* trow = ttg + (size_t)(en.x + toff) * FR;
     const int ntiles = (en.z + 15) >> 4;
 
 #pragma unroll
@@ -270,8 +292,8 @@ __global__ void __launch_bounds__((RadialShape<Real, SvwfIdx<LMAX, MMAX>::B, T>:
         for (int gl = 0; gl < NGS; ++gl) {
           const int gg = sp * NGS + gl;
           if (gg < NG) {
-            const Real* src = trow + ((size_t)((i * G + hq) * NG + gg) * 16 + col) * 4;
-            if constexpr (sizeof(Real) == 4) {
+            const TapT* src = trow + ((size_t)((i * G + hq) * NG + gg) * 16 + col) * 4;
+            if constexpr (sizeof(TapT) == 4) {
               const float4 v = *reinterpret_cast<const float4*>(src);
               af[i][gl][0] = v.x;
               af[i][gl][1] = v.y;
@@ -315,7 +337,12 @@ __global__ void __launch_bounds__((RadialShape<Real, SvwfIdx<LMAX, MMAX>::B, T>:
         if (SMH_X_PREFETCH) load_x(rec1, x1);
         const Rec rec = rec0;
         const bool valid = tl * 16 + col < en.z;
-        const Real wq = valid ? tap_weight<Real>(a.interp, tap, rec.w) : Real(0);
+        Real wq;
+        if constexpr (VOL)
+          wq = ((tap >> 1) ? rec.w : Real(1) - rec.w) * ((tap & 1) ? rec.w2 : Real(1) - rec.w2);
+        else
+          wq = tap_weight<Real>(a.interp, tap, rec.w);
+        if (!valid) wq = Real(0);
         // e^{i m phi}, m = -MMAX..MMAX
         cplx<Real> E[NE];
         E[MMX] = {Real(1), Real(0)};

@@ -1,0 +1,608 @@
+// Volume-lookup coupling operator (CouplingMatrixVolumeLookupCUDA,
+// linearsystem/linear_system.py:688-783; tables of prepare_lookup.py:16-250):
+// particles at arbitrary heights in one layer. The coupling block of a pair
+// is Lpl(rho, z1 + z2) + Lmn(rho, z1 - z2), both complex64 tables on a
+// uniform (rho, z) grid of one resolution, bilinearly interpolated.
+//
+// Table build (FP64, stored complex64 as the reference stores them):
+//   Lpl[r, j] = float( sum_k J_|dm|(kappa_k rho_r) (c+pl_k e^{i kz (sz_j - 2 z_s)} + c-pl_k e^{-i kz (sz_j - 2 z_s)}) )
+//   Lmn[r, j] = float(direct(rho_r, dz_j)) + float( same with c+-mn and e^{+-i kz dz_j} )
+//   direct(rho, dz) = sum_ld coef[n1, n2, ld] h_ld(k d) P_ld^|dm|(dz / d),  d = sqrt(rho^2 + dz^2),
+// zero for rho below the mask radius; the contour weights and layer response
+// are folded into c+-pl/mn on the host.
+//
+// Matvec: the binned MFMA lookup kernel of the radial operator, once per
+// table, with bilinear taps (r, j), (r, j+1), (r+1, j), (r+1, j+1): pairs are
+// binned by (receiver chunk, r, j) so every bin entry shares its four tap
+// rows. The FP64 kernel reads the same FP32 tap table (complex64 data, exact).
+#include <cmath>
+
+#include "smh_lookup.h"
+#include "smh_special.h"
+
+namespace {
+
+constexpr int kMaxL2 = 16;  // 2 l_max <= 16
+
+// Normalized associated Legendre P_l^m(cos t) for real (ct, st), l <= lmax,
+// m <= l, P[l * (lmax + 1) + m] (the normalization of smuthi's
+// legendre_normalized, utility/math.py:38-104).
+__device__ void legendre_p(int lmax, double ct, double st, double* P) {
+  const int W = lmax + 1;
+  for (int j = 0; j < W * W; ++j) P[j] = 0.0;
+  P[0] = sqrt(0.5);
+  if (lmax >= 1) P[W] = sqrt(1.5) * ct;
+  for (int l = 1; l < lmax; ++l) {
+    const double a = sqrt((2.0 * l + 1) * (2.0 * l + 3)) / (l + 1);
+    const double b = (double)l / (l + 1) * sqrt((2.0 * l + 3) / (2.0 * l - 1));
+    P[(l + 1) * W] = a * ct * P[l * W] - b * P[(l - 1) * W];
+  }
+  double prod = 1.0;
+  for (int m = 1; m <= lmax; ++m) {
+    prod *= (2.0 * m + 1) / (2.0 * m);
+    double stm = 1.0;
+    for (int i = 0; i < m; ++i) stm *= st;
+    P[m * W + m] = sqrt(prod / 2.0) * stm;
+    for (int l = m; l < lmax; ++l) {
+      const double den = (double)(l + 1 - m) * (l + 1 + m);
+      const double a = sqrt((2.0 * l + 1) * (2.0 * l + 3) / den);
+      const double b = sqrt((2.0 * l + 3) * (l - m) * (l + m) / ((2.0 * l - 1) * den));
+      P[(l + 1) * W + m] = a * ct * P[l * W + m] - (l - 1 >= m ? b * P[(l - 1) * W + m] : 0.0);
+    }
+  }
+}
+
+// Direct part of the minus table, rounded to complex64; one block per node.
+__global__ void vol_direct_kernel(double rho_min, double dz_min, double h, int64_t len_rho, int64_t len_z, int B,
+                                  int L2, double kre, double kim, double mask, const double2* __restrict__ coef,
+                                  const int* __restrict__ dm_of_p, float2* table) {
+  const int64_t node = blockIdx.x;
+  const int64_t r = node / len_z, j = node - r * len_z;
+  const double rho = rho_min + (double)r * h, dz = dz_min + (double)j * h;
+  const int BB = B * B;
+  float2* row = table + node * BB;
+  if (rho < mask) {
+    for (int p = threadIdx.x; p < BB; p += blockDim.x) row[p] = make_float2(0.f, 0.f);
+    return;
+  }
+  __shared__ zc hl[kMaxL2 + 1];
+  __shared__ double P[(kMaxL2 + 1) * (kMaxL2 + 1)];
+  if (threadIdx.x == 0) {
+    double d = sqrt(rho * rho + dz * dz);
+    if (d == 0.0) d = 1e-20;
+    smh::spherical_hankel_all(L2, zc{kre * d, kim * d}, hl);
+    legendre_p(L2, dz / d, rho / d, P);
+  }
+  __syncthreads();
+  for (int p = threadIdx.x; p < BB; p += blockDim.x) {
+    const int dm = dm_of_p[p];
+    zc s{0.0, 0.0};
+    for (int ld = dm; ld <= L2; ++ld) {
+      const double2 c = coef[(size_t)p * (L2 + 1) + ld];
+      const double pl = P[ld * (L2 + 1) + dm];
+      s = smh::zadd(s, smh::zmul(zc{c.x * pl, c.y * pl}, hl[ld]));
+    }
+    row[p] = make_float2((float)s.re, (float)s.im);
+  }
+}
+
+// Layer-mediated parts of both tables. Block: one rho node and kZT z nodes;
+// per chunk of kappa the Bessel values and the z phases are staged in LDS,
+// each thread owns one (n1, n2) entry. plus += float(layer), minus = float(minus) + float(layer).
+constexpr int kZT = 8;
+constexpr int kKC = 16;
+constexpr int kNJ = 9;
+__global__ void vol_layer_kernel(double rho_min, double sz_min, double dz_min, double h, int64_t len_z, int B,
+                                 int nmax, double z_s, const int* __restrict__ dm_of_p, int64_t n_kappa,
+                                 const double2* __restrict__ kappa, const double2* __restrict__ kz,
+                                 const double2* __restrict__ c, float2* plus, float2* minus) {
+  __shared__ zc J[kKC][kNJ];
+  __shared__ zc E[kKC][4][kZT];  // e^{+i kz (sz - 2 z_s)}, e^{-i ...}, e^{+i kz dz}, e^{-i kz dz}
+  const int64_t r = blockIdx.x;
+  const int64_t j0 = (int64_t)blockIdx.y * kZT;
+  const int nz = (int)min<int64_t>(kZT, len_z - j0);
+  const double rho = rho_min + (double)r * h;
+  const int BB = B * B;
+  for (int p0 = 0; p0 < BB; p0 += blockDim.x) {
+    const int p = p0 + threadIdx.x;
+    const bool act = p < BB;
+    const int dm = act ? dm_of_p[p] : 0;
+    zc apl[kZT], amn[kZT];
+    for (int z = 0; z < kZT; ++z) apl[z] = amn[z] = zc{0.0, 0.0};
+    for (int64_t k0 = 0; k0 < n_kappa; k0 += kKC) {
+      const int kc = (int)min<int64_t>(kKC, n_kappa - k0);
+      __syncthreads();
+      for (int e = threadIdx.x; e < kKC; e += blockDim.x) {
+        if (e < kc) {
+          zc buf[smh::kMaxBesselOrder + 1];
+          const double2 kp = kappa[k0 + e];
+          smh::bessel_j_all(nmax, zc{kp.x * rho, kp.y * rho}, buf);
+          for (int n = 0; n <= nmax; ++n) J[e][n] = buf[n];
+        }
+      }
+      for (int e = threadIdx.x; e < kKC * kZT; e += blockDim.x) {
+        const int kk = e / kZT, z = e - kk * kZT;
+        if (kk < kc && z < nz) {
+          const double2 q = kz[k0 + kk];
+          const double sz = sz_min + (double)(j0 + z) * h - 2.0 * z_s, dz = dz_min + (double)(j0 + z) * h;
+          // e^{i q t} = e^{-Im q t} (cos(Re q t) + i sin(Re q t))
+          auto cis = [&](double t, double sgn) {
+            const double m = exp(-sgn * q.y * t);
+            double sn, cn;
+            sincos(sgn * q.x * t, &sn, &cn);
+            return zc{m * cn, m * sn};
+          };
+          E[kk][0][z] = cis(sz, 1.0);
+          E[kk][1][z] = cis(sz, -1.0);
+          E[kk][2][z] = cis(dz, 1.0);
+          E[kk][3][z] = cis(dz, -1.0);
+        }
+      }
+      __syncthreads();
+      if (act) {
+        for (int kk = 0; kk < kc; ++kk) {
+          const size_t base = ((size_t)(k0 + kk) * 4) * BB + p;
+          const double2 cp = c[base], cm = c[base + BB], dp = c[base + 2 * (size_t)BB], dn = c[base + 3 * (size_t)BB];
+          const zc jj = J[kk][dm];
+          const zc jcp = smh::zmul(jj, zc{cp.x, cp.y}), jcm = smh::zmul(jj, zc{cm.x, cm.y});
+          const zc jdp = smh::zmul(jj, zc{dp.x, dp.y}), jdn = smh::zmul(jj, zc{dn.x, dn.y});
+          for (int z = 0; z < kZT; ++z) {
+            apl[z] = smh::zadd(apl[z], smh::zadd(smh::zmul(jcp, E[kk][0][z]), smh::zmul(jcm, E[kk][1][z])));
+            amn[z] = smh::zadd(amn[z], smh::zadd(smh::zmul(jdp, E[kk][2][z]), smh::zmul(jdn, E[kk][3][z])));
+          }
+        }
+      }
+    }
+    if (act)
+      for (int z = 0; z < nz; ++z) {
+        const size_t o = ((size_t)r * len_z + j0 + z) * BB + p;
+        plus[o] = make_float2((float)apl[z].re, (float)apl[z].im);
+        const float2 d = minus[o];
+        minus[o] = make_float2(d.x + (float)amn[z].re, d.y + (float)amn[z].im);
+      }
+  }
+}
+
+// FP32 tap table in fragment order (T = 4, one lane quarter per tap):
+// row j holds F[i][g][rr][e] = A_j(16 i + rr, n2 = 4 g + e), A rows [0, Bh) = Re, [Bh, 2 Bh) = Im.
+__global__ void vol_taps_kernel(const float2* __restrict__ table, int64_t rows, int B, int Bh, int NG, int FR,
+                                float* tt) {
+  const int64_t total = rows * FR;
+  for (int64_t f = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; f < total; f += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t row = f / FR;
+    int rem = (int)(f - row * FR);
+    const int e = rem & 3, rr = (rem >> 2) & 15;
+    rem >>= 6;
+    const int g = rem % NG, i = rem / NG;
+    const int m = 16 * i + rr, n2 = 4 * g + e;
+    float v = 0.f;
+    if (n2 < B) {
+      if (m < B)
+        v = table[(row * B + m) * B + n2].x;
+      else if (m >= Bh && m < Bh + B)
+        v = table[(row * B + (m - Bh)) * B + n2].y;
+    }
+    tt[f] = v;
+  }
+}
+
+// Bilinear cell of a pair in one table: distance node r and z node j of
+// zval = z1 + zsgn * z2, with weights inside the cell.
+__host__ __device__ inline void vol_geometry(const double* a, const double* b, double rho_min, double h, double zmin,
+                                             double zsgn, int& r, int& j, double& wr, double& wz, double& rho,
+                                             double& dx, double& dy) {
+  dx = a[0] - b[0];
+  dy = a[1] - b[1];
+  double w;
+  pair_geometry(dx, dy, rho_min, h, r, w, rho);
+  wr = w;
+  const double t = (a[2] + zsgn * b[2] - zmin) / h;
+  const double fl = floor(t);
+  j = (int)fl;
+  wz = t - fl;
+}
+
+__global__ void vol_pair_bin_kernel(const double* __restrict__ xyz, int n, double rho_min, double h, int len_rho,
+                                    double zmin, double zsgn, int len_z, int Rc, int* cnt, const int* off,
+                                    int* cursor, uint2* pairs, int* err) {
+  const int nb2 = (n + 255) / 256;
+  const int64_t b = blockIdx.x;
+  const int i1 = (int)(b / nb2);
+  const int i2 = (int)(b % nb2) * 256 + threadIdx.x;
+  if (i1 >= n || i2 >= n) return;
+  int r, j;
+  double wr, wz, rho, dx, dy;
+  vol_geometry(xyz + 3 * (size_t)i1, xyz + 3 * (size_t)i2, rho_min, h, zmin, zsgn, r, j, wr, wz, rho, dx, dy);
+  if (r < 0 || r >= len_rho - 1 || j < 0 || j >= len_z - 1) {
+    atomicOr(err, 1);
+    return;
+  }
+  const int64_t key = ((int64_t)(i1 / Rc) * len_rho + r) * len_z + j;
+  if (pairs == nullptr) {
+    atomicAdd(&cnt[key], 1);
+  } else {
+    const int pos = off[key] + atomicAdd(&cursor[key], 1);
+    pairs[pos] = make_uint2((unsigned)i1, (unsigned)i2);
+  }
+}
+
+template <typename Real>
+__global__ void vol_pair_record_kernel(const uint2* __restrict__ pairs, int64_t n_pairs, const double* __restrict__ xyz,
+                                       double rho_min, double h, double zmin, double zsgn, int Rc, PairRec2<Real>* rec) {
+  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n_pairs) return;
+  const uint2 pr = pairs[p];
+  int r, j;
+  double wr, wz, rho, dx, dy;
+  vol_geometry(xyz + 3 * (size_t)pr.x, xyz + 3 * (size_t)pr.y, rho_min, h, zmin, zsgn, r, j, wr, wz, rho, dx, dy);
+  double c = 1.0, sn = 0.0;
+  if (rho > 0.0) {
+    c = dx / rho;
+    sn = dy / rho;
+  }
+  PairRec2<Real> out{};
+  out.ij = pr.y | ((pr.x % (unsigned)Rc) << kPairShift);
+  out.w = (Real)wr;
+  out.w2 = (Real)wz;
+  out.c = (Real)c;
+  out.s = (Real)sn;
+  rec[p] = out;
+}
+
+template <int L, int Mx, typename Real>
+KernelInfo vol_kernel_info() {
+  using Sh = RadialShape<Real, SvwfIdx<L, Mx>::B, 4>;
+  KernelInfo k;
+  k.fn = reinterpret_cast<void*>(&lookup_matvec_kernel<Real, float, L, Mx, 4, true>);
+  k.threads = Sh::kThreads;
+  k.sy = Sh::SY;
+  return k;
+}
+
+KernelInfo volume_kernel(int precision, int l, int m) {
+#define SMH_V(L, Mx)                                                                                    \
+  if (l == L && m == Mx)                                                                                \
+    return precision == SMH_FP32 ? vol_kernel_info<L, Mx, float>() : vol_kernel_info<L, Mx, double>();
+  SMH_V(1, 1)
+  SMH_V(2, 1)
+  SMH_V(2, 2)
+  SMH_V(3, 1)
+  SMH_V(3, 2)
+  SMH_V(3, 3)
+  SMH_V(4, 1)
+  SMH_V(4, 2)
+  SMH_V(4, 3)
+  SMH_V(4, 4)
+#undef SMH_V
+  return KernelInfo{};
+}
+
+// Bin the pairs of table tab (0: z1 + z2, 1: z1 - z2) for precision pi.
+template <typename Real>
+int vol_bin_pairs(smh_op* op, int tab, int pi, int Rc) {
+  smh_ctx* ctx = op->ctx;
+  smh_op::Binning& bn = op->vbin[tab][pi];
+  hipStream_t s = ctx->stream;
+  const int64_t n_part = op->n_part, len_rho = op->len_rho, len_z = op->len_z;
+  const double zmin = tab == 0 ? op->sz_min : op->dz_min, zsgn = tab == 0 ? 1.0 : -1.0;
+  bn.Rc = Rc;
+  bn.n_chunks = (int)((n_part + Rc - 1) / Rc);
+  const int64_t nkeys = (int64_t)bn.n_chunks * len_rho * len_z;
+  SMH_CHECK(ctx, nkeys < ((int64_t)1 << 31), SMH_ERR_UNSUPPORTED,
+            "volume table too large for the pair binning (%lld bins)", (long long)nkeys);
+  int *d_cnt = nullptr, *d_cur = nullptr, *d_err = nullptr, *d_off = nullptr;
+  uint2* d_pairs = nullptr;
+  auto cleanup = [&]() {
+    void* p[] = {d_cnt, d_cur, d_err, d_off, d_pairs};
+    for (void* q : p)
+      if (q) hipFree(q);
+  };
+  if (hipMalloc(&d_cnt, (nkeys + 1) * sizeof(int)) != hipSuccess ||
+      hipMalloc(&d_cur, (nkeys + 1) * sizeof(int)) != hipSuccess || hipMalloc(&d_err, sizeof(int)) != hipSuccess ||
+      hipMalloc(&d_off, (nkeys + 1) * sizeof(int)) != hipSuccess) {
+    cleanup();
+    return smh_fail(ctx, SMH_ERR_HIP, "device allocation failed (volume binning)");
+  }
+  hipMemsetAsync(d_cnt, 0, (nkeys + 1) * sizeof(int), s);
+  hipMemsetAsync(d_cur, 0, (nkeys + 1) * sizeof(int), s);
+  hipMemsetAsync(d_err, 0, sizeof(int), s);
+  const int nb2 = (int)((n_part + 255) / 256);
+  const int64_t nblocks = n_part * nb2;
+  hipLaunchKernelGGL(vol_pair_bin_kernel, dim3((unsigned)nblocks), dim3(256), 0, s, op->d_xyz, (int)n_part,
+                     op->rho_min, op->h, (int)len_rho, zmin, zsgn, (int)len_z, Rc, d_cnt, (const int*)nullptr, d_cur,
+                     (uint2*)nullptr, d_err);
+  std::vector<int> cnt(nkeys);
+  int herr = 0;
+  if (hipMemcpyAsync(cnt.data(), d_cnt, nkeys * sizeof(int), hipMemcpyDeviceToHost, s) != hipSuccess ||
+      hipMemcpyAsync(&herr, d_err, sizeof(int), hipMemcpyDeviceToHost, s) != hipSuccess ||
+      hipStreamSynchronize(s) != hipSuccess) {
+    cleanup();
+    return smh_fail(ctx, SMH_ERR_HIP, "volume binning count failed");
+  }
+  if (herr) {
+    cleanup();
+    return smh_fail(ctx, SMH_ERR_ARG, "a particle pair lies outside the volume lookup table");
+  }
+  std::vector<int> off(nkeys + 1);
+  int64_t acc = 0;
+  for (int64_t k2 = 0; k2 < nkeys; ++k2) {
+    off[k2] = (int)acc;
+    acc += cnt[k2];
+  }
+  off[nkeys] = (int)acc;
+  if (acc >= (int64_t)INT32_MAX) {
+    cleanup();
+    return smh_fail(ctx, SMH_ERR_UNSUPPORTED, "too many particle pairs for 32-bit offsets");
+  }
+  if (hipMalloc(&d_pairs, std::max<int64_t>(acc, 1) * sizeof(uint2)) != hipSuccess ||
+      hipMemcpyAsync(d_off, off.data(), (nkeys + 1) * sizeof(int), hipMemcpyHostToDevice, s) != hipSuccess ||
+      hipMalloc(&bn.d_rec, std::max<int64_t>(acc, 1) * sizeof(PairRec2<Real>)) != hipSuccess) {
+    cleanup();
+    return smh_fail(ctx, SMH_ERR_HIP, "device allocation failed (volume pairs)");
+  }
+  hipLaunchKernelGGL(vol_pair_bin_kernel, dim3((unsigned)nblocks), dim3(256), 0, s, op->d_xyz, (int)n_part,
+                     op->rho_min, op->h, (int)len_rho, zmin, zsgn, (int)len_z, Rc, d_cnt, (const int*)d_off, d_cur,
+                     d_pairs, d_err);
+  if (acc > 0)
+    hipLaunchKernelGGL(vol_pair_record_kernel<Real>, dim3((unsigned)((acc + 255) / 256)), dim3(256), 0, s, d_pairs,
+                       acc, op->d_xyz, op->rho_min, op->h, zmin, zsgn, Rc, (PairRec2<Real>*)bn.d_rec);
+  if (hipStreamSynchronize(s) != hipSuccess) {
+    cleanup();
+    return smh_fail(ctx, SMH_ERR_HIP, "volume pair records failed");
+  }
+  cleanup();
+  op->n_pairs = acc;
+  // a (chunk, r, j) key is a bin; its entry starts at table row r * len_z + j
+  return build_work_items<PairRec2<Real>>(op, bn, cnt, off, len_rho * len_z, [](int64_t rj) { return (int)rj; });
+}
+
+// FP32 tap tables, row sums and per-item statistics from the complex64 tables.
+int vol_make_taps(smh_op* op) {
+  smh_ctx* ctx = op->ctx;
+  hipStream_t s = ctx->stream;
+  const TapGeom g(op->B, 4);
+  op->FR = g.FR;
+  const int64_t rows = op->len_rho * op->len_z;
+  for (int t = 0; t < 2; ++t) {
+    if (!op->d_vtt[t]) {
+      SMH_HIP(ctx, hipMalloc(&op->d_vtt[t], (size_t)rows * g.FR * sizeof(float)));
+      SMH_HIP(ctx, hipMalloc(&op->d_vrowsum[t], (size_t)rows * sizeof(float)));
+    }
+    hipLaunchKernelGGL(vol_taps_kernel, dim3(4096), dim3(256), 0, s, op->d_vtab[t], rows, op->B, op->Bh, g.NG, g.FR,
+                       op->d_vtt[t]);
+    hipLaunchKernelGGL(tap_rowsum_kernel<float2>, dim3(blocks_for(rows)), dim3(256), 0, s, op->d_vtab[t],
+                       (const float2*)nullptr, rows, op->B, 0, op->d_vrowsum[t]);
+    smh_op::Binning& bn = op->vbin[t][0];
+    if (bn.n_items > 0)
+      hipLaunchKernelGGL(item_lsum_kernel, dim3(bn.n_items), dim3(256), 0, s, bn.d_entries, bn.d_items, bn.n_items,
+                         bn.n_entries, 4, (int)op->len_z, op->d_vrowsum[t], bn.d_item_lsum);
+  }
+  SMH_HIP(ctx, hipGetLastError());
+  SMH_HIP(ctx, hipStreamSynchronize(s));
+  op->has_table = true;
+  return SMH_OK;
+}
+
+}  // namespace
+
+namespace smh {
+
+int vol_apply(smh_op* op, int precision, const void* x_pad, void* y_pad, bool time_kernel) {
+  smh_ctx* ctx = op->ctx;
+  const int pi = precision == SMH_FP32 ? 0 : 1;
+  const KernelInfo k = volume_kernel(precision, op->l_max, op->m_max);
+  SMH_CHECK(ctx, k.fn != nullptr && op->vbin[0][pi].Rc > 0, SMH_ERR_UNSUPPORTED,
+            "no volume matvec kernel for l_max %d m_max %d", op->l_max, op->m_max);
+  hipStream_t s = ctx->stream;
+  if (pi == 0)
+    hipLaunchKernelGGL(xmax_kernel, dim3(1), dim3(1024), 0, s, (const float2*)x_pad, op->n_part * op->B, op->d_xmax);
+  double ms_total = 0;
+  for (int t = 0; t < 2; ++t) {
+    const smh_op::Binning& bn = op->vbin[t][pi];
+    RadialKArgs a;
+    a.recs = bn.d_rec;
+    a.entries = bn.d_entries;
+    a.items = bn.d_items;
+    a.tt = op->d_vtt[t];
+    a.x = x_pad;
+    a.ypart = op->d_ypart;
+    a.xmax = op->d_xmax;
+    a.item_lsum = bn.d_item_lsum;
+    a.item_en = bn.d_item_en;
+    a.n_items = bn.n_items;
+    a.n_entries = bn.n_entries;
+    a.interp = SMH_INTERP_LINEAR;
+    a.Rc = bn.Rc;
+    a.len_z = (int)op->len_z;
+    SMH_HIP(ctx, hipFuncSetAttribute(k.fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bn.lds));
+    if (bn.n_items > 0) {
+      if (time_kernel) SMH_HIP(ctx, hipEventRecord(ctx->ev0, s));
+      void* args[] = {&a};
+      SMH_HIP(ctx, hipLaunchKernel(k.fn, dim3(bn.n_items), dim3(k.threads), args, bn.lds, s));
+      if (time_kernel) {
+        SMH_HIP(ctx, hipEventRecord(ctx->ev1, s));
+        SMH_HIP(ctx, hipEventSynchronize(ctx->ev1));
+        float ms = 0;
+        SMH_HIP(ctx, hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+        ms_total += ms;
+      }
+    }
+    const int64_t n = op->n_part * op->B;
+    if (pi == 0)
+      hipLaunchKernelGGL(reduce_kernel<float>, dim3(blocks_for(n)), dim3(256), 0, s, (const float*)op->d_ypart,
+                         bn.d_chunk_items, bn.Rc, op->B, op->n_part, (float2*)y_pad, t);
+    else
+      hipLaunchKernelGGL(reduce_kernel<double>, dim3(blocks_for(n)), dim3(256), 0, s, (const double*)op->d_ypart,
+                         bn.d_chunk_items, bn.Rc, op->B, op->n_part, (double2*)y_pad, t);
+    SMH_HIP(ctx, hipGetLastError());
+  }
+  if (time_kernel) op->last_kernel_ms = (float)ms_total;
+  return SMH_OK;
+}
+
+int vol_upload(smh_op* op, const double* plus, const double* minus) {
+  smh_ctx* ctx = op->ctx;
+  const size_t n = (size_t)op->len_rho * op->len_z * op->B * op->B;
+  std::vector<float2> h(n);
+  const double* src[2] = {plus, minus};
+  for (int t = 0; t < 2; ++t) {
+    for (size_t i = 0; i < n; ++i) h[i] = make_float2((float)src[t][2 * i], (float)src[t][2 * i + 1]);
+    if (!op->d_vtab[t]) SMH_HIP(ctx, hipMalloc(&op->d_vtab[t], n * sizeof(float2)));
+    SMH_HIP(ctx, hipMemcpy(op->d_vtab[t], h.data(), n * sizeof(float2), hipMemcpyHostToDevice));
+  }
+  return vol_make_taps(op);
+}
+
+int vol_download(smh_op* op, double* plus, double* minus) {
+  smh_ctx* ctx = op->ctx;
+  SMH_CHECK(ctx, op->d_vtab[0] != nullptr, SMH_ERR_STATE, "no table");
+  const size_t n = (size_t)op->len_rho * op->len_z * op->B * op->B;
+  std::vector<float2> h(n);
+  double* dst[2] = {plus, minus};
+  for (int t = 0; t < 2; ++t) {
+    if (!dst[t]) continue;
+    SMH_HIP(ctx, hipMemcpy(h.data(), op->d_vtab[t], n * sizeof(float2), hipMemcpyDeviceToHost));
+    for (size_t i = 0; i < n; ++i) {
+      dst[t][2 * i] = h[i].x;
+      dst[t][2 * i + 1] = h[i].y;
+    }
+  }
+  return SMH_OK;
+}
+
+}  // namespace smh
+
+extern "C" int smh_volume_create(smh_ctx* ctx, int64_t n_part, const double* xyz, const int* lmax, const int* mmax,
+                                 int l_max, int m_max, double rho_min, double sz_min, double dz_min, double resolution,
+                                 int64_t len_rho, int64_t len_z, int interp, smh_op** out) {
+  if (!ctx || !out) return SMH_ERR_ARG;
+  *out = nullptr;
+  SMH_CHECK(ctx, n_part >= 1 && xyz && lmax && mmax, SMH_ERR_ARG, "need at least one particle");
+  SMH_CHECK(ctx, n_part <= (int64_t)kPairMask, SMH_ERR_ARG, "too many particles (pair records index 23 bits)");
+  SMH_CHECK(ctx, l_max >= 1 && m_max >= 1 && m_max <= l_max && 2 * l_max <= kMaxL2, SMH_ERR_ARG,
+            "invalid l_max %d / m_max %d", l_max, m_max);
+  SMH_CHECK(ctx, resolution > 0 && len_rho >= 2 && len_z >= 2 && len_rho * len_z < (1 << 30), SMH_ERR_ARG,
+            "invalid table grid");
+  SMH_CHECK(ctx, interp == SMH_INTERP_LINEAR, SMH_ERR_UNSUPPORTED,
+            "the volume lookup interpolates bilinearly (interp must be SMH_INTERP_LINEAR)");
+  SMH_CHECK(ctx, volume_kernel(SMH_FP32, l_max, m_max).fn != nullptr, SMH_ERR_UNSUPPORTED,
+            "no kernel for l_max %d m_max %d", l_max, m_max);
+  SMH_HIP(ctx, hipSetDevice(ctx->device));
+  smh_op* op = new smh_op();
+  op->ctx = ctx;
+  op->kind = 1;
+  op->n_part = n_part;
+  op->l_max = l_max;
+  op->m_max = m_max;
+  op->B = smh_blocksize(l_max, m_max);
+  op->Bh = (op->B + 15) / 16 * 16;
+  op->MP = 2 * op->Bh;
+  op->interp = interp;
+  op->T = 4;
+  op->rho_min = rho_min;
+  op->sz_min = sz_min;
+  op->dz_min = dz_min;
+  op->h = resolution;
+  op->len_rho = len_rho;
+  op->len_z = len_z;
+  auto fail = [&](int code) {
+    smh_op_free(op);
+    delete op;
+    return code;
+  };
+  if (int code = op_index_map(op, lmax, mmax)) return fail(code);
+  hipStream_t s = ctx->stream;
+  if (hipMalloc(&op->d_xyz, n_part * 3 * sizeof(double)) != hipSuccess ||
+      hipMemcpyAsync(op->d_xyz, xyz, n_part * 3 * sizeof(double), hipMemcpyHostToDevice, s) != hipSuccess ||
+      hipMalloc(&op->d_xmax, sizeof(float)) != hipSuccess) {
+    smh_fail(ctx, SMH_ERR_HIP, "device allocation failed");
+    return fail(SMH_ERR_HIP);
+  }
+  size_t ypart = 0;
+  for (int pi = 0; pi < 2; ++pi) {
+    const KernelInfo k = volume_kernel(pi == 0 ? SMH_FP32 : SMH_FP64, l_max, m_max);
+    const size_t rs = pi == 0 ? sizeof(float) : sizeof(double);
+    size_t lds = 0;
+    const int Rc = chunk_size(op, rs, k.sy, lds);
+    if (Rc == 0) continue;
+    for (int t = 0; t < 2; ++t) {
+      op->vbin[t][pi].lds = lds;
+      const int code = pi == 0 ? vol_bin_pairs<float>(op, t, 0, Rc) : vol_bin_pairs<double>(op, t, 1, Rc);
+      if (code) return fail(code);
+      ypart = std::max(ypart, (size_t)op->vbin[t][pi].n_items * Rc * 2 * op->B * rs);
+    }
+  }
+  op->ypart_bytes = std::max<size_t>(ypart, 16);
+  const int64_t n_pad = n_part * op->B;
+  if (hipMalloc(&op->d_ypart, op->ypart_bytes) != hipSuccess ||
+      hipMalloc(&op->d_x32, n_pad * sizeof(float2)) != hipSuccess ||
+      hipMalloc(&op->d_y32, n_pad * sizeof(float2)) != hipSuccess ||
+      hipMalloc(&op->d_x64, n_pad * sizeof(double2)) != hipSuccess ||
+      hipMalloc(&op->d_y64, n_pad * sizeof(double2)) != hipSuccess || hipStreamSynchronize(s) != hipSuccess) {
+    smh_fail(ctx, SMH_ERR_HIP, "device allocation failed (vectors)");
+    return fail(SMH_ERR_HIP);
+  }
+  ctx->live_ops++;
+  *out = op;
+  return SMH_OK;
+}
+
+extern "C" int smh_volume_build_table(smh_op* op, double k_re, double k_im, double mask_radius, double z_s,
+                                      const double* direct_coef, int64_t n_kappa, const double* kappa,
+                                      const double* kz, const double* layer_coef) {
+  if (!op) return SMH_ERR_ARG;
+  smh_ctx* ctx = op->ctx;
+  SMH_CHECK(ctx, op->kind == 1, SMH_ERR_ARG, "not a volume operator");
+  SMH_CHECK(ctx, direct_coef != nullptr, SMH_ERR_ARG, "direct coefficients are NULL");
+  SMH_CHECK(ctx, n_kappa == 0 || (kappa && kz && layer_coef && n_kappa >= 2), SMH_ERR_ARG,
+            "layer-mediated part needs kappa, kz and coefficients");
+  SMH_HIP(ctx, hipSetDevice(ctx->device));
+  hipStream_t s = ctx->stream;
+  const int B = op->B, BB = B * B, L2 = 2 * op->l_max;
+  const int64_t nodes = op->len_rho * op->len_z;
+  for (int t = 0; t < 2; ++t)
+    if (!op->d_vtab[t]) SMH_HIP(ctx, hipMalloc(&op->d_vtab[t], (size_t)nodes * BB * sizeof(float2)));
+  std::vector<int> tau, ll, mm;
+  smh_index_tables(op->l_max, op->m_max, tau, ll, mm);
+  std::vector<int> dm(BB);
+  for (int n1 = 0; n1 < B; ++n1)
+    for (int n2 = 0; n2 < B; ++n2) dm[n1 * B + n2] = abs(mm[n1] - mm[n2]);
+  double2 *d_coef = nullptr, *d_kappa = nullptr, *d_kz = nullptr, *d_lc = nullptr;
+  int* d_dm = nullptr;
+  auto release = [&]() {
+    void* p[] = {d_coef, d_kappa, d_kz, d_lc, d_dm};
+    for (void* q : p)
+      if (q) hipFree(q);
+  };
+  const size_t cbytes = (size_t)BB * (L2 + 1) * sizeof(double2);
+  if (hipMalloc(&d_coef, cbytes) != hipSuccess || hipMalloc(&d_dm, BB * sizeof(int)) != hipSuccess ||
+      hipMemcpyAsync(d_coef, direct_coef, cbytes, hipMemcpyHostToDevice, s) != hipSuccess ||
+      hipMemcpyAsync(d_dm, dm.data(), BB * sizeof(int), hipMemcpyHostToDevice, s) != hipSuccess) {
+    release();
+    return smh_fail(ctx, SMH_ERR_HIP, "direct coefficient upload failed");
+  }
+  hipLaunchKernelGGL(vol_direct_kernel, dim3((unsigned)nodes), dim3(256), 0, s, op->rho_min, op->dz_min, op->h,
+                     op->len_rho, op->len_z, B, L2, k_re, k_im, mask_radius, d_coef, d_dm, op->d_vtab[1]);
+  if (n_kappa > 0) {
+    const size_t lbytes = (size_t)n_kappa * 4 * BB * sizeof(double2);
+    if (hipMalloc(&d_kappa, n_kappa * sizeof(double2)) != hipSuccess ||
+        hipMalloc(&d_kz, n_kappa * sizeof(double2)) != hipSuccess || hipMalloc(&d_lc, lbytes) != hipSuccess ||
+        hipMemcpyAsync(d_kappa, kappa, n_kappa * sizeof(double2), hipMemcpyHostToDevice, s) != hipSuccess ||
+        hipMemcpyAsync(d_kz, kz, n_kappa * sizeof(double2), hipMemcpyHostToDevice, s) != hipSuccess ||
+        hipMemcpyAsync(d_lc, layer_coef, lbytes, hipMemcpyHostToDevice, s) != hipSuccess) {
+      release();
+      return smh_fail(ctx, SMH_ERR_HIP, "layer-mediated coefficient upload failed");
+    }
+    const int nmax = 2 * op->m_max;
+    SMH_CHECK(ctx, nmax < kNJ, SMH_ERR_UNSUPPORTED, "m_max %d too large for the table kernel", op->m_max);
+    dim3 grid((unsigned)op->len_rho, (unsigned)((op->len_z + kZT - 1) / kZT));
+    hipLaunchKernelGGL(vol_layer_kernel, grid, dim3(256), 0, s, op->rho_min, op->sz_min, op->dz_min, op->h,
+                       op->len_z, B, nmax, z_s, d_dm, n_kappa, d_kappa, d_kz, d_lc, op->d_vtab[0], op->d_vtab[1]);
+  } else {
+    SMH_HIP(ctx, hipMemsetAsync(op->d_vtab[0], 0, (size_t)nodes * BB * sizeof(float2), s));
+  }
+  const hipError_t e = hipStreamSynchronize(s);
+  release();
+  SMH_CHECK(ctx, e == hipSuccess, SMH_ERR_HIP, "volume table kernels failed: %s", hipGetErrorString(e));
+  return vol_make_taps(op);
+}

@@ -96,3 +96,50 @@ def layer_coefficients_radial(vacuum_wavelength, layer_system, k_parallel, i_s, 
     # the table integrand is 2 i^|dm| sum_k (I_k + I_{k+1}) dk_k with I_k = J(kappa_k rho) jac_k belbe_k
     ipow = np.array([1, 1j, -1, -1j])[dm % 4]
     return 2 * ipow[None] * (trapezoid_weights(k_parallel) * jac)[:, None, None] * belbe
+
+
+def volume_grids(xyz, resolution):
+    """rho, z1+z2 and z1-z2 grids of the volume tables (prepare_lookup.py:16-80):
+    each arange(min - 3h, max + 3h, h); the two z grids must have one length."""
+    xyz = np.asarray(xyz, dtype=float)
+    rho_max, rho_min_pair = pair_distance_extremes(xyz[:, :2])
+    zmin, zmax = xyz[:, 2].min(), xyz[:, 2].max()
+    rho = np.arange(-3 * resolution, rho_max + 3 * resolution, resolution)
+    sz = np.arange(2 * zmin - 3 * resolution, 2 * zmax + 3 * resolution, resolution)
+    dz = np.arange((zmin - zmax) - 3 * resolution, (zmax - zmin) + 3 * resolution, resolution)
+    if len(sz) != len(dz):
+        raise ValueError('z1+z2 and z1-z2 grids differ in length (%d, %d)' % (len(sz), len(dz)))
+    return rho, sz, dz, rho_min_pair
+
+
+def direct_coefficients_volume(l_max, m_max):
+    """coef[n1, n2, ld] with direct(rho, dz) = sum_ld coef h_ld(k d) P_ld^|m1-m2|(dz / d)."""
+    B = flds.blocksize(l_max, m_max)
+    tau, ll, mm = flds.index_arrays(l_max, m_max)
+    coef = np.zeros((B, B, 2 * l_max + 1), complex)
+    for n1 in range(B):
+        for n2 in range(B):
+            l1, m1, l2, m2 = ll[n1], mm[n1], ll[n2], mm[n2]
+            for ld in range(max(abs(l1 - l2), abs(m1 - m2)), l1 + l2 + 1):
+                a5, b5 = sf.ab5_coefficients(l2, m2, l1, m1, ld)
+                coef[n1, n2, ld] = a5 if tau[n1] == tau[n2] else b5
+    return coef
+
+
+def layer_coefficients_volume(vacuum_wavelength, layer_system, k_parallel, i_s, l_max, m_max):
+    """C[k, 4, n1, n2] of the volume tables' layer-mediated parts
+    (prepare_lookup.py:138-250): with I(q) = 2 i^|dm| w_k kappa_k / (kz_k k) q,
+      plus  = sum_k J (C0 e^{i kz (sz - 2 z_s)} + C1 e^{-i kz (sz - 2 z_s)}),
+      minus = sum_k J (C2 e^{i kz dz} + C3 e^{-i kz dz}),
+    C0 = I(sum_pol L01 Bd+ B-), C1 = I(sum_pol L10 Bd- B+), C2 = I(sum_pol L00 Bd+ B+), C3 = I(sum_pol L11 Bd- B-)."""
+    k_is, kz, L, Bd, Bn = layer_factors(vacuum_wavelength, layer_system, k_parallel, i_s, l_max, m_max)
+    _, _, mm = flds.index_arrays(l_max, m_max)
+    nk, B = len(k_parallel), len(mm)
+    C = np.zeros((nk, 4, B, B), complex)
+    for q, (a, b) in enumerate(((0, 1), (1, 0), (0, 0), (1, 1))):
+        for pol in range(2):
+            C[:, q] += L[pol, a, b][:, None, None] * Bd[pol, a].T[:, :, None] * Bn[pol, b].T[:, None, :]
+    jac = k_parallel / (kz * k_is)
+    dm = np.abs(mm[:, None] - mm[None, :])
+    ipow = np.array([1, 1j, -1, -1j])[dm % 4]
+    return C * (2 * ipow[None, None] * (trapezoid_weights(k_parallel) * jac)[:, None, None, None]), kz

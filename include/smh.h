@@ -105,13 +105,19 @@ int smh_radial_build_table(smh_op *op, double k_re, double k_im, double mask_rad
 int smh_op_download_table(smh_op *op, double *table, double *table_minus);
 
 /* y = W x for nrhs vectors. x, y: host complex128 [nrhs][n_unk] in smuthi's
- * system-vector layout (replaces the CUDA matvec closure, linear_system.py:888-898). */
+ * system-vector layout (replaces the CUDA matvec closure, linear_system.py:888-898).
+ * Radial operators apply nrhs > 1 in blocks of 8 with one batched launch each. */
 int smh_op_matvec(smh_op *op, const double *x, double *y, int64_t nrhs, int precision);
 
 /* Time `iters` matvecs on device-resident synthetic vectors; returns the
  * average time of the coupling kernel (HIP events on the operator's stream)
  * and of the whole matvec in milliseconds. */
 int smh_op_bench_matvec(smh_op *op, int precision, int iters, double *kernel_ms, double *total_ms);
+
+/* The same for the batched kernel: one launch applies W to 8 right-hand
+ * sides (smh_op_matvec uses it for nrhs > 1 on radial operators). No
+ * counterpart in smuthi, which has no multi-RHS path. */
+int smh_op_bench_matvec_batch(smh_op *op, int precision, int iters, double *kernel_ms, double *total_ms);
 
 /* Solve (1 - T W) b = T a (LinearSystem.solve with solver_type='gmres',
  * linear_system.py:254-344) for nrhs right-hand sides.

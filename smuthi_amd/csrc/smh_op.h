@@ -43,6 +43,7 @@ struct smh_op {
     int* d_item_en = nullptr;        // per item: ceil(log2) of the most pairs of one receiver
     size_t lds = 0;
   } bin[2];
+  Binning bbin[2];  // the same pairs binned for the kBatch-RHS kernel (smaller receiver chunks)
   int64_t n_pairs = 0;
 
   // volume lookup: complex64 tables [len_rho][len_z][B][B] over (rho, z1+z2)
@@ -59,6 +60,8 @@ struct smh_op {
   double2* d_pk_in = nullptr;
   double2* d_pk_out = nullptr;
   int64_t pk_cap = 0;  // capacity in complex entries
+  void* d_xb = nullptr;  // kBatch-RHS padded vectors (allocated on first use, FP64 size)
+  void* d_yb = nullptr;
   float2* d_x32 = nullptr;
   float2* d_y32 = nullptr;
   double2* d_x64 = nullptr;
@@ -69,12 +72,20 @@ struct smh_op {
 void smh_op_free(smh_op* op);
 
 namespace smh {
+// right-hand sides per batched matvec / batched GMRES block
+constexpr int kBatchRhs = 8;
 // packed -> padded index map of the particles' multipole blocks
 int op_index_map(smh_op* op, const int* lmax, const int* mmax);
 // volume operator: matvec, table upload / download (smh_volume.hip)
 int vol_apply(smh_op* op, int precision, const void* x_pad, void* y_pad, bool time_kernel);
 int vol_upload(smh_op* op, const double* plus, const double* minus);
 int vol_download(smh_op* op, double* plus, double* minus);
+// kBatch-RHS matvec on padded [n_part][B][kBatch] vectors (radial operators)
+bool op_has_batch(const smh_op* op, int precision);
+int op_apply_batch(smh_op* op, int precision, const void* x_pad, void* y_pad, bool time_kernel);
+int op_unpack_batch(smh_op* op, int precision, const double2* pk, int nrhs, void* pad);
+int op_pack_batch(smh_op* op, int precision, const void* pad, int nrhs, double2* pk);
+int op_reserve_batch(smh_op* op);
 // Launch one coupling matvec on padded device vectors (x -> y, [n_part][B]).
 int op_apply(smh_op* op, int precision, const void* x_pad, void* y_pad, bool time_kernel);
 // Ensure packed host staging buffers for nrhs vectors.

@@ -281,7 +281,389 @@ struct Solver {
   }
 };
 
+
+// ---- kBatch right-hand sides interleaved: element e = i * kBatch + r ----
+constexpr int kB = smh::kBatchRhs;
+
+// partial[(b * kMaxBasis + j) * kB + r] = block b's share of conj(V_j[., r]) . w[., r]
+__global__ void bmulti_dot_kernel(const double2* __restrict__ V, int64_t ld, int nv, const double2* __restrict__ w,
+                                  int64_t n8, double2* partial) {
+  __shared__ double2 red[kDotThreads];
+  for (int j = 0; j < nv; ++j) {
+    const double2* v = V + (size_t)j * ld;
+    double re = 0, im = 0;
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n8; e += (int64_t)gridDim.x * blockDim.x) {
+      const double2 a = v[e], b = w[e];
+      re += a.x * b.x + a.y * b.y;
+      im += a.x * b.y - a.y * b.x;
+    }
+    red[threadIdx.x] = make_double2(re, im);
+    __syncthreads();
+    for (int s = blockDim.x / 2; s >= kB; s >>= 1) {  // strides keep e % kB
+      if ((int)threadIdx.x < s) {
+        red[threadIdx.x].x += red[threadIdx.x + s].x;
+        red[threadIdx.x].y += red[threadIdx.x + s].y;
+      }
+      __syncthreads();
+    }
+    if ((int)threadIdx.x < kB) partial[((size_t)blockIdx.x * kMaxBasis + j) * kB + threadIdx.x] = red[threadIdx.x];
+    __syncthreads();
+  }
+}
+
+__global__ void breduce_partials_kernel(const double2* __restrict__ partial, int nblocks, int nv, double2* out) {
+  const int jr = threadIdx.x;
+  if (jr >= nv * kB) return;
+  double re = 0, im = 0;
+  for (int b = 0; b < nblocks; ++b) {
+    re += partial[(size_t)b * kMaxBasis * kB + jr].x;
+    im += partial[(size_t)b * kMaxBasis * kB + jr].y;
+  }
+  out[jr] = make_double2(re, im);
+}
+
+// w[e] += sgn * sum_j V_j[e] c[j][e % kB]
+__global__ void bmulti_axpy_kernel(const double2* __restrict__ V, int64_t ld, int nv, const double2* __restrict__ c,
+                                   double sgn, double2* w, int64_t n8) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n8) return;
+  const int r = (int)(e % kB);
+  double2 acc = w[e];
+  for (int j = 0; j < nv; ++j) {
+    const double2 v = V[(size_t)j * ld + e], cj = c[j * kB + r];
+    acc.x += sgn * (v.x * cj.x - v.y * cj.y);
+    acc.y += sgn * (v.x * cj.y + v.y * cj.x);
+  }
+  w[e] = acc;
+}
+
+__global__ void bscale_copy_kernel(const double2* __restrict__ src, const double* __restrict__ sc, double2* dst,
+                                   int64_t n8) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n8) return;
+  const double f = sc[e % kB];
+  dst[e] = make_double2(src[e].x * f, src[e].y * f);
+}
+
+// out = v - t o y with t per unknown, shared by the right-hand sides
+__global__ void bmaster_fp64_kernel(const double2* __restrict__ v, const double2* __restrict__ t,
+                                    const double2* __restrict__ y, double2* out, int64_t n8) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n8) return;
+  const double2 a = t[e / kB], b = y[e];
+  out[e] = make_double2(v[e].x - (a.x * b.x - a.y * b.y), v[e].y - (a.x * b.y + a.y * b.x));
+}
+
+__global__ void bmaster_fp32_kernel(const double2* __restrict__ v, const double2* __restrict__ t,
+                                    const float2* __restrict__ y, double2* out, int64_t n8) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n8) return;
+  const double2 a = t[e / kB];
+  const double bx = y[e].x, by = y[e].y;
+  out[e] = make_double2(v[e].x - (a.x * bx - a.y * by), v[e].y - (a.x * by + a.y * bx));
+}
+
+// Restarted GMRES on kB systems at once, sharing every coupling matvec.
+struct BatchSolver {
+  smh_op* op;
+  smh_ctx* ctx;
+  hipStream_t s;
+  int64_t n, n8;  // padded length, times kB
+  int restart;
+  double2* V = nullptr;        // (restart + 1) basis blocks [n][kB]
+  double2* w = nullptr;
+  double2* t = nullptr;        // T diagonal (padded, [n])
+  double2* partial = nullptr;  // [kDotBlocks][kMaxBasis][kB]
+  double2* coef = nullptr;     // [kMaxBasis][kB]
+  double* scl = nullptr;       // [kB]
+  double2* y64 = nullptr;      // coupling output FP64 [n][kB]
+  float2* x32 = nullptr;
+  float2* y32 = nullptr;
+  int64_t mv64 = 0, mv32 = 0, iters = 0;
+
+  int init() {
+    SMH_HIP(ctx, hipMalloc(&V, (size_t)(restart + 1) * n8 * sizeof(double2)));
+    SMH_HIP(ctx, hipMalloc(&w, n8 * sizeof(double2)));
+    SMH_HIP(ctx, hipMalloc(&t, n * sizeof(double2)));
+    SMH_HIP(ctx, hipMalloc(&partial, (size_t)kDotBlocks * kMaxBasis * kB * sizeof(double2)));
+    SMH_HIP(ctx, hipMalloc(&coef, (size_t)kMaxBasis * kB * sizeof(double2)));
+    SMH_HIP(ctx, hipMalloc(&scl, kB * sizeof(double)));
+    SMH_HIP(ctx, hipMalloc(&y64, n8 * sizeof(double2)));
+    SMH_HIP(ctx, hipMalloc(&x32, n8 * sizeof(float2)));
+    SMH_HIP(ctx, hipMalloc(&y32, n8 * sizeof(float2)));
+    return SMH_OK;
+  }
+  void release() {
+    void* p[] = {V, w, t, partial, coef, scl, y64, x32, y32};
+    for (void* q : p)
+      if (q) hipFree(q);
+  }
+
+  int master(const double2* v, double2* out, int precision) {
+    int rc;
+    if (precision == SMH_FP64) {
+      if ((rc = smh::op_apply_batch(op, SMH_FP64, v, y64, false))) return rc;
+      hipLaunchKernelGGL(bmaster_fp64_kernel, dim3(blocks_for(n8)), dim3(256), 0, s, v, t, y64, out, n8);
+      ++mv64;
+    } else {
+      hipLaunchKernelGGL(to_fp32_kernel, dim3(blocks_for(n8)), dim3(256), 0, s, v, x32, n8);
+      if ((rc = smh::op_apply_batch(op, SMH_FP32, x32, y32, false))) return rc;
+      hipLaunchKernelGGL(bmaster_fp32_kernel, dim3(blocks_for(n8)), dim3(256), 0, s, v, t, y32, out, n8);
+      ++mv32;
+    }
+    SMH_HIP(ctx, hipGetLastError());
+    return SMH_OK;
+  }
+
+  // h[j][r] = V_j[., r]^H w[., r] for j < nv (device -> host)
+  int dots(const double2* Vb, int nv, const double2* wv, zd* h) {
+    hipLaunchKernelGGL(bmulti_dot_kernel, dim3(kDotBlocks), dim3(kDotThreads), 0, s, Vb, n8, nv, wv, n8, partial);
+    hipLaunchKernelGGL(breduce_partials_kernel, dim3(1), dim3(kMaxBasis * kB), 0, s, partial, kDotBlocks, nv, coef);
+    SMH_HIP(ctx, hipMemcpyAsync(h, coef, (size_t)nv * kB * sizeof(double2), hipMemcpyDeviceToHost, s));
+    SMH_HIP(ctx, hipStreamSynchronize(s));
+    return SMH_OK;
+  }
+  int norms(const double2* v, double* out) {
+    zd h[kB];
+    int rc = dots(v, 1, v, h);
+    for (int r = 0; r < kB; ++r) out[r] = std::sqrt(std::max(0.0, h[r].real()));
+    return rc;
+  }
+  int set_scale(const double* f, const double2* src, double2* dst) {
+    SMH_HIP(ctx, hipMemcpyAsync(scl, f, kB * sizeof(double), hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL(bscale_copy_kernel, dim3(blocks_for(n8)), dim3(256), 0, s, src, scl, dst, n8);
+    return SMH_OK;
+  }
+
+  // GMRES on M x = b for the systems r with active[r] (others are left
+  // untouched), each to ||b - M x|| <= atol[r]; x_zero / trust_estimate as
+  // for the single-RHS solver. res_out[r]: residual (or estimate) of x.
+  int gmres(double2* x, const double2* b, const double* atol, const bool* active, int maxiter, int precision,
+            double* res_out, int& steps, bool x_zero, bool trust_estimate) {
+    std::vector<zd> H((size_t)kB * (restart + 1) * restart), cs((size_t)kB * restart), sn((size_t)kB * restart),
+        g((size_t)kB * (restart + 1)), yv((size_t)kMaxBasis * kB), hc((size_t)(restart + 1) * kB),
+        hc2((size_t)(restart + 1) * kB);
+    int rc;
+    steps = 0;
+    for (int outer = 0;; ++outer) {
+      if (outer == 0 && x_zero) {
+        SMH_HIP(ctx, hipMemcpyAsync(w, b, n8 * sizeof(double2), hipMemcpyDeviceToDevice, s));
+      } else {
+        if ((rc = master(x, w, precision))) return rc;
+        hipLaunchKernelGGL(sub_kernel, dim3(blocks_for(n8)), dim3(256), 0, s, b, w, w, n8);
+      }
+      double beta[kB];
+      if ((rc = norms(w, beta))) return rc;
+      bool cont[kB], live[kB], any = false;
+      int kr[kB];
+      double f[kB];
+      for (int r = 0; r < kB; ++r) {
+        res_out[r] = beta[r];
+        cont[r] = active[r] && beta[r] > atol[r] && beta[r] > 0.0;
+        live[r] = cont[r];
+        kr[r] = 0;
+        f[r] = cont[r] ? 1.0 / beta[r] : 0.0;
+        any = any || cont[r];
+      }
+      if (!any || steps >= maxiter) return SMH_OK;
+      set_scale(f, w, V);
+      std::fill(g.begin(), g.end(), zd(0));
+      for (int r = 0; r < kB; ++r) g[(size_t)r * (restart + 1)] = cont[r] ? beta[r] : 0.0;
+      int k = 0;
+      for (; k < restart && steps < maxiter; ++k, ++steps) {
+        double2* vk = V + (size_t)k * n8;
+        if ((rc = master(vk, w, precision))) return rc;
+        // CGS2 against V_0..V_k, all systems at once
+        if ((rc = dots(V, k + 1, w, hc.data()))) return rc;
+        SMH_HIP(ctx, hipMemcpyAsync(coef, hc.data(), (size_t)(k + 1) * kB * sizeof(double2), hipMemcpyHostToDevice, s));
+        hipLaunchKernelGGL(bmulti_axpy_kernel, dim3(blocks_for(n8)), dim3(256), 0, s, V, n8, k + 1, coef, -1.0, w, n8);
+        if ((rc = dots(V, k + 1, w, hc2.data()))) return rc;
+        SMH_HIP(ctx, hipMemcpyAsync(coef, hc2.data(), (size_t)(k + 1) * kB * sizeof(double2), hipMemcpyHostToDevice,
+                                    s));
+        hipLaunchKernelGGL(bmulti_axpy_kernel, dim3(blocks_for(n8)), dim3(256), 0, s, V, n8, k + 1, coef, -1.0, w, n8);
+        double hn[kB];
+        if ((rc = norms(w, hn))) return rc;
+        for (int r = 0; r < kB; ++r) f[r] = (live[r] && hn[r] > 0.0) ? 1.0 / hn[r] : 0.0;
+        set_scale(f, w, V + (size_t)(k + 1) * n8);
+        bool any_live = false;
+        for (int r = 0; r < kB; ++r) {
+          if (!live[r]) continue;
+          zd* Hr = H.data() + (size_t)r * (restart + 1) * restart;
+          zd* csr = cs.data() + (size_t)r * restart;
+          zd* snr = sn.data() + (size_t)r * restart;
+          zd* gr = g.data() + (size_t)r * (restart + 1);
+          std::vector<zd> col(k + 2);
+          for (int j = 0; j <= k; ++j) col[j] = hc[(size_t)j * kB + r] + hc2[(size_t)j * kB + r];
+          col[k + 1] = hn[r];
+          for (int j = 0; j < k; ++j) {
+            const zd a = col[j], bb = col[j + 1];
+            col[j] = std::conj(csr[j]) * a + std::conj(snr[j]) * bb;
+            col[j + 1] = -snr[j] * a + csr[j] * bb;
+          }
+          const zd a = col[k], bb = col[k + 1];
+          const double den = std::sqrt(std::norm(a) + std::norm(bb));
+          csr[k] = den > 0 ? a / den : zd(1);
+          snr[k] = den > 0 ? bb / den : zd(0);
+          col[k] = std::conj(csr[k]) * a + std::conj(snr[k]) * bb;
+          gr[k + 1] = -snr[k] * gr[k];
+          gr[k] = std::conj(csr[k]) * gr[k];
+          for (int j = 0; j <= k; ++j) Hr[(size_t)j * restart + k] = col[j];
+          kr[r] = k + 1;
+          if (std::abs(gr[k + 1]) <= atol[r] || hn[r] == 0.0)
+            live[r] = false;
+          else
+            any_live = true;
+        }
+        ++iters;
+        if (!any_live) {
+          ++k;
+          ++steps;
+          break;
+        }
+      }
+      // per system back substitution, then x += V y for all systems at once
+      std::fill(yv.begin(), yv.end(), zd(0));
+      bool all_est = true;
+      for (int r = 0; r < kB; ++r) {
+        if (!cont[r]) continue;
+        const zd* Hr = H.data() + (size_t)r * (restart + 1) * restart;
+        const zd* gr = g.data() + (size_t)r * (restart + 1);
+        const int m = kr[r];
+        for (int i = m - 1; i >= 0; --i) {
+          zd sum = gr[i];
+          for (int j = i + 1; j < m; ++j) sum -= Hr[(size_t)i * restart + j] * yv[(size_t)j * kB + r];
+          yv[(size_t)i * kB + r] = sum / Hr[(size_t)i * restart + i];
+        }
+        if (live[r]) all_est = false;
+        res_out[r] = std::abs(gr[m]);
+      }
+      if (k > 0) {
+        SMH_HIP(ctx, hipMemcpyAsync(coef, yv.data(), (size_t)k * kB * sizeof(double2), hipMemcpyHostToDevice, s));
+        hipLaunchKernelGGL(bmulti_axpy_kernel, dim3(blocks_for(n8)), dim3(256), 0, s, V, n8, k, coef, 1.0, x, n8);
+      }
+      if (trust_estimate && all_est) return SMH_OK;
+    }
+  }
+};
+
 }  // namespace
+
+// Blocks of kB right-hand sides solved together (every coupling matvec is one
+// batched launch for the whole block); same modes and stopping rule as below.
+static int solve_batched(smh_op* op, const double* t_diag, const double* rhs, double* x, int64_t nrhs, double tol,
+                         int restart, int maxiter, int mode, double* stats,
+                         std::chrono::steady_clock::time_point t0) {
+  smh_ctx* ctx = op->ctx;
+  hipStream_t s = ctx->stream;
+  BatchSolver S;
+  S.op = op;
+  S.ctx = ctx;
+  S.s = s;
+  S.n = op->n_part * op->B;
+  S.n8 = S.n * kB;
+  S.restart = restart;
+  int rc = S.init();
+  double2 *b_pad = nullptr, *x_pad = nullptr, *d_pad = nullptr, *r_pad = nullptr;
+  auto done = [&](int code) {
+    S.release();
+    void* p[] = {b_pad, x_pad, d_pad, r_pad};
+    for (void* q : p)
+      if (q) hipFree(q);
+    return code;
+  };
+  if (rc) return done(rc);
+  if ((rc = smh::op_reserve_packed(op, kB))) return done(rc);
+  if (hipMalloc(&b_pad, S.n8 * sizeof(double2)) != hipSuccess || hipMalloc(&x_pad, S.n8 * sizeof(double2)) != hipSuccess ||
+      hipMalloc(&d_pad, S.n8 * sizeof(double2)) != hipSuccess || hipMalloc(&r_pad, S.n8 * sizeof(double2)) != hipSuccess)
+    return done(smh_fail(ctx, SMH_ERR_HIP, "solver allocation failed"));
+  SMH_HIP(ctx, hipMemcpyAsync(op->d_pk_in, t_diag, op->n_unk * sizeof(double2), hipMemcpyHostToDevice, s));
+  if ((rc = smh::op_unpack(op, SMH_FP64, op->d_pk_in, S.t))) return done(rc);
+  double worst = 0, sweeps_total = 0;
+  int64_t steps_total = 0;
+  bool converged_all = true;
+  for (int64_t r0 = 0; r0 < nrhs; r0 += kB) {
+    const int nb = (int)std::min<int64_t>(kB, nrhs - r0);
+    SMH_HIP(ctx, hipMemcpyAsync(op->d_pk_in, rhs + 2 * r0 * op->n_unk, (size_t)nb * op->n_unk * sizeof(double2),
+                                hipMemcpyHostToDevice, s));
+    if ((rc = smh::op_unpack_batch(op, SMH_FP64, op->d_pk_in, nb, b_pad))) return done(rc);
+    double bnorm[kB], atol[kB], res[kB];
+    bool active[kB];
+    if ((rc = S.norms(b_pad, bnorm))) return done(rc);
+    for (int r = 0; r < kB; ++r) {
+      atol[r] = tol * bnorm[r];
+      active[r] = r < nb && bnorm[r] > 0.0;
+    }
+    int steps = 0;
+    if (mode == SMH_SOLVE_FP64) {
+      SMH_HIP(ctx, hipMemcpyAsync(x_pad, b_pad, S.n8 * sizeof(double2), hipMemcpyDeviceToDevice, s));
+      if ((rc = S.gmres(x_pad, b_pad, atol, active, maxiter, SMH_FP64, res, steps, false, false))) return done(rc);
+      steps_total += steps;
+      sweeps_total += 1;
+    } else {
+      SMH_HIP(ctx, hipMemsetAsync(x_pad, 0, S.n8 * sizeof(double2), s));
+      SMH_HIP(ctx, hipMemcpyAsync(r_pad, b_pad, S.n8 * sizeof(double2), hipMemcpyDeviceToDevice, s));
+      for (int r = 0; r < kB; ++r) res[r] = bnorm[r];
+      bool fresh = true;
+      int budget = maxiter;
+      for (int sweep = 0; sweep < 50; ++sweep) {
+        if (sweep > 0) {
+          if ((rc = S.master(x_pad, r_pad, SMH_FP64))) return done(rc);
+          hipLaunchKernelGGL(sub_kernel, dim3(blocks_for(S.n8)), dim3(256), 0, s, b_pad, r_pad, r_pad, S.n8);
+          if ((rc = S.norms(r_pad, res))) return done(rc);
+          fresh = true;
+        }
+        bool act[kB], any = false;
+        double inner_atol[kB];
+        for (int r = 0; r < kB; ++r) {
+          act[r] = active[r] && res[r] > atol[r];
+          inner_atol[r] = std::max(atol[r] * 0.5, 1e-4 * res[r]);
+          any = any || act[r];
+        }
+        if (!any || budget <= 0) break;
+        SMH_HIP(ctx, hipMemsetAsync(d_pad, 0, S.n8 * sizeof(double2), s));
+        double inner_res[kB];
+        int inner_steps = 0;
+        if ((rc = S.gmres(d_pad, r_pad, inner_atol, act, budget, SMH_FP32, inner_res, inner_steps, true, true)))
+          return done(rc);
+        budget -= inner_steps;
+        steps_total += inner_steps;
+        hipLaunchKernelGGL(add_kernel, dim3(blocks_for(S.n8)), dim3(256), 0, s, x_pad, d_pad, S.n8);
+        sweeps_total += 1;
+        fresh = false;
+      }
+      if (!fresh) {
+        if ((rc = S.master(x_pad, r_pad, SMH_FP64))) return done(rc);
+        hipLaunchKernelGGL(sub_kernel, dim3(blocks_for(S.n8)), dim3(256), 0, s, b_pad, r_pad, r_pad, S.n8);
+        if ((rc = S.norms(r_pad, res))) return done(rc);
+      }
+    }
+    for (int r = 0; r < nb; ++r) {
+      const double rel = bnorm[r] > 0 ? res[r] / bnorm[r] : 0.0;
+      worst = std::max(worst, rel);
+      if (rel > tol) converged_all = false;
+    }
+    if ((rc = smh::op_pack_batch(op, SMH_FP64, x_pad, nb, op->d_pk_out))) return done(rc);
+    SMH_HIP(ctx, hipMemcpyAsync(x + 2 * r0 * op->n_unk, op->d_pk_out, (size_t)nb * op->n_unk * sizeof(double2),
+                                hipMemcpyDeviceToHost, s));
+  }
+  SMH_HIP(ctx, hipStreamSynchronize(s));
+  const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  if (stats) {
+    stats[0] = (double)steps_total;
+    stats[1] = (double)S.mv64;
+    stats[2] = (double)S.mv32;
+    stats[3] = worst;
+    stats[4] = secs;
+    stats[5] = sweeps_total;
+    stats[6] = (double)kB;  // right-hand sides per batched matvec
+    stats[7] = 0;
+  }
+  if (!converged_all) {
+    smh_fail(ctx, SMH_ERR_NOCONV, "GMRES stopped at relative residual %.3e (tol %.1e)", worst, tol);
+    return done(SMH_ERR_NOCONV);
+  }
+  return done(SMH_OK);
+}
 
 extern "C" int smh_solve(smh_op* op, const double* t_diag, const double* rhs, double* x, int64_t nrhs, double tol,
                          int restart, int maxiter, int mode, double* stats) {
@@ -323,6 +705,10 @@ extern "C" int smh_solve(smh_op* op, const double* t_diag, const double* rhs, do
     return done(code);
   };
   hipStream_t s = ctx->stream;
+  if (nrhs > 1 && smh::op_has_batch(op, SMH_FP32) && smh::op_has_batch(op, SMH_FP64)) {
+    const int code = solve_batched(op, t_diag, rhs, x, nrhs, tol, restart, maxiter, mode, stats, t0);
+    return done2(code);
+  }
   // T diagonal in padded layout
   if (hipMemcpyAsync(op->d_pk_in, t_diag, op->n_unk * sizeof(double2), hipMemcpyHostToDevice, s) != hipSuccess)
     return done2(smh_fail(ctx, SMH_ERR_HIP, "upload failed"));

@@ -83,6 +83,15 @@ class _GpuCouplingOperator(SystemMatrix):
                    cu.context(), 'smh_op_bench_matvec')
         return k.value, t.value
 
+    def bench_matvec_batch(self, precision='single', iters=10):
+        """(kernel ms, matvec ms) of one batched launch (8 right-hand sides) on device-resident data."""
+        k = ctypes.c_double()
+        t = ctypes.c_double()
+        prec = _lib.FP32 if precision in ('single', 'fp32') else _lib.FP64
+        _lib.check(_lib.load().smh_op_bench_matvec_batch(self._op, prec, iters, ctypes.byref(k), ctypes.byref(t)),
+                   cu.context(), 'smh_op_bench_matvec_batch')
+        return k.value, t.value
+
     def solve_master(self, t_diagonal, rhs, tol, precision='mixed', restart=20, maxiter=None, with_stats=False):
         """Solve (1 - diag(t) W) x = rhs on the GPU (rhs [n] or [nrhs, n]).
 

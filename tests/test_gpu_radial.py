@@ -124,3 +124,22 @@ def test_gpu_matvec_two_particles_and_degenerate_stack(gpu):
     W[:30, 30:] = oc.direct_coupling_block(WL, plist[0], plist[1], ol.LayerSystem(*ls))
     W[30:, :30] = oc.direct_coupling_block(WL, plist[1], plist[0], ol.LayerSystem(*ls))
     assert np.linalg.norm(op.matvec(v, 'double') - W @ v) / np.linalg.norm(W @ v) < 5e-5
+
+
+@pytest.mark.parametrize('kind,cubic,okind', [('linear', 'spline', 'linear'), ('cubic', 'spline', 'cubic'),
+                                              ('cubic', 'catmull-rom', 'catmull-rom')])
+def test_gpu_batched_matvec_matches_oracle(gpu, kind, cubic, okind):
+    # 11 right-hand sides: one full block of 8 and one padded block of 3 through the batched kernel
+    plist = _random_spheres(70, seed=6)
+    kp = ob.reasonable_Sommerfeld_kpar_contour(WL, layer_refractive_indices=[1.52, 1], neff_resolution=1e-2)
+    op = _product_op(plist, ([0, 0], [1.52, 1]), kp, 5.0, kind, cubic)
+    ref_op = oc.RadialLookupCPU(op.lookup_table, op.radial_distance_array, plist, okind)
+    rng = np.random.default_rng(21)
+    V = rng.standard_normal((11, ref_op.n_unk)) + 1j * rng.standard_normal((11, ref_op.n_unk))
+    V[4] *= 1e-3  # right-hand sides of different scale share one fixed-point scale per block
+    W64 = op.matvec(V, precision='double')
+    W32 = op.matvec(V, precision='single')
+    for r in range(11):
+        want = ref_op.matvec(V[r])
+        assert np.linalg.norm(W64[r] - want) / np.linalg.norm(want) < 1e-12, r
+        assert np.linalg.norm(W32[r] - want) / np.linalg.norm(want) < 3e-6, r

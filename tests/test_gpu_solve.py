@@ -107,3 +107,29 @@ def test_multiple_rhs_match_single(gpu):
     X = lsys.solve(np.stack(rhs))
     for j in range(3):
         assert np.linalg.norm(X[j] - sols[j]) / np.linalg.norm(sols[j]) < 1e-7
+
+
+@pytest.mark.parametrize('precision', ['double', 'mixed'])
+def test_batched_solve_matches_oracle_lu(gpu, precision):
+    """Ten right-hand sides (one full batch of 8 and one of 2) against the oracle's LU."""
+    from smuthi_amd import ensembles
+    pp = ensembles.small_substrate_ensemble(seed=2)[:30]
+    plist = [osys.Sphere(p.position, p.refractive_index, p.radius, 3, 3) for p in pp]
+    kp = ob.reasonable_Sommerfeld_kpar_contour(WL, layer_refractive_indices=[1.52, 1], neff_resolution=1e-2)
+    from smuthi_amd.layers import LayerSystem
+    from smuthi_amd.linear_system import CouplingMatrixRadialLookupCUDA
+    op = CouplingMatrixRadialLookupCUDA(WL, plist, LayerSystem([0, 0], [1.52, 1]), k_parallel=kp, resolution=5.0,
+                                        interpolator_kind='cubic')
+    W = oc.RadialLookupCPU(op.lookup_table, op.radial_distance_array, plist, 'cubic').dense()
+    rng = np.random.default_rng(4)
+    n = W.shape[0]
+    t = 0.3 + 0.1j + 0.05 * (rng.standard_normal(n) + 1j * rng.standard_normal(n))
+    R = rng.standard_normal((10, n)) + 1j * rng.standard_normal((10, n))
+    R[7] *= 1e3  # right-hand sides of different scale in one batch
+    M = np.eye(n) - t[:, None] * W
+    X_ref = np.linalg.solve(M, R.T).T
+    X, stats, ok = op.solve_master(t, R, tol=1e-9, precision=precision, with_stats=True)
+    assert ok and stats['relative_residual'] <= 1e-9
+    for j in range(10):
+        assert np.linalg.norm(X[j] - X_ref[j]) / np.linalg.norm(X_ref[j]) < 1e-7, j
+        assert np.linalg.norm(R[j] - M @ X[j]) / np.linalg.norm(R[j]) < 2e-9, j

@@ -6,13 +6,15 @@ U(1.5,2.0) + 0.01i) at z = 200 nm in a 60 x 60 um square with 20 nm minimum
 gaps, on glass (n = 1.52) in air, lambda = 550 nm, l_max = m_max = 3
 (300,000 unknowns), radial lookup with cubic (not-a-knot spline)
 interpolation at 2 nm, Sommerfeld contour with neff_resolution = neff_imag =
-2e-3. A step is one right-hand-side solve (plane wave, incidence angle and
-polarisation vary per step) to relative residual 1e-6: GMRES whose coupling
-matvecs run in FP32 on MFMA, wrapped in FP64 iterative refinement with FP64
-residuals. The coefficients of step 0 are compared with a plain FP64 GPU
-GMRES solve (tol 1e-9) of the same right-hand side.
+2e-3. A step solves a batch of right-hand sides (plane waves; incidence angle and
+polarisation vary; 8 per step by default) to relative residual 1e-6: GMRES
+for the whole batch in lockstep whose coupling matvecs run in FP32 on MFMA
+(one launch per 8 RHS), wrapped in FP64 iterative refinement with FP64
+residuals. The coefficients of right-hand side 0 are compared with a plain
+FP64 GPU GMRES solve (tol 1e-9) of the same right-hand side; single-RHS
+solve and plain FP64 GMRES throughputs are reported beside.
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--particles N]
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--particles N] [--batch R]
 """
 import argparse
 import contextlib
@@ -56,11 +58,13 @@ def rhs_for_step(j):
 TRAFFIC_FILE = os.path.join(ROOT, 'profiles', 'radial_matvec_10k_cubic_traffic.md')
 
 
-def measured_traffic():
-    """HBM bytes per FP32 matvec launch from the committed counter summary, or None."""
+def measured_traffic(kind='single'):
+    """HBM bytes per FP32 matvec launch (single-RHS or 8-RHS kernel) from the
+    committed counter summary, or None."""
+    key = 'traffic_bytes_per_launch' if kind == 'single' else 'traffic_bytes_per_launch_8rhs'
     try:
         for line in open(TRAFFIC_FILE):
-            if line.startswith('traffic_bytes_per_launch'):
+            if line.split('=')[0].strip() == key:
                 return float(line.split('=')[1])
     except OSError:
         pass
@@ -68,9 +72,9 @@ def measured_traffic():
 
 
 def traffic_probe(n_particles, copy_bytes=512 << 20):
-    """Run a plain copy of known size, then three FP32 matvec launches of the
-    bench operator (one warm-up + two timed): the workload the FETCH_SIZE /
-    WRITE_SIZE counter passes are taken on."""
+    """Run a plain copy of known size, then three single-RHS and three 8-RHS
+    FP32 matvec launches of the bench operator (one warm-up + two timed each):
+    the workload the FETCH_SIZE / WRITE_SIZE counter passes are taken on."""
     import ctypes
     from smuthi_amd import _lib, cuda
     from smuthi_amd.linear_system import CouplingMatrixRadialLookupCUDA
@@ -82,7 +86,8 @@ def traffic_probe(n_particles, copy_bytes=512 << 20):
     _lib.check(_lib.load().smh_bench_copy(cuda.context(), copy_bytes, ctypes.byref(ms)), cuda.context(),
                'smh_bench_copy')
     k, _ = op.bench_matvec('single', iters=2)
-    log('copy %d bytes: %.3f ms; FP32 matvec kernel %.2f ms' % (copy_bytes, ms.value, k))
+    kb, _ = op.bench_matvec_batch('single', iters=2)
+    log('copy %d bytes: %.3f ms; FP32 matvec kernel %.2f ms, 8-RHS kernel %.2f ms' % (copy_bytes, ms.value, k, kb))
 
 
 def cpu_baseline(table, rho, n_full, iters_per_solve, n_sample=400, seed=5, budget_s=20.0):
@@ -122,10 +127,11 @@ def cpu_baseline(table, rho, n_full, iters_per_solve, n_sample=400, seed=5, budg
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument('--gpus', type=int, default=1)
-    ap.add_argument('--steps', type=int, default=3)
+    ap.add_argument('--steps', type=int, default=2)
     ap.add_argument('--warmup', type=int, default=1)
     ap.add_argument('--particles', type=int, default=10000)
     ap.add_argument('--tol', type=float, default=1e-6)
+    ap.add_argument('--batch', type=int, default=8, help='right-hand sides solved together per step')
     ap.add_argument('--no-cpu-baseline', action='store_true')
     ap.add_argument('--matvec-iters', type=int, default=5)
     ap.add_argument('--traffic-probe', action='store_true',
@@ -161,60 +167,68 @@ def main():
     op = base.coupling_matrix
     log('prepare %.2f s (table %.2f s), len_rho %d' % (prep_s, op.table_seconds, len(op.radial_distance_array)))
 
-    # right-hand sides of all steps (setup, outside the timed region)
+    # right-hand sides of all steps (setup, outside the timed region): a step is a
+    # batch of `batch` incidence directions / polarisations
+    nb = args.batch
+    # (T is the same for every incidence: rhs_j = T a_j)
     rhs = []
-    for j in range(args.warmup + args.steps):
-        s = make_system(j, 'mixed', args.tol)
-        s.compute_initial_field_coefficients()
-        s.compute_t_matrix()
-        rhs.append(s.t_matrix.right_hand_side())
-    base.compute_initial_field_coefficients()  # restore step-0 fields on the particles
+    for j in range(nb * (args.warmup + args.steps)):
+        polar, az, pol = rhs_for_step(j)
+        rhs.append(base.t_matrix.diagonal * PlaneWave(WL, polar, az, pol).swe_coefficients(plist, ls))
+    rhs = np.stack(rhs)
 
-    # coupling matvec kernel timing (HIP events on the operator's stream)
+    # coupling matvec kernels (HIP events on the operator's stream)
     k32, tot32 = op.bench_matvec('single', iters=args.matvec_iters)
     k64, tot64 = op.bench_matvec('double', iters=max(2, args.matvec_iters // 2))
+    kb32, totb32 = op.bench_matvec_batch('single', iters=max(2, args.matvec_iters // 2))
+    kb64, totb64 = op.bench_matvec_batch('double', iters=2)
     n_unk = op.shape[0]
     E = float(n_unk) ** 2
-    flops_alg = E * (4 * 4 + 8 * 1)  # cubic: 4 taps per complex entry, one RHS
-    log('matvec FP32 %.2f ms, FP64 %.2f ms' % (k32, k64))
+    flops_1 = E * (4 * 4 + 8 * 1)  # cubic: 4 taps per complex entry, one RHS
+    flops_8 = E * (4 * 4 + 8 * 8)  # the same entries applied to 8 RHS
+    log('matvec FP32 %.2f ms, FP64 %.2f ms; 8-RHS FP32 %.2f ms, FP64 %.2f ms' % (k32, k64, kb32, kb64))
 
-    # FP64 reference for step 0 (plain FP64 GMRES) and its throughput
-    ref = make_system(0, 'double', 1e-9)
-    ref.t_matrix = base.t_matrix
-    ref.coupling_matrix = op
+    def solve(j0, count, precision, tol):
+        sysj = make_system(j0, precision, tol)
+        sysj.t_matrix = base.t_matrix
+        sysj.coupling_matrix = op
+        x = sysj.solve(rhs[j0:j0 + count])
+        return x, dict(sysj.solve_stats)
+
+    # FP64 reference for right-hand side 0 and the plain FP64 GMRES throughput (one RHS)
     t0 = time.time()
-    x_ref = ref.solve(rhs[0][None, :])[0]
-    fp64_ref_s = time.time() - t0
-    ref_stats = dict(ref.solve_stats)
-    log('FP64 reference solve (tol 1e-9): %.2f s, %d iterations' % (fp64_ref_s, ref_stats['iterations']))
-    plain = make_system(0, 'double', args.tol)
-    plain.t_matrix = base.t_matrix
-    plain.coupling_matrix = op
+    x_ref = solve(0, 1, 'double', 1e-9)[0][0]
+    log('FP64 reference solve (tol 1e-9): %.2f s' % (time.time() - t0))
     t0 = time.time()
-    plain.solve(rhs[0][None, :])
+    _, plain_stats = solve(0, 1, 'double', args.tol)
     fp64_plain_s = time.time() - t0
-    plain_stats = dict(plain.solve_stats)
     log('plain FP64 GMRES (tol %.0e): %.2f s, %d iterations' % (args.tol, fp64_plain_s, plain_stats['iterations']))
+    # one mixed-precision solve of a single RHS (no batching) for comparison
+    t0 = time.time()
+    x_single, single_stats = solve(0, 1, 'mixed', args.tol)
+    single_s = time.time() - t0
+    log('single-RHS mixed solve: %.2f s' % single_s)
 
-    solver = make_system(0, 'mixed', args.tol)
-    solver.t_matrix = base.t_matrix
-    solver.coupling_matrix = op
-    for j in range(args.warmup):
-        solver.solve(rhs[j][None, :])
+    # timed region: batched mixed-precision solves
+    for w in range(args.warmup):
+        solve(nb * w, nb, 'mixed', args.tol)
     t0 = time.time()
     sols, stats = [], []
-    for j in range(args.warmup, args.warmup + args.steps):
-        sols.append(solver.solve(rhs[j][None, :])[0])
-        stats.append(dict(solver.solve_stats))
+    for st in range(args.warmup, args.warmup + args.steps):
+        x, stt = solve(nb * st, nb, 'mixed', args.tol)
+        sols.append(x)
+        stats.append(stt)
     elapsed = time.time() - t0
-    # error of the mixed-precision solution vs the FP64 solution (same RHS as step 0)
-    x_mixed0 = solver.solve(rhs[0][None, :])[0]
-    coef_err = float(np.linalg.norm(x_mixed0 - x_ref) / np.linalg.norm(x_ref))
+    # accuracy: RHS 0 solved inside a batch vs the FP64 solution
+    x_b0 = solve(0, nb, 'mixed', args.tol)[0][0]
+    coef_err = float(np.linalg.norm(x_b0 - x_ref) / np.linalg.norm(x_ref))
+    single_err = float(np.linalg.norm(x_single[0] - x_ref) / np.linalg.norm(x_ref))
     worst_res = max(s['relative_residual'] for s in stats)
     ok = coef_err <= 1e-5 and worst_res <= args.tol
-    value = args.steps / elapsed if ok else 0.0
-    log('mixed solves: %.2f s for %d steps, residual %.2e, coefficient error vs FP64 %.2e'
-        % (elapsed, args.steps, worst_res, coef_err))
+    n_solved = nb * args.steps
+    value = n_solved / elapsed if ok else 0.0
+    log('batched mixed solves: %.2f s for %d RHS, residual %.2e, coefficient error vs FP64 %.2e'
+        % (elapsed, n_solved, worst_res, coef_err))
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -222,7 +236,9 @@ def main():
         cpu = cpu_baseline(op.lookup_table, op.radial_distance_array, len(plist), plain_stats['iterations'])
         log('cpu baseline %.1f s' % (time.time() - t0))
 
-    achieved = flops_alg / (k32 * 1e-3) / 1e12
+    batched = nb > 1
+    kern_ms = kb32 if batched else k32
+    achieved = (flops_8 if batched else flops_1) / (kern_ms * 1e-3) / 1e12
     out = {
         'metric': 'RHS-solves/s at 10k spheres, err<=1e-5 vs FP64; coupling matvec GB/s',
         'value': value,
@@ -238,24 +254,34 @@ def main():
         'data': 'synthetic',
         'config': {
             'workload': ('radial-lookup coupling solve, %d spheres monolayer on glass, l_max=3, cubic spline 2 nm, '
-                         'GMRES tol %.0e, FP32-in-FP64 refinement' % (len(plist), args.tol)),
+                         'GMRES tol %.0e, FP32-in-FP64 refinement, %d right-hand sides (plane waves) per step'
+                         % (len(plist), args.tol, nb)),
             'particles': len(plist), 'unknowns': n_unk, 'lookup_rows': int(len(op.radial_distance_array)),
             'n_kappa': int(len(kp)), 'side_nm': side, 'seed': 2, 'tolerance': args.tol,
-            'interpolation': 'cubic (not-a-knot spline)', 'resolution_nm': 2.0,
+            'interpolation': 'cubic (not-a-knot spline)', 'resolution_nm': 2.0, 'rhs_per_step': nb,
         },
         'coupling_matvec_gbps_equiv_dense': E * 8 / (k32 * 1e-3) / 1e9,
-        'matvec_ms': {'fp32_kernel': k32, 'fp32_total': tot32, 'fp64_kernel': k64, 'fp64_total': tot64},
+        'matvec_ms': {'fp32_kernel': k32, 'fp32_total': tot32, 'fp64_kernel': k64, 'fp64_total': tot64,
+                      'fp32_kernel_8rhs': kb32, 'fp32_total_8rhs': totb32, 'fp64_kernel_8rhs': kb64,
+                      'fp64_total_8rhs': totb64},
         'coef_err_vs_fp64': coef_err,
         'worst_relative_residual': worst_res,
         'solve_stats': stats,
+        'single_rhs_mixed': {'seconds': single_s, 'rhs_solves_per_s': 1.0 / single_s, 'coef_err_vs_fp64': single_err,
+                             **single_stats},
         'fp64_plain_gmres': {'seconds': fp64_plain_s, 'rhs_solves_per_s': 1.0 / fp64_plain_s, **plain_stats},
         'speedup_vs_fp64_gmres': (value * fp64_plain_s) if value else None,
         'prepare_seconds': prep_s,
         'table_seconds': op.table_seconds,
         'roofline': {'bound': 'mfma', 'achieved': achieved, 'peak': MFMA_FP32_PEAK_TFLOPS, 'unit': 'TFLOP/s',
-                     'frac': achieved / MFMA_FP32_PEAK_TFLOPS, 'traffic': measured_traffic(),
-                     'kernel': 'radial_matvec_kernel<float,3,3,4>',
-                     'flops_per_launch': flops_alg},
+                     'frac': achieved / MFMA_FP32_PEAK_TFLOPS,
+                     'traffic': measured_traffic('batch' if batched else 'single'),
+                     'kernel': ('batch_slots_kernel<3,3,4>' if batched
+                                else 'lookup_matvec_kernel<float,float,3,3,4,false>'),
+                     'flops_per_launch': flops_8 if batched else flops_1},
+        'roofline_single_rhs_kernel': {'achieved': flops_1 / (k32 * 1e-3) / 1e12, 'peak': MFMA_FP32_PEAK_TFLOPS,
+                                       'frac': flops_1 / (k32 * 1e-3) / 1e12 / MFMA_FP32_PEAK_TFLOPS,
+                                       'traffic': measured_traffic('single')},
         'cpu_baseline': cpu,
         'device': dev, 'compute_units': n_cu, 'host': platform.node(), 'host_cpus': os.cpu_count(),
     }

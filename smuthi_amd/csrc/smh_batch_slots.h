@@ -1,0 +1,294 @@
+// FP32 lookup matvec for kBatch right-hand sides in "slot" order.
+//
+// Same product as batch_matvec_kernel (per pair: interpolate the lookup
+// block, then one MFMA 16x16x4 tile per K-step applies it to all right-hand
+// sides, columns = (rhs, re|im)), with the multipoles permuted inside the
+// kernel so that the azimuthal phases cost no per-lane selection:
+//
+//   slot order: multipoles grouped by |m| (m = 0 first); inside a group the
+//   m > 0 ones before the m < 0 ones; every group padded to a multiple of 4
+//   slots (padding slots have zero taps and zero inputs).
+//
+// An MFMA K-step covers 4 slots, so its |m| is known at compile time and
+// e^{i m2 phi} x reduces to z = cos(|m| phi) a + sin(|m| phi) b with lane
+// operands (a, b) = (Re x, Im x) or (Im x, Re x) and a sign that is uniform
+// per step or + for slots 0, 1 and - for slots 2, 3 of the step. The 4 result
+// rows a lane holds (4q + v) are one slot group as well, so the receiver
+// rotation e^{-i m1 phi} needs one phase per lane and tile. The re/im
+// combination of a (rhs, re) / (rhs, im) lane pair is one DPP exchange.
+//
+// Layouts (S = slots per particle = Bh, NS = S / 4 K-steps):
+//   coefficients [bins][T][MT][NS/4][64 lanes][4]: the interpolant on a bin
+//         as a polynomial in the bin fraction w (evaluated by Horner, T - 1
+//         FMAs per value); lane (q, rr) of tile i gets
+//         c_k(slot 16 (i mod MH) + rr, slot 4 s + q), s = 4 s4 + j, rows of
+//         tiles i < MH from Re L, the others from Im L (one dwordx4 per s4);
+//   x     [n_part][4 (q)][kBatch][NS] float2: the NS inputs a lane needs are
+//         contiguous (NS/2 dwordx4 loads per pair);
+//   LDS   [Rc][2 (hi, lo)][16 columns][S + 1] int32 (two-level fixed point);
+//   ypart [n_items][Rc][16 columns][S] float.
+#pragma once
+
+#include "smh_op.h"
+#include "smh_radial_kernel.h"
+
+namespace smh {
+
+template <int L, int M>
+struct SlotOrder {
+  static constexpr int B = SvwfIdx<L, M>::B, S = SvwfIdx<L, M>::Bh, NS = S / 4;
+  int n_of[S];     // multipole of slot k (-1: padding)
+  int slot_of[B];  // slot of multipole n
+  int absm[NS];    // |m| of K-step / row group g (slots 4g..4g+3)
+  int sgn[NS];     // sign of m in group g: +1, -1, or 0 (+ for slots 4g, 4g+1, - for 4g+2, 4g+3)
+  bool ok;
+  constexpr SlotOrder() : n_of(), slot_of(), absm(), sgn(), ok(true) {
+    int sg[S] = {};
+    int k = 0;
+    for (int a = 0; a <= M; ++a) {
+      const int k0 = k;
+      for (int pass = 0; pass < (a ? 2 : 1); ++pass)
+        for (int n = 0; n < B; ++n) {
+          const int m = SvwfIdx<L, M>::m_of(n);
+          if (m == (pass ? -a : a) && k < S) {
+            n_of[k] = n;
+            slot_of[n] = k;
+            sg[k] = pass ? -1 : 1;
+            ++k;
+          }
+        }
+      while (k % 4 && k < S) {
+        n_of[k] = -1;
+        sg[k] = 0;
+        ++k;
+      }
+      for (int g = k0 / 4; g < k / 4; ++g) absm[g] = a;
+    }
+    if (k > S) ok = false;
+    while (k < S) {
+      n_of[k] = -1;
+      sg[k] = 0;
+      ++k;
+    }
+    for (int g = 0; g < NS; ++g) {
+      bool pos = false, neg = false;
+      for (int j = 0; j < 4; ++j) {
+        pos = pos || sg[4 * g + j] > 0;
+        neg = neg || sg[4 * g + j] < 0;
+      }
+      if (pos && neg) {
+        sgn[g] = 0;
+        // mixed groups must be (+, +, -, -)
+        if (sg[4 * g] < 0 || sg[4 * g + 1] < 0 || sg[4 * g + 2] > 0 || sg[4 * g + 3] > 0) ok = false;
+      } else {
+        sgn[g] = neg ? -1 : 1;
+      }
+    }
+    int cnt = 0;
+    for (int s = 0; s < S; ++s) cnt += n_of[s] >= 0;
+    if (cnt != B) ok = false;
+  }
+};
+
+// Host copy of the slot order: slot_of[B] then n_of[S].
+template <int L, int M>
+void slot_tables(int* out) {
+  constexpr SlotOrder<L, M> so{};
+  static_assert(so.ok, "slot order");
+  for (int n = 0; n < so.B; ++n) out[n] = so.slot_of[n];
+  for (int k = 0; k < so.S; ++k) out[so.B + k] = so.n_of[k];
+}
+
+// Shape of the slot kernel (FP32, 512 threads, one tap fragment set of at
+// most 128 registers per lane).
+template <int L, int M, int T>
+struct SlotShape {
+  static constexpr int S = SvwfIdx<L, M>::Bh, NS = S / 4, MT = 2 * S / 16, MH = MT / 2;
+  static constexpr int SC = S + 1;          // LDS column stride (words)
+  static constexpr int LO = 2 * kBatchRhs * SC;  // offset of the low words
+  static constexpr int SY = 2 * LO;         // LDS words per receiver
+  static constexpr int FRB = MT * NS * 64;  // floats per tap row
+  static constexpr bool fits = T * MT * NS <= 128;
+};
+
+__device__ inline float xor_sign(float v, unsigned m) { return __uint_as_float(__float_as_uint(v) ^ m); }
+// v + c * (value of lane l ^ 1)
+__device__ inline float fma_xor1(float v, float c, float other) {
+  return fmaf(c, __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(other), 0xB1, 0xF, 0xF, false)), v);
+}
+
+template <int LMAX, int MMAX, int T>
+__global__ void __launch_bounds__(512) batch_slots_kernel(RadialKArgs a) {
+  using Sh = SlotShape<LMAX, MMAX, T>;
+  constexpr int S = Sh::S, NS = Sh::NS, MT = Sh::MT, MH = Sh::MH, SC = Sh::SC, LO = Sh::LO, SY = Sh::SY;
+  constexpr int KB = kBatchRhs;
+  static_assert(KB * 2 == 16, "16 result columns = kBatch right-hand sides x (re, im)");
+  static_assert(Sh::fits, "tap fragments exceed the register budget");
+  static_assert(MMAX <= 3, "phase selection covers |m| <= 3");
+  constexpr SlotOrder<LMAX, MMAX> so{};
+  static_assert(so.ok, "slot order");
+
+  extern __shared__ __align__(16) int yacc[];
+  int* next_entry = yacc + (size_t)a.Rc * SY;
+  float sh = 0.f, ish = 0.f, sl = 0.f, isl = 0.f;
+  fixed_scales(a.item_lsum[blockIdx.x], *a.xmax, a.item_en[blockIdx.x], sh, ish, sl, isl);
+  const float slo = sl * ish;  // low-word scale relative to the high word
+
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int item = blockIdx.x;
+  const int2 it = a.items[item];
+  const int e_end = item + 1 < a.n_items ? a.items[item + 1].y : a.n_entries;
+  const PairRec<float>* recs = reinterpret_cast<const PairRec<float>*>(a.recs);
+  const float2* xg = reinterpret_cast<const float2*>(a.x);
+  const float4* tt4 = reinterpret_cast<const float4*>(a.tt);
+  for (int j = tid; j < a.Rc * SY; j += blockDim.x) yacc[j] = 0;
+  if (tid == 0) *next_entry = it.y;
+  __syncthreads();
+
+  const int rr = lane & 15, q = lane >> 4;
+  const int rhs = rr >> 1, zp = rr & 1;
+  const unsigned zmask = zp ? 0u : 0x80000000u;      // z: sign of the b term for the re column
+  const unsigned qmask = q >= 2 ? 0x80000000u : 0u;  // mixed-sign steps: slots 2, 3 have m < 0
+  const float sgnz = zp ? 1.f : -1.f;
+  // result rows of tile i are slot group 4 i + q: its |m| and the sign masks of rows v < 2 / v >= 2
+  // (as one-hot weights hot[i][a] = [|m| == a], so the per-pair phase of a lane is
+  // a short FMA chain over the |m| values that occur in the tile, no branches)
+  float hot[MH][4];
+  unsigned smlo[MH], smhi[MH];
+#pragma unroll
+  for (int i = 0; i < MH; ++i) {
+    const int g0 = 4 * i;
+    const int ra = q == 0 ? so.absm[g0] : (q == 1 ? so.absm[g0 + 1] : (q == 2 ? so.absm[g0 + 2] : so.absm[g0 + 3]));
+    const int sg = q == 0 ? so.sgn[g0] : (q == 1 ? so.sgn[g0 + 1] : (q == 2 ? so.sgn[g0 + 2] : so.sgn[g0 + 3]));
+#pragma unroll
+    for (int am = 0; am < 4; ++am) hot[i][am] = ra == am ? 1.f : 0.f;
+    smlo[i] = sg < 0 ? 0x80000000u : 0u;
+    smhi[i] = sg > 0 ? 0u : 0x80000000u;
+  }
+  const int colbase = (rhs * 2 + zp) * SC + 4 * q;
+  const int64_t xlane = (int64_t)(q * KB + rhs) * NS;
+
+  for (;;) {
+    int e0 = 0;
+    if (lane == 0) e0 = atomicAdd(next_entry, 1);
+    const int e = __builtin_amdgcn_readfirstlane(e0);
+    if (e >= e_end) break;
+    const int4 en = a.entries[e];
+    const int bin = (en.x - a.row_o) / a.row_s;
+    // polynomial coefficients tv[t][i][s] = c_t(tile i row rr, slot 4 s + q) of the entry's bin
+    float tv[T][MT][NS];
+#pragma unroll
+    for (int t = 0; t < T; ++t) {
+      const float4* row = tt4 + ((size_t)bin * T + t) * (Sh::FRB / 4);
+#pragma unroll
+      for (int i = 0; i < MT; ++i)
+#pragma unroll
+        for (int s4 = 0; s4 < NS / 4; ++s4) {
+          const float4 v = row[(i * (NS / 4) + s4) * 64 + lane];
+          tv[t][i][4 * s4 + 0] = v.x;
+          tv[t][i][4 * s4 + 1] = v.y;
+          tv[t][i][4 * s4 + 2] = v.z;
+          tv[t][i][4 * s4 + 3] = v.w;
+        }
+    }
+    const int p_end = en.z;
+    auto load_x = [&](const PairRec<float>& r, float4 (&xs)[NS / 2]) {
+      const float4* xp = reinterpret_cast<const float4*>(xg + (int64_t)(r.ij & kPairMask) * S * KB + xlane);
+#pragma unroll
+      for (int k = 0; k < NS / 2; ++k) xs[k] = xp[k];
+    };
+    PairRec<float> rec_n = recs[en.y];
+    float4 xc[NS / 2];
+    load_x(rec_n, xc);
+    for (int p = 0; p < p_end; ++p) {
+      const PairRec<float> rec = rec_n;
+      rec_n = recs[en.y + (p + 1 < p_end ? p + 1 : p)];
+
+      const float w = rec.w;
+      // cos / sin of k phi, k = 1..3
+      const float c1 = rec.c, s1 = rec.s;
+      const float c2 = c1 * c1 - s1 * s1, s2 = 2.f * c1 * s1;
+      const float c3 = c2 * c1 - s2 * s1, s3 = s2 * c1 + c2 * s1;
+      const float cs[4] = {1.f, c1, c2, c3};
+      const float sn[4] = {0.f, s1, s2, s3};
+      float ez[4] = {0.f, 0.f, 0.f, 0.f};  // sin(k phi) with the re-column sign
+#pragma unroll
+      for (int k = 1; k <= MMAX; ++k) ez[k] = xor_sign(sn[k], zmask);
+
+      // B operand per K-step: (e^{i m2 phi} x[slot 4 s + q][rhs]).re or .im
+      float z[NS];
+#pragma unroll
+      for (int s = 0; s < NS; ++s) {
+        const float4 x4 = xc[s >> 1];
+        const float xr = (s & 1) ? x4.z : x4.x, xi = (s & 1) ? x4.w : x4.y;
+        const float av0 = zp ? xi : xr, bv0 = zp ? xr : xi;
+        const int am = so.absm[s];
+        if (am == 0) {
+          z[s] = av0;
+        } else {
+          const int sg = so.sgn[s];
+          const float e = sg > 0 ? ez[am] : (sg < 0 ? -ez[am] : xor_sign(ez[am], qmask));
+          z[s] = fmaf(cs[am], av0, e * bv0);
+        }
+      }
+      // the input of the next pair goes into the same registers
+      load_x(rec_n, xc);
+
+      typename Mfma<float>::v4 acc[MT];
+#pragma unroll
+      for (int i = 0; i < MT; ++i) acc[i] = typename Mfma<float>::v4{0, 0, 0, 0};
+#pragma unroll
+      for (int s = 0; s < NS; ++s) {
+#pragma unroll
+        for (int i = 0; i < MT; ++i) {
+          float v = tv[T - 1][i][s];
+#pragma unroll
+          for (int t = T - 2; t >= 0; --t) v = fmaf(v, w, tv[t][i][s]);
+          acc[i] = Mfma<float>::op(v, z[s], acc[i]);
+        }
+      }
+
+      // y (rhs, re|im) of rows 16 i + 4 q + v, rotated by e^{-i m1 phi}, scaled by S_hi
+      int* yrow = yacc + (size_t)(rec.ij >> kPairShift) * SY + colbase;
+#pragma unroll
+      for (int i = 0; i < MH; ++i) {
+        float er = 0.f, ei = 0.f;
+#pragma unroll
+        for (int am = 0; am <= MMAX; ++am) {
+          // |m| values outside this tile's four row groups drop out at compile time
+          if (so.absm[4 * i] != am && so.absm[4 * i + 1] != am && so.absm[4 * i + 2] != am &&
+              so.absm[4 * i + 3] != am)
+            continue;
+          er = fmaf(hot[i][am], cs[am], er);
+          if (am > 0) ei = fmaf(hot[i][am], sn[am], ei);
+        }
+        er *= sh;
+        ei *= sh;
+        const float elo = xor_sign(ei, smlo[i]), ehi = xor_sign(ei, smhi[i]);
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const float eps = v < 2 ? elo : ehi;
+          const float al = acc[i][v], be = acc[i + MH][v];
+          const float own = fmaf(al, er, be * eps);
+          const float snd = fmaf(be, er, -(al * eps));
+          const float val = fma_xor1(own, sgnz, snd);
+          const float h = rintf(val);
+          const float l = rintf((val - h) * slo);
+          atomicAdd(yrow + 16 * i + v, (int)h);
+          atomicAdd(yrow + LO + 16 * i + v, (int)l);
+        }
+      }
+    }
+  }
+  __syncthreads();
+  // partial of this item: [Rc][16 columns][S]
+  float* yp = reinterpret_cast<float*>(a.ypart) + (size_t)item * a.Rc * 2 * KB * S;
+  for (int j = tid; j < a.Rc * 2 * KB * S; j += blockDim.x) {
+    const int il = j / (2 * KB * S), k = j - il * 2 * KB * S;
+    const int c = k / S, sl2 = k - c * S;
+    const int w0 = il * SY + c * SC + sl2;
+    yp[j] = (float)((double)yacc[w0] * (double)ish + (double)yacc[w0 + LO] * (double)isl);
+  }
+}
+
+}  // namespace smh

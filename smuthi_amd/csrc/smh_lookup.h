@@ -19,7 +19,9 @@ inline int blocks_for(int64_t n, int t = 256) { return (int)((n + t - 1) / t); }
 struct KernelInfo {
   void* fn = nullptr;
   int threads = 0;
-  int sy = 0;  // accumulator stride per receiver
+  int sy = 0;     // accumulator stride per receiver
+  int slots = 0;  // kBatch-RHS kernels in slot order: slots per particle (0: plain layout)
+  void (*slot_tables)(int*) = nullptr;  // host slot order: slot_of[B], then n_of[slots]
 };
 
 // Receiver chunk of a precision: as many receivers (at most 512, the pair
@@ -156,15 +158,22 @@ __global__ void xmax_kernel(const float2* __restrict__ x, int64_t n, float* xmax
   if (threadIdx.x == 0) *xmax = red[0];
 }
 
+// Work items per compute unit: items of one receiver chunk run one after the
+// other on a CU's single workgroup slot, so ~16 rounds keep the last,
+// partly filled round short.
+constexpr int kItemsPerCu = 16;
+
 // Host: bin entries (at most kEntryPairs pairs each) of every non-empty
-// (chunk, bin) key, grouped into work items of similar cost (a tile of 16
-// pairs costs 1, loading a bin's taps 1/4), then uploaded with the per-item
-// pair statistics. cnt / off: pair count and first pair of each key
-// (chunk-major, bins_per_chunk keys per chunk); row_of(bin) gives the bin's
-// first tap row.
+// (chunk, bin) key, grouped into work items of similar cost, then uploaded
+// with the per-item pair statistics. Cost of an entry: single-vector kernels
+// one unit per tile of 16 pairs plus 1/4 for loading the bin's taps; the
+// kBatch-RHS kernels (batch set) one unit per pair plus one for the taps.
+// cnt / off: pair count and first pair of each key (chunk-major,
+// bins_per_chunk keys per chunk); row_of(bin) gives the bin's first tap row.
 template <typename Rec, typename RowOf>
 int build_work_items(smh_op* op, smh_op::Binning& bn, const std::vector<int>& cnt, const std::vector<int>& off,
-                     int64_t bins_per_chunk, RowOf row_of) {
+                     int64_t bins_per_chunk, RowOf row_of, bool batch = false) {
+  auto cost = [batch](int np) { return batch ? np + 1.0 : (np + 15) / 16 + 0.25; };
   smh_ctx* ctx = op->ctx;
   hipStream_t s = ctx->stream;
   std::vector<int4> entries;
@@ -178,10 +187,10 @@ int build_work_items(smh_op* op, smh_op::Binning& bn, const std::vector<int>& cn
         const int np = std::min(kEntryPairs, n - p0);
         entries.push_back(make_int4(row_of(r), off[key] + p0, np, 0));
         ent_chunk.push_back(c);
-        total += (np + 15) / 16 + 0.25;
+        total += cost(np);
       }
     }
-  const int64_t target_items = std::max<int64_t>(4 * (int64_t)ctx->num_cu, bn.n_chunks);
+  const int64_t target_items = std::max<int64_t>(kItemsPerCu * (int64_t)ctx->num_cu, bn.n_chunks);
   const double budget = std::max(1.0, total / (double)target_items);
   std::vector<int2> items;
   std::vector<int> chunk_items(bn.n_chunks + 1, 0);
@@ -194,7 +203,7 @@ int build_work_items(smh_op* op, smh_op::Binning& bn, const std::vector<int>& cn
       items.push_back(make_int2(c, (int)e));
       run = 0;
     }
-    run += (entries[e].z + 15) / 16 + 0.25;
+    run += cost(entries[e].z);
   }
   while (cur_chunk < bn.n_chunks) chunk_items[++cur_chunk] = (int)items.size();
   bn.n_entries = (int)entries.size();

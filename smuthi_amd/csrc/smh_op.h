@@ -28,6 +28,12 @@ struct smh_op {
   double* d_tt64 = nullptr;          // tap table [n_tt][FR] in MFMA fragment order
   float* d_tt32 = nullptr;
   int FR = 0;                        // reals per tap row
+  // FP32 kBatch-RHS kernel in slot order (multipoles grouped by |m|): its tap
+  // table, the slot tables (slot_of[B], n_of[b32_S]) and the slots per
+  // particle of its input layout (b32_S = B and no tables: plain layout)
+  float* d_tt32b = nullptr;
+  int* d_b32_tab = nullptr;
+  int b32_S = 0;
   float* d_rowsum = nullptr;         // per tap row: max_n1 sum_n2 |Re| + |Im|
   float* d_xmax = nullptr;           // largest |Re| + |Im| of the current FP32 input
 
@@ -74,13 +80,22 @@ void smh_op_free(smh_op* op);
 namespace smh {
 // right-hand sides per batched matvec / batched GMRES block
 constexpr int kBatchRhs = 8;
+// Index of (particle p, multipole n, rhs r) in the FP32 kBatch-RHS input:
+// slot layout [n_part][4][kBatch][S/4] (slot k = slot[n] sits at (k % 4, r,
+// k / 4)) when a slot table is given, else the plain layout [n_part][S = B][kBatch].
+__host__ __device__ inline int64_t b32_index(int64_t p, int n, int r, int S, const int* slot) {
+  if (!slot) return (p * S + n) * kBatchRhs + r;
+  const int k = slot[n];
+  return p * S * kBatchRhs + ((k & 3) * kBatchRhs + r) * (S / 4) + (k >> 2);
+}
 // packed -> padded index map of the particles' multipole blocks
 int op_index_map(smh_op* op, const int* lmax, const int* mmax);
 // volume operator: matvec, table upload / download (smh_volume.hip)
 int vol_apply(smh_op* op, int precision, const void* x_pad, void* y_pad, bool time_kernel);
 int vol_upload(smh_op* op, const double* plus, const double* minus);
 int vol_download(smh_op* op, double* plus, double* minus);
-// kBatch-RHS matvec on padded [n_part][B][kBatch] vectors (radial operators)
+// kBatch-RHS matvec (radial operators): FP64 input and every output on padded
+// [n_part][B][kBatch] vectors; the FP32 input in the layout of b32_index()
 bool op_has_batch(const smh_op* op, int precision);
 int op_apply_batch(smh_op* op, int precision, const void* x_pad, void* y_pad, bool time_kernel);
 int op_unpack_batch(smh_op* op, int precision, const double2* pk, int nrhs, void* pad);

@@ -4,6 +4,7 @@
 
 #include "smh_op.h"
 #include "smh_batch_kernel.h"
+#include "smh_batch_slots.h"
 #include "smh_lookup.h"
 
 using namespace smh;
@@ -122,6 +123,66 @@ __global__ void taps_kernel(const double2* __restrict__ table, const double2* __
   }
 }
 
+// Coefficient table of the FP32 slot-order kBatch-RHS kernel: the
+// interpolant on bin r (rho in [rho_r, rho_r + h), w = (rho - rho_r) / h) as
+// a polynomial sum_k c_k w^k, k < T, stored as rows r T + k of floats
+//   F[i][s4][lane][e] = c_k(slot 16 (i mod MH) + rr, slot 4 (4 s4 + e) + q),  lane = 16 q + rr,
+// rows from Re L for i < MH and from Im L for i >= MH, zero for padding slots
+// (n_of[slot] < 0). The coefficients expand the tap weights of tap_weight():
+//   linear       c0 = y_r, c1 = y_{r+1} - y_r;
+//   Catmull-Rom  c0 = y_r, c1 = (y_{r+1} - y_{r-1}) / 2,
+//                c2 = y_{r-1} - 5/2 y_r + 2 y_{r+1} - 1/2 y_{r+2},
+//                c3 = (-y_{r-1} + 3 y_r - 3 y_{r+1} + y_{r+2}) / 2;
+//   spline       c0 = y_r, c1 = y_{r+1} - y_r - 2 Mh_r - Mh_{r+1}, c2 = 3 Mh_r, c3 = Mh_{r+1} - Mh_r.
+// Bins no pair uses (at the table ends) read clamped rows.
+__global__ void poly_slot_kernel(const double2* __restrict__ table, const double2* __restrict__ mh, int64_t len,
+                                 int B, int S, const int* __restrict__ n_of, int interp, int T, float* tt) {
+  const int NS = S / 4, MH = S / 16, FRB = 2 * MH * NS * 64;
+  const int64_t total = len * T * FRB;
+  for (int64_t f = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; f < total; f += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t row = f / FRB;
+    int rem = (int)(f - row * FRB);
+    const int e = rem & 3;
+    const int ln = (rem >> 2) & 63;
+    rem >>= 8;
+    const int s4 = rem % (NS / 4);
+    const int i = rem / (NS / 4);
+    const int q = ln >> 4, rr = ln & 15;
+    const int n1 = n_of[16 * (i % MH) + rr];
+    const int n2 = n_of[4 * (4 * s4 + e) + q];
+    const int64_t r = row / T;
+    const int k = (int)(row - r * T);
+    float v = 0.f;
+    if (n1 >= 0 && n2 >= 0) {
+      const int64_t nn = (int64_t)n1 * B + n2;
+      auto Y = [&](int64_t j) {
+        j = j < 0 ? 0 : (j >= len ? len - 1 : j);
+        const double2 c = table[j * B * B + nn];
+        return i < MH ? c.x : c.y;
+      };
+      auto Mh = [&](int64_t j) {
+        j = j < 0 ? 0 : (j >= len ? len - 1 : j);
+        const double2 c = mh[j * B * B + nn];
+        return i < MH ? c.x : c.y;
+      };
+      double c = 0.0;
+      if (interp == SMH_INTERP_LINEAR) {
+        c = k == 0 ? Y(r) : Y(r + 1) - Y(r);
+      } else if (interp == SMH_INTERP_CATMULL_ROM) {
+        const double p0 = Y(r - 1), p1 = Y(r), p2 = Y(r + 1), p3 = Y(r + 2);
+        c = k == 0 ? p1
+                   : (k == 1 ? 0.5 * (p2 - p0)
+                             : (k == 2 ? p0 - 2.5 * p1 + 2.0 * p2 - 0.5 * p3 : 0.5 * (-p0 + 3.0 * p1 - 3.0 * p2 + p3)));
+      } else {
+        const double y0 = Y(r), y1 = Y(r + 1), m0 = Mh(r), m1 = Mh(r + 1);
+        c = k == 0 ? y0 : (k == 1 ? y1 - y0 - 2.0 * m0 - m1 : (k == 2 ? 3.0 * m0 : m1 - m0));
+      }
+      v = (float)c;
+    }
+    tt[f] = v;
+  }
+}
+
 // Not-a-knot cubic spline moments, scaled: Mh_j = h^2/6 S''(rho_j), one
 // thread per complex column (n1, n2). With uniform spacing the system is
 //   Mh_{j-1} + 4 Mh_j + Mh_{j+1} = y_{j-1} - 2 y_j + y_{j+1},  j = 1..n-2,
@@ -200,13 +261,23 @@ KernelInfo pick(int l, int m) {
   return KernelInfo{};
 }
 
+// FP32 operators whose tap fragments fit one register set use the slot-order
+// kernel; the others (and FP64) the plain kBatch-RHS kernel.
 template <int L, int Mx, int T, typename Real>
 KernelInfo batch_kernel_info() {
-  using Sh = BatchShape<Real, SvwfIdx<L, Mx>::B, T>;
   KernelInfo k;
-  k.fn = reinterpret_cast<void*>(&batch_matvec_kernel<Real, Real, L, Mx, T, false>);
-  k.threads = Sh::kThreads;
-  k.sy = Sh::SY;
+  if constexpr (std::is_same<Real, float>::value && SlotShape<L, Mx, T>::fits && Mx <= 3) {
+    k.fn = reinterpret_cast<void*>(&batch_slots_kernel<L, Mx, T>);
+    k.threads = 512;
+    k.sy = SlotShape<L, Mx, T>::SY;
+    k.slots = SlotShape<L, Mx, T>::S;
+    k.slot_tables = &slot_tables<L, Mx>;
+  } else {
+    using Sh = BatchShape<Real, SvwfIdx<L, Mx>::B, T>;
+    k.fn = reinterpret_cast<void*>(&batch_matvec_kernel<Real, Real, L, Mx, T, false>);
+    k.threads = Sh::kThreads;
+    k.sy = Sh::SY;
+  }
   return k;
 }
 
@@ -309,6 +380,12 @@ int op_make_taps(smh_op* op) {
   }
   hipLaunchKernelGGL(taps_kernel, dim3(4096), dim3(256), 0, s, op->d_table, d_mh, len, op->B, op->Bh, g.G, g.KS, g.NG,
                      g.FR, op->interp == SMH_INTERP_SPLINE ? 1 : 0, op->d_tt64, op->d_tt32);
+  if (op->d_b32_tab) {
+    const size_t frb = (size_t)2 * op->b32_S * op->b32_S;  // MT * NS * 64 floats
+    if (!op->d_tt32b) SMH_HIP(ctx, hipMalloc(&op->d_tt32b, (size_t)len * op->T * frb * sizeof(float)));
+    hipLaunchKernelGGL(poly_slot_kernel, dim3(4096), dim3(256), 0, s, op->d_table, d_mh, len, op->B, op->b32_S,
+                       op->d_b32_tab + op->B, op->interp, op->T, op->d_tt32b);
+  }
   if (!op->d_rowsum) {
     SMH_HIP(ctx, hipMalloc(&op->d_rowsum, (size_t)op->n_tt * sizeof(float)));
     SMH_HIP(ctx, hipMalloc(&op->d_xmax, sizeof(float)));
@@ -384,15 +461,19 @@ int op_apply(smh_op* op, int precision, const void* x_pad, void* y_pad, bool tim
 namespace {
 
 // kBatch-vector layouts: packed [nrhs][n_unk] (smuthi) <-> padded [n_part][B][kBatch]
-// (right-hand sides nrhs..kBatch-1 are zero).
+// (right-hand sides nrhs..kBatch-1 are zero); the FP32 input may use the slot
+// layout of b32_index() instead (S slots per particle, slot table given).
 template <typename Real>
 __global__ void scatter_batch_kernel(const double2* __restrict__ pk, const int* __restrict__ pk2pad, int64_t n_unk,
-                                     int nrhs, typename Vec2<Real>::t* pad) {
+                                     int nrhs, int B, int S, const int* __restrict__ slot,
+                                     typename Vec2<Real>::t* pad) {
   const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= n_unk * nrhs) return;
   const int64_t r = j / n_unk, u = j - r * n_unk;
   const double2 v = pk[j];
-  pad[(int64_t)pk2pad[u] * kBatch + r] = typename Vec2<Real>::t{Real(v.x), Real(v.y)};
+  const int ip = pk2pad[u];
+  const int64_t p = ip / B;
+  pad[b32_index(p, (int)(ip - p * B), (int)r, S, slot)] = typename Vec2<Real>::t{Real(v.x), Real(v.y)};
 }
 
 template <typename Real>
@@ -405,23 +486,26 @@ __global__ void pack_batch_kernel(const typename Vec2<Real>::t* __restrict__ pad
   pk[j] = make_double2((double)v.x, (double)v.y);
 }
 
-// Sum the work items of each chunk for the kBatch-RHS kernel: partial
-// [Rc][kBatch][2][B] -> y [n_part][B][kBatch].
+// Sum the work items of each chunk for the kBatch-RHS kernels: partial
+// [Rc][kBatch][2][S] (slot order when slot is given, else S = B) -> y
+// [n_part][B][kBatch].
 template <typename Real>
 __global__ void reduce_batch_kernel(const Real* __restrict__ ypart, const int* __restrict__ chunk_items, int Rc,
-                                    int B, int64_t n_part, typename Vec2<Real>::t* y) {
+                                    int B, int S, const int* __restrict__ slot, int64_t n_part,
+                                    typename Vec2<Real>::t* y) {
   const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= n_part * B * kBatch) return;
   const int r = (int)(j % kBatch);
   const int64_t in = j / kBatch;
   const int64_t i1 = in / B;
   const int n1 = (int)(in - i1 * B);
+  const int k1 = slot ? slot[n1] : n1;
   const int c = (int)(i1 / Rc), il = (int)(i1 - (int64_t)c * Rc);
   Real re = 0, im = 0;
   for (int w = chunk_items[c]; w < chunk_items[c + 1]; ++w) {
-    const Real* p = ypart + ((size_t)w * Rc + il) * 2 * kBatch * B + (size_t)(2 * r) * B;
-    re += p[n1];
-    im += p[B + n1];
+    const Real* p = ypart + ((size_t)w * Rc + il) * 2 * kBatch * S + (size_t)(2 * r) * S;
+    re += p[k1];
+    im += p[S + k1];
   }
   y[j] = typename Vec2<Real>::t{re, im};
 }
@@ -432,16 +516,16 @@ namespace smh {
 
 int op_unpack_batch(smh_op* op, int precision, const double2* pk, int nrhs, void* pad) {
   hipStream_t s = op->ctx->stream;
-  const int64_t n_pad = op->n_part * op->B * kBatch;
   const int64_t n = op->n_unk * nrhs;
   if (precision == SMH_FP32) {
-    SMH_HIP(op->ctx, hipMemsetAsync(pad, 0, n_pad * sizeof(float2), s));
+    const int* slot = op->d_b32_tab;
+    SMH_HIP(op->ctx, hipMemsetAsync(pad, 0, (size_t)op->n_part * op->b32_S * kBatch * sizeof(float2), s));
     hipLaunchKernelGGL(scatter_batch_kernel<float>, dim3(blocks_for(n)), dim3(256), 0, s, pk, op->d_pk2pad,
-                       op->n_unk, nrhs, (float2*)pad);
+                       op->n_unk, nrhs, op->B, op->b32_S, slot, (float2*)pad);
   } else {
-    SMH_HIP(op->ctx, hipMemsetAsync(pad, 0, n_pad * sizeof(double2), s));
+    SMH_HIP(op->ctx, hipMemsetAsync(pad, 0, (size_t)op->n_part * op->B * kBatch * sizeof(double2), s));
     hipLaunchKernelGGL(scatter_batch_kernel<double>, dim3(blocks_for(n)), dim3(256), 0, s, pk, op->d_pk2pad,
-                       op->n_unk, nrhs, (double2*)pad);
+                       op->n_unk, nrhs, op->B, op->B, (const int*)nullptr, (double2*)pad);
   }
   SMH_HIP(op->ctx, hipGetLastError());
   return SMH_OK;
@@ -462,7 +546,9 @@ int op_pack_batch(smh_op* op, int precision, const void* pad, int nrhs, double2*
 
 int op_reserve_batch(smh_op* op) {
   if (op->d_xb) return SMH_OK;
-  const size_t bytes = (size_t)op->n_part * op->B * kBatch * sizeof(double2);
+  // FP64 padded vectors or the FP32 input in its (slot) layout, whichever is larger
+  const size_t bytes = (size_t)op->n_part * kBatch *
+                       std::max<size_t>((size_t)op->B * sizeof(double2), (size_t)op->b32_S * sizeof(float2));
   SMH_HIP(op->ctx, hipMalloc(&op->d_xb, bytes));
   SMH_HIP(op->ctx, hipMalloc(&op->d_yb, bytes));
   return SMH_OK;
@@ -483,7 +569,9 @@ int op_apply_batch(smh_op* op, int precision, const void* x_pad, void* y_pad, bo
   a.recs = bn.d_rec;
   a.entries = bn.d_entries;
   a.items = bn.d_items;
-  a.tt = pi == 0 ? (const void*)op->d_tt32 : (const void*)op->d_tt64;
+  const bool slots = pi == 0 && k.slots > 0;
+  SMH_CHECK(ctx, !slots || (op->d_tt32b && op->b32_S == k.slots), SMH_ERR_STATE, "slot-order tap table missing");
+  a.tt = pi == 0 ? (slots ? (const void*)op->d_tt32b : (const void*)op->d_tt32) : (const void*)op->d_tt64;
   a.x = x_pad;
   a.ypart = op->d_ypart;
   a.xmax = op->d_xmax;
@@ -494,11 +582,16 @@ int op_apply_batch(smh_op* op, int precision, const void* x_pad, void* y_pad, bo
   a.interp = op->interp;
   a.Rc = bn.Rc;
   a.len_z = 0;
+  a.row_s = op->s_mode;
+  a.row_o = op->o_mode;
   hipStream_t s = ctx->stream;
   SMH_HIP(ctx, hipFuncSetAttribute(k.fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bn.lds));
   const int64_t nx = op->n_part * op->B * kBatch;
+  const int S = slots ? k.slots : op->B;
+  const int* slot = slots ? op->d_b32_tab : nullptr;
   if (pi == 0)
-    hipLaunchKernelGGL(xmax_kernel, dim3(1), dim3(1024), 0, s, (const float2*)x_pad, nx, op->d_xmax);
+    hipLaunchKernelGGL(xmax_kernel, dim3(1), dim3(1024), 0, s, (const float2*)x_pad, op->n_part * S * kBatch,
+                       op->d_xmax);
   if (bn.n_items > 0) {
     if (time_kernel) SMH_HIP(ctx, hipEventRecord(ctx->ev0, s));
     void* args[] = {&a};
@@ -507,10 +600,11 @@ int op_apply_batch(smh_op* op, int precision, const void* x_pad, void* y_pad, bo
   }
   if (pi == 0)
     hipLaunchKernelGGL(reduce_batch_kernel<float>, dim3(blocks_for(nx)), dim3(256), 0, s, (const float*)op->d_ypart,
-                       bn.d_chunk_items, bn.Rc, op->B, op->n_part, (float2*)y_pad);
+                       bn.d_chunk_items, bn.Rc, op->B, S, slot, op->n_part, (float2*)y_pad);
   else
     hipLaunchKernelGGL(reduce_batch_kernel<double>, dim3(blocks_for(nx)), dim3(256), 0, s,
-                       (const double*)op->d_ypart, bn.d_chunk_items, bn.Rc, op->B, op->n_part, (double2*)y_pad);
+                       (const double*)op->d_ypart, bn.d_chunk_items, bn.Rc, op->B, op->B, (const int*)nullptr,
+                       op->n_part, (double2*)y_pad);
   SMH_HIP(ctx, hipGetLastError());
   if (time_kernel && bn.n_items > 0) {
     SMH_HIP(ctx, hipEventSynchronize(ctx->ev1));
@@ -552,7 +646,7 @@ int op_index_map(smh_op* op, const int* lmax, const int* mmax) {
 
 void smh_op_free(smh_op* op) {
   void* ptrs[] = {op->d_xy, op->d_xyz, op->d_pk2pad, op->d_table, op->d_table_minus, op->d_tt64, op->d_tt32,
-                  op->d_ypart, op->d_pk_in, op->d_pk_out, op->d_x32, op->d_y32, op->d_x64, op->d_y64,
+                  op->d_tt32b, op->d_b32_tab, op->d_ypart, op->d_pk_in, op->d_pk_out, op->d_x32, op->d_y32, op->d_x64, op->d_y64,
                   op->d_rowsum, op->d_xmax, op->d_xb, op->d_yb};
   for (void* p : ptrs)
     if (p) hipFree(p);
@@ -575,7 +669,7 @@ void smh_op_free(smh_op* op) {
 // Bin all ordered pairs by (receiver chunk, table interval) for one precision,
 // write their records and the bin entries / work items the matvec kernel walks.
 template <typename Real>
-static int bin_pairs(smh_op* op, smh_op::Binning& bn, int Rc) {
+static int bin_pairs(smh_op* op, smh_op::Binning& bn, int Rc, bool batch) {
   smh_ctx* ctx = op->ctx;
   hipStream_t s = ctx->stream;
   const int64_t n_part = op->n_part, len_rho = op->len_rho;
@@ -658,8 +752,8 @@ static int bin_pairs(smh_op* op, smh_op::Binning& bn, int Rc) {
   }
   cleanup();
 
-  return build_work_items<PairRec<Real>>(op, bn, cnt, off, len_rho,
-                                         [&](int64_t r) { return (int)(op->s_mode * r + op->o_mode); });
+  return build_work_items<PairRec<Real>>(
+      op, bn, cnt, off, len_rho, [&](int64_t r) { return (int)(op->s_mode * r + op->o_mode); }, batch);
 }
 
 extern "C" int smh_radial_create(smh_ctx* ctx, int64_t n_part, const double* xy, const int* lmax, const int* mmax,
@@ -733,7 +827,8 @@ extern "C" int smh_radial_create(smh_ctx* ctx, int64_t n_part, const double* xy,
     const int Rc = chunk_size(op, rs, k.sy, lds);
     if (Rc == 0) continue;  // no kernel of this precision fits; matvecs in it report UNSUPPORTED
     op->bin[pi].lds = lds;
-    const int code = pi == 0 ? bin_pairs<float>(op, op->bin[0], Rc) : bin_pairs<double>(op, op->bin[1], Rc);
+    const int code =
+        pi == 0 ? bin_pairs<float>(op, op->bin[0], Rc, false) : bin_pairs<double>(op, op->bin[1], Rc, false);
     if (code) return fail(code);
     ypart = std::max(ypart, (size_t)op->bin[pi].n_items * Rc * 2 * op->B * rs);
     // the same for the kBatch-RHS kernel, whose accumulator holds kBatch vectors per receiver
@@ -742,9 +837,23 @@ extern "C" int smh_radial_create(smh_ctx* ctx, int64_t n_part, const double* xy,
     const int Rb = kb.fn ? chunk_size(op, rs, kb.sy, ldsb) : 0;
     if (Rb == 0) continue;
     op->bbin[pi].lds = ldsb;
-    const int codeb = pi == 0 ? bin_pairs<float>(op, op->bbin[0], Rb) : bin_pairs<double>(op, op->bbin[1], Rb);
+    const int codeb =
+        pi == 0 ? bin_pairs<float>(op, op->bbin[0], Rb, true) : bin_pairs<double>(op, op->bbin[1], Rb, true);
     if (codeb) return fail(codeb);
-    ypart = std::max(ypart, (size_t)op->bbin[pi].n_items * Rb * 2 * kBatch * op->B * rs);
+    const int Sb = kb.slots > 0 ? kb.slots : op->B;
+    if (pi == 0) {
+      op->b32_S = Sb;
+      if (kb.slots > 0) {
+        std::vector<int> tab(op->B + Sb);
+        kb.slot_tables(tab.data());
+        if (hipMalloc(&op->d_b32_tab, tab.size() * sizeof(int)) != hipSuccess ||
+            hipMemcpy(op->d_b32_tab, tab.data(), tab.size() * sizeof(int), hipMemcpyHostToDevice) != hipSuccess) {
+          smh_fail(ctx, SMH_ERR_HIP, "device allocation failed (slot tables)");
+          return fail(SMH_ERR_HIP);
+        }
+      }
+    }
+    ypart = std::max(ypart, (size_t)op->bbin[pi].n_items * Rb * 2 * kBatch * Sb * rs);
   }
   op->ypart_bytes = std::max<size_t>(ypart, 16);
   const int64_t n_pad = n_part * op->B;
@@ -839,12 +948,13 @@ extern "C" int smh_op_matvec(smh_op* op, const double* x, double* y, int64_t nrh
 }
 
 namespace {
+// Synthetic input: n entries into whichever of x32 / x64 is given.
 __global__ void fill_pattern_kernel(float2* x32, double2* x64, int64_t n) {
   const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (j < n) {
     const double a = sin(0.37 * (double)j + 0.1), b = cos(0.11 * (double)j);
-    x32[j] = make_float2((float)a, (float)b);
-    x64[j] = make_double2(a, b);
+    if (x32) x32[j] = make_float2((float)a, (float)b);
+    if (x64) x64[j] = make_double2(a, b);
   }
 }
 }  // namespace
@@ -859,13 +969,13 @@ extern "C" int smh_op_bench_matvec_batch(smh_op* op, int precision, int iters, d
   int rc = op_reserve_batch(op);
   if (rc) return rc;
   hipStream_t s = ctx->stream;
-  const int64_t n = op->n_part * op->B * kBatch;
   // synthetic input in the precision's layout (the other buffer is scratch for the unused copy)
+  const int64_t n = op->n_part * (precision == SMH_FP32 ? op->b32_S : op->B) * kBatch;
   if (precision == SMH_FP32)
     hipLaunchKernelGGL(fill_pattern_kernel, dim3(blocks_for(n)), dim3(256), 0, s, (float2*)op->d_xb,
-                       (double2*)op->d_yb, n);
+                       (double2*)nullptr, n);
   else
-    hipLaunchKernelGGL(fill_pattern_kernel, dim3(blocks_for(n)), dim3(256), 0, s, (float2*)op->d_yb,
+    hipLaunchKernelGGL(fill_pattern_kernel, dim3(blocks_for(n)), dim3(256), 0, s, (float2*)nullptr,
                        (double2*)op->d_xb, n);
   if ((rc = op_apply_batch(op, precision, op->d_xb, op->d_yb, false))) return rc;  // warm-up
   SMH_HIP(ctx, hipStreamSynchronize(s));

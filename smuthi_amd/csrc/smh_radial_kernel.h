@@ -135,6 +135,7 @@ struct RadialKArgs {
   const int* item_en;       // FP32: per item, ceil(log2) of the most pairs one receiver has in it
   int n_items, n_entries, interp, Rc;
   int len_z;                // 2-D lookups: table rows per distance node (tap t sits at (t/2) len_z + t%2)
+  int row_s = 1, row_o = 0;  // entry tap row = row_s * bin + row_o (kernels on per-bin coefficient rows)
 };
 
 // FP32 receiver accumulation in LDS is fixed point: a contribution v is split

@@ -9,9 +9,12 @@
 // SMH_SOLVE_FP64: restarted GMRES with FP64 coupling matvecs, x0 = rhs and
 //   stopping rule ||rhs - M x|| <= tol ||rhs|| (smuthi's scipy call,
 //   linear_system.py:285-286).
-// SMH_SOLVE_MIXED: FP64 iterative refinement: r = rhs - M x with the FP64
-//   matvec, an inner GMRES with FP32 coupling matvecs solves M d = r to a
-//   loose tolerance, x += d, until the FP64 residual meets tol.
+// SMH_SOLVE_MIXED: FP64 iterative refinement from x = 0: an inner GMRES with
+//   FP32 coupling matvecs (FP64 Arnoldi) solves M d = r to
+//   max(tol/2 ||rhs||, kInnerRel ||r||), x += d, r = rhs - M x with the FP64
+//   matvec, until the FP64 residual meets tol. kInnerRel sits at the FP32
+//   operator's resolution, so for tolerances down to ~1e-6 the first sweep
+//   usually suffices and one FP64 matvec checks it.
 #include <chrono>
 #include <cmath>
 #include <complex>
@@ -27,6 +30,7 @@ int blocks_for(int64_t n, int t = 256) { return (int)((n + t - 1) / t); }
 constexpr int kDotThreads = 256;
 constexpr int kDotBlocks = 512;
 constexpr int kMaxBasis = 128;
+constexpr double kInnerRel = 2e-7;  // inner (FP32-matvec) GMRES target relative to the current FP64 residual
 
 // partial[b][j] = sum over this block's slice of conj(V_j) . w
 __global__ void multi_dot_kernel(const double2* __restrict__ V, int64_t ld, int nv, const double2* __restrict__ w,
@@ -354,6 +358,16 @@ __global__ void bmaster_fp64_kernel(const double2* __restrict__ v, const double2
   out[e] = make_double2(v[e].x - (a.x * b.x - a.y * b.y), v[e].y - (a.x * b.y + a.y * b.x));
 }
 
+// FP64 padded [n][kB] -> FP32 input of the batched kernel (b32_index layout)
+__global__ void to_fp32_b32_kernel(const double2* __restrict__ v, float2* out, int64_t n8, int B, int S,
+                                   const int* __restrict__ slot) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n8) return;
+  const int r = (int)(e % kB);
+  const int64_t ip = e / kB, p = ip / B;
+  out[smh::b32_index(p, (int)(ip - p * B), r, S, slot)] = make_float2((float)v[e].x, (float)v[e].y);
+}
+
 __global__ void bmaster_fp32_kernel(const double2* __restrict__ v, const double2* __restrict__ t,
                                     const float2* __restrict__ y, double2* out, int64_t n8) {
   const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -389,7 +403,10 @@ struct BatchSolver {
     SMH_HIP(ctx, hipMalloc(&coef, (size_t)kMaxBasis * kB * sizeof(double2)));
     SMH_HIP(ctx, hipMalloc(&scl, kB * sizeof(double)));
     SMH_HIP(ctx, hipMalloc(&y64, n8 * sizeof(double2)));
-    SMH_HIP(ctx, hipMalloc(&x32, n8 * sizeof(float2)));
+    // FP32 input in the batched kernel's layout (padding slots stay zero)
+    const size_t n32 = (size_t)op->n_part * op->b32_S * kB;
+    SMH_HIP(ctx, hipMalloc(&x32, n32 * sizeof(float2)));
+    SMH_HIP(ctx, hipMemsetAsync(x32, 0, n32 * sizeof(float2), s));
     SMH_HIP(ctx, hipMalloc(&y32, n8 * sizeof(float2)));
     return SMH_OK;
   }
@@ -406,7 +423,8 @@ struct BatchSolver {
       hipLaunchKernelGGL(bmaster_fp64_kernel, dim3(blocks_for(n8)), dim3(256), 0, s, v, t, y64, out, n8);
       ++mv64;
     } else {
-      hipLaunchKernelGGL(to_fp32_kernel, dim3(blocks_for(n8)), dim3(256), 0, s, v, x32, n8);
+      hipLaunchKernelGGL(to_fp32_b32_kernel, dim3(blocks_for(n8)), dim3(256), 0, s, v, x32, n8, op->B, op->b32_S,
+                         (const int*)op->d_b32_tab);
       if ((rc = smh::op_apply_batch(op, SMH_FP32, x32, y32, false))) return rc;
       hipLaunchKernelGGL(bmaster_fp32_kernel, dim3(blocks_for(n8)), dim3(256), 0, s, v, t, y32, out, n8);
       ++mv32;
@@ -616,7 +634,7 @@ static int solve_batched(smh_op* op, const double* t_diag, const double* rhs, do
         double inner_atol[kB];
         for (int r = 0; r < kB; ++r) {
           act[r] = active[r] && res[r] > atol[r];
-          inner_atol[r] = std::max(atol[r] * 0.5, 1e-4 * res[r]);
+          inner_atol[r] = std::max(atol[r] * 0.5, kInnerRel * res[r]);
           any = any || act[r];
         }
         if (!any || budget <= 0) break;
@@ -749,10 +767,9 @@ extern "C" int smh_solve(smh_op* op, const double* t_diag, const double* rhs, do
         }
         if (res <= atol || budget <= 0) break;
         SMH_HIP(ctx, hipMemsetAsync(d_pad, 0, S.n * sizeof(double2), s));
-        // the FP32 operator resolves ~1e-7 relative; ask the inner solve for 1e-4 of this residual
         double inner_res = 0;
         int inner_steps = 0;
-        const double inner_atol = std::max(atol * 0.5, 1e-4 * res);
+        const double inner_atol = std::max(atol * 0.5, kInnerRel * res);
         if ((rc = S.gmres(d_pad, r_pad, inner_atol, budget, SMH_FP32, inner_res, inner_steps, true, true)))
           return done2(rc);
         budget -= inner_steps;

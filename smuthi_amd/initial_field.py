@@ -55,19 +55,23 @@ class PlaneWave:
         return up, dn
 
     def swe_coefficients(self, particle_list, layer_system):
-        """Regular SWE coefficients a of all particles as one system vector."""
-        blocks = []
-        cache = {}
-        for p in particle_list:
-            i = layer_system.layer_number(p.position[2])
-            if i not in cache:
-                cache[i] = self.plane_wave_expansion(layer_system, i)
-            up, dn = cache[i]
-            a = np.zeros(flds.blocksize(p.l_max, p.m_max), complex)
-            for pwe in (up, dn):
-                a += _discrete_pwe_to_swe(pwe, p.l_max, p.m_max, np.asarray(p.position)[None, :])[0]
-            blocks.append(a)
-        return np.concatenate(blocks)
+        """Regular SWE coefficients a of all particles as one system vector
+        (particles grouped by layer and multipole cut-off, one array
+        evaluation per group)."""
+        sizes = [flds.blocksize(p.l_max, p.m_max) for p in particle_list]
+        offsets = np.concatenate([[0], np.cumsum(sizes)])
+        out = np.zeros(offsets[-1], complex)
+        groups = {}
+        for j, p in enumerate(particle_list):
+            key = (layer_system.layer_number(p.position[2]), p.l_max, p.m_max)
+            groups.setdefault(key, []).append(j)
+        for (i, l_max, m_max), idx in groups.items():
+            up, dn = self.plane_wave_expansion(layer_system, i)
+            pos = np.array([particle_list[j].position for j in idx], dtype=float)
+            a = _discrete_pwe_to_swe(up, l_max, m_max, pos) + _discrete_pwe_to_swe(dn, l_max, m_max, pos)
+            rows = offsets[np.asarray(idx)][:, None] + np.arange(a.shape[1])[None, :]
+            out[rows] = a
+        return out
 
     def swe_coefficients_batched(self, positions, l_max, m_max, layer_system):
         """[n_particles, blocksize] coefficients for particles sharing (l_max, m_max) and one layer."""

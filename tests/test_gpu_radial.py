@@ -64,6 +64,29 @@ def test_gpu_table_matches_oracle(gpu):
     assert (err <= 2e-7 * np.abs(want) + 1e-12 * scale).all(), err.max() / scale
 
 
+@pytest.mark.parametrize('order', [(1, 1), (2, 1), (2, 2), (3, 1), (3, 2), (4, 1), (4, 4)])
+def test_gpu_batched_matvec_orders(gpu, order):
+    # every multipole cut-off of the 8-RHS kernels: the slot-order FP32 kernel (blocks up to 32 rows)
+    # and the plain one (l_max = m_max = 4), 9 right-hand sides = one full and one padded block
+    rng = np.random.default_rng(31)
+    pts = []
+    while len(pts) < 16:
+        p = rng.uniform(-1200, 1200, 2)
+        if all(np.hypot(*(p - q)) > 220 for q in pts):
+            pts.append(p)
+    plist = [OSphere([p[0], p[1], 200.0], 1.5 + 0.01j, 100, *order) for p in pts]
+    kp = ob.reasonable_Sommerfeld_kpar_contour(WL, layer_refractive_indices=[1.52, 1], neff_resolution=1e-2)
+    op = _product_op(plist, ([0, 0], [1.52, 1]), kp, 5.0, 'cubic')
+    ref_op = oc.RadialLookupCPU(op.lookup_table, op.radial_distance_array, plist, 'cubic')
+    V = rng.standard_normal((9, ref_op.n_unk)) + 1j * rng.standard_normal((9, ref_op.n_unk))
+    W64 = op.matvec(V, precision='double')
+    W32 = op.matvec(V, precision='single')
+    for r in range(9):
+        want = ref_op.matvec(V[r])
+        assert np.linalg.norm(W64[r] - want) / np.linalg.norm(want) < 1e-12, r
+        assert np.linalg.norm(W32[r] - want) / np.linalg.norm(want) < 3e-6, r
+
+
 @pytest.mark.parametrize('kind,cubic,okind', [('linear', 'spline', 'linear'), ('cubic', 'spline', 'cubic'),
                                               ('cubic', 'catmull-rom', 'catmull-rom')])
 def test_gpu_matvec_matches_oracle_three_spheres(gpu, kind, cubic, okind):

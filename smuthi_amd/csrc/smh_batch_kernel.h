@@ -71,11 +71,12 @@ __global__ void __launch_bounds__((BatchShape<Real, SvwfIdx<LMAX, MMAX>::B, T>::
   extern __shared__ __align__(16) unsigned char smem_raw[];
   AccT* yacc = reinterpret_cast<AccT*>(smem_raw);
   int* next_entry = reinterpret_cast<int*>(yacc + (size_t)a.Rc * SY);
+  // work item of this block (launch order maps blocks to items, see RadialKArgs::order)
+  const int item = a.order ? a.order[blockIdx.x] : (int)blockIdx.x;
   float sh = 0.f, ish = 0.f, sl = 0.f, isl = 0.f;
-  if (kFixed) fixed_scales(a.item_lsum[blockIdx.x], *a.xmax, a.item_en[blockIdx.x], sh, ish, sl, isl);
+  if (kFixed) fixed_scales(a.item_lsum[item], *a.xmax, a.item_en[item], sh, ish, sl, isl);
 
   const int tid = threadIdx.x, lane = tid & 63;
-  const int item = blockIdx.x;
   const int2 it = a.items[item];
   const int e_end = item + 1 < a.n_items ? a.items[item + 1].y : a.n_entries;
   const Rec* recs = reinterpret_cast<const Rec*>(a.recs);

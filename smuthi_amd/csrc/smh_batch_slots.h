@@ -130,12 +130,13 @@ __global__ void __launch_bounds__(512) batch_slots_kernel(RadialKArgs a) {
 
   extern __shared__ __align__(16) int yacc[];
   int* next_entry = yacc + (size_t)a.Rc * SY;
+  // work item of this block (launch order maps blocks to items, see RadialKArgs::order)
+  const int item = a.order ? a.order[blockIdx.x] : (int)blockIdx.x;
   float sh = 0.f, ish = 0.f, sl = 0.f, isl = 0.f;
-  fixed_scales(a.item_lsum[blockIdx.x], *a.xmax, a.item_en[blockIdx.x], sh, ish, sl, isl);
+  fixed_scales(a.item_lsum[item], *a.xmax, a.item_en[item], sh, ish, sl, isl);
   const float slo = sl * ish;  // low-word scale relative to the high word
 
   const int tid = threadIdx.x, lane = tid & 63;
-  const int item = blockIdx.x;
   const int2 it = a.items[item];
   const int e_end = item + 1 < a.n_items ? a.items[item + 1].y : a.n_entries;
   const PairRec<float>* recs = reinterpret_cast<const PairRec<float>*>(a.recs);

@@ -208,11 +208,26 @@ int build_work_items(smh_op* op, smh_op::Binning& bn, const std::vector<int>& cn
   while (cur_chunk < bn.n_chunks) chunk_items[++cur_chunk] = (int)items.size();
   bn.n_entries = (int)entries.size();
   bn.n_items = (int)items.size();
+  // Launch order: items sorted by their first table row; blocks are dealt to
+  // the XCDs round-robin (XCD = block mod kXcds), so XCD x gets the x-th
+  // contiguous share of the sorted list and its CUs run items of neighbouring
+  // bins (from different receiver chunks) at the same time.
+  constexpr int kXcds = 8;
+  const int ni = bn.n_items;
+  std::vector<int> sorted(ni), order(ni);
+  for (int k = 0; k < ni; ++k) sorted[k] = k;
+  std::stable_sort(sorted.begin(), sorted.end(),
+                   [&](int p, int q) { return entries[items[p].y].x < entries[items[q].y].x; });
+  std::vector<int> xoff(kXcds + 1, 0);
+  for (int x = 0; x < kXcds; ++x) xoff[x + 1] = xoff[x] + (ni - x + kXcds - 1) / kXcds;
+  for (int b = 0; b < ni; ++b) order[b] = sorted[xoff[b % kXcds] + b / kXcds];
   if (hipMalloc(&bn.d_entries, std::max<size_t>(entries.size(), 1) * sizeof(int4)) != hipSuccess ||
       hipMalloc(&bn.d_items, std::max<size_t>(items.size(), 1) * sizeof(int2)) != hipSuccess ||
       hipMalloc(&bn.d_chunk_items, chunk_items.size() * sizeof(int)) != hipSuccess ||
       hipMalloc(&bn.d_item_lsum, std::max(bn.n_items, 1) * sizeof(float)) != hipSuccess ||
       hipMalloc(&bn.d_item_en, std::max(bn.n_items, 1) * sizeof(int)) != hipSuccess ||
+      hipMalloc(&bn.d_order, std::max(bn.n_items, 1) * sizeof(int)) != hipSuccess ||
+      hipMemcpyAsync(bn.d_order, order.data(), order.size() * sizeof(int), hipMemcpyHostToDevice, s) != hipSuccess ||
       hipMemcpyAsync(bn.d_entries, entries.data(), entries.size() * sizeof(int4), hipMemcpyHostToDevice, s) !=
           hipSuccess ||
       hipMemcpyAsync(bn.d_items, items.data(), items.size() * sizeof(int2), hipMemcpyHostToDevice, s) != hipSuccess ||

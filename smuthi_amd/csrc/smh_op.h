@@ -47,6 +47,7 @@ struct smh_op {
     int* d_chunk_items = nullptr;    // items of chunk c: [chunk_items[c], chunk_items[c+1])
     float* d_item_lsum = nullptr;    // per item: largest tap-row sum over its bins
     int* d_item_en = nullptr;        // per item: ceil(log2) of the most pairs of one receiver
+    int* d_order = nullptr;          // block -> item launch order (XCD-aware)
     size_t lds = 0;
   } bin[2];
   Binning bbin[2];  // the same pairs binned for the kBatch-RHS kernel (smaller receiver chunks)

@@ -419,6 +419,7 @@ int op_apply(smh_op* op, int precision, const void* x_pad, void* y_pad, bool tim
   a.recs = bn.d_rec;
   a.entries = bn.d_entries;
   a.items = bn.d_items;
+  a.order = bn.d_order;
   a.tt = pi == 0 ? (const void*)op->d_tt32 : (const void*)op->d_tt64;
   a.x = x_pad;
   a.ypart = op->d_ypart;
@@ -569,6 +570,7 @@ int op_apply_batch(smh_op* op, int precision, const void* x_pad, void* y_pad, bo
   a.recs = bn.d_rec;
   a.entries = bn.d_entries;
   a.items = bn.d_items;
+  a.order = bn.d_order;
   const bool slots = pi == 0 && k.slots > 0;
   SMH_CHECK(ctx, !slots || (op->d_tt32b && op->b32_S == k.slots), SMH_ERR_STATE, "slot-order tap table missing");
   a.tt = pi == 0 ? (slots ? (const void*)op->d_tt32b : (const void*)op->d_tt32) : (const void*)op->d_tt64;
@@ -651,7 +653,7 @@ void smh_op_free(smh_op* op) {
   for (void* p : ptrs)
     if (p) hipFree(p);
   auto free_bin = [](smh_op::Binning& b) {
-    void* bp[] = {b.d_rec, b.d_entries, b.d_items, b.d_chunk_items, b.d_item_lsum, b.d_item_en};
+    void* bp[] = {b.d_rec, b.d_entries, b.d_items, b.d_chunk_items, b.d_item_lsum, b.d_item_en, b.d_order};
     for (void* p : bp)
       if (p) hipFree(p);
   };

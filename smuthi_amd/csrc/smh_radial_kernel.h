@@ -136,6 +136,10 @@ struct RadialKArgs {
   int n_items, n_entries, interp, Rc;
   int len_z;                // 2-D lookups: table rows per distance node (tap t sits at (t/2) len_z + t%2)
   int row_s = 1, row_o = 0;  // entry tap row = row_s * bin + row_o (kernels on per-bin coefficient rows)
+  // block -> work item (null: identity). Items sorted by first table row and
+  // dealt so that the blocks one XCD runs (blockIdx mod 8) walk neighbouring
+  // bins together and share the tap rows in that XCD's L2.
+  const int* order = nullptr;
 };
 
 // FP32 receiver accumulation in LDS is fixed point: a contribution v is split
@@ -256,11 +260,12 @@ __global__ void __launch_bounds__((RadialShape<Real, SvwfIdx<LMAX, MMAX>::B, T>:
   using AccT = typename std::conditional<kFixed, int, double>::type;
   AccT* yacc = reinterpret_cast<AccT*>(smem_raw);
   int* next_entry = reinterpret_cast<int*>(yacc + (size_t)a.Rc * SY);
+  // work item of this block (launch order maps blocks to items, see RadialKArgs::order)
+  const int item = a.order ? a.order[blockIdx.x] : (int)blockIdx.x;
   float sh = 0.f, ish = 0.f, sl = 0.f, isl = 0.f;
-  if (kFixed) fixed_scales(a.item_lsum[blockIdx.x], *a.xmax, a.item_en[blockIdx.x], sh, ish, sl, isl);
+  if (kFixed) fixed_scales(a.item_lsum[item], *a.xmax, a.item_en[item], sh, ish, sl, isl);
 
   const int tid = threadIdx.x, lane = tid & 63;
-  const int item = blockIdx.x;
   const int2 it = a.items[item];
   const int e_end = item + 1 < a.n_items ? a.items[item + 1].y : a.n_entries;
   const Rec* recs = reinterpret_cast<const Rec*>(a.recs);

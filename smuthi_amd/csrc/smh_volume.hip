@@ -403,6 +403,7 @@ int vol_apply(smh_op* op, int precision, const void* x_pad, void* y_pad, bool ti
     a.recs = bn.d_rec;
     a.entries = bn.d_entries;
     a.items = bn.d_items;
+    a.order = bn.d_order;
     a.tt = op->d_vtt[t];
     a.x = x_pad;
     a.ypart = op->d_ypart;

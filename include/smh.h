@@ -63,8 +63,11 @@ int smh_radial_create(smh_ctx *ctx, int64_t n_part, const double *xy, const int 
  * xyz: [n_part][3] (all particles in one layer); complex64 tables over
  * (rho, z1+z2) ("plus") and (rho, z1-z2) ("minus") on the grids
  * rho_min + i h (i < len_rho), sz_min + j h and dz_min + j h (j < len_z),
- * interpolated bilinearly (interp must be SMH_INTERP_LINEAR, which is the
- * reference's GPU 'linear' and CPU RectBivariateSpline kx=ky=1). The call
+ * interpolated with interp = SMH_INTERP_LINEAR (bilinear: the reference's GPU
+ * 'linear' and CPU RectBivariateSpline kx=ky=1), SMH_INTERP_SPLINE (tensor
+ * not-a-knot spline: CPU RectBivariateSpline kx=ky=3, linear_system.py:652-663;
+ * needs 7 nodes per axis) or SMH_INTERP_CATMULL_ROM (tensor Catmull-Rom: the
+ * reference's CUDA 'cubic' kernel, linear_system_cuda.py:149-256). The call
  * bins every particle pair by table cell for the matvec. */
 int smh_volume_create(smh_ctx *ctx, int64_t n_part, const double *xyz, const int *lmax, const int *mmax,
                       int l_max, int m_max, double rho_min, double sz_min, double dz_min, double resolution,

@@ -91,17 +91,21 @@ __global__ void tap_rowsum_kernel(const Src* __restrict__ table, const Src* __re
 }
 
 // Per work item: largest row sum over the tap rows its bin entries use (tap t
-// of an entry at row x sits at x + t, or at x + (t/2) len_z + t%2 for 2-D lookups).
+// of an entry at row x sits at x + t for 1-D lookups, vmode 0; 2-D lookups:
+// at x + vol_tap_offset(vmode, s, t, len_z) over all of their z slices s).
 __global__ void item_lsum_kernel(const int4* __restrict__ entries, const int2* __restrict__ items, int n_items,
-                                 int n_entries, int T, int len_z, const float* __restrict__ rowsum, float* item_lsum) {
+                                 int n_entries, int T, int vmode, int len_z, const float* __restrict__ rowsum,
+                                 float* item_lsum) {
   const int k = blockIdx.x;
   const int e0 = items[k].y, e1 = k + 1 < n_items ? items[k + 1].y : n_entries;
+  const int ns = vmode >= 2 ? 4 : 1;
   float m = 0.f;
   for (int e = e0 + (int)threadIdx.x; e < e1; e += blockDim.x)
-    for (int t = 0; t < T; ++t) {
-      const int off = len_z > 0 ? (t >> 1) * len_z + (t & 1) : t;
-      m = fmaxf(m, rowsum[entries[e].x + off]);
-    }
+    for (int sl = 0; sl < ns; ++sl)
+      for (int t = 0; t < T; ++t) {
+        const int off = vmode ? vol_tap_offset(vmode, sl, t, len_z) : t;
+        m = fmaxf(m, rowsum[entries[e].x + off]);
+      }
   __shared__ float red[256];
   red[threadIdx.x] = m;
   __syncthreads();

@@ -395,7 +395,7 @@ int op_make_taps(smh_op* op) {
   for (smh_op::Binning* bn : {&op->bin[0], &op->bbin[0]})
     if (bn->n_items > 0)
       hipLaunchKernelGGL(item_lsum_kernel, dim3(bn->n_items), dim3(256), 0, s, bn->d_entries, bn->d_items,
-                         bn->n_items, bn->n_entries, op->T, 0, op->d_rowsum, bn->d_item_lsum);
+                         bn->n_items, bn->n_entries, op->T, 0, 0, op->d_rowsum, bn->d_item_lsum);
   SMH_HIP(ctx, hipGetLastError());
   SMH_HIP(ctx, hipStreamSynchronize(s));
   if (d_mh) hipFree(d_mh);
@@ -662,7 +662,7 @@ void smh_op_free(smh_op* op) {
   for (auto& row : op->vbin)
     for (auto& b : row) free_bin(b);
   for (int t = 0; t < 2; ++t) {
-    void* vp[] = {op->d_vtab[t], op->d_vtt[t], op->d_vrowsum[t]};
+    void* vp[] = {op->d_vtab[t], op->d_vtt[t], op->d_vtt64[t], op->d_vrowsum[t]};
     for (void* p : vp)
       if (p) hipFree(p);
   }

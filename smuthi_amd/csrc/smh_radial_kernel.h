@@ -136,6 +136,7 @@ struct RadialKArgs {
   int n_items, n_entries, interp, Rc;
   int len_z;                // 2-D lookups: table rows per distance node (tap t sits at (t/2) len_z + t%2)
   int row_s = 1, row_o = 0;  // entry tap row = row_s * bin + row_o (kernels on per-bin coefficient rows)
+  int vmode = 1, vslice = 0;  // volume lookups: interpolation mode and z slice (vol_tap_offset)
   // block -> work item (null: identity). Items sorted by first table row and
   // dealt so that the blocks one XCD runs (blockIdx mod 8) walk neighbouring
   // bins together and share the tap rows in that XCD's L2.
@@ -236,11 +237,34 @@ struct RadialShape {
   static constexpr int SY = sizeof(Real) == 4 ? 4 * B + 1 : 2 * B + 1;
 };
 
+// Volume lookups over nodes (distance r, z index j). vmode 1, bilinear: one
+// launch per table, nodes at row r * len_z + j, taps (r, j), (r, j+1),
+// (r+1, j), (r+1, j+1). vmode 2 (tensor Catmull-Rom) and 3 (tensor not-a-knot
+// spline) run as four launches per table, one per z tap s ("slice"), each a
+// 4-tap interpolation along rho weighted by the slice's z weight:
+//   Catmull-Rom: tap t at node (r + t - 1, j + s - 1), one row per node;
+//   spline: four rows per node (value, rho moment, z moment, mixed moment,
+//   moments Mh = h^2/6 S''), tap t = (node r + t/2, rho kind t%2), slice
+//   s = (node j + s/2, z kind s%2), weights of tap_weight() in each direction.
+// Offsets are relative to the entry's row (node (r, j)).
+__host__ __device__ inline int vol_tap_offset(int vmode, int s, int t, int len_z) {
+  if (vmode == 2) return (t - 1) * len_z + (s - 1);
+  if (vmode == 3) return 4 * ((t >> 1) * len_z + (s >> 1)) + (t & 1) + 2 * (s & 1);
+  return (t >> 1) * len_z + (t & 1);
+}
+template <typename Real>
+__host__ __device__ inline Real vol_tap_weight(int vmode, int s, int t, Real w, Real w2) {
+  if (vmode == 2)
+    return tap_weight<Real>(SMH_INTERP_CATMULL_ROM, t, w) * tap_weight<Real>(SMH_INTERP_CATMULL_ROM, s, w2);
+  if (vmode == 3) return tap_weight<Real>(SMH_INTERP_SPLINE, t, w) * tap_weight<Real>(SMH_INTERP_SPLINE, s, w2);
+  return ((t >> 1) ? w : Real(1) - w) * ((t & 1) ? w2 : Real(1) - w2);
+}
+
 // Lookup matvec. VOL = false: radial lookup, taps are consecutive tap-table
-// rows, weights from the 1-D interpolant. VOL = true: bilinear volume lookup
-// over nodes (distance r, z index j) laid out row r * len_z + j, taps
-// (r, j), (r, j+1), (r+1, j), (r+1, j+1). TapT: storage type of the tap table
-// (float for FP32, and for FP64 over complex64 tables, which float holds exactly).
+// rows, weights from the 1-D interpolant. VOL = true: volume lookup, taps and
+// weights of vol_tap_offset / vol_tap_weight for a.vmode and a.vslice.
+// TapT: storage type of the tap table (float for FP32, and for FP64 over
+// complex64 tables, which float holds exactly; double for FP64 tables).
 template <typename Real, typename TapT, int LMAX, int MMAX, int T, bool VOL>
 __global__ void __launch_bounds__((RadialShape<Real, SvwfIdx<LMAX, MMAX>::B, T>::kThreads))
     lookup_matvec_kernel(RadialKArgs a) {
@@ -284,7 +308,7 @@ __global__ void __launch_bounds__((RadialShape<Real, SvwfIdx<LMAX, MMAX>::B, T>:
     e = __shfl(e, 0);
     if (e >= e_end) break;
     const int4 en = a.entries[e];  // (first tap row, first pair, pairs)
-    const int toff = VOL ? (tap >> 1) * a.len_z + (tap & 1) : tap;
+    const int toff = VOL ? vol_tap_offset(a.vmode, a.vslice, tap, a.len_z) : tap;
     const TapT* trow = ttg + (size_t)(en.x + toff) * FR;
     const int ntiles = (en.z + 15) >> 4;
 
@@ -345,7 +369,7 @@ __global__ void __launch_bounds__((RadialShape<Real, SvwfIdx<LMAX, MMAX>::B, T>:
         const bool valid = tl * 16 + col < en.z;
         Real wq;
         if constexpr (VOL)
-          wq = ((tap >> 1) ? rec.w : Real(1) - rec.w) * ((tap & 1) ? rec.w2 : Real(1) - rec.w2);
+          wq = vol_tap_weight<Real>(a.vmode, a.vslice, tap, rec.w, rec.w2);
         else
           wq = tap_weight<Real>(a.interp, tap, rec.w);
         if (!valid) wq = Real(0);

@@ -186,6 +186,125 @@ __global__ void vol_taps_kernel(const float2* __restrict__ table, int64_t rows, 
   }
 }
 
+// Not-a-knot spline moments Mh = h^2/6 S'' along one axis of a complex array
+// [outer][n][inner] (FP64), one thread per (outer, inner) column: the
+// tridiagonal system of the uniform-grid spline (rows 1 and n-2 closed by the
+// not-a-knot conditions, Thomas algorithm with c' from the host), as the
+// radial lookup's spline_kernel.
+template <typename Src>
+__global__ void spline_axis_kernel(const Src* __restrict__ y, const double* __restrict__ cprime, int64_t n,
+                                   int64_t inner, int64_t outer, double2* mh) {
+  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= inner * outer) return;
+  const int64_t o = c / inner, ci = c - o * inner;
+  const int64_t base = o * n * inner + ci;
+  auto Y = [&](int64_t j) {
+    const Src v = y[base + j * inner];
+    return make_double2((double)v.x, (double)v.y);
+  };
+  auto M = [&](int64_t j) -> double2& { return mh[base + j * inner]; };
+  auto rhs = [&](int64_t j) {
+    const double2 a = Y(j - 1), b = Y(j), d = Y(j + 1);
+    return make_double2(a.x - 2.0 * b.x + d.x, a.y - 2.0 * b.y + d.y);
+  };
+  const double2 r1 = rhs(1), rn2 = rhs(n - 2);
+  const double2 m1 = make_double2(r1.x / 6.0, r1.y / 6.0);
+  const double2 mn2 = make_double2(rn2.x / 6.0, rn2.y / 6.0);
+  double2 dprev = make_double2(0, 0);
+  for (int64_t j = 2; j <= n - 3; ++j) {
+    double2 d = rhs(j);
+    if (j == 2) {
+      d.x -= m1.x;
+      d.y -= m1.y;
+    }
+    if (j == n - 3) {
+      d.x -= mn2.x;
+      d.y -= mn2.y;
+    }
+    const double den = (j == 2) ? 4.0 : 4.0 - cprime[j - 1];
+    const double2 dp = (j == 2) ? make_double2(d.x / 4.0, d.y / 4.0)
+                                : make_double2((d.x - dprev.x) / den, (d.y - dprev.y) / den);
+    M(j) = dp;
+    dprev = dp;
+  }
+  double2 next = M(n - 3);
+  for (int64_t j = n - 4; j >= 2; --j) {
+    const double2 dp = M(j);
+    const double2 v = make_double2(dp.x - cprime[j] * next.x, dp.y - cprime[j] * next.y);
+    M(j) = v;
+    next = v;
+  }
+  const double2 m2 = M(2), mn3 = M(n - 3);
+  M(1) = m1;
+  M(n - 2) = mn2;
+  M(0) = make_double2(2.0 * m1.x - m2.x, 2.0 * m1.y - m2.y);
+  M(n - 1) = make_double2(2.0 * mn2.x - mn3.x, 2.0 * mn2.y - mn3.y);
+}
+
+// Tensor-spline tap tables: row 4 node + q holds quantity q (0 value,
+// 1 rho moment, 2 z moment, 3 mixed moment) of the node in fragment order
+// (as vol_taps_kernel), FP32 and FP64.
+__global__ void vol_taps4_kernel(const float2* __restrict__ table, const double2* __restrict__ mr,
+                                 const double2* __restrict__ mz, const double2* __restrict__ mrz, int64_t nodes, int B,
+                                 int Bh, int NG, int FR, float* tt32, double* tt64) {
+  const int64_t total = nodes * 4 * FR;
+  for (int64_t f = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; f < total; f += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t row = f / FR;
+    int rem = (int)(f - row * FR);
+    const int e = rem & 3, rr = (rem >> 2) & 15;
+    rem >>= 6;
+    const int g = rem % NG, i = rem / NG;
+    const int m = 16 * i + rr, n2 = 4 * g + e;
+    const int64_t node = row >> 2;
+    const int q = (int)(row & 3);
+    double v = 0.0;
+    if (n2 < B && (m < B || (m >= Bh && m < Bh + B))) {
+      const int n1 = m < B ? m : m - Bh;
+      const int64_t o = (node * B + n1) * B + n2;
+      double2 c;
+      if (q == 0) {
+        const float2 t = table[o];
+        c = make_double2((double)t.x, (double)t.y);
+      } else {
+        c = q == 1 ? mr[o] : (q == 2 ? mz[o] : mrz[o]);
+      }
+      v = m < B ? c.x : c.y;
+    }
+    tt32[f] = (float)v;
+    tt64[f] = v;
+  }
+}
+
+// Row sums of a fragment-order tap table: per row max over n1 of
+// sum_n2 |Re A(n1, n2)| + |Im A(n1, n2)| (slightly enlarged), one block per row.
+__global__ void frag_rowsum_kernel(const float* __restrict__ tt, int64_t rows, int B, int Bh, int NG, int FR,
+                                   float* rowsum) {
+  __shared__ float sums[64];
+  __shared__ float red[256];
+  for (int64_t row = blockIdx.x; row < rows; row += gridDim.x) {
+    for (int j = threadIdx.x; j < 64; j += blockDim.x) sums[j] = 0.f;
+    __syncthreads();
+    for (int rem0 = threadIdx.x; rem0 < FR; rem0 += blockDim.x) {
+      int rem = rem0;
+      const int rr = (rem >> 2) & 15;
+      rem >>= 6;
+      const int i = rem / NG;
+      const int m = 16 * i + rr;
+      const int n1 = m < Bh ? m : m - Bh;
+      if (n1 < B) atomicAdd(&sums[n1], fabsf(tt[row * FR + rem0]));
+    }
+    __syncthreads();
+    red[threadIdx.x] = (int)threadIdx.x < B ? sums[threadIdx.x] : 0.f;
+    __syncthreads();
+    for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+      if ((int)threadIdx.x < w) red[threadIdx.x] = fmaxf(red[threadIdx.x], red[threadIdx.x + w]);
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) rowsum[row] = red[0] * 1.001f;
+    __syncthreads();
+  }
+}
+
 // Bilinear cell of a pair in one table: distance node r and z node j of
 // zval = z1 + zsgn * z2, with weights inside the cell.
 __host__ __device__ inline void vol_geometry(const double* a, const double* b, double rho_min, double h, double zmin,
@@ -202,9 +321,12 @@ __host__ __device__ inline void vol_geometry(const double* a, const double* b, d
   wz = t - fl;
 }
 
+// A pair is valid when its interpolation stencil lies inside the table: nodes
+// r - lo .. r + hi - 1 in each direction (lo = 0, hi = 2 bilinear and spline;
+// lo = 1, hi = 3 Catmull-Rom).
 __global__ void vol_pair_bin_kernel(const double* __restrict__ xyz, int n, double rho_min, double h, int len_rho,
-                                    double zmin, double zsgn, int len_z, int Rc, int* cnt, const int* off,
-                                    int* cursor, uint2* pairs, int* err) {
+                                    double zmin, double zsgn, int len_z, int lo, int hi, int Rc, int* cnt,
+                                    const int* off, int* cursor, uint2* pairs, int* err) {
   const int nb2 = (n + 255) / 256;
   const int64_t b = blockIdx.x;
   const int i1 = (int)(b / nb2);
@@ -213,7 +335,7 @@ __global__ void vol_pair_bin_kernel(const double* __restrict__ xyz, int n, doubl
   int r, j;
   double wr, wz, rho, dx, dy;
   vol_geometry(xyz + 3 * (size_t)i1, xyz + 3 * (size_t)i2, rho_min, h, zmin, zsgn, r, j, wr, wz, rho, dx, dy);
-  if (r < 0 || r >= len_rho - 1 || j < 0 || j >= len_z - 1) {
+  if (r < lo || r > len_rho - hi || j < lo || j > len_z - hi) {
     atomicOr(err, 1);
     return;
   }
@@ -249,20 +371,24 @@ __global__ void vol_pair_record_kernel(const uint2* __restrict__ pairs, int64_t 
   rec[p] = out;
 }
 
-template <int L, int Mx, typename Real>
+template <int L, int Mx, typename Real, typename TapT>
 KernelInfo vol_kernel_info() {
   using Sh = RadialShape<Real, SvwfIdx<L, Mx>::B, 4>;
   KernelInfo k;
-  k.fn = reinterpret_cast<void*>(&lookup_matvec_kernel<Real, float, L, Mx, 4, true>);
+  k.fn = reinterpret_cast<void*>(&lookup_matvec_kernel<Real, TapT, L, Mx, 4, true>);
   k.threads = Sh::kThreads;
   k.sy = Sh::SY;
   return k;
 }
 
-KernelInfo volume_kernel(int precision, int l, int m) {
+// tap64: FP64 kernel over the FP64 tap table (spline moments); otherwise the
+// FP32 tap table, which holds the complex64 table values exactly.
+KernelInfo volume_kernel(int precision, int l, int m, bool tap64 = false) {
 #define SMH_V(L, Mx)                                                                                    \
   if (l == L && m == Mx)                                                                                \
-    return precision == SMH_FP32 ? vol_kernel_info<L, Mx, float>() : vol_kernel_info<L, Mx, double>();
+    return precision == SMH_FP32 ? vol_kernel_info<L, Mx, float, float>()                               \
+                                 : (tap64 ? vol_kernel_info<L, Mx, double, double>()                     \
+                                          : vol_kernel_info<L, Mx, double, float>());
   SMH_V(1, 1)
   SMH_V(2, 1)
   SMH_V(2, 2)
@@ -285,6 +411,7 @@ int vol_bin_pairs(smh_op* op, int tab, int pi, int Rc) {
   hipStream_t s = ctx->stream;
   const int64_t n_part = op->n_part, len_rho = op->len_rho, len_z = op->len_z;
   const double zmin = tab == 0 ? op->sz_min : op->dz_min, zsgn = tab == 0 ? 1.0 : -1.0;
+  const int lo = op->vmode == 2 ? 1 : 0, hi = op->vmode == 2 ? 3 : 2;
   bn.Rc = Rc;
   bn.n_chunks = (int)((n_part + Rc - 1) / Rc);
   const int64_t nkeys = (int64_t)bn.n_chunks * len_rho * len_z;
@@ -309,8 +436,8 @@ int vol_bin_pairs(smh_op* op, int tab, int pi, int Rc) {
   const int nb2 = (int)((n_part + 255) / 256);
   const int64_t nblocks = n_part * nb2;
   hipLaunchKernelGGL(vol_pair_bin_kernel, dim3((unsigned)nblocks), dim3(256), 0, s, op->d_xyz, (int)n_part,
-                     op->rho_min, op->h, (int)len_rho, zmin, zsgn, (int)len_z, Rc, d_cnt, (const int*)nullptr, d_cur,
-                     (uint2*)nullptr, d_err);
+                     op->rho_min, op->h, (int)len_rho, zmin, zsgn, (int)len_z, lo, hi, Rc, d_cnt, (const int*)nullptr,
+                     d_cur, (uint2*)nullptr, d_err);
   std::vector<int> cnt(nkeys);
   int herr = 0;
   if (hipMemcpyAsync(cnt.data(), d_cnt, nkeys * sizeof(int), hipMemcpyDeviceToHost, s) != hipSuccess ||
@@ -341,8 +468,8 @@ int vol_bin_pairs(smh_op* op, int tab, int pi, int Rc) {
     return smh_fail(ctx, SMH_ERR_HIP, "device allocation failed (volume pairs)");
   }
   hipLaunchKernelGGL(vol_pair_bin_kernel, dim3((unsigned)nblocks), dim3(256), 0, s, op->d_xyz, (int)n_part,
-                     op->rho_min, op->h, (int)len_rho, zmin, zsgn, (int)len_z, Rc, d_cnt, (const int*)d_off, d_cur,
-                     d_pairs, d_err);
+                     op->rho_min, op->h, (int)len_rho, zmin, zsgn, (int)len_z, lo, hi, Rc, d_cnt, (const int*)d_off,
+                     d_cur, d_pairs, d_err);
   if (acc > 0)
     hipLaunchKernelGGL(vol_pair_record_kernel<Real>, dim3((unsigned)((acc + 255) / 256)), dim3(256), 0, s, d_pairs,
                        acc, op->d_xyz, op->rho_min, op->h, zmin, zsgn, Rc, (PairRec2<Real>*)bn.d_rec);
@@ -352,33 +479,83 @@ int vol_bin_pairs(smh_op* op, int tab, int pi, int Rc) {
   }
   cleanup();
   op->n_pairs = acc;
-  // a (chunk, r, j) key is a bin; its entry starts at table row r * len_z + j
-  return build_work_items<PairRec2<Real>>(op, bn, cnt, off, len_rho * len_z, [](int64_t rj) { return (int)rj; });
+  // a (chunk, r, j) key is a bin; its entry starts at the node's first tap row (r * len_z + j) * vq
+  const int vq = op->vq;
+  return build_work_items<PairRec2<Real>>(op, bn, cnt, off, len_rho * len_z,
+                                          [vq](int64_t rj) { return (int)(rj * vq); });
 }
 
-// FP32 tap tables, row sums and per-item statistics from the complex64 tables.
+// Tap tables, row sums and per-item statistics from the complex64 tables:
+// bilinear / Catmull-Rom one FP32 row per node; spline four rows per node
+// (value and FP64 moments along rho, along z and mixed), FP32 and FP64.
 int vol_make_taps(smh_op* op) {
   smh_ctx* ctx = op->ctx;
   hipStream_t s = ctx->stream;
   const TapGeom g(op->B, 4);
   op->FR = g.FR;
-  const int64_t rows = op->len_rho * op->len_z;
+  const int64_t nodes = op->len_rho * op->len_z, rows = nodes * op->vq;
+  const int64_t BB = (int64_t)op->B * op->B;
+  double2 *mr = nullptr, *mz = nullptr, *mrz = nullptr;
+  double *cpr = nullptr, *cpz = nullptr;
+  auto cleanup = [&]() {
+    void* p[] = {mr, mz, mrz, cpr, cpz};
+    for (void* q : p)
+      if (q) hipFree(q);
+  };
+  if (op->vq == 4) {
+    auto cprime = [](int64_t n) {
+      std::vector<double> cp(n, 0.0);
+      for (int64_t j = 2; j <= n - 3; ++j) cp[j] = (j == 2) ? 0.25 : 1.0 / (4.0 - cp[j - 1]);
+      return cp;
+    };
+    const std::vector<double> hr = cprime(op->len_rho), hz = cprime(op->len_z);
+    if (hipMalloc(&mr, nodes * BB * sizeof(double2)) != hipSuccess ||
+        hipMalloc(&mz, nodes * BB * sizeof(double2)) != hipSuccess ||
+        hipMalloc(&mrz, nodes * BB * sizeof(double2)) != hipSuccess ||
+        hipMalloc(&cpr, hr.size() * sizeof(double)) != hipSuccess ||
+        hipMalloc(&cpz, hz.size() * sizeof(double)) != hipSuccess ||
+        hipMemcpyAsync(cpr, hr.data(), hr.size() * sizeof(double), hipMemcpyHostToDevice, s) != hipSuccess ||
+        hipMemcpyAsync(cpz, hz.data(), hz.size() * sizeof(double), hipMemcpyHostToDevice, s) != hipSuccess) {
+      cleanup();
+      return smh_fail(ctx, SMH_ERR_HIP, "device allocation failed (spline moments)");
+    }
+  }
   for (int t = 0; t < 2; ++t) {
     if (!op->d_vtt[t]) {
-      SMH_HIP(ctx, hipMalloc(&op->d_vtt[t], (size_t)rows * g.FR * sizeof(float)));
-      SMH_HIP(ctx, hipMalloc(&op->d_vrowsum[t], (size_t)rows * sizeof(float)));
+      if (hipMalloc(&op->d_vtt[t], (size_t)rows * g.FR * sizeof(float)) != hipSuccess ||
+          hipMalloc(&op->d_vrowsum[t], (size_t)rows * sizeof(float)) != hipSuccess ||
+          (op->vq == 4 && hipMalloc(&op->d_vtt64[t], (size_t)rows * g.FR * sizeof(double)) != hipSuccess)) {
+        cleanup();
+        return smh_fail(ctx, SMH_ERR_HIP, "device allocation failed (volume tap tables)");
+      }
     }
-    hipLaunchKernelGGL(vol_taps_kernel, dim3(4096), dim3(256), 0, s, op->d_vtab[t], rows, op->B, op->Bh, g.NG, g.FR,
-                       op->d_vtt[t]);
-    hipLaunchKernelGGL(tap_rowsum_kernel<float2>, dim3(blocks_for(rows)), dim3(256), 0, s, op->d_vtab[t],
-                       (const float2*)nullptr, rows, op->B, 0, op->d_vrowsum[t]);
+    if (op->vq == 1) {
+      hipLaunchKernelGGL(vol_taps_kernel, dim3(4096), dim3(256), 0, s, op->d_vtab[t], rows, op->B, op->Bh, g.NG,
+                         g.FR, op->d_vtt[t]);
+    } else {
+      const int64_t colr = op->len_z * BB, colz = op->len_rho * BB;
+      hipLaunchKernelGGL(spline_axis_kernel<float2>, dim3(blocks_for(colr)), dim3(256), 0, s, op->d_vtab[t], cpr,
+                         op->len_rho, colr, (int64_t)1, mr);
+      hipLaunchKernelGGL(spline_axis_kernel<float2>, dim3(blocks_for(colz)), dim3(256), 0, s, op->d_vtab[t], cpz,
+                         op->len_z, BB, op->len_rho, mz);
+      hipLaunchKernelGGL(spline_axis_kernel<double2>, dim3(blocks_for(colz)), dim3(256), 0, s, (const double2*)mr,
+                         cpz, op->len_z, BB, op->len_rho, mrz);
+      hipLaunchKernelGGL(vol_taps4_kernel, dim3(4096), dim3(256), 0, s, op->d_vtab[t], mr, mz, mrz, nodes, op->B,
+                         op->Bh, g.NG, g.FR, op->d_vtt[t], op->d_vtt64[t]);
+    }
+    hipLaunchKernelGGL(frag_rowsum_kernel, dim3((unsigned)std::min<int64_t>(rows, 65536)), dim3(256), 0, s,
+                       op->d_vtt[t], rows, op->B, op->Bh, g.NG, g.FR, op->d_vrowsum[t]);
     smh_op::Binning& bn = op->vbin[t][0];
     if (bn.n_items > 0)
       hipLaunchKernelGGL(item_lsum_kernel, dim3(bn.n_items), dim3(256), 0, s, bn.d_entries, bn.d_items, bn.n_items,
-                         bn.n_entries, 4, (int)op->len_z, op->d_vrowsum[t], bn.d_item_lsum);
+                         bn.n_entries, 4, op->vmode, (int)op->len_z, op->d_vrowsum[t], bn.d_item_lsum);
+    // the moments of this table must be consumed before the next one overwrites them
+    if (hipStreamSynchronize(s) != hipSuccess) {
+      cleanup();
+      return smh_fail(ctx, SMH_ERR_HIP, "volume tap tables failed: %s", hipGetErrorString(hipGetLastError()));
+    }
   }
-  SMH_HIP(ctx, hipGetLastError());
-  SMH_HIP(ctx, hipStreamSynchronize(s));
+  cleanup();
   op->has_table = true;
   return SMH_OK;
 }
@@ -390,52 +567,60 @@ namespace smh {
 int vol_apply(smh_op* op, int precision, const void* x_pad, void* y_pad, bool time_kernel) {
   smh_ctx* ctx = op->ctx;
   const int pi = precision == SMH_FP32 ? 0 : 1;
-  const KernelInfo k = volume_kernel(precision, op->l_max, op->m_max);
+  const bool tap64 = pi == 1 && op->vq == 4;
+  const KernelInfo k = volume_kernel(precision, op->l_max, op->m_max, tap64);
   SMH_CHECK(ctx, k.fn != nullptr && op->vbin[0][pi].Rc > 0, SMH_ERR_UNSUPPORTED,
             "no volume matvec kernel for l_max %d m_max %d", op->l_max, op->m_max);
   hipStream_t s = ctx->stream;
   if (pi == 0)
     hipLaunchKernelGGL(xmax_kernel, dim3(1), dim3(1024), 0, s, (const float2*)x_pad, op->n_part * op->B, op->d_xmax);
   double ms_total = 0;
+  const int n_slices = op->vmode >= 2 ? 4 : 1;
   for (int t = 0; t < 2; ++t) {
     const smh_op::Binning& bn = op->vbin[t][pi];
-    RadialKArgs a;
-    a.recs = bn.d_rec;
-    a.entries = bn.d_entries;
-    a.items = bn.d_items;
-    a.order = bn.d_order;
-    a.tt = op->d_vtt[t];
-    a.x = x_pad;
-    a.ypart = op->d_ypart;
-    a.xmax = op->d_xmax;
-    a.item_lsum = bn.d_item_lsum;
-    a.item_en = bn.d_item_en;
-    a.n_items = bn.n_items;
-    a.n_entries = bn.n_entries;
-    a.interp = SMH_INTERP_LINEAR;
-    a.Rc = bn.Rc;
-    a.len_z = (int)op->len_z;
-    SMH_HIP(ctx, hipFuncSetAttribute(k.fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bn.lds));
-    if (bn.n_items > 0) {
-      if (time_kernel) SMH_HIP(ctx, hipEventRecord(ctx->ev0, s));
-      void* args[] = {&a};
-      SMH_HIP(ctx, hipLaunchKernel(k.fn, dim3(bn.n_items), dim3(k.threads), args, bn.lds, s));
-      if (time_kernel) {
-        SMH_HIP(ctx, hipEventRecord(ctx->ev1, s));
-        SMH_HIP(ctx, hipEventSynchronize(ctx->ev1));
-        float ms = 0;
-        SMH_HIP(ctx, hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
-        ms_total += ms;
+    for (int sl = 0; sl < n_slices; ++sl) {
+      RadialKArgs a;
+      a.recs = bn.d_rec;
+      a.entries = bn.d_entries;
+      a.items = bn.d_items;
+      a.order = bn.d_order;
+      a.tt = tap64 ? (const void*)op->d_vtt64[t] : (const void*)op->d_vtt[t];
+      a.x = x_pad;
+      a.ypart = op->d_ypart;
+      a.xmax = op->d_xmax;
+      a.item_lsum = bn.d_item_lsum;
+      a.item_en = bn.d_item_en;
+      a.n_items = bn.n_items;
+      a.n_entries = bn.n_entries;
+      a.interp = op->interp;
+      a.Rc = bn.Rc;
+      a.len_z = (int)op->len_z;
+      a.vmode = op->vmode;
+      a.vslice = sl;
+      SMH_HIP(ctx, hipFuncSetAttribute(k.fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bn.lds));
+      if (bn.n_items > 0) {
+        if (time_kernel) SMH_HIP(ctx, hipEventRecord(ctx->ev0, s));
+        void* args[] = {&a};
+        SMH_HIP(ctx, hipLaunchKernel(k.fn, dim3(bn.n_items), dim3(k.threads), args, bn.lds, s));
+        if (time_kernel) {
+          SMH_HIP(ctx, hipEventRecord(ctx->ev1, s));
+          SMH_HIP(ctx, hipEventSynchronize(ctx->ev1));
+          float ms = 0;
+          SMH_HIP(ctx, hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+          ms_total += ms;
+        }
       }
+      // the first launch writes y, the others add to it
+      const int accumulate = (t > 0 || sl > 0) ? 1 : 0;
+      const int64_t n = op->n_part * op->B;
+      if (pi == 0)
+        hipLaunchKernelGGL(reduce_kernel<float>, dim3(blocks_for(n)), dim3(256), 0, s, (const float*)op->d_ypart,
+                           bn.d_chunk_items, bn.Rc, op->B, op->n_part, (float2*)y_pad, accumulate);
+      else
+        hipLaunchKernelGGL(reduce_kernel<double>, dim3(blocks_for(n)), dim3(256), 0, s, (const double*)op->d_ypart,
+                           bn.d_chunk_items, bn.Rc, op->B, op->n_part, (double2*)y_pad, accumulate);
+      SMH_HIP(ctx, hipGetLastError());
     }
-    const int64_t n = op->n_part * op->B;
-    if (pi == 0)
-      hipLaunchKernelGGL(reduce_kernel<float>, dim3(blocks_for(n)), dim3(256), 0, s, (const float*)op->d_ypart,
-                         bn.d_chunk_items, bn.Rc, op->B, op->n_part, (float2*)y_pad, t);
-    else
-      hipLaunchKernelGGL(reduce_kernel<double>, dim3(blocks_for(n)), dim3(256), 0, s, (const double*)op->d_ypart,
-                         bn.d_chunk_items, bn.Rc, op->B, op->n_part, (double2*)y_pad, t);
-    SMH_HIP(ctx, hipGetLastError());
   }
   if (time_kernel) op->last_kernel_ms = (float)ms_total;
   return SMH_OK;
@@ -484,8 +669,12 @@ extern "C" int smh_volume_create(smh_ctx* ctx, int64_t n_part, const double* xyz
             "invalid l_max %d / m_max %d", l_max, m_max);
   SMH_CHECK(ctx, resolution > 0 && len_rho >= 2 && len_z >= 2 && len_rho * len_z < (1 << 30), SMH_ERR_ARG,
             "invalid table grid");
-  SMH_CHECK(ctx, interp == SMH_INTERP_LINEAR, SMH_ERR_UNSUPPORTED,
-            "the volume lookup interpolates bilinearly (interp must be SMH_INTERP_LINEAR)");
+  SMH_CHECK(ctx, interp == SMH_INTERP_LINEAR || interp == SMH_INTERP_CATMULL_ROM || interp == SMH_INTERP_SPLINE,
+            SMH_ERR_ARG, "invalid interpolation %d", interp);
+  SMH_CHECK(ctx, interp != SMH_INTERP_SPLINE || (len_rho >= 7 && len_z >= 7 && len_rho * len_z < (1 << 28)),
+            SMH_ERR_ARG, "spline interpolation needs at least 7 nodes per axis (and < 2^28 nodes)");
+  SMH_CHECK(ctx, interp != SMH_INTERP_CATMULL_ROM || (len_rho >= 4 && len_z >= 4), SMH_ERR_ARG,
+            "Catmull-Rom interpolation needs at least 4 nodes per axis");
   SMH_CHECK(ctx, volume_kernel(SMH_FP32, l_max, m_max).fn != nullptr, SMH_ERR_UNSUPPORTED,
             "no kernel for l_max %d m_max %d", l_max, m_max);
   SMH_HIP(ctx, hipSetDevice(ctx->device));
@@ -500,6 +689,8 @@ extern "C" int smh_volume_create(smh_ctx* ctx, int64_t n_part, const double* xyz
   op->MP = 2 * op->Bh;
   op->interp = interp;
   op->T = 4;
+  op->vmode = interp == SMH_INTERP_LINEAR ? 1 : (interp == SMH_INTERP_CATMULL_ROM ? 2 : 3);
+  op->vq = interp == SMH_INTERP_SPLINE ? 4 : 1;
   op->rho_min = rho_min;
   op->sz_min = sz_min;
   op->dz_min = dz_min;

@@ -327,7 +327,8 @@ class LinearSystem:
                                                                   cubic_interpolant=self.cubic_interpolant, **kw)
         else:
             from .volume import CouplingMatrixVolumeLookupCUDA
-            self.coupling_matrix = CouplingMatrixVolumeLookupCUDA(cuda_blocksize=self.cuda_blocksize, **kw)
+            self.coupling_matrix = CouplingMatrixVolumeLookupCUDA(cuda_blocksize=self.cuda_blocksize,
+                                                                  cubic_interpolant=self.cubic_interpolant, **kw)
 
     def solve(self, rhs=None):
         """Solve for the scattered-field coefficients; rhs may hold several right-hand sides [nrhs, n]."""

@@ -2,7 +2,10 @@
 CouplingMatrixVolumeLookupCUDA (linearsystem/linear_system.py:688-783), for
 particles at different heights inside one layer. Both tables (over
 (rho, z1+z2) and (rho, z1-z2), prepare_lookup.py:16-250) are built on the GPU
-and interpolated bilinearly by the binned MFMA matvec of libsmh."""
+and interpolated by the binned MFMA matvec of libsmh: bilinearly, or bicubic
+as a tensor not-a-knot spline (smuthi's CPU RectBivariateSpline(kx=ky=3),
+linear_system.py:652-663) or tensor Catmull-Rom (smuthi's CUDA kernel,
+linear_system_cuda.py:149-256)."""
 import ctypes
 import time
 
@@ -15,16 +18,31 @@ from . import lookup as look
 from .linear_system import _GpuCouplingOperator
 
 
+def _interp_code(interpolator_kind, cubic_interpolant):
+    if interpolator_kind == 'linear':
+        return _lib.INTERP_LINEAR
+    if interpolator_kind != 'cubic':
+        raise ValueError("interpolator_kind must be 'linear' or 'cubic'")
+    if cubic_interpolant == 'spline':
+        return _lib.INTERP_SPLINE
+    if cubic_interpolant == 'catmull-rom':
+        return _lib.INTERP_CATMULL_ROM
+    raise ValueError("cubic_interpolant must be 'spline' or 'catmull-rom'")
+
+
 class CouplingMatrixVolumeLookupCUDA(_GpuCouplingOperator):
     """Attributes as smuthi's class: lookup_table_plus / lookup_table_minus
     [len_rho, len_z, B, B] (complex64 values), rho_array, sum_z_array,
-    diff_z_array, linear_operator. interpolator_kind must be 'linear'
-    (bilinear, the reference's GPU 'linear' and CPU kx=ky=1 spline)."""
+    diff_z_array, linear_operator. interpolator_kind 'linear' (bilinear, the
+    reference's GPU 'linear' and CPU kx=ky=1 spline) or 'cubic'; extra keyword
+    cubic_interpolant: 'spline' (default, tensor not-a-knot spline = smuthi
+    CPU RectBivariateSpline(kx=ky=3)) or 'catmull-rom' (smuthi's CUDA
+    kernel)."""
 
     def __init__(self, vacuum_wavelength, particle_list, layer_system, k_parallel='default', resolution=None,
-                 cuda_blocksize=None, interpolator_kind='linear'):
-        if interpolator_kind != 'linear':
-            raise NotImplementedError("the GPU volume lookup interpolates bilinearly: interpolator_kind='linear'")
+                 cuda_blocksize=None, interpolator_kind='linear', cubic_interpolant='spline'):
+        self._interp = _interp_code(interpolator_kind, cubic_interpolant)
+        self.cubic_interpolant = cubic_interpolant
         layers = {layer_system.layer_number(p.position[2]) for p in particle_list}
         if len(layers) != 1:
             raise ValueError('volume lookup needs all particles in one layer')
@@ -63,7 +81,7 @@ class CouplingMatrixVolumeLookupCUDA(_GpuCouplingOperator):
                                                  _lib.iptr(self._mmax), self.l_max, self.m_max,
                                                  float(self.rho_array[0]), float(self.sum_z_array[0]),
                                                  float(self.diff_z_array[0]), h, len(self.rho_array),
-                                                 len(self.sum_z_array), _lib.INTERP_LINEAR, ctypes.byref(handle)),
+                                                 len(self.sum_z_array), self._interp, ctypes.byref(handle)),
                    cu.context(), 'smh_volume_create')
         self._op = handle
 
@@ -84,12 +102,15 @@ class CouplingMatrixVolumeLookupCUDA(_GpuCouplingOperator):
         return self._tables()[1]
 
     @classmethod
-    def from_tables(cls, table_plus, table_minus, rho_array, sum_z_array, diff_z_array, particle_list, resolution):
+    def from_tables(cls, table_plus, table_minus, rho_array, sum_z_array, diff_z_array, particle_list, resolution,
+                    interpolator_kind='linear', cubic_interpolant='spline'):
         """Operator over given host tables (e.g. smuthi's own)."""
         self = cls.__new__(cls)
+        self._interp = _interp_code(interpolator_kind, cubic_interpolant)
+        self.cubic_interpolant = cubic_interpolant
         self._init_common(particle_list)
         self.resolution = resolution
-        self.interpolator_kind = 'linear'
+        self.interpolator_kind = interpolator_kind
         self.rho_array, self.sum_z_array, self.diff_z_array = (np.asarray(a) for a in (rho_array, sum_z_array,
                                                                                     diff_z_array))
         xyz = np.ascontiguousarray([p.position for p in particle_list], dtype=np.float64)

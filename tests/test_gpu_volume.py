@@ -90,3 +90,34 @@ def test_volume_solve_matches_oracle_lu(gpu):
     for mode in ('double', 'mixed'):
         x = op.solve_master(t, rhs, tol=1e-9, precision=mode)
         assert np.linalg.norm(x - x_ref) / np.linalg.norm(x_ref) < 1e-7, mode
+
+
+@pytest.mark.parametrize('cubic,okind', [('spline', 'cubic'), ('catmull-rom', 'catmull-rom')])
+def test_volume_bicubic_matvec_matches_oracle(gpu, cubic, okind):
+    # bicubic volume lookup: tensor not-a-knot spline (= RectBivariateSpline kx=ky=3,
+    # linear_system.py:652-663) and tensor Catmull-Rom (linear_system_cuda.py:149-256)
+    from smuthi_amd.layers import LayerSystem
+    from smuthi_amd.volume import CouplingMatrixVolumeLookupCUDA
+    plist = _film_particles(10, 4)
+    plist[2] = OSphere(plist[2].position, 1.7, 70, 2, 1)  # mixed multipole cut-offs
+    kp = _contour()
+    tpl, tmn, rho, sz, dz = oc.volumetric_coupling_lookup_table(WL, plist, ol.LayerSystem(THICK, NIDX), kp, 10.0)
+    ref = oc.VolumeLookupCPU(tpl, tmn, rho, sz, dz, plist, okind)
+    rng = np.random.default_rng(9)
+    v = rng.standard_normal(ref.n_unk) + 1j * rng.standard_normal(ref.n_unk)
+    want = ref.matvec(v)
+    up = CouplingMatrixVolumeLookupCUDA.from_tables(tpl, tmn, rho, sz, dz, plist, 10.0, interpolator_kind='cubic',
+                                                    cubic_interpolant=cubic)
+    built = CouplingMatrixVolumeLookupCUDA(WL, plist, LayerSystem(THICK, NIDX), k_parallel=kp, resolution=10.0,
+                                           interpolator_kind='cubic', cubic_interpolant=cubic)
+    for op in (up, built):
+        e64 = np.linalg.norm(op.matvec(v, 'double') - want) / np.linalg.norm(want)
+        e32 = np.linalg.norm(op.matvec(v, 'single') - want) / np.linalg.norm(want)
+        assert e64 < (1e-11 if op is up else 2e-6), e64
+        assert e32 < 5e-6, e32
+    # the mixed-precision solve on the bicubic operator against the oracle's LU
+    W = ref.dense()
+    t = 0.2 + 0.1j + 0.05 * (rng.standard_normal(W.shape[0]) + 1j * rng.standard_normal(W.shape[0]))
+    x_ref = np.linalg.solve(np.eye(W.shape[0]) - t[:, None] * W, v)
+    x = up.solve_master(t, v, tol=1e-9, precision='mixed')
+    assert np.linalg.norm(x - x_ref) / np.linalg.norm(x_ref) < 1e-7

@@ -179,3 +179,52 @@ def test_lookup_operator_vs_explicit_tolerance_contract():
     for kind, tol in (('linear', 1e-2), ('cubic', 5e-4), ('catmull-rom', 5e-4)):
         got = cp.RadialLookupCPU(table, rho, plist, kind).matvec(v)
         assert np.linalg.norm(got - ref) / np.linalg.norm(ref) < tol, kind
+
+
+def test_volume_interpolants_closed_forms():
+    # VolumeLookupCPU (linear_system.py:593-685; tensor Catmull-Rom of linear_system_cuda.py:149-256):
+    # bilinear is exact on bilinear data, tensor Catmull-Rom on tensor-quadratic data and the
+    # RectBivariateSpline(kx=ky=3) path on tensor-cubic data; Lpl and Lmn add
+    rho = np.arange(-15.0, 300.0, 5.0)
+    sz = np.arange(80.0, 200.0, 5.0)
+    dz = np.arange(-60.0, 60.0, 5.0)
+    rng = np.random.default_rng(3)
+    plist = [Sphere([0, 0, 100], 1.5, 50, 1, 1), Sphere([200, 30, 90], 1.5, 50, 1, 1)]
+    r = rng.uniform(10, 280, 40)
+    s = rng.uniform(sz[2], sz[-3], 40)
+    d = rng.uniform(dz[2], dz[-3], 40)
+    for deg, kinds in ((1, ('linear', 'cubic', 'catmull-rom')), (2, ('cubic', 'catmull-rom')), (3, ('cubic',))):
+        cp_ = rng.standard_normal((deg + 1, deg + 1, 2, 2)) + 1j * rng.standard_normal((deg + 1, deg + 1, 2, 2))
+        cm_ = rng.standard_normal((deg + 1, deg + 1, 2, 2)) + 1j * rng.standard_normal((deg + 1, deg + 1, 2, 2))
+
+        def poly(c, x, y):
+            x, y = np.asarray(x, float), np.asarray(y, float)
+            return sum(c[a, b] * (x[..., None, None] / 100) ** a * (y[..., None, None] / 100) ** b
+                       for a in range(deg + 1) for b in range(deg + 1))
+
+        tpl = poly(cp_, rho[:, None], sz[None, :])
+        tmn = poly(cm_, rho[:, None], dz[None, :])
+        want = poly(cp_, r, s) + poly(cm_, r, d)
+        for kind in kinds:
+            got = cp.VolumeLookupCPU(tpl, tmn, rho, sz, dz, plist, kind).entries(r, s, d)
+            assert np.abs(got - want).max() / np.abs(want).max() < 1e-9, (deg, kind)
+
+
+def test_volume_spline_is_tensor_not_a_knot():
+    # the 'cubic' volume path (RectBivariateSpline kx=ky=3, s=0) equals not-a-knot spline interpolation
+    # along rho followed by one along z (scipy make_interp_spline, k=3) - the form the GPU evaluates
+    import scipy.interpolate as si
+    rho = np.arange(-15.0, 120.0, 5.0)
+    sz = np.arange(80.0, 160.0, 5.0)
+    dz = np.arange(-40.0, 40.0, 5.0)
+    rng = np.random.default_rng(4)
+    tpl = rng.standard_normal((len(rho), len(sz), 1, 1)) + 1j * rng.standard_normal((len(rho), len(sz), 1, 1))
+    tmn = np.zeros((len(rho), len(dz), 1, 1), complex)
+    plist = [Sphere([0, 0, 100], 1.5, 50, 1, 1)]
+    op = cp.VolumeLookupCPU(tpl, tmn, rho, sz, dz, plist, 'cubic')
+    r = rng.uniform(rho[0], rho[-1], 30)
+    s = rng.uniform(sz[0], sz[-1], 30)
+    got = op.entries(r, s, np.zeros(30))[:, 0, 0]
+    cols = np.array([si.make_interp_spline(rho, tpl[:, j, 0, 0], k=3)(r) for j in range(len(sz))])
+    want = np.array([si.make_interp_spline(sz, cols[:, i], k=3)(s[i]) for i in range(30)])
+    assert np.abs(got - want).max() / np.abs(want).max() < 1e-12

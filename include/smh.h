@@ -157,6 +157,22 @@ int smh_pwe_assemble(smh_ctx *ctx, int64_t n_part, const double *xyz, int l_max,
                      double z_ref, int64_t n_k, const double *kp, const double *kz, const double *btab, int64_t n_a,
                      const double *alpha, double *g, int precision, double *kernel_ms);
 
+/* ---- spherical wave expansion fields ----------------------------------
+ * Direct field of the particles' outgoing spherical wave expansions, summed
+ * at points: replaces SphericalWaveExpansion.electric_field / magnetic_field
+ * (fields/expansions.py:348-424, SVWFs of fields/vector_wave_functions.py:64-155)
+ * over the scattered-field SWEs that PiecewiseFieldExpansion adds
+ * (postprocessing/scattered_field.py:72-75). pos: [n_part][3]; radius:
+ * [n_part]; zlim: [n_part][2] (z range of the particle's layer, +-inf
+ * allowed); coef: [n_part][blocksize(l_max, m_max)] complex in (tau, l, m)
+ * order (zeros for multipoles a particle does not have). A particle's SWE
+ * counts at points with zlim[0] <= z < zlim[1] strictly outside its sphere.
+ * field 0: E = sum b N; 1: H = -i k / omega sum b_(1-tau) N_tau. out:
+ * [3][n_pts] complex. kernel_ms (may be NULL): device time. */
+int smh_swe_field(smh_ctx *ctx, int64_t n_part, const double *pos, const double *radius, const double *zlim,
+                  int l_max, int m_max, const double *coef, double k_re, double k_im, int64_t n_pts,
+                  const double *pts, int field, double omega, int precision, double *out, double *kernel_ms);
+
 /* ---- measurement helper (no counterpart in smuthi) ----------------------
  * Copy `bytes` (a multiple of 16) between two fresh device buffers with one
  * plain streaming kernel and return its time; used to calibrate the HBM

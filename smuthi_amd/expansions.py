@@ -73,6 +73,108 @@ class SphericalWaveExpansion(FieldExpansion):
     def coefficients_tlm(self, tau, l, m):
         return self.coefficients[flds.multi_to_single_index(tau, l, m, self.l_max, self.m_max)]
 
+    def valid(self, x, y, z):
+        z = np.asarray(z)
+        vld = np.logical_and(z >= self.lower_z, z < self.upper_z)
+        return np.logical_and(vld, FieldExpansion.valid(self, x, y, z))
+
+    def _field(self, field, x, y, z, omega, precision):
+        if self.kind != 'outgoing':
+            raise NotImplementedError('the GPU evaluates outgoing spherical wave expansions')
+        x, y, z = np.array(x, ndmin=1), np.array(y, ndmin=1), np.array(z, ndmin=1)
+        out = [np.zeros(x.shape, dtype=complex) for _ in range(3)]
+        vld = self.valid(x, y, z)
+        if vld.any():
+            pts = np.stack([x[vld], y[vld], z[vld]], axis=1)
+            res = swe_field_sum(np.asarray([self.reference_point], dtype=float), np.zeros(1),
+                                np.array([[-np.inf, np.inf]]), self.coefficients[None, :], self.l_max, self.m_max,
+                                self.k, pts, field, omega, precision)
+            for c in range(3):
+                out[c][vld] = res[c]
+        return tuple(out)
+
+    def electric_field(self, x, y, z, precision='single'):
+        """E at points (fields/expansions.py:348-381), zero where not valid."""
+        return self._field(0, x, y, z, 1.0, precision)
+
+    def magnetic_field(self, x, y, z, vacuum_wavelength, precision='single'):
+        """H at points (fields/expansions.py:383-424), zero where not valid."""
+        return self._field(1, x, y, z, flds.angular_frequency(vacuum_wavelength), precision)
+
+
+def swe_field_sum(positions, radii, zlim, coefficients, l_max, m_max, k, points, field=0, omega=1.0,
+                  precision='single'):
+    """Sum of outgoing SWEs at points on the GPU (libsmh smh_swe_field):
+    positions [n][3], radii [n], zlim [n][2], coefficients [n][blocksize(l_max,
+    m_max)]; particle p counts at points with zlim[p, 0] <= z < zlim[p, 1]
+    outside its sphere. field 0: E, 1: H. Returns (Fx, Fy, Fz)."""
+    pos = np.ascontiguousarray(positions, dtype=np.float64)
+    rad = np.ascontiguousarray(radii, dtype=np.float64)
+    zl = np.ascontiguousarray(zlim, dtype=np.float64)
+    cf = np.ascontiguousarray(coefficients, dtype=np.complex128)
+    pts = np.ascontiguousarray(points, dtype=np.float64)
+    n_pts = pts.shape[0]
+    out = np.empty((3, n_pts), dtype=np.complex128)
+    ms = ctypes.c_double()
+    prec = _lib.FP32 if precision in ('single', 'fp32') else _lib.FP64
+    kk = complex(k)
+    _lib.check(_lib.load().smh_swe_field(cu.context(), len(pos), _lib.dptr(pos), _lib.dptr(rad), _lib.dptr(zl),
+                                         int(l_max), int(m_max), _lib.dptr(cf), kk.real, kk.imag, n_pts,
+                                         _lib.dptr(pts), int(field), float(omega), prec, _lib.dptr(out),
+                                         ctypes.byref(ms)), cu.context(), 'smh_swe_field')
+    swe_field_sum.last_kernel_ms = ms.value
+    return out[0], out[1], out[2]
+
+
+class SphericalWaveSum(FieldExpansion):
+    """The outgoing scattered-field SWEs of the particles of one layer as one
+    expansion: the direct field that PiecewiseFieldExpansion adds particle by
+    particle (postprocessing/scattered_field.py:72-75), summed on the GPU.
+    Each SWE counts in [lower_z, upper_z) outside its own sphere."""
+
+    def __init__(self, k, particles, lower_z=-np.inf, upper_z=np.inf):
+        FieldExpansion.__init__(self)
+        self.k = k
+        self.lower_z, self.upper_z = lower_z, upper_z
+        # a particle's scattered field: a SphericalWaveExpansion, or its coefficient array
+        orders = [(getattr(p.scattered_field, 'l_max', p.l_max), getattr(p.scattered_field, 'm_max', p.m_max))
+                  for p in particles]
+        self.l_max = max(o[0] for o in orders)
+        self.m_max = max(o[1] for o in orders)
+        B = flds.blocksize(self.l_max, self.m_max)
+        self.positions = np.array([p.position for p in particles], dtype=float)
+        self.radii = np.array([p.radius for p in particles], dtype=float)
+        self.coefficients = np.zeros((len(particles), B), dtype=complex)
+        for i, (p, (lp, mp)) in enumerate(zip(particles, orders)):
+            cp = np.asarray(getattr(p.scattered_field, 'coefficients', p.scattered_field), dtype=complex)
+            for tau in range(2):
+                for l in range(1, lp + 1):
+                    for m in range(-min(l, mp), min(l, mp) + 1):
+                        self.coefficients[i, flds.multi_to_single_index(tau, l, m, self.l_max, self.m_max)] = \
+                            cp[flds.multi_to_single_index(tau, l, m, lp, mp)]
+        self._particles = particles
+
+    def valid(self, x, y, z):
+        z = np.asarray(z)
+        vld = np.logical_and(z >= self.lower_z, z < self.upper_z)
+        if len(self._particles) == 1:
+            vld = np.logical_and(vld, self._particles[0].is_outside(x, y, z))
+        return np.logical_and(vld, FieldExpansion.valid(self, x, y, z))
+
+    def _field(self, field, x, y, z, omega, precision):
+        x, y, z = np.array(x, ndmin=1), np.array(y, ndmin=1), np.array(z, ndmin=1)
+        pts = np.stack([x.ravel(), y.ravel(), z.ravel()], axis=1)
+        zlim = np.tile([self.lower_z, self.upper_z], (len(self.radii), 1))
+        res = swe_field_sum(self.positions, self.radii, zlim, self.coefficients, self.l_max, self.m_max, self.k,
+                            pts, field, omega, precision)
+        return tuple(r.reshape(x.shape) for r in res)
+
+    def electric_field(self, x, y, z, precision='single'):
+        return self._field(0, x, y, z, 1.0, precision)
+
+    def magnetic_field(self, x, y, z, vacuum_wavelength, precision='single'):
+        return self._field(1, x, y, z, flds.angular_frequency(vacuum_wavelength), precision)
+
 
 class PlaneWaveExpansion(FieldExpansion):
     """g_j(kappa, alpha) on a (k_parallel, azimuthal_angles) grid; a single

@@ -13,7 +13,7 @@ from . import _lib
 from . import cuda as cu
 from . import fields as flds
 from . import special as sf
-from .expansions import PiecewiseFieldExpansion, PlaneWaveExpansion, layer_response
+from .expansions import PiecewiseFieldExpansion, PlaneWaveExpansion, SphericalWaveSum, layer_response
 
 
 def _coefficients(particle):
@@ -100,18 +100,16 @@ def scattered_field_pwe(vacuum_wavelength, particle_list, layer_system, layer_nu
 
 def scattered_field_piecewise_expansion(vacuum_wavelength, particle_list, layer_system, k_parallel='default',
                                         azimuthal_angles='default', layer_numbers=None, precision='single'):
-    """Scattered field as a PiecewiseFieldExpansion of plane wave expansions
+    """Scattered field as a PiecewiseFieldExpansion
     (postprocessing/scattered_field.py:13-77): per layer the layer-mediated
     upgoing (not in the bottom layer) and downgoing (not in the top layer)
-    waves, plus, in each layer holding scatterers, their direct field as a
-    PWE on the side of the scatterers where it converges. Points between the
-    lowest and highest scatterer of a layer are not covered (there the
-    direct field needs the spherical wave expansions)."""
+    plane wave expansions, plus the direct field, the particles' outgoing
+    spherical wave expansions, summed on the GPU per layer
+    (expansions.SphericalWaveSum); points inside a particle get no field."""
     if layer_numbers is None:
         layer_numbers = range(layer_system.number_of_layers())
     nl = layer_system.number_of_layers()
     sfld = PiecewiseFieldExpansion()
-    occupied = {layer_system.layer_number(p.position[2]) for p in particle_list}
     for i in layer_numbers:
         up, dn = scattered_field_pwe(vacuum_wavelength, particle_list, layer_system, i, k_parallel, azimuthal_angles,
                                      include_direct=False, include_layer_response=True, precision=precision)
@@ -119,11 +117,13 @@ def scattered_field_piecewise_expansion(vacuum_wavelength, particle_list, layer_
             sfld.expansion_list.append(up)
         if i < nl - 1:
             sfld.expansion_list.append(dn)
-        if i in occupied:
-            du, dd = scattered_field_pwe(vacuum_wavelength, particle_list, layer_system, i, k_parallel,
-                                         azimuthal_angles, include_direct=True, include_layer_response=False,
-                                         precision=precision)
-            sfld.expansion_list.extend([du, dd])
+    by_layer = {}
+    for p in particle_list:
+        by_layer.setdefault(layer_system.layer_number(p.position[2]), []).append(p)
+    for i, parts in sorted(by_layer.items()):
+        k = flds.angular_frequency(vacuum_wavelength) * layer_system.refractive_indices[i]
+        sfld.expansion_list.append(SphericalWaveSum(k, parts, layer_system.lower_zlimit(i),
+                                                    layer_system.upper_zlimit(i)))
     for p in particle_list:
         sfld.validity_conditions.append(p.is_outside)
     return sfld

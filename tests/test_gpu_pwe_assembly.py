@@ -83,6 +83,10 @@ def test_extinction_cross_section_matches_oracle(gpu):
 
 
 def test_piecewise_expansion_sums_valid_parts(gpu):
+    # scattered_field_piecewise_expansion (postprocessing/scattered_field.py:13-77): layer-mediated
+    # PWEs per layer plus the particles' SWEs (direct field) in their layer, nothing inside a particle;
+    # points below, above and inside the particle slab
+    from oracle import swe as osw
     from smuthi_amd.layers import LayerSystem
     from smuthi_amd.scattered_field import scattered_field_piecewise_expansion
     thick, n = [0, 0], [1.52, 1.0]
@@ -95,10 +99,25 @@ def test_piecewise_expansion_sums_valid_parts(gpu):
     ls = ol.LayerSystem(thick, n)
     x = np.linspace(-2000, 2000, 40)
     y = np.zeros_like(x)
-    for zz, layer in ((-300.0, 0), (900.0, 1)):
+    k1 = ob.angular_frequency(WL) * n[1]
+    swes = [(p.scattered_field, p.l_max, p.m_max, p.position, p.radius) for p in plist]
+    for zz in (-300.0, 900.0, 200.0, 260.0):
         z = np.full_like(x, zz)
-        got = sfld.electric_field(x, y, z, precision='double')
-        up, dn = osys.scattered_field_pwe(WL, plist, ls, layer, kp, alpha)
-        want = [a + b for a, b in zip(up.electric_field(x, y, z), dn.electric_field(x, y, z))]
+        z[0], x[0] = plist[3].position[2], plist[3].position[0]  # a particle centre: no field
+        y0 = y.copy()
+        y0[0] = plist[3].position[1]
+        got = sfld.electric_field(x, y0, z, precision='double')
+        layer = 0 if zz < 0 else 1
+        up, dn = osys.scattered_field_pwe(WL, plist, ls, layer, kp, alpha, include_direct=False)
+        parts = [dn.electric_field(x, y0, z)] if layer == 0 else [up.electric_field(x, y0, z)]
+        if layer == 1:
+            parts.append(osw.direct_field(swes, k1, x, y0, z, z_range=(0.0, np.inf)))
+        want = [sum(p[c] for p in parts) for c in range(3)]
+        inside = np.zeros(x.shape, bool)
+        for p in plist:
+            inside |= ~p.is_outside(x, y0, z)
+        for c in range(3):
+            want[c][inside] = 0
         scale = max(np.abs(w).max() for w in want)
-        assert max(np.abs(g - w).max() for g, w in zip(got, want)) < 1e-10 * scale
+        assert max(np.abs(g - w).max() for g, w in zip(got, want)) < 1e-10 * scale, zz
+        assert all(g[0] == 0 for g in got)

@@ -15,6 +15,7 @@ FP64 GPU GMRES solve (tol 1e-9) of the same right-hand side; single-RHS
 solve and plain FP64 GMRES throughputs are reported beside.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--particles N] [--batch R]
+       python bench.py --near-field    (near-field kernels at the same ensemble, one JSON line)
 """
 import argparse
 import contextlib
@@ -90,6 +91,67 @@ def traffic_probe(n_particles, copy_bytes=512 << 20):
     log('copy %d bytes: %.3f ms; FP32 matvec kernel %.2f ms, 8-RHS kernel %.2f ms' % (copy_bytes, ms.value, k, kb))
 
 
+# flops per (point, particle) pair of the SWE direct sum at l_max = m_max = 3: 38 per (l, m) for both
+# tau, 50 per m (rotation and Cartesian accumulation), ~150 for geometry, Hankel and Legendre
+SWE_FLOPS_PER_PAIR = 15 * 38 + 7 * 50 + 150
+# flop-equivalents per (point, kappa, alpha) term of the PWE field sum (3 complex FMAs, phase,
+# 3 transcendentals at quarter rate)
+PWE_FLOP_EQ_PER_TERM = 62
+
+
+def near_field_bench(n_particles, n_grid=512, n_kappa=2000, n_alpha=180, seed=7):
+    """Time the two near-field kernels at the scale of the bench ensemble:
+    the direct field of every particle's outgoing SWE (l_max = m_max = 3) and
+    one plane wave expansion (n_kappa x n_alpha), both on an n_grid^2 xz-plane
+    grid above the particle slab (x in [-30, 30] um, z in [330, 3000] nm).
+    Expansion coefficients are synthetic (seeded); the kernel time does not
+    depend on their values. HIP-event times of the kernels alone."""
+    from smuthi_amd import cuda, fields as flds
+    from smuthi_amd.expansions import PlaneWaveExpansion, swe_field_sum
+    if not cuda.enable_gpu():
+        raise SystemExit('no MI355X device')
+    plist, ls, kp_, side = build_workload(n_particles)
+    rng = np.random.default_rng(seed)
+    k = 2 * np.pi / WL
+    B = flds.blocksize(3, 3)
+    pos = np.array([p.position for p in plist], dtype=float)
+    rad = np.array([p.radius for p in plist], dtype=float)
+    zlim = np.tile([0.0, np.inf], (len(plist), 1))
+    coef = (rng.standard_normal((len(plist), B)) + 1j * rng.standard_normal((len(plist), B))) * 1e-3
+    X, Z = np.meshgrid(np.linspace(-side / 2, side / 2, n_grid), np.linspace(330.0, 3000.0, n_grid))
+    pts = np.stack([X.ravel(), np.zeros(X.size), Z.ravel()], axis=1)
+    pairs = float(len(pts)) * len(plist)
+    out = {'metric': 'near-field kernels, %d spheres, %dx%d points' % (len(plist), n_grid, n_grid),
+           'points': len(pts), 'particles': len(plist), 'data': 'synthetic expansion coefficients'}
+    for prec, peak in (('single', 157.3), ('double', MFMA_FP64_PEAK_TFLOPS)):
+        swe_field_sum(pos, rad, zlim, coef, 3, 3, k, pts[:4096], 0, 1.0, prec)  # warm-up
+        swe_field_sum(pos, rad, zlim, coef, 3, 3, k, pts, 0, 1.0, prec)
+        ms = swe_field_sum.last_kernel_ms
+        tf = pairs * SWE_FLOPS_PER_PAIR / (ms * 1e-3) / 1e12
+        out['swe_direct_sum_' + ('fp32' if prec == 'single' else 'fp64')] = {
+            'kernel': 'swe_field_kernel<%s,3,3>' % ('float' if prec == 'single' else 'double'), 'ms': ms,
+            'pairs': pairs, 'svwf_terms': pairs * B, 'flops_per_pair': SWE_FLOPS_PER_PAIR, 'tflops': tf,
+            'peak': peak, 'frac': tf / peak}
+        log('SWE direct sum %s: %.2f ms, %.1f TFLOP/s' % (prec, ms, tf))
+    kpw = flds.reasonable_Sommerfeld_kpar_contour(WL, layer_refractive_indices=[1.52, 1], neff_resolution=1.4e-3,
+                                                  neff_imag=1e-2)
+    kpw = kpw[:n_kappa] if len(kpw) >= n_kappa else np.concatenate(
+        [kpw, kpw[-1] + np.arange(1, n_kappa + 1 - len(kpw)) * (kpw[-1] - kpw[-2])])
+    alpha = np.linspace(0, 2 * np.pi, n_alpha)
+    pwe = PlaneWaveExpansion(k, kpw, alpha, 'upgoing', [0, 0, 0], 0, np.inf)
+    pwe.coefficients = (rng.standard_normal((2, n_kappa, n_alpha)) + 1j * rng.standard_normal((2, n_kappa, n_alpha)))
+    pwe.electric_field(pts[:4096, 0], pts[:4096, 1], pts[:4096, 2], precision='single')  # warm-up
+    pwe.electric_field(pts[:, 0], pts[:, 1], pts[:, 2], precision='single')
+    ms = pwe.last_kernel_ms
+    terms = float(len(pts)) * n_kappa * n_alpha
+    tf = terms * PWE_FLOP_EQ_PER_TERM / (ms * 1e-3) / 1e12
+    out['pwe_field_fp32'] = {'kernel': 'pwe_field_kernel<float>', 'ms': ms, 'terms': terms,
+                             'flop_eq_per_term': PWE_FLOP_EQ_PER_TERM, 'tflops_equiv': tf, 'peak': 157.3,
+                             'frac': tf / 157.3}
+    log('PWE field fp32: %.2f ms, %.1f TFLOP/s-equivalent' % (ms, tf))
+    return json.dumps(out)
+
+
 def cpu_baseline(table, rho, n_full, iters_per_solve, n_sample=400, seed=5, budget_s=20.0):
     """Time the oracle's restatement of smuthi's CPU radial-lookup matvec
     (CouplingMatrixRadialLookupCPU.matvec: per-(n1, n2) scipy interp1d 'cubic',
@@ -136,6 +198,8 @@ def main():
     ap.add_argument('--matvec-iters', type=int, default=5)
     ap.add_argument('--traffic-probe', action='store_true',
                     help='only run the copy calibration and three FP32 matvecs (for counter passes)')
+    ap.add_argument('--near-field', action='store_true',
+                    help='time the SWE direct-sum and PWE near-field kernels instead (one JSON line)')
     args = ap.parse_args()
     if args.gpus != 1:
         print('bench.py measures one GPU: this build has no multi-GPU path (run with --gpus 1)', file=sys.stderr)
@@ -143,6 +207,8 @@ def main():
     if args.traffic_probe:
         traffic_probe(args.particles)
         return None
+    if args.near_field:
+        return near_field_bench(args.particles)
 
     from smuthi_amd import cuda
     from smuthi_amd.initial_field import PlaneWave

@@ -11,9 +11,10 @@
 // Layout. One thread per point, 256 points per workgroup; the particles are
 // staged through LDS in chunks (centre, squared radius, z range, the B
 // coefficients), every thread reading the same particle at the same time (LDS
-// broadcast). Per (point, particle) the geometry, e^{ikr} and the spherical
-// Hankel recurrence run in FP64 (large k r stays accurate), the normalized
-// Legendre functions and the multipole sum in Real (FP32 or FP64). The sum is
+// broadcast). Per (point, particle) the distance runs in FP64 and, in the FP32
+// kernel, the phase Re(k) r is reduced to [-pi, pi] in FP64 (large k r stays
+// accurate); the Hankel recurrence, the normalized Legendre functions and the
+// multipole sum run in Real (FP32 or FP64). The sum is
 // formed in spherical components per m, rotated by e^{i m phi} once per m and
 // added in Cartesian components.
 #include <cmath>
@@ -92,29 +93,37 @@ __global__ void __launch_bounds__(kSwePoints) swe_field_kernel(SweArgs a) {
         cp = (Real)(dx * irho);
         sn = (Real)(dy * irho);
       }
-      // spherical Hankel h_l(x), x = k r, and d/dx(x h_l) / x, in FP64
-      using Z = cplx<double>;
-      const Z x{a.k_re * r, a.k_im * r};
-      const Z ix = cdiv(Z{1.0, 0.0}, x);
-      const double dmp = exp(-x.im);
-      const Z eix{dmp * cos(x.re), dmp * sin(x.re)};
+      // spherical Hankel h_l(x), x = k r, and d/dx(x h_l) / x. FP64 kernel: all in FP64. FP32
+      // kernel: the phase Re(k) r is reduced to [-pi, pi] in FP64, the rest runs in FP32.
+      using Z = cplx<Real>;
+      const Z x{(Real)(a.k_re * r), (Real)(a.k_im * r)};
+      const Z ix = cdiv(Z{Real(1), Real(0)}, x);
+      Z eix;
+      if constexpr (sizeof(Real) == 8) {
+        const double dmp = exp(-x.im);
+        eix = Z{dmp * cos(x.re), dmp * sin(x.re)};
+      } else {
+        const double ph = a.k_re * r;
+        const float red = (float)(ph - 6.283185307179586477 * rint(ph * 0.15915494309189533577));
+        const float dmp = expf((float)(-a.k_im * r));
+        float sr, cr;
+        sincosf(red, &sr, &cr);
+        eix = Z{dmp * cr, dmp * sr};
+      }
       Z h[LMAX + 1];
       h[0] = cmul(Z{eix.im, -eix.re}, ix);  // -i e^{ix} / x
-      h[1] = cscale(cmul(cmul(eix, Z{x.re, x.im + 1.0}), cmul(ix, ix)), -1.0);
+      h[1] = cscale(cmul(cmul(eix, Z{x.re, x.im + Real(1)}), cmul(ix, ix)), Real(-1));
 #pragma unroll
-      for (int l = 1; l < LMAX; ++l) h[l + 1] = csub(cscale(cmul(ix, h[l]), double(2 * l + 1)), h[l - 1]);
+      for (int l = 1; l < LMAX; ++l) h[l + 1] = csub(cscale(cmul(ix, h[l]), Real(2 * l + 1)), h[l - 1]);
       // radial factors per l: A = h / sqrt(2l(l+1)), Bv = l(l+1) h / x / sqrt(..), Cv = (x h_(l-1) - l h_l) / x / sqrt(..)
       C A[LMAX + 1], Bv[LMAX + 1], Cv[LMAX + 1];
 #pragma unroll
       for (int l = 1; l <= LMAX; ++l) {
-        const double pf = 1.0 / sqrt(2.0 * l * (l + 1));
+        const Real pf = (Real)(1.0 / sqrt(2.0 * l * (l + 1)));
         const Z hx = cmul(h[l], ix);
-        const Z al = cscale(h[l], pf);
-        const Z bl = cscale(hx, pf * l * (l + 1));
-        const Z cl = cscale(csub(h[l - 1], cscale(hx, double(l))), pf);
-        A[l] = C{(Real)al.re, (Real)al.im};
-        Bv[l] = C{(Real)bl.re, (Real)bl.im};
-        Cv[l] = C{(Real)cl.re, (Real)cl.im};
+        A[l] = cscale(h[l], pf);
+        Bv[l] = cscale(hx, pf * Real(l * (l + 1)));
+        Cv[l] = cscale(csub(h[l - 1], cscale(hx, Real(l))), pf);
       }
       // normalized associated Legendre P, pi, tau of cos(theta) (utility/math.py:38-104, real argument)
       Real P[LMAX + 1][MMAX + 1], PI[LMAX + 1][MMAX + 1], TA[LMAX + 1][MMAX + 1];

@@ -149,6 +149,25 @@ def near_field_bench(n_particles, n_grid=512, n_kappa=2000, n_alpha=180, seed=7)
                              'flop_eq_per_term': PWE_FLOP_EQ_PER_TERM, 'tflops_equiv': tf, 'peak': 157.3,
                              'frac': tf / 157.3}
     log('PWE field fp32: %.2f ms, %.1f TFLOP/s-equivalent' % (ms, tf))
+    # scattered far field of one solution vector on the default half-degree grid
+    from smuthi_amd import far_field as ffm
+    from smuthi_amd.scattered_field import scattered_field_pwe
+    az, po = flds.angular_arrays(np.pi / 360)
+    x = (rng.standard_normal(len(plist) * B) + 1j * rng.standard_normal(len(plist) * B)) * 1e-3
+    parts = ffm._per_particle(plist, x)
+    for prec in ('single', 'double'):
+        t0 = time.time()
+        ffm.scattered_far_field(WL, parts, ls, po, az, precision=prec)
+        wall = time.time() - t0
+        kms = scattered_field_pwe.last_kernel_ms  # the bottom-hemisphere call
+        t0 = time.time()
+        ffm.scattered_far_field(WL, parts, ls, po, az, precision=prec)
+        wall = time.time() - t0
+        out['scattered_far_field_' + ('fp32' if prec == 'single' else 'fp64')] = {
+            'seconds': wall, 'polar': len(po), 'azimuthal': len(az),
+            'assembly_kernel_ms_bottom': kms,
+            'terms_per_hemisphere': float(len(plist)) * (len(po) // 2) * len(az)}
+        log('scattered far field %s: %.3f s' % (prec, wall))
     return json.dumps(out)
 
 

@@ -374,3 +374,92 @@ class LinearSystemOracle:
     def scatter_solution(self, b):
         for i, p in enumerate(self.particles):
             p.scattered_field = b[self.offs[i]:self.offs[i + 1]].copy()
+
+
+class FarField:
+    """Far-field amplitude [2, n_beta, n_alpha] and intensity (postprocessing/far_field.py:9-209)."""
+
+    def __init__(self, polar_angles, azimuthal_angles):
+        self.polar_angles = np.asarray(polar_angles)
+        self.azimuthal_angles = np.asarray(azimuthal_angles)
+        self.amplitude = np.full((2, len(self.polar_angles), len(self.azimuthal_angles)), np.nan, dtype=complex)
+        self.signal = np.full((2, len(self.polar_angles), len(self.azimuthal_angles)), np.nan)
+
+    def azimuthal_integral_times_sin_beta(self):
+        return np.trapezoid(self.signal, self.azimuthal_angles[None, None, :]) * np.sin(self.polar_angles[None, :])
+
+    def integral(self):
+        return np.trapezoid(self.azimuthal_integral_times_sin_beta(), self.polar_angles[None, :])
+
+    def append(self, other):
+        if max(self.polar_angles) <= min(other.polar_angles):
+            self.polar_angles = np.concatenate((self.polar_angles, other.polar_angles))
+            self.signal = np.concatenate((self.signal, other.signal), 1)
+            self.amplitude = np.concatenate((self.amplitude, other.amplitude), 1)
+        elif min(self.polar_angles) >= max(other.polar_angles):
+            self.polar_angles = np.concatenate((other.polar_angles, self.polar_angles))
+            self.signal = np.concatenate((other.signal, self.signal), 1)
+            self.amplitude = np.concatenate((other.amplitude, self.amplitude), 1)
+        else:
+            raise ValueError('far fields have overlapping polar angle domains')
+
+
+def pwe_to_ff_conversion(vacuum_wavelength, pwe):
+    """Far field of a PWE (far_field.py:223-259): amplitude -2 pi k kz g, intensity
+    2 pi^2 / omega k kz^2 |g|^2, polar angles asin(kappa / k) (pi - .. when downgoing)."""
+    omega = fb.angular_frequency(vacuum_wavelength)
+    k = pwe.k
+    kp = pwe.k_parallel
+    polar = np.arcsin(kp / k) if pwe.kind == 'upgoing' else np.pi - np.arcsin(kp / k)
+    kz = fb.k_z(k_parallel=kp, k=k)
+    ampli = -2 * np.pi * (kz * k)[None, :, None] * pwe.coefficients
+    intens = (2 * np.pi ** 2 / omega * (kz ** 2 * k)[None, :, None] * abs(pwe.coefficients) ** 2).real
+    idx = np.argsort(polar)
+    ff = FarField(np.real(polar[idx]), pwe.azimuthal_angles)
+    ff.amplitude = ampli[:, idx, :]
+    ff.signal = intens[:, idx, :]
+    return ff
+
+
+def scattered_far_field(vacuum_wavelength, particle_list, layer_system, polar_angles, azimuthal_angles):
+    """Scattered far field in the top and bottom layers (far_field.py:363-427)."""
+    omega = fb.angular_frequency(vacuum_wavelength)
+    polar_angles = np.asarray(polar_angles, dtype=float)
+    i_top = layer_system.number_of_layers() - 1
+    top = polar_angles[polar_angles < np.pi / 2]
+    bot = polar_angles[polar_angles > np.pi / 2]
+    n_top = np.sort(np.sin(top) * layer_system.refractive_indices[i_top])
+    n_bot = np.sort(np.sin(bot) * layer_system.refractive_indices[0])
+    ff_top = ff_bot = None
+    if len(top) > 1 and np.imag(layer_system.refractive_indices[i_top]) == 0:
+        up, _ = scattered_field_pwe(vacuum_wavelength, particle_list, layer_system, i_top, n_top * omega,
+                                    azimuthal_angles)
+        ff_top = pwe_to_ff_conversion(vacuum_wavelength, up)
+    if len(bot) > 1 and np.imag(layer_system.refractive_indices[0]) == 0:
+        _, dn = scattered_field_pwe(vacuum_wavelength, particle_list, layer_system, 0, n_bot * omega,
+                                    azimuthal_angles)
+        ff_bot = pwe_to_ff_conversion(vacuum_wavelength, dn)
+    if ff_top is not None:
+        if ff_bot is not None:
+            ff_top.append(ff_bot)
+        return ff_top
+    return ff_bot
+
+
+def scattering_cross_section(initial_field, particle_list, layer_system, polar_angles, azimuthal_angles):
+    """Differential scattering cross section: far-field intensity / |A|^2 n_P / 2 (far_field.py:429-478)."""
+    i_top = layer_system.number_of_layers() - 1
+    n_p = layer_system.refractive_indices[0 if np.cos(initial_field.polar_angle) > 0 else i_top]
+    if np.imag(n_p):
+        raise ValueError('plane wave from absorbing layer: cross section undefined')
+    i0 = abs(initial_field.amplitude) ** 2 * np.real(n_p) / 2
+    ff = scattered_far_field(initial_field.vacuum_wavelength, particle_list, layer_system, polar_angles,
+                             azimuthal_angles)
+    ff.signal = ff.signal / i0
+    return ff
+
+
+def total_scattering_cross_section(initial_field, particle_list, layer_system, polar_angles, azimuthal_angles):
+    """Angle integral of the differential scattering cross section (far_field.py:480-515)."""
+    s = scattering_cross_section(initial_field, particle_list, layer_system, polar_angles, azimuthal_angles).integral()
+    return s[0] + s[1]

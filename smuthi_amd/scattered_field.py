@@ -33,13 +33,17 @@ def _assemble(vacuum_wavelength, particles, layer_system, i_swe, k_parallel, azi
         raise NotImplementedError('PWE assembly supports m_max <= 4')
     B = flds.blocksize(l_max, m_max)
     coef = np.zeros((len(particles), B), complex)
+    # particles grouped by their own (l_max, m_max): one index map and one array copy per group
+    groups = {}
     for i, p in enumerate(particles):
-        c = _coefficients(p)
-        for tau in range(2):
-            for l in range(1, p.l_max + 1):
-                for m in range(-min(l, p.m_max), min(l, p.m_max) + 1):
-                    coef[i, flds.multi_to_single_index(tau, l, m, l_max, m_max)] = \
-                        c[flds.multi_to_single_index(tau, l, m, p.l_max, p.m_max)]
+        groups.setdefault((p.l_max, p.m_max), []).append(i)
+    for (lp, mp), idx in groups.items():
+        dst = [flds.multi_to_single_index(tau, l, m, l_max, m_max)
+               for tau in range(2) for l in range(1, lp + 1) for m in range(-min(l, mp), min(l, mp) + 1)]
+        src = [flds.multi_to_single_index(tau, l, m, lp, mp)
+               for tau in range(2) for l in range(1, lp + 1) for m in range(-min(l, mp), min(l, mp) + 1)]
+        block = np.stack([_coefficients(particles[i]) for i in idx])
+        coef[np.asarray(idx)[:, None], np.asarray(dst)[None, :]] = block[:, src]
     kp = np.ascontiguousarray(np.array(k_parallel, ndmin=1), dtype=complex)
     kz = flds.k_z(k_parallel=kp, k=k).astype(complex)
     # B_{n,pol}(+-kz/k, kappa/k) / (2 pi kz k): btab[k][dir][pol][n]
@@ -79,12 +83,14 @@ def scattered_field_pwe(vacuum_wavelength, particle_list, layer_system, layer_nu
     groups = {}
     for p in particle_list:
         groups.setdefault(layer_system.layer_number(p.position[2]), []).append(p)
+    scattered_field_pwe.last_kernel_ms = 0.0
     for i_swe, parts in sorted(groups.items()):
         direct = i_swe == layer_number and include_direct
         if not (direct or include_layer_response):
             continue
         gu, gd, k_swe, kp, al, ms = _assemble(vacuum_wavelength, parts, layer_system, i_swe, k_parallel,
                                               azimuthal_angles, precision)
+        scattered_field_pwe.last_kernel_ms += ms
         ref = [0, 0, layer_system.reference_z(i_swe)]
         zs = [p.position[2] for p in parts]
         pu = PlaneWaveExpansion(k_swe, kp, al, 'upgoing', ref, max(zs), layer_system.upper_zlimit(i_swe))

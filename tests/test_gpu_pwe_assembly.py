@@ -121,3 +121,56 @@ def test_piecewise_expansion_sums_valid_parts(gpu):
         scale = max(np.abs(w).max() for w in want)
         assert max(np.abs(g - w).max() for g, w in zip(got, want)) < 1e-10 * scale, zz
         assert all(g[0] == 0 for g in got)
+
+def test_far_field_and_scattering_cross_section_match_oracle(gpu):
+    # scattered_far_field / pwe_to_ff_conversion / scattering_cross_section
+    # (postprocessing/far_field.py:223-515) on the GPU assembly vs the FP64 oracle
+    from smuthi_amd.layers import LayerSystem
+    from smuthi_amd.initial_field import PlaneWave
+    from smuthi_amd import far_field as ff
+    thick, n = [0, 400, 0], [1.5, 2.0, 1.0]
+    plist = _particles(30, 11)
+    po = np.arange(0, np.pi + 1e-9, np.pi / 90)
+    az = np.arange(0, 2 * np.pi + 1e-9, np.pi / 36)
+    want = osys.scattered_far_field(WL, plist, ol.LayerSystem(thick, n), po, az)
+    got = ff.scattered_far_field(WL, plist, LayerSystem(thick, n), po, az, precision='double')
+    np.testing.assert_allclose(got.polar_angles, want.polar_angles, rtol=0, atol=1e-14)
+    assert _rel(got.signal, want.signal) < 1e-10
+    assert _rel(got.amplitude, want.amplitude) < 1e-10
+    opw = osys.PlaneWave(WL, np.pi * 7 / 8, np.pi / 3, 0, 1, [0, 0, 400])
+    pw = PlaneWave(WL, np.pi * 7 / 8, np.pi / 3, 0, amplitude=1, reference_point=[0, 0, 400])
+    s_want = osys.total_scattering_cross_section(opw, plist, ol.LayerSystem(thick, n), po, az)
+    s_got = ff.total_scattering_cross_section(initial_field=pw, particle_list=plist, layer_system=LayerSystem(thick, n),
+                                              polar_angles=po, azimuthal_angles=az, precision='double')
+    assert abs(s_got - s_want) / abs(s_want) < 1e-10
+
+
+def test_cross_sections_of_many_solutions(gpu):
+    # one particle configuration, several solution vectors: per-RHS scattering and extinction
+    # cross sections equal the oracle's with each solution stored on the particles
+    from smuthi_amd.layers import LayerSystem
+    from smuthi_amd.initial_field import PlaneWave
+    from smuthi_amd import far_field as ff
+    thick, n = [0, 0], [1.52, 1.0]
+    plist = _particles(12, 13)
+    for p in plist:
+        p.position[2] = 150.0
+    rng = np.random.default_rng(2)
+    nu = sum(ob.blocksize(p.l_max, p.m_max) for p in plist)
+    X = (rng.standard_normal((3, nu)) + 1j * rng.standard_normal((3, nu))) * 0.1
+    angles = [(np.pi, 0.0, 0), (np.pi * 0.9, 0.3, 1), (np.pi * 0.8, 1.1, 0)]
+    pws = [PlaneWave(WL, b, a, pol) for b, a, pol in angles]
+    po = np.arange(0, np.pi + 1e-9, np.pi / 60)
+    az = np.arange(0, 2 * np.pi + 1e-9, np.pi / 30)
+    scs, ecs = ff.scattering_cross_sections(pws, plist, LayerSystem(thick, n), X, po, az)
+    for j, (b, a, pol) in enumerate(angles):
+        off = 0
+        for p in plist:
+            nb = ob.blocksize(p.l_max, p.m_max)
+            p.scattered_field = X[j, off:off + nb]
+            off += nb
+        opw = osys.PlaneWave(WL, b, a, pol)
+        s = osys.total_scattering_cross_section(opw, plist, ol.LayerSystem(thick, n), po, az)
+        e = osys.extinction_cross_section(opw, plist, ol.LayerSystem(thick, n))
+        assert abs(scs[j] - s) / abs(s) < 1e-10
+        assert abs(ecs[j] - e) / abs(e) < 1e-10

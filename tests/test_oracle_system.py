@@ -143,3 +143,42 @@ def test_pwe_to_swe_selection_rule():
             assert abs(c[n]) < 1e-14
         else:
             assert abs(c[n]) > 1e-3
+
+
+def test_scattering_cross_section_mie_closed_form():
+    # one sphere in a homogeneous medium: the far-field integral of the scattered intensity
+    # (postprocessing/far_field.py:223-515) equals the Mie scattering cross section
+    # 2 pi / k^2 sum_l (2l+1) (|Q_te|^2 + |Q_tm|^2) up to the angular quadrature
+    kp = fb.reasonable_Sommerfeld_kpar_contour(WL, layer_refractive_indices=[1.3, 1.3], neff_resolution=1e-2)
+    ls = lay.LayerSystem([0, 0], [1.3, 1.3])
+    p = osys.Sphere([0, 0, 100], 1.9 + 0.05j, 120, 6, 6)
+    pw = osys.PlaneWave(WL, np.pi, 0, 1)
+    lsys = osys.LinearSystemOracle([p], pw, ls, kp)
+    p.scattered_field = lsys.solve_lu(lsys.explicit_coupling())
+    az, po = np.arange(0, 2 * np.pi + 1e-9, np.pi / 180), np.arange(0, np.pi + 1e-9, np.pi / 360)
+    sig = osys.total_scattering_cross_section(pw, [p], ls, po, az)
+    k = 2 * np.pi / WL * 1.3
+    mie = 0.0
+    for l in range(1, 7):
+        q_te = lay.mie_coefficient(0, l, k, 2 * np.pi / WL * p.refractive_index, 120)
+        q_tm = lay.mie_coefficient(1, l, k, 2 * np.pi / WL * p.refractive_index, 120)
+        mie += (2 * l + 1) * (abs(q_te) ** 2 + abs(q_tm) ** 2)
+    mie = 2 * np.pi / k ** 2 * mie
+    assert abs(sig - mie) / mie < 1e-4
+
+
+def test_optical_theorem_lossless_waveguide():
+    # tests/system_tests/conservation_of_energy_tests/test_conservation_of_energy_planewave.py:14-72:
+    # lossless spheres in a lossless layer system, scattering = extinction cross section to 1e-4
+    kp = fb.reasonable_Sommerfeld_kpar_contour(WL, neff_waypoints=[0, 0.5, 0.8 - 0.01j, 2 - 0.01j, 2.5, 5],
+                                               neff_resolution=5e-3)
+    plist = [osys.Sphere([100, 100, 150], 2.4, 110, 4, 4), osys.Sphere([-100, -100, 250], 1.9, 120, 3, 3),
+             osys.Sphere([-200, 100, 300], 1.7, 90, 3, 3)]
+    ls = lay.LayerSystem([0, 400, 0], [2, 1.3, 2])
+    pw = osys.PlaneWave(WL, np.pi * 7 / 8, np.pi / 3, 0, 1, [0, 0, 400])
+    lsys = osys.LinearSystemOracle(plist, pw, ls, kp)
+    lsys.scatter_solution(lsys.solve_lu(lsys.explicit_coupling()))
+    az, po = np.arange(0, 2 * np.pi + 1e-9, np.pi / 360), np.arange(0, np.pi + 1e-9, np.pi / 360)
+    scs = osys.total_scattering_cross_section(pw, plist, ls, po, az)
+    ecs = osys.extinction_cross_section(pw, plist, ls)
+    assert abs(scs - ecs) / abs(scs) < 1e-4

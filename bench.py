@@ -16,6 +16,7 @@ solve and plain FP64 GMRES throughputs are reported beside.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--particles N] [--batch R]
        python bench.py --near-field    (near-field kernels at the same ensemble, one JSON line)
+       python bench.py --explicit N    (explicit coupling matrix of N spheres: build and matvecs, one JSON line)
 """
 import argparse
 import contextlib
@@ -40,14 +41,21 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def build_workload(n_particles, seed=2):
-    from smuthi_amd import ensembles, fields as flds
+def workload_medium():
+    """Glass (n = 1.52) below air, and the Sommerfeld contour of the workload."""
+    from smuthi_amd import fields as flds
     from smuthi_amd.layers import LayerSystem
-    side = 60000.0 * np.sqrt(n_particles / 10000.0)
-    plist = ensembles.monolayer(n_particles, side, 200.0, seed=seed)
     ls = LayerSystem([0, 0], [1.52, 1])
     kp = flds.reasonable_Sommerfeld_kpar_contour(WL, layer_refractive_indices=[1.52, 1], neff_resolution=2e-3,
                                                  neff_imag=2e-3)
+    return ls, kp
+
+
+def build_workload(n_particles, seed=2):
+    from smuthi_amd import ensembles
+    side = 60000.0 * np.sqrt(n_particles / 10000.0)
+    plist = ensembles.monolayer(n_particles, side, 200.0, seed=seed)
+    ls, kp = workload_medium()
     return plist, ls, kp, side
 
 
@@ -171,6 +179,71 @@ def near_field_bench(n_particles, n_grid=512, n_kappa=2000, n_alpha=180, seed=7)
     return json.dumps(out)
 
 
+# executed FP64 flops per (receiver, emitter, kappa, matrix entry) of the
+# explicit layer-mediated sum: t = d0 T1 + d1 T0 (2 complex products and a
+# sum, 14 flops) and acc += J t (a complex multiply-add, 8 flops)
+EXPLICIT_FLOPS_PER_TERM = 22
+
+
+def explicit_bench(n_particles=1000, seed=11, cpu_pairs=20):
+    """Build the explicit (dense FP64) coupling matrix of n_particles spheres
+    at heights U(200, 400) nm in air above glass (lambda 550 nm, l_max = m_max
+    = 3, the bench's Sommerfeld contour), then time its dense matvecs.
+    Reports the build kernels' HIP-event time, the layer-mediated sum's FP64
+    rate against the FP64 peak, the matvecs' W bandwidth against HBM, and the
+    oracle's per-pair CPU time (explicit direct + layer-mediated blocks, as
+    smuthi's CouplingMatrixExplicit builds them) on a sample of pairs."""
+    from smuthi_amd import cuda, ensembles, fields as flds
+    from smuthi_amd.linear_system import CouplingMatrixExplicit
+    if not cuda.enable_gpu():
+        raise SystemExit('no MI355X device')
+    ls, kp = workload_medium()
+    side = 60000.0 * np.sqrt(n_particles / 10000.0)
+    flat = ensembles.monolayer(n_particles, side, 300.0, seed=seed)
+    rng = np.random.default_rng(seed)
+    for p in flat:
+        p.position = [p.position[0], p.position[1], float(rng.uniform(200.0, 400.0))]
+    B = flds.blocksize(3, 3)
+    n = n_particles * B
+    log('explicit: %d spheres, %d unknowns, N_kappa %d' % (n_particles, n, len(kp)))
+    CouplingMatrixExplicit(WL, flat[:64], ls, k_parallel=kp)  # warm-up (kernel load)
+    t0 = time.time()
+    op = CouplingMatrixExplicit(WL, flat, ls, k_parallel=kp)
+    wall = time.time() - t0
+    terms = float(n_particles) ** 2 * len(kp) * B * B
+    tf = terms * EXPLICIT_FLOPS_PER_TERM / (op.build_kernel_ms * 1e-3) / 1e12
+    out = {'metric': 'explicit coupling matrix build, %d spheres (%d unknowns)' % (n_particles, n),
+           'value': op.build_kernel_ms / 1e3, 'unit': 's (build kernels)', 'higher_is_better': False,
+           'wall_s': wall, 'host_coefficients_s': op.host_seconds, 'n_kappa': len(kp), 'dtype': 'f64',
+           'data': 'synthetic (seeded positions)',
+           'roofline': {'bound': 'fp64', 'achieved': tf, 'peak': MFMA_FP64_PEAK_TFLOPS, 'unit': 'TFLOP/s',
+                        'frac': tf / MFMA_FP64_PEAK_TFLOPS, 'traffic': None,
+                        'note': 'all build kernels timed together; %d flops per (pair, kappa, entry)'
+                                % EXPLICIT_FLOPS_PER_TERM}}
+    log('explicit build: %.3f s kernels (%.1f TFLOP/s), %.2f s wall' % (op.build_kernel_ms / 1e3, tf, wall))
+    for prec, esz in (('double', 16), ('single', 8)):
+        k_ms, t_ms = op.bench_matvec(prec, iters=10)
+        gbs = float(n) * n * esz / (k_ms * 1e-3) / 1e9
+        out['matvec_' + ('fp64' if prec == 'double' else 'fp32')] = {
+            'kernel': 'explicit_gemv_kernel<%s>' % ('double' if prec == 'double' else 'float'), 'ms': k_ms,
+            'matvec_ms': t_ms, 'GB/s': gbs, 'frac_hbm': gbs / HBM_PEAK_GBS}
+        log('explicit matvec %s: %.3f ms, %.0f GB/s' % (prec, k_ms, gbs))
+    # CPU: the oracle's explicit blocks (smuthi's per-pair Python path) on a sample of pairs
+    from oracle import coupling as oc, layers as ol
+    from oracle.system import Sphere as OSphere
+    ls_o = ol.LayerSystem(ls.thicknesses, ls.refractive_indices)
+    sample = [OSphere(p.position, p.refractive_index, p.radius, 3, 3) for p in flat[:cpu_pairs + 1]]
+    t0 = time.time()
+    for q in range(cpu_pairs):
+        oc.direct_coupling_block(WL, sample[0], sample[q + 1], ls_o)
+        oc.layer_mediated_coupling_block(WL, sample[0], sample[q + 1], ls_o, kp)
+    per_pair = (time.time() - t0) / cpu_pairs
+    out['cpu_baseline'] = {'value': per_pair * n_particles ** 2, 'unit': 's (extrapolated build)', 'cores': 1,
+                           'kind': 'port', 'sample': '%d pairs, %.3f s per pair' % (cpu_pairs, per_pair)}
+    log('oracle CPU: %.3f s per pair -> %.0f s for the matrix' % (per_pair, per_pair * n_particles ** 2))
+    return json.dumps(out)
+
+
 def cpu_baseline(table, rho, n_full, iters_per_solve, n_sample=400, seed=5, budget_s=20.0):
     """Time the oracle's restatement of smuthi's CPU radial-lookup matvec
     (CouplingMatrixRadialLookupCPU.matvec: per-(n1, n2) scipy interp1d 'cubic',
@@ -217,6 +290,8 @@ def main():
     ap.add_argument('--matvec-iters', type=int, default=5)
     ap.add_argument('--traffic-probe', action='store_true',
                     help='only run the copy calibration and three FP32 matvecs (for counter passes)')
+    ap.add_argument('--explicit', type=int, default=0, metavar='N',
+                    help='build the explicit coupling matrix of N spheres and time its matvecs (one JSON line)')
     ap.add_argument('--near-field', action='store_true',
                     help='time the SWE direct-sum and PWE near-field kernels instead (one JSON line)')
     args = ap.parse_args()
@@ -228,6 +303,8 @@ def main():
         return None
     if args.near_field:
         return near_field_bench(args.particles)
+    if args.explicit:
+        return explicit_bench(args.explicit)
 
     from smuthi_amd import cuda
     from smuthi_amd.initial_field import PlaneWave

@@ -94,6 +94,32 @@ int smh_op_destroy(smh_op *op);
  * (plus, minus), stored complex64. B = blocksize(l_max, m_max). */
 int smh_op_upload_table(smh_op *op, const double *table, const double *table_minus);
 
+/* Explicit coupling operator (CouplingMatrixExplicit.__init__, linearsystem/linear_system.py:453-486):
+ * the dense coupling matrix of every particle pair, built on the GPU in FP64
+ * at the pairs' exact positions (no lookup). Block of receiver i, emitter j:
+ *   W_ij[n1][n2] = e^{i (m2 - m1) phi_ij} (D_ij + R_ij)
+ *   D_ij = sum_ld direct_coef[n1][n2][ld] h_ld(k_{L_i} d_ij) P_ld^|m1-m2|(dz_ij / d_ij)
+ *          (direct_coupling_block, direct_coupling.py:18-105), for i != j in one
+ *          layer, or every i != j when direct_all is set (a layer system of one medium);
+ *   R_ij = sum_k J_|m1-m2|(kappa_k rho_ij) sum_{a,b in 0,1} layer_coef[L_i][L_j][k][2a+b][n1][n2]
+ *          e^{i s_a kz[L_i][k] z_rel_i} e^{-i s_b kz[L_j][k] z_rel_j},   s_0 = +1, s_1 = -1
+ *          (layer_mediated_coupling_block, layer_mediated_coupling.py:215-340; the
+ *          Jacobian, layer response and quadrature weights folded into layer_coef).
+ * layer: [n_part] index of each particle's layer among n_layers occupied layers;
+ * k_layer: complex128 [n_layers]; z_rel: [n_part] height above the layer's
+ * reference; direct_coef: complex128 [B][B][2 l_max + 1]; kappa: complex128
+ * [n_kappa]; kz: complex128 [n_layers][n_kappa]; layer_coef: complex128
+ * [n_layers][n_layers][n_kappa][4][B][B] (n_kappa = 0 and NULL pointers: no
+ * layer-mediated part). l_max <= 8, m_max <= 4. The matrix needs 24 bytes per
+ * entry of device memory ((n_part B)^2 entries); build_ms (may be NULL)
+ * receives the build kernels' time. smh_op_matvec and smh_solve apply it as a
+ * dense matrix; smh_op_download_table returns it on the padded index
+ * i * B + n (B = blocksize(l_max, m_max)), complex128 [n_part B][n_part B]. */
+int smh_explicit_create(smh_ctx *ctx, int64_t n_part, const double *xyz, const int *lmax, const int *mmax, int l_max,
+                        int m_max, const int *layer, int n_layers, const double *k_layer, const double *z_rel,
+                        int direct_all, const double *direct_coef, int64_t n_kappa, const double *kappa,
+                        const double *kz, const double *layer_coef, double *build_ms, smh_op **out);
+
 /* Build the radial table on the GPU in FP64 (replaces radial_coupling_lookup_table,
  * prepare_lookup.py:253-464, and its helper kernel, prepare_lookup_cuda.py:55-95):
  *   L[j,n1,n2] = c64(direct[j,n1,n2]) + sum_k J_|m1-m2|(kappa_k rho_j) * lm_coef[k,n1,n2]

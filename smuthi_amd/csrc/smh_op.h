@@ -5,7 +5,7 @@
 
 struct smh_op {
   smh_ctx* ctx = nullptr;
-  int kind = 0;  // 0 radial, 1 volume
+  int kind = 0;  // 0 radial, 1 volume, 2 explicit (dense FP64 / FP32 matrix)
   int64_t n_part = 0, n_unk = 0;
   int l_max = 0, m_max = 0, B = 0, Bh = 0, MP = 0;
   int interp = 0, T = 0, s_mode = 1, o_mode = 0;
@@ -63,6 +63,10 @@ struct smh_op {
   float* d_vrowsum[2] = {nullptr, nullptr};
   Binning vbin[2][2];
 
+  // explicit operator: dense W [n_part * B][n_part * B] on padded vectors
+  double2* d_W64 = nullptr;
+  float2* d_W32 = nullptr;
+
   // work buffers
   void* d_ypart = nullptr;
   size_t ypart_bytes = 0;
@@ -97,6 +101,9 @@ int op_index_map(smh_op* op, const int* lmax, const int* mmax);
 int vol_apply(smh_op* op, int precision, const void* x_pad, void* y_pad, bool time_kernel);
 int vol_upload(smh_op* op, const double* plus, const double* minus);
 int vol_download(smh_op* op, double* plus, double* minus);
+// explicit operator: dense matvec and the padded matrix (smh_explicit.hip)
+int explicit_apply(smh_op* op, int precision, const void* x_pad, void* y_pad, bool time_kernel);
+int explicit_download(smh_op* op, double* W);
 // kBatch-RHS matvec (radial operators): FP64 input and every output on padded
 // [n_part][B][kBatch] vectors; the FP32 input in the layout of b32_index()
 bool op_has_batch(const smh_op* op, int precision);

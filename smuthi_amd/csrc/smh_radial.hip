@@ -408,6 +408,7 @@ int op_apply(smh_op* op, int precision, const void* x_pad, void* y_pad, bool tim
   smh_ctx* ctx = op->ctx;
   SMH_CHECK(ctx, op->has_table, SMH_ERR_STATE, "coupling table not set");
   if (op->kind == 1) return vol_apply(op, precision, x_pad, y_pad, time_kernel);
+  if (op->kind == 2) return explicit_apply(op, precision, x_pad, y_pad, time_kernel);
   const int pi = precision == SMH_FP32 ? 0 : 1;
   const smh_op::Binning& bn = op->bin[pi];
   SMH_CHECK(ctx, bn.Rc > 0, SMH_ERR_UNSUPPORTED, "no %s kernel fits LDS for blocksize %d (l_max %d, m_max %d)",
@@ -648,7 +649,7 @@ int op_index_map(smh_op* op, const int* lmax, const int* mmax) {
 
 void smh_op_free(smh_op* op) {
   void* ptrs[] = {op->d_xy, op->d_xyz, op->d_pk2pad, op->d_table, op->d_table_minus, op->d_tt64, op->d_tt32,
-                  op->d_tt32b, op->d_b32_tab, op->d_ypart, op->d_pk_in, op->d_pk_out, op->d_x32, op->d_y32, op->d_x64, op->d_y64,
+                  op->d_tt32b, op->d_b32_tab, op->d_W64, op->d_W32, op->d_ypart, op->d_pk_in, op->d_pk_out, op->d_x32, op->d_y32, op->d_x64, op->d_y64,
                   op->d_rowsum, op->d_xmax, op->d_xb, op->d_yb};
   for (void* p : ptrs)
     if (p) hipFree(p);
@@ -894,6 +895,7 @@ extern "C" int smh_op_upload_table(smh_op* op, const double* table, const double
   smh_ctx* ctx = op->ctx;
   SMH_CHECK(ctx, table != nullptr, SMH_ERR_ARG, "table is NULL");
   SMH_HIP(ctx, hipSetDevice(ctx->device));
+  SMH_CHECK(ctx, op->kind != 2, SMH_ERR_ARG, "the explicit operator builds its matrix itself");
   if (op->kind == 1) {
     SMH_CHECK(ctx, table_minus != nullptr, SMH_ERR_ARG, "volume operator needs the minus table");
     return vol_upload(op, table, table_minus);
@@ -909,6 +911,7 @@ extern "C" int smh_op_download_table(smh_op* op, double* table, double* table_mi
   smh_ctx* ctx = op->ctx;
   SMH_HIP(ctx, hipSetDevice(ctx->device));
   if (op->kind == 1) return vol_download(op, table, table_minus);
+  if (op->kind == 2) return explicit_download(op, table);
   SMH_CHECK(ctx, op->d_table != nullptr, SMH_ERR_STATE, "no table");
   const size_t bytes = (size_t)op->len_rho * op->B * op->B * sizeof(double2);
   SMH_HIP(ctx, hipMemcpyAsync(table, op->d_table, bytes, hipMemcpyDeviceToHost, ctx->stream));

@@ -6,6 +6,7 @@
 //     Negative real part uses J_n(-z) = (-1)^n J_n(z).
 //   * spherical Hankel h_n^(1)(z) by upward recurrence from h_0 = -i e^{iz}/z
 //     and h_1 = -e^{iz} (z + i) / z^2.
+//   * normalized associated Legendre P_l^m(cos t) of real argument.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -83,10 +84,11 @@ __host__ __device__ inline void bessel_j_all(int nmax, zc z, zc* out) {
   } else if (az <= 25.0) {
     int N = 2 * (int)((1.3 * az + 30.0) * 0.5);
     if (N < nmax + 20) N = nmax + 20 + (nmax & 1);
+    // the unnormalised values are kept in out[] itself (no local array, so
+    // that callers may pass LDS and keep their registers)
     zc fp1 = {0.0, 0.0}, f = {1e-30, 0.0};
     zc norm = {0.0, 0.0};
-    zc vals[kMaxBesselOrder + 1];
-    for (int n = 0; n <= nmax; ++n) vals[n] = {0.0, 0.0};
+    for (int n = 0; n <= nmax; ++n) out[n] = {0.0, 0.0};
     zc inv_z = zdiv(zc{1.0, 0.0}, z);
     for (int n = N; n >= 1; --n) {
       // f_{n-1} = (2n/z) f_n - f_{n+1}
@@ -94,18 +96,18 @@ __host__ __device__ inline void bessel_j_all(int nmax, zc z, zc* out) {
       fp1 = f;
       f = fm1;
       int idx = n - 1;
-      if (idx <= nmax) vals[idx] = f;
+      if (idx <= nmax) out[idx] = f;
       if (idx > 0 && (idx % 2) == 0) norm = zadd(norm, zscale(f, 2.0));
       double a = zabs(f);
       if (a > 1e200) {
         f = zscale(f, 1e-200);
         fp1 = zscale(fp1, 1e-200);
         norm = zscale(norm, 1e-200);
-        for (int q = 0; q <= nmax; ++q) vals[q] = zscale(vals[q], 1e-200);
+        for (int q = 0; q <= nmax; ++q) out[q] = zscale(out[q], 1e-200);
       }
     }
     norm = zadd(norm, f);  // + f_0
-    for (int n = 0; n <= nmax; ++n) out[n] = zdiv(vals[n], norm);
+    for (int n = 0; n <= nmax; ++n) out[n] = zdiv(out[n], norm);
   } else {
     // Hankel asymptotic expansion for J_0 and J_1.
     zc inv8z = zdiv(zc{1.0, 0.0}, zscale(z, 8.0));
@@ -152,6 +154,34 @@ __host__ __device__ inline void spherical_hankel_all(int nmax, zc z, zc* out) {
   zc h1 = zmul(zscale(eiz, -1.0), zmul(zadd(z, zc{0.0, 1.0}), zmul(inv_z, inv_z)));
   out[1] = h1;
   for (int n = 1; n < nmax; ++n) out[n + 1] = zsub(zscale(zmul(inv_z, out[n]), 2.0 * n + 1.0), out[n - 1]);
+}
+
+// Normalized associated Legendre P_l^m(cos t) for real (ct, st), l <= lmax,
+// m <= l, P[l * (lmax + 1) + m] (the normalization of smuthi's
+// legendre_normalized, utility/math.py:38-104).
+__host__ __device__ inline void legendre_p(int lmax, double ct, double st, double* P) {
+  const int W = lmax + 1;
+  for (int j = 0; j < W * W; ++j) P[j] = 0.0;
+  P[0] = sqrt(0.5);
+  if (lmax >= 1) P[W] = sqrt(1.5) * ct;
+  for (int l = 1; l < lmax; ++l) {
+    const double a = sqrt((2.0 * l + 1) * (2.0 * l + 3)) / (l + 1);
+    const double b = (double)l / (l + 1) * sqrt((2.0 * l + 3) / (2.0 * l - 1));
+    P[(l + 1) * W] = a * ct * P[l * W] - b * P[(l - 1) * W];
+  }
+  double prod = 1.0;
+  for (int m = 1; m <= lmax; ++m) {
+    prod *= (2.0 * m + 1) / (2.0 * m);
+    double stm = 1.0;
+    for (int i = 0; i < m; ++i) stm *= st;
+    P[m * W + m] = sqrt(prod / 2.0) * stm;
+    for (int l = m; l < lmax; ++l) {
+      const double den = (double)(l + 1 - m) * (l + 1 + m);
+      const double a = sqrt((2.0 * l + 1) * (2.0 * l + 3) / den);
+      const double b = sqrt((2.0 * l + 3) * (l - m) * (l + m) / ((2.0 * l - 1) * den));
+      P[(l + 1) * W + m] = a * ct * P[l * W + m] - (l - 1 >= m ? b * P[(l - 1) * W + m] : 0.0);
+    }
+  }
 }
 
 }  // namespace smh

@@ -24,34 +24,6 @@ namespace {
 
 constexpr int kMaxL2 = 16;  // 2 l_max <= 16
 
-// Normalized associated Legendre P_l^m(cos t) for real (ct, st), l <= lmax,
-// m <= l, P[l * (lmax + 1) + m] (the normalization of smuthi's
-// legendre_normalized, utility/math.py:38-104).
-__device__ void legendre_p(int lmax, double ct, double st, double* P) {
-  const int W = lmax + 1;
-  for (int j = 0; j < W * W; ++j) P[j] = 0.0;
-  P[0] = sqrt(0.5);
-  if (lmax >= 1) P[W] = sqrt(1.5) * ct;
-  for (int l = 1; l < lmax; ++l) {
-    const double a = sqrt((2.0 * l + 1) * (2.0 * l + 3)) / (l + 1);
-    const double b = (double)l / (l + 1) * sqrt((2.0 * l + 3) / (2.0 * l - 1));
-    P[(l + 1) * W] = a * ct * P[l * W] - b * P[(l - 1) * W];
-  }
-  double prod = 1.0;
-  for (int m = 1; m <= lmax; ++m) {
-    prod *= (2.0 * m + 1) / (2.0 * m);
-    double stm = 1.0;
-    for (int i = 0; i < m; ++i) stm *= st;
-    P[m * W + m] = sqrt(prod / 2.0) * stm;
-    for (int l = m; l < lmax; ++l) {
-      const double den = (double)(l + 1 - m) * (l + 1 + m);
-      const double a = sqrt((2.0 * l + 1) * (2.0 * l + 3) / den);
-      const double b = sqrt((2.0 * l + 3) * (l - m) * (l + m) / ((2.0 * l - 1) * den));
-      P[(l + 1) * W + m] = a * ct * P[l * W + m] - (l - 1 >= m ? b * P[(l - 1) * W + m] : 0.0);
-    }
-  }
-}
-
 // Direct part of the minus table, rounded to complex64; one block per node.
 __global__ void vol_direct_kernel(double rho_min, double dz_min, double h, int64_t len_rho, int64_t len_z, int B,
                                   int L2, double kre, double kim, double mask, const double2* __restrict__ coef,
@@ -71,7 +43,7 @@ __global__ void vol_direct_kernel(double rho_min, double dz_min, double h, int64
     double d = sqrt(rho * rho + dz * dz);
     if (d == 0.0) d = 1e-20;
     smh::spherical_hankel_all(L2, zc{kre * d, kim * d}, hl);
-    legendre_p(L2, dz / d, rho / d, P);
+    smh::legendre_p(L2, dz / d, rho / d, P);
   }
   __syncthreads();
   for (int p = threadIdx.x; p < BB; p += blockDim.x) {

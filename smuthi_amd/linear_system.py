@@ -20,6 +20,9 @@ from . import cuda as cu
 from . import fields as flds
 from . import lookup as look
 
+# relative residual of the GPU solve that stands in for smuthi's LU solve
+_LU_TOLERANCE = 1e-12
+
 _INTERP = {'linear': _lib.INTERP_LINEAR, 'catmull-rom': _lib.INTERP_CATMULL_ROM, 'spline': _lib.INTERP_SPLINE}
 
 
@@ -212,6 +215,81 @@ class CouplingMatrixRadialLookupCUDA(_GpuCouplingOperator):
         return self
 
 
+class CouplingMatrixExplicit(_GpuCouplingOperator):
+    """Explicit (dense) coupling matrix built on the MI355X; drop-in for
+    smuthi's class of the same name (linear_system.py:441-486).
+
+    Every block is direct_coupling_block (direct_coupling.py:18-105) plus
+    layer_mediated_coupling_block (layer_mediated_coupling.py:215-340) for the
+    pair's exact positions, in FP64, including pairs in different layers.
+    The layer-mediated Sommerfeld integral uses the reference's quadrature,
+    which leaves out the contour's last interval. The matrix stays on the
+    device (complex128 plus a complex64 copy for FP32 matvecs);
+    `coupling_matrix` downloads it in smuthi's system layout."""
+
+    def __init__(self, vacuum_wavelength, particle_list, layer_system, k_parallel='default', use_pvwf_coupling=False,
+                 pvwf_coupling_k_parallel=None):
+        if use_pvwf_coupling:
+            raise NotImplementedError('plane-wave mediated direct coupling is not built on the GPU path')
+        self._init_common(particle_list)
+        if isinstance(k_parallel, str) and k_parallel == 'default':
+            k_parallel = flds.default_Sommerfeld_k_parallel_array
+        layers = [layer_system.layer_number(p.position[2]) for p in particle_list]
+        occupied = sorted(set(layers))
+        lidx = {l: q for q, l in enumerate(occupied)}
+        layer = np.ascontiguousarray([lidx[l] for l in layers], dtype=np.int32)
+        k_layer = np.ascontiguousarray([complex(layer_system.wavenumber(l, vacuum_wavelength)) for l in occupied],
+                                       dtype=np.complex128)
+        z_rel = np.ascontiguousarray([p.position[2] - layer_system.reference_z(l)
+                                      for p, l in zip(particle_list, layers)], dtype=np.float64)
+        xyz = np.ascontiguousarray([p.position for p in particle_list], dtype=np.float64)
+        t0 = time.time()
+        coef = np.ascontiguousarray(look.direct_coefficients_volume(self.l_max, self.m_max))
+        kp = kz = lc = None
+        n_kappa = 0
+        if not layer_system.is_degenerate():
+            kp = np.ascontiguousarray(k_parallel, dtype=np.complex128)
+            n_kappa = len(kp)
+            w = look.trapezoid_weights_drop_last(kp)
+            nl = len(occupied)
+            B = self.blocksize
+            lc = np.empty((nl, nl, n_kappa, 4, B, B), dtype=np.complex128)
+            kz = np.empty((nl, n_kappa), dtype=np.complex128)
+            for a, l1 in enumerate(occupied):
+                for b, l2 in enumerate(occupied):
+                    lc[a, b], kz1, _ = look.layer_coefficients_pair(vacuum_wavelength, layer_system, kp, l1, l2,
+                                                                     self.l_max, self.m_max, w)
+                kz[a] = kz1
+        self.host_seconds = time.time() - t0
+        lib = _lib.load()
+        ctx = cu.context()
+        handle = ctypes.c_void_p()
+        ms = ctypes.c_double()
+        _lib.check(lib.smh_explicit_create(ctx, len(particle_list), _lib.dptr(xyz), _lib.iptr(self._lmax),
+                                           _lib.iptr(self._mmax), self.l_max, self.m_max, _lib.iptr(layer),
+                                           len(occupied), _lib.dptr(k_layer), _lib.dptr(z_rel),
+                                           int(layer_system.is_degenerate()), _lib.dptr(coef), n_kappa,
+                                           _lib.dptr(kp), _lib.dptr(kz), _lib.dptr(lc), ctypes.byref(ms),
+                                           ctypes.byref(handle)), ctx, 'smh_explicit_create')
+        self._op = handle
+        self.build_kernel_ms = ms.value
+        self.table_seconds = time.time() - t0
+        self._make_linear_operator()
+
+    @property
+    def coupling_matrix(self):
+        """The FP64 matrix [n, n] in smuthi's system layout (copied from the device on access)."""
+        n_pad = len(self.particle_list) * self.blocksize
+        full = np.empty((n_pad, n_pad), dtype=np.complex128)
+        _lib.check(_lib.load().smh_op_download_table(self._op, _lib.dptr(full), None), cu.context(),
+                   'smh_op_download_table')
+        idx = np.concatenate([i * self.blocksize + np.array(
+            [flds.multi_to_single_index(t, l, m, self.l_max, self.m_max)
+             for t in range(2) for l in range(1, p.l_max + 1) for m in range(-min(l, p.m_max), min(l, p.m_max) + 1)])
+            for i, p in enumerate(self.particle_list)])
+        return full[np.ix_(idx, idx)]
+
+
 class TMatrix(SystemMatrix):
     """Block-diagonal T of the particles (linear_system.py:410-438); spheres give a diagonal."""
 
@@ -250,8 +328,8 @@ class MasterMatrix(SystemMatrix):
 
 class LinearSystem:
     """Assemble and solve (1 - T W) b = T a with smuthi's constructor arguments
-    (linear_system.py:42-344). The coupling must use a lookup table; the solve
-    runs on the GPU. Extra arguments: precision ('mixed': FP32 coupling
+    (linear_system.py:42-344). The coupling is a lookup operator or the
+    explicit matrix, both built on the GPU; the solve runs on the GPU. Extra arguments: precision ('mixed': FP32 coupling
     matvecs inside FP64 iterative refinement; 'double': plain FP64 GMRES),
     cubic_interpolant, restart and maxiter."""
 
@@ -306,18 +384,31 @@ class LinearSystem:
         self.t_matrix = TMatrix(self.particle_list)
 
     def compute_coupling_matrix(self):
-        if self.coupling_matrix_lookup_resolution is None:
-            raise NotImplementedError('the GPU path couples particles through a lookup table: '
-                                      'set coupling_matrix_lookup_resolution')
-        if self.solver_type.lower() == 'lu' or self.store_coupling_matrix:
-            warnings.warn('explicit coupling matrices are not built on the GPU path; using the lookup with GMRES')
-            self.solver_type = 'gmres'
+        """Lookup operator when a resolution is given and every particle sits in
+        one layer, else the explicit matrix (linear_system.py:149-263)."""
         if not cu.use_gpu:
             raise RuntimeError('smuthi_amd provides the MI355X operators only: call '
                                'smuthi_amd.cuda.enable_gpu() first')
-        layers = {self.layer_system.layer_number(p.position[2]) for p in self.particle_list}
-        if len(layers) != 1:
-            raise NotImplementedError('lookup coupling needs all particles in one layer')
+        if self.solver_type.lower() == 'lu':
+            self.store_coupling_matrix = True
+        if self.coupling_matrix_lookup_resolution is not None:
+            layers = {self.layer_system.layer_number(p.position[2]) for p in self.particle_list}
+            if len(layers) != 1:
+                warnings.warn('Particles are not all in same layer. '
+                              'Fall back to direct coupling matrix computation (no lookup).')
+                self.coupling_matrix_lookup_resolution = None
+            elif self.store_coupling_matrix:
+                warnings.warn('Explicit matrix computation using lookup currently not implemented. '
+                              'Disabling lookup.')
+                self.coupling_matrix_lookup_resolution = None
+        if self.coupling_matrix_lookup_resolution is None:
+            if not self.store_coupling_matrix:
+                warnings.warn('With lookup disabled, coupling matrix needs to be stored.')
+                self.store_coupling_matrix = True
+            self.coupling_matrix = CouplingMatrixExplicit(vacuum_wavelength=self.initial_field.vacuum_wavelength,
+                                                          particle_list=self.particle_list,
+                                                          layer_system=self.layer_system, k_parallel=self.k_parallel)
+            return
         z = [p.position[2] for p in self.particle_list]
         kw = dict(vacuum_wavelength=self.initial_field.vacuum_wavelength, particle_list=self.particle_list,
                   layer_system=self.layer_system, k_parallel=self.k_parallel,
@@ -337,9 +428,14 @@ class LinearSystem:
             raise ValueError('This solver type is currently not implemented.')
         single = rhs is None
         b = self.t_matrix.right_hand_side() if single else rhs
+        tol, precision = self.solver_tolerance, self.precision
+        if self.solver_type.lower() == 'lu':
+            # the dense matrix stays on the device: FP64 GMRES to a residual a
+            # direct solve would leave
+            tol, precision = _LU_TOLERANCE, 'double'
         x, self.solve_stats, converged = self.coupling_matrix.solve_master(
-            self.t_matrix.diagonal, b, self.solver_tolerance, precision=self.precision, restart=self.restart,
-            maxiter=self.maxiter, with_stats=True)
+            self.t_matrix.diagonal, b, tol, precision=precision, restart=self.restart, maxiter=self.maxiter,
+            with_stats=True)
         if not converged:
             warnings.warn('GMRES did not reach the tolerance: %s'
                           % _lib.load().smh_last_error(cu.context()).decode(errors='replace'))

@@ -143,3 +143,49 @@ def layer_coefficients_volume(vacuum_wavelength, layer_system, k_parallel, i_s, 
     dm = np.abs(mm[:, None] - mm[None, :])
     ipow = np.array([1, 1j, -1, -1j])[dm % 4]
     return C * (2 * ipow[None, None] * (trapezoid_weights(k_parallel) * jac)[:, None, None, None]), kz
+
+
+def trapezoid_weights_drop_last(k_parallel):
+    """Weights of the explicit layer-mediated quadrature: numba_trapz
+    (layer_mediated_coupling.py:13-23) sums the intervals 0..n-3 only, so the
+    contour's last interval is left out: sum_k w_k f_k = sum_{k<n-2} (f_k + f_{k+1}) dk_k."""
+    dk = np.diff(k_parallel)[:-1]
+    w = np.zeros(len(k_parallel), complex)
+    w[:-2] += dk
+    w[1:-1] += dk
+    return w
+
+
+def layer_coefficients_pair(vacuum_wavelength, layer_system, k_parallel, is1, is2, l_max, m_max, weights):
+    """C[k, 2a+b, n1, n2] of the explicit layer-mediated block between a
+    receiver in layer is1 and an emitter in layer is2
+    (layer_mediated_coupling.py:215-340):
+      W^R = e^{i (m2-m1) phi} sum_k J_|m1-m2|(kappa_k rho) sum_ab C_ab e^{i s_a kz1 z1} e^{-i s_b kz2 z2},
+    s_0 = +1, s_1 = -1, z1 / z2 relative to the layers' reference heights,
+    C_ab = 2 i^|m1-m2| w_k kappa_k / (kz2_k k2) sum_pol L_ab(pol) B-dagger_a(pol, n1) B_b(pol, n2)
+    with L the response from layer is2 to layer is1 and w the quadrature weights.
+    Returns (C, kz1, kz2)."""
+    omega = flds.angular_frequency(vacuum_wavelength)
+    kp = np.asarray(k_parallel, dtype=complex)
+    k1 = complex(layer_system.wavenumber(is1, vacuum_wavelength))
+    k2 = complex(layer_system.wavenumber(is2, vacuum_wavelength))
+    kz1 = flds.k_z(k_parallel=kp, k=k1)
+    kz2 = flds.k_z(k_parallel=kp, k=k2)
+    L = np.stack([lay.layersystem_response_matrix(pol, layer_system.thicknesses, layer_system.refractive_indices,
+                                                  kp, omega, is2, is1) for pol in range(2)])
+    Bd = np.stack([sf.transformation_arrays(l_max, m_max, kz1 / k1, kp / k1, True),
+                   sf.transformation_arrays(l_max, m_max, -kz1 / k1, kp / k1, True)], axis=1)
+    Bn = np.stack([sf.transformation_arrays(l_max, m_max, kz2 / k2, kp / k2, False),
+                   sf.transformation_arrays(l_max, m_max, -kz2 / k2, kp / k2, False)], axis=1)
+    _, _, mm = flds.index_arrays(l_max, m_max)
+    B = len(mm)
+    C = np.zeros((len(kp), 4, B, B), complex)
+    for a in range(2):
+        for b in range(2):
+            for pol in range(2):
+                C[:, 2 * a + b] += L[pol, a, b][:, None, None] * Bd[pol, a].T[:, :, None] * Bn[pol, b].T[:, None, :]
+    jac = kp / (kz2 * k2)
+    dm = np.abs(mm[:, None] - mm[None, :])
+    ipow = np.array([1, 1j, -1, -1j])[dm % 4]
+    C *= 2 * ipow[None, None] * (np.asarray(weights) * jac)[:, None, None, None]
+    return C, kz1, kz2

@@ -114,3 +114,43 @@ def test_plane_wave_rhs_matches_oracle():
         want = np.concatenate([osys.initial_field_coefficients(opw, p, ols) for p in oparts])
         got = ppw.swe_coefficients(pparts, pls)
         np.testing.assert_allclose(got, want, rtol=1e-12, atol=1e-14)
+
+
+def test_explicit_quadrature_weights_drop_the_last_interval():
+    # the weights reproduce the reference's numba_trapz (oracle _trapz_drop_last) on random data
+    rng = np.random.default_rng(3)
+    kp = np.cumsum(rng.uniform(0.001, 0.01, 40)) * (1 - 0.02j)
+    y = rng.standard_normal(40) + 1j * rng.standard_normal(40)
+    w = look.trapezoid_weights_drop_last(kp)
+    want = oc._trapz_drop_last(y, kp)
+    assert abs(np.sum(w * y) / 2 - want) < 1e-13 * abs(want)
+    assert w[-1] == 0 and abs(np.sum(w) / 2 - (kp[-2] - kp[0])) < 1e-15
+
+
+def test_explicit_layer_coefficients_reproduce_oracle_blocks():
+    # sum_k J (sum_ab C_ab e^{i s_a kz1 z1} e^{-i s_b kz2 z2}) e^{i (m2-m1) phi} is the oracle's
+    # layer_mediated_coupling_block for same-layer, cross-layer and self pairs
+    wl = 550.0
+    thick, nidx = [0, 300, 0], [1.5, 2.0 + 0.01j, 1.0]
+    ls_o, ls = ol.LayerSystem(thick, nidx), lay.LayerSystem(thick, nidx)
+    kp = ob.reasonable_Sommerfeld_kpar_contour(wl, layer_refractive_indices=[1.5, 2.0, 1.0], neff_resolution=1e-2)
+    l_max = m_max = 3
+    _, _, mm = flds.index_arrays(l_max, m_max)
+    w = look.trapezoid_weights_drop_last(kp)
+    p0 = OSphere([0.0, 0.0, 100.0], 1.5, 50, l_max, m_max)
+    pairs = [(p0, OSphere([250.0, 130.0, 220.0], 1.5, 50, l_max, m_max)),
+             (p0, OSphere([-90.0, 40.0, -80.0], 1.5, 50, l_max, m_max)),
+             (OSphere([10.0, 0.0, 450.0], 1.5, 50, l_max, m_max), p0), (p0, p0)]
+    for p1, p2 in pairs:
+        want = oc.layer_mediated_coupling_block(wl, p1, p2, ls_o, kp)
+        is1, is2 = ls.layer_number(p1.position[2]), ls.layer_number(p2.position[2])
+        C, kz1, kz2 = look.layer_coefficients_pair(wl, ls, kp, is1, is2, l_max, m_max, w)
+        z1, z2 = p1.position[2] - ls.reference_z(is1), p2.position[2] - ls.reference_z(is2)
+        d = np.subtract(p1.position, p2.position)
+        rho, phi = np.hypot(d[0], d[1]), np.arctan2(d[1], d[0])
+        rx = [np.exp(1j * kz1 * z1), np.exp(-1j * kz1 * z1)]
+        fx = [np.exp(-1j * kz2 * z2), np.exp(1j * kz2 * z2)]
+        S = sum(C[:, 2 * a + b] * (rx[a] * fx[b])[:, None, None] for a in range(2) for b in range(2))
+        J = scipy.special.jv(np.abs(mm[:, None] - mm[None, :])[None], kp[:, None, None] * rho)
+        got = (J * S).sum(0) * np.exp(1j * (mm[None, :] - mm[:, None]) * phi)
+        assert np.abs(got - want).max() < 1e-12 * np.abs(want).max(), (is1, is2)

@@ -17,6 +17,7 @@ solve and plain FP64 GMRES throughputs are reported beside.
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--particles N] [--batch R]
        python bench.py --near-field    (near-field kernels at the same ensemble, one JSON line)
        python bench.py --explicit N    (explicit coupling matrix of N spheres: build and matvecs, one JSON line)
+       python bench.py --rhs           (Gaussian-beam and dipole right-hand sides at the ensemble, one JSON line)
 """
 import argparse
 import contextlib
@@ -244,6 +245,71 @@ def explicit_bench(n_particles=1000, seed=11, cpu_pairs=20):
     return json.dumps(out)
 
 
+# FP64 flop-equivalents per (point, kappa, alpha) term of the PWE-to-SWE
+# kernel: u = cos(a) dx + sin(a) dy (3), e^{i kappa u} (sincos and, off the
+# real axis, exp: counted as 20), 2 (2 m_max + 1) complex multiply-adds (8 each)
+P2S_FLOP_EQ_PER_TERM_33 = 3 + 20 + 14 * 8
+
+
+def rhs_bench(n_particles, n_dipoles=16, seed=13, cpu_particles=3):
+    """Right-hand sides other than plane waves at the bench ensemble: the
+    regular SWE coefficients of a Gaussian beam (waist 5 um, from the top, on
+    the bench's Sommerfeld contour truncated to the top medium's propagating
+    range x the default 0.5-degree azimuthal grid) and of a collection of
+    point dipoles (explicit blocks from every dipole to every sphere). HIP-event
+    kernel times; the oracle's per-particle CPU time beside."""
+    from smuthi_amd import cuda, fields as flds
+    from smuthi_amd import initial_field as inf
+    from smuthi_amd import linear_system as lsm
+    if not cuda.enable_gpu():
+        raise SystemExit('no MI355X device')
+    plist, ls, kp, side = build_workload(n_particles)
+    k0 = 2 * np.pi / WL
+    kpb = kp[np.real(kp) < 0.999 * k0]
+    al, _ = flds.angular_arrays(np.pi / 360)
+    beam = inf.GaussianBeam(WL, np.pi, 0.0, 0, 5000.0, k_parallel_array=kpb, azimuthal_angles_array=al)
+    beam.swe_coefficients(plist[:256], ls)  # warm-up
+    t0 = time.time()
+    beam.swe_coefficients(plist, ls)
+    wall = time.time() - t0
+    ms = inf.pwe_to_swe.last_kernel_ms  # the downgoing expansion's launch (the upgoing one is the same size)
+    terms = float(len(plist)) * len(kpb) * len(al)
+    tf = terms * P2S_FLOP_EQ_PER_TERM_33 / (ms * 1e-3) / 1e12
+    out = {'metric': 'non-plane-wave right-hand sides, %d spheres' % len(plist), 'dtype': 'f64',
+           'data': 'synthetic (seeded positions)',
+           'gaussian_beam': {'kernel': 'pwe_to_swe_kernel<3,3>', 'ms_per_expansion': ms, 'wall_s_both': wall,
+                             'n_kappa': len(kpb), 'n_alpha': len(al), 'terms': terms,
+                             'flop_eq_per_term': P2S_FLOP_EQ_PER_TERM_33, 'tflops_equiv': tf,
+                             'peak': MFMA_FP64_PEAK_TFLOPS, 'frac': tf / MFMA_FP64_PEAK_TFLOPS}}
+    log('gaussian beam: %.2f ms per expansion (%.1f TFLOP/s-eq), %.2f s wall' % (ms, tf, wall))
+    rng = np.random.default_rng(seed)
+    coll = inf.DipoleCollection(WL, k_parallel_array=kp)
+    for _ in range(n_dipoles):
+        pos = [rng.uniform(-side / 2, side / 2), rng.uniform(-side / 2, side / 2), 450.0]
+        coll.append(inf.DipoleSource(WL, list(rng.standard_normal(3) * 1e7), pos, k_parallel_array=kp))
+    coll.swe_coefficients(plist[:256], ls)  # warm-up
+    t0 = time.time()
+    coll.swe_coefficients(plist, ls)
+    wall = time.time() - t0
+    kms = lsm.explicit_blocks.last_kernel_ms
+    layer_flops = float(len(plist)) * n_dipoles * len(kp) * 900 * EXPLICIT_FLOPS_PER_TERM
+    out['dipoles'] = {'dipoles': n_dipoles, 'kernel_ms': kms, 'wall_s': wall,
+                      'tflops': layer_flops / (kms * 1e-3) / 1e12, 'n_kappa': len(kp)}
+    log('%d dipoles: %.2f ms kernels, %.2f s wall' % (n_dipoles, kms, wall))
+    # CPU: the oracle's Gaussian-beam conversion per particle
+    from oracle import initial as oin, layers as ol, system as osys
+    ls_o = ol.LayerSystem(ls.thicknesses, ls.refractive_indices)
+    beam_o = oin.GaussianBeam(WL, np.pi, 0.0, 0, 5000.0, kpb, al)
+    t0 = time.time()
+    for p in plist[:cpu_particles]:
+        osys.initial_field_coefficients(beam_o, osys.Sphere(p.position, 1.5, 100, 3, 3), ls_o)
+    per = (time.time() - t0) / cpu_particles
+    out['cpu_baseline'] = {'value': per * len(plist), 'unit': 's (extrapolated, Gaussian beam)', 'cores': 1,
+                           'kind': 'port', 'sample': '%d particles, %.2f s each' % (cpu_particles, per)}
+    log('oracle CPU: %.2f s per particle' % per)
+    return json.dumps(out)
+
+
 def cpu_baseline(table, rho, n_full, iters_per_solve, n_sample=400, seed=5, budget_s=20.0):
     """Time the oracle's restatement of smuthi's CPU radial-lookup matvec
     (CouplingMatrixRadialLookupCPU.matvec: per-(n1, n2) scipy interp1d 'cubic',
@@ -292,6 +358,8 @@ def main():
                     help='only run the copy calibration and three FP32 matvecs (for counter passes)')
     ap.add_argument('--explicit', type=int, default=0, metavar='N',
                     help='build the explicit coupling matrix of N spheres and time its matvecs (one JSON line)')
+    ap.add_argument('--rhs', action='store_true',
+                    help='time the Gaussian-beam and dipole right-hand sides at the ensemble (one JSON line)')
     ap.add_argument('--near-field', action='store_true',
                     help='time the SWE direct-sum and PWE near-field kernels instead (one JSON line)')
     args = ap.parse_args()
@@ -305,6 +373,8 @@ def main():
         return near_field_bench(args.particles)
     if args.explicit:
         return explicit_bench(args.explicit)
+    if args.rhs:
+        return rhs_bench(args.particles)
 
     from smuthi_amd import cuda
     from smuthi_amd.initial_field import PlaneWave

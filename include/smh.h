@@ -120,6 +120,21 @@ int smh_explicit_create(smh_ctx *ctx, int64_t n_part, const double *xyz, const i
                         int direct_all, const double *direct_coef, int64_t n_kappa, const double *kappa,
                         const double *kz, const double *layer_coef, double *build_ms, smh_op **out);
 
+/* Explicit blocks between two particle sets, returned to the host: the
+ * same W_ij as smh_explicit_create for receivers i = 0 .. n_recv - 1 and
+ * emitters j = n_recv .. n_recv + n_emit - 1 of one particle array (xyz,
+ * layer, z_rel of n_recv + n_emit entries), all padded to (l_max, m_max).
+ * With x == NULL, out receives W: complex128 [n_recv B][n_emit B]; else
+ * out = W x (x: complex128 [n_emit B], out: [n_recv B]) formed on the
+ * device. Replaces the per-source block calls of
+ * DipoleSource.spherical_wave_expansion (initial_field.py:390-411):
+ * (direct_coupling_block + layer_mediated_coupling_block)(particle, dipole)
+ * applied to the dipole's outgoing SWE. */
+int smh_explicit_blocks(smh_ctx *ctx, int64_t n_recv, int64_t n_emit, const double *xyz, int l_max, int m_max,
+                        const int *layer, int n_layers, const double *k_layer, const double *z_rel, int direct_all,
+                        const double *direct_coef, int64_t n_kappa, const double *kappa, const double *kz,
+                        const double *layer_coef, const double *x, double *out, double *build_ms);
+
 /* Build the radial table on the GPU in FP64 (replaces radial_coupling_lookup_table,
  * prepare_lookup.py:253-464, and its helper kernel, prepare_lookup_cuda.py:55-95):
  *   L[j,n1,n2] = c64(direct[j,n1,n2]) + sum_k J_|m1-m2|(kappa_k rho_j) * lm_coef[k,n1,n2]
@@ -198,6 +213,20 @@ int smh_pwe_assemble(smh_ctx *ctx, int64_t n_part, const double *xyz, int l_max,
 int smh_swe_field(smh_ctx *ctx, int64_t n_part, const double *pos, const double *radius, const double *zlim,
                   int l_max, int m_max, const double *coef, double k_re, double k_im, int64_t n_pts,
                   const double *pts, int field, double omega, int precision, double *out, double *kernel_ms);
+
+/* Plane wave expansion -> regular SWE at n_pts reference points
+ * (pwe_to_swe_conversion, fields/transformations.py:77-130, continuous
+ * branch), FP64:
+ *   out[p][n] = sum_k w_k e^{i kz_k dz} sum_pol bdag[k][pol][n]
+ *               sum_j w_a[j] e^{-i m_n alpha_j} coef[pol][k][j] e^{i kappa_k (cos alpha_j dx + sin alpha_j dy)}
+ * with (dx, dy, dz) = pts[p] (the points relative to the expansion's
+ * reference point), kz signed by the expansion's kind, w_k = 4 kappa_k times
+ * the kappa trapezoid weight, w_a the alpha trapezoid weights, bdag the
+ * B-dagger coefficients of (l_max, m_max) at (kz / k, kappa / k).
+ * kp, kz, w_k, bdag, coef, out: complex128; out [n_pts][blocksize]. */
+int smh_pwe_to_swe(smh_ctx *ctx, int64_t n_k, const double *kp, const double *kz, const double *w_k, int64_t n_a,
+                   const double *alpha, const double *w_a, const double *coef, const double *bdag, int l_max,
+                   int m_max, int64_t n_pts, const double *pts, double *out, double *kernel_ms);
 
 /* ---- measurement helper (no counterpart in smuthi) ----------------------
  * Copy `bytes` (a multiple of 16) between two fresh device buffers with one

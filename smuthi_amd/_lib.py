@@ -63,6 +63,10 @@ SIGNATURES = {
                                  _dp, _c.c_int, _c.c_double, _c.c_int, _dp, _dp]),
     'smh_explicit_create': (_c.c_int, [_p, _i64, _dp, _ip, _ip, _c.c_int, _c.c_int, _ip, _c.c_int, _dp, _dp,
                                        _c.c_int, _dp, _i64, _dp, _dp, _dp, _dp, _c.POINTER(_p)]),
+    'smh_explicit_blocks': (_c.c_int, [_p, _i64, _i64, _dp, _c.c_int, _c.c_int, _ip, _c.c_int, _dp, _dp, _c.c_int,
+                                       _dp, _i64, _dp, _dp, _dp, _dp, _dp, _dp]),
+    'smh_pwe_to_swe': (_c.c_int, [_p, _i64, _dp, _dp, _dp, _i64, _dp, _dp, _dp, _dp, _c.c_int, _c.c_int, _i64, _dp,
+                                  _dp, _dp]),
     'smh_bench_copy': (_c.c_int, [_p, _i64, _dp]),
     'smh_host_bessel_j': (_c.c_int, [_c.c_int, _i64, _dp, _dp]),
     'smh_host_spherical_hankel': (_c.c_int, [_c.c_int, _i64, _dp, _dp]),

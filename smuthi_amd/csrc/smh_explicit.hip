@@ -50,7 +50,8 @@ struct ExplArgs {
   const int* emit;       // emitters sorted by layer
   const int2* groups;    // (first index into emit, count <= kEG), one layer per group
   int n_groups;
-  int n_part;
+  int e0, n_emit;        // emitters are particles e0 .. e0 + n_emit - 1 (receivers 0 .. n_recv - 1)
+  int64_t ldw;           // row length of W: n_emit * B
   int B, L2, nj, n_layers;  // nj = 2 m_max + 1 Bessel orders
   const int* mdiff;      // [B*B] m2 - m1
   const double2* dcoef;  // [B*B][L2+1] direct coefficients (a5 / b5)
@@ -91,14 +92,14 @@ __global__ void explicit_phase_kernel(int64_t n_part, int64_t n_kappa, const int
   T[2 * t + 1] = make_double2(m.re, m.im);
 }
 
-// 2. Jt[r][j][k][n] = J_n(kappa_k rho_{i0 + r, j}) for one slab of receivers
-__global__ void explicit_bessel_kernel(int i0, int slab, int64_t n_part, int64_t n_kappa, int nj,
+// 2. Jt[r][j - e0][k][n] = J_n(kappa_k rho_{i0 + r, j}) for one slab of receivers
+__global__ void explicit_bessel_kernel(int i0, int slab, int e0, int64_t n_emit, int64_t n_kappa, int nj,
                                        const double* __restrict__ xyz, const double2* __restrict__ kappa,
                                        double2* __restrict__ Jt) {
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= (int64_t)slab * n_part * n_kappa) return;
+  if (t >= (int64_t)slab * n_emit * n_kappa) return;
   const int64_t rj = t / n_kappa, k = t - rj * n_kappa;
-  const int64_t r = rj / n_part, j = rj - r * n_part;
+  const int64_t r = rj / n_emit, j = e0 + (rj - r * n_emit);
   const int64_t i = i0 + r;
   const double dx = xyz[3 * i] - xyz[3 * j], dy = xyz[3 * i + 1] - xyz[3 * j + 1];
   const double rho = sqrt(dx * dx + dy * dy);
@@ -158,7 +159,7 @@ __global__ void __launch_bounds__(kThreads) explicit_direct_kernel(ExplArgs a) {
   }
   __syncthreads();
   const int BB = a.B * a.B, W = a.L2 + 1;
-  const int64_t ldw = (int64_t)a.n_part * a.B;
+  const int64_t ldw = a.ldw;
   for (int p = threadIdx.x; p < BB; p += blockDim.x) {
     const int md = a.mdiff[p], dm = md < 0 ? -md : md;
     const int n1 = p / a.B, n2 = p - n1 * a.B;
@@ -175,7 +176,7 @@ __global__ void __launch_bounds__(kThreads) explicit_direct_kernel(ExplArgs a) {
         }
         v = smh::zmul(v, phase_of(PH[e], md));
       }
-      a.W64[((int64_t)i * a.B + n1) * ldw + (int64_t)j * a.B + n2] = make_double2(v.re, v.im);
+      a.W64[((int64_t)i * a.B + n1) * ldw + (int64_t)(j - a.e0) * a.B + n2] = make_double2(v.re, v.im);
     }
   }
 }
@@ -205,7 +206,7 @@ __global__ void __launch_bounds__(kThreads)
   const double2* Tj[kEG];
 #pragma unroll
   for (int e = 0; e < kEG; ++e) Tj[e] = T + (size_t)emit[grp.x + (e < grp.y ? e : 0)] * nk * 2;
-  const double2* Jr = Jt + (size_t)r * a.n_part * nk * nj;
+  const double2* Jr = Jt + (size_t)r * a.n_emit * nk * nj;
   for (int p0 = 0; p0 < BB; p0 += blockDim.x) {
     const int p = p0 + threadIdx.x;
     const bool act = p < BB;
@@ -221,7 +222,7 @@ __global__ void __launch_bounds__(kThreads)
         const int e = t / (kKC * nj), q = t - e * (kKC * nj), kk = q / nj, n = q - kk * nj;
         const int j = jidx[e];
         zc v{0.0, 0.0};
-        if (j >= 0 && kk < kc) v = ld(Jr + ((size_t)j * nk + k0 + kk) * nj + n);
+        if (j >= 0 && kk < kc) v = ld(Jr + ((size_t)(j - a.e0) * nk + k0 + kk) * nj + n);
         J[e][kk][n] = v;
       }
       __syncthreads();
@@ -252,12 +253,12 @@ __global__ void __launch_bounds__(kThreads)
     }
     if (act) {
       const int n1 = p / a.B, n2 = p - n1 * a.B;
-      const int64_t ldw = (int64_t)a.n_part * a.B;
+      const int64_t ldw = a.ldw;
       for (int e = 0; e < kEG; ++e) {
         const int j = jidx[e];
         if (j < 0) break;
         const zc v = smh::zmul(acc[e], phase_of(PH[e], md));
-        double2* o = W64 + ((int64_t)i * a.B + n1) * ldw + (int64_t)j * a.B + n2;
+        double2* o = W64 + ((int64_t)i * a.B + n1) * ldw + (int64_t)(j - a.e0) * a.B + n2;
         const double2 w = *o;
         *o = make_double2(w.x + v.re, w.y + v.im);
       }
@@ -272,13 +273,13 @@ __global__ void to_c64_kernel(const double2* __restrict__ src, float2* __restric
   }
 }
 
-// y[r] = sum_c W[r][c] x[c]; one wavefront per row, 4 rows per block.
+// y[r] = sum_c W[r][c] x[c] for W [rows][n]; one wavefront per row, 4 rows per block.
 template <typename Real>
 __global__ void __launch_bounds__(256) explicit_gemv_kernel(const void* __restrict__ Wv, const void* __restrict__ xv,
-                                                           void* __restrict__ yv, int64_t n) {
+                                                           void* __restrict__ yv, int64_t rows, int64_t n) {
   const int lane = threadIdx.x & 63;
   const int64_t row = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
-  if (row >= n) return;
+  if (row >= rows) return;
   Real re = 0, im = 0;
   if constexpr (sizeof(Real) == 8) {
     const double2* W = (const double2*)Wv + row * n;
@@ -355,9 +356,11 @@ int explicit_apply(smh_op* op, int precision, const void* x_pad, void* y_pad, bo
   const dim3 grid((unsigned)((n + 3) / 4));
   if (time_kernel) SMH_HIP(ctx, hipEventRecord(ctx->ev0, s));
   if (precision == SMH_FP32)
-    hipLaunchKernelGGL(explicit_gemv_kernel<float>, grid, dim3(256), 0, s, (const void*)op->d_W32, x_pad, y_pad, n);
+    hipLaunchKernelGGL(explicit_gemv_kernel<float>, grid, dim3(256), 0, s, (const void*)op->d_W32, x_pad, y_pad, n,
+                       n);
   else
-    hipLaunchKernelGGL(explicit_gemv_kernel<double>, grid, dim3(256), 0, s, (const void*)op->d_W64, x_pad, y_pad, n);
+    hipLaunchKernelGGL(explicit_gemv_kernel<double>, grid, dim3(256), 0, s, (const void*)op->d_W64, x_pad, y_pad, n,
+                       n);
   SMH_HIP(ctx, hipGetLastError());
   if (time_kernel) {
     SMH_HIP(ctx, hipEventRecord(ctx->ev1, s));
@@ -379,6 +382,143 @@ int explicit_download(smh_op* op, double* W) {
 
 }  // namespace smh
 
+namespace {
+
+// The explicit blocks of receivers 0 .. n_recv - 1 and emitters e0 .. e0 +
+// n_emit - 1 of one particle set (n_tot = max(n_recv, e0 + n_emit) entries of
+// xyz / layer / z_rel) into the device matrix W64 [n_recv B][n_emit B].
+int explicit_build(smh_ctx* ctx, int64_t n_recv, int64_t e0, int64_t n_emit, const double* xyz, const int* layer,
+                   const double* z_rel, int l_max, int m_max, int n_layers, const double* k_layer, int direct_all,
+                   const double* direct_coef, int64_t n_kappa, const double* kappa, const double* kz,
+                   const double* layer_coef, double2* W64, float* build_ms) {
+  hipStream_t s = ctx->stream;
+  const int64_t n_tot = std::max<int64_t>(n_recv, e0 + n_emit);
+  const int B = smh_blocksize(l_max, m_max), BB = B * B, L2 = 2 * l_max;
+  std::vector<int> tau, ll, mm;
+  smh_index_tables(l_max, m_max, tau, ll, mm);
+  std::vector<int> mdiff(BB);
+  for (int n1 = 0; n1 < B; ++n1)
+    for (int n2 = 0; n2 < B; ++n2) mdiff[n1 * B + n2] = mm[n2] - mm[n1];
+  // emitters sorted by layer, cut into groups of kEG of one layer
+  std::vector<int> emit(n_emit);
+  for (int64_t j = 0; j < n_emit; ++j) emit[j] = (int)(e0 + j);
+  std::stable_sort(emit.begin(), emit.end(), [&](int x, int y) { return layer[x] < layer[y]; });
+  std::vector<int2> groups;
+  for (int64_t g0 = 0; g0 < n_emit;) {
+    int cnt = 1;
+    while (cnt < kEG && g0 + cnt < n_emit && layer[emit[g0 + cnt]] == layer[emit[g0]]) ++cnt;
+    groups.push_back(make_int2((int)g0, cnt));
+    g0 += cnt;
+  }
+  const int64_t blocks = n_recv * (int64_t)groups.size();
+  SMH_CHECK(ctx, blocks < (int64_t)1 << 31, SMH_ERR_ARG, "too many particle pairs");
+  std::vector<void*> tmp;
+  auto dev = [&](const void* src, size_t bytes) -> void* {
+    void* d = nullptr;
+    if (hipMalloc(&d, std::max<size_t>(bytes, 16)) != hipSuccess) return nullptr;
+    tmp.push_back(d);
+    if (src && hipMemcpyAsync(d, src, bytes, hipMemcpyHostToDevice, s) != hipSuccess) return nullptr;
+    return d;
+  };
+  auto release = [&]() {
+    for (void* p : tmp) (void)hipFree(p);
+    tmp.clear();
+  };
+  ExplArgs a;
+  a.xyz = (const double*)dev(xyz, n_tot * 3 * sizeof(double));
+  const double* d_zrel = (const double*)dev(z_rel, n_tot * sizeof(double));
+  a.layer = (const int*)dev(layer, n_tot * sizeof(int));
+  a.emit = (const int*)dev(emit.data(), n_emit * sizeof(int));
+  a.groups = (const int2*)dev(groups.data(), groups.size() * sizeof(int2));
+  a.mdiff = (const int*)dev(mdiff.data(), BB * sizeof(int));
+  a.dcoef = (const double2*)dev(direct_coef, (size_t)BB * (L2 + 1) * sizeof(double2));
+  a.klay = (const double2*)dev(k_layer, n_layers * sizeof(double2));
+  const int nj = 2 * m_max + 1;
+  // receivers per slab of the Bessel table: enough for 2048 blocks of the
+  // layer kernel, at most 8 GB of table
+  const int64_t per_recv = std::max<int64_t>(1, n_emit * n_kappa * nj * (int64_t)sizeof(double2));
+  const int64_t want = (2048 + (int64_t)groups.size() - 1) / (int64_t)groups.size();
+  const int slab = (int)std::max<int64_t>(1, std::min<int64_t>({n_recv, want, ((int64_t)8 << 30) / per_recv}));
+  const double2* d_kappa = nullptr;
+  const double2* d_kz = nullptr;
+  double2* d_T = nullptr;
+  double2* d_J = nullptr;
+  bool ok = a.xyz && d_zrel && a.layer && a.emit && a.groups && a.mdiff && a.dcoef && a.klay;
+  if (ok && n_kappa > 0) {
+    d_kappa = (const double2*)dev(kappa, n_kappa * sizeof(double2));
+    d_kz = (const double2*)dev(kz, (size_t)n_layers * n_kappa * sizeof(double2));
+    a.C = (const double2*)dev(layer_coef, (size_t)n_layers * n_layers * n_kappa * 4 * BB * sizeof(double2));
+    d_T = (double2*)dev(nullptr, (size_t)n_tot * n_kappa * 2 * sizeof(double2));
+    d_J = (double2*)dev(nullptr, (size_t)slab * per_recv);
+    ok = d_kappa && d_kz && a.C && d_T && d_J;
+  }
+  if (!ok) {
+    release();
+    return smh_fail(ctx, SMH_ERR_HIP, "device allocation failed (explicit coupling coefficients)");
+  }
+  a.n_groups = (int)groups.size();
+  a.e0 = (int)e0;
+  a.n_emit = (int)n_emit;
+  a.ldw = n_emit * B;
+  a.B = B;
+  a.L2 = L2;
+  a.nj = nj;
+  a.n_layers = n_layers;
+  a.direct_all = direct_all;
+  a.n_kappa = n_kappa;
+  a.W64 = W64;
+  const int threads = std::min(kThreads, (BB + 63) / 64 * 64);
+  hipEvent_t ev0, ev1;
+  SMH_HIP(ctx, hipEventCreate(&ev0));
+  SMH_HIP(ctx, hipEventCreate(&ev1));
+  SMH_HIP(ctx, hipEventRecord(ev0, s));
+  hipLaunchKernelGGL(explicit_direct_kernel, dim3((unsigned)blocks), dim3(threads), 0, s, a);
+  if (n_kappa > 0) {
+    const int64_t nt = n_tot * n_kappa;
+    hipLaunchKernelGGL(explicit_phase_kernel, dim3((unsigned)((nt + 255) / 256)), dim3(256), 0, s, n_tot, n_kappa,
+                       a.layer, d_zrel, d_kz, d_T);
+    for (int i0 = 0; i0 < n_recv; i0 += slab) {
+      const int cnt = (int)std::min<int64_t>(slab, n_recv - i0);
+      const int64_t nb = (int64_t)cnt * n_emit * n_kappa;
+      hipLaunchKernelGGL(explicit_bessel_kernel, dim3((unsigned)((nb + 255) / 256)), dim3(256), 0, s, i0, cnt,
+                         (int)e0, n_emit, n_kappa, nj, a.xyz, d_kappa, d_J);
+      a.i0 = i0;
+      hipLaunchKernelGGL(explicit_layer_kernel, dim3((unsigned)(cnt * a.n_groups)), dim3(threads), 0, s, a,
+                         (const double2*)d_T, (const double2*)d_J, a.C, a.emit, W64);
+    }
+  }
+  SMH_HIP(ctx, hipEventRecord(ev1, s));
+  const hipError_t le = hipGetLastError();
+  const hipError_t se = hipStreamSynchronize(s);
+  float ms = 0;
+  (void)hipEventElapsedTime(&ms, ev0, ev1);
+  (void)hipEventDestroy(ev0);
+  (void)hipEventDestroy(ev1);
+  release();
+  if (le != hipSuccess || se != hipSuccess)
+    return smh_fail(ctx, SMH_ERR_HIP, "explicit coupling kernels failed: %s",
+                    hipGetErrorString(le != hipSuccess ? le : se));
+  if (build_ms) *build_ms = ms;
+  return SMH_OK;
+}
+
+int explicit_check(smh_ctx* ctx, int64_t n_tot, const int* layer, int l_max, int m_max, int n_layers,
+                   const double* direct_coef, int64_t n_kappa, const double* kappa, const double* kz,
+                   const double* layer_coef) {
+  SMH_CHECK(ctx, l_max >= 1 && m_max >= 1 && m_max <= l_max && 2 * l_max <= kMaxL2 && 2 * m_max < kNJ, SMH_ERR_ARG,
+            "invalid l_max %d / m_max %d (explicit coupling: l_max <= 8, m_max <= 4)", l_max, m_max);
+  SMH_CHECK(ctx, n_layers >= 1, SMH_ERR_ARG, "need at least one layer");
+  SMH_CHECK(ctx, direct_coef != nullptr, SMH_ERR_ARG, "direct coefficients are NULL");
+  SMH_CHECK(ctx, n_kappa == 0 || (kappa && kz && layer_coef && n_kappa >= 2), SMH_ERR_ARG,
+            "layer-mediated part needs kappa, kz and coefficients");
+  for (int64_t i = 0; i < n_tot; ++i)
+    SMH_CHECK(ctx, layer[i] >= 0 && layer[i] < n_layers, SMH_ERR_ARG, "particle %lld: layer index %d out of range",
+              (long long)i, layer[i]);
+  return SMH_OK;
+}
+
+}  // namespace
+
 extern "C" int smh_explicit_create(smh_ctx* ctx, int64_t n_part, const double* xyz, const int* lmax,
                                    const int* mmax, int l_max, int m_max, const int* layer, int n_layers,
                                    const double* k_layer, const double* z_rel, int direct_all,
@@ -388,15 +528,9 @@ extern "C" int smh_explicit_create(smh_ctx* ctx, int64_t n_part, const double* x
   *out = nullptr;
   SMH_CHECK(ctx, n_part >= 1 && n_part < (1 << 24) && xyz && lmax && mmax && layer && z_rel && k_layer, SMH_ERR_ARG,
             "invalid particle arguments");
-  SMH_CHECK(ctx, l_max >= 1 && m_max >= 1 && m_max <= l_max && 2 * l_max <= kMaxL2 && 2 * m_max < kNJ, SMH_ERR_ARG,
-            "invalid l_max %d / m_max %d (explicit operator: l_max <= 8, m_max <= 4)", l_max, m_max);
-  SMH_CHECK(ctx, n_layers >= 1, SMH_ERR_ARG, "need at least one layer");
-  SMH_CHECK(ctx, direct_coef != nullptr, SMH_ERR_ARG, "direct coefficients are NULL");
-  SMH_CHECK(ctx, n_kappa == 0 || (kappa && kz && layer_coef && n_kappa >= 2), SMH_ERR_ARG,
-            "layer-mediated part needs kappa, kz and coefficients");
-  for (int64_t i = 0; i < n_part; ++i)
-    SMH_CHECK(ctx, layer[i] >= 0 && layer[i] < n_layers, SMH_ERR_ARG, "particle %lld: layer index %d out of range",
-              (long long)i, layer[i]);
+  if (int code = explicit_check(ctx, n_part, layer, l_max, m_max, n_layers, direct_coef, n_kappa, kappa, kz,
+                                layer_coef))
+    return code;
   SMH_HIP(ctx, hipSetDevice(ctx->device));
   smh_op* op = new smh_op();
   op->ctx = ctx;
@@ -413,117 +547,23 @@ extern "C" int smh_explicit_create(smh_ctx* ctx, int64_t n_part, const double* x
     return code;
   };
   if (int code = smh::op_index_map(op, lmax, mmax)) return fail(code);
-  hipStream_t s = ctx->stream;
-  const int B = op->B, BB = B * B, L2 = 2 * l_max;
-  const int64_t n = n_part * B;
-  std::vector<int> tau, ll, mm;
-  smh_index_tables(l_max, m_max, tau, ll, mm);
-  std::vector<int> mdiff(BB);
-  for (int n1 = 0; n1 < B; ++n1)
-    for (int n2 = 0; n2 < B; ++n2) mdiff[n1 * B + n2] = mm[n2] - mm[n1];
-  // emitters sorted by layer, cut into groups of kEG of one layer
-  std::vector<int> emit(n_part);
-  for (int64_t i = 0; i < n_part; ++i) emit[i] = (int)i;
-  std::stable_sort(emit.begin(), emit.end(), [&](int x, int y) { return layer[x] < layer[y]; });
-  std::vector<int2> groups;
-  for (int64_t g0 = 0; g0 < n_part;) {
-    int cnt = 1;
-    while (cnt < kEG && g0 + cnt < n_part && layer[emit[g0 + cnt]] == layer[emit[g0]]) ++cnt;
-    groups.push_back(make_int2((int)g0, cnt));
-    g0 += cnt;
-  }
-  const int64_t blocks = n_part * (int64_t)groups.size();
-  SMH_CHECK(ctx, blocks < (int64_t)1 << 31, SMH_ERR_ARG, "too many particle pairs");
-  std::vector<void*> tmp;
-  auto dev = [&](const void* src, size_t bytes) -> void* {
-    void* d = nullptr;
-    if (hipMalloc(&d, std::max<size_t>(bytes, 16)) != hipSuccess) return nullptr;
-    tmp.push_back(d);
-    if (src && hipMemcpyAsync(d, src, bytes, hipMemcpyHostToDevice, s) != hipSuccess) return nullptr;
-    return d;
-  };
-  auto release = [&]() {
-    for (void* p : tmp) (void)hipFree(p);
-    tmp.clear();
-  };
-  ExplArgs a;
-  a.xyz = (const double*)dev(xyz, n_part * 3 * sizeof(double));
-  const double* d_zrel = (const double*)dev(z_rel, n_part * sizeof(double));
-  a.layer = (const int*)dev(layer, n_part * sizeof(int));
-  a.emit = (const int*)dev(emit.data(), n_part * sizeof(int));
-  a.groups = (const int2*)dev(groups.data(), groups.size() * sizeof(int2));
-  a.mdiff = (const int*)dev(mdiff.data(), BB * sizeof(int));
-  a.dcoef = (const double2*)dev(direct_coef, (size_t)BB * (L2 + 1) * sizeof(double2));
-  a.klay = (const double2*)dev(k_layer, n_layers * sizeof(double2));
-  const int nj = 2 * m_max + 1;
-  // receivers per slab of the Bessel table: enough for 2048 blocks of the
-  // layer kernel, at most 8 GB of table
-  const int64_t per_recv = std::max<int64_t>(1, n_part * n_kappa * nj * (int64_t)sizeof(double2));
-  const int64_t want = (2048 + (int64_t)groups.size() - 1) / (int64_t)groups.size();
-  const int slab = (int)std::max<int64_t>(1, std::min<int64_t>({n_part, want, ((int64_t)8 << 30) / per_recv}));
-  const double2* d_kappa = nullptr;
-  const double2* d_kz = nullptr;
-  double2* d_T = nullptr;
-  double2* d_J = nullptr;
-  bool ok = a.xyz && d_zrel && a.layer && a.emit && a.groups && a.mdiff && a.dcoef && a.klay;
-  if (ok && n_kappa > 0) {
-    d_kappa = (const double2*)dev(kappa, n_kappa * sizeof(double2));
-    d_kz = (const double2*)dev(kz, (size_t)n_layers * n_kappa * sizeof(double2));
-    a.C = (const double2*)dev(layer_coef, (size_t)n_layers * n_layers * n_kappa * 4 * BB * sizeof(double2));
-    d_T = (double2*)dev(nullptr, (size_t)n_part * n_kappa * 2 * sizeof(double2));
-    d_J = (double2*)dev(nullptr, (size_t)slab * per_recv);
-    ok = d_kappa && d_kz && a.C && d_T && d_J;
-  }
-  ok = ok && hipMalloc(&op->d_W64, (size_t)n * n * sizeof(double2)) == hipSuccess &&
-       hipMalloc(&op->d_W32, (size_t)n * n * sizeof(float2)) == hipSuccess &&
-       hipMalloc(&op->d_x32, n * sizeof(float2)) == hipSuccess && hipMalloc(&op->d_y32, n * sizeof(float2)) == hipSuccess &&
-       hipMalloc(&op->d_x64, n * sizeof(double2)) == hipSuccess && hipMalloc(&op->d_y64, n * sizeof(double2)) == hipSuccess;
-  if (!ok) {
-    release();
+  const int64_t n = n_part * op->B;
+  if (hipMalloc(&op->d_W64, (size_t)n * n * sizeof(double2)) != hipSuccess ||
+      hipMalloc(&op->d_W32, (size_t)n * n * sizeof(float2)) != hipSuccess ||
+      hipMalloc(&op->d_x32, n * sizeof(float2)) != hipSuccess || hipMalloc(&op->d_y32, n * sizeof(float2)) != hipSuccess ||
+      hipMalloc(&op->d_x64, n * sizeof(double2)) != hipSuccess || hipMalloc(&op->d_y64, n * sizeof(double2)) != hipSuccess) {
     smh_fail(ctx, SMH_ERR_HIP, "device allocation failed (dense matrix of %lld x %lld needs %.1f GB)", (long long)n,
              (long long)n, (double)n * n * 24 / 1e9);
     return fail(SMH_ERR_HIP);
   }
-  a.n_groups = (int)groups.size();
-  a.n_part = (int)n_part;
-  a.B = B;
-  a.L2 = L2;
-  a.nj = nj;
-  a.n_layers = n_layers;
-  a.direct_all = direct_all;
-  a.n_kappa = n_kappa;
-  a.W64 = op->d_W64;
-  const int threads = std::min(kThreads, (BB + 63) / 64 * 64);
-  hipEvent_t e0, e1;
-  SMH_HIP(ctx, hipEventCreate(&e0));
-  SMH_HIP(ctx, hipEventCreate(&e1));
-  SMH_HIP(ctx, hipEventRecord(e0, s));
-  hipLaunchKernelGGL(explicit_direct_kernel, dim3((unsigned)(n_part * a.n_groups)), dim3(threads), 0, s, a);
-  if (n_kappa > 0) {
-    const int64_t nt = n_part * n_kappa;
-    hipLaunchKernelGGL(explicit_phase_kernel, dim3((unsigned)((nt + 255) / 256)), dim3(256), 0, s, n_part, n_kappa,
-                       a.layer, d_zrel, d_kz, d_T);
-    for (int i0 = 0; i0 < n_part; i0 += slab) {
-      const int cnt = (int)std::min<int64_t>(slab, n_part - i0);
-      const int64_t nb = (int64_t)cnt * n_part * n_kappa;
-      hipLaunchKernelGGL(explicit_bessel_kernel, dim3((unsigned)((nb + 255) / 256)), dim3(256), 0, s, i0, cnt, n_part,
-                         n_kappa, nj, a.xyz, d_kappa, d_J);
-      a.i0 = i0;
-      hipLaunchKernelGGL(explicit_layer_kernel, dim3((unsigned)(cnt * a.n_groups)), dim3(threads), 0, s, a,
-                         (const double2*)d_T, (const double2*)d_J, a.C, a.emit, op->d_W64);
-    }
-  }
-  hipLaunchKernelGGL(to_c64_kernel, dim3(4096), dim3(256), 0, s, (const double2*)op->d_W64, op->d_W32, n * n);
-  SMH_HIP(ctx, hipEventRecord(e1, s));
-  const hipError_t le = hipGetLastError();
-  const hipError_t se = hipStreamSynchronize(s);
   float ms = 0;
-  (void)hipEventElapsedTime(&ms, e0, e1);
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
-  release();
-  if (le != hipSuccess || se != hipSuccess) {
-    smh_fail(ctx, SMH_ERR_HIP, "explicit coupling kernels failed: %s", hipGetErrorString(le != hipSuccess ? le : se));
+  if (int code = explicit_build(ctx, n_part, 0, n_part, xyz, layer, z_rel, l_max, m_max, n_layers, k_layer,
+                                direct_all, direct_coef, n_kappa, kappa, kz, layer_coef, op->d_W64, &ms))
+    return fail(code);
+  hipLaunchKernelGGL(to_c64_kernel, dim3(4096), dim3(256), 0, ctx->stream, (const double2*)op->d_W64, op->d_W32, n * n);
+  const hipError_t e = hipStreamSynchronize(ctx->stream);
+  if (e != hipSuccess) {
+    smh_fail(ctx, SMH_ERR_HIP, "explicit matrix conversion failed: %s", hipGetErrorString(e));
     return fail(SMH_ERR_HIP);
   }
   if (build_ms) *build_ms = ms;
@@ -531,4 +571,49 @@ extern "C" int smh_explicit_create(smh_ctx* ctx, int64_t n_part, const double* x
   ctx->live_ops++;
   *out = op;
   return SMH_OK;
+}
+
+extern "C" int smh_explicit_blocks(smh_ctx* ctx, int64_t n_recv, int64_t n_emit, const double* xyz, int l_max,
+                                   int m_max, const int* layer, int n_layers, const double* k_layer,
+                                   const double* z_rel, int direct_all, const double* direct_coef, int64_t n_kappa,
+                                   const double* kappa, const double* kz, const double* layer_coef, const double* x,
+                                   double* out, double* build_ms) {
+  if (!ctx) return SMH_ERR_ARG;
+  SMH_CHECK(ctx, n_recv >= 1 && n_emit >= 1 && n_recv + n_emit < (1 << 24) && xyz && layer && z_rel && k_layer && out,
+            SMH_ERR_ARG, "invalid particle arguments");
+  if (int code = explicit_check(ctx, n_recv + n_emit, layer, l_max, m_max, n_layers, direct_coef, n_kappa, kappa, kz,
+                                layer_coef))
+    return code;
+  SMH_HIP(ctx, hipSetDevice(ctx->device));
+  const int B = smh_blocksize(l_max, m_max);
+  const size_t bytes = (size_t)n_recv * B * n_emit * B * sizeof(double2);
+  double2* d_W = nullptr;
+  SMH_CHECK(ctx, hipMalloc(&d_W, bytes) == hipSuccess, SMH_ERR_HIP, "device allocation failed (%.1f GB of blocks)",
+            bytes / 1e9);
+  float ms = 0;
+  int code = explicit_build(ctx, n_recv, n_recv, n_emit, xyz, layer, z_rel, l_max, m_max, n_layers, k_layer,
+                            direct_all, direct_coef, n_kappa, kappa, kz, layer_coef, d_W, &ms);
+  if (code == SMH_OK && !x && hipMemcpy(out, d_W, bytes, hipMemcpyDeviceToHost) != hipSuccess)
+    code = smh_fail(ctx, SMH_ERR_HIP, "block download failed");
+  if (code == SMH_OK && x) {
+    // out = W x on the device
+    const int64_t rows = n_recv * B, cols = n_emit * B;
+    double2 *d_x = nullptr, *d_y = nullptr;
+    hipError_t e = hipMalloc(&d_x, cols * sizeof(double2));
+    if (e == hipSuccess) e = hipMalloc(&d_y, rows * sizeof(double2));
+    if (e == hipSuccess) e = hipMemcpyAsync(d_x, x, cols * sizeof(double2), hipMemcpyHostToDevice, ctx->stream);
+    if (e == hipSuccess) {
+      hipLaunchKernelGGL(explicit_gemv_kernel<double>, dim3((unsigned)((rows + 3) / 4)), dim3(256), 0, ctx->stream,
+                         (const void*)d_W, (const void*)d_x, (void*)d_y, rows, cols);
+      e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipMemcpyAsync(out, d_y, rows * sizeof(double2), hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    if (d_x) (void)hipFree(d_x);
+    if (d_y) (void)hipFree(d_y);
+    if (e != hipSuccess) code = smh_fail(ctx, SMH_ERR_HIP, "block product failed: %s", hipGetErrorString(e));
+  }
+  (void)hipFree(d_W);
+  if (code == SMH_OK && build_ms) *build_ms = ms;
+  return code;
 }

@@ -215,6 +215,78 @@ class CouplingMatrixRadialLookupCUDA(_GpuCouplingOperator):
         return self
 
 
+def _explicit_inputs(vacuum_wavelength, positions, layer_system, k_parallel, l_max, m_max):
+    """Host arrays of the explicit coupling kernels for particles at `positions`:
+    layer index among the occupied layers, each layer's wavenumber, heights
+    above the layer's reference, direct coefficients and, unless the layer
+    system is one medium, the per-layer-pair layer-mediated coefficients with
+    the explicit quadrature's weights (its last contour interval left out)."""
+    if isinstance(k_parallel, str) and k_parallel == 'default':
+        k_parallel = flds.default_Sommerfeld_k_parallel_array
+    layers = [layer_system.layer_number(p[2]) for p in positions]
+    occupied = sorted(set(layers))
+    lidx = {l: q for q, l in enumerate(occupied)}
+    out = dict(
+        layer=np.ascontiguousarray([lidx[l] for l in layers], dtype=np.int32),
+        n_layers=len(occupied),
+        k_layer=np.ascontiguousarray([complex(layer_system.wavenumber(l, vacuum_wavelength)) for l in occupied],
+                                     dtype=np.complex128),
+        z_rel=np.ascontiguousarray([p[2] - layer_system.reference_z(l) for p, l in zip(positions, layers)],
+                                   dtype=np.float64),
+        xyz=np.ascontiguousarray(positions, dtype=np.float64),
+        direct_all=int(layer_system.is_degenerate()),
+        coef=np.ascontiguousarray(look.direct_coefficients_volume(l_max, m_max)),
+        kp=None, kz=None, lc=None, n_kappa=0)
+    if not layer_system.is_degenerate():
+        kp = np.ascontiguousarray(k_parallel, dtype=np.complex128)
+        w = look.trapezoid_weights_drop_last(kp)
+        nl = len(occupied)
+        B = flds.blocksize(l_max, m_max)
+        lc = np.empty((nl, nl, len(kp), 4, B, B), dtype=np.complex128)
+        kz = np.empty((nl, len(kp)), dtype=np.complex128)
+        for a, l1 in enumerate(occupied):
+            for b, l2 in enumerate(occupied):
+                lc[a, b], kz1, _ = look.layer_coefficients_pair(vacuum_wavelength, layer_system, kp, l1, l2, l_max,
+                                                                 m_max, w)
+            kz[a] = kz1
+        out.update(kp=kp, kz=kz, lc=lc, n_kappa=len(kp))
+    return out
+
+
+def explicit_blocks(vacuum_wavelength, receivers, emitters, layer_system, k_parallel, l_max, m_max, x=None):
+    """Explicit coupling blocks W[i, :, j, :] (receiver i, emitter j) of two
+    position sets, padded to (l_max, m_max): complex128 [n_recv, B, n_emit, B],
+    built on the GPU (direct_coupling_block + layer_mediated_coupling_block,
+    direct_coupling.py:18-105, layer_mediated_coupling.py:215-340). With x
+    ([n_emit, B]) the product W x ([n_recv, B]) is returned instead, formed on
+    the device."""
+    receivers = np.asarray(receivers, dtype=float).reshape(-1, 3)
+    emitters = np.asarray(emitters, dtype=float).reshape(-1, 3)
+    B = flds.blocksize(l_max, m_max)
+    h = _explicit_inputs(vacuum_wavelength, np.concatenate([receivers, emitters]), layer_system, k_parallel, l_max,
+                         m_max)
+    if x is None:
+        out = np.empty((len(receivers) * B, len(emitters) * B), dtype=np.complex128)
+    else:
+        x = np.ascontiguousarray(x, dtype=np.complex128).reshape(len(emitters) * B)
+        out = np.empty(len(receivers) * B, dtype=np.complex128)
+    ms = ctypes.c_double()
+    ctx = cu.context()
+    _lib.check(_lib.load().smh_explicit_blocks(ctx, len(receivers), len(emitters), _lib.dptr(h['xyz']), l_max, m_max,
+                                               _lib.iptr(h['layer']), h['n_layers'], _lib.dptr(h['k_layer']),
+                                               _lib.dptr(h['z_rel']), h['direct_all'], _lib.dptr(h['coef']),
+                                               h['n_kappa'], _lib.dptr(h['kp']), _lib.dptr(h['kz']),
+                                               _lib.dptr(h['lc']), _lib.dptr(x), _lib.dptr(out), ctypes.byref(ms)),
+               ctx, 'smh_explicit_blocks')
+    explicit_blocks.last_kernel_ms = ms.value
+    if x is None:
+        return out.reshape(len(receivers), B, len(emitters), B)
+    return out.reshape(len(receivers), B)
+
+
+explicit_blocks.last_kernel_ms = 0.0
+
+
 class CouplingMatrixExplicit(_GpuCouplingOperator):
     """Explicit (dense) coupling matrix built on the MI355X; drop-in for
     smuthi's class of the same name (linear_system.py:441-486).
@@ -232,44 +304,19 @@ class CouplingMatrixExplicit(_GpuCouplingOperator):
         if use_pvwf_coupling:
             raise NotImplementedError('plane-wave mediated direct coupling is not built on the GPU path')
         self._init_common(particle_list)
-        if isinstance(k_parallel, str) and k_parallel == 'default':
-            k_parallel = flds.default_Sommerfeld_k_parallel_array
-        layers = [layer_system.layer_number(p.position[2]) for p in particle_list]
-        occupied = sorted(set(layers))
-        lidx = {l: q for q, l in enumerate(occupied)}
-        layer = np.ascontiguousarray([lidx[l] for l in layers], dtype=np.int32)
-        k_layer = np.ascontiguousarray([complex(layer_system.wavenumber(l, vacuum_wavelength)) for l in occupied],
-                                       dtype=np.complex128)
-        z_rel = np.ascontiguousarray([p.position[2] - layer_system.reference_z(l)
-                                      for p, l in zip(particle_list, layers)], dtype=np.float64)
-        xyz = np.ascontiguousarray([p.position for p in particle_list], dtype=np.float64)
         t0 = time.time()
-        coef = np.ascontiguousarray(look.direct_coefficients_volume(self.l_max, self.m_max))
-        kp = kz = lc = None
-        n_kappa = 0
-        if not layer_system.is_degenerate():
-            kp = np.ascontiguousarray(k_parallel, dtype=np.complex128)
-            n_kappa = len(kp)
-            w = look.trapezoid_weights_drop_last(kp)
-            nl = len(occupied)
-            B = self.blocksize
-            lc = np.empty((nl, nl, n_kappa, 4, B, B), dtype=np.complex128)
-            kz = np.empty((nl, n_kappa), dtype=np.complex128)
-            for a, l1 in enumerate(occupied):
-                for b, l2 in enumerate(occupied):
-                    lc[a, b], kz1, _ = look.layer_coefficients_pair(vacuum_wavelength, layer_system, kp, l1, l2,
-                                                                     self.l_max, self.m_max, w)
-                kz[a] = kz1
+        h = _explicit_inputs(vacuum_wavelength, [p.position for p in particle_list], layer_system, k_parallel,
+                             self.l_max, self.m_max)
         self.host_seconds = time.time() - t0
         lib = _lib.load()
         ctx = cu.context()
         handle = ctypes.c_void_p()
         ms = ctypes.c_double()
-        _lib.check(lib.smh_explicit_create(ctx, len(particle_list), _lib.dptr(xyz), _lib.iptr(self._lmax),
-                                           _lib.iptr(self._mmax), self.l_max, self.m_max, _lib.iptr(layer),
-                                           len(occupied), _lib.dptr(k_layer), _lib.dptr(z_rel),
-                                           int(layer_system.is_degenerate()), _lib.dptr(coef), n_kappa,
-                                           _lib.dptr(kp), _lib.dptr(kz), _lib.dptr(lc), ctypes.byref(ms),
+        _lib.check(lib.smh_explicit_create(ctx, len(particle_list), _lib.dptr(h['xyz']), _lib.iptr(self._lmax),
+                                           _lib.iptr(self._mmax), self.l_max, self.m_max, _lib.iptr(h['layer']),
+                                           h['n_layers'], _lib.dptr(h['k_layer']), _lib.dptr(h['z_rel']),
+                                           h['direct_all'], _lib.dptr(h['coef']), h['n_kappa'], _lib.dptr(h['kp']),
+                                           _lib.dptr(h['kz']), _lib.dptr(h['lc']), ctypes.byref(ms),
                                            ctypes.byref(handle)), ctx, 'smh_explicit_create')
         self._op = handle
         self.build_kernel_ms = ms.value

@@ -154,3 +154,45 @@ def test_explicit_layer_coefficients_reproduce_oracle_blocks():
         J = scipy.special.jv(np.abs(mm[:, None] - mm[None, :])[None], kp[:, None, None] * rho)
         got = (J * S).sum(0) * np.exp(1j * (mm[None, :] - mm[:, None]) * phi)
         assert np.abs(got - want).max() < 1e-12 * np.abs(want).max(), (is1, is2)
+
+
+def test_gaussian_beam_expansion_matches_oracle():
+    # host side of GaussianBeam: the angular spectrum and its layer response in every layer
+    from oracle import initial as oin
+    from smuthi_amd import initial_field as inf
+    wl = 550.0
+    thick, nidx = [0, 250, 0], [1.5, 1.9 + 0.02j, 1.0]
+    kp = ob.reasonable_Sommerfeld_kpar_contour(wl, layer_refractive_indices=[1.5, 1.9, 1.0], neff_resolution=1e-2)
+    al = np.linspace(0, 2 * np.pi, 73)
+    for polar, pol in ((0.3, 0), (np.pi - 0.4, 1)):
+        ob_ = oin.GaussianBeam(wl, polar, 0.7, pol, 1500.0, kp, al, amplitude=1.3, reference_point=[40, -30, 120])
+        pb = inf.GaussianBeam(wl, polar, 0.7, pol, 1500.0, k_parallel_array=kp, azimuthal_angles_array=al,
+                              amplitude=1.3, reference_point=[40, -30, 120])
+        for i in range(3):
+            wants = ob_.plane_wave_expansion(ol.LayerSystem(thick, nidx), i)
+            gots = pb.plane_wave_expansion(lay.LayerSystem(thick, nidx), i)
+            # scale of the layer's field: an expansion that is zero in exact arithmetic (the wave
+            # entering an outer half-space from outside) holds transfer-matrix roundoff on both sides
+            scale = max(np.abs(w.coefficients).max() for w in wants)
+            for want, got in zip(wants, gots):
+                assert np.abs(got.coefficients - want.coefficients).max() <= 1e-12 * scale, (polar, i)
+                assert got.kind == want.kind and list(got.reference_point) == list(want.reference_point)
+
+
+def test_dipole_outgoing_expansion_matches_oracle():
+    from oracle import initial as oin
+    from smuthi_amd import initial_field as inf
+    ls_o, ls = ol.LayerSystem([0, 200, 0], [1.5, 2, 1]), lay.LayerSystem([0, 200, 0], [1.5, 2, 1])
+    for pos in ([30, -20, 100], [0, 0, -50], [5, 5, 400]):
+        d = inf.DipoleSource(550, [1e7, -2e7, 3e7 + 1e6j], pos, k_parallel_array=np.array([0, 1.0]))
+        do = oin.DipoleSource(550, [1e7, -2e7, 3e7 + 1e6j], pos, np.array([0, 1.0]))
+        want = do.outgoing_coefficients(ls_o)
+        assert np.abs(d.outgoing_spherical_wave_expansion(ls).coefficients - want).max() <= 1e-15 * np.abs(want).max()
+
+
+def test_trapezoid_weights_are_numpy_trapz():
+    from smuthi_amd import initial_field as inf
+    rng = np.random.default_rng(1)
+    x = np.cumsum(rng.uniform(0.1, 1, 30)) * (1 - 0.05j)
+    y = rng.standard_normal(30) + 1j * rng.standard_normal(30)
+    assert abs(np.sum(inf._trapezoid_weights(x) * y) - np.trapezoid(y, x)) < 1e-13 * np.abs(y).sum()

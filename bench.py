@@ -8,9 +8,10 @@ gaps, on glass (n = 1.52) in air, lambda = 550 nm, l_max = m_max = 3
 interpolation at 2 nm, Sommerfeld contour with neff_resolution = neff_imag =
 2e-3. A step solves a batch of right-hand sides (plane waves; incidence angle and
 polarisation vary; 8 per step by default) to relative residual 1e-6: GMRES
-for the whole batch in lockstep whose coupling matvecs run in FP32 on MFMA
-(one launch per 8 RHS), wrapped in FP64 iterative refinement with FP64
-residuals. The coefficients of right-hand side 0 are compared with a plain
+for the whole batch in lockstep whose coupling matvecs run with FP16
+operands on MFMA (one launch per 8 RHS; --precision mixed: FP32), wrapped in
+FP64 iterative refinement (FP32 residuals between sweeps, the FP64 residual
+certifies the result). The coefficients of right-hand side 0 are compared with a plain
 FP64 GPU GMRES solve (tol 1e-9) of the same right-hand side; single-RHS
 solve and plain FP64 GMRES throughputs are reported beside.
 
@@ -33,6 +34,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 MFMA_FP32_PEAK_TFLOPS = 157.3
+MFMA_FP16_PEAK_TFLOPS = 2500.0
 MFMA_FP64_PEAK_TFLOPS = 78.6
 HBM_PEAK_GBS = 8000.0
 WL = 550.0
@@ -66,14 +68,16 @@ def rhs_for_step(j):
 
 
 TRAFFIC_FILE = os.path.join(ROOT, 'profiles', 'radial_matvec_10k_cubic_traffic.md')
+TRAFFIC_FILE16 = os.path.join(ROOT, 'profiles', 'radial_matvec_10k_fp16_8rhs.md')
 
 
 def measured_traffic(kind='single'):
-    """HBM bytes per FP32 matvec launch (single-RHS or 8-RHS kernel) from the
-    committed counter summary, or None."""
-    key = 'traffic_bytes_per_launch' if kind == 'single' else 'traffic_bytes_per_launch_8rhs'
+    """HBM bytes per matvec launch (single-RHS FP32, 8-RHS FP32 or 8-RHS FP16
+    kernel) from the committed counter summary, or None."""
+    key = {'single': 'traffic_bytes_per_launch', 'batch': 'traffic_bytes_per_launch_8rhs',
+           'batch16': 'traffic_bytes_per_launch_8rhs_fp16'}[kind]
     try:
-        for line in open(TRAFFIC_FILE):
+        for line in open(TRAFFIC_FILE16 if kind == 'batch16' else TRAFFIC_FILE):
             if line.split('=')[0].strip() == key:
                 return float(line.split('=')[1])
     except OSError:
@@ -82,9 +86,10 @@ def measured_traffic(kind='single'):
 
 
 def traffic_probe(n_particles, copy_bytes=512 << 20):
-    """Run a plain copy of known size, then three single-RHS and three 8-RHS
-    FP32 matvec launches of the bench operator (one warm-up + two timed each):
-    the workload the FETCH_SIZE / WRITE_SIZE counter passes are taken on."""
+    """Run a plain copy of known size, then three single-RHS FP32, three 8-RHS
+    FP32 and three 8-RHS FP16 matvec launches of the bench operator (one
+    warm-up + two timed each): the workload the FETCH_SIZE / WRITE_SIZE
+    counter passes are taken on."""
     import ctypes
     from smuthi_amd import _lib, cuda
     from smuthi_amd.linear_system import CouplingMatrixRadialLookupCUDA
@@ -97,7 +102,9 @@ def traffic_probe(n_particles, copy_bytes=512 << 20):
                'smh_bench_copy')
     k, _ = op.bench_matvec('single', iters=2)
     kb, _ = op.bench_matvec_batch('single', iters=2)
-    log('copy %d bytes: %.3f ms; FP32 matvec kernel %.2f ms, 8-RHS kernel %.2f ms' % (copy_bytes, ms.value, k, kb))
+    kh, _ = op.bench_matvec_batch('half', iters=2)
+    log('copy %d bytes: %.3f ms; FP32 matvec kernel %.2f ms, 8-RHS FP32 kernel %.2f ms, 8-RHS FP16 kernel %.2f ms'
+        % (copy_bytes, ms.value, k, kb, kh))
 
 
 # flops per (point, particle) pair of the SWE direct sum at l_max = m_max = 3: 38 per (l, m) for both
@@ -352,6 +359,8 @@ def main():
     ap.add_argument('--particles', type=int, default=10000)
     ap.add_argument('--tol', type=float, default=1e-6)
     ap.add_argument('--batch', type=int, default=8, help='right-hand sides solved together per step')
+    ap.add_argument('--precision', default='mixed16', choices=['mixed16', 'mixed', 'double'],
+                    help='inner matvec precision of the timed solves (mixed16: FP16 operands, mixed: FP32)')
     ap.add_argument('--no-cpu-baseline', action='store_true')
     ap.add_argument('--matvec-iters', type=int, default=5)
     ap.add_argument('--traffic-probe', action='store_true',
@@ -413,12 +422,14 @@ def main():
     k32, tot32 = op.bench_matvec('single', iters=args.matvec_iters)
     k64, tot64 = op.bench_matvec('double', iters=max(2, args.matvec_iters // 2))
     kb32, totb32 = op.bench_matvec_batch('single', iters=max(2, args.matvec_iters // 2))
+    kb16, totb16 = op.bench_matvec_batch('half', iters=max(2, args.matvec_iters // 2))
     kb64, totb64 = op.bench_matvec_batch('double', iters=2)
     n_unk = op.shape[0]
     E = float(n_unk) ** 2
     flops_1 = E * (4 * 4 + 8 * 1)  # cubic: 4 taps per complex entry, one RHS
     flops_8 = E * (4 * 4 + 8 * 8)  # the same entries applied to 8 RHS
-    log('matvec FP32 %.2f ms, FP64 %.2f ms; 8-RHS FP32 %.2f ms, FP64 %.2f ms' % (k32, k64, kb32, kb64))
+    log('matvec FP32 %.2f ms, FP64 %.2f ms; 8-RHS FP16 %.2f ms, FP32 %.2f ms, FP64 %.2f ms'
+        % (k32, k64, kb16, kb32, kb64))
 
     def solve(j0, count, precision, tol):
         sysj = make_system(j0, precision, tol)
@@ -441,26 +452,27 @@ def main():
     single_s = time.time() - t0
     log('single-RHS mixed solve: %.2f s' % single_s)
 
-    # timed region: batched mixed-precision solves
+    # timed region: batched solves at the requested inner precision
+    prec = args.precision
     for w in range(args.warmup):
-        solve(nb * w, nb, 'mixed', args.tol)
+        solve(nb * w, nb, prec, args.tol)
     t0 = time.time()
     sols, stats = [], []
     for st in range(args.warmup, args.warmup + args.steps):
-        x, stt = solve(nb * st, nb, 'mixed', args.tol)
+        x, stt = solve(nb * st, nb, prec, args.tol)
         sols.append(x)
         stats.append(stt)
     elapsed = time.time() - t0
     # accuracy: RHS 0 solved inside a batch vs the FP64 solution
-    x_b0 = solve(0, nb, 'mixed', args.tol)[0][0]
+    x_b0 = solve(0, nb, prec, args.tol)[0][0]
     coef_err = float(np.linalg.norm(x_b0 - x_ref) / np.linalg.norm(x_ref))
     single_err = float(np.linalg.norm(x_single[0] - x_ref) / np.linalg.norm(x_ref))
     worst_res = max(s['relative_residual'] for s in stats)
     ok = coef_err <= 1e-5 and worst_res <= args.tol
     n_solved = nb * args.steps
     value = n_solved / elapsed if ok else 0.0
-    log('batched mixed solves: %.2f s for %d RHS, residual %.2e, coefficient error vs FP64 %.2e'
-        % (elapsed, n_solved, worst_res, coef_err))
+    log('batched %s solves: %.2f s for %d RHS, residual %.2e, coefficient error vs FP64 %.2e'
+        % (prec, elapsed, n_solved, worst_res, coef_err))
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -469,8 +481,10 @@ def main():
         log('cpu baseline %.1f s' % (time.time() - t0))
 
     batched = nb > 1
-    kern_ms = kb32 if batched else k32
+    half = prec == 'mixed16'
+    kern_ms = (kb16 if half else kb32) if batched else k32
     achieved = (flops_8 if batched else flops_1) / (kern_ms * 1e-3) / 1e12
+    peak = MFMA_FP16_PEAK_TFLOPS if (half and batched) else MFMA_FP32_PEAK_TFLOPS
     out = {
         'metric': 'RHS-solves/s at 10k spheres, err<=1e-5 vs FP64; coupling matvec GB/s',
         'value': value,
@@ -482,18 +496,22 @@ def main():
         'higher_is_better': True,
         'scaling': 'weak',
         'vs_baseline': None,
-        'dtype': 'fp32 coupling matvec (MFMA), fp64 refinement/Arnoldi/residual',
+        'dtype': ('fp16 operands / fp32 accumulation coupling matvec (MFMA), fp32 sweep residuals, '
+                  'fp64 refinement/Arnoldi/certifying residual' if half else
+                  'fp32 coupling matvec (MFMA), fp64 refinement/Arnoldi/residual'),
         'data': 'synthetic',
         'config': {
             'workload': ('radial-lookup coupling solve, %d spheres monolayer on glass, l_max=3, cubic spline 2 nm, '
-                         'GMRES tol %.0e, FP32-in-FP64 refinement, %d right-hand sides (plane waves) per step'
-                         % (len(plist), args.tol, nb)),
+                         'GMRES tol %.0e, %s-in-FP64 refinement, %d right-hand sides (plane waves) per step'
+                         % (len(plist), args.tol, 'FP16' if half else ('FP32' if prec == 'mixed' else 'no'), nb)),
+            'inner_precision': prec,
             'particles': len(plist), 'unknowns': n_unk, 'lookup_rows': int(len(op.radial_distance_array)),
             'n_kappa': int(len(kp)), 'side_nm': side, 'seed': 2, 'tolerance': args.tol,
             'interpolation': 'cubic (not-a-knot spline)', 'resolution_nm': 2.0, 'rhs_per_step': nb,
         },
         'coupling_matvec_gbps_equiv_dense': E * 8 / (k32 * 1e-3) / 1e9,
         'matvec_ms': {'fp32_kernel': k32, 'fp32_total': tot32, 'fp64_kernel': k64, 'fp64_total': tot64,
+                      'fp16_kernel_8rhs': kb16, 'fp16_total_8rhs': totb16,
                       'fp32_kernel_8rhs': kb32, 'fp32_total_8rhs': totb32, 'fp64_kernel_8rhs': kb64,
                       'fp64_total_8rhs': totb64},
         'coef_err_vs_fp64': coef_err,
@@ -505,12 +523,16 @@ def main():
         'speedup_vs_fp64_gmres': (value * fp64_plain_s) if value else None,
         'prepare_seconds': prep_s,
         'table_seconds': op.table_seconds,
-        'roofline': {'bound': 'mfma', 'achieved': achieved, 'peak': MFMA_FP32_PEAK_TFLOPS, 'unit': 'TFLOP/s',
-                     'frac': achieved / MFMA_FP32_PEAK_TFLOPS,
-                     'traffic': measured_traffic('batch' if batched else 'single'),
-                     'kernel': ('batch_slots_kernel<3,3,4>' if batched
+        'roofline': {'bound': 'mfma', 'achieved': achieved, 'peak': peak, 'unit': 'TFLOP/s',
+                     'frac': achieved / peak,
+                     'traffic': measured_traffic(('batch16' if half else 'batch') if batched else 'single'),
+                     'kernel': (('batch_half_kernel<3,3,4>' if half else 'batch_slots_kernel<3,3,4>') if batched
                                 else 'lookup_matvec_kernel<float,float,3,3,4,false>'),
                      'flops_per_launch': flops_8 if batched else flops_1},
+        'roofline_fp32_8rhs_kernel': {'kernel': 'batch_slots_kernel<3,3,4>', 'ms': kb32,
+                                      'achieved': flops_8 / (kb32 * 1e-3) / 1e12, 'peak': MFMA_FP32_PEAK_TFLOPS,
+                                      'frac': flops_8 / (kb32 * 1e-3) / 1e12 / MFMA_FP32_PEAK_TFLOPS,
+                                      'traffic': measured_traffic('batch')},
         'roofline_single_rhs_kernel': {'achieved': flops_1 / (k32 * 1e-3) / 1e12, 'peak': MFMA_FP32_PEAK_TFLOPS,
                                        'frac': flops_1 / (k32 * 1e-3) / 1e12 / MFMA_FP32_PEAK_TFLOPS,
                                        'traffic': measured_traffic('single')},

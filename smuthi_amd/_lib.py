@@ -24,9 +24,11 @@ INTERP_SPLINE = 2
 
 FP32 = 0
 FP64 = 1
+FP16 = 2
 
 SOLVE_FP64 = 0
 SOLVE_MIXED = 1
+SOLVE_MIXED16 = 2
 
 _ERR_NAMES = {1: 'invalid argument', 2: 'HIP error', 3: 'unsupported', 4: 'invalid state', 5: 'not converged'}
 

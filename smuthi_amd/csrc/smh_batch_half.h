@@ -1,0 +1,248 @@
+// FP16-input lookup matvec for kBatch right-hand sides in slot order.
+//
+// The product of batch_slots_kernel (per pair: interpolate the lookup block,
+// apply it to all right-hand sides, rotate, accumulate in fixed point) with
+// FP16 operands on the 16x16x16 FP16 MFMA (FP32 accumulation):
+//
+//   * the bin's interpolant is stored as FP16 polynomial coefficients scaled
+//     by a per-bin power of two (hscale[bin]; |value| <= 2^12), evaluated by
+//     Horner in packed FP16 (two slots per instruction);
+//   * the rotated input e^{i m2 phi} x / xmax is formed in FP32 and packed to
+//     FP16 per 4 slots;
+//   * one MFMA covers 16 slots (4 slot groups): lane (q, rr) holds the 4 slots
+//     of group 4 c + q of chunk c as its K values, so its azimuthal phase
+//     is chosen per lane from the |m| values of the chunk's groups (one-hot
+//     weights, as the receiver rotation of batch_slots_kernel);
+//   * result rows, the receiver rotation, the fixed-point accumulation and the
+//     partial-sum layout are those of batch_slots_kernel, with the scales
+//     hscale[bin] * xmax folded into the rotation.
+//
+// The operands carry FP16 rounding (relative 2^-11 of the bin / input scale):
+// a matvec is accurate to a few 1e-4 relative, which the FP64 iterative
+// refinement of the mixed solve corrects.
+//
+// Layouts (S slots per particle, NC = S / 16 chunks):
+//   coefficients [bins][T][MT][64 lanes][NC][4] FP16: lane (q, rr) of tile i
+//         gets c_t(slot 16 (i mod MH) + rr, slot 16 c + 4 q + j), tiles
+//         i < MH from Re L, the others from Im L;
+//   x     [n_part][4 (q)][kBatch][NC][4 (j)] float2 (b16_index): the 4 NC
+//         inputs a lane needs are contiguous (the (rhs, re) and (rhs, im)
+//         lanes read the same bytes and pick their operand order).
+#pragma once
+
+#include "smh_batch_slots.h"
+
+namespace smh {
+
+typedef _Float16 half2_t __attribute__((ext_vector_type(2)));
+typedef _Float16 half4_t __attribute__((ext_vector_type(4)));
+
+template <int LMAX, int MMAX, int T>
+__global__ void __launch_bounds__(512) batch_half_kernel(RadialKArgs a, const float* __restrict__ hscale) {
+  using Sh = SlotShape<LMAX, MMAX, T>;
+  constexpr int S = Sh::S, MT = Sh::MT, MH = Sh::MH, SC = Sh::SC, LO = Sh::LO, SY = Sh::SY;
+  constexpr int NC = S / 16;
+  constexpr int KB = kBatchRhs;
+  static_assert(KB * 2 == 16, "16 result columns = kBatch right-hand sides x (re, im)");
+  static_assert(S % 16 == 0, "slots per particle must be a multiple of 16");
+  static_assert(MMAX <= 3, "phase selection covers |m| <= 3");
+  constexpr SlotOrder<LMAX, MMAX> so{};
+  static_assert(so.ok, "slot order");
+
+  extern __shared__ __align__(16) int yacc[];
+  int* next_entry = yacc + (size_t)a.Rc * SY;
+  const int item = a.order ? a.order[blockIdx.x] : (int)blockIdx.x;
+  const float xmax = *a.xmax;
+  float sh = 0.f, ish = 0.f, sl = 0.f, isl = 0.f;
+  fixed_scales(a.item_lsum[item], xmax, a.item_en[item], sh, ish, sl, isl);
+  const float slo = sl * ish;
+  const float xinv = xmax > 0.f ? 1.f / xmax : 0.f;
+
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int2 it = a.items[item];
+  const int e_end = item + 1 < a.n_items ? a.items[item + 1].y : a.n_entries;
+  const PairRec<float>* recs = reinterpret_cast<const PairRec<float>*>(a.recs);
+  const float2* xg = reinterpret_cast<const float2*>(a.x);
+  const uint4* tth = reinterpret_cast<const uint4*>(a.tt);
+  for (int j = tid; j < a.Rc * SY; j += blockDim.x) yacc[j] = 0;
+  if (tid == 0) *next_entry = it.y;
+  __syncthreads();
+
+  const int rr = lane & 15, q = lane >> 4;
+  const int rhs = rr >> 1, zp = rr & 1;
+  const unsigned zmask = zp ? 0u : 0x80000000u;  // z: sign of the b term for the re column
+  const float sgnz = zp ? 1.f : -1.f;
+  // receiver rotation, as batch_slots_kernel: result rows of tile i are slot group 4 i + q
+  float hot[MH][4];
+  unsigned smlo[MH], smhi[MH];
+#pragma unroll
+  for (int i = 0; i < MH; ++i) {
+    const int g0 = 4 * i;
+    const int ra = q == 0 ? so.absm[g0] : (q == 1 ? so.absm[g0 + 1] : (q == 2 ? so.absm[g0 + 2] : so.absm[g0 + 3]));
+    const int sg = q == 0 ? so.sgn[g0] : (q == 1 ? so.sgn[g0 + 1] : (q == 2 ? so.sgn[g0 + 2] : so.sgn[g0 + 3]));
+#pragma unroll
+    for (int am = 0; am < 4; ++am) hot[i][am] = ra == am ? 1.f : 0.f;
+    smlo[i] = sg < 0 ? 0x80000000u : 0u;
+    smhi[i] = sg > 0 ? 0u : 0x80000000u;
+  }
+  // input rotation: the K values of chunk c are the 4 slots of group 4 c + q
+  // (the same group structure as the result rows of tile c)
+  float zhot[NC][4];
+  unsigned zlo[NC], zhi[NC];
+#pragma unroll
+  for (int c = 0; c < NC; ++c) {
+    const int g0 = 4 * c;
+    const int ra = q == 0 ? so.absm[g0] : (q == 1 ? so.absm[g0 + 1] : (q == 2 ? so.absm[g0 + 2] : so.absm[g0 + 3]));
+    const int sg = q == 0 ? so.sgn[g0] : (q == 1 ? so.sgn[g0 + 1] : (q == 2 ? so.sgn[g0 + 2] : so.sgn[g0 + 3]));
+#pragma unroll
+    for (int am = 0; am < 4; ++am) zhot[c][am] = ra == am ? 1.f : 0.f;
+    zlo[c] = (sg < 0 ? 0x80000000u : 0u) ^ zmask;  // slots j < 2
+    zhi[c] = (sg > 0 ? 0u : 0x80000000u) ^ zmask;  // slots j >= 2 (m < 0 in mixed groups)
+  }
+  const int colbase = (rhs * 2 + zp) * SC + 4 * q;
+  const int64_t xlane = (int64_t)(q * KB + rhs) * NC * 4;
+
+  for (;;) {
+    int e0 = 0;
+    if (lane == 0) e0 = atomicAdd(next_entry, 1);
+    const int e = __builtin_amdgcn_readfirstlane(e0);
+    if (e >= e_end) break;
+    const int4 en = a.entries[e];
+    const int bin = (en.x - a.row_o) / a.row_s;
+    const float bscale = hscale[bin] * xmax;
+    // polynomial coefficients: tv[t][i][c] = 4 FP16 values (2 packed pairs)
+    half2_t tv[T][MT][NC][2];
+#pragma unroll
+    for (int t = 0; t < T; ++t)
+#pragma unroll
+      for (int i = 0; i < MT; ++i) {
+        const uint4* row = tth + (((size_t)bin * T + t) * MT + i) * 64 * (NC / 2 > 0 ? NC / 2 : 1);
+        if constexpr (NC % 2 == 0) {
+#pragma unroll
+          for (int c2 = 0; c2 < NC / 2; ++c2) {
+            const uint4 v = row[lane * (NC / 2) + c2];
+            tv[t][i][2 * c2][0] = __builtin_bit_cast(half2_t, v.x);
+            tv[t][i][2 * c2][1] = __builtin_bit_cast(half2_t, v.y);
+            tv[t][i][2 * c2 + 1][0] = __builtin_bit_cast(half2_t, v.z);
+            tv[t][i][2 * c2 + 1][1] = __builtin_bit_cast(half2_t, v.w);
+          }
+        } else {
+          const uint2* row2 = reinterpret_cast<const uint2*>(tth) + (((size_t)bin * T + t) * MT + i) * 64 * NC;
+#pragma unroll
+          for (int c = 0; c < NC; ++c) {
+            const uint2 v = row2[lane * NC + c];
+            tv[t][i][c][0] = __builtin_bit_cast(half2_t, v.x);
+            tv[t][i][c][1] = __builtin_bit_cast(half2_t, v.y);
+          }
+        }
+      }
+    const int p_end = en.z;
+    auto load_x = [&](const PairRec<float>& r, float4 (&xs)[2 * NC]) {
+      const float4* xp = reinterpret_cast<const float4*>(xg + (int64_t)(r.ij & kPairMask) * S * KB + xlane);
+#pragma unroll
+      for (int k = 0; k < 2 * NC; ++k) xs[k] = xp[k];
+    };
+    PairRec<float> rec_n = recs[en.y];
+    float4 xc[2 * NC];
+    load_x(rec_n, xc);
+    for (int p = 0; p < p_end; ++p) {
+      const PairRec<float> rec = rec_n;
+      rec_n = recs[en.y + (p + 1 < p_end ? p + 1 : p)];
+
+      const float c1 = rec.c, s1 = rec.s;
+      const float c2 = c1 * c1 - s1 * s1, s2 = 2.f * c1 * s1;
+      const float c3 = c2 * c1 - s2 * s1, s3 = s2 * c1 + c2 * s1;
+      const float cs[4] = {1.f, c1, c2, c3};
+      const float sn[4] = {0.f, s1, s2, s3};
+
+      // B operands: per chunk the rotated, scaled input of the lane's 4 slots in FP16
+      half4_t zb[NC];
+#pragma unroll
+      for (int c = 0; c < NC; ++c) {
+        float cz = 0.f, sz = 0.f;
+#pragma unroll
+        for (int am = 0; am <= MMAX; ++am) {
+          if (so.absm[4 * c] != am && so.absm[4 * c + 1] != am && so.absm[4 * c + 2] != am &&
+              so.absm[4 * c + 3] != am)
+            continue;
+          cz = fmaf(zhot[c][am], cs[am], cz);
+          if (am > 0) sz = fmaf(zhot[c][am], sn[am], sz);
+        }
+        cz *= xinv;
+        sz *= xinv;
+        const float elo = xor_sign(sz, zlo[c]), ehi = xor_sign(sz, zhi[c]);
+        const float4 xa = xc[2 * c], xb = xc[2 * c + 1];  // slots j = 0, 1 and 2, 3 as (re, im)
+        const float av0 = zp ? xa.y : xa.x, bv0 = zp ? xa.x : xa.y;
+        const float av1 = zp ? xa.w : xa.z, bv1 = zp ? xa.z : xa.w;
+        const float av2 = zp ? xb.y : xb.x, bv2 = zp ? xb.x : xb.y;
+        const float av3 = zp ? xb.w : xb.z, bv3 = zp ? xb.z : xb.w;
+        zb[c][0] = (_Float16)fmaf(cz, av0, elo * bv0);
+        zb[c][1] = (_Float16)fmaf(cz, av1, elo * bv1);
+        zb[c][2] = (_Float16)fmaf(cz, av2, ehi * bv2);
+        zb[c][3] = (_Float16)fmaf(cz, av3, ehi * bv3);
+      }
+      load_x(rec_n, xc);
+
+      const _Float16 wh = (_Float16)rec.w;
+      const half2_t w2 = {wh, wh};
+      typename Mfma<float>::v4 acc[MT];
+#pragma unroll
+      for (int i = 0; i < MT; ++i) acc[i] = typename Mfma<float>::v4{0, 0, 0, 0};
+#pragma unroll
+      for (int c = 0; c < NC; ++c) {
+#pragma unroll
+        for (int i = 0; i < MT; ++i) {
+          half2_t v0 = tv[T - 1][i][c][0], v1 = tv[T - 1][i][c][1];
+#pragma unroll
+          for (int t = T - 2; t >= 0; --t) {
+            v0 = v0 * w2 + tv[t][i][c][0];
+            v1 = v1 * w2 + tv[t][i][c][1];
+          }
+          const half4_t av = {v0[0], v0[1], v1[0], v1[1]};
+          acc[i] = __builtin_amdgcn_mfma_f32_16x16x16f16(av, zb[c], acc[i], 0, 0, 0);
+        }
+      }
+
+      // y (rhs, re|im) of rows 16 i + 4 q + v, rotated by e^{-i m1 phi}, scaled by S_hi and the operand scales
+      int* yrow = yacc + (size_t)(rec.ij >> kPairShift) * SY + colbase;
+      const float scl = sh * bscale;
+#pragma unroll
+      for (int i = 0; i < MH; ++i) {
+        float er = 0.f, ei = 0.f;
+#pragma unroll
+        for (int am = 0; am <= MMAX; ++am) {
+          if (so.absm[4 * i] != am && so.absm[4 * i + 1] != am && so.absm[4 * i + 2] != am &&
+              so.absm[4 * i + 3] != am)
+            continue;
+          er = fmaf(hot[i][am], cs[am], er);
+          if (am > 0) ei = fmaf(hot[i][am], sn[am], ei);
+        }
+        er *= scl;
+        ei *= scl;
+        const float elo = xor_sign(ei, smlo[i]), ehi = xor_sign(ei, smhi[i]);
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const float eps = v < 2 ? elo : ehi;
+          const float al = acc[i][v], be = acc[i + MH][v];
+          const float own = fmaf(al, er, be * eps);
+          const float snd = fmaf(be, er, -(al * eps));
+          const float val = fma_xor1(own, sgnz, snd);
+          const float h = rintf(val);
+          const float l = rintf((val - h) * slo);
+          atomicAdd(yrow + 16 * i + v, (int)h);
+          atomicAdd(yrow + LO + 16 * i + v, (int)l);
+        }
+      }
+    }
+  }
+  __syncthreads();
+  float* yp = reinterpret_cast<float*>(a.ypart) + (size_t)item * a.Rc * 2 * KB * S;
+  for (int j = tid; j < a.Rc * 2 * KB * S; j += blockDim.x) {
+    const int il = j / (2 * KB * S), k = j - il * 2 * KB * S;
+    const int c = k / S, sl2 = k - c * S;
+    const int w0 = il * SY + c * SC + sl2;
+    yp[j] = (float)((double)yacc[w0] * (double)ish + (double)yacc[w0 + LO] * (double)isl);
+  }
+}
+
+}  // namespace smh

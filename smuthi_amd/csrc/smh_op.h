@@ -34,6 +34,10 @@ struct smh_op {
   float* d_tt32b = nullptr;
   int* d_b32_tab = nullptr;
   int b32_S = 0;
+  // FP16-operand kBatch-RHS kernel (slot order, b16_index input layout):
+  // FP16 polynomial coefficients and one power-of-two scale per bin
+  _Float16* d_tth = nullptr;
+  float* d_hscale = nullptr;
   float* d_rowsum = nullptr;         // per tap row: max_n1 sum_n2 |Re| + |Im|
   float* d_xmax = nullptr;           // largest |Re| + |Im| of the current FP32 input
 
@@ -94,6 +98,15 @@ __host__ __device__ inline int64_t b32_index(int64_t p, int n, int r, int S, con
   if (!slot) return (p * S + n) * kBatchRhs + r;
   const int k = slot[n];
   return p * S * kBatchRhs + ((k & 3) * kBatchRhs + r) * (S / 4) + (k >> 2);
+}
+// Index of (particle p, multipole n, rhs r) in the input of the FP16-operand
+// kBatch-RHS kernel: [n_part][4 (q)][kBatch][S/16 (c)][4 (j)] float2 for slot
+// k = slot[n] = 16 c + 4 q + j (the re-column and im-column lanes of a
+// right-hand side read the same 16-byte pairs).
+__host__ __device__ inline int64_t b16_index(int64_t p, int n, int r, int S, const int* slot) {
+  const int k = slot[n];
+  const int c = k >> 4, q = (k >> 2) & 3, j = k & 3;
+  return p * S * kBatchRhs + ((int64_t)(q * kBatchRhs + r) * (S / 16) + c) * 4 + j;
 }
 // packed -> padded index map of the particles' multipole blocks
 int op_index_map(smh_op* op, const int* lmax, const int* mmax);

@@ -4,6 +4,7 @@
 
 #include "smh_op.h"
 #include "smh_batch_kernel.h"
+#include "smh_batch_half.h"
 #include "smh_batch_slots.h"
 #include "smh_lookup.h"
 
@@ -135,6 +136,30 @@ __global__ void taps_kernel(const double2* __restrict__ table, const double2* __
 //                c3 = (-y_{r-1} + 3 y_r - 3 y_{r+1} + y_{r+2}) / 2;
 //   spline       c0 = y_r, c1 = y_{r+1} - y_r - 2 Mh_r - Mh_{r+1}, c2 = 3 Mh_r, c3 = Mh_{r+1} - Mh_r.
 // Bins no pair uses (at the table ends) read clamped rows.
+// Coefficient k of the bin-r interpolant of table entry nn (Re part if re, else Im).
+__device__ inline double poly_coef(const double2* __restrict__ table, const double2* __restrict__ mh, int64_t len,
+                                   int BB, int64_t nn, bool re, int interp, int64_t r, int k) {
+  auto Y = [&](int64_t j) {
+    j = j < 0 ? 0 : (j >= len ? len - 1 : j);
+    const double2 c = table[j * BB + nn];
+    return re ? c.x : c.y;
+  };
+  auto Mh = [&](int64_t j) {
+    j = j < 0 ? 0 : (j >= len ? len - 1 : j);
+    const double2 c = mh[j * BB + nn];
+    return re ? c.x : c.y;
+  };
+  if (interp == SMH_INTERP_LINEAR) return k == 0 ? Y(r) : Y(r + 1) - Y(r);
+  if (interp == SMH_INTERP_CATMULL_ROM) {
+    const double p0 = Y(r - 1), p1 = Y(r), p2 = Y(r + 1), p3 = Y(r + 2);
+    return k == 0 ? p1
+                  : (k == 1 ? 0.5 * (p2 - p0)
+                            : (k == 2 ? p0 - 2.5 * p1 + 2.0 * p2 - 0.5 * p3 : 0.5 * (-p0 + 3.0 * p1 - 3.0 * p2 + p3)));
+  }
+  const double y0 = Y(r), y1 = Y(r + 1), m0 = Mh(r), m1 = Mh(r + 1);
+  return k == 0 ? y0 : (k == 1 ? y1 - y0 - 2.0 * m0 - m1 : (k == 2 ? 3.0 * m0 : m1 - m0));
+}
+
 __global__ void poly_slot_kernel(const double2* __restrict__ table, const double2* __restrict__ mh, int64_t len,
                                  int B, int S, const int* __restrict__ n_of, int interp, int T, float* tt) {
   const int NS = S / 4, MH = S / 16, FRB = 2 * MH * NS * 64;
@@ -153,34 +178,53 @@ __global__ void poly_slot_kernel(const double2* __restrict__ table, const double
     const int64_t r = row / T;
     const int k = (int)(row - r * T);
     float v = 0.f;
-    if (n1 >= 0 && n2 >= 0) {
-      const int64_t nn = (int64_t)n1 * B + n2;
-      auto Y = [&](int64_t j) {
-        j = j < 0 ? 0 : (j >= len ? len - 1 : j);
-        const double2 c = table[j * B * B + nn];
-        return i < MH ? c.x : c.y;
-      };
-      auto Mh = [&](int64_t j) {
-        j = j < 0 ? 0 : (j >= len ? len - 1 : j);
-        const double2 c = mh[j * B * B + nn];
-        return i < MH ? c.x : c.y;
-      };
-      double c = 0.0;
-      if (interp == SMH_INTERP_LINEAR) {
-        c = k == 0 ? Y(r) : Y(r + 1) - Y(r);
-      } else if (interp == SMH_INTERP_CATMULL_ROM) {
-        const double p0 = Y(r - 1), p1 = Y(r), p2 = Y(r + 1), p3 = Y(r + 2);
-        c = k == 0 ? p1
-                   : (k == 1 ? 0.5 * (p2 - p0)
-                             : (k == 2 ? p0 - 2.5 * p1 + 2.0 * p2 - 0.5 * p3 : 0.5 * (-p0 + 3.0 * p1 - 3.0 * p2 + p3)));
-      } else {
-        const double y0 = Y(r), y1 = Y(r + 1), m0 = Mh(r), m1 = Mh(r + 1);
-        c = k == 0 ? y0 : (k == 1 ? y1 - y0 - 2.0 * m0 - m1 : (k == 2 ? 3.0 * m0 : m1 - m0));
-      }
-      v = (float)c;
-    }
+    if (n1 >= 0 && n2 >= 0) v = (float)poly_coef(table, mh, len, B * B, (int64_t)n1 * B + n2, i < MH, interp, r, k);
     tt[f] = v;
   }
+}
+
+// FP16 coefficients of the FP16-input batched kernel, one block per bin r:
+// layout [bins][T][MT][64 lanes][NC][4], lane (q, rr) of tile i holding
+// c_t(slot 16 (i mod MH) + rr, slot 16 c + 4 q + j); the bin's values are
+// divided by hscale[r] = 2^e, the smallest power of two with
+// T * max |c| / 2^e <= 2^12 (every Horner partial sum stays far inside the
+// FP16 range).
+__global__ void poly_half_kernel(const double2* __restrict__ table, const double2* __restrict__ mh, int64_t len,
+                                 int B, int S, const int* __restrict__ n_of, int interp, int T, _Float16* th,
+                                 float* hscale) {
+  const int MH = S / 16, MT = 2 * MH, NC = S / 16;
+  const int per_t = MT * 64 * NC * 4;
+  const int per_bin = T * per_t;
+  const int64_t r = blockIdx.x;
+  __shared__ double red[256];
+  auto value = [&](int f) {
+    const int k = f / per_t;
+    int rem = f - k * per_t;
+    const int j = rem & 3;
+    rem >>= 2;
+    const int c = rem % NC;
+    rem /= NC;
+    const int ln = rem & 63;
+    const int i = rem >> 6;
+    const int q = ln >> 4, rr = ln & 15;
+    const int n1 = n_of[16 * (i % MH) + rr];
+    const int n2 = n_of[16 * c + 4 * q + j];
+    if (n1 < 0 || n2 < 0) return 0.0;
+    return poly_coef(table, mh, len, B * B, (int64_t)n1 * B + n2, i < MH, interp, r, k);
+  };
+  double m = 0.0;
+  for (int f = threadIdx.x; f < per_bin; f += blockDim.x) m = fmax(m, fabs(value(f)));
+  red[threadIdx.x] = m;
+  __syncthreads();
+  for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + w]);
+    __syncthreads();
+  }
+  const double mx = red[0] * T;
+  const int e = mx > 0.0 ? (int)ceil(log2(mx)) - 12 : 0;
+  const double inv = ldexp(1.0, -e);
+  if (threadIdx.x == 0) hscale[r] = ldexpf(1.0f, e);
+  for (int f = threadIdx.x; f < per_bin; f += blockDim.x) th[(size_t)r * per_bin + f] = (_Float16)(value(f) * inv);
 }
 
 // Not-a-knot cubic spline moments, scaled: Mh_j = h^2/6 S''(rho_j), one
@@ -299,6 +343,32 @@ KernelInfo pick_batch(int l, int m) {
   return KernelInfo{};
 }
 
+// FP16-operand kBatch-RHS kernel: where the FP32 slot-order kernel is used
+template <int L, int Mx, int T>
+void* half_fn() {
+  if constexpr (SlotShape<L, Mx, T>::fits && Mx <= 3)
+    return reinterpret_cast<void*>(&batch_half_kernel<L, Mx, T>);
+  else
+    return nullptr;
+}
+
+void* radial_half_kernel(int T, int l, int m) {
+#define SMH_K(L, Mx) \
+  if (l == L && m == Mx) return T == 4 ? half_fn<L, Mx, 4>() : half_fn<L, Mx, 2>();
+  SMH_K(1, 1)
+  SMH_K(2, 1)
+  SMH_K(2, 2)
+  SMH_K(3, 1)
+  SMH_K(3, 2)
+  SMH_K(3, 3)
+  SMH_K(4, 1)
+  SMH_K(4, 2)
+  SMH_K(4, 3)
+  SMH_K(4, 4)
+#undef SMH_K
+  return nullptr;
+}
+
 KernelInfo radial_batch_kernel(int precision, int T, int l, int m) {
   if (precision == SMH_FP32) return T == 4 ? pick_batch<4, float>(l, m) : pick_batch<2, float>(l, m);
   return T == 4 ? pick_batch<4, double>(l, m) : pick_batch<2, double>(l, m);
@@ -385,6 +455,16 @@ int op_make_taps(smh_op* op) {
     if (!op->d_tt32b) SMH_HIP(ctx, hipMalloc(&op->d_tt32b, (size_t)len * op->T * frb * sizeof(float)));
     hipLaunchKernelGGL(poly_slot_kernel, dim3(4096), dim3(256), 0, s, op->d_table, d_mh, len, op->B, op->b32_S,
                        op->d_b32_tab + op->B, op->interp, op->T, op->d_tt32b);
+    if (radial_half_kernel(op->T, op->l_max, op->m_max)) {
+      // T * MT * 64 * NC * 4 halves, MT = 2 S / 16, NC = S / 16: 2 S^2 per tap
+      const size_t per_bin = (size_t)op->T * 2 * op->b32_S * op->b32_S;
+      if (!op->d_tth) {
+        SMH_HIP(ctx, hipMalloc(&op->d_tth, (size_t)len * per_bin * sizeof(_Float16)));
+        SMH_HIP(ctx, hipMalloc(&op->d_hscale, (size_t)len * sizeof(float)));
+      }
+      hipLaunchKernelGGL(poly_half_kernel, dim3((unsigned)len), dim3(256), 0, s, op->d_table, d_mh, len, op->B,
+                         op->b32_S, op->d_b32_tab + op->B, op->interp, op->T, op->d_tth, op->d_hscale);
+    }
   }
   if (!op->d_rowsum) {
     SMH_HIP(ctx, hipMalloc(&op->d_rowsum, (size_t)op->n_tt * sizeof(float)));
@@ -467,7 +547,7 @@ namespace {
 // layout of b32_index() instead (S slots per particle, slot table given).
 template <typename Real>
 __global__ void scatter_batch_kernel(const double2* __restrict__ pk, const int* __restrict__ pk2pad, int64_t n_unk,
-                                     int nrhs, int B, int S, const int* __restrict__ slot,
+                                     int nrhs, int B, int S, const int* __restrict__ slot, int half_layout,
                                      typename Vec2<Real>::t* pad) {
   const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= n_unk * nrhs) return;
@@ -475,7 +555,9 @@ __global__ void scatter_batch_kernel(const double2* __restrict__ pk, const int* 
   const double2 v = pk[j];
   const int ip = pk2pad[u];
   const int64_t p = ip / B;
-  pad[b32_index(p, (int)(ip - p * B), (int)r, S, slot)] = typename Vec2<Real>::t{Real(v.x), Real(v.y)};
+  const int n = (int)(ip - p * B);
+  pad[half_layout ? b16_index(p, n, (int)r, S, slot) : b32_index(p, n, (int)r, S, slot)] =
+      typename Vec2<Real>::t{Real(v.x), Real(v.y)};
 }
 
 template <typename Real>
@@ -519,15 +601,15 @@ namespace smh {
 int op_unpack_batch(smh_op* op, int precision, const double2* pk, int nrhs, void* pad) {
   hipStream_t s = op->ctx->stream;
   const int64_t n = op->n_unk * nrhs;
-  if (precision == SMH_FP32) {
+  if (precision == SMH_FP32 || precision == SMH_FP16) {
     const int* slot = op->d_b32_tab;
     SMH_HIP(op->ctx, hipMemsetAsync(pad, 0, (size_t)op->n_part * op->b32_S * kBatch * sizeof(float2), s));
     hipLaunchKernelGGL(scatter_batch_kernel<float>, dim3(blocks_for(n)), dim3(256), 0, s, pk, op->d_pk2pad,
-                       op->n_unk, nrhs, op->B, op->b32_S, slot, (float2*)pad);
+                       op->n_unk, nrhs, op->B, op->b32_S, slot, precision == SMH_FP16 ? 1 : 0, (float2*)pad);
   } else {
     SMH_HIP(op->ctx, hipMemsetAsync(pad, 0, (size_t)op->n_part * op->B * kBatch * sizeof(double2), s));
     hipLaunchKernelGGL(scatter_batch_kernel<double>, dim3(blocks_for(n)), dim3(256), 0, s, pk, op->d_pk2pad,
-                       op->n_unk, nrhs, op->B, op->B, (const int*)nullptr, (double2*)pad);
+                       op->n_unk, nrhs, op->B, op->B, (const int*)nullptr, 0, (double2*)pad);
   }
   SMH_HIP(op->ctx, hipGetLastError());
   return SMH_OK;
@@ -536,7 +618,7 @@ int op_unpack_batch(smh_op* op, int precision, const double2* pk, int nrhs, void
 int op_pack_batch(smh_op* op, int precision, const void* pad, int nrhs, double2* pk) {
   hipStream_t s = op->ctx->stream;
   const int64_t n = op->n_unk * nrhs;
-  if (precision == SMH_FP32)
+  if (precision == SMH_FP32 || precision == SMH_FP16)
     hipLaunchKernelGGL(pack_batch_kernel<float>, dim3(blocks_for(n)), dim3(256), 0, s, (const float2*)pad,
                        op->d_pk2pad, op->n_unk, nrhs, pk);
   else
@@ -548,7 +630,7 @@ int op_pack_batch(smh_op* op, int precision, const void* pad, int nrhs, double2*
 
 int op_reserve_batch(smh_op* op) {
   if (op->d_xb) return SMH_OK;
-  // FP64 padded vectors or the FP32 input in its (slot) layout, whichever is larger
+  // FP64 padded vectors or the FP32 / FP16-kernel input in its slot layout, whichever is larger
   const size_t bytes = (size_t)op->n_part * kBatch *
                        std::max<size_t>((size_t)op->B * sizeof(double2), (size_t)op->b32_S * sizeof(float2));
   SMH_HIP(op->ctx, hipMalloc(&op->d_xb, bytes));
@@ -557,16 +639,19 @@ int op_reserve_batch(smh_op* op) {
 }
 
 bool op_has_batch(const smh_op* op, int precision) {
+  if (precision == SMH_FP16) return op->kind == 0 && op->bbin[0].Rc > 0 && op->d_tth != nullptr;
   return op->kind == 0 && op->bbin[precision == SMH_FP32 ? 0 : 1].Rc > 0;
 }
 
 int op_apply_batch(smh_op* op, int precision, const void* x_pad, void* y_pad, bool time_kernel) {
   smh_ctx* ctx = op->ctx;
   SMH_CHECK(ctx, op->has_table, SMH_ERR_STATE, "coupling table not set");
-  const int pi = precision == SMH_FP32 ? 0 : 1;
+  const bool half = precision == SMH_FP16;
+  const int pi = (precision == SMH_FP32 || half) ? 0 : 1;
   SMH_CHECK(ctx, op_has_batch(op, precision), SMH_ERR_UNSUPPORTED, "no batched matvec for this operator");
   const smh_op::Binning& bn = op->bbin[pi];
-  const KernelInfo k = radial_batch_kernel(precision, op->T, op->l_max, op->m_max);
+  const KernelInfo k = radial_batch_kernel(pi == 0 ? SMH_FP32 : SMH_FP64, op->T, op->l_max, op->m_max);
+  void* fn = half ? radial_half_kernel(op->T, op->l_max, op->m_max) : k.fn;
   RadialKArgs a;
   a.recs = bn.d_rec;
   a.entries = bn.d_entries;
@@ -574,7 +659,9 @@ int op_apply_batch(smh_op* op, int precision, const void* x_pad, void* y_pad, bo
   a.order = bn.d_order;
   const bool slots = pi == 0 && k.slots > 0;
   SMH_CHECK(ctx, !slots || (op->d_tt32b && op->b32_S == k.slots), SMH_ERR_STATE, "slot-order tap table missing");
-  a.tt = pi == 0 ? (slots ? (const void*)op->d_tt32b : (const void*)op->d_tt32) : (const void*)op->d_tt64;
+  SMH_CHECK(ctx, !half || (fn && slots), SMH_ERR_STATE, "FP16 tap table missing");
+  a.tt = half ? (const void*)op->d_tth
+              : (pi == 0 ? (slots ? (const void*)op->d_tt32b : (const void*)op->d_tt32) : (const void*)op->d_tt64);
   a.x = x_pad;
   a.ypart = op->d_ypart;
   a.xmax = op->d_xmax;
@@ -588,7 +675,7 @@ int op_apply_batch(smh_op* op, int precision, const void* x_pad, void* y_pad, bo
   a.row_s = op->s_mode;
   a.row_o = op->o_mode;
   hipStream_t s = ctx->stream;
-  SMH_HIP(ctx, hipFuncSetAttribute(k.fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bn.lds));
+  SMH_HIP(ctx, hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bn.lds));
   const int64_t nx = op->n_part * op->B * kBatch;
   const int S = slots ? k.slots : op->B;
   const int* slot = slots ? op->d_b32_tab : nullptr;
@@ -597,8 +684,10 @@ int op_apply_batch(smh_op* op, int precision, const void* x_pad, void* y_pad, bo
                        op->d_xmax);
   if (bn.n_items > 0) {
     if (time_kernel) SMH_HIP(ctx, hipEventRecord(ctx->ev0, s));
-    void* args[] = {&a};
-    SMH_HIP(ctx, hipLaunchKernel(k.fn, dim3(bn.n_items), dim3(k.threads), args, bn.lds, s));
+    const float* hs = op->d_hscale;
+    void* args1[] = {&a};
+    void* args2[] = {&a, &hs};
+    SMH_HIP(ctx, hipLaunchKernel(fn, dim3(bn.n_items), dim3(k.threads), half ? args2 : args1, bn.lds, s));
     if (time_kernel) SMH_HIP(ctx, hipEventRecord(ctx->ev1, s));
   }
   if (pi == 0)
@@ -649,7 +738,7 @@ int op_index_map(smh_op* op, const int* lmax, const int* mmax) {
 
 void smh_op_free(smh_op* op) {
   void* ptrs[] = {op->d_xy, op->d_xyz, op->d_pk2pad, op->d_table, op->d_table_minus, op->d_tt64, op->d_tt32,
-                  op->d_tt32b, op->d_b32_tab, op->d_W64, op->d_W32, op->d_ypart, op->d_pk_in, op->d_pk_out, op->d_x32, op->d_y32, op->d_x64, op->d_y64,
+                  op->d_tt32b, op->d_b32_tab, op->d_tth, op->d_hscale, op->d_W64, op->d_W32, op->d_ypart, op->d_pk_in, op->d_pk_out, op->d_x32, op->d_y32, op->d_x64, op->d_y64,
                   op->d_rowsum, op->d_xmax, op->d_xb, op->d_yb};
   for (void* p : ptrs)
     if (p) hipFree(p);
@@ -923,13 +1012,16 @@ extern "C" int smh_op_matvec(smh_op* op, const double* x, double* y, int64_t nrh
   if (!op) return SMH_ERR_ARG;
   smh_ctx* ctx = op->ctx;
   SMH_CHECK(ctx, x && y && nrhs >= 1, SMH_ERR_ARG, "invalid matvec arguments");
-  SMH_CHECK(ctx, precision == SMH_FP32 || precision == SMH_FP64, SMH_ERR_ARG, "invalid precision %d", precision);
+  SMH_CHECK(ctx, precision == SMH_FP32 || precision == SMH_FP64 || precision == SMH_FP16, SMH_ERR_ARG,
+            "invalid precision %d", precision);
+  SMH_CHECK(ctx, precision != SMH_FP16 || op_has_batch(op, SMH_FP16), SMH_ERR_UNSUPPORTED,
+            "FP16 matvecs exist for radial operators with the slot-order kernel only");
   SMH_HIP(ctx, hipSetDevice(ctx->device));
   int rc = op_reserve_packed(op, nrhs);
   if (rc) return rc;
   hipStream_t s = ctx->stream;
   SMH_HIP(ctx, hipMemcpyAsync(op->d_pk_in, x, op->n_unk * nrhs * sizeof(double2), hipMemcpyHostToDevice, s));
-  if (nrhs > 1 && op_has_batch(op, precision)) {
+  if ((nrhs > 1 || precision == SMH_FP16) && op_has_batch(op, precision)) {
     // blocks of kBatch right-hand sides through the batched kernel
     if ((rc = op_reserve_batch(op))) return rc;
     for (int64_t r0 = 0; r0 < nrhs; r0 += kBatch) {
@@ -975,8 +1067,8 @@ extern "C" int smh_op_bench_matvec_batch(smh_op* op, int precision, int iters, d
   if (rc) return rc;
   hipStream_t s = ctx->stream;
   // synthetic input in the precision's layout (the other buffer is scratch for the unused copy)
-  const int64_t n = op->n_part * (precision == SMH_FP32 ? op->b32_S : op->B) * kBatch;
-  if (precision == SMH_FP32)
+  const int64_t n = op->n_part * (precision == SMH_FP64 ? op->B : op->b32_S) * kBatch;
+  if (precision != SMH_FP64)
     hipLaunchKernelGGL(fill_pattern_kernel, dim3(blocks_for(n)), dim3(256), 0, s, (float2*)op->d_xb,
                        (double2*)nullptr, n);
   else

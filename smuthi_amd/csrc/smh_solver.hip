@@ -15,6 +15,12 @@
 //   matvec, until the FP64 residual meets tol. kInnerRel sits at the FP32
 //   operator's resolution, so for tolerances down to ~1e-6 the first sweep
 //   usually suffices and one FP64 matvec checks it.
+// SMH_SOLVE_MIXED16: the same refinement with FP16-operand matvecs in the
+//   inner GMRES (batched radial solves), whose target kInnerRel16 sits at
+//   that operator's resolution (a few 1e-4): each sweep gains about three
+//   digits, so tolerances near 1e-6 take two sweeps. Between sweeps the
+//   residual comes from the FP32 matvec; the FP64 residual certifies the
+//   result once that meets the tolerance.
 #include <chrono>
 #include <cmath>
 #include <complex>
@@ -30,7 +36,11 @@ int blocks_for(int64_t n, int t = 256) { return (int)((n + t - 1) / t); }
 constexpr int kDotThreads = 256;
 constexpr int kDotBlocks = 512;
 constexpr int kMaxBasis = 128;
-constexpr double kInnerRel = 2e-7;  // inner (FP32-matvec) GMRES target relative to the current FP64 residual
+constexpr double kInnerRel = 2e-7;    // inner (FP32-matvec) GMRES target relative to the current FP64 residual
+constexpr double kInnerRel16 = 1e-3;  // the same for FP16-operand matvecs
+// FP32 residuals drive FP16 sweeps while above this fraction of ||b|| (about
+// 30 times the FP32 matvec's error)
+constexpr double kFp32ResidualFloor = 1e-5;
 
 // partial[b][j] = sum over this block's slice of conj(V_j) . w
 __global__ void multi_dot_kernel(const double2* __restrict__ V, int64_t ld, int nv, const double2* __restrict__ w,
@@ -368,6 +378,16 @@ __global__ void to_fp32_b32_kernel(const double2* __restrict__ v, float2* out, i
   out[smh::b32_index(p, (int)(ip - p * B), r, S, slot)] = make_float2((float)v[e].x, (float)v[e].y);
 }
 
+// FP64 padded [n][kB] -> FP32 input of the FP16-operand kernel (b16_index layout)
+__global__ void to_fp32_b16_kernel(const double2* __restrict__ v, float2* out, int64_t n8, int B, int S,
+                                   const int* __restrict__ slot) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n8) return;
+  const int r = (int)(e % kB);
+  const int64_t ip = e / kB, p = ip / B;
+  out[smh::b16_index(p, (int)(ip - p * B), r, S, slot)] = make_float2((float)v[e].x, (float)v[e].y);
+}
+
 __global__ void bmaster_fp32_kernel(const double2* __restrict__ v, const double2* __restrict__ t,
                                     const float2* __restrict__ y, double2* out, int64_t n8) {
   const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -392,8 +412,9 @@ struct BatchSolver {
   double* scl = nullptr;       // [kB]
   double2* y64 = nullptr;      // coupling output FP64 [n][kB]
   float2* x32 = nullptr;
+  float2* x16 = nullptr;  // FP32 input in the FP16-operand kernel's layout (allocated on first use)
   float2* y32 = nullptr;
-  int64_t mv64 = 0, mv32 = 0, iters = 0;
+  int64_t mv64 = 0, mv32 = 0, mv16 = 0, iters = 0;
 
   int init() {
     SMH_HIP(ctx, hipMalloc(&V, (size_t)(restart + 1) * n8 * sizeof(double2)));
@@ -411,7 +432,7 @@ struct BatchSolver {
     return SMH_OK;
   }
   void release() {
-    void* p[] = {V, w, t, partial, coef, scl, y64, x32, y32};
+    void* p[] = {V, w, t, partial, coef, scl, y64, x32, x16, y32};
     for (void* q : p)
       if (q) hipFree(q);
   }
@@ -422,6 +443,17 @@ struct BatchSolver {
       if ((rc = smh::op_apply_batch(op, SMH_FP64, v, y64, false))) return rc;
       hipLaunchKernelGGL(bmaster_fp64_kernel, dim3(blocks_for(n8)), dim3(256), 0, s, v, t, y64, out, n8);
       ++mv64;
+    } else if (precision == SMH_FP16) {
+      if (!x16) {
+        const size_t n16 = (size_t)op->n_part * op->b32_S * kB;
+        SMH_HIP(ctx, hipMalloc(&x16, n16 * sizeof(float2)));
+        SMH_HIP(ctx, hipMemsetAsync(x16, 0, n16 * sizeof(float2), s));
+      }
+      hipLaunchKernelGGL(to_fp32_b16_kernel, dim3(blocks_for(n8)), dim3(256), 0, s, v, x16, n8, op->B, op->b32_S,
+                         (const int*)op->d_b32_tab);
+      if ((rc = smh::op_apply_batch(op, SMH_FP16, x16, y32, false))) return rc;
+      hipLaunchKernelGGL(bmaster_fp32_kernel, dim3(blocks_for(n8)), dim3(256), 0, s, v, t, y32, out, n8);
+      ++mv16;
     } else {
       hipLaunchKernelGGL(to_fp32_b32_kernel, dim3(blocks_for(n8)), dim3(256), 0, s, v, x32, n8, op->B, op->b32_S,
                          (const int*)op->d_b32_tab);
@@ -622,26 +654,53 @@ static int solve_batched(smh_op* op, const double* t_diag, const double* rhs, do
       SMH_HIP(ctx, hipMemcpyAsync(r_pad, b_pad, S.n8 * sizeof(double2), hipMemcpyDeviceToDevice, s));
       for (int r = 0; r < kB; ++r) res[r] = bnorm[r];
       bool fresh = true;
+      bool fp64_residuals = mode != SMH_SOLVE_MIXED16;
+      double prev_res[kB];  // residual each system's last sweep started from
       int budget = maxiter;
       for (int sweep = 0; sweep < 50; ++sweep) {
         if (sweep > 0) {
-          if ((rc = S.master(x_pad, r_pad, SMH_FP64))) return done(rc);
-          hipLaunchKernelGGL(sub_kernel, dim3(blocks_for(S.n8)), dim3(256), 0, s, b_pad, r_pad, r_pad, S.n8);
-          if ((rc = S.norms(r_pad, res))) return done(rc);
-          fresh = true;
+          // FP16 inner solves: the residual that drives the next sweep comes
+          // from the FP32 matvec (accurate to a few 1e-7 of ||b||, far below
+          // the FP16 sweep's own error) while it is well above that floor;
+          // once it meets the tolerance or nears the floor, FP64 residuals
+          // certify the result and drive any further sweeps
+          bool certified = fp64_residuals;
+          if (!certified) {
+            if ((rc = S.master(x_pad, r_pad, SMH_FP32))) return done(rc);
+            hipLaunchKernelGGL(sub_kernel, dim3(blocks_for(S.n8)), dim3(256), 0, s, b_pad, r_pad, r_pad, S.n8);
+            if ((rc = S.norms(r_pad, res))) return done(rc);
+            bool need64 = false, last = true;
+            for (int r = 0; r < kB; ++r) {
+              if (!active[r]) continue;
+              need64 = need64 || res[r] <= std::max(atol[r], kFp32ResidualFloor * bnorm[r]);
+              // the next sweep gains the same factor as this one: if that is
+              // enough for every system, its residual is taken in FP64 directly
+              last = last && prev_res[r] > 0.0 && res[r] * (res[r] / prev_res[r]) <= 0.5 * atol[r];
+            }
+            certified = need64;
+            fp64_residuals = need64 || last;
+          }
+          if (certified) {
+            if ((rc = S.master(x_pad, r_pad, SMH_FP64))) return done(rc);
+            hipLaunchKernelGGL(sub_kernel, dim3(blocks_for(S.n8)), dim3(256), 0, s, b_pad, r_pad, r_pad, S.n8);
+            if ((rc = S.norms(r_pad, res))) return done(rc);
+            fresh = true;
+          }
         }
         bool act[kB], any = false;
         double inner_atol[kB];
         for (int r = 0; r < kB; ++r) {
+          prev_res[r] = res[r];
           act[r] = active[r] && res[r] > atol[r];
-          inner_atol[r] = std::max(atol[r] * 0.5, kInnerRel * res[r]);
+          inner_atol[r] = std::max(atol[r] * 0.5, (mode == SMH_SOLVE_MIXED16 ? kInnerRel16 : kInnerRel) * res[r]);
           any = any || act[r];
         }
         if (!any || budget <= 0) break;
         SMH_HIP(ctx, hipMemsetAsync(d_pad, 0, S.n8 * sizeof(double2), s));
         double inner_res[kB];
         int inner_steps = 0;
-        if ((rc = S.gmres(d_pad, r_pad, inner_atol, act, budget, SMH_FP32, inner_res, inner_steps, true, true)))
+        if ((rc = S.gmres(d_pad, r_pad, inner_atol, act, budget, mode == SMH_SOLVE_MIXED16 ? SMH_FP16 : SMH_FP32,
+                          inner_res, inner_steps, true, true)))
           return done(rc);
         budget -= inner_steps;
         steps_total += inner_steps;
@@ -674,7 +733,7 @@ static int solve_batched(smh_op* op, const double* t_diag, const double* rhs, do
     stats[4] = secs;
     stats[5] = sweeps_total;
     stats[6] = (double)kB;  // right-hand sides per batched matvec
-    stats[7] = 0;
+    stats[7] = (double)S.mv16;
   }
   if (!converged_all) {
     smh_fail(ctx, SMH_ERR_NOCONV, "GMRES stopped at relative residual %.3e (tol %.1e)", worst, tol);
@@ -690,7 +749,10 @@ extern "C" int smh_solve(smh_op* op, const double* t_diag, const double* rhs, do
   SMH_CHECK(ctx, t_diag && rhs && x && nrhs >= 1, SMH_ERR_ARG, "invalid solve arguments");
   SMH_CHECK(ctx, tol > 0 && restart >= 1 && restart < kMaxBasis && maxiter >= 1, SMH_ERR_ARG,
             "invalid tolerance / restart / maxiter");
-  SMH_CHECK(ctx, mode == SMH_SOLVE_FP64 || mode == SMH_SOLVE_MIXED, SMH_ERR_ARG, "invalid mode %d", mode);
+  SMH_CHECK(ctx, mode == SMH_SOLVE_FP64 || mode == SMH_SOLVE_MIXED || mode == SMH_SOLVE_MIXED16, SMH_ERR_ARG,
+            "invalid mode %d", mode);
+  SMH_CHECK(ctx, mode != SMH_SOLVE_MIXED16 || (smh::op_has_batch(op, SMH_FP16) && smh::op_has_batch(op, SMH_FP64)),
+            SMH_ERR_UNSUPPORTED, "FP16-operand solves need a radial operator with the batched FP16 kernel");
   SMH_CHECK(ctx, op->has_table, SMH_ERR_STATE, "coupling table not set");
   SMH_HIP(ctx, hipSetDevice(ctx->device));
   const auto t0 = std::chrono::steady_clock::now();
@@ -723,7 +785,7 @@ extern "C" int smh_solve(smh_op* op, const double* t_diag, const double* rhs, do
     return done(code);
   };
   hipStream_t s = ctx->stream;
-  if (nrhs > 1 && smh::op_has_batch(op, SMH_FP32) && smh::op_has_batch(op, SMH_FP64)) {
+  if ((nrhs > 1 && smh::op_has_batch(op, SMH_FP32) && smh::op_has_batch(op, SMH_FP64)) || mode == SMH_SOLVE_MIXED16) {
     const int code = solve_batched(op, t_diag, rhs, x, nrhs, tol, restart, maxiter, mode, stats, t0);
     return done2(code);
   }

@@ -26,6 +26,14 @@ _LU_TOLERANCE = 1e-12
 _INTERP = {'linear': _lib.INTERP_LINEAR, 'catmull-rom': _lib.INTERP_CATMULL_ROM, 'spline': _lib.INTERP_SPLINE}
 
 
+def _precision_code(precision):
+    if precision in ('single', 'fp32', _lib.FP32):
+        return _lib.FP32
+    if precision in ('half', 'fp16', _lib.FP16):
+        return _lib.FP16
+    return _lib.FP64
+
+
 def _interp_code(interpolator_kind, cubic_interpolant):
     if interpolator_kind == 'linear':
         return _lib.INTERP_LINEAR
@@ -68,11 +76,13 @@ class _GpuCouplingOperator(SystemMatrix):
         self._mmax = np.ascontiguousarray([p.m_max for p in particle_list], dtype=np.int32)
 
     def matvec(self, vector, precision='single'):
-        """W x for one vector or a [nrhs, n] block (complex128 in and out)."""
+        """W x for one vector or a [nrhs, n] block (complex128 in and out).
+        precision: 'single' (FP32), 'double' (FP64) or 'half' (FP16 operands,
+        FP32 accumulation; radial operators, a few 1e-4 relative error)."""
         v = np.ascontiguousarray(vector, dtype=np.complex128)
         nrhs = 1 if v.ndim == 1 else v.shape[0]
         out = np.empty_like(v)
-        prec = _lib.FP32 if precision in ('single', 'fp32', _lib.FP32) else _lib.FP64
+        prec = _precision_code(precision)
         _lib.check(_lib.load().smh_op_matvec(self._op, _lib.dptr(v), _lib.dptr(out), nrhs, prec), cu.context(),
                    'smh_op_matvec')
         return out
@@ -90,7 +100,7 @@ class _GpuCouplingOperator(SystemMatrix):
         """(kernel ms, matvec ms) of one batched launch (8 right-hand sides) on device-resident data."""
         k = ctypes.c_double()
         t = ctypes.c_double()
-        prec = _lib.FP32 if precision in ('single', 'fp32') else _lib.FP64
+        prec = _precision_code(precision)
         _lib.check(_lib.load().smh_op_bench_matvec_batch(self._op, prec, iters, ctypes.byref(k), ctypes.byref(t)),
                    cu.context(), 'smh_op_bench_matvec_batch')
         return k.value, t.value
@@ -107,7 +117,7 @@ class _GpuCouplingOperator(SystemMatrix):
         nrhs = 1 if b.ndim == 1 else b.shape[0]
         x = np.empty_like(b)
         stats = np.zeros(8)
-        mode = _lib.SOLVE_MIXED if precision == 'mixed' else _lib.SOLVE_FP64
+        mode = {'mixed': _lib.SOLVE_MIXED, 'mixed16': _lib.SOLVE_MIXED16}.get(precision, _lib.SOLVE_FP64)
         maxiter = 10 * self.shape[0] if maxiter is None else maxiter
         maxiter = int(min(maxiter, 2 ** 31 - 1))
         code = _lib.load().smh_solve(self._op, _lib.dptr(t), _lib.dptr(b), _lib.dptr(x), nrhs, float(tol),
@@ -116,7 +126,8 @@ class _GpuCouplingOperator(SystemMatrix):
         if not with_stats or converged:
             _lib.check(code, cu.context(), 'smh_solve')
         st = dict(iterations=int(stats[0]), matvecs_fp64=int(stats[1]), matvecs_fp32=int(stats[2]),
-                  relative_residual=float(stats[3]), seconds=float(stats[4]), sweeps=int(stats[5]))
+                  relative_residual=float(stats[3]), seconds=float(stats[4]), sweeps=int(stats[5]),
+                  matvecs_fp16=int(stats[7]))
         return (x, st, converged) if with_stats else x
 
     def _make_linear_operator(self):
@@ -376,8 +387,10 @@ class MasterMatrix(SystemMatrix):
 class LinearSystem:
     """Assemble and solve (1 - T W) b = T a with smuthi's constructor arguments
     (linear_system.py:42-344). The coupling is a lookup operator or the
-    explicit matrix, both built on the GPU; the solve runs on the GPU. Extra arguments: precision ('mixed': FP32 coupling
-    matvecs inside FP64 iterative refinement; 'double': plain FP64 GMRES),
+    explicit matrix, both built on the GPU; the solve runs on the GPU.
+    Extra arguments: precision ('mixed': FP32 coupling matvecs inside FP64
+    iterative refinement; 'mixed16': FP16-operand matvecs inside the same
+    refinement, radial lookups; 'double': plain FP64 GMRES),
     cubic_interpolant, restart and maxiter."""
 
     def __init__(self, particle_list, initial_field, layer_system, k_parallel='default', solver_type='gmres',

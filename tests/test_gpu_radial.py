@@ -81,10 +81,15 @@ def test_gpu_batched_matvec_orders(gpu, order):
     V = rng.standard_normal((9, ref_op.n_unk)) + 1j * rng.standard_normal((9, ref_op.n_unk))
     W64 = op.matvec(V, precision='double')
     W32 = op.matvec(V, precision='single')
+    # FP16 operands (FP16 interpolant coefficients scaled per bin, FP16 inputs scaled by max |x|):
+    # the slot-order orders only
+    W16 = op.matvec(V, precision='half') if order != (4, 4) else None
     for r in range(9):
         want = ref_op.matvec(V[r])
         assert np.linalg.norm(W64[r] - want) / np.linalg.norm(want) < 1e-12, r
         assert np.linalg.norm(W32[r] - want) / np.linalg.norm(want) < 3e-6, r
+        if W16 is not None:
+            assert np.linalg.norm(W16[r] - want) / np.linalg.norm(want) < 2e-3, r
 
 
 @pytest.mark.parametrize('kind,cubic,okind', [('linear', 'spline', 'linear'), ('cubic', 'spline', 'cubic'),

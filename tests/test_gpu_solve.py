@@ -109,7 +109,7 @@ def test_multiple_rhs_match_single(gpu):
         assert np.linalg.norm(X[j] - sols[j]) / np.linalg.norm(sols[j]) < 1e-7
 
 
-@pytest.mark.parametrize('precision', ['double', 'mixed'])
+@pytest.mark.parametrize('precision', ['double', 'mixed', 'mixed16'])
 def test_batched_solve_matches_oracle_lu(gpu, precision):
     """Ten right-hand sides (one full batch of 8 and one of 2) against the oracle's LU."""
     from smuthi_amd import ensembles
@@ -130,6 +130,30 @@ def test_batched_solve_matches_oracle_lu(gpu, precision):
     X_ref = np.linalg.solve(M, R.T).T
     X, stats, ok = op.solve_master(t, R, tol=1e-9, precision=precision, with_stats=True)
     assert ok and stats['relative_residual'] <= 1e-9
+    if precision == 'mixed16':
+        # FP16 inner matvecs; FP32 residuals between sweeps; FP64 certificates
+        assert stats['matvecs_fp16'] > 0 and stats['matvecs_fp64'] >= 1
     for j in range(10):
         assert np.linalg.norm(X[j] - X_ref[j]) / np.linalg.norm(X_ref[j]) < 1e-7, j
         assert np.linalg.norm(R[j] - M @ X[j]) / np.linalg.norm(R[j]) < 2e-9, j
+
+
+def test_single_rhs_mixed16_solve(gpu):
+    """One right-hand side with FP16-operand inner matvecs (padded into one batched launch)."""
+    from smuthi_amd import ensembles
+    from smuthi_amd.layers import LayerSystem
+    from smuthi_amd.linear_system import CouplingMatrixRadialLookupCUDA
+    pp = ensembles.small_substrate_ensemble(seed=3)[:20]
+    plist = [osys.Sphere(p.position, p.refractive_index, p.radius, 3, 2) for p in pp]
+    kp = ob.reasonable_Sommerfeld_kpar_contour(WL, layer_refractive_indices=[1.52, 1], neff_resolution=1e-2)
+    op = CouplingMatrixRadialLookupCUDA(WL, plist, LayerSystem([0, 0], [1.52, 1]), k_parallel=kp, resolution=5.0,
+                                        interpolator_kind='linear')
+    W = oc.RadialLookupCPU(op.lookup_table, op.radial_distance_array, plist, 'linear').dense()
+    rng = np.random.default_rng(6)
+    n = W.shape[0]
+    t = 0.25 - 0.1j + 0.05 * (rng.standard_normal(n) + 1j * rng.standard_normal(n))
+    r = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    M = np.eye(n) - t[:, None] * W
+    x, stats, ok = op.solve_master(t, r, tol=1e-6, precision='mixed16', with_stats=True)
+    assert ok and stats['relative_residual'] <= 1e-6 and stats['matvecs_fp16'] > 0
+    assert np.linalg.norm(x - np.linalg.solve(M, r)) / np.linalg.norm(x) < 1e-5

@@ -7,8 +7,8 @@
 //   * the bin's interpolant is stored as FP16 polynomial coefficients scaled
 //     by a per-bin power of two (hscale[bin]; |value| <= 2^12), evaluated by
 //     Horner in packed FP16 (two slots per instruction);
-//   * the rotated input e^{i m2 phi} x / xmax is formed in FP32 and packed to
-//     FP16 per 4 slots;
+//   * the input is stored in FP16 as x / xmax (re and im planes per 4 slots)
+//     and rotated by e^{i m2 phi} in packed FP16;
 //   * one MFMA covers 16 slots (4 slot groups): lane (q, rr) holds the 4 slots
 //     of group 4 c + q of chunk c as its K values, so its azimuthal phase
 //     is chosen per lane from the |m| values of the chunk's groups (one-hot
@@ -25,9 +25,10 @@
 //   coefficients [bins][T][MT][64 lanes][NC][4] FP16: lane (q, rr) of tile i
 //         gets c_t(slot 16 (i mod MH) + rr, slot 16 c + 4 q + j), tiles
 //         i < MH from Re L, the others from Im L;
-//   x     [n_part][4 (q)][kBatch][NC][4 (j)] float2 (b16_index): the 4 NC
-//         inputs a lane needs are contiguous (the (rhs, re) and (rhs, im)
-//         lanes read the same bytes and pick their operand order).
+//   x     [n_part][4 (q)][kBatch][NC][2 (re, im)][4 (j)] FP16 (b16_index): a
+//         lane reads its chunk's 4 values of one plane as its first operand
+//         (re for the re column, im for the im column) and the other plane
+//         as the second, two 8-byte loads per chunk.
 #pragma once
 
 #include "smh_batch_slots.h"
@@ -56,13 +57,12 @@ __global__ void __launch_bounds__(512) batch_half_kernel(RadialKArgs a, const fl
   float sh = 0.f, ish = 0.f, sl = 0.f, isl = 0.f;
   fixed_scales(a.item_lsum[item], xmax, a.item_en[item], sh, ish, sl, isl);
   const float slo = sl * ish;
-  const float xinv = xmax > 0.f ? 1.f / xmax : 0.f;
 
   const int tid = threadIdx.x, lane = tid & 63;
   const int2 it = a.items[item];
   const int e_end = item + 1 < a.n_items ? a.items[item + 1].y : a.n_entries;
   const PairRec<float>* recs = reinterpret_cast<const PairRec<float>*>(a.recs);
-  const float2* xg = reinterpret_cast<const float2*>(a.x);
+  const _Float16* xg = reinterpret_cast<const _Float16*>(a.x);
   const uint4* tth = reinterpret_cast<const uint4*>(a.tt);
   for (int j = tid; j < a.Rc * SY; j += blockDim.x) yacc[j] = 0;
   if (tid == 0) *next_entry = it.y;
@@ -100,7 +100,8 @@ __global__ void __launch_bounds__(512) batch_half_kernel(RadialKArgs a, const fl
     zhi[c] = (sg > 0 ? 0u : 0x80000000u) ^ zmask;  // slots j >= 2 (m < 0 in mixed groups)
   }
   const int colbase = (rhs * 2 + zp) * SC + 4 * q;
-  const int64_t xlane = (int64_t)(q * KB + rhs) * NC * 4;
+  // first / second operand plane of the lane: (re, im) for the re column, (im, re) for the im column
+  const int64_t xlane = (int64_t)(q * KB + rhs) * NC * 8 + 4 * zp, xother = 4 - 8 * zp;
 
   for (;;) {
     int e0 = 0;
@@ -137,13 +138,16 @@ __global__ void __launch_bounds__(512) batch_half_kernel(RadialKArgs a, const fl
         }
       }
     const int p_end = en.z;
-    auto load_x = [&](const PairRec<float>& r, float4 (&xs)[2 * NC]) {
-      const float4* xp = reinterpret_cast<const float4*>(xg + (int64_t)(r.ij & kPairMask) * S * KB + xlane);
+    auto load_x = [&](const PairRec<float>& r, uint2 (&xs)[2 * NC]) {
+      const _Float16* xp = xg + (int64_t)(r.ij & kPairMask) * S * KB * 2 + xlane;
 #pragma unroll
-      for (int k = 0; k < 2 * NC; ++k) xs[k] = xp[k];
+      for (int c = 0; c < NC; ++c) {
+        xs[2 * c] = *reinterpret_cast<const uint2*>(xp + 8 * c);
+        xs[2 * c + 1] = *reinterpret_cast<const uint2*>(xp + 8 * c + xother);
+      }
     };
     PairRec<float> rec_n = recs[en.y];
-    float4 xc[2 * NC];
+    uint2 xc[2 * NC];
     load_x(rec_n, xc);
     for (int p = 0; p < p_end; ++p) {
       const PairRec<float> rec = rec_n;
@@ -155,7 +159,7 @@ __global__ void __launch_bounds__(512) batch_half_kernel(RadialKArgs a, const fl
       const float cs[4] = {1.f, c1, c2, c3};
       const float sn[4] = {0.f, s1, s2, s3};
 
-      // B operands: per chunk the rotated, scaled input of the lane's 4 slots in FP16
+      // B operands: per chunk the rotated input of the lane's 4 slots, z = cz a + e b in packed FP16
       half4_t zb[NC];
 #pragma unroll
       for (int c = 0; c < NC; ++c) {
@@ -168,18 +172,13 @@ __global__ void __launch_bounds__(512) batch_half_kernel(RadialKArgs a, const fl
           cz = fmaf(zhot[c][am], cs[am], cz);
           if (am > 0) sz = fmaf(zhot[c][am], sn[am], sz);
         }
-        cz *= xinv;
-        sz *= xinv;
-        const float elo = xor_sign(sz, zlo[c]), ehi = xor_sign(sz, zhi[c]);
-        const float4 xa = xc[2 * c], xb = xc[2 * c + 1];  // slots j = 0, 1 and 2, 3 as (re, im)
-        const float av0 = zp ? xa.y : xa.x, bv0 = zp ? xa.x : xa.y;
-        const float av1 = zp ? xa.w : xa.z, bv1 = zp ? xa.z : xa.w;
-        const float av2 = zp ? xb.y : xb.x, bv2 = zp ? xb.x : xb.y;
-        const float av3 = zp ? xb.w : xb.z, bv3 = zp ? xb.z : xb.w;
-        zb[c][0] = (_Float16)fmaf(cz, av0, elo * bv0);
-        zb[c][1] = (_Float16)fmaf(cz, av1, elo * bv1);
-        zb[c][2] = (_Float16)fmaf(cz, av2, ehi * bv2);
-        zb[c][3] = (_Float16)fmaf(cz, av3, ehi * bv3);
+        const _Float16 czh = (_Float16)cz, eloh = (_Float16)xor_sign(sz, zlo[c]), ehih = (_Float16)xor_sign(sz, zhi[c]);
+        const half2_t cz2 = {czh, czh}, elo2 = {eloh, eloh}, ehi2 = {ehih, ehih};
+        const half2_t a01 = __builtin_bit_cast(half2_t, xc[2 * c].x), a23 = __builtin_bit_cast(half2_t, xc[2 * c].y);
+        const half2_t b01 = __builtin_bit_cast(half2_t, xc[2 * c + 1].x);
+        const half2_t b23 = __builtin_bit_cast(half2_t, xc[2 * c + 1].y);
+        const half2_t z01 = a01 * cz2 + b01 * elo2, z23 = a23 * cz2 + b23 * ehi2;
+        zb[c] = half4_t{z01[0], z01[1], z23[0], z23[1]};
       }
       load_x(rec_n, xc);
 

@@ -162,6 +162,25 @@ __global__ void xmax_kernel(const float2* __restrict__ x, int64_t n, float* xmax
   if (threadIdx.x == 0) *xmax = red[0];
 }
 
+// The same bound over an FP64 input (the FP16-operand kernel's input is
+// scaled by it while being converted).
+__global__ void xmax64_kernel(const double2* __restrict__ x, int64_t n, float* xmax) {
+  __shared__ float red[1024];
+  float m = 0.f;
+  for (int64_t j = threadIdx.x; j < n; j += blockDim.x) {
+    const double2 v = x[j];
+    m = fmaxf(m, (float)(fabs(v.x) + fabs(v.y)));
+  }
+  red[threadIdx.x] = m;
+  __syncthreads();
+  for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) red[threadIdx.x] = fmaxf(red[threadIdx.x], red[threadIdx.x + w]);
+    __syncthreads();
+  }
+  // one ulp of headroom: the scaled inputs stay inside [-1, 1]
+  if (threadIdx.x == 0) *xmax = red[0] * (1.f + 1.2e-7f);
+}
+
 // Work items per compute unit: items of one receiver chunk run one after the
 // other on a CU's single workgroup slot, so ~16 rounds keep the last,
 // partly filled round short.

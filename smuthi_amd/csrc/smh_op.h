@@ -99,15 +99,18 @@ __host__ __device__ inline int64_t b32_index(int64_t p, int n, int r, int S, con
   const int k = slot[n];
   return p * S * kBatchRhs + ((k & 3) * kBatchRhs + r) * (S / 4) + (k >> 2);
 }
-// Index of (particle p, multipole n, rhs r) in the input of the FP16-operand
-// kBatch-RHS kernel: [n_part][4 (q)][kBatch][S/16 (c)][4 (j)] float2 for slot
-// k = slot[n] = 16 c + 4 q + j (the re-column and im-column lanes of a
-// right-hand side read the same 16-byte pairs).
+// Index (in halves) of Re x for (particle p, multipole n, rhs r) in the FP16
+// input of the FP16-operand kBatch-RHS kernel: [n_part][4 (q)][kBatch][S/16
+// (c)][2 (re, im)][4 (j)] for slot k = slot[n] = 16 c + 4 q + j; Im x sits 4
+// halves later. The values are x / xmax, xmax the input's largest |Re| + |Im|.
 __host__ __device__ inline int64_t b16_index(int64_t p, int n, int r, int S, const int* slot) {
   const int k = slot[n];
   const int c = k >> 4, q = (k >> 2) & 3, j = k & 3;
-  return p * S * kBatchRhs + ((int64_t)(q * kBatchRhs + r) * (S / 16) + c) * 4 + j;
+  return p * S * kBatchRhs * 2 + ((int64_t)(q * kBatchRhs + r) * (S / 16) + c) * 8 + j;
 }
+// FP64 padded kBatch-RHS vectors [n_part][B][kBatch] -> the FP16 input above
+// (with the scale written to op->d_xmax); padding slots of out must be zero.
+int op_to_half_batch(smh_op* op, const double2* v, void* out);
 // packed -> padded index map of the particles' multipole blocks
 int op_index_map(smh_op* op, const int* lmax, const int* mmax);
 // volume operator: matvec, table upload / download (smh_volume.hip)

@@ -547,7 +547,7 @@ namespace {
 // layout of b32_index() instead (S slots per particle, slot table given).
 template <typename Real>
 __global__ void scatter_batch_kernel(const double2* __restrict__ pk, const int* __restrict__ pk2pad, int64_t n_unk,
-                                     int nrhs, int B, int S, const int* __restrict__ slot, int half_layout,
+                                     int nrhs, int B, int S, const int* __restrict__ slot,
                                      typename Vec2<Real>::t* pad) {
   const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= n_unk * nrhs) return;
@@ -555,9 +555,22 @@ __global__ void scatter_batch_kernel(const double2* __restrict__ pk, const int* 
   const double2 v = pk[j];
   const int ip = pk2pad[u];
   const int64_t p = ip / B;
-  const int n = (int)(ip - p * B);
-  pad[half_layout ? b16_index(p, n, (int)r, S, slot) : b32_index(p, n, (int)r, S, slot)] =
-      typename Vec2<Real>::t{Real(v.x), Real(v.y)};
+  pad[b32_index(p, (int)(ip - p * B), (int)r, S, slot)] = typename Vec2<Real>::t{Real(v.x), Real(v.y)};
+}
+
+// FP64 padded [n_part][B][kBatch] -> FP16 input of the FP16-operand kernel
+// (b16_index layout, values divided by *xmax)
+__global__ void to_half_b16_kernel(const double2* __restrict__ v, int64_t n8, int B, int S,
+                                   const int* __restrict__ slot, const float* __restrict__ xmax, _Float16* out) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n8) return;
+  const int r = (int)(e % kBatch);
+  const int64_t ip = e / kBatch, p = ip / B;
+  const float m = *xmax;
+  const double inv = m > 0.f ? 1.0 / (double)m : 0.0;
+  const int64_t k = b16_index(p, (int)(ip - p * B), r, S, slot);
+  out[k] = (_Float16)(v[e].x * inv);
+  out[k + 4] = (_Float16)(v[e].y * inv);
 }
 
 template <typename Real>
@@ -601,16 +614,35 @@ namespace smh {
 int op_unpack_batch(smh_op* op, int precision, const double2* pk, int nrhs, void* pad) {
   hipStream_t s = op->ctx->stream;
   const int64_t n = op->n_unk * nrhs;
-  if (precision == SMH_FP32 || precision == SMH_FP16) {
+  if (precision == SMH_FP32) {
     const int* slot = op->d_b32_tab;
     SMH_HIP(op->ctx, hipMemsetAsync(pad, 0, (size_t)op->n_part * op->b32_S * kBatch * sizeof(float2), s));
     hipLaunchKernelGGL(scatter_batch_kernel<float>, dim3(blocks_for(n)), dim3(256), 0, s, pk, op->d_pk2pad,
-                       op->n_unk, nrhs, op->B, op->b32_S, slot, precision == SMH_FP16 ? 1 : 0, (float2*)pad);
+                       op->n_unk, nrhs, op->B, op->b32_S, slot, (float2*)pad);
+  } else if (precision == SMH_FP16) {
+    // through the FP64 padded layout (in the output buffer, free until the matvec runs)
+    SMH_CHECK(op->ctx, op->d_yb != nullptr, SMH_ERR_STATE, "batched buffers not reserved");
+    SMH_HIP(op->ctx, hipMemsetAsync(op->d_yb, 0, (size_t)op->n_part * op->B * kBatch * sizeof(double2), s));
+    hipLaunchKernelGGL(scatter_batch_kernel<double>, dim3(blocks_for(n)), dim3(256), 0, s, pk, op->d_pk2pad,
+                       op->n_unk, nrhs, op->B, op->B, (const int*)nullptr, (double2*)op->d_yb);
+    SMH_HIP(op->ctx, hipGetLastError());
+    return op_to_half_batch(op, (const double2*)op->d_yb, pad);
   } else {
     SMH_HIP(op->ctx, hipMemsetAsync(pad, 0, (size_t)op->n_part * op->B * kBatch * sizeof(double2), s));
     hipLaunchKernelGGL(scatter_batch_kernel<double>, dim3(blocks_for(n)), dim3(256), 0, s, pk, op->d_pk2pad,
-                       op->n_unk, nrhs, op->B, op->B, (const int*)nullptr, 0, (double2*)pad);
+                       op->n_unk, nrhs, op->B, op->B, (const int*)nullptr, (double2*)pad);
   }
+  SMH_HIP(op->ctx, hipGetLastError());
+  return SMH_OK;
+}
+
+int op_to_half_batch(smh_op* op, const double2* v, void* out) {
+  hipStream_t s = op->ctx->stream;
+  const int64_t n8 = op->n_part * op->B * kBatch;
+  hipLaunchKernelGGL(xmax64_kernel, dim3(1), dim3(1024), 0, s, v, n8, op->d_xmax);
+  SMH_HIP(op->ctx, hipMemsetAsync(out, 0, (size_t)op->n_part * op->b32_S * kBatch * 2 * sizeof(_Float16), s));
+  hipLaunchKernelGGL(to_half_b16_kernel, dim3(blocks_for(n8)), dim3(256), 0, s, v, n8, op->B, op->b32_S,
+                     (const int*)op->d_b32_tab, (const float*)op->d_xmax, (_Float16*)out);
   SMH_HIP(op->ctx, hipGetLastError());
   return SMH_OK;
 }
@@ -679,7 +711,8 @@ int op_apply_batch(smh_op* op, int precision, const void* x_pad, void* y_pad, bo
   const int64_t nx = op->n_part * op->B * kBatch;
   const int S = slots ? k.slots : op->B;
   const int* slot = slots ? op->d_b32_tab : nullptr;
-  if (pi == 0)
+  // (the FP16 input was scaled by the xmax its conversion wrote)
+  if (pi == 0 && !half)
     hipLaunchKernelGGL(xmax_kernel, dim3(1), dim3(1024), 0, s, (const float2*)x_pad, op->n_part * S * kBatch,
                        op->d_xmax);
   if (bn.n_items > 0) {
@@ -1066,9 +1099,15 @@ extern "C" int smh_op_bench_matvec_batch(smh_op* op, int precision, int iters, d
   int rc = op_reserve_batch(op);
   if (rc) return rc;
   hipStream_t s = ctx->stream;
-  // synthetic input in the precision's layout (the other buffer is scratch for the unused copy)
+  // synthetic input in the precision's layout (the other buffer is scratch for the unused copy;
+  // the FP16 input is converted from an FP64 pattern in the output buffer)
   const int64_t n = op->n_part * (precision == SMH_FP64 ? op->B : op->b32_S) * kBatch;
-  if (precision != SMH_FP64)
+  if (precision == SMH_FP16) {
+    const int64_t n64 = op->n_part * op->B * kBatch;
+    hipLaunchKernelGGL(fill_pattern_kernel, dim3(blocks_for(n64)), dim3(256), 0, s, (float2*)nullptr,
+                       (double2*)op->d_yb, n64);
+    if ((rc = op_to_half_batch(op, (const double2*)op->d_yb, op->d_xb))) return rc;
+  } else if (precision == SMH_FP32)
     hipLaunchKernelGGL(fill_pattern_kernel, dim3(blocks_for(n)), dim3(256), 0, s, (float2*)op->d_xb,
                        (double2*)nullptr, n);
   else

@@ -378,16 +378,6 @@ __global__ void to_fp32_b32_kernel(const double2* __restrict__ v, float2* out, i
   out[smh::b32_index(p, (int)(ip - p * B), r, S, slot)] = make_float2((float)v[e].x, (float)v[e].y);
 }
 
-// FP64 padded [n][kB] -> FP32 input of the FP16-operand kernel (b16_index layout)
-__global__ void to_fp32_b16_kernel(const double2* __restrict__ v, float2* out, int64_t n8, int B, int S,
-                                   const int* __restrict__ slot) {
-  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= n8) return;
-  const int r = (int)(e % kB);
-  const int64_t ip = e / kB, p = ip / B;
-  out[smh::b16_index(p, (int)(ip - p * B), r, S, slot)] = make_float2((float)v[e].x, (float)v[e].y);
-}
-
 __global__ void bmaster_fp32_kernel(const double2* __restrict__ v, const double2* __restrict__ t,
                                     const float2* __restrict__ y, double2* out, int64_t n8) {
   const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -412,7 +402,7 @@ struct BatchSolver {
   double* scl = nullptr;       // [kB]
   double2* y64 = nullptr;      // coupling output FP64 [n][kB]
   float2* x32 = nullptr;
-  float2* x16 = nullptr;  // FP32 input in the FP16-operand kernel's layout (allocated on first use)
+  void* x16 = nullptr;  // FP16 input of the FP16-operand kernel (allocated on first use)
   float2* y32 = nullptr;
   int64_t mv64 = 0, mv32 = 0, mv16 = 0, iters = 0;
 
@@ -444,13 +434,8 @@ struct BatchSolver {
       hipLaunchKernelGGL(bmaster_fp64_kernel, dim3(blocks_for(n8)), dim3(256), 0, s, v, t, y64, out, n8);
       ++mv64;
     } else if (precision == SMH_FP16) {
-      if (!x16) {
-        const size_t n16 = (size_t)op->n_part * op->b32_S * kB;
-        SMH_HIP(ctx, hipMalloc(&x16, n16 * sizeof(float2)));
-        SMH_HIP(ctx, hipMemsetAsync(x16, 0, n16 * sizeof(float2), s));
-      }
-      hipLaunchKernelGGL(to_fp32_b16_kernel, dim3(blocks_for(n8)), dim3(256), 0, s, v, x16, n8, op->B, op->b32_S,
-                         (const int*)op->d_b32_tab);
+      if (!x16) SMH_HIP(ctx, hipMalloc(&x16, (size_t)op->n_part * op->b32_S * kB * 2 * sizeof(_Float16)));
+      if ((rc = smh::op_to_half_batch(op, v, x16))) return rc;
       if ((rc = smh::op_apply_batch(op, SMH_FP16, x16, y32, false))) return rc;
       hipLaunchKernelGGL(bmaster_fp32_kernel, dim3(blocks_for(n8)), dim3(256), 0, s, v, t, y32, out, n8);
       ++mv16;

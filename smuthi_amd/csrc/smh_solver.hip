@@ -658,9 +658,11 @@ static int solve_batched(smh_op* op, const double* t_diag, const double* rhs, do
             for (int r = 0; r < kB; ++r) {
               if (!active[r]) continue;
               need64 = need64 || res[r] <= std::max(atol[r], kFp32ResidualFloor * bnorm[r]);
-              // the next sweep gains the same factor as this one: if that is
-              // enough for every system, its residual is taken in FP64 directly
-              last = last && prev_res[r] > 0.0 && res[r] * (res[r] / prev_res[r]) <= 0.5 * atol[r];
+              // the next sweep gains about the factor this one did: if that
+              // takes every system to the tolerance or to the FP32 floor, its
+              // residual is taken in FP64 directly
+              last = last && prev_res[r] > 0.0 &&
+                     res[r] * (res[r] / prev_res[r]) <= std::max(0.5 * atol[r], kFp32ResidualFloor * bnorm[r]);
             }
             certified = need64;
             fp64_residuals = need64 || last;

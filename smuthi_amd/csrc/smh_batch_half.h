@@ -2,17 +2,18 @@
 //
 // The product of batch_slots_kernel (per pair: interpolate the lookup block,
 // apply it to all right-hand sides, rotate, accumulate in fixed point) with
-// FP16 operands on the 16x16x16 FP16 MFMA (FP32 accumulation):
+// FP16 operands on the FP16 MFMA (16x16x32, FP32 accumulation):
 //
 //   * the bin's interpolant is stored as FP16 polynomial coefficients scaled
 //     by a per-bin power of two (hscale[bin]; |value| <= 2^12), evaluated by
 //     Horner in packed FP16 (two slots per instruction);
 //   * the input is stored in FP16 as x / xmax (re and im planes per 4 slots)
 //     and rotated by e^{i m2 phi} in packed FP16;
-//   * one MFMA covers 16 slots (4 slot groups): lane (q, rr) holds the 4 slots
+//   * a chunk covers 16 slots (4 slot groups): lane (q, rr) holds the 4 slots
 //     of group 4 c + q of chunk c as its K values, so its azimuthal phase
 //     is chosen per lane from the |m| values of the chunk's groups (one-hot
-//     weights, as the receiver rotation of batch_slots_kernel);
+//     weights, as the receiver rotation of batch_slots_kernel); one
+//     16x16x32 MFMA takes two chunks;
 //   * result rows, the receiver rotation, the fixed-point accumulation and the
 //     partial-sum layout are those of batch_slots_kernel, with the scales
 //     hscale[bin] * xmax folded into the rotation.
@@ -37,9 +38,14 @@ namespace smh {
 
 typedef _Float16 half2_t __attribute__((ext_vector_type(2)));
 typedef _Float16 half4_t __attribute__((ext_vector_type(4)));
+typedef _Float16 half8_t __attribute__((ext_vector_type(8)));
+
+// 16 waves per workgroup: the receiver accumulator takes the CU's LDS, so the
+// workgroup is the CU's only one, and 4 waves per SIMD hide the gather latency
+constexpr int kHalfThreads = 1024;
 
 template <int LMAX, int MMAX, int T>
-__global__ void __launch_bounds__(512) batch_half_kernel(RadialKArgs a, const float* __restrict__ hscale) {
+__global__ void __launch_bounds__(kHalfThreads) batch_half_kernel(RadialKArgs a, const float* __restrict__ hscale) {
   using Sh = SlotShape<LMAX, MMAX, T>;
   constexpr int S = Sh::S, MT = Sh::MT, MH = Sh::MH, SC = Sh::SC, LO = Sh::LO, SY = Sh::SY;
   constexpr int NC = S / 16;
@@ -52,10 +58,13 @@ __global__ void __launch_bounds__(512) batch_half_kernel(RadialKArgs a, const fl
 
   extern __shared__ __align__(16) int yacc[];
   int* next_entry = yacc + (size_t)a.Rc * SY;
+  int* n_adds = next_entry + 1;  // pairs accumulated per receiver
   const int item = a.order ? a.order[blockIdx.x] : (int)blockIdx.x;
   const float xmax = *a.xmax;
   float sh = 0.f, ish = 0.f, sl = 0.f, isl = 0.f;
-  fixed_scales(a.item_lsum[item], xmax, a.item_en[item], sh, ish, sl, isl);
+  // at least 2^9 additions of headroom: every value added to either word is
+  // then below 2^21 in magnitude, the range of the rounding by kMagic below
+  fixed_scales(a.item_lsum[item], xmax, max(a.item_en[item], 9), sh, ish, sl, isl);
   const float slo = sl * ish;
 
   const int tid = threadIdx.x, lane = tid & 63;
@@ -65,13 +74,23 @@ __global__ void __launch_bounds__(512) batch_half_kernel(RadialKArgs a, const fl
   const _Float16* xg = reinterpret_cast<const _Float16*>(a.x);
   const uint4* tth = reinterpret_cast<const uint4*>(a.tt);
   for (int j = tid; j < a.Rc * SY; j += blockDim.x) yacc[j] = 0;
+  for (int j = tid; j < a.Rc; j += blockDim.x) n_adds[j] = 0;
   if (tid == 0) *next_entry = it.y;
   __syncthreads();
+  // every pair adds once to each word of its receiver: count them first
+  for (int e = it.y + (tid >> 6); e < e_end; e += blockDim.x >> 6) {
+    const int4 en = a.entries[e];
+    for (int p = lane; p < en.z; p += 64) atomicAdd(n_adds + (recs[en.y + p].ij >> kPairShift), 1);
+  }
 
   const int rr = lane & 15, q = lane >> 4;
   const int rhs = rr >> 1, zp = rr & 1;
   const unsigned zmask = zp ? 0u : 0x80000000u;  // z: sign of the b term for the re column
   const float sgnz = zp ? 1.f : -1.f;
+  // 1.5 * 2^23: v + kMagic rounds v (|v| < 2^22) to the nearest integer n,
+  // and the bits of the sum are kMagicBits + n; the words accumulate those
+  // bits, and the finish subtracts kMagicBits once per accumulated pair
+  constexpr float kMagic = 12582912.0f;
   // receiver rotation, as batch_slots_kernel: result rows of tile i are slot group 4 i + q
   float hot[MH][4];
   unsigned smlo[MH], smhi[MH];
@@ -103,22 +122,15 @@ __global__ void __launch_bounds__(512) batch_half_kernel(RadialKArgs a, const fl
   // first / second operand plane of the lane: (re, im) for the re column, (im, re) for the im column
   const int64_t xlane = (int64_t)(q * KB + rhs) * NC * 8 + 4 * zp, xother = 4 - 8 * zp;
 
-  for (;;) {
-    int e0 = 0;
-    if (lane == 0) e0 = atomicAdd(next_entry, 1);
-    const int e = __builtin_amdgcn_readfirstlane(e0);
-    if (e >= e_end) break;
-    const int4 en = a.entries[e];
-    const int bin = (en.x - a.row_o) / a.row_s;
-    const float bscale = hscale[bin] * xmax;
-    // polynomial coefficients: tv[t][i][c] = 4 FP16 values (2 packed pairs)
-    half2_t tv[T][MT][NC][2];
+  typedef half2_t Coef[T][MT][NC][2];
+  // polynomial coefficients of a bin: tv[t][i][c] = 4 FP16 values (2 packed pairs)
+  auto load_coef = [&](int bin, Coef& tv) {
 #pragma unroll
     for (int t = 0; t < T; ++t)
 #pragma unroll
       for (int i = 0; i < MT; ++i) {
-        const uint4* row = tth + (((size_t)bin * T + t) * MT + i) * 64 * (NC / 2 > 0 ? NC / 2 : 1);
         if constexpr (NC % 2 == 0) {
+          const uint4* row = tth + (((size_t)bin * T + t) * MT + i) * 64 * (NC / 2);
 #pragma unroll
           for (int c2 = 0; c2 < NC / 2; ++c2) {
             const uint4 v = row[lane * (NC / 2) + c2];
@@ -137,18 +149,27 @@ __global__ void __launch_bounds__(512) batch_half_kernel(RadialKArgs a, const fl
           }
         }
       }
-    const int p_end = en.z;
-    auto load_x = [&](const PairRec<float>& r, uint2 (&xs)[2 * NC]) {
-      const _Float16* xp = xg + (int64_t)(r.ij & kPairMask) * S * KB * 2 + xlane;
+  };
+  auto load_x = [&](const PairRec<float>& r, uint2 (&xs)[2 * NC]) {
+    const _Float16* xp = xg + (int64_t)(r.ij & kPairMask) * S * KB * 2 + xlane;
 #pragma unroll
-      for (int c = 0; c < NC; ++c) {
-        xs[2 * c] = *reinterpret_cast<const uint2*>(xp + 8 * c);
-        xs[2 * c + 1] = *reinterpret_cast<const uint2*>(xp + 8 * c + xother);
-      }
-    };
-    PairRec<float> rec_n = recs[en.y];
-    uint2 xc[2 * NC];
-    load_x(rec_n, xc);
+    for (int c = 0; c < NC; ++c) {
+      xs[2 * c] = *reinterpret_cast<const uint2*>(xp + 8 * c);
+      xs[2 * c + 1] = *reinterpret_cast<const uint2*>(xp + 8 * c + xother);
+    }
+  };
+  auto grab = [&]() {
+    int e0 = 0;
+    if (lane == 0) e0 = atomicAdd(next_entry, 1);
+    return __builtin_amdgcn_readfirstlane(e0);
+  };
+  auto bin_of = [&](const int4& en) { return (en.x - a.row_o) / a.row_s; };
+  uint2 xc[2 * NC];  // input of the next pair to run
+  // the pairs of one entry (one bin) with its coefficients tv; rec_n is its
+  // first pair, whose input is in xc
+  auto run_entry = [&](const int4& en, const Coef& tv, PairRec<float> rec_n) {
+    const float bscale = hscale[bin_of(en)] * xmax;
+    const int p_end = en.z;
     for (int p = 0; p < p_end; ++p) {
       const PairRec<float> rec = rec_n;
       rec_n = recs[en.y + (p + 1 < p_end ? p + 1 : p)];
@@ -187,23 +208,36 @@ __global__ void __launch_bounds__(512) batch_half_kernel(RadialKArgs a, const fl
       typename Mfma<float>::v4 acc[MT];
 #pragma unroll
       for (int i = 0; i < MT; ++i) acc[i] = typename Mfma<float>::v4{0, 0, 0, 0};
+      // interpolated coefficients of tile i, chunk c (Horner in packed FP16)
+      auto coef = [&](int i, int c) {
+        half2_t v0 = tv[T - 1][i][c][0], v1 = tv[T - 1][i][c][1];
 #pragma unroll
-      for (int c = 0; c < NC; ++c) {
+        for (int t = T - 2; t >= 0; --t) {
+          v0 = v0 * w2 + tv[t][i][c][0];
+          v1 = v1 * w2 + tv[t][i][c][1];
+        }
+        return half4_t{v0[0], v0[1], v1[0], v1[1]};
+      };
+      // two chunks per 16x16x32 MFMA (the full-rate FP16 form): the lane's 8
+      // K values are its 4 of chunk c and its 4 of chunk c + 1, in A and B alike
+#pragma unroll
+      for (int c = 0; c + 1 < NC; c += 2) {
+        const half8_t bv = __builtin_shufflevector(zb[c], zb[c + 1], 0, 1, 2, 3, 4, 5, 6, 7);
 #pragma unroll
         for (int i = 0; i < MT; ++i) {
-          half2_t v0 = tv[T - 1][i][c][0], v1 = tv[T - 1][i][c][1];
-#pragma unroll
-          for (int t = T - 2; t >= 0; --t) {
-            v0 = v0 * w2 + tv[t][i][c][0];
-            v1 = v1 * w2 + tv[t][i][c][1];
-          }
-          const half4_t av = {v0[0], v0[1], v1[0], v1[1]};
-          acc[i] = __builtin_amdgcn_mfma_f32_16x16x16f16(av, zb[c], acc[i], 0, 0, 0);
+          const half8_t av = __builtin_shufflevector(coef(i, c), coef(i, c + 1), 0, 1, 2, 3, 4, 5, 6, 7);
+          acc[i] = __builtin_amdgcn_mfma_f32_16x16x32_f16(av, bv, acc[i], 0, 0, 0);
         }
+      }
+      if constexpr (NC % 2 == 1) {
+#pragma unroll
+        for (int i = 0; i < MT; ++i)
+          acc[i] = __builtin_amdgcn_mfma_f32_16x16x16f16(coef(i, NC - 1), zb[NC - 1], acc[i], 0, 0, 0);
       }
 
       // y (rhs, re|im) of rows 16 i + 4 q + v, rotated by e^{-i m1 phi}, scaled by S_hi and the operand scales
-      int* yrow = yacc + (size_t)(rec.ij >> kPairShift) * SY + colbase;
+      const int il = rec.ij >> kPairShift;
+      int* yrow = yacc + (size_t)il * SY + colbase;
       const float scl = sh * bscale;
 #pragma unroll
       for (int i = 0; i < MH; ++i) {
@@ -223,16 +257,28 @@ __global__ void __launch_bounds__(512) batch_half_kernel(RadialKArgs a, const fl
         for (int v = 0; v < 4; ++v) {
           const float eps = v < 2 ? elo : ehi;
           const float al = acc[i][v], be = acc[i + MH][v];
-          const float own = fmaf(al, er, be * eps);
-          const float snd = fmaf(be, er, -(al * eps));
-          const float val = fma_xor1(own, sgnz, snd);
-          const float h = rintf(val);
-          const float l = rintf((val - h) * slo);
-          atomicAdd(yrow + 16 * i + v, (int)h);
-          atomicAdd(yrow + LO + 16 * i + v, (int)l);
+          // (al er + be eps) of this column + sgnz (be' er - al' eps) of the
+          // other column of the right-hand side (lane ^ 1)
+          const float u = fma_xor1(al, sgnz, be), w = fma_xor1(be, -sgnz, al);
+          const float val = fmaf(er, u, eps * w);
+          const float th = val + kMagic;
+          const float tl = fmaf(val - (th - kMagic), slo, kMagic);
+          atomicAdd(yrow + 16 * i + v, __float_as_int(th));
+          atomicAdd(yrow + LO + 16 * i + v, __float_as_int(tl));
         }
       }
     }
+  };
+
+  Coef tv;
+  for (;;) {
+    const int e = grab();
+    if (e >= e_end) break;
+    const int4 en = a.entries[e];
+    load_coef(bin_of(en), tv);
+    const PairRec<float> rec0 = recs[en.y];
+    load_x(rec0, xc);
+    run_entry(en, tv, rec0);
   }
   __syncthreads();
   float* yp = reinterpret_cast<float*>(a.ypart) + (size_t)item * a.Rc * 2 * KB * S;
@@ -240,7 +286,9 @@ __global__ void __launch_bounds__(512) batch_half_kernel(RadialKArgs a, const fl
     const int il = j / (2 * KB * S), k = j - il * 2 * KB * S;
     const int c = k / S, sl2 = k - c * S;
     const int w0 = il * SY + c * SC + sl2;
-    yp[j] = (float)((double)yacc[w0] * (double)ish + (double)yacc[w0 + LO] * (double)isl);
+    const unsigned bias = (unsigned)n_adds[il] * (unsigned)__float_as_int(kMagic);  // modulo 2^32
+    const int hi = (int)((unsigned)yacc[w0] - bias), lo = (int)((unsigned)yacc[w0 + LO] - bias);
+    yp[j] = (float)((double)hi * (double)ish + (double)lo * (double)isl);
   }
 }
 

@@ -25,11 +25,11 @@ struct KernelInfo {
 };
 
 // Receiver chunk of a precision: as many receivers (at most 512, the pair
-// record's receiver field) as the LDS accumulator [Rc][SY] fits in one
-// workgroup's LDS, less 1 KiB.
+// record's receiver field) as the LDS accumulator [Rc][SY] and one counter
+// int per receiver fit in one workgroup's LDS, less 1 KiB.
 inline int chunk_size(const smh_op* op, size_t real_size, int sy, size_t& lds) {
   const size_t limit = std::min<size_t>(op->ctx->lds_per_block ? op->ctx->lds_per_block : 65536, 163840) - 1024;
-  const size_t per = (size_t)sy * real_size;
+  const size_t per = (size_t)sy * real_size + sizeof(int);
   int rc = (int)std::min<size_t>(512, (limit - 16) / per);
   if (rc < 16) {
     lds = 0;

@@ -720,7 +720,8 @@ int op_apply_batch(smh_op* op, int precision, const void* x_pad, void* y_pad, bo
     const float* hs = op->d_hscale;
     void* args1[] = {&a};
     void* args2[] = {&a, &hs};
-    SMH_HIP(ctx, hipLaunchKernel(fn, dim3(bn.n_items), dim3(k.threads), half ? args2 : args1, bn.lds, s));
+    SMH_HIP(ctx, hipLaunchKernel(fn, dim3(bn.n_items), dim3(half ? kHalfThreads : k.threads), half ? args2 : args1,
+                                 bn.lds, s));
     if (time_kernel) SMH_HIP(ctx, hipEventRecord(ctx->ev1, s));
   }
   if (pi == 0)

@@ -74,7 +74,9 @@ __global__ void __launch_bounds__((BatchShape<Real, SvwfIdx<LMAX, MMAX>::B, T>::
   // work item of this block (launch order maps blocks to items, see RadialKArgs::order)
   const int item = a.order ? a.order[blockIdx.x] : (int)blockIdx.x;
   float sh = 0.f, ish = 0.f, sl = 0.f, isl = 0.f;
-  if (kFixed) fixed_scales(a.item_lsum[item], *a.xmax, a.item_en[item], sh, ish, sl, isl);
+  // headroom for the additions: 2^eN pairs per receiver, SPL parts each
+  const int eN = a.item_en[item] + (SPL > 1 ? 32 - __builtin_clz(SPL - 1) : 0);
+  if (kFixed) fixed_scales(a.item_lsum[item], *a.xmax, eN, sh, ish, sl, isl);
 
   const int tid = threadIdx.x, lane = tid & 63;
   const int2 it = a.items[item];

@@ -14,9 +14,10 @@
 //     is chosen per lane from the |m| values of the chunk's groups (one-hot
 //     weights, as the receiver rotation of batch_slots_kernel); one
 //     16x16x32 MFMA takes two chunks;
-//   * result rows, the receiver rotation, the fixed-point accumulation and the
-//     partial-sum layout are those of batch_slots_kernel, with the scales
-//     hscale[bin] * xmax folded into the rotation.
+//   * result rows, the receiver rotation and the partial-sum layout are those
+//     of batch_slots_kernel, with the scales hscale[bin] * xmax folded into
+//     the rotation; the receiver sums are those of batch_slots_kernel (one
+//     int64 fixed-point word per result, fixed_scale64).
 //
 // The operands carry FP16 rounding (relative 2^-11 of the bin / input scale):
 // a matvec is accurate to a few 1e-4 relative, which the FP64 iterative
@@ -47,7 +48,7 @@ constexpr int kHalfThreads = 1024;
 template <int LMAX, int MMAX, int T>
 __global__ void __launch_bounds__(kHalfThreads) batch_half_kernel(RadialKArgs a, const float* __restrict__ hscale) {
   using Sh = SlotShape<LMAX, MMAX, T>;
-  constexpr int S = Sh::S, MT = Sh::MT, MH = Sh::MH, SC = Sh::SC, LO = Sh::LO, SY = Sh::SY;
+  constexpr int S = Sh::S, MT = Sh::MT, MH = Sh::MH, SC = Sh::SC, LO = Sh::LO;
   constexpr int NC = S / 16;
   constexpr int KB = kBatchRhs;
   static_assert(KB * 2 == 16, "16 result columns = kBatch right-hand sides x (re, im)");
@@ -56,16 +57,16 @@ __global__ void __launch_bounds__(kHalfThreads) batch_half_kernel(RadialKArgs a,
   constexpr SlotOrder<LMAX, MMAX> so{};
   static_assert(so.ok, "slot order");
 
-  extern __shared__ __align__(16) int yacc[];
-  int* next_entry = yacc + (size_t)a.Rc * SY;
+  // [Rc][LO] int64 fixed-point words, as batch_slots_kernel
+  extern __shared__ __align__(16) unsigned long long half_acc[];
+  unsigned long long* yacc = half_acc;
+  int* next_entry = reinterpret_cast<int*>(yacc + (size_t)a.Rc * LO);
   int* n_adds = next_entry + 1;  // pairs accumulated per receiver
   const int item = a.order ? a.order[blockIdx.x] : (int)blockIdx.x;
   const float xmax = *a.xmax;
-  float sh = 0.f, ish = 0.f, sl = 0.f, isl = 0.f;
-  // at least 2^9 additions of headroom: every value added to either word is
-  // then below 2^21 in magnitude, the range of the rounding by kMagic below
-  fixed_scales(a.item_lsum[item], xmax, max(a.item_en[item], 9), sh, ish, sl, isl);
-  const float slo = sl * ish;
+  float sh = 0.f;
+  double ish = 0.0;
+  fixed_scale64(a.item_lsum[item], xmax, a.item_en[item], sh, ish);
 
   const int tid = threadIdx.x, lane = tid & 63;
   const int2 it = a.items[item];
@@ -73,7 +74,7 @@ __global__ void __launch_bounds__(kHalfThreads) batch_half_kernel(RadialKArgs a,
   const PairRec<float>* recs = reinterpret_cast<const PairRec<float>*>(a.recs);
   const _Float16* xg = reinterpret_cast<const _Float16*>(a.x);
   const uint4* tth = reinterpret_cast<const uint4*>(a.tt);
-  for (int j = tid; j < a.Rc * SY; j += blockDim.x) yacc[j] = 0;
+  for (int j = tid; j < a.Rc * LO; j += blockDim.x) yacc[j] = 0ull;
   for (int j = tid; j < a.Rc; j += blockDim.x) n_adds[j] = 0;
   if (tid == 0) *next_entry = it.y;
   __syncthreads();
@@ -87,10 +88,6 @@ __global__ void __launch_bounds__(kHalfThreads) batch_half_kernel(RadialKArgs a,
   const int rhs = rr >> 1, zp = rr & 1;
   const unsigned zmask = zp ? 0u : 0x80000000u;  // z: sign of the b term for the re column
   const float sgnz = zp ? 1.f : -1.f;
-  // 1.5 * 2^23: v + kMagic rounds v (|v| < 2^22) to the nearest integer n,
-  // and the bits of the sum are kMagicBits + n; the words accumulate those
-  // bits, and the finish subtracts kMagicBits once per accumulated pair
-  constexpr float kMagic = 12582912.0f;
   // receiver rotation, as batch_slots_kernel: result rows of tile i are slot group 4 i + q
   float hot[MH][4];
   unsigned smlo[MH], smhi[MH];
@@ -237,7 +234,7 @@ __global__ void __launch_bounds__(kHalfThreads) batch_half_kernel(RadialKArgs a,
 
       // y (rhs, re|im) of rows 16 i + 4 q + v, rotated by e^{-i m1 phi}, scaled by S_hi and the operand scales
       const int il = rec.ij >> kPairShift;
-      int* yrow = yacc + (size_t)il * SY + colbase;
+      unsigned long long* yrow = yacc + (size_t)il * LO + colbase;
       const float scl = sh * bscale;
 #pragma unroll
       for (int i = 0; i < MH; ++i) {
@@ -261,10 +258,7 @@ __global__ void __launch_bounds__(kHalfThreads) batch_half_kernel(RadialKArgs a,
           // other column of the right-hand side (lane ^ 1)
           const float u = fma_xor1(al, sgnz, be), w = fma_xor1(be, -sgnz, al);
           const float val = fmaf(er, u, eps * w);
-          const float th = val + kMagic;
-          const float tl = fmaf(val - (th - kMagic), slo, kMagic);
-          atomicAdd(yrow + 16 * i + v, __float_as_int(th));
-          atomicAdd(yrow + LO + 16 * i + v, __float_as_int(tl));
+          atomicAdd(yrow + 16 * i + v, fixed_bits64(val));
         }
       }
     }
@@ -285,10 +279,9 @@ __global__ void __launch_bounds__(kHalfThreads) batch_half_kernel(RadialKArgs a,
   for (int j = tid; j < a.Rc * 2 * KB * S; j += blockDim.x) {
     const int il = j / (2 * KB * S), k = j - il * 2 * KB * S;
     const int c = k / S, sl2 = k - c * S;
-    const int w0 = il * SY + c * SC + sl2;
-    const unsigned bias = (unsigned)n_adds[il] * (unsigned)__float_as_int(kMagic);  // modulo 2^32
-    const int hi = (int)((unsigned)yacc[w0] - bias), lo = (int)((unsigned)yacc[w0 + LO] - bias);
-    yp[j] = (float)((double)hi * (double)ish + (double)lo * (double)isl);
+    const int w0 = il * LO + c * SC + sl2;
+    const long long sum = (long long)(yacc[w0] - (unsigned long long)n_adds[il] * kMagic64Bits);  // modulo 2^64
+    yp[j] = (float)((double)sum * ish);
   }
 }
 

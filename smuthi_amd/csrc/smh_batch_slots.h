@@ -25,7 +25,7 @@
 //         tiles i < MH from Re L, the others from Im L (one dwordx4 per s4);
 //   x     [n_part][4 (q)][kBatch][NS] float2: the NS inputs a lane needs are
 //         contiguous (NS/2 dwordx4 loads per pair);
-//   LDS   [Rc][2 (hi, lo)][16 columns][S + 1] int32 (two-level fixed point);
+//   LDS   [Rc][16 columns][S + 1] int64 (fixed point, fixed_scale64);
 //   ypart [n_items][Rc][16 columns][S] float.
 #pragma once
 
@@ -105,11 +105,34 @@ template <int L, int M, int T>
 struct SlotShape {
   static constexpr int S = SvwfIdx<L, M>::Bh, NS = S / 4, MT = 2 * S / 16, MH = MT / 2;
   static constexpr int SC = S + 1;          // LDS column stride (words)
-  static constexpr int LO = 2 * kBatchRhs * SC;  // offset of the low words
-  static constexpr int SY = 2 * LO;         // LDS words per receiver
+  static constexpr int LO = 2 * kBatchRhs * SC;  // int64 accumulator words per receiver
+  static constexpr int SY = 2 * LO;         // LDS int32 words per receiver
   static constexpr int FRB = MT * NS * 64;  // floats per tap row
   static constexpr bool fits = T * MT * NS <= 128;
 };
+
+// Receiver sums of the slot kernels: one int64 fixed-point word per result.
+// A value v of a work item (|v| <= V = 8 lsum xmax) is added as
+// rint(v sh) through kMagic64 (1.5 * 2^52: the bits of u + kMagic64 are
+// kMagic64Bits + rint(u) for |u| <= 2^51), sh = 2^k with |v| sh <=
+// 2^min(51, 62 - eN), so 2^eN additions stay within 2^62; the words
+// accumulate those bits and the finish subtracts kMagic64Bits once per
+// accumulated pair (modulo 2^64). Resolution V' 2^-min(51, 62 - eN), V' the
+// power of two above V: far below the FP32 rounding of the values, and the
+// sums are order independent.
+constexpr double kMagic64 = 6755399441055744.0;
+constexpr unsigned long long kMagic64Bits = 0x4338000000000000ull;
+__device__ inline void fixed_scale64(float lsum, float xmax, int eN, float& sh, double& ish) {
+  const double V = 8.0 * (double)lsum * (double)xmax;
+  int k = 0;
+  if (V > 0.0 && V < 1e30) k = min(51, 62 - eN) - ((int)floor(log2(V)) + 1);
+  k = max(-100, min(k, 100));
+  sh = ldexpf(1.f, k);
+  ish = ldexp(1.0, -k);
+}
+__device__ inline unsigned long long fixed_bits64(float v) {
+  return (unsigned long long)__double_as_longlong((double)v + kMagic64);
+}
 
 __device__ inline float xor_sign(float v, unsigned m) { return __uint_as_float(__float_as_uint(v) ^ m); }
 // v + c * (value of lane l ^ 1)
@@ -120,7 +143,7 @@ __device__ inline float fma_xor1(float v, float c, float other) {
 template <int LMAX, int MMAX, int T>
 __global__ void __launch_bounds__(512) batch_slots_kernel(RadialKArgs a) {
   using Sh = SlotShape<LMAX, MMAX, T>;
-  constexpr int S = Sh::S, NS = Sh::NS, MT = Sh::MT, MH = Sh::MH, SC = Sh::SC, LO = Sh::LO, SY = Sh::SY;
+  constexpr int S = Sh::S, NS = Sh::NS, MT = Sh::MT, MH = Sh::MH, SC = Sh::SC, LO = Sh::LO;
   constexpr int KB = kBatchRhs;
   static_assert(KB * 2 == 16, "16 result columns = kBatch right-hand sides x (re, im)");
   static_assert(Sh::fits, "tap fragments exceed the register budget");
@@ -128,13 +151,15 @@ __global__ void __launch_bounds__(512) batch_slots_kernel(RadialKArgs a) {
   constexpr SlotOrder<LMAX, MMAX> so{};
   static_assert(so.ok, "slot order");
 
-  extern __shared__ __align__(16) int yacc[];
-  int* next_entry = yacc + (size_t)a.Rc * SY;
+  extern __shared__ __align__(16) unsigned long long slot_acc[];
+  unsigned long long* yacc = slot_acc;  // [Rc][LO] int64 words
+  int* next_entry = reinterpret_cast<int*>(yacc + (size_t)a.Rc * LO);
+  int* n_adds = next_entry + 1;  // pairs accumulated per receiver
   // work item of this block (launch order maps blocks to items, see RadialKArgs::order)
   const int item = a.order ? a.order[blockIdx.x] : (int)blockIdx.x;
-  float sh = 0.f, ish = 0.f, sl = 0.f, isl = 0.f;
-  fixed_scales(a.item_lsum[item], *a.xmax, a.item_en[item], sh, ish, sl, isl);
-  const float slo = sl * ish;  // low-word scale relative to the high word
+  float sh = 0.f;
+  double ish = 0.0;
+  fixed_scale64(a.item_lsum[item], *a.xmax, a.item_en[item], sh, ish);
 
   const int tid = threadIdx.x, lane = tid & 63;
   const int2 it = a.items[item];
@@ -142,9 +167,15 @@ __global__ void __launch_bounds__(512) batch_slots_kernel(RadialKArgs a) {
   const PairRec<float>* recs = reinterpret_cast<const PairRec<float>*>(a.recs);
   const float2* xg = reinterpret_cast<const float2*>(a.x);
   const float4* tt4 = reinterpret_cast<const float4*>(a.tt);
-  for (int j = tid; j < a.Rc * SY; j += blockDim.x) yacc[j] = 0;
+  for (int j = tid; j < a.Rc * LO; j += blockDim.x) yacc[j] = 0ull;
+  for (int j = tid; j < a.Rc; j += blockDim.x) n_adds[j] = 0;
   if (tid == 0) *next_entry = it.y;
   __syncthreads();
+  // every pair adds once to each word of its receiver: count them first
+  for (int e = it.y + (tid >> 6); e < e_end; e += blockDim.x >> 6) {
+    const int4 en = a.entries[e];
+    for (int p = lane; p < en.z; p += 64) atomicAdd(n_adds + (recs[en.y + p].ij >> kPairShift), 1);
+  }
 
   const int rr = lane & 15, q = lane >> 4;
   const int rhs = rr >> 1, zp = rr & 1;
@@ -250,7 +281,7 @@ __global__ void __launch_bounds__(512) batch_slots_kernel(RadialKArgs a) {
       }
 
       // y (rhs, re|im) of rows 16 i + 4 q + v, rotated by e^{-i m1 phi}, scaled by S_hi
-      int* yrow = yacc + (size_t)(rec.ij >> kPairShift) * SY + colbase;
+      unsigned long long* yrow = yacc + (size_t)(rec.ij >> kPairShift) * LO + colbase;
 #pragma unroll
       for (int i = 0; i < MH; ++i) {
         float er = 0.f, ei = 0.f;
@@ -273,10 +304,7 @@ __global__ void __launch_bounds__(512) batch_slots_kernel(RadialKArgs a) {
           const float own = fmaf(al, er, be * eps);
           const float snd = fmaf(be, er, -(al * eps));
           const float val = fma_xor1(own, sgnz, snd);
-          const float h = rintf(val);
-          const float l = rintf((val - h) * slo);
-          atomicAdd(yrow + 16 * i + v, (int)h);
-          atomicAdd(yrow + LO + 16 * i + v, (int)l);
+          atomicAdd(yrow + 16 * i + v, fixed_bits64(val));
         }
       }
     }
@@ -287,8 +315,9 @@ __global__ void __launch_bounds__(512) batch_slots_kernel(RadialKArgs a) {
   for (int j = tid; j < a.Rc * 2 * KB * S; j += blockDim.x) {
     const int il = j / (2 * KB * S), k = j - il * 2 * KB * S;
     const int c = k / S, sl2 = k - c * S;
-    const int w0 = il * SY + c * SC + sl2;
-    yp[j] = (float)((double)yacc[w0] * (double)ish + (double)yacc[w0 + LO] * (double)isl);
+    const int w0 = il * LO + c * SC + sl2;
+    const long long sum = (long long)(yacc[w0] - (unsigned long long)n_adds[il] * kMagic64Bits);  // modulo 2^64
+    yp[j] = (float)((double)sum * ish);
   }
 }
 

@@ -34,6 +34,10 @@ struct smh_op {
   float* d_tt32b = nullptr;
   int* d_b32_tab = nullptr;
   int b32_S = 0;
+  // FP64 kBatch-RHS kernel in the same slot order: FP64 polynomial
+  // coefficients and its input in the b32_index layout
+  double* d_tt64b = nullptr;
+  double2* d_x64s = nullptr;
   // FP16-operand kBatch-RHS kernel (slot order, b16_index input layout):
   // FP16 polynomial coefficients and one power-of-two scale per bin
   _Float16* d_tth = nullptr;

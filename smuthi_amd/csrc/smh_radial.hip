@@ -6,6 +6,7 @@
 #include "smh_batch_kernel.h"
 #include "smh_batch_half.h"
 #include "smh_batch_slots.h"
+#include "smh_batch_slots64.h"
 #include "smh_lookup.h"
 
 using namespace smh;
@@ -183,6 +184,30 @@ __global__ void poly_slot_kernel(const double2* __restrict__ table, const double
   }
 }
 
+// The same coefficients in FP64 for the FP64 slot-order kernel, layout
+// [rows][MT][NS/2][64 lanes][2]: lane (q, rr) of tile i holds
+// c_k(slot 16 (i mod MH) + rr, slot 4 (2 s2 + e) + q).
+__global__ void poly_slot64_kernel(const double2* __restrict__ table, const double2* __restrict__ mh, int64_t len,
+                                   int B, int S, const int* __restrict__ n_of, int interp, int T, double* tt) {
+  const int NS = S / 4, MH = S / 16, FRB = 2 * MH * NS * 64;
+  const int64_t total = len * T * FRB;
+  for (int64_t f = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; f < total; f += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t row = f / FRB;
+    int rem = (int)(f - row * FRB);
+    const int e = rem & 1;
+    const int ln = (rem >> 1) & 63;
+    rem >>= 7;
+    const int s2 = rem % (NS / 2);
+    const int i = rem / (NS / 2);
+    const int q = ln >> 4, rr = ln & 15;
+    const int n1 = n_of[16 * (i % MH) + rr];
+    const int n2 = n_of[4 * (2 * s2 + e) + q];
+    const int64_t r = row / T;
+    const int k = (int)(row - r * T);
+    tt[f] = (n1 >= 0 && n2 >= 0) ? poly_coef(table, mh, len, B * B, (int64_t)n1 * B + n2, i < MH, interp, r, k) : 0.0;
+  }
+}
+
 // FP16 coefficients of the FP16-input batched kernel, one block per bin r:
 // layout [bins][T][MT][64 lanes][NC][4], lane (q, rr) of tile i holding
 // c_t(slot 16 (i mod MH) + rr, slot 16 c + 4 q + j); the bin's values are
@@ -305,15 +330,22 @@ KernelInfo pick(int l, int m) {
   return KernelInfo{};
 }
 
-// FP32 operators whose tap fragments fit one register set use the slot-order
-// kernel; the others (and FP64) the plain kBatch-RHS kernel.
+// Operators whose coefficient fragments fit one register set use the
+// slot-order kernels (FP32 and FP64); the others the plain kBatch-RHS kernel.
 template <int L, int Mx, int T, typename Real>
 KernelInfo batch_kernel_info() {
   KernelInfo k;
   if constexpr (std::is_same<Real, float>::value && SlotShape<L, Mx, T>::fits && Mx <= 3) {
     k.fn = reinterpret_cast<void*>(&batch_slots_kernel<L, Mx, T>);
     k.threads = 512;
-    k.sy = SlotShape<L, Mx, T>::SY;
+    k.sy = SlotShape<L, Mx, T>::SY;  // int32 words (one int64 word per result)
+    k.slots = SlotShape<L, Mx, T>::S;
+    k.slot_tables = &slot_tables<L, Mx>;
+  } else if constexpr (std::is_same<Real, double>::value && SlotShape<L, Mx, T>::fits && Mx <= 3 &&
+                       SlotShape<L, Mx, T>::NS % 2 == 0) {
+    k.fn = reinterpret_cast<void*>(&batch_slots64_kernel<L, Mx, T>);
+    k.threads = kSlots64Threads;
+    k.sy = SlotShape<L, Mx, T>::LO;  // doubles
     k.slots = SlotShape<L, Mx, T>::S;
     k.slot_tables = &slot_tables<L, Mx>;
   } else {
@@ -455,6 +487,11 @@ int op_make_taps(smh_op* op) {
     if (!op->d_tt32b) SMH_HIP(ctx, hipMalloc(&op->d_tt32b, (size_t)len * op->T * frb * sizeof(float)));
     hipLaunchKernelGGL(poly_slot_kernel, dim3(4096), dim3(256), 0, s, op->d_table, d_mh, len, op->B, op->b32_S,
                        op->d_b32_tab + op->B, op->interp, op->T, op->d_tt32b);
+    if (radial_batch_kernel(SMH_FP64, op->T, op->l_max, op->m_max).slots == op->b32_S) {
+      if (!op->d_tt64b) SMH_HIP(ctx, hipMalloc(&op->d_tt64b, (size_t)len * op->T * frb * sizeof(double)));
+      hipLaunchKernelGGL(poly_slot64_kernel, dim3(4096), dim3(256), 0, s, op->d_table, d_mh, len, op->B, op->b32_S,
+                         op->d_b32_tab + op->B, op->interp, op->T, op->d_tt64b);
+    }
     if (radial_half_kernel(op->T, op->l_max, op->m_max)) {
       // T * MT * 64 * NC * 4 halves, MT = 2 S / 16, NC = S / 16: 2 S^2 per tap
       const size_t per_bin = (size_t)op->T * 2 * op->b32_S * op->b32_S;
@@ -556,6 +593,17 @@ __global__ void scatter_batch_kernel(const double2* __restrict__ pk, const int* 
   const int ip = pk2pad[u];
   const int64_t p = ip / B;
   pad[b32_index(p, (int)(ip - p * B), (int)r, S, slot)] = typename Vec2<Real>::t{Real(v.x), Real(v.y)};
+}
+
+// FP64 padded [n_part][B][kBatch] -> FP64 input of the FP64 slot-order kernel
+// (b32_index layout)
+__global__ void to_slots64_kernel(const double2* __restrict__ v, int64_t n8, int B, int S,
+                                  const int* __restrict__ slot, double2* out) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n8) return;
+  const int r = (int)(e % kBatch);
+  const int64_t ip = e / kBatch, p = ip / B;
+  out[b32_index(p, (int)(ip - p * B), r, S, slot)] = v[e];
 }
 
 // FP64 padded [n_part][B][kBatch] -> FP16 input of the FP16-operand kernel
@@ -689,12 +737,27 @@ int op_apply_batch(smh_op* op, int precision, const void* x_pad, void* y_pad, bo
   a.entries = bn.d_entries;
   a.items = bn.d_items;
   a.order = bn.d_order;
-  const bool slots = pi == 0 && k.slots > 0;
-  SMH_CHECK(ctx, !slots || (op->d_tt32b && op->b32_S == k.slots), SMH_ERR_STATE, "slot-order tap table missing");
+  const bool slots = k.slots > 0;
+  SMH_CHECK(ctx, !slots || (op->d_b32_tab && op->b32_S == k.slots && (pi == 0 ? (void*)op->d_tt32b : (void*)op->d_tt64b)),
+            SMH_ERR_STATE, "slot-order tap table missing");
   SMH_CHECK(ctx, !half || (fn && slots), SMH_ERR_STATE, "FP16 tap table missing");
   a.tt = half ? (const void*)op->d_tth
-              : (pi == 0 ? (slots ? (const void*)op->d_tt32b : (const void*)op->d_tt32) : (const void*)op->d_tt64);
+              : (pi == 0 ? (slots ? (const void*)op->d_tt32b : (const void*)op->d_tt32)
+                         : (slots ? (const void*)op->d_tt64b : (const void*)op->d_tt64));
   a.x = x_pad;
+  hipStream_t s = ctx->stream;
+  if (pi == 1 && slots) {
+    // FP64 padded [n_part][B][kBatch] -> the slot layout (padding slots stay zero)
+    const int64_t n8 = op->n_part * op->B * kBatch;
+    if (!op->d_x64s) {
+      const size_t bytes = (size_t)op->n_part * k.slots * kBatch * sizeof(double2);
+      SMH_HIP(ctx, hipMalloc(&op->d_x64s, bytes));
+      SMH_HIP(ctx, hipMemsetAsync(op->d_x64s, 0, bytes, s));
+    }
+    hipLaunchKernelGGL(to_slots64_kernel, dim3(blocks_for(n8)), dim3(256), 0, s, (const double2*)x_pad, n8, op->B,
+                       k.slots, (const int*)op->d_b32_tab, op->d_x64s);
+    a.x = op->d_x64s;
+  }
   a.ypart = op->d_ypart;
   a.xmax = op->d_xmax;
   a.item_lsum = bn.d_item_lsum;
@@ -706,7 +769,6 @@ int op_apply_batch(smh_op* op, int precision, const void* x_pad, void* y_pad, bo
   a.len_z = 0;
   a.row_s = op->s_mode;
   a.row_o = op->o_mode;
-  hipStream_t s = ctx->stream;
   SMH_HIP(ctx, hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bn.lds));
   const int64_t nx = op->n_part * op->B * kBatch;
   const int S = slots ? k.slots : op->B;
@@ -729,8 +791,8 @@ int op_apply_batch(smh_op* op, int precision, const void* x_pad, void* y_pad, bo
                        bn.d_chunk_items, bn.Rc, op->B, S, slot, op->n_part, (float2*)y_pad);
   else
     hipLaunchKernelGGL(reduce_batch_kernel<double>, dim3(blocks_for(nx)), dim3(256), 0, s,
-                       (const double*)op->d_ypart, bn.d_chunk_items, bn.Rc, op->B, op->B, (const int*)nullptr,
-                       op->n_part, (double2*)y_pad);
+                       (const double*)op->d_ypart, bn.d_chunk_items, bn.Rc, op->B, S, slot, op->n_part,
+                       (double2*)y_pad);
   SMH_HIP(ctx, hipGetLastError());
   if (time_kernel && bn.n_items > 0) {
     SMH_HIP(ctx, hipEventSynchronize(ctx->ev1));
@@ -772,7 +834,7 @@ int op_index_map(smh_op* op, const int* lmax, const int* mmax) {
 
 void smh_op_free(smh_op* op) {
   void* ptrs[] = {op->d_xy, op->d_xyz, op->d_pk2pad, op->d_table, op->d_table_minus, op->d_tt64, op->d_tt32,
-                  op->d_tt32b, op->d_b32_tab, op->d_tth, op->d_hscale, op->d_W64, op->d_W32, op->d_ypart, op->d_pk_in, op->d_pk_out, op->d_x32, op->d_y32, op->d_x64, op->d_y64,
+                  op->d_tt32b, op->d_tt64b, op->d_x64s, op->d_b32_tab, op->d_tth, op->d_hscale, op->d_W64, op->d_W32, op->d_ypart, op->d_pk_in, op->d_pk_out, op->d_x32, op->d_y32, op->d_x64, op->d_y64,
                   op->d_rowsum, op->d_xmax, op->d_xb, op->d_yb};
   for (void* p : ptrs)
     if (p) hipFree(p);

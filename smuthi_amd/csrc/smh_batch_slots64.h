@@ -17,7 +17,10 @@
 //     the sign of m1 in mixed groups depends on q);
 //   * the re/im columns of a right-hand side are combined with one DPP
 //     exchange (lane ^ 1) and each value is added to the receiver's LDS
-//     accumulator with one FP64 LDS atomic.
+//     accumulator with one FP64 LDS atomic;
+//   * the K-steps run in two parts, each a pass over the entry's pairs with
+//     its own epilogue, so one part's coefficients (128 registers) leave room
+//     for two waves per SIMD.
 //
 // Layouts (S = slots per particle, NS = S / 4 K-steps):
 //   coefficients [bins][T][MT][NS/2][64 lanes][2] FP64: lane (q, rr) of tile i
@@ -44,7 +47,9 @@ __device__ inline double lane_xor1_64(double v) {
   return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
 }
 
-constexpr int kSlots64Threads = 256;
+// two parts of the K-steps per pair, two waves per SIMD
+constexpr int kSlots64Parts = 2;
+constexpr int kSlots64Threads = 512;
 
 template <int LMAX, int MMAX, int T>
 __global__ void __launch_bounds__(kSlots64Threads) batch_slots64_kernel(RadialKArgs a) {
@@ -53,7 +58,8 @@ __global__ void __launch_bounds__(kSlots64Threads) batch_slots64_kernel(RadialKA
   constexpr int KB = kBatchRhs;
   static_assert(KB * 2 == 16, "16 result columns = kBatch right-hand sides x (re, im)");
   static_assert(Sh::fits, "coefficients exceed the register budget");
-  static_assert(NS % 2 == 0, "coefficient pairs per lane");
+  constexpr int NSH = NS / kSlots64Parts;  // K-steps per part
+  static_assert(NS % (2 * kSlots64Parts) == 0, "coefficient pairs per lane and part");
   static_assert(MMAX <= 3, "phase selection covers |m| <= 3");
   constexpr SlotOrder<LMAX, MMAX> so{};
   static_assert(so.ok, "slot order");
@@ -92,90 +98,97 @@ __global__ void __launch_bounds__(kSlots64Threads) batch_slots64_kernel(RadialKA
     if (e >= e_end) break;
     const int4 en = a.entries[e];
     const int bin = (en.x - a.row_o) / a.row_s;
-    double tv[T][MT][NS];
-#pragma unroll
-    for (int t = 0; t < T; ++t)
-#pragma unroll
-      for (int i = 0; i < MT; ++i) {
-        const double2* row = tt2 + (((size_t)bin * T + t) * MT + i) * (NS / 2) * 64;
-#pragma unroll
-        for (int s2 = 0; s2 < NS / 2; ++s2) {
-          const double2 v = row[s2 * 64 + lane];
-          tv[t][i][2 * s2] = v.x;
-          tv[t][i][2 * s2 + 1] = v.y;
-        }
-      }
     const int p_end = en.z;
-    auto load_x = [&](const PairRec<double>& r, double (&xs)[2 * NS]) {
-      const double* xp = xg + (int64_t)(r.ij & kPairMask) * S * KB * 2 + xlane;
+    // the K-steps in kSlots64Parts parts of NSH, each with its own pass over
+    // the entry's pairs and its own epilogue (coefficients of one part in
+    // registers: two waves per SIMD)
 #pragma unroll
-      for (int s = 0; s < NS; ++s) {
-        xs[2 * s] = xp[2 * s];
-        xs[2 * s + 1] = xp[2 * s + xother];
-      }
-    };
-    PairRec<double> rec_n = recs[en.y];
-    double xc[2 * NS];
-    load_x(rec_n, xc);
-    for (int p = 0; p < p_end; ++p) {
-      const PairRec<double> rec = rec_n;
-      rec_n = recs[en.y + (p + 1 < p_end ? p + 1 : p)];
-
-      const double w = rec.w;
-      const double c1 = rec.c, s1 = rec.s;
-      const double c2 = c1 * c1 - s1 * s1, s2 = 2.0 * c1 * s1;
-      const double c3 = c2 * c1 - s2 * s1, s3 = s2 * c1 + c2 * s1;
-      const double cs[4] = {1.0, c1, c2, c3};
-      const double sn[4] = {0.0, s1, s2, s3};
-
-      // B operand per K-step: (e^{i m2 phi} x[slot 4 s + q][rhs]).re or .im
-      double z[NS];
+    for (int h = 0; h < kSlots64Parts; ++h) {
+      double tv[T][MT][NSH];
 #pragma unroll
-      for (int s = 0; s < NS; ++s) {
-        const int am = so.absm[s];
-        const double av = xc[2 * s], bv = xc[2 * s + 1];
-        if (am == 0) {
-          z[s] = av;
-        } else {
-          const int sg = so.sgn[s];
-          const double ez = xor_sign64(sn[am], zmask);
-          const double e = sg > 0 ? ez : (sg < 0 ? -ez : xor_sign64(ez, qmask));
-          z[s] = fma(cs[am], av, e * bv);
-        }
-      }
-      load_x(rec_n, xc);
-
-      V4 acc[MT];
-#pragma unroll
-      for (int i = 0; i < MT; ++i) acc[i] = V4{0, 0, 0, 0};
-#pragma unroll
-      for (int s = 0; s < NS; ++s) {
+      for (int t = 0; t < T; ++t)
 #pragma unroll
         for (int i = 0; i < MT; ++i) {
-          double v = tv[T - 1][i][s];
+          const double2* row = tt2 + (((size_t)bin * T + t) * MT + i) * (NS / 2) * 64;
 #pragma unroll
-          for (int t = T - 2; t >= 0; --t) v = fma(v, w, tv[t][i][s]);
-          acc[i] = Mfma<double>::op(v, z[s], acc[i]);
+          for (int s2 = 0; s2 < NSH / 2; ++s2) {
+            const double2 v = row[(h * (NSH / 2) + s2) * 64 + lane];
+            tv[t][i][2 * s2] = v.x;
+            tv[t][i][2 * s2 + 1] = v.y;
+          }
         }
-      }
+      auto load_x = [&](const PairRec<double>& r, double (&xs)[2 * NSH]) {
+        const double* xp = xg + (int64_t)(r.ij & kPairMask) * S * KB * 2 + xlane + 2 * h * NSH;
+#pragma unroll
+        for (int s = 0; s < NSH; ++s) {
+          xs[2 * s] = xp[2 * s];
+          xs[2 * s + 1] = xp[2 * s + xother];
+        }
+      };
+      PairRec<double> rec_n = recs[en.y];
+      double xc[2 * NSH];
+      load_x(rec_n, xc);
+      for (int p = 0; p < p_end; ++p) {
+        const PairRec<double> rec = rec_n;
+        rec_n = recs[en.y + (p + 1 < p_end ? p + 1 : p)];
 
-      // register v of tile i: slot 16 i + 4 v + q (group 4 i + v), rotated by e^{-i m1 phi}
-      double* yrow = yacc + (size_t)(rec.ij >> kPairShift) * LO + colbase;
+        const double w = rec.w;
+        const double c1 = rec.c, s1 = rec.s;
+        const double c2 = c1 * c1 - s1 * s1, s2 = 2.0 * c1 * s1;
+        const double c3 = c2 * c1 - s2 * s1, s3 = s2 * c1 + c2 * s1;
+        const double cs[4] = {1.0, c1, c2, c3};
+        const double sn[4] = {0.0, s1, s2, s3};
+
+        // B operand per K-step: (e^{i m2 phi} x[slot 4 s + q][rhs]).re or .im
+        double z[NSH];
 #pragma unroll
-      for (int i = 0; i < MH; ++i) {
+        for (int sl = 0; sl < NSH; ++sl) {
+          const int s = h * NSH + sl;
+          const int am = so.absm[s];
+          const double av = xc[2 * sl], bv = xc[2 * sl + 1];
+          if (am == 0) {
+            z[sl] = av;
+          } else {
+            const int sg = so.sgn[s];
+            const double ez = xor_sign64(sn[am], zmask);
+            const double ee = sg > 0 ? ez : (sg < 0 ? -ez : xor_sign64(ez, qmask));
+            z[sl] = fma(cs[am], av, ee * bv);
+          }
+        }
+        load_x(rec_n, xc);
+
+        V4 acc[MT];
 #pragma unroll
-        for (int v = 0; v < 4; ++v) {
-          const int g = 4 * i + v;
-          const int am = so.absm[g], sg = so.sgn[g];
-          const double er = cs[am];
-          const double eps = am == 0 ? 0.0 : (sg > 0 ? sn[am] : (sg < 0 ? -sn[am] : xor_sign64(sn[am], qmask)));
-          const double al = acc[i][v], be = acc[i + MH][v];
-          const double own = fma(al, er, be * eps);
-          const double snd = fma(be, er, -(al * eps));
-          const double val = fma(sgnz, lane_xor1_64(snd), own);
-          if (so.n_of[16 * i + 4 * v] >= 0 || so.n_of[16 * i + 4 * v + 1] >= 0 || so.n_of[16 * i + 4 * v + 2] >= 0 ||
-              so.n_of[16 * i + 4 * v + 3] >= 0)
-            atomicAdd(yrow + 16 * i + 4 * v, val);
+        for (int i = 0; i < MT; ++i) acc[i] = V4{0, 0, 0, 0};
+#pragma unroll
+        for (int sl = 0; sl < NSH; ++sl) {
+#pragma unroll
+          for (int i = 0; i < MT; ++i) {
+            double v = tv[T - 1][i][sl];
+#pragma unroll
+            for (int t = T - 2; t >= 0; --t) v = fma(v, w, tv[t][i][sl]);
+            acc[i] = Mfma<double>::op(v, z[sl], acc[i]);
+          }
+        }
+
+        // register v of tile i: slot 16 i + 4 v + q (group 4 i + v), rotated by e^{-i m1 phi}
+        double* yrow = yacc + (size_t)(rec.ij >> kPairShift) * LO + colbase;
+#pragma unroll
+        for (int i = 0; i < MH; ++i) {
+#pragma unroll
+          for (int v = 0; v < 4; ++v) {
+            const int g = 4 * i + v;
+            const int am = so.absm[g], sg = so.sgn[g];
+            const double er = cs[am];
+            const double eps = am == 0 ? 0.0 : (sg > 0 ? sn[am] : (sg < 0 ? -sn[am] : xor_sign64(sn[am], qmask)));
+            const double al = acc[i][v], be = acc[i + MH][v];
+            const double own = fma(al, er, be * eps);
+            const double snd = fma(be, er, -(al * eps));
+            const double val = fma(sgnz, lane_xor1_64(snd), own);
+            if (so.n_of[16 * i + 4 * v] >= 0 || so.n_of[16 * i + 4 * v + 1] >= 0 ||
+                so.n_of[16 * i + 4 * v + 2] >= 0 || so.n_of[16 * i + 4 * v + 3] >= 0)
+              atomicAdd(yrow + 16 * i + 4 * v, val);
+          }
         }
       }
     }

@@ -342,7 +342,7 @@ KernelInfo batch_kernel_info() {
     k.slots = SlotShape<L, Mx, T>::S;
     k.slot_tables = &slot_tables<L, Mx>;
   } else if constexpr (std::is_same<Real, double>::value && SlotShape<L, Mx, T>::fits && Mx <= 3 &&
-                       SlotShape<L, Mx, T>::NS % 2 == 0) {
+                       SlotShape<L, Mx, T>::NS % (2 * kSlots64Parts) == 0) {
     k.fn = reinterpret_cast<void*>(&batch_slots64_kernel<L, Mx, T>);
     k.threads = kSlots64Threads;
     k.sy = SlotShape<L, Mx, T>::LO;  // doubles

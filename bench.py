@@ -87,9 +87,9 @@ def measured_traffic(kind='single'):
 
 def traffic_probe(n_particles, copy_bytes=512 << 20):
     """Run a plain copy of known size, then three single-RHS FP32, three 8-RHS
-    FP32 and three 8-RHS FP16 matvec launches of the bench operator (one
-    warm-up + two timed each): the workload the FETCH_SIZE / WRITE_SIZE
-    counter passes are taken on."""
+    FP32, three 8-RHS FP16 and three 8-RHS FP64 matvec launches of the bench
+    operator (one warm-up + two timed each): the workload the FETCH_SIZE /
+    WRITE_SIZE counter passes are taken on."""
     import ctypes
     from smuthi_amd import _lib, cuda
     from smuthi_amd.linear_system import CouplingMatrixRadialLookupCUDA
@@ -103,8 +103,9 @@ def traffic_probe(n_particles, copy_bytes=512 << 20):
     k, _ = op.bench_matvec('single', iters=2)
     kb, _ = op.bench_matvec_batch('single', iters=2)
     kh, _ = op.bench_matvec_batch('half', iters=2)
-    log('copy %d bytes: %.3f ms; FP32 matvec kernel %.2f ms, 8-RHS FP32 kernel %.2f ms, 8-RHS FP16 kernel %.2f ms'
-        % (copy_bytes, ms.value, k, kb, kh))
+    kd, _ = op.bench_matvec_batch('double', iters=2)
+    log('copy %d bytes: %.3f ms; FP32 matvec kernel %.2f ms, 8-RHS FP32 kernel %.2f ms, 8-RHS FP16 kernel %.2f ms, '
+        '8-RHS FP64 kernel %.2f ms' % (copy_bytes, ms.value, k, kb, kh, kd))
 
 
 # flops per (point, particle) pair of the SWE direct sum at l_max = m_max = 3: 38 per (l, m) for both

@@ -165,6 +165,15 @@ int smh_op_bench_matvec(smh_op *op, int precision, int iters, double *kernel_ms,
  * counterpart in smuthi, which has no multi-RHS path. */
 int smh_op_bench_matvec_batch(smh_op *op, int precision, int iters, double *kernel_ms, double *total_ms);
 
+/* Interpolation of the FP16 operator used by the mixed16 inner solves (and
+ * smh_op_matvec with SMH_FP16): *terms = 2 when it interpolates linearly
+ * between the table nodes (chosen when that error, measured on the table,
+ * stays within 2^-10 of each used bin's scale), else the operator's own
+ * number of Horner terms (0 before a table is set); *lin_ratio = the
+ * measured largest error / scale (-1 when not measured). No counterpart in
+ * smuthi, which has no reduced-precision operator. */
+int smh_op_half_interp(smh_op *op, int *terms, double *lin_ratio);
+
 /* Solve (1 - T W) b = T a (LinearSystem.solve with solver_type='gmres',
  * linear_system.py:254-344) for nrhs right-hand sides.
  * t_diag: complex128 [n_unk] diagonal sphere T-matrix; rhs, x: complex128

@@ -56,6 +56,7 @@ SIGNATURES = {
     'smh_op_matvec': (_c.c_int, [_p, _dp, _dp, _i64, _c.c_int]),
     'smh_op_bench_matvec': (_c.c_int, [_p, _c.c_int, _c.c_int, _dp, _dp]),
     'smh_op_bench_matvec_batch': (_c.c_int, [_p, _c.c_int, _c.c_int, _dp, _dp]),
+    'smh_op_half_interp': (_c.c_int, [_p, _ip, _dp]),
     'smh_solve': (_c.c_int, [_p, _dp, _dp, _dp, _i64, _c.c_double, _c.c_int, _c.c_int, _c.c_int, _dp]),
     'smh_pwe_field': (_c.c_int, [_p, _c.c_int, _i64, _dp, _dp, _i64, _dp, _dp, _c.c_double, _c.c_double, _i64, _dp,
                                  _dp, _c.c_int, _dp]),

@@ -42,6 +42,8 @@ struct smh_op {
   // FP16 polynomial coefficients and one power-of-two scale per bin
   _Float16* d_tth = nullptr;
   float* d_hscale = nullptr;
+  int T_half = 0;         // its Horner terms: 2 (linear between nodes) or T
+  float lin_ratio = -1;   // largest linear-interpolation error / bin scale over the used bins
   float* d_rowsum = nullptr;         // per tap row: max_n1 sum_n2 |Re| + |Im|
   float* d_xmax = nullptr;           // largest |Re| + |Im| of the current FP32 input
 

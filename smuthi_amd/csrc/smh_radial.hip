@@ -252,6 +252,55 @@ __global__ void poly_half_kernel(const double2* __restrict__ table, const double
   for (int f = threadIdx.x; f < per_bin; f += blockDim.x) th[(size_t)r * per_bin + f] = (_Float16)(value(f) * inv);
 }
 
+// Linear interpolation error of each bin relative to its scale: bin r
+// (between nodes r and r + 1) gets max_n max(|d2_r|, |d2_{r+1}|) / 8 over
+// max_n max(|y_r|, |y_{r+1}|), d2_j = y_{j-1} - 2 y_j + y_{j+1} (h^2 y'' at
+// node j) of the Re and Im parts; one block per bin.
+__global__ void lin_ratio_kernel(const double2* __restrict__ table, int64_t len, int BB, float* ratio) {
+  const int64_t r = blockIdx.x;
+  __shared__ double rd[256], ry[256];
+  auto d2 = [&](int64_t j, int n) {
+    if (j < 1 || j > len - 2) return 0.0;
+    const double2 a = table[(j - 1) * BB + n], b = table[j * BB + n], c = table[(j + 1) * BB + n];
+    return fmax(fabs(a.x - 2.0 * b.x + c.x), fabs(a.y - 2.0 * b.y + c.y));
+  };
+  double md = 0.0, my = 0.0;
+  for (int n = threadIdx.x; n < BB; n += blockDim.x) {
+    md = fmax(md, fmax(d2(r, n), d2(r + 1, n)));
+    const double2 y0 = table[r * BB + n], y1 = table[(r + 1 < len ? r + 1 : r) * BB + n];
+    my = fmax(my, fmax(fmax(fabs(y0.x), fabs(y0.y)), fmax(fabs(y1.x), fabs(y1.y))));
+  }
+  rd[threadIdx.x] = md;
+  ry[threadIdx.x] = my;
+  __syncthreads();
+  for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) {
+      rd[threadIdx.x] = fmax(rd[threadIdx.x], rd[threadIdx.x + w]);
+      ry[threadIdx.x] = fmax(ry[threadIdx.x], ry[threadIdx.x + w]);
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) ratio[r] = ry[0] > 0.0 ? (float)(rd[0] / (8.0 * ry[0])) : 0.f;
+}
+
+// Largest ratio[] over the bins the entries use (one block).
+__global__ void used_max_kernel(const int4* __restrict__ entries, int n_entries, int row_o, int row_s,
+                                const float* __restrict__ ratio, int64_t len, float* out) {
+  __shared__ float red[1024];
+  float m = 0.f;
+  for (int e = threadIdx.x; e < n_entries; e += blockDim.x) {
+    const int64_t bin = (entries[e].x - row_o) / row_s;
+    if (bin >= 0 && bin < len) m = fmaxf(m, ratio[bin]);
+  }
+  red[threadIdx.x] = m;
+  __syncthreads();
+  for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) red[threadIdx.x] = fmaxf(red[threadIdx.x], red[threadIdx.x + w]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *out = red[0];
+}
+
 // Not-a-knot cubic spline moments, scaled: Mh_j = h^2/6 S''(rho_j), one
 // thread per complex column (n1, n2). With uniform spacing the system is
 //   Mh_{j-1} + 4 Mh_j + Mh_{j+1} = y_{j-1} - 2 y_j + y_{j+1},  j = 1..n-2,
@@ -492,15 +541,40 @@ int op_make_taps(smh_op* op) {
       hipLaunchKernelGGL(poly_slot64_kernel, dim3(4096), dim3(256), 0, s, op->d_table, d_mh, len, op->B, op->b32_S,
                          op->d_b32_tab + op->B, op->interp, op->T, op->d_tt64b);
     }
-    if (radial_half_kernel(op->T, op->l_max, op->m_max)) {
+    // The FP16 operator (inner solves only) interpolates linearly between
+    // the table nodes when that error stays within two FP16 roundings (2^-10
+    // of each bin's scale) on every bin a pair uses: half the Horner work and
+    // coefficient traffic, at an operator error the inner solves' 1e-3
+    // target does not see (the FP32 / FP64 residuals of the refinement use
+    // the operator's own interpolation). Otherwise it keeps that one.
+    op->T_half = op->T;
+    const smh_op::Binning& hb = op->bbin[0];
+    if (op->T > 2 && hb.n_entries > 0 && radial_half_kernel(2, op->l_max, op->m_max)) {
+      float* d_ratio = nullptr;
+      SMH_HIP(ctx, hipMalloc(&d_ratio, (size_t)(len + 1) * sizeof(float)));
+      hipLaunchKernelGGL(lin_ratio_kernel, dim3((unsigned)len), dim3(256), 0, s, op->d_table, len, op->B * op->B,
+                         d_ratio);
+      hipLaunchKernelGGL(used_max_kernel, dim3(1), dim3(1024), 0, s, hb.d_entries, hb.n_entries, op->o_mode,
+                         op->s_mode, d_ratio, len, d_ratio + len);
+      float worst = 1.f;
+      const hipError_t e1 = hipMemcpyAsync(&worst, d_ratio + len, sizeof(float), hipMemcpyDeviceToHost, s);
+      const hipError_t e2 = hipStreamSynchronize(s);
+      hipFree(d_ratio);
+      SMH_CHECK(ctx, e1 == hipSuccess && e2 == hipSuccess, SMH_ERR_HIP, "interpolation check failed");
+      op->lin_ratio = worst;
+      if (worst <= 0x1p-10f) op->T_half = 2;
+    }
+    if (radial_half_kernel(op->T_half, op->l_max, op->m_max)) {
       // T * MT * 64 * NC * 4 halves, MT = 2 S / 16, NC = S / 16: 2 S^2 per tap
-      const size_t per_bin = (size_t)op->T * 2 * op->b32_S * op->b32_S;
+      // (allocated for the operator's T, the larger of the two)
+      const size_t per_bin = (size_t)std::max(op->T, 2) * 2 * op->b32_S * op->b32_S;
       if (!op->d_tth) {
         SMH_HIP(ctx, hipMalloc(&op->d_tth, (size_t)len * per_bin * sizeof(_Float16)));
         SMH_HIP(ctx, hipMalloc(&op->d_hscale, (size_t)len * sizeof(float)));
       }
       hipLaunchKernelGGL(poly_half_kernel, dim3((unsigned)len), dim3(256), 0, s, op->d_table, d_mh, len, op->B,
-                         op->b32_S, op->d_b32_tab + op->B, op->interp, op->T, op->d_tth, op->d_hscale);
+                         op->b32_S, op->d_b32_tab + op->B, op->T_half == 2 ? (int)SMH_INTERP_LINEAR : op->interp,
+                         op->T_half, op->d_tth, op->d_hscale);
     }
   }
   if (!op->d_rowsum) {
@@ -731,7 +805,7 @@ int op_apply_batch(smh_op* op, int precision, const void* x_pad, void* y_pad, bo
   SMH_CHECK(ctx, op_has_batch(op, precision), SMH_ERR_UNSUPPORTED, "no batched matvec for this operator");
   const smh_op::Binning& bn = op->bbin[pi];
   const KernelInfo k = radial_batch_kernel(pi == 0 ? SMH_FP32 : SMH_FP64, op->T, op->l_max, op->m_max);
-  void* fn = half ? radial_half_kernel(op->T, op->l_max, op->m_max) : k.fn;
+  void* fn = half ? radial_half_kernel(op->T_half, op->l_max, op->m_max) : k.fn;
   RadialKArgs a;
   a.recs = bn.d_rec;
   a.entries = bn.d_entries;
@@ -1151,6 +1225,13 @@ __global__ void fill_pattern_kernel(float2* x32, double2* x64, int64_t n) {
   }
 }
 }  // namespace
+
+extern "C" int smh_op_half_interp(smh_op* op, int* terms, double* lin_ratio) {
+  if (!op) return SMH_ERR_ARG;
+  if (terms) *terms = op->T_half;
+  if (lin_ratio) *lin_ratio = (double)op->lin_ratio;
+  return SMH_OK;
+}
 
 extern "C" int smh_op_bench_matvec_batch(smh_op* op, int precision, int iters, double* kernel_ms,
                                          double* total_ms) {

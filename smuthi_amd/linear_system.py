@@ -105,6 +105,15 @@ class _GpuCouplingOperator(SystemMatrix):
                    cu.context(), 'smh_op_bench_matvec_batch')
         return k.value, t.value
 
+    def half_interpolation(self):
+        """(Horner terms, measured linear-interpolation error / bin scale) of the FP16 operator of the
+        mixed16 inner solves: 2 terms = linear between the table nodes."""
+        terms = ctypes.c_int()
+        ratio = ctypes.c_double()
+        _lib.check(_lib.load().smh_op_half_interp(self._op, ctypes.byref(terms), ctypes.byref(ratio)), cu.context(),
+                   'smh_op_half_interp')
+        return terms.value, ratio.value
+
     def solve_master(self, t_diagonal, rhs, tol, precision='mixed', restart=20, maxiter=None, with_stats=False):
         """Solve (1 - diag(t) W) x = rhs on the GPU (rhs [n] or [nrhs, n]).
 

@@ -157,3 +157,37 @@ def test_single_rhs_mixed16_solve(gpu):
     x, stats, ok = op.solve_master(t, r, tol=1e-6, precision='mixed16', with_stats=True)
     assert ok and stats['relative_residual'] <= 1e-6 and stats['matvecs_fp16'] > 0
     assert np.linalg.norm(x - np.linalg.solve(M, r)) / np.linalg.norm(x) < 1e-5
+
+
+def test_mixed16_linear_fp16_operator(gpu):
+    """At a 2 nm table the FP16 inner operator interpolates linearly between the nodes (the error, measured on
+    the table, stays within 2^-10 of every used bin's scale): its matvec stays within the FP16 tolerance of the
+    cubic oracle, and mixed16 solves still certify against the cubic FP64 operator."""
+    from smuthi_amd import ensembles
+    from smuthi_amd.layers import LayerSystem
+    from smuthi_amd.linear_system import CouplingMatrixRadialLookupCUDA
+    pp = ensembles.small_substrate_ensemble(seed=2)[:24]
+    plist = [osys.Sphere(p.position, p.refractive_index, p.radius, 3, 3) for p in pp]
+    kp = ob.reasonable_Sommerfeld_kpar_contour(WL, layer_refractive_indices=[1.52, 1], neff_resolution=1e-2)
+    op = CouplingMatrixRadialLookupCUDA(WL, plist, LayerSystem([0, 0], [1.52, 1]), k_parallel=kp, resolution=2.0,
+                                        interpolator_kind='cubic')
+    terms, ratio = op.half_interpolation()
+    print('FP16 operator: %d terms, linear error / bin scale %.3e' % (terms, ratio), flush=True)
+    assert terms == 2 and 0.0 <= ratio <= 2.0 ** -10
+    ref = oc.RadialLookupCPU(op.lookup_table, op.radial_distance_array, plist, 'cubic')
+    rng = np.random.default_rng(8)
+    n = ref.n_unk
+    V = rng.standard_normal((9, n)) + 1j * rng.standard_normal((9, n))
+    W16 = op.matvec(V, precision='half')
+    for r in range(9):
+        want = ref.matvec(V[r])
+        assert np.linalg.norm(W16[r] - want) / np.linalg.norm(want) < 2e-3, r
+    W = ref.dense()
+    t = 0.3 + 0.1j + 0.05 * (rng.standard_normal(n) + 1j * rng.standard_normal(n))
+    R = rng.standard_normal((8, n)) + 1j * rng.standard_normal((8, n))
+    M = np.eye(n) - t[:, None] * W
+    X, stats, ok = op.solve_master(t, R, tol=1e-9, precision='mixed16', with_stats=True)
+    assert ok and stats['relative_residual'] <= 1e-9 and stats['matvecs_fp16'] > 0
+    X_ref = np.linalg.solve(M, R.T).T
+    for j in range(8):
+        assert np.linalg.norm(X[j] - X_ref[j]) / np.linalg.norm(X_ref[j]) < 1e-7, j

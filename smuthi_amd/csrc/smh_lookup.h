@@ -145,13 +145,17 @@ __global__ void item_count_kernel(const Rec* __restrict__ recs, const int4* __re
   }
 }
 
-// Largest |Re x| + |Im x| of the input (one workgroup).
-__global__ void xmax_kernel(const float2* __restrict__ x, int64_t n, float* xmax) {
-  __shared__ float red[1024];
+// Largest |Re x| + |Im x| of an input, in two passes: kXmaxBlocks workgroups
+// write their maxima after xmax[0], one workgroup takes the largest (times a
+// headroom factor) into xmax[0]. The buffer holds 1 + kXmaxBlocks floats.
+constexpr int kXmaxBlocks = 256;
+template <typename V2>
+__global__ void __launch_bounds__(256) xmax_part_kernel(const V2* __restrict__ x, int64_t n, float* xmax) {
+  __shared__ float red[256];
   float m = 0.f;
-  for (int64_t j = threadIdx.x; j < n; j += blockDim.x) {
-    const float2 v = x[j];
-    m = fmaxf(m, fabsf(v.x) + fabsf(v.y));
+  for (int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j < n; j += (int64_t)gridDim.x * blockDim.x) {
+    const V2 v = x[j];
+    m = fmaxf(m, (float)(fabs((double)v.x) + fabs((double)v.y)));
   }
   red[threadIdx.x] = m;
   __syncthreads();
@@ -159,26 +163,28 @@ __global__ void xmax_kernel(const float2* __restrict__ x, int64_t n, float* xmax
     if ((int)threadIdx.x < w) red[threadIdx.x] = fmaxf(red[threadIdx.x], red[threadIdx.x + w]);
     __syncthreads();
   }
-  if (threadIdx.x == 0) *xmax = red[0];
+  if (threadIdx.x == 0) xmax[1 + blockIdx.x] = red[0];
 }
-
-// The same bound over an FP64 input (the FP16-operand kernel's input is
-// scaled by it while being converted).
-__global__ void xmax64_kernel(const double2* __restrict__ x, int64_t n, float* xmax) {
-  __shared__ float red[1024];
-  float m = 0.f;
-  for (int64_t j = threadIdx.x; j < n; j += blockDim.x) {
-    const double2 v = x[j];
-    m = fmaxf(m, (float)(fabs(v.x) + fabs(v.y)));
-  }
-  red[threadIdx.x] = m;
+__global__ void __launch_bounds__(kXmaxBlocks) xmax_final_kernel(float* xmax, float headroom) {
+  __shared__ float red[kXmaxBlocks];
+  red[threadIdx.x] = xmax[1 + threadIdx.x];
   __syncthreads();
   for (int w = blockDim.x / 2; w > 0; w >>= 1) {
     if ((int)threadIdx.x < w) red[threadIdx.x] = fmaxf(red[threadIdx.x], red[threadIdx.x + w]);
     __syncthreads();
   }
-  // one ulp of headroom: the scaled inputs stay inside [-1, 1]
-  if (threadIdx.x == 0) *xmax = red[0] * (1.f + 1.2e-7f);
+  if (threadIdx.x == 0) xmax[0] = red[0] * headroom;
+}
+// FP32 input: the bound of the fixed-point scales.
+inline void launch_xmax(const float2* x, int64_t n, float* xmax, hipStream_t s) {
+  hipLaunchKernelGGL(xmax_part_kernel<float2>, dim3(kXmaxBlocks), dim3(256), 0, s, x, n, xmax);
+  hipLaunchKernelGGL(xmax_final_kernel, dim3(1), dim3(kXmaxBlocks), 0, s, xmax, 1.f);
+}
+// FP64 input (the FP16-operand kernel's input is scaled by it while being
+// converted): one ulp of headroom, so the scaled inputs stay inside [-1, 1].
+inline void launch_xmax(const double2* x, int64_t n, float* xmax, hipStream_t s) {
+  hipLaunchKernelGGL(xmax_part_kernel<double2>, dim3(kXmaxBlocks), dim3(256), 0, s, x, n, xmax);
+  hipLaunchKernelGGL(xmax_final_kernel, dim3(1), dim3(kXmaxBlocks), 0, s, xmax, 1.f + 1.2e-7f);
 }
 
 // Work items per compute unit: items of one receiver chunk run one after the

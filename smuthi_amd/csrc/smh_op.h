@@ -45,7 +45,7 @@ struct smh_op {
   int T_half = 0;         // its Horner terms: 2 (linear between nodes) or T
   float lin_ratio = -1;   // largest linear-interpolation error / bin scale over the used bins
   float* d_rowsum = nullptr;         // per tap row: max_n1 sum_n2 |Re| + |Im|
-  float* d_xmax = nullptr;           // largest |Re| + |Im| of the current FP32 input
+  float* d_xmax = nullptr;           // [0] largest |Re| + |Im| of the current input; [1..256] partial maxima
 
   // pair binning, one per precision (index 0: FP32, 1: FP64); the receiver
   // chunk size follows the LDS accumulator of that precision

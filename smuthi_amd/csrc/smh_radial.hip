@@ -579,7 +579,7 @@ int op_make_taps(smh_op* op) {
   }
   if (!op->d_rowsum) {
     SMH_HIP(ctx, hipMalloc(&op->d_rowsum, (size_t)op->n_tt * sizeof(float)));
-    SMH_HIP(ctx, hipMalloc(&op->d_xmax, sizeof(float)));
+    SMH_HIP(ctx, hipMalloc(&op->d_xmax, (1 + kXmaxBlocks) * sizeof(float)));
   }
   hipLaunchKernelGGL(tap_rowsum_kernel<double2>, dim3(blocks_for(op->n_tt)), dim3(256), 0, s, op->d_table, d_mh,
                      op->n_tt, op->B, op->interp == SMH_INTERP_SPLINE ? 1 : 0, op->d_rowsum);
@@ -625,7 +625,7 @@ int op_apply(smh_op* op, int precision, const void* x_pad, void* y_pad, bool tim
   hipStream_t s = ctx->stream;
   SMH_HIP(ctx, hipFuncSetAttribute(k.fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bn.lds));
   if (pi == 0)
-    hipLaunchKernelGGL(xmax_kernel, dim3(1), dim3(1024), 0, s, (const float2*)x_pad, op->n_part * op->B, op->d_xmax);
+    launch_xmax((const float2*)x_pad, op->n_part * op->B, op->d_xmax, s);
   if (bn.n_items > 0) {
     if (time_kernel) SMH_HIP(ctx, hipEventRecord(ctx->ev0, s));
     void* args[] = {&a};
@@ -761,7 +761,7 @@ int op_unpack_batch(smh_op* op, int precision, const double2* pk, int nrhs, void
 int op_to_half_batch(smh_op* op, const double2* v, void* out) {
   hipStream_t s = op->ctx->stream;
   const int64_t n8 = op->n_part * op->B * kBatch;
-  hipLaunchKernelGGL(xmax64_kernel, dim3(1), dim3(1024), 0, s, v, n8, op->d_xmax);
+  launch_xmax(v, n8, op->d_xmax, s);
   SMH_HIP(op->ctx, hipMemsetAsync(out, 0, (size_t)op->n_part * op->b32_S * kBatch * 2 * sizeof(_Float16), s));
   hipLaunchKernelGGL(to_half_b16_kernel, dim3(blocks_for(n8)), dim3(256), 0, s, v, n8, op->B, op->b32_S,
                      (const int*)op->d_b32_tab, (const float*)op->d_xmax, (_Float16*)out);
@@ -849,8 +849,7 @@ int op_apply_batch(smh_op* op, int precision, const void* x_pad, void* y_pad, bo
   const int* slot = slots ? op->d_b32_tab : nullptr;
   // (the FP16 input was scaled by the xmax its conversion wrote)
   if (pi == 0 && !half)
-    hipLaunchKernelGGL(xmax_kernel, dim3(1), dim3(1024), 0, s, (const float2*)x_pad, op->n_part * S * kBatch,
-                       op->d_xmax);
+    launch_xmax((const float2*)x_pad, op->n_part * S * kBatch, op->d_xmax, s);
   if (bn.n_items > 0) {
     if (time_kernel) SMH_HIP(ctx, hipEventRecord(ctx->ev0, s));
     const float* hs = op->d_hscale;

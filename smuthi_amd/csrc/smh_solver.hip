@@ -195,6 +195,7 @@ struct Solver {
 
   // h[0..nv) = V[0..nv)^H w (device -> host)
   int dots(const double2* Vb, int nv, const double2* wv, zd* h) {
+    if (nv < 1 || nv > kMaxBasis) return smh_fail(ctx, SMH_ERR_ARG, "basis size out of range");
     hipLaunchKernelGGL(multi_dot_kernel, dim3(kDotBlocks), dim3(kDotThreads), 0, s, Vb, n, nv, wv, n, partial);
     hipLaunchKernelGGL(reduce_partials_kernel, dim3(1), dim3(kMaxBasis), 0, s, partial, kDotBlocks, nv, coef);
     SMH_HIP(ctx, hipMemcpyAsync(h, coef, nv * sizeof(double2), hipMemcpyDeviceToHost, s));
@@ -325,15 +326,27 @@ __global__ void bmulti_dot_kernel(const double2* __restrict__ V, int64_t ld, int
   }
 }
 
-__global__ void breduce_partials_kernel(const double2* __restrict__ partial, int nblocks, int nv, double2* out) {
-  const int jr = threadIdx.x;
-  if (jr >= nv * kB) return;
+// one workgroup per (basis vector, right-hand side), a fixed-order tree over the partials
+__global__ void __launch_bounds__(256) breduce_partials_kernel(const double2* __restrict__ partial, int nblocks,
+                                                               double2* out) {
+  const int jr = blockIdx.x;
+  __shared__ double2 red[256];
   double re = 0, im = 0;
-  for (int b = 0; b < nblocks; ++b) {
-    re += partial[(size_t)b * kMaxBasis * kB + jr].x;
-    im += partial[(size_t)b * kMaxBasis * kB + jr].y;
+  for (int b = threadIdx.x; b < nblocks; b += blockDim.x) {
+    const double2 p = partial[(size_t)b * kMaxBasis * kB + jr];
+    re += p.x;
+    im += p.y;
   }
-  out[jr] = make_double2(re, im);
+  red[threadIdx.x] = make_double2(re, im);
+  __syncthreads();
+  for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) {
+      red[threadIdx.x].x += red[threadIdx.x + w].x;
+      red[threadIdx.x].y += red[threadIdx.x + w].y;
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[jr] = red[0];
 }
 
 // w[e] += sgn * sum_j V_j[e] c[j][e % kB]
@@ -452,8 +465,9 @@ struct BatchSolver {
 
   // h[j][r] = V_j[., r]^H w[., r] for j < nv (device -> host)
   int dots(const double2* Vb, int nv, const double2* wv, zd* h) {
+    if (nv < 1 || nv > kMaxBasis) return smh_fail(ctx, SMH_ERR_ARG, "basis size out of range");
     hipLaunchKernelGGL(bmulti_dot_kernel, dim3(kDotBlocks), dim3(kDotThreads), 0, s, Vb, n8, nv, wv, n8, partial);
-    hipLaunchKernelGGL(breduce_partials_kernel, dim3(1), dim3(kMaxBasis * kB), 0, s, partial, kDotBlocks, nv, coef);
+    hipLaunchKernelGGL(breduce_partials_kernel, dim3(nv * kB), dim3(256), 0, s, partial, kDotBlocks, coef);
     SMH_HIP(ctx, hipMemcpyAsync(h, coef, (size_t)nv * kB * sizeof(double2), hipMemcpyDeviceToHost, s));
     SMH_HIP(ctx, hipStreamSynchronize(s));
     return SMH_OK;

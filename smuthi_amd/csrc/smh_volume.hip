@@ -545,7 +545,7 @@ int vol_apply(smh_op* op, int precision, const void* x_pad, void* y_pad, bool ti
             "no volume matvec kernel for l_max %d m_max %d", op->l_max, op->m_max);
   hipStream_t s = ctx->stream;
   if (pi == 0)
-    hipLaunchKernelGGL(xmax_kernel, dim3(1), dim3(1024), 0, s, (const float2*)x_pad, op->n_part * op->B, op->d_xmax);
+    launch_xmax((const float2*)x_pad, op->n_part * op->B, op->d_xmax, s);
   double ms_total = 0;
   const int n_slices = op->vmode >= 2 ? 4 : 1;
   for (int t = 0; t < 2; ++t) {
@@ -678,7 +678,7 @@ extern "C" int smh_volume_create(smh_ctx* ctx, int64_t n_part, const double* xyz
   hipStream_t s = ctx->stream;
   if (hipMalloc(&op->d_xyz, n_part * 3 * sizeof(double)) != hipSuccess ||
       hipMemcpyAsync(op->d_xyz, xyz, n_part * 3 * sizeof(double), hipMemcpyHostToDevice, s) != hipSuccess ||
-      hipMalloc(&op->d_xmax, sizeof(float)) != hipSuccess) {
+      hipMalloc(&op->d_xmax, (1 + kXmaxBlocks) * sizeof(float)) != hipSuccess) {
     smh_fail(ctx, SMH_ERR_HIP, "device allocation failed");
     return fail(SMH_ERR_HIP);
   }

@@ -531,8 +531,8 @@ def main():
         'roofline': {'bound': 'mfma', 'achieved': achieved, 'peak': peak, 'unit': 'TFLOP/s',
                      'frac': achieved / peak,
                      'traffic': measured_traffic(('batch16' if half else 'batch') if batched else 'single'),
-                     'kernel': (('batch_half_kernel<3,3,4>' if half else 'batch_slots_kernel<3,3,4>') if batched
-                                else 'lookup_matvec_kernel<float,float,3,3,4,false>'),
+                     'kernel': (('batch_half_kernel<3,3,%d>' % half_terms if half else 'batch_slots_kernel<3,3,4>')
+                                if batched else 'lookup_matvec_kernel<float,float,3,3,4,false>'),
                      'flops_per_launch': flops_k},
         'fp16_inner_operator': {'interpolation': ('linear between table nodes' if half_terms == 2
                                                   else 'the operator\'s own (%d Horner terms)' % half_terms),

@@ -12,12 +12,16 @@
 //   * a chunk covers 16 slots (4 slot groups): lane (q, rr) holds the 4 slots
 //     of group 4 c + q of chunk c as its K values, so its azimuthal phase
 //     is chosen per lane from the |m| values of the chunk's groups (one-hot
-//     weights, as the receiver rotation of batch_slots_kernel); one
-//     16x16x32 MFMA takes two chunks;
-//   * result rows, the receiver rotation and the partial-sum layout are those
-//     of batch_slots_kernel, with the scales hscale[bin] * xmax folded into
-//     the rotation; the receiver sums are those of batch_slots_kernel (one
-//     int64 fixed-point word per result, fixed_scale64).
+//     weights, as the receiver rotation of batch_slots_kernel);
+//   * the complex product runs inside the MFMA: one 16x16x32 MFMA per row
+//     tile and chunk takes K = [Re L | Im L] of the chunk's slots against
+//     [Re z; -Im z] (re columns) or [Im z; Re z] (im columns), so each lane
+//     ends with its column of y = L z, and the receiver rotation
+//     e^{-i m1 phi} needs one DPP exchange (lane ^ 1) per value;
+//   * result rows and the partial-sum layout are those of batch_slots_kernel,
+//     with the scales hscale[bin] * xmax folded into the rotation; the
+//     receiver sums are those of batch_slots_kernel (one int64 fixed-point
+//     word per result, fixed_scale64).
 //
 // The operands carry FP16 rounding (relative 2^-11 of the bin / input scale):
 // a matvec is accurate to a few 1e-4 relative, which the FP64 iterative
@@ -40,6 +44,7 @@ namespace smh {
 typedef _Float16 half2_t __attribute__((ext_vector_type(2)));
 typedef _Float16 half4_t __attribute__((ext_vector_type(4)));
 typedef _Float16 half8_t __attribute__((ext_vector_type(8)));
+typedef unsigned u32x2_t __attribute__((ext_vector_type(2)));
 
 // 16 waves per workgroup: the receiver accumulator takes the CU's LDS, so the
 // workgroup is the CU's only one, and 4 waves per SIMD hide the gather latency
@@ -87,7 +92,6 @@ __global__ void __launch_bounds__(kHalfThreads) batch_half_kernel(RadialKArgs a,
   const int rr = lane & 15, q = lane >> 4;
   const int rhs = rr >> 1, zp = rr & 1;
   const unsigned zmask = zp ? 0u : 0x80000000u;  // z: sign of the b term for the re column
-  const float sgnz = zp ? 1.f : -1.f;
   // receiver rotation, as batch_slots_kernel: result rows of tile i are slot group 4 i + q
   float hot[MH][4];
   unsigned smlo[MH], smhi[MH];
@@ -98,8 +102,9 @@ __global__ void __launch_bounds__(kHalfThreads) batch_half_kernel(RadialKArgs a,
     const int sg = q == 0 ? so.sgn[g0] : (q == 1 ? so.sgn[g0 + 1] : (q == 2 ? so.sgn[g0 + 2] : so.sgn[g0 + 3]));
 #pragma unroll
     for (int am = 0; am < 4; ++am) hot[i][am] = ra == am ? 1.f : 0.f;
-    smlo[i] = sg < 0 ? 0x80000000u : 0u;
-    smhi[i] = sg > 0 ? 0u : 0x80000000u;
+    // (the im column takes -sin: y'_im = cos y_im - sin y_re)
+    smlo[i] = (sg < 0 ? 0x80000000u : 0u) ^ (zp ? 0x80000000u : 0u);
+    smhi[i] = (sg > 0 ? 0u : 0x80000000u) ^ (zp ? 0x80000000u : 0u);
   }
   // input rotation: the K values of chunk c are the 4 slots of group 4 c + q
   // (the same group structure as the result rows of tile c)
@@ -115,6 +120,9 @@ __global__ void __launch_bounds__(kHalfThreads) batch_half_kernel(RadialKArgs a,
     zlo[c] = (sg < 0 ? 0x80000000u : 0u) ^ zmask;  // slots j < 2
     zhi[c] = (sg > 0 ? 0u : 0x80000000u) ^ zmask;  // slots j >= 2 (m < 0 in mixed groups)
   }
+  // the Lim half of K takes -Im z (re column) or Re z (im column):
+  // (zp ? 1 : -1) (cz b - e a) with zb's cz and e, the sign as FP16 sign bits
+  const unsigned hsign = zp ? 0u : 0x80008000u;
   const int colbase = (rhs * 2 + zp) * SC + 4 * q;
   // first / second operand plane of the lane: (re, im) for the re column, (im, re) for the im column
   const int64_t xlane = (int64_t)(q * KB + rhs) * NC * 8 + 4 * zp, xother = 4 - 8 * zp;
@@ -177,8 +185,10 @@ __global__ void __launch_bounds__(kHalfThreads) batch_half_kernel(RadialKArgs a,
       const float cs[4] = {1.f, c1, c2, c3};
       const float sn[4] = {0.f, s1, s2, s3};
 
-      // B operands: per chunk the rotated input of the lane's 4 slots, z = cz a + e b in packed FP16
-      half4_t zb[NC];
+      // B operands per chunk, FP16: zb = the lane's column of z = e^{i m2 phi} x
+      // (cz a + e b), zb2 = the column of the other part with the sign of the
+      // complex product (-Im z for the re column, Re z for the im column)
+      half4_t zb[NC], zb2[NC];
 #pragma unroll
       for (int c = 0; c < NC; ++c) {
         float cz = 0.f, sz = 0.f;
@@ -197,14 +207,16 @@ __global__ void __launch_bounds__(kHalfThreads) batch_half_kernel(RadialKArgs a,
         const half2_t b23 = __builtin_bit_cast(half2_t, xc[2 * c + 1].y);
         const half2_t z01 = a01 * cz2 + b01 * elo2, z23 = a23 * cz2 + b23 * ehi2;
         zb[c] = half4_t{z01[0], z01[1], z23[0], z23[1]};
+        const half2_t y01 = b01 * cz2 - a01 * elo2, y23 = b23 * cz2 - a23 * ehi2;
+        zb2[c] = __builtin_bit_cast(half4_t, __builtin_bit_cast(u32x2_t, half4_t{y01[0], y01[1], y23[0], y23[1]}) ^ hsign);
       }
       load_x(rec_n, xc);
 
       const _Float16 wh = (_Float16)rec.w;
       const half2_t w2 = {wh, wh};
-      typename Mfma<float>::v4 acc[MT];
+      typename Mfma<float>::v4 acc[MH];
 #pragma unroll
-      for (int i = 0; i < MT; ++i) acc[i] = typename Mfma<float>::v4{0, 0, 0, 0};
+      for (int i = 0; i < MH; ++i) acc[i] = typename Mfma<float>::v4{0, 0, 0, 0};
       // interpolated coefficients of tile i, chunk c (Horner in packed FP16)
       auto coef = [&](int i, int c) {
         half2_t v0 = tv[T - 1][i][c][0], v1 = tv[T - 1][i][c][1];
@@ -215,21 +227,18 @@ __global__ void __launch_bounds__(kHalfThreads) batch_half_kernel(RadialKArgs a,
         }
         return half4_t{v0[0], v0[1], v1[0], v1[1]};
       };
-      // two chunks per 16x16x32 MFMA (the full-rate FP16 form): the lane's 8
-      // K values are its 4 of chunk c and its 4 of chunk c + 1, in A and B alike
+      // one 16x16x32 MFMA (the full-rate FP16 form) per row tile and chunk:
+      // the lane's 8 K values are its 4 slots of chunk c from Re L (times zb)
+      // and the same 4 from Im L (times zb2), so the column sums are the
+      // complex product's re or im part directly
 #pragma unroll
-      for (int c = 0; c + 1 < NC; c += 2) {
-        const half8_t bv = __builtin_shufflevector(zb[c], zb[c + 1], 0, 1, 2, 3, 4, 5, 6, 7);
+      for (int c = 0; c < NC; ++c) {
+        const half8_t bv = __builtin_shufflevector(zb[c], zb2[c], 0, 1, 2, 3, 4, 5, 6, 7);
 #pragma unroll
-        for (int i = 0; i < MT; ++i) {
-          const half8_t av = __builtin_shufflevector(coef(i, c), coef(i, c + 1), 0, 1, 2, 3, 4, 5, 6, 7);
+        for (int i = 0; i < MH; ++i) {
+          const half8_t av = __builtin_shufflevector(coef(i, c), coef(i + MH, c), 0, 1, 2, 3, 4, 5, 6, 7);
           acc[i] = __builtin_amdgcn_mfma_f32_16x16x32_f16(av, bv, acc[i], 0, 0, 0);
         }
-      }
-      if constexpr (NC % 2 == 1) {
-#pragma unroll
-        for (int i = 0; i < MT; ++i)
-          acc[i] = __builtin_amdgcn_mfma_f32_16x16x16f16(coef(i, NC - 1), zb[NC - 1], acc[i], 0, 0, 0);
       }
 
       // y (rhs, re|im) of rows 16 i + 4 q + v, rotated by e^{-i m1 phi}, scaled by S_hi and the operand scales
@@ -253,11 +262,8 @@ __global__ void __launch_bounds__(kHalfThreads) batch_half_kernel(RadialKArgs a,
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
           const float eps = v < 2 ? elo : ehi;
-          const float al = acc[i][v], be = acc[i + MH][v];
-          // (al er + be eps) of this column + sgnz (be' er - al' eps) of the
-          // other column of the right-hand side (lane ^ 1)
-          const float u = fma_xor1(al, sgnz, be), w = fma_xor1(be, -sgnz, al);
-          const float val = fmaf(er, u, eps * w);
+          // y' = e^{-i m1 phi} y: er y_own + eps y_other, y_other from lane ^ 1
+          const float val = fma_xor1(er * acc[i][v], eps, acc[i][v]);
           atomicAdd(yrow + 16 * i + v, fixed_bits64(val));
         }
       }

@@ -14,7 +14,9 @@
 //   max(tol/2 ||rhs||, kInnerRel ||r||), x += d, r = rhs - M x with the FP64
 //   matvec, until the FP64 residual meets tol. kInnerRel sits at the FP32
 //   operator's resolution, so for tolerances down to ~1e-6 the first sweep
-//   usually suffices and one FP64 matvec checks it.
+//   usually suffices and one FP64 matvec checks it. Inner solves run one
+//   restart cycle per sweep, so the FP64 residual is taken after every cycle
+//   (a reduced-precision operator never restarts on its own residuals).
 // SMH_SOLVE_MIXED16: the same refinement with FP16-operand matvecs in the
 //   inner GMRES (batched radial solves), whose target kInnerRel16 sits at
 //   that operator's resolution (a few 1e-4): each sweep gains about three
@@ -41,6 +43,7 @@ constexpr double kInnerRel16 = 1e-3;  // the same for FP16-operand matvecs
 // FP32 residuals drive FP16 sweeps while above this fraction of ||b|| (about
 // 30 times the FP32 matvec's error)
 constexpr double kFp32ResidualFloor = 1e-5;
+
 
 // partial[b][j] = sum over this block's slice of conj(V_j) . w
 __global__ void multi_dot_kernel(const double2* __restrict__ V, int64_t ld, int nv, const double2* __restrict__ w,
@@ -211,8 +214,11 @@ struct Solver {
 
   // Restarted GMRES on M x = b from x (in/out), stops at ||b - M x|| <= atol.
   // x_zero: x is zero on entry, so the first residual is b itself.
-  // trust_estimate: return as soon as the Arnoldi residual estimate reaches
-  // atol, without recomputing the true residual (the caller checks it).
+  // trust_estimate (inner solves of the refinement): one restart cycle at
+  // most, returning as soon as the Arnoldi residual estimate reaches atol or
+  // the cycle ends, without recomputing the true residual (the caller takes
+  // it in FP64 and starts the next sweep: a reduced-precision operator whose
+  // error sits above atol would otherwise stall the restarts).
   // On return res_out is the true residual of x (or the estimate, see above)
   // and steps the number of Arnoldi steps taken.
   int gmres(double2* x, const double2* b, double atol, int maxiter, int precision, double& res_out, int& steps,
@@ -237,7 +243,6 @@ struct Solver {
       std::fill(g.begin(), g.end(), zd(0));
       g[0] = beta;
       int k = 0;
-      bool est_converged = false;
       for (; k < restart && steps < maxiter; ++k, ++steps) {
         double2* vk = V + (size_t)k * n;
         if ((rc = master(vk, w, precision))) return rc;
@@ -272,7 +277,6 @@ struct Solver {
         for (int j = 0; j <= k; ++j) H[(size_t)j * restart + k] = hcol[j];
         ++iters;
         if (std::abs(g[k + 1]) <= atol || hn == 0.0) {
-          est_converged = true;
           ++k;
           ++steps;
           break;
@@ -288,7 +292,7 @@ struct Solver {
         SMH_HIP(ctx, hipMemcpyAsync(coef, yv.data(), k * sizeof(double2), hipMemcpyHostToDevice, s));
         hipLaunchKernelGGL(multi_update_kernel, dim3(blocks_for(n)), dim3(256), 0, s, V, n, k, coef, x, n);
       }
-      if (trust_estimate && est_converged) {
+      if (trust_estimate) {
         res_out = std::abs(g[k]);
         return SMH_OK;
       }
@@ -572,7 +576,6 @@ struct BatchSolver {
       }
       // per system back substitution, then x += V y for all systems at once
       std::fill(yv.begin(), yv.end(), zd(0));
-      bool all_est = true;
       for (int r = 0; r < kB; ++r) {
         if (!cont[r]) continue;
         const zd* Hr = H.data() + (size_t)r * (restart + 1) * restart;
@@ -583,14 +586,13 @@ struct BatchSolver {
           for (int j = i + 1; j < m; ++j) sum -= Hr[(size_t)i * restart + j] * yv[(size_t)j * kB + r];
           yv[(size_t)i * kB + r] = sum / Hr[(size_t)i * restart + i];
         }
-        if (live[r]) all_est = false;
         res_out[r] = std::abs(gr[m]);
       }
       if (k > 0) {
         SMH_HIP(ctx, hipMemcpyAsync(coef, yv.data(), (size_t)k * kB * sizeof(double2), hipMemcpyHostToDevice, s));
         hipLaunchKernelGGL(bmulti_axpy_kernel, dim3(blocks_for(n8)), dim3(256), 0, s, V, n8, k, coef, 1.0, x, n8);
       }
-      if (trust_estimate && all_est) return SMH_OK;
+      if (trust_estimate) return SMH_OK;
     }
   }
 };
@@ -655,6 +657,7 @@ static int solve_batched(smh_op* op, const double* t_diag, const double* rhs, do
       bool fresh = true;
       bool fp64_residuals = mode != SMH_SOLVE_MIXED16;
       double prev_res[kB];  // residual each system's last sweep started from
+      const int inner_prec = mode == SMH_SOLVE_MIXED16 ? SMH_FP16 : SMH_FP32;
       int budget = maxiter;
       for (int sweep = 0; sweep < 50; ++sweep) {
         if (sweep > 0) {
@@ -700,8 +703,7 @@ static int solve_batched(smh_op* op, const double* t_diag, const double* rhs, do
         SMH_HIP(ctx, hipMemsetAsync(d_pad, 0, S.n8 * sizeof(double2), s));
         double inner_res[kB];
         int inner_steps = 0;
-        if ((rc = S.gmres(d_pad, r_pad, inner_atol, act, budget, mode == SMH_SOLVE_MIXED16 ? SMH_FP16 : SMH_FP32,
-                          inner_res, inner_steps, true, true)))
+        if ((rc = S.gmres(d_pad, r_pad, inner_atol, act, budget, inner_prec, inner_res, inner_steps, true, true)))
           return done(rc);
         budget -= inner_steps;
         steps_total += inner_steps;

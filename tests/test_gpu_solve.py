@@ -191,3 +191,36 @@ def test_mixed16_linear_fp16_operator(gpu):
     X_ref = np.linalg.solve(M, R.T).T
     for j in range(8):
         assert np.linalg.norm(X[j] - X_ref[j]) / np.linalg.norm(X_ref[j]) < 1e-7, j
+
+
+def test_refinement_on_ill_scaled_operator(gpu):
+    """Overlapping spheres (centres 143 nm apart, radii 100 and 80 nm) make the lookup operator ill-scaled
+    (entries up to ~340, cond(M) ~ 500): with restarts longer than the system (GMRES(20) stagnates near 3e-3
+    on it, as scipy's would), the refinement (one inner restart cycle per sweep, FP64 residuals in between)
+    meets the tolerance against the oracle's dense LU in FP32 and FP16 inner precision (single and batched)."""
+    from smuthi_amd.layers import LayerSystem
+    from smuthi_amd.linear_system import CouplingMatrixRadialLookupCUDA
+    kp = ob.reasonable_Sommerfeld_kpar_contour(WL, layer_refractive_indices=[1.52, 1], neff_resolution=1e-2)
+    plist = [osys.Sphere([0, 0, 150], 2.4 + 0.1j, 100, 3, 3), osys.Sphere([102, -100, 150], 1.9 + 0.2j, 80, 3, 2),
+             osys.Sphere([202, 100, 150], 1.7, 90, 3, 3)]
+    op = CouplingMatrixRadialLookupCUDA(WL, plist, LayerSystem([0, 0], [1.52, 1]), k_parallel=kp, resolution=5.0,
+                                        interpolator_kind='cubic')
+    ref = oc.RadialLookupCPU(op.lookup_table, op.radial_distance_array, plist, 'cubic')
+    rng = np.random.default_rng(0)
+    n = ref.n_unk
+    t = 0.3 + 0.1j + 0.05 * rng.standard_normal(n)
+    M = np.eye(n) - t[:, None] * ref.dense()
+    r = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    x, st, ok = op.solve_master(t, r, tol=1e-8, precision='mixed', restart=100, with_stats=True)
+    print('single mixed:', st, flush=True)
+    assert ok and st['relative_residual'] <= 1e-8
+    assert np.linalg.norm(x - np.linalg.solve(M, r)) / np.linalg.norm(x) < 1e-6
+    R = rng.standard_normal((3, n)) + 1j * rng.standard_normal((3, n))
+    X_ref = np.linalg.solve(M, R.T).T
+    for prec in ('mixed', 'mixed16'):
+        X, st, ok = op.solve_master(t, R, tol=1e-8, precision=prec, restart=100, with_stats=True)
+        print('batched %s:' % prec, st, flush=True)
+        assert ok and st['relative_residual'] <= 1e-8, prec
+        for j in range(3):
+            assert np.linalg.norm(X[j] - X_ref[j]) / np.linalg.norm(X_ref[j]) < 1e-6, (prec, j)
+

@@ -125,8 +125,8 @@ constexpr unsigned long long kMagic64Bits = 0x4338000000000000ull;
 __device__ inline void fixed_scale64(float lsum, float xmax, int eN, float& sh, double& ish) {
   const double V = 8.0 * (double)lsum * (double)xmax;
   int k = 0;
-  if (V > 0.0 && V < 1e30) k = min(51, 62 - eN) - ((int)floor(log2(V)) + 1);
-  k = max(-100, min(k, 100));
+  if (V > 0.0 && V < 1e300) k = min(51, 62 - eN) - ((int)floor(log2(V)) + 1);
+  k = max(-126, min(k, 126));  // the float range of sh
   sh = ldexpf(1.f, k);
   ish = ldexp(1.0, -k);
 }

@@ -72,12 +72,12 @@ TRAFFIC_FILE16 = os.path.join(ROOT, 'profiles', 'radial_matvec_10k_fp16_8rhs.md'
 
 
 def measured_traffic(kind='single'):
-    """HBM bytes per matvec launch (single-RHS FP32, 8-RHS FP32 or 8-RHS FP16
-    kernel) from the committed counter summary, or None."""
+    """HBM bytes per matvec launch (single-RHS FP32, 8-RHS FP32, FP16 or FP64
+    kernel) from the committed counter summaries, or None."""
     key = {'single': 'traffic_bytes_per_launch', 'batch': 'traffic_bytes_per_launch_8rhs',
-           'batch16': 'traffic_bytes_per_launch_8rhs_fp16'}[kind]
+           'batch16': 'traffic_bytes_per_launch_8rhs_fp16', 'batch64': 'traffic_bytes_per_launch_8rhs_fp64'}[kind]
     try:
-        for line in open(TRAFFIC_FILE16 if kind == 'batch16' else TRAFFIC_FILE):
+        for line in open(TRAFFIC_FILE16 if kind in ('batch16', 'batch64') else TRAFFIC_FILE):
             if line.split('=')[0].strip() == key:
                 return float(line.split('=')[1])
     except OSError:
@@ -541,6 +541,10 @@ def main():
                                       'achieved': flops_8 / (kb32 * 1e-3) / 1e12, 'peak': MFMA_FP32_PEAK_TFLOPS,
                                       'frac': flops_8 / (kb32 * 1e-3) / 1e12 / MFMA_FP32_PEAK_TFLOPS,
                                       'traffic': measured_traffic('batch')},
+        'roofline_fp64_8rhs_kernel': {'kernel': 'batch_slots64_kernel<3,3,4>', 'ms': kb64,
+                                      'achieved': flops_8 / (kb64 * 1e-3) / 1e12, 'peak': MFMA_FP64_PEAK_TFLOPS,
+                                      'frac': flops_8 / (kb64 * 1e-3) / 1e12 / MFMA_FP64_PEAK_TFLOPS,
+                                      'traffic': measured_traffic('batch64')},
         'roofline_single_rhs_kernel': {'achieved': flops_1 / (k32 * 1e-3) / 1e12, 'peak': MFMA_FP32_PEAK_TFLOPS,
                                        'frac': flops_1 / (k32 * 1e-3) / 1e12 / MFMA_FP32_PEAK_TFLOPS,
                                        'traffic': measured_traffic('single')},

@@ -399,7 +399,7 @@ def main():
     def make_system(j, precision, tol):
         polar, az, pol = rhs_for_step(j)
         return LinearSystem(plist, PlaneWave(WL, polar, az, pol), ls, kp, solver_type='gmres', solver_tolerance=tol,
-                            coupling_matrix_lookup_resolution=2.0, interpolator_kind='cubic',
+                            store_coupling_matrix=False, coupling_matrix_lookup_resolution=2.0, interpolator_kind='cubic',
                             precision=precision)
 
     t0 = time.time()

@@ -40,6 +40,7 @@ extern "C" {
 #define SMH_SOLVE_FP64 0  /* restarted GMRES, FP64 matvec, FP64 Arnoldi */
 #define SMH_SOLVE_MIXED 1 /* FP64 iterative refinement around FP32-matvec GMRES */
 #define SMH_SOLVE_MIXED16 2 /* FP64 iterative refinement around FP16-matvec GMRES (batched radial solves) */
+#define SMH_SOLVE_LU 3      /* LU factorization of the explicit master matrix (solver_type 'LU') */
 
 typedef struct smh_ctx smh_ctx;
 typedef struct smh_op smh_op;
@@ -74,6 +75,33 @@ int smh_radial_create(smh_ctx *ctx, int64_t n_part, const double *xy, const int 
 int smh_volume_create(smh_ctx *ctx, int64_t n_part, const double *xyz, const int *lmax, const int *mmax,
                       int l_max, int m_max, double rho_min, double sz_min, double dz_min, double resolution,
                       int64_t len_rho, int64_t len_z, int interp, smh_op **out);
+
+/* Several particle ensembles in one operator (no counterpart in smuthi,
+ * which runs one Simulation per ensemble, simulation.py:363-414, and builds
+ * its table each time, linear_system.py:183-225): ensemble e holds particles
+ * [ens_off[e], ens_off[e+1]) of xy / xyz (ens_off[0] = 0, n_ens + 1 offsets,
+ * every ensemble non-empty). Pairs couple particles of one ensemble only; all
+ * ensembles share the table (sized by the caller for the largest), every
+ * matvec launch covers all of them, and smh_solve treats each (ensemble,
+ * right-hand side) as a system of its own with its own convergence. System
+ * vectors concatenate the ensembles' vectors. Other arguments as the
+ * single-ensemble constructors. */
+int smh_radial_create_ensembles(smh_ctx *ctx, int n_ens, const int64_t *ens_off, const double *xy, const int *lmax,
+                                const int *mmax, int l_max, int m_max, double rho_min, double resolution,
+                                int64_t len_rho, int interp, smh_op **out);
+int smh_volume_create_ensembles(smh_ctx *ctx, int n_ens, const int64_t *ens_off, const double *xyz, const int *lmax,
+                                const int *mmax, int l_max, int m_max, double rho_min, double sz_min, double dz_min,
+                                double resolution, int64_t len_rho, int64_t len_z, int interp, smh_op **out);
+
+/* Right block preconditioner for smh_solve (no counterpart in smuthi, whose
+ * GMRES is unpreconditioned, linear_system.py:271-286): blocks = complex128
+ * [n_part][B][B], B = blocksize(l_max, m_max) of the operator, on the padded
+ * multipole index (rows and columns of multipoles a particle does not have
+ * are ignored: those vector entries stay zero). smh_solve then runs GMRES on
+ * M P y = b and returns x = P y, with the same FP64 stopping rule on
+ * ||b - M x||. NULL removes it. Needs the batched kernels (radial or volume
+ * operators). */
+int smh_op_set_preconditioner(smh_op *op, const double *blocks);
 
 /* Build both volume tables on the GPU (volumetric_coupling_lookup_table,
  * prepare_lookup.py:16-250, and its helper kernel prepare_lookup_cuda.py:6-52),
@@ -174,13 +202,19 @@ int smh_op_bench_matvec_batch(smh_op *op, int precision, int iters, double *kern
  * smuthi, which has no reduced-precision operator. */
 int smh_op_half_interp(smh_op *op, int *terms, double *lin_ratio);
 
-/* Solve (1 - T W) b = T a (LinearSystem.solve with solver_type='gmres',
- * linear_system.py:254-344) for nrhs right-hand sides.
+/* Solve (1 - T W) b = T a (LinearSystem.solve, linear_system.py:254-344)
+ * for nrhs right-hand sides.
  * t_diag: complex128 [n_unk] diagonal sphere T-matrix; rhs, x: complex128
- * [nrhs][n_unk]. SMH_SOLVE_FP64 starts from x0 = rhs as smuthi does;
- * SMH_SOLVE_MIXED refines from x0 = 0. Both stop at ||rhs - M x|| <= tol ||rhs||
- * measured in FP64. stats (out, 8 doubles): iterations, FP64 matvecs, FP32
- * matvecs, worst relative residual, solve seconds, refinement sweeps, 0, 0. */
+ * [nrhs][n_unk]. SMH_SOLVE_FP64 (solver_type 'gmres') starts from x0 = rhs as
+ * smuthi does; SMH_SOLVE_MIXED / MIXED16 refine from x0 = 0. They stop at
+ * ||rhs - M x|| <= tol ||rhs|| measured in FP64 and return SMH_ERR_NOCONV
+ * when maxiter Arnoldi steps do not get there. SMH_SOLVE_LU (solver_type
+ * 'LU', linear_system.py:259-268, explicit operators only) factors M on the
+ * device once per T (kept on the operator) and solves directly; tol, restart
+ * and maxiter are not used. stats (out, 8 doubles): Arnoldi steps, FP64
+ * matvecs, FP32 matvecs, worst relative FP64 residual, solve seconds,
+ * refinement sweeps, right-hand sides per batched matvec (0 when the solve
+ * was not batched), FP16 matvecs. */
 int smh_solve(smh_op *op, const double *t_diag, const double *rhs, double *x, int64_t nrhs, double tol,
               int restart, int maxiter, int mode, double *stats);
 

@@ -29,6 +29,7 @@ FP16 = 2
 SOLVE_FP64 = 0
 SOLVE_MIXED = 1
 SOLVE_MIXED16 = 2
+SOLVE_LU = 3
 
 _ERR_NAMES = {1: 'invalid argument', 2: 'HIP error', 3: 'unsupported', 4: 'invalid state', 5: 'not converged'}
 
@@ -47,6 +48,12 @@ SIGNATURES = {
                                      _c.c_int, _c.POINTER(_p)]),
     'smh_volume_create': (_c.c_int, [_p, _i64, _dp, _ip, _ip, _c.c_int, _c.c_int, _c.c_double, _c.c_double,
                                      _c.c_double, _c.c_double, _i64, _i64, _c.c_int, _c.POINTER(_p)]),
+    'smh_radial_create_ensembles': (_c.c_int, [_p, _c.c_int, _c.POINTER(_i64), _dp, _ip, _ip, _c.c_int, _c.c_int,
+                                               _c.c_double, _c.c_double, _i64, _c.c_int, _c.POINTER(_p)]),
+    'smh_volume_create_ensembles': (_c.c_int, [_p, _c.c_int, _c.POINTER(_i64), _dp, _ip, _ip, _c.c_int, _c.c_int,
+                                               _c.c_double, _c.c_double, _c.c_double, _c.c_double, _i64, _i64,
+                                               _c.c_int, _c.POINTER(_p)]),
+    'smh_op_set_preconditioner': (_c.c_int, [_p, _dp]),
     'smh_volume_build_table': (_c.c_int, [_p, _c.c_double, _c.c_double, _c.c_double, _c.c_double, _dp, _i64, _dp,
                                           _dp, _dp]),
     'smh_op_destroy': (_c.c_int, [_p]),
@@ -110,6 +117,11 @@ def dptr(a):
         return None
     assert a.flags['C_CONTIGUOUS'] and a.dtype in (np.float64, np.complex128), (a.dtype, a.flags)
     return a.ctypes.data_as(_dp)
+
+
+def i64ptr(a):
+    assert a.flags['C_CONTIGUOUS'] and a.dtype == np.int64
+    return a.ctypes.data_as(_c.POINTER(_i64))
 
 
 def iptr(a):

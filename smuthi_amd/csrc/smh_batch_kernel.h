@@ -11,6 +11,9 @@
 // epilogue combines the re/im columns of a lane pair, rotates by
 // e^{-i m1 phi} and adds to the receiver's LDS accumulator [Rc][kBatch][2][B]
 // (FP32: two-level int32 fixed point as the single-RHS kernel; FP64: doubles).
+// Volume lookups (VOL) take the taps and weights of vol_tap_offset /
+// vol_tap_weight for a.vmode and the z slice a.vslice, one launch per table
+// and slice as the single-vector volume kernel.
 #pragma once
 
 #include "smh_op.h"
@@ -109,7 +112,7 @@ __global__ void __launch_bounds__((BatchShape<Real, SvwfIdx<LMAX, MMAX>::B, T>::
     Real tv[T][MT][SP];
 #pragma unroll
     for (int t = 0; t < T; ++t) {
-      const int toff = VOL ? (t >> 1) * a.len_z + (t & 1) : t;
+      const int toff = VOL ? vol_tap_offset(a.vmode, a.vslice, t, a.len_z) : t;
       const TapT* row = ttg + (size_t)(en.x + toff) * FR;
 #pragma unroll
       for (int i = 0; i < MT; ++i)
@@ -146,7 +149,7 @@ __global__ void __launch_bounds__((BatchShape<Real, SvwfIdx<LMAX, MMAX>::B, T>::
 #pragma unroll
       for (int t = 0; t < T; ++t) {
         if constexpr (VOL)
-          w[t] = ((t >> 1) ? rec.w : Real(1) - rec.w) * ((t & 1) ? rec.w2 : Real(1) - rec.w2);
+          w[t] = vol_tap_weight<Real>(a.vmode, a.vslice, t, rec.w, rec.w2);
         else
           w[t] = tap_weight<Real>(a.interp, t, rec.w);
       }

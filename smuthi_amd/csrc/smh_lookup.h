@@ -62,6 +62,34 @@ __global__ void reduce_kernel(const Real* __restrict__ ypart, const int* __restr
   y[j] = typename Vec2<Real>::t{re, im};
 }
 
+// Sum the work items of each chunk for the kBatch-RHS kernels: partial
+// [Rc][kBatch][2][S] (slot order when slot is given, else S = B) -> y
+// [n_part][B][kBatch] (y = sum, or y += sum when accumulate is set).
+template <typename Real>
+__global__ void reduce_batch_kernel(const Real* __restrict__ ypart, const int* __restrict__ chunk_items, int Rc,
+                                    int B, int S, const int* __restrict__ slot, int64_t n_part,
+                                    typename Vec2<Real>::t* y, int accumulate) {
+  const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n_part * B * kBatchRhs) return;
+  const int r = (int)(j % kBatchRhs);
+  const int64_t in = j / kBatchRhs;
+  const int64_t i1 = in / B;
+  const int n1 = (int)(in - i1 * B);
+  const int k1 = slot ? slot[n1] : n1;
+  const int c = (int)(i1 / Rc), il = (int)(i1 - (int64_t)c * Rc);
+  Real re = 0, im = 0;
+  for (int w = chunk_items[c]; w < chunk_items[c + 1]; ++w) {
+    const Real* p = ypart + ((size_t)w * Rc + il) * 2 * kBatchRhs * S + (size_t)(2 * r) * S;
+    re += p[k1];
+    im += p[S + k1];
+  }
+  if (accumulate) {
+    re += y[j].x;
+    im += y[j].y;
+  }
+  y[j] = typename Vec2<Real>::t{re, im};
+}
+
 // Row sums of tap rows: rowsum[row] = max_n1 sum_n2 (|Re| + |Im|) of row
 // `row` of a [rows][B][B] table (with `interleave`, even rows come from
 // table and odd rows from mh, both indexed row / 2), with a small margin for
@@ -185,6 +213,18 @@ inline void launch_xmax(const float2* x, int64_t n, float* xmax, hipStream_t s) 
 inline void launch_xmax(const double2* x, int64_t n, float* xmax, hipStream_t s) {
   hipLaunchKernelGGL(xmax_part_kernel<double2>, dim3(kXmaxBlocks), dim3(256), 0, s, x, n, xmax);
   hipLaunchKernelGGL(xmax_final_kernel, dim3(1), dim3(kXmaxBlocks), 0, s, xmax, 1.f + 1.2e-7f);
+}
+
+// Zero-fill n 32-bit words with a kernel (callers may capture the launch in a
+// graph, where a memset node is avoided).
+__global__ void zero_words_kernel(uint32_t* __restrict__ p, int64_t n) {
+  for (int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j < n; j += (int64_t)gridDim.x * blockDim.x)
+    p[j] = 0u;
+}
+inline void fill_zero(void* p, size_t bytes, hipStream_t s) {
+  const int64_t n = (int64_t)(bytes / 4);
+  hipLaunchKernelGGL(zero_words_kernel, dim3((unsigned)std::min<int64_t>((n + 255) / 256, 4096)), dim3(256), 0, s,
+                     (uint32_t*)p, n);
 }
 
 // Work items per compute unit: items of one receiver chunk run one after the

@@ -72,10 +72,35 @@ struct smh_op {
   int vmode = 1, vq = 1;  // interpolation mode (vol_tap_offset) and tap rows per (rho, z) node
   float* d_vrowsum[2] = {nullptr, nullptr};
   Binning vbin[2][2];
+  Binning vbbin[2][2];  // the same pairs binned for the kBatch-RHS kernel
 
   // explicit operator: dense W [n_part * B][n_part * B] on padded vectors
   double2* d_W64 = nullptr;
   float2* d_W32 = nullptr;
+  // LU factors of M = 1 - diag(t) W (explicit operator, solver_type 'LU'),
+  // the T they were formed with and the rocBLAS handle of the factorization
+  double2* d_lu = nullptr;
+  int* d_piv = nullptr;
+  void* blas = nullptr;
+  std::vector<double> lu_t;
+
+  // particle ensembles ("segments") of one operator: particles
+  // [seg_off[s], seg_off[s+1]) form ensemble s; pairs couple particles of one
+  // ensemble only and the solver treats every (ensemble, right-hand side) as
+  // a system of its own. One segment unless created for several ensembles.
+  int n_seg = 1;
+  std::vector<int64_t> seg_off;  // host, n_seg + 1 particle offsets
+  int64_t* d_seg_off = nullptr;  // device copy
+  int* d_seg_of = nullptr;       // device: ensemble of every particle (null: one ensemble)
+
+  // right block preconditioner P = blockdiag(C_i), complex128 [n_part][B][B]
+  // on the padded index (null: none); GMRES then solves M P y = b, x = P y
+  double2* d_P = nullptr;
+
+  // solver workspace (smh_solver.hip), grown on demand, freed with the operator
+  void* solver_ws = nullptr;
+  // kernels whose dynamic LDS limit has been raised for this operator
+  std::vector<void*> attr_set;
 
   // work buffers
   void* d_ypart = nullptr;
@@ -121,11 +146,25 @@ int op_to_half_batch(smh_op* op, const double2* v, void* out);
 int op_index_map(smh_op* op, const int* lmax, const int* mmax);
 // volume operator: matvec, table upload / download (smh_volume.hip)
 int vol_apply(smh_op* op, int precision, const void* x_pad, void* y_pad, bool time_kernel);
+int vol_apply_batch(smh_op* op, int precision, const void* x_pad, void* y_pad, bool time_kernel);
+// particle ensembles of an operator: n_ens offsets ens_off[0] = 0 < ... < ens_off[n_ens] = n_part
+int op_set_segments(smh_op* op, int n_ens, const int64_t* ens_off);
 int vol_upload(smh_op* op, const double* plus, const double* minus);
 int vol_download(smh_op* op, double* plus, double* minus);
 // explicit operator: dense matvec and the padded matrix (smh_explicit.hip)
 int explicit_apply(smh_op* op, int precision, const void* x_pad, void* y_pad, bool time_kernel);
 int explicit_download(smh_op* op, double* W);
+// solver workspace release, and the drop of its captured graphs when the
+// operator's buffers change (smh_solver.hip)
+void solver_release(smh_op* op);
+void solver_drop_graphs(smh_op* op);
+// raise a kernel's dynamic LDS limit once per operator (outside stream capture)
+int op_set_lds(smh_op* op, void* fn, size_t lds);
+// y[p] = P_p x[p] (+ y[p] when accumulate) for kBatch padded vectors [n_part][B][kBatch]
+int op_apply_precond_batch(smh_op* op, const double2* x, double2* y, bool accumulate);
+// explicit operator: LU solve on the device (smh_lu.hip) and its release
+int lu_solve(smh_op* op, const double* t_diag, const double* rhs, double* x, int64_t nrhs, double* stats);
+void lu_release(smh_op* op);
 // kBatch-RHS matvec (radial operators): FP64 input and every output on padded
 // [n_part][B][kBatch] vectors; the FP32 input in the layout of b32_index()
 bool op_has_batch(const smh_op* op, int precision);

@@ -15,14 +15,17 @@ namespace {
 
 constexpr int kThreads = 256;
 
+// Grid: (receiver i1, block of 256 emitters of i1's ensemble); pairs couple
+// particles of one ensemble only (seg_of null: one ensemble).
 __global__ void pair_bin_kernel(const double2* __restrict__ xy, int n, double rho_min, double h, int len_rho,
-                                int s_mode, int o_mode, int T, int64_t n_tt, int Rc, int* cnt, const int* off,
-                                int* cursor, uint2* pairs, int* err) {
-  const int nb2 = (n + 255) / 256;
-  const int64_t b = blockIdx.x;
-  const int i1 = (int)(b / nb2);
-  const int i2 = (int)(b % nb2) * 256 + threadIdx.x;
-  if (i1 >= n || i2 >= n) return;
+                                int s_mode, int o_mode, int T, int64_t n_tt, int Rc, const int* __restrict__ seg_of,
+                                const int64_t* __restrict__ seg_off, int* cnt, const int* off, int* cursor,
+                                uint2* pairs, int* err) {
+  const int i1 = blockIdx.x;
+  const int sg = seg_of ? seg_of[i1] : 0;
+  const int64_t i2l = (int64_t)seg_off[sg] + (int64_t)blockIdx.y * 256 + threadIdx.x;
+  if (i1 >= n || i2l >= seg_off[sg + 1]) return;
+  const int i2 = (int)i2l;
   const double dx = xy[i1].x - xy[i2].x, dy = xy[i1].y - xy[i2].y;
   int r;
   double w, rho;
@@ -465,6 +468,16 @@ KernelInfo radial_kernel(int precision, int T, int l, int m) {
 
 namespace smh {
 
+int op_set_lds(smh_op* op, void* fn, size_t lds) {
+  for (void* f : op->attr_set)
+    if (f == fn) return SMH_OK;
+  // the device's limit, so launches of this kernel by any operator fit
+  const size_t lim = std::max(lds, op->ctx->lds_per_block);
+  SMH_HIP(op->ctx, hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lim));
+  op->attr_set.push_back(fn);
+  return SMH_OK;
+}
+
 int op_reserve_packed(smh_op* op, int64_t nrhs) {
   const int64_t need = op->n_unk * nrhs;
   if (need <= op->pk_cap) return SMH_OK;
@@ -623,7 +636,7 @@ int op_apply(smh_op* op, int precision, const void* x_pad, void* y_pad, bool tim
   a.interp = op->interp;
   a.Rc = bn.Rc;
   hipStream_t s = ctx->stream;
-  SMH_HIP(ctx, hipFuncSetAttribute(k.fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bn.lds));
+  if (int rc_ = op_set_lds(op, k.fn, bn.lds)) return rc_;
   if (pi == 0)
     launch_xmax((const float2*)x_pad, op->n_part * op->B, op->d_xmax, s);
   if (bn.n_items > 0) {
@@ -705,30 +718,6 @@ __global__ void pack_batch_kernel(const typename Vec2<Real>::t* __restrict__ pad
   pk[j] = make_double2((double)v.x, (double)v.y);
 }
 
-// Sum the work items of each chunk for the kBatch-RHS kernels: partial
-// [Rc][kBatch][2][S] (slot order when slot is given, else S = B) -> y
-// [n_part][B][kBatch].
-template <typename Real>
-__global__ void reduce_batch_kernel(const Real* __restrict__ ypart, const int* __restrict__ chunk_items, int Rc,
-                                    int B, int S, const int* __restrict__ slot, int64_t n_part,
-                                    typename Vec2<Real>::t* y) {
-  const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= n_part * B * kBatch) return;
-  const int r = (int)(j % kBatch);
-  const int64_t in = j / kBatch;
-  const int64_t i1 = in / B;
-  const int n1 = (int)(in - i1 * B);
-  const int k1 = slot ? slot[n1] : n1;
-  const int c = (int)(i1 / Rc), il = (int)(i1 - (int64_t)c * Rc);
-  Real re = 0, im = 0;
-  for (int w = chunk_items[c]; w < chunk_items[c + 1]; ++w) {
-    const Real* p = ypart + ((size_t)w * Rc + il) * 2 * kBatch * S + (size_t)(2 * r) * S;
-    re += p[k1];
-    im += p[S + k1];
-  }
-  y[j] = typename Vec2<Real>::t{re, im};
-}
-
 }  // namespace
 
 namespace smh {
@@ -762,7 +751,7 @@ int op_to_half_batch(smh_op* op, const double2* v, void* out) {
   hipStream_t s = op->ctx->stream;
   const int64_t n8 = op->n_part * op->B * kBatch;
   launch_xmax(v, n8, op->d_xmax, s);
-  SMH_HIP(op->ctx, hipMemsetAsync(out, 0, (size_t)op->n_part * op->b32_S * kBatch * 2 * sizeof(_Float16), s));
+  fill_zero(out, (size_t)op->n_part * op->b32_S * kBatch * 2 * sizeof(_Float16), s);
   hipLaunchKernelGGL(to_half_b16_kernel, dim3(blocks_for(n8)), dim3(256), 0, s, v, n8, op->B, op->b32_S,
                      (const int*)op->d_b32_tab, (const float*)op->d_xmax, (_Float16*)out);
   SMH_HIP(op->ctx, hipGetLastError());
@@ -794,12 +783,37 @@ int op_reserve_batch(smh_op* op) {
 
 bool op_has_batch(const smh_op* op, int precision) {
   if (precision == SMH_FP16) return op->kind == 0 && op->bbin[0].Rc > 0 && op->d_tth != nullptr;
+  if (op->kind == 1) return op->vbbin[0][precision == SMH_FP32 ? 0 : 1].Rc > 0;
   return op->kind == 0 && op->bbin[precision == SMH_FP32 ? 0 : 1].Rc > 0;
+}
+
+int op_set_segments(smh_op* op, int n_ens, const int64_t* ens_off) {
+  smh_ctx* ctx = op->ctx;
+  SMH_CHECK(ctx, n_ens >= 1 && n_ens <= (1 << 16) && ens_off[0] == 0 && ens_off[n_ens] == op->n_part, SMH_ERR_ARG,
+            "ensemble offsets must run from 0 to the particle count");
+  for (int e = 0; e < n_ens; ++e)
+    SMH_CHECK(ctx, ens_off[e + 1] > ens_off[e], SMH_ERR_ARG, "ensemble %d is empty", e);
+  op->n_seg = n_ens;
+  op->seg_off.assign(ens_off, ens_off + n_ens + 1);
+  std::vector<int> seg_of(op->n_part);
+  for (int e = 0; e < n_ens; ++e)
+    for (int64_t i = ens_off[e]; i < ens_off[e + 1]; ++i) seg_of[i] = e;
+  if (op->d_seg_off) hipFree(op->d_seg_off);
+  op->d_seg_off = nullptr;
+  SMH_HIP(ctx, hipMalloc(&op->d_seg_off, (n_ens + 1) * sizeof(int64_t)));
+  SMH_HIP(ctx, hipMalloc(&op->d_seg_of, op->n_part * sizeof(int)));
+  SMH_HIP(ctx, hipMemcpy(op->d_seg_off, op->seg_off.data(), (n_ens + 1) * sizeof(int64_t), hipMemcpyHostToDevice));
+  SMH_HIP(ctx, hipMemcpy(op->d_seg_of, seg_of.data(), op->n_part * sizeof(int), hipMemcpyHostToDevice));
+  return SMH_OK;
 }
 
 int op_apply_batch(smh_op* op, int precision, const void* x_pad, void* y_pad, bool time_kernel) {
   smh_ctx* ctx = op->ctx;
   SMH_CHECK(ctx, op->has_table, SMH_ERR_STATE, "coupling table not set");
+  if (op->kind == 1) {
+    SMH_CHECK(ctx, precision != SMH_FP16, SMH_ERR_UNSUPPORTED, "no FP16 volume matvec");
+    return vol_apply_batch(op, precision, x_pad, y_pad, time_kernel);
+  }
   const bool half = precision == SMH_FP16;
   const int pi = (precision == SMH_FP32 || half) ? 0 : 1;
   SMH_CHECK(ctx, op_has_batch(op, precision), SMH_ERR_UNSUPPORTED, "no batched matvec for this operator");
@@ -843,7 +857,7 @@ int op_apply_batch(smh_op* op, int precision, const void* x_pad, void* y_pad, bo
   a.len_z = 0;
   a.row_s = op->s_mode;
   a.row_o = op->o_mode;
-  SMH_HIP(ctx, hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bn.lds));
+  if (int rc_ = op_set_lds(op, fn, bn.lds)) return rc_;
   const int64_t nx = op->n_part * op->B * kBatch;
   const int S = slots ? k.slots : op->B;
   const int* slot = slots ? op->d_b32_tab : nullptr;
@@ -861,11 +875,11 @@ int op_apply_batch(smh_op* op, int precision, const void* x_pad, void* y_pad, bo
   }
   if (pi == 0)
     hipLaunchKernelGGL(reduce_batch_kernel<float>, dim3(blocks_for(nx)), dim3(256), 0, s, (const float*)op->d_ypart,
-                       bn.d_chunk_items, bn.Rc, op->B, S, slot, op->n_part, (float2*)y_pad);
+                       bn.d_chunk_items, bn.Rc, op->B, S, slot, op->n_part, (float2*)y_pad, 0);
   else
     hipLaunchKernelGGL(reduce_batch_kernel<double>, dim3(blocks_for(nx)), dim3(256), 0, s,
                        (const double*)op->d_ypart, bn.d_chunk_items, bn.Rc, op->B, S, slot, op->n_part,
-                       (double2*)y_pad);
+                       (double2*)y_pad, 0);
   SMH_HIP(ctx, hipGetLastError());
   if (time_kernel && bn.n_items > 0) {
     SMH_HIP(ctx, hipEventSynchronize(ctx->ev1));
@@ -896,6 +910,12 @@ int op_index_map(smh_op* op, const int* lmax, const int* mmax) {
     n_unk += smh_blocksize(lmax[i], mmax[i]);
   }
   op->n_unk = n_unk;
+  // one ensemble unless the operator was created for several
+  op->n_seg = 1;
+  op->seg_off = {0, op->n_part};
+  SMH_HIP(ctx, hipMalloc(&op->d_seg_off, 2 * sizeof(int64_t)));
+  SMH_HIP(ctx, hipMemcpyAsync(op->d_seg_off, op->seg_off.data(), 2 * sizeof(int64_t), hipMemcpyHostToDevice,
+                              ctx->stream));
   SMH_HIP(ctx, hipMalloc(&op->d_pk2pad, n_unk * sizeof(int)));
   SMH_HIP(ctx, hipMemcpyAsync(op->d_pk2pad, op->pk2pad.data(), n_unk * sizeof(int), hipMemcpyHostToDevice,
                               ctx->stream));
@@ -906,6 +926,11 @@ int op_index_map(smh_op* op, const int* lmax, const int* mmax) {
 }  // namespace smh
 
 void smh_op_free(smh_op* op) {
+  smh::lu_release(op);
+  smh::solver_release(op);
+  void* sp[] = {op->d_seg_off, op->d_seg_of, op->d_P};
+  for (void* p : sp)
+    if (p) hipFree(p);
   void* ptrs[] = {op->d_xy, op->d_xyz, op->d_pk2pad, op->d_table, op->d_table_minus, op->d_tt64, op->d_tt32,
                   op->d_tt32b, op->d_tt64b, op->d_x64s, op->d_b32_tab, op->d_tth, op->d_hscale, op->d_W64, op->d_W32, op->d_ypart, op->d_pk_in, op->d_pk_out, op->d_x32, op->d_y32, op->d_x64, op->d_y64,
                   op->d_rowsum, op->d_xmax, op->d_xb, op->d_yb};
@@ -919,6 +944,8 @@ void smh_op_free(smh_op* op) {
   for (auto& b : op->bin) free_bin(b);
   for (auto& b : op->bbin) free_bin(b);
   for (auto& row : op->vbin)
+    for (auto& b : row) free_bin(b);
+  for (auto& row : op->vbbin)
     for (auto& b : row) free_bin(b);
   for (int t = 0; t < 2; ++t) {
     void* vp[] = {op->d_vtab[t], op->d_vtt[t], op->d_vtt64[t], op->d_vrowsum[t]};
@@ -956,11 +983,16 @@ static int bin_pairs(smh_op* op, smh_op::Binning& bn, int Rc, bool batch) {
   hipMemsetAsync(d_cnt, 0, (nkeys + 1) * sizeof(int), s);
   hipMemsetAsync(d_cur, 0, (nkeys + 1) * sizeof(int), s);
   hipMemsetAsync(d_err, 0, sizeof(int), s);
-  const int nb2 = (int)((n_part + 255) / 256);
-  const int64_t nblocks = n_part * nb2;
-  hipLaunchKernelGGL(pair_bin_kernel, dim3((unsigned)nblocks), dim3(256), 0, s, op->d_xy, (int)n_part, op->rho_min,
-                     op->h, (int)len_rho, op->s_mode, op->o_mode, op->T, op->n_tt, Rc, d_cnt, (const int*)nullptr,
-                     d_cur, (uint2*)nullptr, d_err);
+  int64_t max_seg = 0;
+  for (int sg = 0; sg < op->n_seg; ++sg) max_seg = std::max<int64_t>(max_seg, op->seg_off[sg + 1] - op->seg_off[sg]);
+  const dim3 grid((unsigned)n_part, (unsigned)((max_seg + 255) / 256));
+  if (grid.y > 65535) {
+    cleanup();
+    return smh_fail(ctx, SMH_ERR_ARG, "ensemble too large for the pair binning");
+  }
+  hipLaunchKernelGGL(pair_bin_kernel, grid, dim3(256), 0, s, op->d_xy, (int)n_part, op->rho_min, op->h, (int)len_rho,
+                     op->s_mode, op->o_mode, op->T, op->n_tt, Rc, (const int*)op->d_seg_of,
+                     (const int64_t*)op->d_seg_off, d_cnt, (const int*)nullptr, d_cur, (uint2*)nullptr, d_err);
   std::vector<int> cnt(nkeys);
   int herr = 0;
   if (hipMemcpyAsync(cnt.data(), d_cnt, nkeys * sizeof(int), hipMemcpyDeviceToHost, s) != hipSuccess ||
@@ -995,9 +1027,9 @@ static int bin_pairs(smh_op* op, smh_op::Binning& bn, int Rc, bool batch) {
     smh_fail(ctx, SMH_ERR_HIP, "device allocation failed (pairs)");
     return SMH_ERR_HIP;
   }
-  hipLaunchKernelGGL(pair_bin_kernel, dim3((unsigned)nblocks), dim3(256), 0, s, op->d_xy, (int)n_part, op->rho_min,
-                     op->h, (int)len_rho, op->s_mode, op->o_mode, op->T, op->n_tt, Rc, d_cnt, (const int*)d_off,
-                     d_cur, d_pairs, d_err);
+  hipLaunchKernelGGL(pair_bin_kernel, grid, dim3(256), 0, s, op->d_xy, (int)n_part, op->rho_min, op->h, (int)len_rho,
+                     op->s_mode, op->o_mode, op->T, op->n_tt, Rc, (const int*)op->d_seg_of,
+                     (const int64_t*)op->d_seg_off, d_cnt, (const int*)d_off, d_cur, d_pairs, d_err);
   if (hipMalloc(&bn.d_rec, std::max<int64_t>(acc, 1) * sizeof(PairRec<Real>)) != hipSuccess) {
     cleanup();
     smh_fail(ctx, SMH_ERR_HIP, "device allocation failed (pair records, %lld pairs)", (long long)acc);
@@ -1017,9 +1049,11 @@ static int bin_pairs(smh_op* op, smh_op::Binning& bn, int Rc, bool batch) {
       op, bn, cnt, off, len_rho, [&](int64_t r) { return (int)(op->s_mode * r + op->o_mode); }, batch);
 }
 
-extern "C" int smh_radial_create(smh_ctx* ctx, int64_t n_part, const double* xy, const int* lmax, const int* mmax,
-                                 int l_max, int m_max, double rho_min, double resolution, int64_t len_rho,
-                                 int interp, smh_op** out) {
+// Operator over n_ens ensembles (particles [ens_off[e], ens_off[e+1])), or one
+// ensemble when ens_off is null.
+static int radial_create(smh_ctx* ctx, int n_ens, const int64_t* ens_off, int64_t n_part, const double* xy,
+                         const int* lmax, const int* mmax, int l_max, int m_max, double rho_min, double resolution,
+                         int64_t len_rho, int interp, smh_op** out) {
   if (!ctx || !out) return SMH_ERR_ARG;
   *out = nullptr;
   SMH_CHECK(ctx, n_part >= 1 && xy && lmax && mmax, SMH_ERR_ARG, "need at least one particle");
@@ -1067,6 +1101,8 @@ extern "C" int smh_radial_create(smh_ctx* ctx, int64_t n_part, const double* xy,
     return code;
   };
   if (int code = op_index_map(op, lmax, mmax)) return fail(code);
+  if (ens_off)
+    if (int code = op_set_segments(op, n_ens, ens_off)) return fail(code);
   if (hipMalloc(&op->d_xy, n_part * sizeof(double2)) != hipSuccess) {
     smh_fail(ctx, SMH_ERR_HIP, "device allocation failed");
     return fail(SMH_ERR_HIP);
@@ -1134,6 +1170,22 @@ extern "C" int smh_radial_create(smh_ctx* ctx, int64_t n_part, const double* xy,
   ctx->live_ops++;
   *out = op;
   return SMH_OK;
+}
+
+extern "C" int smh_radial_create(smh_ctx* ctx, int64_t n_part, const double* xy, const int* lmax, const int* mmax,
+                                 int l_max, int m_max, double rho_min, double resolution, int64_t len_rho,
+                                 int interp, smh_op** out) {
+  return radial_create(ctx, 1, nullptr, n_part, xy, lmax, mmax, l_max, m_max, rho_min, resolution, len_rho, interp,
+                       out);
+}
+
+extern "C" int smh_radial_create_ensembles(smh_ctx* ctx, int n_ens, const int64_t* ens_off, const double* xy,
+                                           const int* lmax, const int* mmax, int l_max, int m_max, double rho_min,
+                                           double resolution, int64_t len_rho, int interp, smh_op** out) {
+  if (!ctx || !out) return SMH_ERR_ARG;
+  SMH_CHECK(ctx, n_ens >= 1 && ens_off, SMH_ERR_ARG, "need at least one ensemble");
+  return radial_create(ctx, n_ens, ens_off, ens_off[n_ens], xy, lmax, mmax, l_max, m_max, rho_min, resolution, len_rho,
+                       interp, out);
 }
 
 extern "C" int smh_op_destroy(smh_op* op) {

@@ -3,8 +3,22 @@
 // Vectors live on the device in the padded layout [n_part][B] (complex FP64);
 // slots a particle does not have carry T = 0 and stay zero. Arnoldi uses
 // classical Gram-Schmidt applied twice (CGS2) in FP64 with fused multi-dot
-// and multi-axpy kernels; the (restart+1) x restart Hessenberg least-squares
-// problem is solved on the host with Givens rotations.
+// and multi-axpy kernels.
+//
+// Two drivers share the stopping rules below:
+// * one right-hand side at a time (operators without the kBatch-RHS kernels,
+//   e.g. the explicit matrix, and single right-hand sides): the small
+//   (restart+1) x restart least-squares problem is solved on the host;
+// * kBatch right-hand sides at a time over every ensemble ("segment") of the
+//   operator: every (segment, right-hand side) pair is a system of its own
+//   (its own Arnoldi scalars, Givens rotations and convergence), all systems
+//   share each coupling matvec launch, and the Givens rotations, the
+//   back-substitution and the restart bookkeeping run on the device. One
+//   Arnoldi step (matvec, CGS2, norm, rotation, normalisation) is a fixed
+//   launch sequence that is captured once per (step, precision,
+//   preconditioner) in a hipGraph and replayed; the host reads one counter
+//   of still-active systems per step. With a right block preconditioner P
+//   (op->d_P) the steps apply M P and the update is x += P V y.
 //
 // SMH_SOLVE_FP64: restarted GMRES with FP64 coupling matvecs, x0 = rhs and
 //   stopping rule ||rhs - M x|| <= tol ||rhs|| (smuthi's scipy call,
@@ -26,6 +40,8 @@
 #include <chrono>
 #include <cmath>
 #include <complex>
+#include <cstdlib>
+#include <map>
 
 #include "smh_op.h"
 
@@ -44,6 +60,9 @@ constexpr double kInnerRel16 = 1e-3;  // the same for FP16-operand matvecs
 // 30 times the FP32 matvec's error)
 constexpr double kFp32ResidualFloor = 1e-5;
 
+// ---------------------------------------------------------------------------
+// single right-hand side
+// ---------------------------------------------------------------------------
 
 // partial[b][j] = sum over this block's slice of conj(V_j) . w
 __global__ void multi_dot_kernel(const double2* __restrict__ V, int64_t ld, int nv, const double2* __restrict__ w,
@@ -148,48 +167,379 @@ __global__ void add_kernel(double2* a, const double2* __restrict__ b, int64_t n)
   if (i < n) a[i] = make_double2(a[i].x + b[i].x, a[i].y + b[i].y);
 }
 
+__global__ void zero2_kernel(double2* p, int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    p[i] = make_double2(0.0, 0.0);
+}
+__global__ void zero_int_kernel(int* p, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0;
+}
+__global__ void zero_f2_kernel(float2* p, int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    p[i] = make_float2(0.f, 0.f);
+}
+
+// ---------------------------------------------------------------------------
+// kBatch right-hand sides x segments: element e = (particle * B + n) * kB + r
+// of a padded vector; system sys = segment(particle) * kB + r
+// ---------------------------------------------------------------------------
+constexpr int kB = smh::kBatchRhs;
+
+// partial[((s * nblk + b) * kMaxBasis + j) * kB + r]: block b's share of
+// conj(V_j) . w over segment s and right-hand side r (grid: nblk x n_seg; a
+// thread sees one r, since segments start at multiples of kB elements and
+// the block stride is a multiple of kB)
+__global__ void __launch_bounds__(kDotThreads) sdot_kernel(const double2* __restrict__ V, int64_t ld, int nv,
+                                                           const double2* __restrict__ w, int64_t per_part,
+                                                           const int64_t* __restrict__ seg_off, double2* partial) {
+  __shared__ double2 red[kDotThreads];
+  const int s = blockIdx.y, b = blockIdx.x, nblk = gridDim.x;
+  const int64_t e0 = seg_off[s] * per_part, e1 = seg_off[s + 1] * per_part;
+  for (int j = 0; j < nv; ++j) {
+    const double2* v = V + (size_t)j * ld;
+    double re = 0, im = 0;
+    for (int64_t e = e0 + (int64_t)b * blockDim.x + threadIdx.x; e < e1; e += (int64_t)nblk * blockDim.x) {
+      const double2 a = v[e], c = w[e];
+      re += a.x * c.x + a.y * c.y;
+      im += a.x * c.y - a.y * c.x;
+    }
+    red[threadIdx.x] = make_double2(re, im);
+    __syncthreads();
+    for (int st = blockDim.x / 2; st >= kB; st >>= 1) {  // strides keep e % kB
+      if ((int)threadIdx.x < st) {
+        red[threadIdx.x].x += red[threadIdx.x + st].x;
+        red[threadIdx.x].y += red[threadIdx.x + st].y;
+      }
+      __syncthreads();
+    }
+    if ((int)threadIdx.x < kB)
+      partial[(((size_t)s * nblk + b) * kMaxBasis + j) * kB + threadIdx.x] = red[threadIdx.x];
+    __syncthreads();
+  }
+}
+
+// out[(s * kB + r) * ldo + j] = sum over the nblk partials, fixed-order tree;
+// one workgroup per (j, r) and segment
+__global__ void __launch_bounds__(256) sreduce_kernel(const double2* __restrict__ partial, int nblk, double2* out,
+                                                      int ldo) {
+  const int j = blockIdx.x / kB, r = blockIdx.x % kB, s = blockIdx.y;
+  __shared__ double2 red[256];
+  double re = 0, im = 0;
+  for (int b = threadIdx.x; b < nblk; b += blockDim.x) {
+    const double2 p = partial[(((size_t)s * nblk + b) * kMaxBasis + j) * kB + r];
+    re += p.x;
+    im += p.y;
+  }
+  red[threadIdx.x] = make_double2(re, im);
+  __syncthreads();
+  for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) {
+      red[threadIdx.x].x += red[threadIdx.x + w].x;
+      red[threadIdx.x].y += red[threadIdx.x + w].y;
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[((size_t)s * kB + r) * ldo + j] = red[0];
+}
+
+__device__ inline int sys_of(int64_t e, int64_t per_part, const int* __restrict__ seg_of) {
+  const int r = (int)(e % kB);
+  return (seg_of ? seg_of[e / per_part] : 0) * kB + r;
+}
+
+// w[e] += sgn * sum_j V_j[e] c[sys(e) * ldc + j]
+__global__ void saxpy_kernel(const double2* __restrict__ V, int64_t ld, int nv, const double2* __restrict__ c,
+                             int ldc, double sgn, double2* w, int64_t n8, int64_t per_part,
+                             const int* __restrict__ seg_of) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n8) return;
+  const double2* cs = c + (size_t)sys_of(e, per_part, seg_of) * ldc;
+  double2 acc = w[e];
+  for (int j = 0; j < nv; ++j) {
+    const double2 v = V[(size_t)j * ld + e], cj = cs[j];
+    acc.x += sgn * (v.x * cj.x - v.y * cj.y);
+    acc.y += sgn * (v.x * cj.y + v.y * cj.x);
+  }
+  w[e] = acc;
+}
+
+__global__ void sscale_kernel(const double2* __restrict__ src, const double* __restrict__ sc, double2* dst,
+                              int64_t n8, int64_t per_part, const int* __restrict__ seg_of) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n8) return;
+  const double f = sc[sys_of(e, per_part, seg_of)];
+  dst[e] = make_double2(src[e].x * f, src[e].y * f);
+}
+
+// out = v - t o y with t per unknown, shared by the right-hand sides
+__global__ void bmaster_fp64_kernel(const double2* __restrict__ v, const double2* __restrict__ t,
+                                    const double2* __restrict__ y, double2* out, int64_t n8) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n8) return;
+  const double2 a = t[e / kB], b = y[e];
+  out[e] = make_double2(v[e].x - (a.x * b.x - a.y * b.y), v[e].y - (a.x * b.y + a.y * b.x));
+}
+
+// FP64 padded [n][kB] -> FP32 input of the batched kernel (b32_index layout)
+__global__ void to_fp32_b32_kernel(const double2* __restrict__ v, float2* out, int64_t n8, int B, int S,
+                                   const int* __restrict__ slot) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n8) return;
+  const int r = (int)(e % kB);
+  const int64_t ip = e / kB, p = ip / B;
+  out[smh::b32_index(p, (int)(ip - p * B), r, S, slot)] = make_float2((float)v[e].x, (float)v[e].y);
+}
+
+__global__ void bmaster_fp32_kernel(const double2* __restrict__ v, const double2* __restrict__ t,
+                                    const float2* __restrict__ y, double2* out, int64_t n8) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n8) return;
+  const double2 a = t[e / kB];
+  const double bx = y[e].x, by = y[e].y;
+  out[e] = make_double2(v[e].x - (a.x * bx - a.y * by), v[e].y - (a.x * by + a.y * bx));
+}
+
+__device__ inline double2 cmul2(double2 a, double2 b) { return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x); }
+__device__ inline double2 cconj2(double2 a) { return make_double2(a.x, -a.y); }
+__device__ inline double2 cdiv2(double2 a, double2 b) {
+  const double d = b.x * b.x + b.y * b.y;
+  return make_double2((a.x * b.x + a.y * b.y) / d, (a.y * b.x - a.x * b.y) / d);
+}
+
+// Start of a restart cycle (one thread per system): beta = ||r|| from the norm
+// dot, the system runs this cycle if active and beta > atol; nlive[restart]
+// counts the systems that run.
+__global__ void cycle_init_kernel(const double2* __restrict__ nrm, const double* __restrict__ atol,
+                                  const int* __restrict__ active, int nsys, int restart, double2* g, int* live,
+                                  double* scale, double* res, int* kr, int* nlive) {
+  const int sys = blockIdx.x * blockDim.x + threadIdx.x;
+  if (sys >= nsys) return;
+  const double beta = sqrt(fmax(0.0, nrm[sys].x));
+  res[sys] = beta;
+  const bool c = active[sys] && beta > atol[sys] && beta > 0.0;
+  live[sys] = c ? 1 : 0;
+  kr[sys] = 0;
+  scale[sys] = c ? 1.0 / beta : 0.0;
+  double2* gs = g + (size_t)sys * (restart + 1);
+  for (int j = 0; j <= restart; ++j) gs[j] = make_double2(0.0, 0.0);
+  gs[0] = make_double2(c ? beta : 0.0, 0.0);
+  if (c) atomicAdd(&nlive[restart], 1);
+}
+
+// Arnoldi step k of every live system (one thread per system): the new
+// Hessenberg column h1 + h2 (the two CGS passes) with h_{k+1,k} = ||w||, the
+// previous Givens rotations applied to it, a new rotation zeroing h_{k+1,k},
+// the rotated right-hand side g and the normalisation of w. A system stops
+// when its least-squares residual |g_{k+1}| reaches atol; nlive[k] counts the
+// systems still running after this step.
+__global__ void givens_kernel(int k, const double2* __restrict__ h1, const double2* __restrict__ h2, int ldh,
+                              const double2* __restrict__ hn2, const double* __restrict__ atol, int nsys, int restart,
+                              int* live, double2* H, double2* cs, double2* sn, double2* g, int* kr, double* scale,
+                              int* nlive) {
+  const int sys = blockIdx.x * blockDim.x + threadIdx.x;
+  if (sys >= nsys) return;
+  if (!live[sys]) {
+    scale[sys] = 0.0;
+    return;
+  }
+  const double hn = sqrt(fmax(0.0, hn2[sys].x));
+  double2* Hs = H + (size_t)sys * (restart + 1) * restart;  // H[j][k] at Hs[j * restart + k]
+  double2* css = cs + (size_t)sys * restart;
+  double2* sns = sn + (size_t)sys * restart;
+  double2* gs = g + (size_t)sys * (restart + 1);
+  for (int j = 0; j <= k; ++j) {
+    const double2 a = h1[(size_t)sys * ldh + j], b = h2[(size_t)sys * ldh + j];
+    Hs[(size_t)j * restart + k] = make_double2(a.x + b.x, a.y + b.y);
+  }
+  for (int j = 0; j < k; ++j) {
+    const double2 a = Hs[(size_t)j * restart + k], b = Hs[(size_t)(j + 1) * restart + k];
+    const double2 c = css[j], s = sns[j];
+    const double2 u = cmul2(cconj2(c), a), v = cmul2(cconj2(s), b);
+    const double2 p = cmul2(s, a), q = cmul2(c, b);
+    Hs[(size_t)j * restart + k] = make_double2(u.x + v.x, u.y + v.y);
+    Hs[(size_t)(j + 1) * restart + k] = make_double2(q.x - p.x, q.y - p.y);
+  }
+  const double2 a = Hs[(size_t)k * restart + k];
+  const double den = sqrt(a.x * a.x + a.y * a.y + hn * hn);
+  const double2 c = den > 0 ? make_double2(a.x / den, a.y / den) : make_double2(1.0, 0.0);
+  const double2 s = den > 0 ? make_double2(hn / den, 0.0) : make_double2(0.0, 0.0);
+  const double2 u = cmul2(cconj2(c), a);
+  Hs[(size_t)k * restart + k] = make_double2(u.x + s.x * hn, u.y - s.y * hn);
+  const double2 gk = gs[k];
+  const double2 gn = cmul2(s, gk);
+  gs[k + 1] = make_double2(-gn.x, -gn.y);
+  gs[k] = cmul2(cconj2(c), gk);
+  css[k] = c;
+  sns[k] = s;
+  kr[sys] = k + 1;
+  scale[sys] = hn > 0 ? 1.0 / hn : 0.0;
+  const double gr = sqrt(gs[k + 1].x * gs[k + 1].x + gs[k + 1].y * gs[k + 1].y);
+  if (gr <= atol[sys] || hn == 0.0)
+    live[sys] = 0;
+  else
+    atomicAdd(&nlive[k], 1);
+}
+
+// End of a cycle (one thread per system): y = R^{-1} g over the system's
+// kr steps (zero beyond), and its least-squares residual estimate |g_kr|.
+__global__ void backsub_kernel(int nsys, int restart, const double2* __restrict__ H, const double2* __restrict__ g,
+                               const int* __restrict__ kr, double2* yv, int ldy, double* res) {
+  const int sys = blockIdx.x * blockDim.x + threadIdx.x;
+  if (sys >= nsys) return;
+  const int m = kr[sys];
+  const double2* Hs = H + (size_t)sys * (restart + 1) * restart;
+  const double2* gs = g + (size_t)sys * (restart + 1);
+  double2* y = yv + (size_t)sys * ldy;
+  for (int j = 0; j < ldy; ++j) y[j] = make_double2(0.0, 0.0);
+  for (int i = m - 1; i >= 0; --i) {
+    double2 sum = gs[i];
+    for (int j = i + 1; j < m; ++j) {
+      const double2 p = cmul2(Hs[(size_t)i * restart + j], y[j]);
+      sum.x -= p.x;
+      sum.y -= p.y;
+    }
+    y[i] = cdiv2(sum, Hs[(size_t)i * restart + i]);
+  }
+  if (m > 0) res[sys] = sqrt(gs[m].x * gs[m].x + gs[m].y * gs[m].y);
+}
+
+// y[p] = P_p x[p] (+ y[p]) for kB padded vectors: one workgroup per particle,
+// thread (n1, r)
+__global__ void precond_kernel(const double2* __restrict__ P, const double2* __restrict__ x, double2* y, int B,
+                               int accumulate) {
+  extern __shared__ double2 xs[];  // [B][kB]
+  const int64_t p = blockIdx.x;
+  const double2* xp = x + (size_t)p * B * kB;
+  for (int j = threadIdx.x; j < B * kB; j += blockDim.x) xs[j] = xp[j];
+  __syncthreads();
+  const double2* Pp = P + (size_t)p * B * B;
+  for (int t = threadIdx.x; t < B * kB; t += blockDim.x) {
+    const int n1 = t / kB, r = t - n1 * kB;
+    double re = 0, im = 0;
+    for (int n2 = 0; n2 < B; ++n2) {
+      const double2 a = Pp[(size_t)n1 * B + n2], b = xs[n2 * kB + r];
+      re += a.x * b.x - a.y * b.y;
+      im += a.x * b.y + a.y * b.x;
+    }
+    double2* out = y + (size_t)p * B * kB + t;
+    if (accumulate) {
+      re += out->x;
+      im += out->y;
+    }
+    *out = make_double2(re, im);
+  }
+}
+
+// ---- per-operator workspace ------------------------------------------------
+struct Ws {
+  // single right-hand side
+  int64_t n1 = 0;
+  int restart1 = 0;
+  double2 *V1 = nullptr, *w1 = nullptr, *t1 = nullptr, *partial1 = nullptr, *coef1 = nullptr, *y1 = nullptr;
+  double2 *b1 = nullptr, *x1 = nullptr, *d1 = nullptr, *r1 = nullptr;
+  float2* x32_1 = nullptr;
+  // batched systems
+  int64_t n8 = 0;
+  int restart = 0, nsys = 0, nblk = 0;
+  double2 *V = nullptr, *w = nullptr, *t = nullptr, *partial = nullptr, *h1 = nullptr, *h2 = nullptr,
+          *hn = nullptr, *H = nullptr, *cs = nullptr, *sn = nullptr, *g = nullptr, *yv = nullptr, *y64 = nullptr,
+          *b = nullptr, *x = nullptr, *d = nullptr, *r = nullptr, *pv = nullptr, *tmp = nullptr;
+  double *atol = nullptr, *scale = nullptr, *res = nullptr;
+  int *live = nullptr, *kr = nullptr, *nlive = nullptr, *active = nullptr;
+  float2 *x32 = nullptr, *y32 = nullptr;
+  void* x16 = nullptr;
+  int* h_pin = nullptr;  // pinned host word: the live-system count of the last step
+  std::map<int, hipGraphExec_t> graphs;
+  bool warmed[3][2] = {};
+
+  void free_single() {
+    void* p[] = {V1, w1, t1, partial1, coef1, y1, b1, x1, d1, r1, x32_1};
+    for (void* q : p)
+      if (q) (void)hipFree(q);
+    V1 = w1 = t1 = partial1 = coef1 = y1 = b1 = x1 = d1 = r1 = nullptr;
+    x32_1 = nullptr;
+    n1 = 0;
+    restart1 = 0;
+  }
+  void drop_graphs() {
+    for (auto& kv : graphs) (void)hipGraphExecDestroy(kv.second);
+    graphs.clear();
+    for (auto& w3 : warmed) w3[0] = w3[1] = false;
+  }
+  void free_batch() {
+    drop_graphs();
+    void* p[] = {V, w, t, partial, h1, h2, hn, H, cs, sn, g, yv, y64, b, x, d, r, pv, tmp,
+                 atol, scale, res, live, kr, nlive, active, x32, y32, x16};
+    for (void* q : p)
+      if (q) (void)hipFree(q);
+    V = w = t = partial = h1 = h2 = hn = H = cs = sn = g = yv = y64 = b = x = d = r = pv = tmp = nullptr;
+    atol = scale = res = nullptr;
+    live = kr = nlive = active = nullptr;
+    x32 = y32 = nullptr;
+    x16 = nullptr;
+    n8 = 0;
+    restart = nsys = nblk = 0;
+  }
+  ~Ws() {
+    free_single();
+    free_batch();
+    if (h_pin) (void)hipHostFree(h_pin);
+  }
+};
+
+Ws* ws_of(smh_op* op) {
+  if (!op->solver_ws) op->solver_ws = new Ws();
+  return static_cast<Ws*>(op->solver_ws);
+}
+
+// SMH_GRAPHS=0 in the environment runs the Arnoldi steps as plain launches
+bool graphs_enabled() {
+  const char* e = getenv("SMH_GRAPHS");
+  return !(e && e[0] == '0');
+}
+
+// Single right-hand side GMRES (host least squares).
 struct Solver {
   smh_op* op;
   smh_ctx* ctx;
   hipStream_t s;
+  Ws* ws;
   int64_t n;  // padded length
   int restart;
-  double2* V = nullptr;        // (restart + 1) basis vectors
-  double2* w = nullptr;
-  double2* t = nullptr;        // T diagonal (padded)
-  double2* partial = nullptr;  // [kDotBlocks][kMaxBasis]
-  double2* coef = nullptr;     // kMaxBasis
-  double2* y64 = nullptr;      // coupling output FP64
-  float2* x32 = nullptr;       // FP32 staging of the matvec input
   int64_t mv64 = 0, mv32 = 0, iters = 0;
 
   int init() {
-    SMH_HIP(ctx, hipMalloc(&V, (size_t)(restart + 1) * n * sizeof(double2)));
-    SMH_HIP(ctx, hipMalloc(&w, n * sizeof(double2)));
-    SMH_HIP(ctx, hipMalloc(&t, n * sizeof(double2)));
-    SMH_HIP(ctx, hipMalloc(&partial, (size_t)kDotBlocks * kMaxBasis * sizeof(double2)));
-    SMH_HIP(ctx, hipMalloc(&coef, kMaxBasis * sizeof(double2)));
-    SMH_HIP(ctx, hipMalloc(&y64, n * sizeof(double2)));
-    SMH_HIP(ctx, hipMalloc(&x32, n * sizeof(float2)));
+    if (ws->n1 == n && ws->restart1 >= restart) return SMH_OK;
+    ws->free_single();
+    auto alloc = [&](void** p, size_t bytes) { return hipMalloc(p, bytes) == hipSuccess; };
+    const bool ok = alloc((void**)&ws->V1, (size_t)(restart + 1) * n * sizeof(double2)) &&
+                    alloc((void**)&ws->w1, n * sizeof(double2)) && alloc((void**)&ws->t1, n * sizeof(double2)) &&
+                    alloc((void**)&ws->partial1, (size_t)kDotBlocks * kMaxBasis * sizeof(double2)) &&
+                    alloc((void**)&ws->coef1, kMaxBasis * sizeof(double2)) &&
+                    alloc((void**)&ws->y1, n * sizeof(double2)) && alloc((void**)&ws->b1, n * sizeof(double2)) &&
+                    alloc((void**)&ws->x1, n * sizeof(double2)) && alloc((void**)&ws->d1, n * sizeof(double2)) &&
+                    alloc((void**)&ws->r1, n * sizeof(double2)) && alloc((void**)&ws->x32_1, n * sizeof(float2));
+    if (!ok) {
+      ws->free_single();
+      return smh_fail(ctx, SMH_ERR_HIP, "solver allocation failed (%lld unknowns, restart %d)", (long long)n, restart);
+    }
+    ws->n1 = n;
+    ws->restart1 = restart;
     return SMH_OK;
-  }
-  void release() {
-    void* p[] = {V, w, t, partial, coef, y64, x32};
-    for (void* q : p)
-      if (q) hipFree(q);
   }
 
   // out = M v (padded), precision of the coupling matvec as requested
   int master(const double2* v, double2* out, int precision) {
     int rc;
     if (precision == SMH_FP64) {
-      if ((rc = smh::op_apply(op, SMH_FP64, v, y64, false))) return rc;
-      hipLaunchKernelGGL(master_fp64_kernel, dim3(blocks_for(n)), dim3(256), 0, s, v, t, y64, out, n);
+      if ((rc = smh::op_apply(op, SMH_FP64, v, ws->y1, false))) return rc;
+      hipLaunchKernelGGL(master_fp64_kernel, dim3(blocks_for(n)), dim3(256), 0, s, v, ws->t1, ws->y1, out, n);
       ++mv64;
     } else {
-      hipLaunchKernelGGL(to_fp32_kernel, dim3(blocks_for(n)), dim3(256), 0, s, v, x32, n);
-      if ((rc = smh::op_apply(op, SMH_FP32, x32, op->d_y32, false))) return rc;
-      hipLaunchKernelGGL(master_fp32_kernel, dim3(blocks_for(n)), dim3(256), 0, s, v, t, op->d_y32, out, n);
+      hipLaunchKernelGGL(to_fp32_kernel, dim3(blocks_for(n)), dim3(256), 0, s, v, ws->x32_1, n);
+      if ((rc = smh::op_apply(op, SMH_FP32, ws->x32_1, op->d_y32, false))) return rc;
+      hipLaunchKernelGGL(master_fp32_kernel, dim3(blocks_for(n)), dim3(256), 0, s, v, ws->t1, op->d_y32, out, n);
       ++mv32;
     }
     SMH_HIP(ctx, hipGetLastError());
@@ -199,9 +549,10 @@ struct Solver {
   // h[0..nv) = V[0..nv)^H w (device -> host)
   int dots(const double2* Vb, int nv, const double2* wv, zd* h) {
     if (nv < 1 || nv > kMaxBasis) return smh_fail(ctx, SMH_ERR_ARG, "basis size out of range");
-    hipLaunchKernelGGL(multi_dot_kernel, dim3(kDotBlocks), dim3(kDotThreads), 0, s, Vb, n, nv, wv, n, partial);
-    hipLaunchKernelGGL(reduce_partials_kernel, dim3(1), dim3(kMaxBasis), 0, s, partial, kDotBlocks, nv, coef);
-    SMH_HIP(ctx, hipMemcpyAsync(h, coef, nv * sizeof(double2), hipMemcpyDeviceToHost, s));
+    hipLaunchKernelGGL(multi_dot_kernel, dim3(kDotBlocks), dim3(kDotThreads), 0, s, Vb, n, nv, wv, n, ws->partial1);
+    hipLaunchKernelGGL(reduce_partials_kernel, dim3(1), dim3(kMaxBasis), 0, s, ws->partial1, kDotBlocks, nv,
+                       ws->coef1);
+    SMH_HIP(ctx, hipMemcpyAsync(h, ws->coef1, nv * sizeof(double2), hipMemcpyDeviceToHost, s));
     SMH_HIP(ctx, hipStreamSynchronize(s));
     return SMH_OK;
   }
@@ -225,6 +576,8 @@ struct Solver {
             bool x_zero = false, bool trust_estimate = false) {
     std::vector<zd> H((size_t)(restart + 1) * restart), cs(restart), sn(restart), g(restart + 1), hcol(restart + 1),
         hcol2(restart + 1), yv(restart);
+    double2* V = ws->V1;
+    double2* w = ws->w1;
     int rc;
     steps = 0;
     double beta = 0;
@@ -248,11 +601,11 @@ struct Solver {
         if ((rc = master(vk, w, precision))) return rc;
         // CGS2 against V_0..V_k
         if ((rc = dots(V, k + 1, w, hcol.data()))) return rc;
-        SMH_HIP(ctx, hipMemcpyAsync(coef, hcol.data(), (k + 1) * sizeof(double2), hipMemcpyHostToDevice, s));
-        hipLaunchKernelGGL(multi_axpy_kernel, dim3(blocks_for(n)), dim3(256), 0, s, V, n, k + 1, coef, w, n);
+        SMH_HIP(ctx, hipMemcpyAsync(ws->coef1, hcol.data(), (k + 1) * sizeof(double2), hipMemcpyHostToDevice, s));
+        hipLaunchKernelGGL(multi_axpy_kernel, dim3(blocks_for(n)), dim3(256), 0, s, V, n, k + 1, ws->coef1, w, n);
         if ((rc = dots(V, k + 1, w, hcol2.data()))) return rc;
-        SMH_HIP(ctx, hipMemcpyAsync(coef, hcol2.data(), (k + 1) * sizeof(double2), hipMemcpyHostToDevice, s));
-        hipLaunchKernelGGL(multi_axpy_kernel, dim3(blocks_for(n)), dim3(256), 0, s, V, n, k + 1, coef, w, n);
+        SMH_HIP(ctx, hipMemcpyAsync(ws->coef1, hcol2.data(), (k + 1) * sizeof(double2), hipMemcpyHostToDevice, s));
+        hipLaunchKernelGGL(multi_axpy_kernel, dim3(blocks_for(n)), dim3(256), 0, s, V, n, k + 1, ws->coef1, w, n);
         for (int j = 0; j <= k; ++j) hcol[j] += hcol2[j];
         double hn;
         if ((rc = norm(w, hn))) return rc;
@@ -289,8 +642,8 @@ struct Solver {
         yv[i] = sum / H[(size_t)i * restart + i];
       }
       if (k > 0) {
-        SMH_HIP(ctx, hipMemcpyAsync(coef, yv.data(), k * sizeof(double2), hipMemcpyHostToDevice, s));
-        hipLaunchKernelGGL(multi_update_kernel, dim3(blocks_for(n)), dim3(256), 0, s, V, n, k, coef, x, n);
+        SMH_HIP(ctx, hipMemcpyAsync(ws->coef1, yv.data(), k * sizeof(double2), hipMemcpyHostToDevice, s));
+        hipLaunchKernelGGL(multi_update_kernel, dim3(blocks_for(n)), dim3(256), 0, s, V, n, k, ws->coef1, x, n);
       }
       if (trust_estimate) {
         res_out = std::abs(g[k]);
@@ -300,307 +653,258 @@ struct Solver {
   }
 };
 
-
-// ---- kBatch right-hand sides interleaved: element e = i * kBatch + r ----
-constexpr int kB = smh::kBatchRhs;
-
-// partial[(b * kMaxBasis + j) * kB + r] = block b's share of conj(V_j[., r]) . w[., r]
-__global__ void bmulti_dot_kernel(const double2* __restrict__ V, int64_t ld, int nv, const double2* __restrict__ w,
-                                  int64_t n8, double2* partial) {
-  __shared__ double2 red[kDotThreads];
-  for (int j = 0; j < nv; ++j) {
-    const double2* v = V + (size_t)j * ld;
-    double re = 0, im = 0;
-    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n8; e += (int64_t)gridDim.x * blockDim.x) {
-      const double2 a = v[e], b = w[e];
-      re += a.x * b.x + a.y * b.y;
-      im += a.x * b.y - a.y * b.x;
-    }
-    red[threadIdx.x] = make_double2(re, im);
-    __syncthreads();
-    for (int s = blockDim.x / 2; s >= kB; s >>= 1) {  // strides keep e % kB
-      if ((int)threadIdx.x < s) {
-        red[threadIdx.x].x += red[threadIdx.x + s].x;
-        red[threadIdx.x].y += red[threadIdx.x + s].y;
-      }
-      __syncthreads();
-    }
-    if ((int)threadIdx.x < kB) partial[((size_t)blockIdx.x * kMaxBasis + j) * kB + threadIdx.x] = red[threadIdx.x];
-    __syncthreads();
-  }
-}
-
-// one workgroup per (basis vector, right-hand side), a fixed-order tree over the partials
-__global__ void __launch_bounds__(256) breduce_partials_kernel(const double2* __restrict__ partial, int nblocks,
-                                                               double2* out) {
-  const int jr = blockIdx.x;
-  __shared__ double2 red[256];
-  double re = 0, im = 0;
-  for (int b = threadIdx.x; b < nblocks; b += blockDim.x) {
-    const double2 p = partial[(size_t)b * kMaxBasis * kB + jr];
-    re += p.x;
-    im += p.y;
-  }
-  red[threadIdx.x] = make_double2(re, im);
-  __syncthreads();
-  for (int w = blockDim.x / 2; w > 0; w >>= 1) {
-    if ((int)threadIdx.x < w) {
-      red[threadIdx.x].x += red[threadIdx.x + w].x;
-      red[threadIdx.x].y += red[threadIdx.x + w].y;
-    }
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) out[jr] = red[0];
-}
-
-// w[e] += sgn * sum_j V_j[e] c[j][e % kB]
-__global__ void bmulti_axpy_kernel(const double2* __restrict__ V, int64_t ld, int nv, const double2* __restrict__ c,
-                                   double sgn, double2* w, int64_t n8) {
-  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= n8) return;
-  const int r = (int)(e % kB);
-  double2 acc = w[e];
-  for (int j = 0; j < nv; ++j) {
-    const double2 v = V[(size_t)j * ld + e], cj = c[j * kB + r];
-    acc.x += sgn * (v.x * cj.x - v.y * cj.y);
-    acc.y += sgn * (v.x * cj.y + v.y * cj.x);
-  }
-  w[e] = acc;
-}
-
-__global__ void bscale_copy_kernel(const double2* __restrict__ src, const double* __restrict__ sc, double2* dst,
-                                   int64_t n8) {
-  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= n8) return;
-  const double f = sc[e % kB];
-  dst[e] = make_double2(src[e].x * f, src[e].y * f);
-}
-
-// out = v - t o y with t per unknown, shared by the right-hand sides
-__global__ void bmaster_fp64_kernel(const double2* __restrict__ v, const double2* __restrict__ t,
-                                    const double2* __restrict__ y, double2* out, int64_t n8) {
-  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= n8) return;
-  const double2 a = t[e / kB], b = y[e];
-  out[e] = make_double2(v[e].x - (a.x * b.x - a.y * b.y), v[e].y - (a.x * b.y + a.y * b.x));
-}
-
-// FP64 padded [n][kB] -> FP32 input of the batched kernel (b32_index layout)
-__global__ void to_fp32_b32_kernel(const double2* __restrict__ v, float2* out, int64_t n8, int B, int S,
-                                   const int* __restrict__ slot) {
-  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= n8) return;
-  const int r = (int)(e % kB);
-  const int64_t ip = e / kB, p = ip / B;
-  out[smh::b32_index(p, (int)(ip - p * B), r, S, slot)] = make_float2((float)v[e].x, (float)v[e].y);
-}
-
-__global__ void bmaster_fp32_kernel(const double2* __restrict__ v, const double2* __restrict__ t,
-                                    const float2* __restrict__ y, double2* out, int64_t n8) {
-  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= n8) return;
-  const double2 a = t[e / kB];
-  const double bx = y[e].x, by = y[e].y;
-  out[e] = make_double2(v[e].x - (a.x * bx - a.y * by), v[e].y - (a.x * by + a.y * bx));
-}
-
-// Restarted GMRES on kB systems at once, sharing every coupling matvec.
+// kBatch right-hand sides x n_seg segments in lockstep, device-side Givens.
 struct BatchSolver {
   smh_op* op;
   smh_ctx* ctx;
   hipStream_t s;
-  int64_t n, n8;  // padded length, times kB
-  int restart;
-  double2* V = nullptr;        // (restart + 1) basis blocks [n][kB]
-  double2* w = nullptr;
-  double2* t = nullptr;        // T diagonal (padded, [n])
-  double2* partial = nullptr;  // [kDotBlocks][kMaxBasis][kB]
-  double2* coef = nullptr;     // [kMaxBasis][kB]
-  double* scl = nullptr;       // [kB]
-  double2* y64 = nullptr;      // coupling output FP64 [n][kB]
-  float2* x32 = nullptr;
-  void* x16 = nullptr;  // FP16 input of the FP16-operand kernel (allocated on first use)
-  float2* y32 = nullptr;
+  Ws* ws;
+  int64_t n, n8, per_part;  // padded length, times kB, elements per particle
+  int restart, nsys, nblk;
+  bool use_P;
   int64_t mv64 = 0, mv32 = 0, mv16 = 0, iters = 0;
 
   int init() {
-    SMH_HIP(ctx, hipMalloc(&V, (size_t)(restart + 1) * n8 * sizeof(double2)));
-    SMH_HIP(ctx, hipMalloc(&w, n8 * sizeof(double2)));
-    SMH_HIP(ctx, hipMalloc(&t, n * sizeof(double2)));
-    SMH_HIP(ctx, hipMalloc(&partial, (size_t)kDotBlocks * kMaxBasis * kB * sizeof(double2)));
-    SMH_HIP(ctx, hipMalloc(&coef, (size_t)kMaxBasis * kB * sizeof(double2)));
-    SMH_HIP(ctx, hipMalloc(&scl, kB * sizeof(double)));
-    SMH_HIP(ctx, hipMalloc(&y64, n8 * sizeof(double2)));
-    // FP32 input in the batched kernel's layout (padding slots stay zero)
+    nsys = op->n_seg * kB;
+    nblk = std::max(8, std::min(kDotBlocks, kDotBlocks / op->n_seg));
+    use_P = op->d_P != nullptr;
+    if (!ws->h_pin) SMH_HIP(ctx, hipHostMalloc((void**)&ws->h_pin, 4 * sizeof(int), hipHostMallocDefault));
+    if (ws->n8 == n8 && ws->restart == restart && ws->nsys == nsys && ws->nblk == nblk) return SMH_OK;
+    ws->free_batch();
     const size_t n32 = (size_t)op->n_part * op->b32_S * kB;
-    SMH_HIP(ctx, hipMalloc(&x32, n32 * sizeof(float2)));
-    SMH_HIP(ctx, hipMemsetAsync(x32, 0, n32 * sizeof(float2), s));
-    SMH_HIP(ctx, hipMalloc(&y32, n8 * sizeof(float2)));
+    const bool half = smh::op_has_batch(op, SMH_FP16);
+    const int ldh = restart + 1;
+    auto alloc = [&](void** p, size_t bytes) { return hipMalloc(p, std::max<size_t>(bytes, 16)) == hipSuccess; };
+    const bool ok =
+        alloc((void**)&ws->V, (size_t)(restart + 1) * n8 * sizeof(double2)) &&
+        alloc((void**)&ws->w, n8 * sizeof(double2)) && alloc((void**)&ws->t, n * sizeof(double2)) &&
+        alloc((void**)&ws->partial, (size_t)op->n_seg * nblk * kMaxBasis * kB * sizeof(double2)) &&
+        alloc((void**)&ws->h1, (size_t)nsys * ldh * sizeof(double2)) &&
+        alloc((void**)&ws->h2, (size_t)nsys * ldh * sizeof(double2)) &&
+        alloc((void**)&ws->hn, (size_t)nsys * sizeof(double2)) &&
+        alloc((void**)&ws->H, (size_t)nsys * (restart + 1) * restart * sizeof(double2)) &&
+        alloc((void**)&ws->cs, (size_t)nsys * restart * sizeof(double2)) &&
+        alloc((void**)&ws->sn, (size_t)nsys * restart * sizeof(double2)) &&
+        alloc((void**)&ws->g, (size_t)nsys * (restart + 1) * sizeof(double2)) &&
+        alloc((void**)&ws->yv, (size_t)nsys * ldh * sizeof(double2)) &&
+        alloc((void**)&ws->y64, n8 * sizeof(double2)) && alloc((void**)&ws->b, n8 * sizeof(double2)) &&
+        alloc((void**)&ws->x, n8 * sizeof(double2)) && alloc((void**)&ws->d, n8 * sizeof(double2)) &&
+        alloc((void**)&ws->r, n8 * sizeof(double2)) && alloc((void**)&ws->pv, n8 * sizeof(double2)) &&
+        alloc((void**)&ws->tmp, n8 * sizeof(double2)) && alloc((void**)&ws->atol, nsys * sizeof(double)) &&
+        alloc((void**)&ws->scale, nsys * sizeof(double)) && alloc((void**)&ws->res, nsys * sizeof(double)) &&
+        alloc((void**)&ws->live, nsys * sizeof(int)) && alloc((void**)&ws->kr, nsys * sizeof(int)) &&
+        alloc((void**)&ws->nlive, (restart + 1) * sizeof(int)) && alloc((void**)&ws->active, nsys * sizeof(int)) &&
+        alloc((void**)&ws->x32, n32 * sizeof(float2)) && alloc((void**)&ws->y32, n8 * sizeof(float2)) &&
+        (!half || alloc(&ws->x16, n32 * 2 * sizeof(_Float16)));
+    if (!ok) {
+      ws->free_batch();
+      return smh_fail(ctx, SMH_ERR_HIP, "batched solver allocation failed (%lld entries, restart %d)", (long long)n8,
+                      restart);
+    }
+    // the FP32 input's padding slots stay zero
+    hipLaunchKernelGGL(zero_f2_kernel, dim3(1024), dim3(256), 0, s, ws->x32, (int64_t)n32);
+    SMH_HIP(ctx, hipGetLastError());
+    ws->n8 = n8;
+    ws->restart = restart;
+    ws->nsys = nsys;
+    ws->nblk = nblk;
     return SMH_OK;
   }
-  void release() {
-    void* p[] = {V, w, t, partial, coef, scl, y64, x32, x16, y32};
-    for (void* q : p)
-      if (q) hipFree(q);
-  }
 
-  int master(const double2* v, double2* out, int precision) {
+  // out = M v, or M P v with the preconditioner (with_P)
+  int master(const double2* v, double2* out, int precision, bool with_P) {
     int rc;
+    const double2* src = v;
+    if (with_P) {
+      if ((rc = smh::op_apply_precond_batch(op, v, ws->pv, false))) return rc;
+      src = ws->pv;
+    }
     if (precision == SMH_FP64) {
-      if ((rc = smh::op_apply_batch(op, SMH_FP64, v, y64, false))) return rc;
-      hipLaunchKernelGGL(bmaster_fp64_kernel, dim3(blocks_for(n8)), dim3(256), 0, s, v, t, y64, out, n8);
+      if ((rc = smh::op_apply_batch(op, SMH_FP64, src, ws->y64, false))) return rc;
+      hipLaunchKernelGGL(bmaster_fp64_kernel, dim3(blocks_for(n8)), dim3(256), 0, s, src, ws->t, ws->y64, out, n8);
       ++mv64;
     } else if (precision == SMH_FP16) {
-      if (!x16) SMH_HIP(ctx, hipMalloc(&x16, (size_t)op->n_part * op->b32_S * kB * 2 * sizeof(_Float16)));
-      if ((rc = smh::op_to_half_batch(op, v, x16))) return rc;
-      if ((rc = smh::op_apply_batch(op, SMH_FP16, x16, y32, false))) return rc;
-      hipLaunchKernelGGL(bmaster_fp32_kernel, dim3(blocks_for(n8)), dim3(256), 0, s, v, t, y32, out, n8);
+      if ((rc = smh::op_to_half_batch(op, src, ws->x16))) return rc;
+      if ((rc = smh::op_apply_batch(op, SMH_FP16, ws->x16, ws->y32, false))) return rc;
+      hipLaunchKernelGGL(bmaster_fp32_kernel, dim3(blocks_for(n8)), dim3(256), 0, s, src, ws->t, ws->y32, out, n8);
       ++mv16;
     } else {
-      hipLaunchKernelGGL(to_fp32_b32_kernel, dim3(blocks_for(n8)), dim3(256), 0, s, v, x32, n8, op->B, op->b32_S,
-                         (const int*)op->d_b32_tab);
-      if ((rc = smh::op_apply_batch(op, SMH_FP32, x32, y32, false))) return rc;
-      hipLaunchKernelGGL(bmaster_fp32_kernel, dim3(blocks_for(n8)), dim3(256), 0, s, v, t, y32, out, n8);
+      hipLaunchKernelGGL(to_fp32_b32_kernel, dim3(blocks_for(n8)), dim3(256), 0, s, src, ws->x32, n8, op->B,
+                         op->b32_S, (const int*)op->d_b32_tab);
+      if ((rc = smh::op_apply_batch(op, SMH_FP32, ws->x32, ws->y32, false))) return rc;
+      hipLaunchKernelGGL(bmaster_fp32_kernel, dim3(blocks_for(n8)), dim3(256), 0, s, src, ws->t, ws->y32, out, n8);
       ++mv32;
     }
     SMH_HIP(ctx, hipGetLastError());
     return SMH_OK;
   }
 
-  // h[j][r] = V_j[., r]^H w[., r] for j < nv (device -> host)
-  int dots(const double2* Vb, int nv, const double2* wv, zd* h) {
+  // out[sys * ldo + j] = V_j[., sys]^H w[., sys], j < nv (device, no sync)
+  int dots(const double2* Vb, int nv, const double2* wv, double2* out, int ldo) {
     if (nv < 1 || nv > kMaxBasis) return smh_fail(ctx, SMH_ERR_ARG, "basis size out of range");
-    hipLaunchKernelGGL(bmulti_dot_kernel, dim3(kDotBlocks), dim3(kDotThreads), 0, s, Vb, n8, nv, wv, n8, partial);
-    hipLaunchKernelGGL(breduce_partials_kernel, dim3(nv * kB), dim3(256), 0, s, partial, kDotBlocks, coef);
-    SMH_HIP(ctx, hipMemcpyAsync(h, coef, (size_t)nv * kB * sizeof(double2), hipMemcpyDeviceToHost, s));
-    SMH_HIP(ctx, hipStreamSynchronize(s));
+    hipLaunchKernelGGL(sdot_kernel, dim3(nblk, op->n_seg), dim3(kDotThreads), 0, s, Vb, n8, nv, wv, per_part,
+                       (const int64_t*)op->d_seg_off, ws->partial);
+    hipLaunchKernelGGL(sreduce_kernel, dim3(nv * kB, op->n_seg), dim3(256), 0, s, ws->partial, nblk, out, ldo);
+    SMH_HIP(ctx, hipGetLastError());
     return SMH_OK;
   }
+  // ||v|| of every system, to the host
   int norms(const double2* v, double* out) {
-    zd h[kB];
-    int rc = dots(v, 1, v, h);
-    for (int r = 0; r < kB; ++r) out[r] = std::sqrt(std::max(0.0, h[r].real()));
-    return rc;
-  }
-  int set_scale(const double* f, const double2* src, double2* dst) {
-    SMH_HIP(ctx, hipMemcpyAsync(scl, f, kB * sizeof(double), hipMemcpyHostToDevice, s));
-    hipLaunchKernelGGL(bscale_copy_kernel, dim3(blocks_for(n8)), dim3(256), 0, s, src, scl, dst, n8);
+    int rc = dots(v, 1, v, ws->hn, 1);
+    if (rc) return rc;
+    std::vector<double2> h(nsys);
+    SMH_HIP(ctx, hipMemcpyAsync(h.data(), ws->hn, nsys * sizeof(double2), hipMemcpyDeviceToHost, s));
+    SMH_HIP(ctx, hipStreamSynchronize(s));
+    for (int k = 0; k < nsys; ++k) out[k] = std::sqrt(std::max(0.0, h[k].x));
     return SMH_OK;
   }
 
-  // GMRES on M x = b for the systems r with active[r] (others are left
-  // untouched), each to ||b - M x|| <= atol[r]; x_zero / trust_estimate as
-  // for the single-RHS solver. res_out[r]: residual (or estimate) of x.
-  int gmres(double2* x, const double2* b, const double* atol, const bool* active, int maxiter, int precision,
-            double* res_out, int& steps, bool x_zero, bool trust_estimate) {
-    std::vector<zd> H((size_t)kB * (restart + 1) * restart), cs((size_t)kB * restart), sn((size_t)kB * restart),
-        g((size_t)kB * (restart + 1)), yv((size_t)kMaxBasis * kB), hc((size_t)(restart + 1) * kB),
-        hc2((size_t)(restart + 1) * kB);
+  // Arnoldi step k of every system: w = M (P) V_k, CGS2, ||w||, rotation,
+  // V_{k+1} = w / ||w||. Device only (captured in a graph).
+  int arnoldi_step(int k, int precision) {
     int rc;
+    const int ldh = restart + 1;
+    const double2* vk = ws->V + (size_t)k * n8;
+    if ((rc = master(vk, ws->w, precision, use_P))) return rc;
+    if ((rc = dots(ws->V, k + 1, ws->w, ws->h1, ldh))) return rc;
+    hipLaunchKernelGGL(saxpy_kernel, dim3(blocks_for(n8)), dim3(256), 0, s, ws->V, n8, k + 1, ws->h1, ldh, -1.0, ws->w,
+                       n8, per_part, (const int*)op->d_seg_of);
+    if ((rc = dots(ws->V, k + 1, ws->w, ws->h2, ldh))) return rc;
+    hipLaunchKernelGGL(saxpy_kernel, dim3(blocks_for(n8)), dim3(256), 0, s, ws->V, n8, k + 1, ws->h2, ldh, -1.0, ws->w,
+                       n8, per_part, (const int*)op->d_seg_of);
+    if ((rc = dots(ws->w, 1, ws->w, ws->hn, 1))) return rc;
+    hipLaunchKernelGGL(givens_kernel, dim3(blocks_for(nsys, 64)), dim3(64), 0, s, k, ws->h1, ws->h2, ldh, ws->hn,
+                       ws->atol, nsys, restart, ws->live, ws->H, ws->cs, ws->sn, ws->g, ws->kr, ws->scale, ws->nlive);
+    hipLaunchKernelGGL(sscale_kernel, dim3(blocks_for(n8)), dim3(256), 0, s, ws->w, ws->scale,
+                       ws->V + (size_t)(k + 1) * n8, n8, per_part, (const int*)op->d_seg_of);
+    SMH_HIP(ctx, hipGetLastError());
+    return SMH_OK;
+  }
+
+  // Step k through its graph: the first step of a configuration runs eagerly
+  // (lazy allocations and kernel attributes happen outside any capture),
+  // later ones are captured once and replayed.
+  int run_step(int k, int precision) {
+    const int pi = precision == SMH_FP64 ? 0 : (precision == SMH_FP32 ? 1 : 2);
+    const int ui = use_P ? 1 : 0;
+    if (!graphs_enabled() || !ws->warmed[pi][ui]) {
+      const int rc = arnoldi_step(k, precision);
+      if (!rc) ws->warmed[pi][ui] = true;
+      return rc;
+    }
+    const int key = ((pi * 2 + ui) << 8) | k;
+    auto it = ws->graphs.find(key);
+    if (it == ws->graphs.end()) {
+      SMH_HIP(ctx, hipStreamBeginCapture(s, hipStreamCaptureModeRelaxed));
+      const int rc = arnoldi_step(k, precision);
+      hipGraph_t graph = nullptr;
+      const hipError_t e = hipStreamEndCapture(s, &graph);
+      if (rc || e != hipSuccess) {
+        if (graph) (void)hipGraphDestroy(graph);
+        return rc ? rc : smh_fail(ctx, SMH_ERR_HIP, "graph capture failed: %s", hipGetErrorString(e));
+      }
+      hipGraphExec_t exec = nullptr;
+      const hipError_t ei = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(graph);
+      if (ei != hipSuccess) return smh_fail(ctx, SMH_ERR_HIP, "graph instantiation failed: %s", hipGetErrorString(ei));
+      it = ws->graphs.emplace(key, exec).first;
+    }
+    SMH_HIP(ctx, hipGraphLaunch(it->second, s));
+    return SMH_OK;
+  }
+
+  // GMRES on M x = b for the systems with active[sys] (others left untouched),
+  // each to ||b - M x|| <= atol[sys]; x_zero / trust_estimate as for the
+  // single right-hand side solver. res_out[sys]: residual (or estimate) of x.
+  int gmres(double2* x, const double2* b, const double* atol, const int* active, int maxiter, int precision,
+            double* res_out, int& steps, bool x_zero, bool trust_estimate) {
+    int rc;
+    const int R = restart;
+    SMH_HIP(ctx, hipMemcpyAsync(ws->atol, atol, nsys * sizeof(double), hipMemcpyHostToDevice, s));
+    SMH_HIP(ctx, hipMemcpyAsync(ws->active, active, nsys * sizeof(int), hipMemcpyHostToDevice, s));
     steps = 0;
     for (int outer = 0;; ++outer) {
       if (outer == 0 && x_zero) {
-        SMH_HIP(ctx, hipMemcpyAsync(w, b, n8 * sizeof(double2), hipMemcpyDeviceToDevice, s));
+        SMH_HIP(ctx, hipMemcpyAsync(ws->w, b, n8 * sizeof(double2), hipMemcpyDeviceToDevice, s));
       } else {
-        if ((rc = master(x, w, precision))) return rc;
-        hipLaunchKernelGGL(sub_kernel, dim3(blocks_for(n8)), dim3(256), 0, s, b, w, w, n8);
+        if ((rc = master(x, ws->w, precision, false))) return rc;
+        hipLaunchKernelGGL(sub_kernel, dim3(blocks_for(n8)), dim3(256), 0, s, b, ws->w, ws->w, n8);
       }
-      double beta[kB];
-      if ((rc = norms(w, beta))) return rc;
-      bool cont[kB], live[kB], any = false;
-      int kr[kB];
-      double f[kB];
-      for (int r = 0; r < kB; ++r) {
-        res_out[r] = beta[r];
-        cont[r] = active[r] && beta[r] > atol[r] && beta[r] > 0.0;
-        live[r] = cont[r];
-        kr[r] = 0;
-        f[r] = cont[r] ? 1.0 / beta[r] : 0.0;
-        any = any || cont[r];
-      }
-      if (!any || steps >= maxiter) return SMH_OK;
-      set_scale(f, w, V);
-      std::fill(g.begin(), g.end(), zd(0));
-      for (int r = 0; r < kB; ++r) g[(size_t)r * (restart + 1)] = cont[r] ? beta[r] : 0.0;
+      if ((rc = dots(ws->w, 1, ws->w, ws->hn, 1))) return rc;
+      hipLaunchKernelGGL(zero_int_kernel, dim3(1), dim3(256), 0, s, ws->nlive, R + 1);
+      hipLaunchKernelGGL(cycle_init_kernel, dim3(blocks_for(nsys, 64)), dim3(64), 0, s, ws->hn, ws->atol, ws->active,
+                         nsys, R, ws->g, ws->live, ws->scale, ws->res, ws->kr, ws->nlive);
+      SMH_HIP(ctx, hipGetLastError());
+      SMH_HIP(ctx, hipMemcpyAsync(res_out, ws->res, nsys * sizeof(double), hipMemcpyDeviceToHost, s));
+      SMH_HIP(ctx, hipMemcpyAsync(ws->h_pin, ws->nlive + R, sizeof(int), hipMemcpyDeviceToHost, s));
+      SMH_HIP(ctx, hipStreamSynchronize(s));
+      if (ws->h_pin[0] == 0 || steps >= maxiter) return SMH_OK;
+      hipLaunchKernelGGL(sscale_kernel, dim3(blocks_for(n8)), dim3(256), 0, s, ws->w, ws->scale, ws->V, n8, per_part,
+                         (const int*)op->d_seg_of);
       int k = 0;
-      for (; k < restart && steps < maxiter; ++k, ++steps) {
-        double2* vk = V + (size_t)k * n8;
-        if ((rc = master(vk, w, precision))) return rc;
-        // CGS2 against V_0..V_k, all systems at once
-        if ((rc = dots(V, k + 1, w, hc.data()))) return rc;
-        SMH_HIP(ctx, hipMemcpyAsync(coef, hc.data(), (size_t)(k + 1) * kB * sizeof(double2), hipMemcpyHostToDevice, s));
-        hipLaunchKernelGGL(bmulti_axpy_kernel, dim3(blocks_for(n8)), dim3(256), 0, s, V, n8, k + 1, coef, -1.0, w, n8);
-        if ((rc = dots(V, k + 1, w, hc2.data()))) return rc;
-        SMH_HIP(ctx, hipMemcpyAsync(coef, hc2.data(), (size_t)(k + 1) * kB * sizeof(double2), hipMemcpyHostToDevice,
-                                    s));
-        hipLaunchKernelGGL(bmulti_axpy_kernel, dim3(blocks_for(n8)), dim3(256), 0, s, V, n8, k + 1, coef, -1.0, w, n8);
-        double hn[kB];
-        if ((rc = norms(w, hn))) return rc;
-        for (int r = 0; r < kB; ++r) f[r] = (live[r] && hn[r] > 0.0) ? 1.0 / hn[r] : 0.0;
-        set_scale(f, w, V + (size_t)(k + 1) * n8);
-        bool any_live = false;
-        for (int r = 0; r < kB; ++r) {
-          if (!live[r]) continue;
-          zd* Hr = H.data() + (size_t)r * (restart + 1) * restart;
-          zd* csr = cs.data() + (size_t)r * restart;
-          zd* snr = sn.data() + (size_t)r * restart;
-          zd* gr = g.data() + (size_t)r * (restart + 1);
-          std::vector<zd> col(k + 2);
-          for (int j = 0; j <= k; ++j) col[j] = hc[(size_t)j * kB + r] + hc2[(size_t)j * kB + r];
-          col[k + 1] = hn[r];
-          for (int j = 0; j < k; ++j) {
-            const zd a = col[j], bb = col[j + 1];
-            col[j] = std::conj(csr[j]) * a + std::conj(snr[j]) * bb;
-            col[j + 1] = -snr[j] * a + csr[j] * bb;
-          }
-          const zd a = col[k], bb = col[k + 1];
-          const double den = std::sqrt(std::norm(a) + std::norm(bb));
-          csr[k] = den > 0 ? a / den : zd(1);
-          snr[k] = den > 0 ? bb / den : zd(0);
-          col[k] = std::conj(csr[k]) * a + std::conj(snr[k]) * bb;
-          gr[k + 1] = -snr[k] * gr[k];
-          gr[k] = std::conj(csr[k]) * gr[k];
-          for (int j = 0; j <= k; ++j) Hr[(size_t)j * restart + k] = col[j];
-          kr[r] = k + 1;
-          if (std::abs(gr[k + 1]) <= atol[r] || hn[r] == 0.0)
-            live[r] = false;
-          else
-            any_live = true;
-        }
+      for (; k < R && steps < maxiter; ++k, ++steps) {
+        if ((rc = run_step(k, precision))) return rc;
         ++iters;
-        if (!any_live) {
+        SMH_HIP(ctx, hipMemcpyAsync(ws->h_pin, ws->nlive + k, sizeof(int), hipMemcpyDeviceToHost, s));
+        SMH_HIP(ctx, hipStreamSynchronize(s));
+        if (ws->h_pin[0] == 0) {
           ++k;
           ++steps;
           break;
         }
       }
-      // per system back substitution, then x += V y for all systems at once
-      std::fill(yv.begin(), yv.end(), zd(0));
-      for (int r = 0; r < kB; ++r) {
-        if (!cont[r]) continue;
-        const zd* Hr = H.data() + (size_t)r * (restart + 1) * restart;
-        const zd* gr = g.data() + (size_t)r * (restart + 1);
-        const int m = kr[r];
-        for (int i = m - 1; i >= 0; --i) {
-          zd sum = gr[i];
-          for (int j = i + 1; j < m; ++j) sum -= Hr[(size_t)i * restart + j] * yv[(size_t)j * kB + r];
-          yv[(size_t)i * kB + r] = sum / Hr[(size_t)i * restart + i];
-        }
-        res_out[r] = std::abs(gr[m]);
-      }
+      // y = R^-1 g per system, then x += V y (or x += P V y) for all at once
+      hipLaunchKernelGGL(backsub_kernel, dim3(blocks_for(nsys, 64)), dim3(64), 0, s, nsys, R, ws->H, ws->g, ws->kr,
+                         ws->yv, R + 1, ws->res);
       if (k > 0) {
-        SMH_HIP(ctx, hipMemcpyAsync(coef, yv.data(), (size_t)k * kB * sizeof(double2), hipMemcpyHostToDevice, s));
-        hipLaunchKernelGGL(bmulti_axpy_kernel, dim3(blocks_for(n8)), dim3(256), 0, s, V, n8, k, coef, 1.0, x, n8);
+        if (use_P) {
+          hipLaunchKernelGGL(zero2_kernel, dim3(1024), dim3(256), 0, s, ws->tmp, n8);
+          hipLaunchKernelGGL(saxpy_kernel, dim3(blocks_for(n8)), dim3(256), 0, s, ws->V, n8, k, ws->yv, R + 1, 1.0,
+                             ws->tmp, n8, per_part, (const int*)op->d_seg_of);
+          if ((rc = smh::op_apply_precond_batch(op, ws->tmp, x, true))) return rc;
+        } else {
+          hipLaunchKernelGGL(saxpy_kernel, dim3(blocks_for(n8)), dim3(256), 0, s, ws->V, n8, k, ws->yv, R + 1, 1.0, x,
+                             n8, per_part, (const int*)op->d_seg_of);
+        }
       }
-      if (trust_estimate) return SMH_OK;
+      SMH_HIP(ctx, hipGetLastError());
+      if (trust_estimate) {
+        SMH_HIP(ctx, hipMemcpyAsync(res_out, ws->res, nsys * sizeof(double), hipMemcpyDeviceToHost, s));
+        SMH_HIP(ctx, hipStreamSynchronize(s));
+        return SMH_OK;
+      }
     }
   }
 };
 
 }  // namespace
 
-// Blocks of kB right-hand sides solved together (every coupling matvec is one
-// batched launch for the whole block); same modes and stopping rule as below.
+namespace smh {
+
+void solver_release(smh_op* op) {
+  if (op->solver_ws) delete static_cast<Ws*>(op->solver_ws);
+  op->solver_ws = nullptr;
+}
+
+void solver_drop_graphs(smh_op* op) {
+  if (op->solver_ws) static_cast<Ws*>(op->solver_ws)->drop_graphs();
+}
+
+int op_apply_precond_batch(smh_op* op, const double2* x, double2* y, bool accumulate) {
+  smh_ctx* ctx = op->ctx;
+  SMH_CHECK(ctx, op->d_P != nullptr, SMH_ERR_STATE, "no preconditioner set");
+  const size_t lds = (size_t)op->B * kB * sizeof(double2);
+  hipLaunchKernelGGL(precond_kernel, dim3((unsigned)op->n_part), dim3(256), lds, ctx->stream, (const double2*)op->d_P,
+                     x, y, op->B, accumulate ? 1 : 0);
+  SMH_HIP(ctx, hipGetLastError());
+  return SMH_OK;
+}
+
+}  // namespace smh
+
+// Blocks of kB right-hand sides over every segment, solved together (every
+// coupling matvec is one batched launch for all systems); same modes and
+// stopping rule as the single right-hand side path.
 static int solve_batched(smh_op* op, const double* t_diag, const double* rhs, double* x, int64_t nrhs, double tol,
                          int restart, int maxiter, int mode, double* stats,
                          std::chrono::steady_clock::time_point t0) {
@@ -610,53 +914,48 @@ static int solve_batched(smh_op* op, const double* t_diag, const double* rhs, do
   S.op = op;
   S.ctx = ctx;
   S.s = s;
+  S.ws = ws_of(op);
   S.n = op->n_part * op->B;
   S.n8 = S.n * kB;
+  S.per_part = (int64_t)op->B * kB;
   S.restart = restart;
   int rc = S.init();
-  double2 *b_pad = nullptr, *x_pad = nullptr, *d_pad = nullptr, *r_pad = nullptr;
-  auto done = [&](int code) {
-    S.release();
-    void* p[] = {b_pad, x_pad, d_pad, r_pad};
-    for (void* q : p)
-      if (q) hipFree(q);
-    return code;
-  };
-  if (rc) return done(rc);
-  if ((rc = smh::op_reserve_packed(op, kB))) return done(rc);
-  if (hipMalloc(&b_pad, S.n8 * sizeof(double2)) != hipSuccess || hipMalloc(&x_pad, S.n8 * sizeof(double2)) != hipSuccess ||
-      hipMalloc(&d_pad, S.n8 * sizeof(double2)) != hipSuccess || hipMalloc(&r_pad, S.n8 * sizeof(double2)) != hipSuccess)
-    return done(smh_fail(ctx, SMH_ERR_HIP, "solver allocation failed"));
+  if (rc) return rc;
+  Ws* ws = S.ws;
+  const int nsys = S.nsys;
+  if ((rc = smh::op_reserve_packed(op, kB))) return rc;
+  if ((rc = smh::op_reserve_batch(op))) return rc;
   SMH_HIP(ctx, hipMemcpyAsync(op->d_pk_in, t_diag, op->n_unk * sizeof(double2), hipMemcpyHostToDevice, s));
-  if ((rc = smh::op_unpack(op, SMH_FP64, op->d_pk_in, S.t))) return done(rc);
+  if ((rc = smh::op_unpack(op, SMH_FP64, op->d_pk_in, ws->t))) return rc;
   double worst = 0, sweeps_total = 0;
   int64_t steps_total = 0;
   bool converged_all = true;
+  std::vector<double> bnorm(nsys), atol(nsys), res(nsys), prev_res(nsys), inner_atol(nsys), inner_res(nsys);
+  std::vector<int> active(nsys), act(nsys);
   for (int64_t r0 = 0; r0 < nrhs; r0 += kB) {
     const int nb = (int)std::min<int64_t>(kB, nrhs - r0);
     SMH_HIP(ctx, hipMemcpyAsync(op->d_pk_in, rhs + 2 * r0 * op->n_unk, (size_t)nb * op->n_unk * sizeof(double2),
                                 hipMemcpyHostToDevice, s));
-    if ((rc = smh::op_unpack_batch(op, SMH_FP64, op->d_pk_in, nb, b_pad))) return done(rc);
-    double bnorm[kB], atol[kB], res[kB];
-    bool active[kB];
-    if ((rc = S.norms(b_pad, bnorm))) return done(rc);
-    for (int r = 0; r < kB; ++r) {
-      atol[r] = tol * bnorm[r];
-      active[r] = r < nb && bnorm[r] > 0.0;
+    if ((rc = smh::op_unpack_batch(op, SMH_FP64, op->d_pk_in, nb, ws->b))) return rc;
+    if ((rc = S.norms(ws->b, bnorm.data()))) return rc;
+    for (int q = 0; q < nsys; ++q) {
+      atol[q] = tol * bnorm[q];
+      active[q] = (q % kB) < nb && bnorm[q] > 0.0;
     }
     int steps = 0;
     if (mode == SMH_SOLVE_FP64) {
-      SMH_HIP(ctx, hipMemcpyAsync(x_pad, b_pad, S.n8 * sizeof(double2), hipMemcpyDeviceToDevice, s));
-      if ((rc = S.gmres(x_pad, b_pad, atol, active, maxiter, SMH_FP64, res, steps, false, false))) return done(rc);
+      // x0 = rhs, as smuthi passes rhs as initial guess
+      SMH_HIP(ctx, hipMemcpyAsync(ws->x, ws->b, S.n8 * sizeof(double2), hipMemcpyDeviceToDevice, s));
+      if ((rc = S.gmres(ws->x, ws->b, atol.data(), active.data(), maxiter, SMH_FP64, res.data(), steps, false, false)))
+        return rc;
       steps_total += steps;
       sweeps_total += 1;
     } else {
-      SMH_HIP(ctx, hipMemsetAsync(x_pad, 0, S.n8 * sizeof(double2), s));
-      SMH_HIP(ctx, hipMemcpyAsync(r_pad, b_pad, S.n8 * sizeof(double2), hipMemcpyDeviceToDevice, s));
-      for (int r = 0; r < kB; ++r) res[r] = bnorm[r];
+      hipLaunchKernelGGL(zero2_kernel, dim3(1024), dim3(256), 0, s, ws->x, S.n8);
+      SMH_HIP(ctx, hipMemcpyAsync(ws->r, ws->b, S.n8 * sizeof(double2), hipMemcpyDeviceToDevice, s));
+      for (int q = 0; q < nsys; ++q) res[q] = bnorm[q];
       bool fresh = true;
       bool fp64_residuals = mode != SMH_SOLVE_MIXED16;
-      double prev_res[kB];  // residual each system's last sweep started from
       const int inner_prec = mode == SMH_SOLVE_MIXED16 ? SMH_FP16 : SMH_FP32;
       int budget = maxiter;
       for (int sweep = 0; sweep < 50; ++sweep) {
@@ -668,61 +967,61 @@ static int solve_batched(smh_op* op, const double* t_diag, const double* rhs, do
           // certify the result and drive any further sweeps
           bool certified = fp64_residuals;
           if (!certified) {
-            if ((rc = S.master(x_pad, r_pad, SMH_FP32))) return done(rc);
-            hipLaunchKernelGGL(sub_kernel, dim3(blocks_for(S.n8)), dim3(256), 0, s, b_pad, r_pad, r_pad, S.n8);
-            if ((rc = S.norms(r_pad, res))) return done(rc);
+            if ((rc = S.master(ws->x, ws->r, SMH_FP32, false))) return rc;
+            hipLaunchKernelGGL(sub_kernel, dim3(blocks_for(S.n8)), dim3(256), 0, s, ws->b, ws->r, ws->r, S.n8);
+            if ((rc = S.norms(ws->r, res.data()))) return rc;
             bool need64 = false, last = true;
-            for (int r = 0; r < kB; ++r) {
-              if (!active[r]) continue;
-              need64 = need64 || res[r] <= std::max(atol[r], kFp32ResidualFloor * bnorm[r]);
+            for (int q = 0; q < nsys; ++q) {
+              if (!active[q]) continue;
+              need64 = need64 || res[q] <= std::max(atol[q], kFp32ResidualFloor * bnorm[q]);
               // the next sweep gains about the factor this one did: if that
               // takes every system to the tolerance or to the FP32 floor, its
               // residual is taken in FP64 directly
-              last = last && prev_res[r] > 0.0 &&
-                     res[r] * (res[r] / prev_res[r]) <= std::max(0.5 * atol[r], kFp32ResidualFloor * bnorm[r]);
+              last = last && prev_res[q] > 0.0 &&
+                     res[q] * (res[q] / prev_res[q]) <= std::max(0.5 * atol[q], kFp32ResidualFloor * bnorm[q]);
             }
             certified = need64;
             fp64_residuals = need64 || last;
           }
           if (certified) {
-            if ((rc = S.master(x_pad, r_pad, SMH_FP64))) return done(rc);
-            hipLaunchKernelGGL(sub_kernel, dim3(blocks_for(S.n8)), dim3(256), 0, s, b_pad, r_pad, r_pad, S.n8);
-            if ((rc = S.norms(r_pad, res))) return done(rc);
+            if ((rc = S.master(ws->x, ws->r, SMH_FP64, false))) return rc;
+            hipLaunchKernelGGL(sub_kernel, dim3(blocks_for(S.n8)), dim3(256), 0, s, ws->b, ws->r, ws->r, S.n8);
+            if ((rc = S.norms(ws->r, res.data()))) return rc;
             fresh = true;
           }
         }
-        bool act[kB], any = false;
-        double inner_atol[kB];
-        for (int r = 0; r < kB; ++r) {
-          prev_res[r] = res[r];
-          act[r] = active[r] && res[r] > atol[r];
-          inner_atol[r] = std::max(atol[r] * 0.5, (mode == SMH_SOLVE_MIXED16 ? kInnerRel16 : kInnerRel) * res[r]);
-          any = any || act[r];
+        bool any = false;
+        for (int q = 0; q < nsys; ++q) {
+          prev_res[q] = res[q];
+          act[q] = active[q] && res[q] > atol[q];
+          inner_atol[q] = std::max(atol[q] * 0.5, (mode == SMH_SOLVE_MIXED16 ? kInnerRel16 : kInnerRel) * res[q]);
+          any = any || act[q];
         }
         if (!any || budget <= 0) break;
-        SMH_HIP(ctx, hipMemsetAsync(d_pad, 0, S.n8 * sizeof(double2), s));
-        double inner_res[kB];
+        hipLaunchKernelGGL(zero2_kernel, dim3(1024), dim3(256), 0, s, ws->d, S.n8);
         int inner_steps = 0;
-        if ((rc = S.gmres(d_pad, r_pad, inner_atol, act, budget, inner_prec, inner_res, inner_steps, true, true)))
-          return done(rc);
+        if ((rc = S.gmres(ws->d, ws->r, inner_atol.data(), act.data(), budget, inner_prec, inner_res.data(),
+                          inner_steps, true, true)))
+          return rc;
         budget -= inner_steps;
         steps_total += inner_steps;
-        hipLaunchKernelGGL(add_kernel, dim3(blocks_for(S.n8)), dim3(256), 0, s, x_pad, d_pad, S.n8);
+        hipLaunchKernelGGL(add_kernel, dim3(blocks_for(S.n8)), dim3(256), 0, s, ws->x, ws->d, S.n8);
         sweeps_total += 1;
         fresh = false;
       }
       if (!fresh) {
-        if ((rc = S.master(x_pad, r_pad, SMH_FP64))) return done(rc);
-        hipLaunchKernelGGL(sub_kernel, dim3(blocks_for(S.n8)), dim3(256), 0, s, b_pad, r_pad, r_pad, S.n8);
-        if ((rc = S.norms(r_pad, res))) return done(rc);
+        if ((rc = S.master(ws->x, ws->r, SMH_FP64, false))) return rc;
+        hipLaunchKernelGGL(sub_kernel, dim3(blocks_for(S.n8)), dim3(256), 0, s, ws->b, ws->r, ws->r, S.n8);
+        if ((rc = S.norms(ws->r, res.data()))) return rc;
       }
     }
-    for (int r = 0; r < nb; ++r) {
-      const double rel = bnorm[r] > 0 ? res[r] / bnorm[r] : 0.0;
+    for (int q = 0; q < nsys; ++q) {
+      if ((q % kB) >= nb) continue;
+      const double rel = bnorm[q] > 0 ? res[q] / bnorm[q] : 0.0;
       worst = std::max(worst, rel);
       if (rel > tol) converged_all = false;
     }
-    if ((rc = smh::op_pack_batch(op, SMH_FP64, x_pad, nb, op->d_pk_out))) return done(rc);
+    if ((rc = smh::op_pack_batch(op, SMH_FP64, ws->x, nb, op->d_pk_out))) return rc;
     SMH_HIP(ctx, hipMemcpyAsync(x + 2 * r0 * op->n_unk, op->d_pk_out, (size_t)nb * op->n_unk * sizeof(double2),
                                 hipMemcpyDeviceToHost, s));
   }
@@ -738,124 +1037,86 @@ static int solve_batched(smh_op* op, const double* t_diag, const double* rhs, do
     stats[6] = (double)kB;  // right-hand sides per batched matvec
     stats[7] = (double)S.mv16;
   }
-  if (!converged_all) {
-    smh_fail(ctx, SMH_ERR_NOCONV, "GMRES stopped at relative residual %.3e (tol %.1e)", worst, tol);
-    return done(SMH_ERR_NOCONV);
-  }
-  return done(SMH_OK);
+  if (!converged_all)
+    return smh_fail(ctx, SMH_ERR_NOCONV, "GMRES stopped at relative residual %.3e (tol %.1e)", worst, tol);
+  return SMH_OK;
 }
 
-extern "C" int smh_solve(smh_op* op, const double* t_diag, const double* rhs, double* x, int64_t nrhs, double tol,
-                         int restart, int maxiter, int mode, double* stats) {
-  if (!op) return SMH_ERR_ARG;
+static int solve_single(smh_op* op, const double* t_diag, const double* rhs, double* x, int64_t nrhs, double tol,
+                        int restart, int maxiter, int mode, double* stats,
+                        std::chrono::steady_clock::time_point t0) {
   smh_ctx* ctx = op->ctx;
-  SMH_CHECK(ctx, t_diag && rhs && x && nrhs >= 1, SMH_ERR_ARG, "invalid solve arguments");
-  SMH_CHECK(ctx, tol > 0 && restart >= 1 && restart < kMaxBasis && maxiter >= 1, SMH_ERR_ARG,
-            "invalid tolerance / restart / maxiter");
-  SMH_CHECK(ctx, mode == SMH_SOLVE_FP64 || mode == SMH_SOLVE_MIXED || mode == SMH_SOLVE_MIXED16, SMH_ERR_ARG,
-            "invalid mode %d", mode);
-  SMH_CHECK(ctx, mode != SMH_SOLVE_MIXED16 || (smh::op_has_batch(op, SMH_FP16) && smh::op_has_batch(op, SMH_FP64)),
-            SMH_ERR_UNSUPPORTED, "FP16-operand solves need a radial operator with the batched FP16 kernel");
-  SMH_CHECK(ctx, op->has_table, SMH_ERR_STATE, "coupling table not set");
-  SMH_HIP(ctx, hipSetDevice(ctx->device));
-  const auto t0 = std::chrono::steady_clock::now();
+  hipStream_t s = ctx->stream;
   Solver S;
   S.op = op;
   S.ctx = ctx;
-  S.s = ctx->stream;
+  S.s = s;
+  S.ws = ws_of(op);
   S.n = op->n_part * op->B;
   S.restart = restart;
   int rc = S.init();
-  if (rc) {
-    S.release();
-    return rc;
-  }
-  auto done = [&](int code) {
-    S.release();
-    return code;
-  };
-  rc = smh::op_reserve_packed(op, 1);
-  if (rc) return done(rc);
-  double2 *b_pad = nullptr, *x_pad = nullptr, *d_pad = nullptr, *r_pad = nullptr;
-  if (hipMalloc(&b_pad, S.n * sizeof(double2)) != hipSuccess || hipMalloc(&x_pad, S.n * sizeof(double2)) != hipSuccess ||
-      hipMalloc(&d_pad, S.n * sizeof(double2)) != hipSuccess || hipMalloc(&r_pad, S.n * sizeof(double2)) != hipSuccess)
-    return done(smh_fail(ctx, SMH_ERR_HIP, "solver allocation failed"));
-  auto done2 = [&](int code) {
-    hipFree(b_pad);
-    hipFree(x_pad);
-    hipFree(d_pad);
-    hipFree(r_pad);
-    return done(code);
-  };
-  hipStream_t s = ctx->stream;
-  if ((nrhs > 1 && smh::op_has_batch(op, SMH_FP32) && smh::op_has_batch(op, SMH_FP64)) || mode == SMH_SOLVE_MIXED16) {
-    const int code = solve_batched(op, t_diag, rhs, x, nrhs, tol, restart, maxiter, mode, stats, t0);
-    return done2(code);
-  }
+  if (rc) return rc;
+  Ws* ws = S.ws;
+  if ((rc = smh::op_reserve_packed(op, 1))) return rc;
   // T diagonal in padded layout
-  if (hipMemcpyAsync(op->d_pk_in, t_diag, op->n_unk * sizeof(double2), hipMemcpyHostToDevice, s) != hipSuccess)
-    return done2(smh_fail(ctx, SMH_ERR_HIP, "upload failed"));
-  if ((rc = smh::op_unpack(op, SMH_FP64, op->d_pk_in, S.t))) return done2(rc);
+  SMH_HIP(ctx, hipMemcpyAsync(op->d_pk_in, t_diag, op->n_unk * sizeof(double2), hipMemcpyHostToDevice, s));
+  if ((rc = smh::op_unpack(op, SMH_FP64, op->d_pk_in, ws->t1))) return rc;
   double worst = 0, sweeps_total = 0;
   int64_t steps_total = 0;
   bool converged_all = true;
   for (int64_t r = 0; r < nrhs; ++r) {
-    if (hipMemcpyAsync(op->d_pk_in, rhs + 2 * r * op->n_unk, op->n_unk * sizeof(double2), hipMemcpyHostToDevice, s) !=
-        hipSuccess)
-      return done2(smh_fail(ctx, SMH_ERR_HIP, "upload failed"));
-    if ((rc = smh::op_unpack(op, SMH_FP64, op->d_pk_in, b_pad))) return done2(rc);
+    SMH_HIP(ctx, hipMemcpyAsync(op->d_pk_in, rhs + 2 * r * op->n_unk, op->n_unk * sizeof(double2),
+                                hipMemcpyHostToDevice, s));
+    if ((rc = smh::op_unpack(op, SMH_FP64, op->d_pk_in, ws->b1))) return rc;
     double bnorm;
-    if ((rc = S.norm(b_pad, bnorm))) return done2(rc);
+    if ((rc = S.norm(ws->b1, bnorm))) return rc;
     const double atol = tol * bnorm;
     double res = 0;
     int steps = 0;
     if (mode == SMH_SOLVE_FP64) {
       // x0 = rhs, as smuthi passes rhs as initial guess; GMRES returns the true residual of x
-      if (hipMemcpyAsync(x_pad, b_pad, S.n * sizeof(double2), hipMemcpyDeviceToDevice, s) != hipSuccess)
-        return done2(smh_fail(ctx, SMH_ERR_HIP, "copy failed"));
-      if ((rc = S.gmres(x_pad, b_pad, atol, maxiter, SMH_FP64, res, steps))) return done2(rc);
+      SMH_HIP(ctx, hipMemcpyAsync(ws->x1, ws->b1, S.n * sizeof(double2), hipMemcpyDeviceToDevice, s));
+      if ((rc = S.gmres(ws->x1, ws->b1, atol, maxiter, SMH_FP64, res, steps))) return rc;
       steps_total += steps;
       sweeps_total += 1;
     } else {
       // iterative refinement from x0 = 0: FP64 residual, FP32-matvec inner GMRES
-      SMH_HIP(ctx, hipMemsetAsync(x_pad, 0, S.n * sizeof(double2), s));
-      SMH_HIP(ctx, hipMemcpyAsync(r_pad, b_pad, S.n * sizeof(double2), hipMemcpyDeviceToDevice, s));
+      hipLaunchKernelGGL(zero2_kernel, dim3(1024), dim3(256), 0, s, ws->x1, S.n);
+      SMH_HIP(ctx, hipMemcpyAsync(ws->r1, ws->b1, S.n * sizeof(double2), hipMemcpyDeviceToDevice, s));
       res = bnorm;
       bool fresh = true;  // res is the FP64 residual of the current x
       int budget = maxiter;
       for (int sweep = 0; sweep < 50; ++sweep) {
         if (sweep > 0) {
-          if ((rc = S.master(x_pad, r_pad, SMH_FP64))) return done2(rc);
-          hipLaunchKernelGGL(sub_kernel, dim3(blocks_for(S.n)), dim3(256), 0, s, b_pad, r_pad, r_pad, S.n);
-          if ((rc = S.norm(r_pad, res))) return done2(rc);
+          if ((rc = S.master(ws->x1, ws->r1, SMH_FP64))) return rc;
+          hipLaunchKernelGGL(sub_kernel, dim3(blocks_for(S.n)), dim3(256), 0, s, ws->b1, ws->r1, ws->r1, S.n);
+          if ((rc = S.norm(ws->r1, res))) return rc;
           fresh = true;
         }
         if (res <= atol || budget <= 0) break;
-        SMH_HIP(ctx, hipMemsetAsync(d_pad, 0, S.n * sizeof(double2), s));
+        hipLaunchKernelGGL(zero2_kernel, dim3(1024), dim3(256), 0, s, ws->d1, S.n);
         double inner_res = 0;
         int inner_steps = 0;
         const double inner_atol = std::max(atol * 0.5, kInnerRel * res);
-        if ((rc = S.gmres(d_pad, r_pad, inner_atol, budget, SMH_FP32, inner_res, inner_steps, true, true)))
-          return done2(rc);
+        if ((rc = S.gmres(ws->d1, ws->r1, inner_atol, budget, SMH_FP32, inner_res, inner_steps, true, true))) return rc;
         budget -= inner_steps;
         steps_total += inner_steps;
-        hipLaunchKernelGGL(add_kernel, dim3(blocks_for(S.n)), dim3(256), 0, s, x_pad, d_pad, S.n);
+        hipLaunchKernelGGL(add_kernel, dim3(blocks_for(S.n)), dim3(256), 0, s, ws->x1, ws->d1, S.n);
         sweeps_total += 1;
         fresh = false;
       }
       if (!fresh) {
-        if ((rc = S.master(x_pad, r_pad, SMH_FP64))) return done2(rc);
-        hipLaunchKernelGGL(sub_kernel, dim3(blocks_for(S.n)), dim3(256), 0, s, b_pad, r_pad, r_pad, S.n);
-        if ((rc = S.norm(r_pad, res))) return done2(rc);
+        if ((rc = S.master(ws->x1, ws->r1, SMH_FP64))) return rc;
+        hipLaunchKernelGGL(sub_kernel, dim3(blocks_for(S.n)), dim3(256), 0, s, ws->b1, ws->r1, ws->r1, S.n);
+        if ((rc = S.norm(ws->r1, res))) return rc;
       }
     }
     const double rel = bnorm > 0 ? res / bnorm : 0.0;
     worst = std::max(worst, rel);
     if (rel > tol) converged_all = false;
-    if ((rc = smh::op_pack(op, SMH_FP64, x_pad, op->d_pk_out))) return done2(rc);
-    if (hipMemcpyAsync(x + 2 * r * op->n_unk, op->d_pk_out, op->n_unk * sizeof(double2), hipMemcpyDeviceToHost, s) !=
-        hipSuccess)
-      return done2(smh_fail(ctx, SMH_ERR_HIP, "download failed"));
+    if ((rc = smh::op_pack(op, SMH_FP64, ws->x1, op->d_pk_out))) return rc;
+    SMH_HIP(ctx, hipMemcpyAsync(x + 2 * r * op->n_unk, op->d_pk_out, op->n_unk * sizeof(double2),
+                                hipMemcpyDeviceToHost, s));
   }
   SMH_HIP(ctx, hipStreamSynchronize(s));
   const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
@@ -869,9 +1130,56 @@ extern "C" int smh_solve(smh_op* op, const double* t_diag, const double* rhs, do
     stats[6] = 0;
     stats[7] = 0;
   }
-  if (!converged_all) {
-    smh_fail(ctx, SMH_ERR_NOCONV, "GMRES stopped at relative residual %.3e (tol %.1e)", worst, tol);
-    return done2(SMH_ERR_NOCONV);
+  if (!converged_all)
+    return smh_fail(ctx, SMH_ERR_NOCONV, "GMRES stopped at relative residual %.3e (tol %.1e)", worst, tol);
+  return SMH_OK;
+}
+
+extern "C" int smh_solve(smh_op* op, const double* t_diag, const double* rhs, double* x, int64_t nrhs, double tol,
+                         int restart, int maxiter, int mode, double* stats) {
+  if (!op) return SMH_ERR_ARG;
+  smh_ctx* ctx = op->ctx;
+  SMH_CHECK(ctx, t_diag && rhs && x && nrhs >= 1, SMH_ERR_ARG, "invalid solve arguments");
+  SMH_CHECK(ctx, mode == SMH_SOLVE_FP64 || mode == SMH_SOLVE_MIXED || mode == SMH_SOLVE_MIXED16 || mode == SMH_SOLVE_LU,
+            SMH_ERR_ARG, "invalid mode %d", mode);
+  if (mode == SMH_SOLVE_LU) {
+    SMH_HIP(ctx, hipSetDevice(ctx->device));
+    return smh::lu_solve(op, t_diag, rhs, x, nrhs, stats);
   }
-  return done2(SMH_OK);
+  SMH_CHECK(ctx, tol > 0 && restart >= 1 && restart < kMaxBasis && maxiter >= 1, SMH_ERR_ARG,
+            "invalid tolerance / restart / maxiter");
+  SMH_CHECK(ctx, mode != SMH_SOLVE_MIXED16 || (smh::op_has_batch(op, SMH_FP16) && smh::op_has_batch(op, SMH_FP64)),
+            SMH_ERR_UNSUPPORTED, "FP16-operand solves need a radial operator with the batched FP16 kernel");
+  SMH_CHECK(ctx, op->has_table, SMH_ERR_STATE, "coupling table not set");
+  SMH_HIP(ctx, hipSetDevice(ctx->device));
+  const auto t0 = std::chrono::steady_clock::now();
+  const bool batch_ok = smh::op_has_batch(op, SMH_FP32) && smh::op_has_batch(op, SMH_FP64);
+  const bool need_batch = mode == SMH_SOLVE_MIXED16 || op->d_P != nullptr || op->n_seg > 1;
+  SMH_CHECK(ctx, batch_ok || !need_batch, SMH_ERR_UNSUPPORTED,
+            "ensembles and preconditioned solves need an operator with the batched kernels");
+  if (batch_ok && (nrhs > 1 || need_batch))
+    return solve_batched(op, t_diag, rhs, x, nrhs, tol, restart, maxiter, mode, stats, t0);
+  return solve_single(op, t_diag, rhs, x, nrhs, tol, restart, maxiter, mode, stats, t0);
+}
+
+extern "C" int smh_op_set_preconditioner(smh_op* op, const double* blocks) {
+  if (!op) return SMH_ERR_ARG;
+  smh_ctx* ctx = op->ctx;
+  SMH_HIP(ctx, hipSetDevice(ctx->device));
+  if (!blocks) {
+    if (op->d_P) {
+      SMH_HIP(ctx, hipStreamSynchronize(ctx->stream));
+      smh::solver_drop_graphs(op);  // graphs hold the buffer's address
+      SMH_HIP(ctx, hipFree(op->d_P));
+      op->d_P = nullptr;
+    }
+    return SMH_OK;
+  }
+  SMH_CHECK(ctx, smh::op_has_batch(op, SMH_FP32) && smh::op_has_batch(op, SMH_FP64), SMH_ERR_UNSUPPORTED,
+            "preconditioned solves need an operator with the batched kernels");
+  const size_t bytes = (size_t)op->n_part * op->B * op->B * sizeof(double2);
+  if (!op->d_P) SMH_HIP(ctx, hipMalloc(&op->d_P, bytes));
+  SMH_HIP(ctx, hipMemcpyAsync(op->d_P, blocks, bytes, hipMemcpyHostToDevice, ctx->stream));
+  SMH_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return SMH_OK;
 }

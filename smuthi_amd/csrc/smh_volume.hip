@@ -17,6 +17,7 @@
 // rows. The FP64 kernel reads the same FP32 tap table (complex64 data, exact).
 #include <cmath>
 
+#include "smh_batch_kernel.h"
 #include "smh_lookup.h"
 #include "smh_special.h"
 
@@ -296,14 +297,17 @@ __host__ __device__ inline void vol_geometry(const double* a, const double* b, d
 // A pair is valid when its interpolation stencil lies inside the table: nodes
 // r - lo .. r + hi - 1 in each direction (lo = 0, hi = 2 bilinear and spline;
 // lo = 1, hi = 3 Catmull-Rom).
+// Grid: (receiver i1, block of 256 emitters of i1's ensemble); pairs couple
+// particles of one ensemble only (seg_of null: one ensemble).
 __global__ void vol_pair_bin_kernel(const double* __restrict__ xyz, int n, double rho_min, double h, int len_rho,
-                                    double zmin, double zsgn, int len_z, int lo, int hi, int Rc, int* cnt,
+                                    double zmin, double zsgn, int len_z, int lo, int hi, int Rc,
+                                    const int* __restrict__ seg_of, const int64_t* __restrict__ seg_off, int* cnt,
                                     const int* off, int* cursor, uint2* pairs, int* err) {
-  const int nb2 = (n + 255) / 256;
-  const int64_t b = blockIdx.x;
-  const int i1 = (int)(b / nb2);
-  const int i2 = (int)(b % nb2) * 256 + threadIdx.x;
-  if (i1 >= n || i2 >= n) return;
+  const int i1 = blockIdx.x;
+  const int sg = seg_of ? seg_of[i1] : 0;
+  const int64_t i2l = (int64_t)seg_off[sg] + (int64_t)blockIdx.y * 256 + threadIdx.x;
+  if (i1 >= n || i2l >= seg_off[sg + 1]) return;
+  const int i2 = (int)i2l;
   int r, j;
   double wr, wz, rho, dx, dy;
   vol_geometry(xyz + 3 * (size_t)i1, xyz + 3 * (size_t)i2, rho_min, h, zmin, zsgn, r, j, wr, wz, rho, dx, dy);
@@ -375,11 +379,42 @@ KernelInfo volume_kernel(int precision, int l, int m, bool tap64 = false) {
   return KernelInfo{};
 }
 
-// Bin the pairs of table tab (0: z1 + z2, 1: z1 - z2) for precision pi.
+// Volume kBatch-RHS kernel (plain layout) of an instance; tap64 as above.
+template <int L, int Mx, typename Real, typename TapT>
+KernelInfo vol_batch_kernel_info() {
+  using Sh = BatchShape<Real, SvwfIdx<L, Mx>::B, 4>;
+  KernelInfo k;
+  k.fn = reinterpret_cast<void*>(&batch_matvec_kernel<Real, TapT, L, Mx, 4, true>);
+  k.threads = Sh::kThreads;
+  k.sy = Sh::SY;
+  return k;
+}
+KernelInfo volume_batch_kernel(int precision, int l, int m, bool tap64 = false) {
+#define SMH_V(L, Mx)                                                                                    \
+  if (l == L && m == Mx)                                                                                \
+    return precision == SMH_FP32 ? vol_batch_kernel_info<L, Mx, float, float>()                         \
+                                 : (tap64 ? vol_batch_kernel_info<L, Mx, double, double>()               \
+                                          : vol_batch_kernel_info<L, Mx, double, float>());
+  SMH_V(1, 1)
+  SMH_V(2, 1)
+  SMH_V(2, 2)
+  SMH_V(3, 1)
+  SMH_V(3, 2)
+  SMH_V(3, 3)
+  SMH_V(4, 1)
+  SMH_V(4, 2)
+  SMH_V(4, 3)
+  SMH_V(4, 4)
+#undef SMH_V
+  return KernelInfo{};
+}
+
+// Bin the pairs of table tab (0: z1 + z2, 1: z1 - z2) for precision pi, for
+// the single-vector kernel or (batch) the kBatch-RHS kernel.
 template <typename Real>
-int vol_bin_pairs(smh_op* op, int tab, int pi, int Rc) {
+int vol_bin_pairs(smh_op* op, int tab, int pi, int Rc, bool batch) {
   smh_ctx* ctx = op->ctx;
-  smh_op::Binning& bn = op->vbin[tab][pi];
+  smh_op::Binning& bn = batch ? op->vbbin[tab][pi] : op->vbin[tab][pi];
   hipStream_t s = ctx->stream;
   const int64_t n_part = op->n_part, len_rho = op->len_rho, len_z = op->len_z;
   const double zmin = tab == 0 ? op->sz_min : op->dz_min, zsgn = tab == 0 ? 1.0 : -1.0;
@@ -405,11 +440,13 @@ int vol_bin_pairs(smh_op* op, int tab, int pi, int Rc) {
   hipMemsetAsync(d_cnt, 0, (nkeys + 1) * sizeof(int), s);
   hipMemsetAsync(d_cur, 0, (nkeys + 1) * sizeof(int), s);
   hipMemsetAsync(d_err, 0, sizeof(int), s);
-  const int nb2 = (int)((n_part + 255) / 256);
-  const int64_t nblocks = n_part * nb2;
-  hipLaunchKernelGGL(vol_pair_bin_kernel, dim3((unsigned)nblocks), dim3(256), 0, s, op->d_xyz, (int)n_part,
-                     op->rho_min, op->h, (int)len_rho, zmin, zsgn, (int)len_z, lo, hi, Rc, d_cnt, (const int*)nullptr,
-                     d_cur, (uint2*)nullptr, d_err);
+  int64_t max_seg = 0;
+  for (int sg = 0; sg < op->n_seg; ++sg) max_seg = std::max<int64_t>(max_seg, op->seg_off[sg + 1] - op->seg_off[sg]);
+  const dim3 grid((unsigned)n_part, (unsigned)((max_seg + 255) / 256));
+  SMH_CHECK(ctx, grid.y <= 65535, SMH_ERR_ARG, "ensemble too large for the pair binning");
+  hipLaunchKernelGGL(vol_pair_bin_kernel, grid, dim3(256), 0, s, op->d_xyz, (int)n_part, op->rho_min, op->h,
+                     (int)len_rho, zmin, zsgn, (int)len_z, lo, hi, Rc, (const int*)op->d_seg_of,
+                     (const int64_t*)op->d_seg_off, d_cnt, (const int*)nullptr, d_cur, (uint2*)nullptr, d_err);
   std::vector<int> cnt(nkeys);
   int herr = 0;
   if (hipMemcpyAsync(cnt.data(), d_cnt, nkeys * sizeof(int), hipMemcpyDeviceToHost, s) != hipSuccess ||
@@ -439,9 +476,9 @@ int vol_bin_pairs(smh_op* op, int tab, int pi, int Rc) {
     cleanup();
     return smh_fail(ctx, SMH_ERR_HIP, "device allocation failed (volume pairs)");
   }
-  hipLaunchKernelGGL(vol_pair_bin_kernel, dim3((unsigned)nblocks), dim3(256), 0, s, op->d_xyz, (int)n_part,
-                     op->rho_min, op->h, (int)len_rho, zmin, zsgn, (int)len_z, lo, hi, Rc, d_cnt, (const int*)d_off,
-                     d_cur, d_pairs, d_err);
+  hipLaunchKernelGGL(vol_pair_bin_kernel, grid, dim3(256), 0, s, op->d_xyz, (int)n_part, op->rho_min, op->h,
+                     (int)len_rho, zmin, zsgn, (int)len_z, lo, hi, Rc, (const int*)op->d_seg_of,
+                     (const int64_t*)op->d_seg_off, d_cnt, (const int*)d_off, d_cur, d_pairs, d_err);
   if (acc > 0)
     hipLaunchKernelGGL(vol_pair_record_kernel<Real>, dim3((unsigned)((acc + 255) / 256)), dim3(256), 0, s, d_pairs,
                        acc, op->d_xyz, op->rho_min, op->h, zmin, zsgn, Rc, (PairRec2<Real>*)bn.d_rec);
@@ -454,7 +491,7 @@ int vol_bin_pairs(smh_op* op, int tab, int pi, int Rc) {
   // a (chunk, r, j) key is a bin; its entry starts at the node's first tap row (r * len_z + j) * vq
   const int vq = op->vq;
   return build_work_items<PairRec2<Real>>(op, bn, cnt, off, len_rho * len_z,
-                                          [vq](int64_t rj) { return (int)(rj * vq); });
+                                          [vq](int64_t rj) { return (int)(rj * vq); }, batch);
 }
 
 // Tap tables, row sums and per-item statistics from the complex64 tables:
@@ -517,10 +554,11 @@ int vol_make_taps(smh_op* op) {
     }
     hipLaunchKernelGGL(frag_rowsum_kernel, dim3((unsigned)std::min<int64_t>(rows, 65536)), dim3(256), 0, s,
                        op->d_vtt[t], rows, op->B, op->Bh, g.NG, g.FR, op->d_vrowsum[t]);
-    smh_op::Binning& bn = op->vbin[t][0];
-    if (bn.n_items > 0)
-      hipLaunchKernelGGL(item_lsum_kernel, dim3(bn.n_items), dim3(256), 0, s, bn.d_entries, bn.d_items, bn.n_items,
-                         bn.n_entries, 4, op->vmode, (int)op->len_z, op->d_vrowsum[t], bn.d_item_lsum);
+    for (smh_op::Binning* bn : {&op->vbin[t][0], &op->vbbin[t][0]})
+      if (bn->n_items > 0)
+        hipLaunchKernelGGL(item_lsum_kernel, dim3(bn->n_items), dim3(256), 0, s, bn->d_entries, bn->d_items,
+                           bn->n_items, bn->n_entries, 4, op->vmode, (int)op->len_z, op->d_vrowsum[t],
+                           bn->d_item_lsum);
     // the moments of this table must be consumed before the next one overwrites them
     if (hipStreamSynchronize(s) != hipSuccess) {
       cleanup();
@@ -569,7 +607,7 @@ int vol_apply(smh_op* op, int precision, const void* x_pad, void* y_pad, bool ti
       a.len_z = (int)op->len_z;
       a.vmode = op->vmode;
       a.vslice = sl;
-      SMH_HIP(ctx, hipFuncSetAttribute(k.fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bn.lds));
+      if (int rc_ = op_set_lds(op, k.fn, bn.lds)) return rc_;
       if (bn.n_items > 0) {
         if (time_kernel) SMH_HIP(ctx, hipEventRecord(ctx->ev0, s));
         void* args[] = {&a};
@@ -591,6 +629,71 @@ int vol_apply(smh_op* op, int precision, const void* x_pad, void* y_pad, bool ti
       else
         hipLaunchKernelGGL(reduce_kernel<double>, dim3(blocks_for(n)), dim3(256), 0, s, (const double*)op->d_ypart,
                            bn.d_chunk_items, bn.Rc, op->B, op->n_part, (double2*)y_pad, accumulate);
+      SMH_HIP(ctx, hipGetLastError());
+    }
+  }
+  if (time_kernel) op->last_kernel_ms = (float)ms_total;
+  return SMH_OK;
+}
+
+// kBatch right-hand sides on the plain layout [n_part][B][kBatch] (FP32 input
+// float2, FP64 double2): one batched launch per table and z slice, their
+// partials summed into y in launch order.
+int vol_apply_batch(smh_op* op, int precision, const void* x_pad, void* y_pad, bool time_kernel) {
+  smh_ctx* ctx = op->ctx;
+  const int pi = precision == SMH_FP32 ? 0 : 1;
+  const bool tap64 = pi == 1 && op->vq == 4;
+  const KernelInfo k = volume_batch_kernel(precision, op->l_max, op->m_max, tap64);
+  SMH_CHECK(ctx, k.fn != nullptr && op->vbbin[0][pi].Rc > 0, SMH_ERR_UNSUPPORTED,
+            "no batched volume matvec kernel for l_max %d m_max %d", op->l_max, op->m_max);
+  hipStream_t s = ctx->stream;
+  if (pi == 0) launch_xmax((const float2*)x_pad, op->n_part * op->B * kBatchRhs, op->d_xmax, s);
+  double ms_total = 0;
+  const int n_slices = op->vmode >= 2 ? 4 : 1;
+  const int64_t nx = op->n_part * op->B * kBatchRhs;
+  for (int t = 0; t < 2; ++t) {
+    const smh_op::Binning& bn = op->vbbin[t][pi];
+    for (int sl = 0; sl < n_slices; ++sl) {
+      RadialKArgs a;
+      a.recs = bn.d_rec;
+      a.entries = bn.d_entries;
+      a.items = bn.d_items;
+      a.order = bn.d_order;
+      a.tt = tap64 ? (const void*)op->d_vtt64[t] : (const void*)op->d_vtt[t];
+      a.x = x_pad;
+      a.ypart = op->d_ypart;
+      a.xmax = op->d_xmax;
+      a.item_lsum = bn.d_item_lsum;
+      a.item_en = bn.d_item_en;
+      a.n_items = bn.n_items;
+      a.n_entries = bn.n_entries;
+      a.interp = op->interp;
+      a.Rc = bn.Rc;
+      a.len_z = (int)op->len_z;
+      a.vmode = op->vmode;
+      a.vslice = sl;
+      if (int rc_ = op_set_lds(op, k.fn, bn.lds)) return rc_;
+      if (bn.n_items > 0) {
+        if (time_kernel) SMH_HIP(ctx, hipEventRecord(ctx->ev0, s));
+        void* args[] = {&a};
+        SMH_HIP(ctx, hipLaunchKernel(k.fn, dim3(bn.n_items), dim3(k.threads), args, bn.lds, s));
+        if (time_kernel) {
+          SMH_HIP(ctx, hipEventRecord(ctx->ev1, s));
+          SMH_HIP(ctx, hipEventSynchronize(ctx->ev1));
+          float ms = 0;
+          SMH_HIP(ctx, hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+          ms_total += ms;
+        }
+      }
+      const int accumulate = (t > 0 || sl > 0) ? 1 : 0;
+      if (pi == 0)
+        hipLaunchKernelGGL(reduce_batch_kernel<float>, dim3(blocks_for(nx)), dim3(256), 0, s,
+                           (const float*)op->d_ypart, bn.d_chunk_items, bn.Rc, op->B, op->B, (const int*)nullptr,
+                           op->n_part, (float2*)y_pad, accumulate);
+      else
+        hipLaunchKernelGGL(reduce_batch_kernel<double>, dim3(blocks_for(nx)), dim3(256), 0, s,
+                           (const double*)op->d_ypart, bn.d_chunk_items, bn.Rc, op->B, op->B, (const int*)nullptr,
+                           op->n_part, (double2*)y_pad, accumulate);
       SMH_HIP(ctx, hipGetLastError());
     }
   }
@@ -630,9 +733,11 @@ int vol_download(smh_op* op, double* plus, double* minus) {
 
 }  // namespace smh
 
-extern "C" int smh_volume_create(smh_ctx* ctx, int64_t n_part, const double* xyz, const int* lmax, const int* mmax,
-                                 int l_max, int m_max, double rho_min, double sz_min, double dz_min, double resolution,
-                                 int64_t len_rho, int64_t len_z, int interp, smh_op** out) {
+// Operator over n_ens ensembles (particles [ens_off[e], ens_off[e+1])), or one
+// ensemble when ens_off is null.
+static int volume_create(smh_ctx* ctx, int n_ens, const int64_t* ens_off, int64_t n_part, const double* xyz,
+                         const int* lmax, const int* mmax, int l_max, int m_max, double rho_min, double sz_min,
+                         double dz_min, double resolution, int64_t len_rho, int64_t len_z, int interp, smh_op** out) {
   if (!ctx || !out) return SMH_ERR_ARG;
   *out = nullptr;
   SMH_CHECK(ctx, n_part >= 1 && xyz && lmax && mmax, SMH_ERR_ARG, "need at least one particle");
@@ -675,6 +780,9 @@ extern "C" int smh_volume_create(smh_ctx* ctx, int64_t n_part, const double* xyz
     return code;
   };
   if (int code = op_index_map(op, lmax, mmax)) return fail(code);
+  if (ens_off)
+    if (int code = op_set_segments(op, n_ens, ens_off)) return fail(code);
+  op->b32_S = op->B;  // the batched kernel's input: plain layout [n_part][B][kBatch]
   hipStream_t s = ctx->stream;
   if (hipMalloc(&op->d_xyz, n_part * 3 * sizeof(double)) != hipSuccess ||
       hipMemcpyAsync(op->d_xyz, xyz, n_part * 3 * sizeof(double), hipMemcpyHostToDevice, s) != hipSuccess ||
@@ -691,9 +799,20 @@ extern "C" int smh_volume_create(smh_ctx* ctx, int64_t n_part, const double* xyz
     if (Rc == 0) continue;
     for (int t = 0; t < 2; ++t) {
       op->vbin[t][pi].lds = lds;
-      const int code = pi == 0 ? vol_bin_pairs<float>(op, t, 0, Rc) : vol_bin_pairs<double>(op, t, 1, Rc);
+      const int code = pi == 0 ? vol_bin_pairs<float>(op, t, 0, Rc, false) : vol_bin_pairs<double>(op, t, 1, Rc, false);
       if (code) return fail(code);
       ypart = std::max(ypart, (size_t)op->vbin[t][pi].n_items * Rc * 2 * op->B * rs);
+    }
+    // the same pairs for the kBatch-RHS kernel (its accumulator holds kBatch vectors per receiver)
+    const KernelInfo kb = volume_batch_kernel(pi == 0 ? SMH_FP32 : SMH_FP64, l_max, m_max);
+    size_t ldsb = 0;
+    const int Rb = kb.fn ? chunk_size(op, rs, kb.sy, ldsb) : 0;
+    if (Rb == 0) continue;
+    for (int t = 0; t < 2; ++t) {
+      op->vbbin[t][pi].lds = ldsb;
+      const int code = pi == 0 ? vol_bin_pairs<float>(op, t, 0, Rb, true) : vol_bin_pairs<double>(op, t, 1, Rb, true);
+      if (code) return fail(code);
+      ypart = std::max(ypart, (size_t)op->vbbin[t][pi].n_items * Rb * 2 * kBatchRhs * op->B * rs);
     }
   }
   op->ypart_bytes = std::max<size_t>(ypart, 16);
@@ -709,6 +828,23 @@ extern "C" int smh_volume_create(smh_ctx* ctx, int64_t n_part, const double* xyz
   ctx->live_ops++;
   *out = op;
   return SMH_OK;
+}
+
+extern "C" int smh_volume_create(smh_ctx* ctx, int64_t n_part, const double* xyz, const int* lmax, const int* mmax,
+                                 int l_max, int m_max, double rho_min, double sz_min, double dz_min, double resolution,
+                                 int64_t len_rho, int64_t len_z, int interp, smh_op** out) {
+  return volume_create(ctx, 1, nullptr, n_part, xyz, lmax, mmax, l_max, m_max, rho_min, sz_min, dz_min, resolution,
+                       len_rho, len_z, interp, out);
+}
+
+extern "C" int smh_volume_create_ensembles(smh_ctx* ctx, int n_ens, const int64_t* ens_off, const double* xyz,
+                                           const int* lmax, const int* mmax, int l_max, int m_max, double rho_min,
+                                           double sz_min, double dz_min, double resolution, int64_t len_rho,
+                                           int64_t len_z, int interp, smh_op** out) {
+  if (!ctx || !out) return SMH_ERR_ARG;
+  SMH_CHECK(ctx, n_ens >= 1 && ens_off, SMH_ERR_ARG, "need at least one ensemble");
+  return volume_create(ctx, n_ens, ens_off, ens_off[n_ens], xyz, lmax, mmax, l_max, m_max, rho_min, sz_min, dz_min,
+                       resolution, len_rho, len_z, interp, out);
 }
 
 extern "C" int smh_volume_build_table(smh_op* op, double k_re, double k_im, double mask_radius, double z_s,

@@ -20,9 +20,6 @@ from . import cuda as cu
 from . import fields as flds
 from . import lookup as look
 
-# relative residual of the GPU solve that stands in for smuthi's LU solve
-_LU_TOLERANCE = 1e-12
-
 _INTERP = {'linear': _lib.INTERP_LINEAR, 'catmull-rom': _lib.INTERP_CATMULL_ROM, 'spline': _lib.INTERP_SPLINE}
 
 
@@ -126,7 +123,8 @@ class _GpuCouplingOperator(SystemMatrix):
         nrhs = 1 if b.ndim == 1 else b.shape[0]
         x = np.empty_like(b)
         stats = np.zeros(8)
-        mode = {'mixed': _lib.SOLVE_MIXED, 'mixed16': _lib.SOLVE_MIXED16}.get(precision, _lib.SOLVE_FP64)
+        mode = {'mixed': _lib.SOLVE_MIXED, 'mixed16': _lib.SOLVE_MIXED16, 'lu': _lib.SOLVE_LU}.get(
+            precision, _lib.SOLVE_FP64)
         maxiter = 10 * self.shape[0] if maxiter is None else maxiter
         maxiter = int(min(maxiter, 2 ** 31 - 1))
         code = _lib.load().smh_solve(self._op, _lib.dptr(t), _lib.dptr(b), _lib.dptr(x), nrhs, float(tol),
@@ -400,12 +398,17 @@ class LinearSystem:
     Extra arguments: precision ('mixed': FP32 coupling matvecs inside FP64
     iterative refinement; 'mixed16': FP16-operand matvecs inside the same
     refinement, radial lookups; 'double': plain FP64 GMRES),
-    cubic_interpolant, restart and maxiter."""
+    cubic_interpolant, restart and maxiter. solver_type 'LU' factors the
+    explicit matrix on the device (and, as the reference, refuses a lookup
+    operator); periodicity and plane-wave mediated direct coupling raise
+    NotImplementedError."""
 
-    def __init__(self, particle_list, initial_field, layer_system, k_parallel='default', solver_type='gmres',
-                 solver_tolerance=1e-4, store_coupling_matrix=False, coupling_matrix_lookup_resolution=None,
-                 interpolator_kind='cubic', cuda_blocksize=None, periodicity=None, precision='mixed',
-                 cubic_interpolant='spline', restart=20, maxiter=None, **unused):
+    def __init__(self, particle_list, initial_field, layer_system, k_parallel='default', solver_type='LU',
+                 solver_tolerance=1e-4, store_coupling_matrix=True, coupling_matrix_lookup_resolution=None,
+                 interpolator_kind='cubic', cuda_blocksize=None, periodicity=None,
+                 ewald_sum_separation_parameter='default', number_of_threads_periodic='default',
+                 use_pvwf_coupling=False, pvwf_coupling_k_parallel=None, precision='mixed',
+                 cubic_interpolant='spline', restart=20, maxiter=None):
         self.particle_list = particle_list
         self.initial_field = initial_field
         self.layer_system = layer_system
@@ -422,6 +425,17 @@ class LinearSystem:
         self.maxiter = maxiter
         if periodicity is not None:
             raise NotImplementedError('periodic particle arrangements are outside the GPU lookup path')
+        self.periodicity = periodicity
+        self.ewald_sum_separation_parameter = ewald_sum_separation_parameter
+        self.number_of_threads_periodic = number_of_threads_periodic
+        self.use_pvwf_coupling = use_pvwf_coupling
+        self.pvwf_coupling_k_parallel = pvwf_coupling_k_parallel
+        if use_pvwf_coupling:
+            if coupling_matrix_lookup_resolution is None:
+                raise NotImplementedError('plane-wave mediated direct coupling is not built on the GPU path')
+            # the reference ignores the flag next to a lookup (linear_system.py:118-122)
+            warnings.warn('The use_pvwf_coupling flag was defined in combination with a lookup. This combination '
+                          'is not yet available. Ignoring the use_pvwf_coupling flag.')
         self.shape = SystemMatrix(particle_list).shape
         sys.stdout.write('Number of unknowns: %i\n' % self.shape[0])
 
@@ -458,8 +472,6 @@ class LinearSystem:
         if not cu.use_gpu:
             raise RuntimeError('smuthi_amd provides the MI355X operators only: call '
                                'smuthi_amd.cuda.enable_gpu() first')
-        if self.solver_type.lower() == 'lu':
-            self.store_coupling_matrix = True
         if self.coupling_matrix_lookup_resolution is not None:
             layers = {self.layer_system.layer_number(p.position[2]) for p in self.particle_list}
             if len(layers) != 1:
@@ -493,15 +505,20 @@ class LinearSystem:
     def solve(self, rhs=None):
         """Solve for the scattered-field coefficients; rhs may hold several right-hand sides [nrhs, n]."""
         from .expansions import SphericalWaveExpansion
-        if self.solver_type.lower() not in ('gmres', 'lu'):
+        kind = self.solver_type.lower()
+        if kind not in ('gmres', 'lgmres', 'lu'):
             raise ValueError('This solver type is currently not implemented.')
         single = rhs is None
         b = self.t_matrix.right_hand_side() if single else rhs
         tol, precision = self.solver_tolerance, self.precision
-        if self.solver_type.lower() == 'lu':
-            # the dense matrix stays on the device: FP64 GMRES to a residual a
-            # direct solve would leave
-            tol, precision = _LU_TOLERANCE, 'double'
+        if kind == 'lu':
+            # linear_system.py:259-268: LU needs the stored (explicit) matrix; the
+            # factors are kept on the device for later right-hand sides
+            if not isinstance(self.coupling_matrix, CouplingMatrixExplicit):
+                raise ValueError('LU factorization only possible with the option "store coupling matrix".')
+            precision = 'lu'
+        elif kind == 'lgmres':
+            warnings.warn("solver_type 'lgmres' runs the GPU GMRES (same stopping rule)")
         x, self.solve_stats, converged = self.coupling_matrix.solve_master(
             self.t_matrix.diagonal, b, tol, precision=precision, restart=self.restart, maxiter=self.maxiter,
             with_stats=True)

@@ -103,13 +103,19 @@ def volume_grids(xyz, resolution):
     each arange(min - 3h, max + 3h, h); the two z grids must have one length."""
     xyz = np.asarray(xyz, dtype=float)
     rho_max, rho_min_pair = pair_distance_extremes(xyz[:, :2])
-    zmin, zmax = xyz[:, 2].min(), xyz[:, 2].max()
+    rho, sz, dz = volume_grids_from_extremes(rho_max, xyz[:, 2].min(), xyz[:, 2].max(), resolution)
+    return rho, sz, dz, rho_min_pair
+
+
+def volume_grids_from_extremes(rho_max, zmin, zmax, resolution):
+    """The grids of volume_grids for a largest in-plane distance and a z range
+    (a table shared by several ensembles takes the extremes over all of them)."""
     rho = np.arange(-3 * resolution, rho_max + 3 * resolution, resolution)
     sz = np.arange(2 * zmin - 3 * resolution, 2 * zmax + 3 * resolution, resolution)
     dz = np.arange((zmin - zmax) - 3 * resolution, (zmax - zmin) + 3 * resolution, resolution)
     if len(sz) != len(dz):
         raise ValueError('z1+z2 and z1-z2 grids differ in length (%d, %d)' % (len(sz), len(dz)))
-    return rho, sz, dz, rho_min_pair
+    return rho, sz, dz
 
 
 def direct_coefficients_volume(l_max, m_max):

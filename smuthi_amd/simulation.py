@@ -18,8 +18,33 @@ class Simulation:
                  neff_max_offset=1, neff_resolution=1e-2, neff_minimal_branchpoint_distance=None,
                  overwrite_default_contours=True, solver_type='LU', solver_tolerance=1e-4,
                  store_coupling_matrix=True, coupling_matrix_lookup_resolution=None,
-                 coupling_matrix_interpolator_kind='linear', log_to_terminal=True, do_sanity_check=True,
-                 precision='mixed', cubic_interpolant='spline', restart=20, maxiter=None, **unused):
+                 coupling_matrix_interpolator_kind='linear', length_unit='length unit', input_file=None,
+                 output_dir='smuthi_output', save_after_run=False, log_to_file=False, log_to_terminal=True,
+                 check_circumscribing_spheres=True, do_sanity_check=True, periodicity=None,
+                 ewald_sum_separation_parameter='default', number_of_threads_periodic='default',
+                 use_pvwf_coupling=False, pvwf_coupling_neff_max=None, pvwf_coupling_neff_resolution=1e-2,
+                 precision='mixed', cubic_interpolant='spline', restart=20, maxiter=None):
+        # arguments of the reference's constructor (simulation.py:92-123) that
+        # select paths this package does not build raise instead of being dropped
+        if periodicity is not None:
+            raise NotImplementedError('periodic particle arrangements are outside the GPU lookup path')
+        if use_pvwf_coupling and coupling_matrix_lookup_resolution is None:
+            raise NotImplementedError('plane-wave mediated direct coupling is not built on the GPU path')
+        if save_after_run or log_to_file:
+            raise NotImplementedError('saving the simulation and logging to a file are not part of this package')
+        self.length_unit = length_unit
+        self.input_file = input_file
+        self.output_dir = output_dir
+        self.save_after_run = save_after_run
+        self.log_to_file = log_to_file
+        self.log_to_terminal = log_to_terminal
+        self.check_circumscribing_spheres = check_circumscribing_spheres
+        self.periodicity = periodicity
+        self.ewald_sum_separation_parameter = ewald_sum_separation_parameter
+        self.number_of_threads_periodic = number_of_threads_periodic
+        self.use_pvwf_coupling = use_pvwf_coupling
+        self.pvwf_coupling_neff_max = pvwf_coupling_neff_max
+        self.pvwf_coupling_neff_resolution = pvwf_coupling_neff_resolution
         self.layer_system = layer_system
         self.particle_list = particle_list
         self.initial_field = initial_field
@@ -71,7 +96,10 @@ class Simulation:
             k_parallel=self.k_parallel, solver_type=self.solver_type, solver_tolerance=self.solver_tolerance,
             store_coupling_matrix=self.store_coupling_matrix,
             coupling_matrix_lookup_resolution=self.coupling_matrix_lookup_resolution,
-            interpolator_kind=self.coupling_matrix_interpolator_kind, precision=self.precision,
+            interpolator_kind=self.coupling_matrix_interpolator_kind, periodicity=self.periodicity,
+            ewald_sum_separation_parameter=self.ewald_sum_separation_parameter,
+            number_of_threads_periodic=self.number_of_threads_periodic, use_pvwf_coupling=self.use_pvwf_coupling,
+            precision=self.precision,
             cubic_interpolant=self.cubic_interpolant, restart=self.restart, maxiter=self.maxiter)
 
     def run(self):

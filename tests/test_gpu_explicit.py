@@ -102,3 +102,34 @@ def test_explicit_lu_path_solves_the_system(gpu):
     sim.run()
     got = np.concatenate([p.scattered_field.coefficients for p in pparts])
     assert np.abs(got - want).max() < 1e-9 * np.abs(want).max()
+
+
+def test_lu_solve_on_overlapping_spheres(gpu):
+    """solver_type 'LU' is a direct solve (linear_system.py:259-268): on the overlapping-sphere system that
+    stagnates restarted GMRES(20) near 3e-3, the device LU of M = 1 - T W (explicit matrix, one medium and a
+    substrate) matches the oracle's dense solve of the same matrix to 1e-12, takes no Arnoldi steps, reuses
+    its factors for a second call, and a lookup operator is refused as the reference refuses it."""
+    from smuthi_amd.layers import LayerSystem
+    from smuthi_amd.linear_system import CouplingMatrixExplicit, CouplingMatrixRadialLookupCUDA
+    kp = ob.reasonable_Sommerfeld_kpar_contour(WL, layer_refractive_indices=[1.52, 1], neff_resolution=1e-2)
+    plist = [osys.Sphere([0, 0, 150], 2.4 + 0.1j, 100, 3, 3), osys.Sphere([102, -100, 150], 1.9 + 0.2j, 80, 3, 2),
+             osys.Sphere([202, 100, 150], 1.7, 90, 3, 3)]
+    ls = LayerSystem([0, 0], [1.52, 1])
+    op = CouplingMatrixExplicit(WL, plist, ls, k_parallel=kp)
+    W = op.coupling_matrix
+    rng = np.random.default_rng(3)
+    n = W.shape[0]
+    t = 0.3 + 0.1j + 0.05 * rng.standard_normal(n)
+    M = np.eye(n) - t[:, None] * W
+    R = rng.standard_normal((3, n)) + 1j * rng.standard_normal((3, n))
+    want = np.linalg.solve(M, R.T).T
+    for call in range(2):
+        X, st, ok = op.solve_master(t, R, tol=1e-4, precision='lu', with_stats=True)
+        print('LU call %d:' % call, st, 'cond', np.linalg.cond(M), flush=True)
+        assert ok and st['iterations'] == 0 and st['relative_residual'] < 1e-12
+        for j in range(3):
+            assert np.linalg.norm(X[j] - want[j]) / np.linalg.norm(want[j]) < 1e-12, (call, j)
+    lookup = CouplingMatrixRadialLookupCUDA(WL, plist, ls, k_parallel=kp, resolution=5.0, interpolator_kind='cubic')
+    from smuthi_amd import _lib
+    with pytest.raises(_lib.SmhError):
+        lookup.solve_master(t, R[0], tol=1e-4, precision='lu')

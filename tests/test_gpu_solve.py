@@ -100,7 +100,8 @@ def test_multiple_rhs_match_single(gpu):
     sols = []
     for j, ang in enumerate((np.pi, np.pi * 0.9, np.pi * 0.8)):
         lsys = LinearSystem(pparts, PlaneWave(WL, ang, 0.3 * j, j % 2), ls, kp, solver_type='gmres',
-                            solver_tolerance=1e-8, coupling_matrix_lookup_resolution=5.0, interpolator_kind='cubic')
+                            solver_tolerance=1e-8, store_coupling_matrix=False, coupling_matrix_lookup_resolution=5.0,
+                            interpolator_kind='cubic')
         lsys.prepare()
         rhs.append(lsys.t_matrix.right_hand_side())
         sols.append(lsys.solve())

@@ -318,44 +318,451 @@ def rhs_bench(n_particles, n_dipoles=16, seed=13, cpu_particles=3):
     return json.dumps(out)
 
 
-def cpu_baseline(table, rho, n_full, iters_per_solve, n_sample=400, seed=5, budget_s=20.0):
-    """Time the oracle's restatement of smuthi's CPU radial-lookup matvec
-    (CouplingMatrixRadialLookupCPU.matvec: per-(n1, n2) scipy interp1d 'cubic',
-    gather, phase, dot) on a sample ensemble of the same density, scale the
-    matvec by (N / N_sample)^2 and multiply by the GMRES iteration count."""
-    from oracle import coupling as oc
-    from oracle.system import Sphere
+def incidences():
+    """The 64 right-hand sides of the batching configuration: polar angles pi - theta_j, theta_j = j 60 deg / 31
+    (j = 0..31), azimuth 0, TE and TM."""
+    th = np.deg2rad(np.arange(32) * 60.0 / 31)
+    return [(np.pi - t, 0.0, pol) for t in th for pol in (0, 1)]
+
+
+def _child(args, timeout=600):
+    """Run `python -m oracle.cpu_bench ...` (a child process that never touches the GPU) and parse its line."""
+    import subprocess
+    res = subprocess.run([sys.executable, '-m', 'oracle.cpu_bench'] + [str(a) for a in args], cwd=ROOT,
+                         capture_output=True, text=True, timeout=timeout)
+    if res.returncode != 0:
+        raise RuntimeError('CPU baseline failed: %s' % res.stderr[-2000:])
+    return json.loads(res.stdout.strip().splitlines()[-1])
+
+
+def cpu_baseline(table, rho, n_full, iters_per_solve, workdir, n_sample=400, seed=5):
+    """The oracle's restatement of smuthi's CPU radial-lookup matvec (CouplingMatrixRadialLookupCPU.matvec:
+    per-(n1, n2) scipy interp1d 'cubic', gather, phase, dot) on a sample ensemble of the same density, its 30
+    row groups over all host cores (up to 16, one BLAS thread each); the matvec time is scaled by
+    (N / N_sample)^2 and multiplied by the FP64 GMRES iteration count."""
     from smuthi_amd import ensembles
     side = 60000.0 * np.sqrt(n_sample / 10000.0)
     pp = ensembles.monolayer(n_sample, side, 200.0, seed=seed)
-    plist = [Sphere(p.position, p.refractive_index, p.radius, 3, 3) for p in pp]
-    # the table of the full ensemble covers every distance of the sample
-    loop = oc.RadialLookupCPUReferenceLoop(table, rho, plist, 'cubic')
-    n_unk = 30 * n_sample
-    v = np.random.default_rng(1).standard_normal(n_unk) + 0j
-    rows, t_used = [], 0.0
+    pos = np.array([p.position for p in pp])
+    d = pos[:, None, :2] - pos[None, :, :2]
+    rmax = float(np.sqrt((d ** 2).sum(-1)).max())
+    rows = int(np.searchsorted(rho, rmax + 3 * (rho[1] - rho[0]))) + 3
+    paths = [os.path.join(workdir, f) for f in ('cpu_table.npy', 'cpu_rho.npy', 'cpu_pos.npy')]
+    np.save(paths[0], np.ascontiguousarray(table[:rows]))
+    np.save(paths[1], rho[:rows])
+    np.save(paths[2], pos)
+    workers = max(1, min(16, os.cpu_count() or 1))
+    r = _child(['matvec'] + paths + ['cubic', workers])
+    t_full = r['matvec_seconds'] * (n_full / n_sample) ** 2
+    t_solve = t_full * max(iters_per_solve, 1)
+    return dict(value=1.0 / t_solve, unit='RHS-solves/s', cores=r['workers'], kind='port',
+                sample=('smuthi CPU radial-lookup matvec restated (oracle RadialLookupCPUReferenceLoop, scipy interp1d '
+                        'cubic) on a %d-sphere sample at the same density: all 30 row groups over %d worker processes '
+                        '(1 BLAS thread each; host has %s CPUs, default BLAS threads %s) in %.2f s; scaled by '
+                        '(N/N_sample)^2 to %.1f s per matvec and by %d FP64 GMRES iterations per solve'
+                        % (n_sample, r['workers'], r['cpu_count'], r['blas_threads_default'], r['matvec_seconds'],
+                           t_full, iters_per_solve)),
+                matvec_seconds_est=t_full, host_cpus=r['cpu_count'])
+
+
+def cfg1_cpu_vs_gpu(workdir, tol=1e-5):
+    """The CPU-runnable configuration (50 spheres r = 100 nm, n = 1.5 + 0.01i at z = 200 nm on glass, radial
+    linear 5 nm, one plane wave, GMRES tol 1e-5): smuthi's CPU lookup solve restated (oracle matvec in scipy
+    GMRES, one core) beside the GPU drop-in (FP64 GMRES), same table; solutions compared."""
+    from smuthi_amd import ensembles, fields as flds
+    from smuthi_amd.initial_field import PlaneWave
+    from smuthi_amd.layers import LayerSystem
+    from smuthi_amd.linear_system import LinearSystem
+    plist = ensembles.small_substrate_ensemble(seed=1)
+    ls = LayerSystem([0, 0], [1.52, 1])
+    kp = flds.reasonable_Sommerfeld_kpar_contour(WL, layer_refractive_indices=[1.52, 1], neff_resolution=1e-2)
+    lsys = LinearSystem(plist, PlaneWave(WL, np.pi, 0.0, 0), ls, kp, solver_type='gmres', solver_tolerance=tol,
+                        store_coupling_matrix=False, coupling_matrix_lookup_resolution=5.0, interpolator_kind='linear',
+                        precision='double')
+    lsys.prepare()
     t0 = time.time()
-    for n1 in range(30):
-        loop.matvec(v, n1_subset=[n1])
-        rows.append(n1)
-        t_used = time.time() - t0
-        if t_used > budget_s:
-            break
-    t_matvec_sample = t_used * 30 / len(rows)
-    t_matvec_full = t_matvec_sample * (n_full / n_sample) ** 2
-    t_solve = t_matvec_full * max(iters_per_solve, 1)
-    return dict(value=1.0 / t_solve, unit='RHS-solves/s', cores=1, kind='port',
-                sample=('smuthi CPU radial-lookup matvec restated (oracle RadialLookupCPUReferenceLoop, scipy '
-                        'interp1d cubic), %d of 30 row groups of a %d-sphere sample at the same density, %.1f s; '
-                        'scaled by (N/N_sample)^2 to %.1f s per matvec and by %d GMRES iterations per solve'
-                        % (len(rows), n_sample, t_used, t_matvec_full, iters_per_solve)),
-                matvec_seconds_est=t_matvec_full)
+    x_gpu = lsys.solve()
+    gpu_s = time.time() - t0
+    op = lsys.coupling_matrix
+    paths = [os.path.join(workdir, f) for f in ('cfg1_table.npy', 'cfg1_rho.npy', 'cfg1.npz')]
+    np.save(paths[0], op.lookup_table)
+    np.save(paths[1], op.radial_distance_array)
+    np.savez(paths[2], pos=np.array([p.position for p in plist]), n=np.array([p.refractive_index for p in plist]),
+             r=np.array([p.radius for p in plist]), t=lsys.t_matrix.diagonal, rhs=lsys.t_matrix.right_hand_side())
+    r = _child(['solve'] + paths + ['linear', tol])
+    x_cpu = np.load(paths[2].replace('.npz', '_x.npy'))
+    return dict(cpu_seconds=r['seconds'], cpu_iterations=r['iterations'], cpu_relative_residual=r['relative_residual'],
+                cpu_rhs_solves_per_s=1.0 / r['seconds'], cpu_cores=1, gpu_seconds=gpu_s,
+                gpu_rhs_solves_per_s=1.0 / gpu_s, gpu_iterations=lsys.solve_stats['iterations'],
+                gpu_relative_residual=lsys.solve_stats['relative_residual'],
+                solution_difference=float(np.linalg.norm(x_gpu - x_cpu) / np.linalg.norm(x_cpu)),
+                workload='50 spheres on glass, radial linear 5 nm, FP64 GMRES tol %.0e (scipy on the CPU, GPU '
+                         'drop-in), one plane wave' % tol)
+
+
+def _errors(x, ref):
+    return [float(np.linalg.norm(a - b) / np.linalg.norm(b)) for a, b in zip(x, ref)]
+
+
+def main_headline(args):
+    import tempfile
+    from smuthi_amd import cuda
+    from smuthi_amd.initial_field import PlaneWave
+    from smuthi_amd.linear_system import LinearSystem
+    if not cuda.enable_gpu():
+        raise SystemExit('no MI355X device')
+    dev, n_cu = cuda.device_info()
+    workdir = tempfile.mkdtemp(prefix='smh_bench_')
+
+    plist, ls, kp, side = build_workload(args.particles)
+    log('workload: %d spheres, side %.0f nm, N_kappa %d' % (len(plist), side, len(kp)))
+    inc = incidences()
+    t0 = time.time()
+    base = LinearSystem(plist, PlaneWave(WL, *inc[0]), ls, kp, solver_type='gmres', solver_tolerance=args.tol,
+                        store_coupling_matrix=False, coupling_matrix_lookup_resolution=2.0, interpolator_kind='cubic',
+                        precision='mixed')
+    base.prepare()
+    prep_s = time.time() - t0
+    op = base.coupling_matrix
+    t_diag = base.t_matrix.diagonal
+    log('prepare %.2f s (table %.2f s), len_rho %d' % (prep_s, op.table_seconds, len(op.radial_distance_array)))
+
+    # right-hand sides (setup, outside the timed region): step j solves incidences nb j .. nb j + nb - 1 of the
+    # 64 distinct ones; T is the same for every incidence, rhs = T a
+    nb = args.batch
+    n_rhs = nb * (args.warmup + args.steps)
+    if n_rhs > len(inc):
+        raise SystemExit('at most %d distinct right-hand sides (warmup + steps <= %d)' % (len(inc), len(inc) // nb))
+    rhs = np.stack([t_diag * PlaneWave(WL, *inc[j]).swe_coefficients(plist, ls) for j in range(n_rhs)])
+
+    # coupling matvec kernels (HIP events on the operator's stream)
+    k32, tot32 = op.bench_matvec('single', iters=args.matvec_iters)
+    k64, tot64 = op.bench_matvec('double', iters=max(2, args.matvec_iters // 2))
+    kb32, totb32 = op.bench_matvec_batch('single', iters=max(2, args.matvec_iters // 2))
+    kb16, totb16 = op.bench_matvec_batch('half', iters=max(2, args.matvec_iters // 2))
+    kb64, totb64 = op.bench_matvec_batch('double', iters=2)
+    n_unk = op.shape[0]
+    E = float(n_unk) ** 2
+    flops_1 = E * (4 * 4 + 8 * 1)  # cubic: 4 taps per complex entry, one RHS
+    flops_8 = E * (4 * 4 + 8 * 8)  # the same entries applied to 8 RHS
+    half_terms, half_lin_ratio = op.half_interpolation()
+    flops_16 = E * (4 * half_terms + 8 * 8)
+    log('matvec FP32 %.2f ms, FP64 %.2f ms; 8-RHS FP16 %.2f ms, FP32 %.2f ms, FP64 %.2f ms'
+        % (k32, k64, kb16, kb32, kb64))
+
+    def solve(rows, precision, tol):
+        x, st, ok = op.solve_master(t_diag, rhs[rows], tol, precision=precision, with_stats=True)
+        return x, st, ok
+
+    # timed region: batched solves at the requested inner precision, distinct incidences every step
+    prec = args.precision
+    for w in range(args.warmup):
+        solve(slice(nb * w, nb * w + nb), prec, args.tol)
+    t0 = time.time()
+    sols, stats = [], []
+    for stp in range(args.warmup, args.warmup + args.steps):
+        x, stt, _ = solve(slice(nb * stp, nb * stp + nb), prec, args.tol)
+        sols.append(x)
+        stats.append(stt)
+    elapsed = time.time() - t0
+    timed = slice(nb * args.warmup, n_rhs)
+    x_timed = np.concatenate(sols)
+    # every timed right-hand side against an FP64 solve of it (FP64 GMRES to 1e-10)
+    t0 = time.time()
+    x_ref, ref_stats, _ = solve(timed, 'double', 1e-10)
+    log('FP64 reference solves (tol 1e-10) of %d RHS: %.2f s' % (len(x_ref), time.time() - t0))
+    errs = _errors(x_timed, x_ref)
+    worst_res = max(s['relative_residual'] for s in stats)
+    n_ok = sum(1 for e in errs if e <= 1e-5) if worst_res <= args.tol else 0
+    value = n_ok / elapsed
+    log('batched %s solves: %.2f s for %d RHS, worst residual %.2e, worst coefficient error vs FP64 %.2e'
+        % (prec, elapsed, len(errs), worst_res, max(errs)))
+
+    # baselines on the same incidences: plain FP64 GMRES, batched (8 RHS per launch) and one RHS at a time
+    # (smuthi's way), and the FP32-inner refinement
+    first = slice(nb * args.warmup, nb * args.warmup + nb)
+    t0 = time.time()
+    _, st64, _ = solve(first, 'double', args.tol)
+    fp64_batched_s = time.time() - t0
+    t0 = time.time()
+    _, st64s, _ = op.solve_master(t_diag, rhs[nb * args.warmup], args.tol, precision='double', with_stats=True)
+    fp64_single_s = time.time() - t0
+    t0 = time.time()
+    x32, st32, _ = solve(first, 'mixed', args.tol)
+    mixed_s = time.time() - t0
+    e32 = _errors(x32, x_ref[:nb])
+    fp64_batched_rate = nb / fp64_batched_s
+    log('FP64 GMRES batched %.2f s per %d RHS, single %.2f s; FP32-inner refinement %.2f s per %d RHS'
+        % (fp64_batched_s, nb, fp64_single_s, mixed_s, nb))
+
+    cpu = cfg1 = None
+    if not args.no_cpu_baseline:
+        t0 = time.time()
+        cpu = cpu_baseline(op.lookup_table, op.radial_distance_array, len(plist), st64s['iterations'], workdir)
+        log('cpu baseline %.1f s' % (time.time() - t0))
+        t0 = time.time()
+        cfg1 = cfg1_cpu_vs_gpu(workdir)
+        log('cfg1 CPU vs GPU %.1f s' % (time.time() - t0))
+
+    half = prec == 'mixed16'
+    kern_ms = kb16 if half else (kb32 if prec == 'mixed' else kb64)
+    flops_k = flops_16 if half else flops_8
+    achieved = flops_k / (kern_ms * 1e-3) / 1e12
+    peak = MFMA_FP16_PEAK_TFLOPS if half else (MFMA_FP32_PEAK_TFLOPS if prec == 'mixed' else MFMA_FP64_PEAK_TFLOPS)
+    out = {
+        'metric': 'RHS-solves/s at 10k spheres, err<=1e-5 vs FP64; coupling matvec GB/s',
+        'value': value,
+        'unit': 'RHS-solves/s',
+        'n_gpus': 1,
+        'steps': args.steps,
+        'warmup': args.warmup,
+        'ms_per_step': elapsed / args.steps * 1e3,
+        'higher_is_better': True,
+        'scaling': 'weak',
+        'vs_baseline': None,
+        'dtype': ('fp16 operands / fp32 accumulation coupling matvec (MFMA), fp32 sweep residuals, '
+                  'fp64 refinement/Arnoldi/certifying residual' if half else
+                  'fp32 coupling matvec (MFMA), fp64 refinement/Arnoldi/residual'),
+        'data': 'synthetic',
+        'config': {
+            'workload': ('radial-lookup coupling solve, %d spheres monolayer on glass, l_max=3, cubic spline 2 nm, '
+                         'GMRES tol %.0e, %s-in-FP64 refinement, %d distinct plane waves per step (polar angles '
+                         '0-60 deg x TE/TM)' % (len(plist), args.tol,
+                                               'FP16' if half else ('FP32' if prec == 'mixed' else 'no'), nb)),
+            'inner_precision': prec,
+            'particles': len(plist), 'unknowns': n_unk, 'lookup_rows': int(len(op.radial_distance_array)),
+            'n_kappa': int(len(kp)), 'side_nm': side, 'seed': 2, 'tolerance': args.tol,
+            'interpolation': 'cubic (not-a-knot spline)', 'resolution_nm': 2.0, 'rhs_per_step': nb,
+        },
+        'coupling_matvec_gbps_equiv_dense': E * 8 / (k32 * 1e-3) / 1e9,
+        'matvec_ms': {'fp32_kernel': k32, 'fp32_total': tot32, 'fp64_kernel': k64, 'fp64_total': tot64,
+                      'fp16_kernel_8rhs': kb16, 'fp16_total_8rhs': totb16,
+                      'fp32_kernel_8rhs': kb32, 'fp32_total_8rhs': totb32, 'fp64_kernel_8rhs': kb64,
+                      'fp64_total_8rhs': totb64},
+        'rhs_checked': len(errs), 'rhs_within_1e-5': n_ok,
+        'coef_err_vs_fp64_worst': max(errs), 'coef_err_vs_fp64_median': float(np.median(errs)),
+        'worst_relative_residual': worst_res,
+        'solve_stats': stats,
+        'fp64_reference_stats': ref_stats,
+        'fp64_gmres_batched': {'seconds_per_batch': fp64_batched_s, 'rhs_solves_per_s': fp64_batched_rate, **st64},
+        'fp64_gmres_single_rhs': {'seconds': fp64_single_s, 'rhs_solves_per_s': 1.0 / fp64_single_s, **st64s},
+        'mixed_fp32_inner': {'seconds_per_batch': mixed_s, 'rhs_solves_per_s': nb / mixed_s,
+                             'coef_err_vs_fp64_worst': max(e32), **st32},
+        'speedup_vs_fp64_gmres_batched': value / fp64_batched_rate,
+        'speedup_vs_fp64_gmres_single_rhs': value * fp64_single_s,
+        'prepare_seconds': prep_s,
+        'table_seconds': op.table_seconds,
+        'roofline': {'bound': 'valu-issue (VALU busy ~71 %, MFMA ~9 % in the SQ counters)' if half else 'mfma',
+                     'achieved': achieved, 'peak': peak, 'unit': 'TFLOP/s',
+                     'frac': achieved / peak,
+                     'traffic': measured_traffic('batch16' if half else 'batch'),
+                     'kernel': ('batch_half_kernel<3,3,%d>' % half_terms if half else 'batch_slots_kernel<3,3,4>'),
+                     'flops_per_launch': flops_k},
+        'fp16_inner_operator': {'interpolation': ('linear between table nodes' if half_terms == 2
+                                                  else 'the operator\'s own (%d Horner terms)' % half_terms),
+                                'linear_error_over_bin_scale': half_lin_ratio},
+        'roofline_fp32_8rhs_kernel': {'kernel': 'batch_slots_kernel<3,3,4>', 'ms': kb32,
+                                      'achieved': flops_8 / (kb32 * 1e-3) / 1e12, 'peak': MFMA_FP32_PEAK_TFLOPS,
+                                      'frac': flops_8 / (kb32 * 1e-3) / 1e12 / MFMA_FP32_PEAK_TFLOPS,
+                                      'traffic': measured_traffic('batch')},
+        'roofline_fp64_8rhs_kernel': {'kernel': 'batch_slots64_kernel<3,3,4>', 'ms': kb64,
+                                      'achieved': flops_8 / (kb64 * 1e-3) / 1e12, 'peak': MFMA_FP64_PEAK_TFLOPS,
+                                      'frac': flops_8 / (kb64 * 1e-3) / 1e12 / MFMA_FP64_PEAK_TFLOPS,
+                                      'traffic': measured_traffic('batch64')},
+        'roofline_single_rhs_kernel': {'achieved': flops_1 / (k32 * 1e-3) / 1e12, 'peak': MFMA_FP32_PEAK_TFLOPS,
+                                       'frac': flops_1 / (k32 * 1e-3) / 1e12 / MFMA_FP32_PEAK_TFLOPS,
+                                       'traffic': measured_traffic('single')},
+        'cpu_baseline': cpu,
+        'cfg1_cpu_vs_gpu': cfg1,
+        'device': dev, 'compute_units': n_cu, 'host': platform.node(), 'host_cpus': os.cpu_count(),
+    }
+    return json.dumps(out)
+
+
+def main_batching(args):
+    """Batching configuration: ensembles of uneven size (log-uniform) in a 1 um film (n = 1.8) on glass, heights
+    U(150, 600) nm, one shared volume lookup table (Catmull-Rom, the reference's CUDA 'cubic'), 8 of the 64
+    plane waves per step solved for every ensemble at once (FP32-inner refinement, FP64 certificate), every
+    (ensemble, field) system checked against an FP64 solve."""
+    from smuthi_amd import cuda, ensembles, fields as flds
+    from smuthi_amd.batch import EnsembleBatch
+    from smuthi_amd.initial_field import PlaneWave
+    from smuthi_amd.layers import LayerSystem
+    if not cuda.enable_gpu():
+        raise SystemExit('no MI355X device')
+    dev, n_cu = cuda.device_info()
+    sizes = ensembles.batching_sizes(args.ensembles, seed=4, low=args.min_size, high=args.max_size)
+    t0 = time.time()
+    ens = [ensembles.batching_ensemble(n, 400 + k) for k, n in enumerate(sizes)]
+    ls = LayerSystem([0, 1000, 0], [1.52, 1.8, 1])
+    kp = flds.reasonable_Sommerfeld_kpar_contour(WL, layer_refractive_indices=[1.52, 1.8, 1], neff_resolution=1e-2)
+    batch = EnsembleBatch(WL, ens, ls, k_parallel=kp, resolution=args.resolution, interpolator_kind='cubic',
+                          cubic_interpolant='catmull-rom')
+    prep = time.time() - t0
+    log('%d ensembles (%d..%d spheres, %d total), table %.1f s, setup %.1f s'
+        % (len(sizes), min(sizes), max(sizes), sum(sizes), batch.table_seconds, prep))
+    inc = incidences()
+    nb = 8
+    n_fields = nb * (args.warmup + args.steps)
+    t0 = time.time()
+    rhs = batch.right_hand_sides([PlaneWave(WL, *inc[j]) for j in range(n_fields)])
+    log('right-hand sides %.1f s' % (time.time() - t0))
+    op = batch.operator
+    k32, tot32 = op.bench_matvec_batch('single', iters=2)
+    k64, tot64 = op.bench_matvec_batch('double', iters=2)
+    for w in range(args.warmup):
+        batch.solve_with_stats(rhs[nb * w:nb * w + nb], tol=args.tol, precision='mixed')
+    t0 = time.time()
+    xs, stats = [], []
+    for stp in range(args.warmup, args.warmup + args.steps):
+        x, st, ok = batch.solve_with_stats(rhs[nb * stp:nb * stp + nb], tol=args.tol, precision='mixed')
+        xs.append(x)
+        stats.append(st)
+    elapsed = time.time() - t0
+    timed = slice(nb * args.warmup, n_fields)
+    t0 = time.time()
+    x_ref, ref_st, _ = batch.solve_with_stats(rhs[timed], tol=1e-10, precision='double')
+    ref_s = time.time() - t0
+    x = np.concatenate(xs)
+    errs = [e for a, b in zip(batch.split(x), batch.split(x_ref)) for e in _errors(a, b)]
+    worst_res = max(s['relative_residual'] for s in stats)
+    n_ok = sum(1 for e in errs if e <= 1e-5) if worst_res <= args.tol else 0
+    t0 = time.time()
+    _, st64, _ = batch.solve_with_stats(rhs[nb * args.warmup:nb * args.warmup + nb], tol=args.tol,
+                                        precision='double')
+    fp64_s = time.time() - t0
+    systems_per_step = nb * len(sizes)
+    value = n_ok / elapsed
+    E = float(sum((30 * n) ** 2 for n in sizes))
+    flops = E * (4 * 32 + 8 * nb)  # volume bicubic: 2 tables x 16 taps per complex entry
+    achieved = flops / (k32 * 1e-3) / 1e12
+    log('batched solves: %.2f s for %d systems, worst residual %.2e, worst error %.2e; FP64 GMRES %.2f s per step'
+        % (elapsed, len(errs), worst_res, max(errs), fp64_s))
+    out = {
+        'metric': 'RHS-solves/s (ensemble x plane wave systems), err<=1e-5 vs FP64',
+        'value': value, 'unit': 'RHS-solves/s', 'n_gpus': 1, 'steps': args.steps, 'warmup': args.warmup,
+        'ms_per_step': elapsed / args.steps * 1e3, 'higher_is_better': True, 'scaling': 'weak', 'vs_baseline': None,
+        'dtype': 'fp32 coupling matvec (MFMA), fp64 refinement/Arnoldi/certifying residual', 'data': 'synthetic',
+        'config': {'workload': ('%d ensembles of %d-%d spheres (log-uniform, seed 4) in a 1 um film (n=1.8) on glass, '
+                                'z U(150,600) nm, one volume table (Catmull-Rom, %.0f nm), 8 plane waves per step '
+                                'for every ensemble, tol %.0e' % (len(sizes), min(sizes), max(sizes),
+                                                                   args.resolution, args.tol)),
+                   'ensembles': len(sizes), 'sizes': sizes, 'particles': sum(sizes),
+                   'systems_per_step': systems_per_step, 'resolution_nm': args.resolution,
+                   'table_rho_rows': int(len(batch.rho_array)), 'table_z_rows': int(len(batch.sum_z_array)),
+                   'n_kappa': int(len(kp))},
+        'systems_checked': len(errs), 'systems_within_1e-5': n_ok, 'coef_err_vs_fp64_worst': max(errs),
+        'worst_relative_residual': worst_res, 'solve_stats': stats, 'fp64_reference_seconds': ref_s,
+        'fp64_reference_stats': ref_st,
+        'fp64_gmres_batched': {'seconds_per_step': fp64_s, 'rhs_solves_per_s': systems_per_step / fp64_s, **st64},
+        'speedup_vs_fp64_gmres_batched': value * fp64_s / systems_per_step,
+        'matvec_ms': {'fp32_8rhs_all_ensembles': k32, 'fp32_8rhs_total': tot32, 'fp64_8rhs_all_ensembles': k64,
+                      'fp64_8rhs_total': tot64},
+        'roofline': {'bound': 'mfma', 'achieved': achieved, 'peak': MFMA_FP32_PEAK_TFLOPS, 'unit': 'TFLOP/s',
+                     'frac': achieved / MFMA_FP32_PEAK_TFLOPS, 'traffic': None,
+                     'kernel': 'batch_matvec_kernel<float,float,3,3,4,true> (8 launches: 2 tables x 4 z slices)',
+                     'flops_per_matvec': flops},
+        'table_seconds': batch.table_seconds, 'setup_seconds': prep,
+        'device': dev, 'compute_units': n_cu,
+    }
+    return json.dumps(out)
+
+
+def main_learned(args):
+    """Learned near-field preconditioner: polydisperse spheres (r 60-180 nm, n 1.4-2.4 + 0-0.05i) inside the
+    500 nm film (n = 2.0) of air / film / glass, volume lookup (Catmull-Rom). The network (K = 16 neighbours,
+    hidden 256, BF16 weights) is trained on the device on solver solutions of training ensembles (random and
+    plane-wave right-hand sides), then a held-out ensemble is solved for 16 plane waves (8 polar angles x
+    TE/TM) with plain GMRES, the untrained first-order structure and the trained network (plain GMRES as
+    fallback); every solution is certified by its FP64 residual and checked against an FP64 solve."""
+    from smuthi_amd import cuda, ensembles, fields as flds
+    from smuthi_amd.batch import EnsembleBatch
+    from smuthi_amd.initial_field import PlaneWave
+    from smuthi_amd.layers import LayerSystem
+    from smuthi_amd.learned import LearnedPreconditioner, solve_with_fallback
+    if not cuda.enable_gpu():
+        raise SystemExit('no MI355X device')
+    dev, n_cu = cuda.device_info()
+    stack = ([0, 500, 0], [1.52, 2.0, 1])
+    ls = LayerSystem(*stack)
+    kp = flds.reasonable_Sommerfeld_kpar_contour(WL, layer_refractive_indices=stack[1], neff_resolution=1e-2)
+    rng = np.random.default_rng(5)
+
+    def make(sizes, seed0):
+        ens = [ensembles.learned_ensemble(n, seed0 + k) for k, n in enumerate(sizes)]
+        return EnsembleBatch(WL, ens, ls, k_parallel=kp, resolution=args.resolution, interpolator_kind='cubic',
+                             cubic_interpolant='catmull-rom')
+
+    def prepare(lp, b):
+        parts = b.particle_list
+        return lp.prepare(b.operator, np.array([p.position for p in parts]), [p.radius for p in parts],
+                          [p.refractive_index for p in parts], 0.0, 500.0, WL, b.t_diagonal,
+                          ensemble_offsets=b.particle_offsets)
+
+    lp = LearnedPreconditioner(K=16, hidden=256, l_max=3, seed=0)
+    t0 = time.time()
+    train = make([args.train_size] * args.train_ensembles, 500)
+    _, feat = prepare(lp, train)
+    th = np.deg2rad(np.linspace(0, 60, 8))
+    fields = [PlaneWave(WL, np.pi - t, 0.2, pol) for t in th for pol in (0, 1)]
+    n = train.n_unknowns
+    R = np.concatenate([rng.standard_normal((8, n)) + 1j * rng.standard_normal((8, n)),
+                        train.right_hand_sides(fields[::2])])
+    X, stx, _ = train.solve_with_stats(R, tol=1e-8, precision='mixed')
+    lp.add_training_ensemble(train.operator, feat, R, X)
+    data_s = time.time() - t0
+    t0 = time.time()
+    L0 = lp.data_loss()[0]
+    lp.train(args.train_steps, lr=1e-3)
+    L1 = lp.data_loss()[0]
+    train_s = time.time() - t0
+    log('training data %.1f s, training %d steps %.1f s, loss %.4e -> %.4e' % (data_s, args.train_steps, train_s, L0,
+                                                                             L1))
+    del train
+    test = make([args.test_size], 900)
+    _, feat_t = prepare(lp, test)
+    rhs = test.right_hand_sides(fields)
+    x_ref, _, _ = test.solve_with_stats(rhs, tol=1e-10, precision='double')
+    res = {}
+    for name in ('plain', 'first_order', 'learned'):
+        if name == 'first_order':
+            lp.apply_prior(test.operator)
+        elif name == 'learned':
+            lp.apply(test.operator, feat_t)
+        t0 = time.time()
+        x, st, fb = solve_with_fallback(test.operator, test.t_diagonal, rhs, args.tol)
+        secs = time.time() - t0
+        errs = _errors(x, x_ref)
+        res[name] = {'seconds': secs, 'rhs_solves_per_s': len(rhs) / secs, 'iterations': st['iterations'],
+                     'fallback': bool(fb), 'worst_relative_residual': st['relative_residual'],
+                     'coef_err_vs_fp64_worst': max(errs), 'stats': st}
+        log('%s: %.2f s, %d iterations, fallback %s, error %.2e' % (name, secs, st['iterations'], fb, max(errs)))
+    value = res['learned']['rhs_solves_per_s'] if res['learned']['coef_err_vs_fp64_worst'] <= 1e-5 else 0.0
+    out = {
+        'metric': 'RHS-solves/s with the learned preconditioner (held-out ensemble), err<=1e-5 vs FP64',
+        'value': value, 'unit': 'RHS-solves/s', 'n_gpus': 1, 'higher_is_better': True, 'vs_baseline': None,
+        'dtype': 'fp32 coupling matvec, fp64 refinement/Arnoldi/preconditioner/residual; network bf16 weights, '
+                 'fp32 accumulation', 'data': 'synthetic',
+        'config': {'workload': ('held-out ensemble of %d polydisperse spheres in a 500 nm film (n=2.0) of air/film/'
+                                'glass, volume lookup (Catmull-Rom, %.0f nm), 16 plane waves, tol %.0e; trained on '
+                                '%d ensembles of %d spheres x 16 right-hand sides'
+                                % (args.test_size, args.resolution, args.tol, args.train_ensembles, args.train_size)),
+                   'K': 16, 'hidden': 256, 'train_steps': args.train_steps},
+        'iterations': {k: v['iterations'] for k, v in res.items()},
+        'fallback_rate': sum(v['fallback'] for v in res.values() if v is not res['plain']) / 2.0,
+        'speedup_vs_plain': res['learned']['rhs_solves_per_s'] / res['plain']['rhs_solves_per_s'],
+        'results': res, 'training': {'loss_untrained': L0, 'loss_trained': L1, 'seconds': train_s,
+                                     'data_seconds': data_s, 'rows': args.train_size * args.train_ensembles,
+                                     'solve_stats': stx},
+        'device': dev, 'compute_units': n_cu,
+    }
+    return json.dumps(out)
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument('--gpus', type=int, default=1)
-    ap.add_argument('--steps', type=int, default=2)
+    ap.add_argument('--steps', type=int, default=4)
     ap.add_argument('--warmup', type=int, default=1)
     ap.add_argument('--particles', type=int, default=10000)
     ap.add_argument('--tol', type=float, default=1e-6)
@@ -364,6 +771,16 @@ def main():
                     help='inner matvec precision of the timed solves (mixed16: FP16 operands, mixed: FP32)')
     ap.add_argument('--no-cpu-baseline', action='store_true')
     ap.add_argument('--matvec-iters', type=int, default=5)
+    ap.add_argument('--config', type=int, default=2, choices=[2, 4, 5],
+                    help='2: the 10k-sphere solve (default); 4: ensembles batched; 5: learned preconditioner')
+    ap.add_argument('--ensembles', type=int, default=32)
+    ap.add_argument('--min-size', type=int, default=100)
+    ap.add_argument('--max-size', type=int, default=2000)
+    ap.add_argument('--resolution', type=float, default=5.0)
+    ap.add_argument('--train-ensembles', type=int, default=4)
+    ap.add_argument('--train-size', type=int, default=1500)
+    ap.add_argument('--test-size', type=int, default=5000)
+    ap.add_argument('--train-steps', type=int, default=400)
     ap.add_argument('--traffic-probe', action='store_true',
                     help='only run the copy calibration and three FP32 matvecs (for counter passes)')
     ap.add_argument('--explicit', type=int, default=0, metavar='N',
@@ -385,173 +802,11 @@ def main():
         return explicit_bench(args.explicit)
     if args.rhs:
         return rhs_bench(args.particles)
-
-    from smuthi_amd import cuda
-    from smuthi_amd.initial_field import PlaneWave
-    from smuthi_amd.linear_system import LinearSystem
-    if not cuda.enable_gpu():
-        raise SystemExit('no MI355X device')
-    dev, n_cu = cuda.device_info()
-
-    plist, ls, kp, side = build_workload(args.particles)
-    log('workload: %d spheres, side %.0f nm, N_kappa %d' % (len(plist), side, len(kp)))
-
-    def make_system(j, precision, tol):
-        polar, az, pol = rhs_for_step(j)
-        return LinearSystem(plist, PlaneWave(WL, polar, az, pol), ls, kp, solver_type='gmres', solver_tolerance=tol,
-                            store_coupling_matrix=False, coupling_matrix_lookup_resolution=2.0, interpolator_kind='cubic',
-                            precision=precision)
-
-    t0 = time.time()
-    base = make_system(0, 'mixed', args.tol)
-    base.prepare()
-    prep_s = time.time() - t0
-    op = base.coupling_matrix
-    log('prepare %.2f s (table %.2f s), len_rho %d' % (prep_s, op.table_seconds, len(op.radial_distance_array)))
-
-    # right-hand sides of all steps (setup, outside the timed region): a step is a
-    # batch of `batch` incidence directions / polarisations
-    nb = args.batch
-    # (T is the same for every incidence: rhs_j = T a_j)
-    rhs = []
-    for j in range(nb * (args.warmup + args.steps)):
-        polar, az, pol = rhs_for_step(j)
-        rhs.append(base.t_matrix.diagonal * PlaneWave(WL, polar, az, pol).swe_coefficients(plist, ls))
-    rhs = np.stack(rhs)
-
-    # coupling matvec kernels (HIP events on the operator's stream)
-    k32, tot32 = op.bench_matvec('single', iters=args.matvec_iters)
-    k64, tot64 = op.bench_matvec('double', iters=max(2, args.matvec_iters // 2))
-    kb32, totb32 = op.bench_matvec_batch('single', iters=max(2, args.matvec_iters // 2))
-    kb16, totb16 = op.bench_matvec_batch('half', iters=max(2, args.matvec_iters // 2))
-    kb64, totb64 = op.bench_matvec_batch('double', iters=2)
-    n_unk = op.shape[0]
-    E = float(n_unk) ** 2
-    flops_1 = E * (4 * 4 + 8 * 1)  # cubic: 4 taps per complex entry, one RHS
-    flops_8 = E * (4 * 4 + 8 * 8)  # the same entries applied to 8 RHS
-    # the FP16 inner operator interpolates with its own number of Horner terms (2: linear between nodes)
-    half_terms, half_lin_ratio = op.half_interpolation()
-    flops_16 = E * (4 * half_terms + 8 * 8)
-    log('matvec FP32 %.2f ms, FP64 %.2f ms; 8-RHS FP16 %.2f ms, FP32 %.2f ms, FP64 %.2f ms'
-        % (k32, k64, kb16, kb32, kb64))
-
-    def solve(j0, count, precision, tol):
-        sysj = make_system(j0, precision, tol)
-        sysj.t_matrix = base.t_matrix
-        sysj.coupling_matrix = op
-        x = sysj.solve(rhs[j0:j0 + count])
-        return x, dict(sysj.solve_stats)
-
-    # FP64 reference for right-hand side 0 and the plain FP64 GMRES throughput (one RHS)
-    t0 = time.time()
-    x_ref = solve(0, 1, 'double', 1e-9)[0][0]
-    log('FP64 reference solve (tol 1e-9): %.2f s' % (time.time() - t0))
-    t0 = time.time()
-    _, plain_stats = solve(0, 1, 'double', args.tol)
-    fp64_plain_s = time.time() - t0
-    log('plain FP64 GMRES (tol %.0e): %.2f s, %d iterations' % (args.tol, fp64_plain_s, plain_stats['iterations']))
-    # one mixed-precision solve of a single RHS (no batching) for comparison
-    t0 = time.time()
-    x_single, single_stats = solve(0, 1, 'mixed', args.tol)
-    single_s = time.time() - t0
-    log('single-RHS mixed solve: %.2f s' % single_s)
-
-    # timed region: batched solves at the requested inner precision
-    prec = args.precision
-    for w in range(args.warmup):
-        solve(nb * w, nb, prec, args.tol)
-    t0 = time.time()
-    sols, stats = [], []
-    for st in range(args.warmup, args.warmup + args.steps):
-        x, stt = solve(nb * st, nb, prec, args.tol)
-        sols.append(x)
-        stats.append(stt)
-    elapsed = time.time() - t0
-    # accuracy: RHS 0 solved inside a batch vs the FP64 solution
-    x_b0 = solve(0, nb, prec, args.tol)[0][0]
-    coef_err = float(np.linalg.norm(x_b0 - x_ref) / np.linalg.norm(x_ref))
-    single_err = float(np.linalg.norm(x_single[0] - x_ref) / np.linalg.norm(x_ref))
-    worst_res = max(s['relative_residual'] for s in stats)
-    ok = coef_err <= 1e-5 and worst_res <= args.tol
-    n_solved = nb * args.steps
-    value = n_solved / elapsed if ok else 0.0
-    log('batched %s solves: %.2f s for %d RHS, residual %.2e, coefficient error vs FP64 %.2e'
-        % (prec, elapsed, n_solved, worst_res, coef_err))
-
-    cpu = None
-    if not args.no_cpu_baseline:
-        t0 = time.time()
-        cpu = cpu_baseline(op.lookup_table, op.radial_distance_array, len(plist), plain_stats['iterations'])
-        log('cpu baseline %.1f s' % (time.time() - t0))
-
-    batched = nb > 1
-    half = prec == 'mixed16'
-    kern_ms = (kb16 if half else kb32) if batched else k32
-    flops_k = ((flops_16 if half else flops_8) if batched else flops_1)
-    achieved = flops_k / (kern_ms * 1e-3) / 1e12
-    peak = MFMA_FP16_PEAK_TFLOPS if (half and batched) else MFMA_FP32_PEAK_TFLOPS
-    out = {
-        'metric': 'RHS-solves/s at 10k spheres, err<=1e-5 vs FP64; coupling matvec GB/s',
-        'value': value,
-        'unit': 'RHS-solves/s',
-        'n_gpus': 1,
-        'steps': args.steps,
-        'warmup': args.warmup,
-        'ms_per_step': elapsed / args.steps * 1e3,
-        'higher_is_better': True,
-        'scaling': 'weak',
-        'vs_baseline': None,
-        'dtype': ('fp16 operands / fp32 accumulation coupling matvec (MFMA), fp32 sweep residuals, '
-                  'fp64 refinement/Arnoldi/certifying residual' if half else
-                  'fp32 coupling matvec (MFMA), fp64 refinement/Arnoldi/residual'),
-        'data': 'synthetic',
-        'config': {
-            'workload': ('radial-lookup coupling solve, %d spheres monolayer on glass, l_max=3, cubic spline 2 nm, '
-                         'GMRES tol %.0e, %s-in-FP64 refinement, %d right-hand sides (plane waves) per step'
-                         % (len(plist), args.tol, 'FP16' if half else ('FP32' if prec == 'mixed' else 'no'), nb)),
-            'inner_precision': prec,
-            'particles': len(plist), 'unknowns': n_unk, 'lookup_rows': int(len(op.radial_distance_array)),
-            'n_kappa': int(len(kp)), 'side_nm': side, 'seed': 2, 'tolerance': args.tol,
-            'interpolation': 'cubic (not-a-knot spline)', 'resolution_nm': 2.0, 'rhs_per_step': nb,
-        },
-        'coupling_matvec_gbps_equiv_dense': E * 8 / (k32 * 1e-3) / 1e9,
-        'matvec_ms': {'fp32_kernel': k32, 'fp32_total': tot32, 'fp64_kernel': k64, 'fp64_total': tot64,
-                      'fp16_kernel_8rhs': kb16, 'fp16_total_8rhs': totb16,
-                      'fp32_kernel_8rhs': kb32, 'fp32_total_8rhs': totb32, 'fp64_kernel_8rhs': kb64,
-                      'fp64_total_8rhs': totb64},
-        'coef_err_vs_fp64': coef_err,
-        'worst_relative_residual': worst_res,
-        'solve_stats': stats,
-        'single_rhs_mixed': {'seconds': single_s, 'rhs_solves_per_s': 1.0 / single_s, 'coef_err_vs_fp64': single_err,
-                             **single_stats},
-        'fp64_plain_gmres': {'seconds': fp64_plain_s, 'rhs_solves_per_s': 1.0 / fp64_plain_s, **plain_stats},
-        'speedup_vs_fp64_gmres': (value * fp64_plain_s) if value else None,
-        'prepare_seconds': prep_s,
-        'table_seconds': op.table_seconds,
-        'roofline': {'bound': 'mfma', 'achieved': achieved, 'peak': peak, 'unit': 'TFLOP/s',
-                     'frac': achieved / peak,
-                     'traffic': measured_traffic(('batch16' if half else 'batch') if batched else 'single'),
-                     'kernel': (('batch_half_kernel<3,3,%d>' % half_terms if half else 'batch_slots_kernel<3,3,4>')
-                                if batched else 'lookup_matvec_kernel<float,float,3,3,4,false>'),
-                     'flops_per_launch': flops_k},
-        'fp16_inner_operator': {'interpolation': ('linear between table nodes' if half_terms == 2
-                                                  else 'the operator\'s own (%d Horner terms)' % half_terms),
-                                'linear_error_over_bin_scale': half_lin_ratio},
-        'roofline_fp32_8rhs_kernel': {'kernel': 'batch_slots_kernel<3,3,4>', 'ms': kb32,
-                                      'achieved': flops_8 / (kb32 * 1e-3) / 1e12, 'peak': MFMA_FP32_PEAK_TFLOPS,
-                                      'frac': flops_8 / (kb32 * 1e-3) / 1e12 / MFMA_FP32_PEAK_TFLOPS,
-                                      'traffic': measured_traffic('batch')},
-        'roofline_fp64_8rhs_kernel': {'kernel': 'batch_slots64_kernel<3,3,4>', 'ms': kb64,
-                                      'achieved': flops_8 / (kb64 * 1e-3) / 1e12, 'peak': MFMA_FP64_PEAK_TFLOPS,
-                                      'frac': flops_8 / (kb64 * 1e-3) / 1e12 / MFMA_FP64_PEAK_TFLOPS,
-                                      'traffic': measured_traffic('batch64')},
-        'roofline_single_rhs_kernel': {'achieved': flops_1 / (k32 * 1e-3) / 1e12, 'peak': MFMA_FP32_PEAK_TFLOPS,
-                                       'frac': flops_1 / (k32 * 1e-3) / 1e12 / MFMA_FP32_PEAK_TFLOPS,
-                                       'traffic': measured_traffic('single')},
-        'cpu_baseline': cpu,
-        'device': dev, 'compute_units': n_cu, 'host': platform.node(), 'host_cpus': os.cpu_count(),
-    }
-    return json.dumps(out)
+    if args.config == 4:
+        return main_batching(args)
+    if args.config == 5:
+        return main_learned(args)
+    return main_headline(args)
 
 
 if __name__ == '__main__':

@@ -93,15 +93,63 @@ int smh_volume_create_ensembles(smh_ctx *ctx, int n_ens, const int64_t *ens_off,
                                 const int *mmax, int l_max, int m_max, double rho_min, double sz_min, double dz_min,
                                 double resolution, int64_t len_rho, int64_t len_z, int interp, smh_op **out);
 
-/* Right block preconditioner for smh_solve (no counterpart in smuthi, whose
- * GMRES is unpreconditioned, linear_system.py:271-286): blocks = complex128
- * [n_part][B][B], B = blocksize(l_max, m_max) of the operator, on the padded
- * multipole index (rows and columns of multipoles a particle does not have
- * are ignored: those vector entries stay zero). smh_solve then runs GMRES on
- * M P y = b and returns x = P y, with the same FP64 stopping rule on
- * ||b - M x||. NULL removes it. Needs the batched kernels (radial or volume
- * operators). */
-int smh_op_set_preconditioner(smh_op *op, const double *blocks);
+/* Right block-sparse preconditioner for smh_solve (no counterpart in smuthi,
+ * whose GMRES is unpreconditioned, linear_system.py:271-286):
+ *   (P v)_i = sum_{k < k1} blocks[i][k] v_{nbr[i][k]}
+ * blocks: complex128 [n_part][k1][B][B], B = blocksize(l_max, m_max) of the
+ * operator, on the padded multipole index (entries of multipoles a particle
+ * does not have act on zeros); nbr: [n_part][k1] emitter particles (-1: empty
+ * slot); k1 = 1 with nbr[i][0] = i is a block-diagonal preconditioner.
+ * smh_solve then runs GMRES on M P y = b and returns x = P y, with the same
+ * FP64 stopping rule on ||b - M x||. blocks = NULL removes it. Needs the
+ * batched kernels (radial or volume operators). */
+int smh_op_set_preconditioner(smh_op *op, int k1, const int *nbr, const double *blocks);
+
+/* ---- learned near-field preconditioner (no counterpart in smuthi) ---------
+ * Near-field blocks A_ij = diag(t_i) W_ij of every particle i and the
+ * emitters nbr[i][k] (k < k1, nbr[i][0] = i, -1: empty; the K nearest
+ * neighbours in the caller's use), W_ij the block the operator's lookup
+ * applies to that pair (the taps and weights of its FP64 matvec, phases
+ * e^{i (m2 - m1) phi} included), t_diag: complex128 [n_unk] T-matrix
+ * diagonal in smuthi's layout. Kept on the operator; the download returns
+ * complex128 [n_part][k1][B][B] on the padded index. */
+int smh_op_near_blocks(smh_op *op, int k1, const int *nbr, const double *t_diag);
+int smh_op_download_near_blocks(smh_op *op, double *out);
+/* Preconditioner from the near blocks and per-channel scales:
+ *   P[i][k] = [k == 0] 1 + diag(s[i][k][ch(n1)]) A[i][k],
+ * ch(n) = tau l_max + l - 1 the (tau, l) channel of the receiving multipole;
+ * s: complex64 as float [n_part][k1][2 l_max][2]. s = 1 everywhere is the
+ * first-order near-field inverse of M = 1 - T W. NULL removes it. */
+int smh_op_set_near_scaling(smh_op *op, const float *s);
+
+/* The network that predicts the scales: x -> relu(x W1 + b1) -> relu(. W2 +
+ * b2) -> . W3 + b3 (s = 1 + output), parameters FP32 (W1 [n_in][n_hidden],
+ * b1, W2 [n_hidden][n_hidden], b2, W3 [n_hidden][n_out], b3, contiguous),
+ * rounded to BF16 where they enter the products (as are the layer inputs and
+ * back-propagated gradients), FP32 accumulation on the BF16 MFMA. The output
+ * layer starts at zero. Rows of x and y are particles. */
+typedef struct smh_mlp smh_mlp;
+int smh_mlp_create(smh_ctx *ctx, int n_in, int n_hidden, int n_out, uint64_t seed, smh_mlp **out);
+int smh_mlp_destroy(smh_mlp *mlp);
+int64_t smh_mlp_num_params(const smh_mlp *mlp);
+int smh_mlp_get_params(smh_mlp *mlp, float *out);
+int smh_mlp_set_params(smh_mlp *mlp, const float *params);
+/* y [rows][n_out] = network(x [rows][n_in]) */
+int smh_mlp_forward(smh_mlp *mlp, int64_t rows, const float *x, float *y);
+/* grad [n_params] = d(sum dy . y) / d params at x */
+int smh_mlp_backward(smh_mlp *mlp, int64_t rows, const float *x, const float *dy, float *grad);
+/* Add one ensemble's particles to the training set: features feat
+ * [n_part][n_in] (float), right-hand sides rhs and the solver's solutions sol
+ * = M^{-1} rhs, complex128 [nrhs][n_unk]; the operator's near blocks must be
+ * computed. The loss is L = sum_{i,r} |(P rhs_r)_i - sol_{i,r}|^2 /
+ * sum |sol|^2 with P from the network's scales. */
+int smh_mlp_add_data(smh_mlp *mlp, smh_op *op, const float *feat, int64_t nrhs, const double *rhs, const double *sol);
+/* L and dL/dy over the training set at outputs y [rows][n_out] (NULL: the
+ * network's outputs); dy may be NULL. */
+int smh_mlp_data_loss(smh_mlp *mlp, const float *y, float *dy, double *loss);
+/* steps of full-batch Adam (beta 0.9 / 0.999) at learning rate lr on the
+ * training set; losses (may be NULL) receives L before each step. */
+int smh_mlp_train(smh_mlp *mlp, int steps, double lr, double *losses);
 
 /* Build both volume tables on the GPU (volumetric_coupling_lookup_table,
  * prepare_lookup.py:16-250, and its helper kernel prepare_lookup_cuda.py:6-52),
@@ -140,7 +188,7 @@ int smh_op_upload_table(smh_op *op, const double *table, const double *table_min
  * reference; direct_coef: complex128 [B][B][2 l_max + 1]; kappa: complex128
  * [n_kappa]; kz: complex128 [n_layers][n_kappa]; layer_coef: complex128
  * [n_layers][n_layers][n_kappa][4][B][B] (n_kappa = 0 and NULL pointers: no
- * layer-mediated part). l_max <= 8, m_max <= 4. The matrix needs 24 bytes per
+ * layer-mediated part). l_max <= 8. The matrix needs 24 bytes per
  * entry of device memory ((n_part B)^2 entries); build_ms (may be NULL)
  * receives the build kernels' time. smh_op_matvec and smh_solve apply it as a
  * dense matrix; smh_op_download_table returns it on the padded index
@@ -192,6 +240,21 @@ int smh_op_bench_matvec(smh_op *op, int precision, int iters, double *kernel_ms,
  * sides (smh_op_matvec uses it for nrhs > 1 on radial operators). No
  * counterpart in smuthi, which has no multi-RHS path. */
 int smh_op_bench_matvec_batch(smh_op *op, int precision, int iters, double *kernel_ms, double *total_ms);
+
+/* Launch-shape autotuner of a lookup operator's matvec (radial or volume,
+ * table set). For each of n_cand candidates c it regroups the pair bins into
+ * work items (about items_per_cu[c] workgroups per compute unit; launch order
+ * dealt to the XCDs by table row when xcd_order[c] != 0, else chunk order),
+ * times `iters` matvecs of the given shape (batch != 0: the 8-RHS kernel in
+ * SMH_FP16 / FP32 / FP64; else the single-vector kernel in FP32 / FP64) into
+ * kernel_ms[c], and keeps the fastest grouping (*best = its index). The FP16
+ * and FP32 8-RHS kernels share one grouping. Results stay within each
+ * kernel's accuracy (the grouping changes summation order and fixed-point
+ * scales only). Drops the solver's captured graphs. No counterpart in
+ * smuthi, whose CUDA kernels launch one thread per matrix row
+ * (linear_system_cuda.py:26-405, cuda_blocksize). */
+int smh_op_autotune(smh_op *op, int batch, int precision, int iters, int n_cand, const int *items_per_cu,
+                    const int *xcd_order, double *kernel_ms, int *best);
 
 /* Interpolation of the FP16 operator used by the mixed16 inner solves (and
  * smh_op_matvec with SMH_FP16): *terms = 2 when it interpolates linearly

@@ -131,7 +131,12 @@ def reasonable_Sommerfeld_kpar_contour(vacuum_wavelength, neff_waypoints=None, l
 
 
 def _prefactor(m):
-    """sqrt((2m+1)!!/(2 (2m)!!)) by the reference's product form (numba_helpers.py:159-175)."""
+    """sqrt((2m+1)!!/(2 (2m)!!)) by the reference's product form (numba_helpers.py:159-175).
+
+    For m > 10 the reference evaluates the same quantity by its asymptotic
+    series in 1/m (numba_helpers.py:177-205), which it truncates where the
+    series reaches double precision; the product form is used here for every
+    m (the two agree to rounding, checked in tests/test_oracle_basics.py)."""
     res = 1.
     for t in range(2, 2 * m + 2, 2):
         res += res / t
@@ -162,9 +167,7 @@ def legendre_normalized(ct, st, lmax):
                            + np.sqrt((2 * (l + 1) + 1) / (2 * (l + 1) - 1)) * ct * pprimel0[l])
         taulm[l + 1, 0] = -st * pprimel0[l + 1]
     for m in range(1, lmax + 1):
-        pref = _prefactor(m) if m <= 10 else None
-        if pref is None:
-            raise ValueError('orders above 10 are outside the hot path')
+        pref = _prefactor(m)
         plm[m, m] = pref * st ** m
         pilm[m, m] = pref * st ** (m - 1)
         taulm[m, m] = m * ct * pilm[m, m]

@@ -429,7 +429,7 @@ class RadialLookupCPUReferenceLoop:
     scipy interp1d objects (linear_system.py:903-975). Used as the timed CPU
     baseline and to cross-check RadialLookupCPU."""
 
-    def __init__(self, table, rho_grid, particle_list, kind='linear'):
+    def __init__(self, table, rho_grid, particle_list, kind='linear', n1_rows=None):
         l_max = max(p.l_max for p in particle_list)
         m_max = max(p.m_max for p in particle_list)
         self.bs = fb.blocksize(l_max, m_max)
@@ -450,9 +450,11 @@ class RadialLookupCPUReferenceLoop:
             off += fb.blocksize(p.l_max, p.m_max)
         self.sys_idx = [np.array(a, dtype=int) for a in self.sys_idx]
         self.pnum = [np.array(a, dtype=int) for a in self.pnum]
+        # (interpolants only for the rows n1_rows when given: a worker of a parallel timing)
+        rows = range(self.bs) if n1_rows is None else set(n1_rows)
         self.lookup = [[scipy.interpolate.interp1d(rho_grid, table[:, n1, n2], kind=kind, axis=-1,
                                                    assume_sorted=True) for n2 in range(self.bs)]
-                       for n1 in range(self.bs)]
+                       if n1 in rows else None for n1 in range(self.bs)]
 
     def matvec(self, v, n1_subset=None):
         out = np.zeros(v.shape, dtype=complex)

@@ -39,6 +39,7 @@ _p = _c.c_void_p
 _i64 = _c.c_int64
 _dp = _c.POINTER(_c.c_double)
 _ip = _c.POINTER(_c.c_int)
+_fp = _c.POINTER(_c.c_float)
 SIGNATURES = {
     'smh_init': (_c.c_int, [_c.c_int, _c.POINTER(_p)]),
     'smh_destroy': (_c.c_int, [_p]),
@@ -53,7 +54,20 @@ SIGNATURES = {
     'smh_volume_create_ensembles': (_c.c_int, [_p, _c.c_int, _c.POINTER(_i64), _dp, _ip, _ip, _c.c_int, _c.c_int,
                                                _c.c_double, _c.c_double, _c.c_double, _c.c_double, _i64, _i64,
                                                _c.c_int, _c.POINTER(_p)]),
-    'smh_op_set_preconditioner': (_c.c_int, [_p, _dp]),
+    'smh_op_set_preconditioner': (_c.c_int, [_p, _c.c_int, _ip, _dp]),
+    'smh_op_near_blocks': (_c.c_int, [_p, _c.c_int, _ip, _dp]),
+    'smh_op_download_near_blocks': (_c.c_int, [_p, _dp]),
+    'smh_op_set_near_scaling': (_c.c_int, [_p, _fp]),
+    'smh_mlp_create': (_c.c_int, [_p, _c.c_int, _c.c_int, _c.c_int, _c.c_uint64, _c.POINTER(_p)]),
+    'smh_mlp_destroy': (_c.c_int, [_p]),
+    'smh_mlp_num_params': (_i64, [_p]),
+    'smh_mlp_get_params': (_c.c_int, [_p, _fp]),
+    'smh_mlp_set_params': (_c.c_int, [_p, _fp]),
+    'smh_mlp_forward': (_c.c_int, [_p, _i64, _fp, _fp]),
+    'smh_mlp_backward': (_c.c_int, [_p, _i64, _fp, _fp, _fp]),
+    'smh_mlp_add_data': (_c.c_int, [_p, _p, _fp, _i64, _dp, _dp]),
+    'smh_mlp_data_loss': (_c.c_int, [_p, _fp, _fp, _dp]),
+    'smh_mlp_train': (_c.c_int, [_p, _c.c_int, _c.c_double, _dp]),
     'smh_volume_build_table': (_c.c_int, [_p, _c.c_double, _c.c_double, _c.c_double, _c.c_double, _dp, _i64, _dp,
                                           _dp, _dp]),
     'smh_op_destroy': (_c.c_int, [_p]),
@@ -64,6 +78,7 @@ SIGNATURES = {
     'smh_op_bench_matvec': (_c.c_int, [_p, _c.c_int, _c.c_int, _dp, _dp]),
     'smh_op_bench_matvec_batch': (_c.c_int, [_p, _c.c_int, _c.c_int, _dp, _dp]),
     'smh_op_half_interp': (_c.c_int, [_p, _ip, _dp]),
+    'smh_op_autotune': (_c.c_int, [_p, _c.c_int, _c.c_int, _c.c_int, _c.c_int, _ip, _ip, _dp, _ip]),
     'smh_solve': (_c.c_int, [_p, _dp, _dp, _dp, _i64, _c.c_double, _c.c_int, _c.c_int, _c.c_int, _dp]),
     'smh_pwe_field': (_c.c_int, [_p, _c.c_int, _i64, _dp, _dp, _i64, _dp, _dp, _c.c_double, _c.c_double, _i64, _dp,
                                  _dp, _c.c_int, _dp]),
@@ -124,7 +139,17 @@ def i64ptr(a):
     return a.ctypes.data_as(_c.POINTER(_i64))
 
 
+def fptr(a):
+    """Pointer to the data of a C-contiguous float32 array."""
+    if a is None:
+        return None
+    assert a.flags['C_CONTIGUOUS'] and a.dtype == np.float32, (a.dtype, a.flags)
+    return a.ctypes.data_as(_fp)
+
+
 def iptr(a):
+    if a is None:
+        return None
     assert a.flags['C_CONTIGUOUS'] and a.dtype == np.int32
     return a.ctypes.data_as(_ip)
 

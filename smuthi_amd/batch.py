@@ -73,7 +73,7 @@ class EnsembleBatch:
                 rho_max, rho_min_pair = max(rho_max, a), min(rho_min_pair, b)
         if not np.isfinite(rho_min_pair):
             rho_max, rho_min_pair = self.resolution, self.resolution
-        op = _EnsembleOperator()
+        op = _EnsembleOperator.__new__(_EnsembleOperator)
         op._init_common(parts)
         self.operator = op
         radial = len(set(xyz[:, 2])) == 1
@@ -132,6 +132,7 @@ class EnsembleBatch:
             p.t_matrix = p.compute_t_matrix(vacuum_wavelength, n_medium)
         self.t_matrix = TMatrix(parts)
         self.t_diagonal = self.t_matrix.diagonal
+        op._make_linear_operator()
 
     @property
     def n_unknowns(self):
@@ -148,14 +149,11 @@ class EnsembleBatch:
         """Per-ensemble parts of [..., n_unknowns] vectors."""
         return [x[..., a:b] for a, b in zip(self.offsets[:-1], self.offsets[1:])]
 
-    def set_preconditioner(self, blocks):
-        """Right block preconditioner for the solves (complex128 [n_particles, B, B] on the padded
-        multipole index of blocksize(l_max, m_max)), or None."""
-        b = None if blocks is None else np.ascontiguousarray(blocks, dtype=np.complex128)
-        if b is not None and b.shape != (len(self.particle_list), self.operator.blocksize, self.operator.blocksize):
-            raise ValueError('preconditioner blocks must be [n_particles, B, B]')
-        _lib.check(_lib.load().smh_op_set_preconditioner(self.operator._op, _lib.dptr(b)), cu.context(),
-                   'smh_op_set_preconditioner')
+    def set_preconditioner(self, blocks, neighbours=None):
+        """Right preconditioner for the solves: block-diagonal blocks [n_particles, B, B], or
+        block-sparse blocks [n_particles, k1, B, B] with neighbours [n_particles, k1] (emitter of each
+        block, -1 empty), on the padded multipole index of blocksize(l_max, m_max); None removes it."""
+        return set_preconditioner(self.operator, blocks, neighbours)
 
     def solve(self, rhs, tol=1e-6, precision='mixed', restart=20, maxiter=None):
         """Solve every ensemble for every right-hand side ([n_fields, n_unknowns], e.g. from
@@ -168,3 +166,24 @@ class EnsembleBatch:
         """As solve, returning (x, stats, converged) instead of raising."""
         return self.operator.solve_master(self.t_diagonal, rhs, tol, precision=precision, restart=restart,
                                           maxiter=maxiter, with_stats=True)
+
+
+def set_preconditioner(operator, blocks, neighbours=None):
+    """smh_op_set_preconditioner on a device operator (see EnsembleBatch.set_preconditioner)."""
+    if blocks is None:
+        _lib.check(_lib.load().smh_op_set_preconditioner(operator._op, 0, None, None), cu.context(),
+                   'smh_op_set_preconditioner')
+        return
+    b = np.asarray(blocks, dtype=np.complex128)
+    n, B = len(operator.particle_list), operator.blocksize
+    if neighbours is None:
+        if b.shape != (n, B, B):
+            raise ValueError('block-diagonal preconditioner blocks must be [n_particles, B, B]')
+        b = b[:, None]
+        neighbours = np.arange(n, dtype=np.int32)[:, None]
+    nb = np.ascontiguousarray(neighbours, dtype=np.int32)
+    if b.ndim != 4 or b.shape[0] != n or b.shape[2:] != (B, B) or nb.shape != b.shape[:2]:
+        raise ValueError('preconditioner blocks must be [n_particles, k1, B, B] with neighbours [n_particles, k1]')
+    b = np.ascontiguousarray(b)
+    _lib.check(_lib.load().smh_op_set_preconditioner(operator._op, nb.shape[1], _lib.iptr(nb), _lib.dptr(b)),
+               cu.context(), 'smh_op_set_preconditioner')

@@ -40,7 +40,7 @@ namespace {
 using smh::zc;
 
 constexpr int kEG = 16;        // emitters per block of the layer kernel
-constexpr int kNJ = 9;         // Bessel orders 0..2 m_max, m_max <= 4
+constexpr int kNJ = 17;        // Bessel orders 0..2 m_max, m_max <= 8
 constexpr int kMaxL2 = 16;     // 2 l_max
 constexpr int kThreads = 1024;
 
@@ -185,7 +185,7 @@ __global__ void __launch_bounds__(kThreads) explicit_direct_kernel(ExplArgs a) {
 // of the block's receiver and emitters are wave-uniform (scalar loads); the
 // Bessel values, indexed by each entry's |m2 - m1|, are staged in LDS per
 // chunk of kKC kappas.
-constexpr int kKC = 16;
+constexpr int kKC = 12;
 __global__ void __launch_bounds__(kThreads)
     explicit_layer_kernel(ExplArgs a, const double2* __restrict__ T, const double2* __restrict__ Jt,
                           const double2* __restrict__ C, const int* __restrict__ emit, double2* __restrict__ W64) {
@@ -506,7 +506,7 @@ int explicit_check(smh_ctx* ctx, int64_t n_tot, const int* layer, int l_max, int
                    const double* direct_coef, int64_t n_kappa, const double* kappa, const double* kz,
                    const double* layer_coef) {
   SMH_CHECK(ctx, l_max >= 1 && m_max >= 1 && m_max <= l_max && 2 * l_max <= kMaxL2 && 2 * m_max < kNJ, SMH_ERR_ARG,
-            "invalid l_max %d / m_max %d (explicit coupling: l_max <= 8, m_max <= 4)", l_max, m_max);
+            "invalid l_max %d / m_max %d (explicit coupling: l_max <= 8)", l_max, m_max);
   SMH_CHECK(ctx, n_layers >= 1, SMH_ERR_ARG, "need at least one layer");
   SMH_CHECK(ctx, direct_coef != nullptr, SMH_ERR_ARG, "direct coefficients are NULL");
   SMH_CHECK(ctx, n_kappa == 0 || (kappa && kz && layer_coef && n_kappa >= 2), SMH_ERR_ARG,

@@ -1,7 +1,8 @@
 // Pieces shared by the radial and the volume lookup operators: work-item
 // construction over bin entries, per-item fixed-point statistics, the chunk
-// reduction and small helper kernels. Included by one translation unit per
-// operator kind (everything here has internal linkage).
+// reduction and small helper kernels. Included by the translation unit of
+// each operator kind and by the autotuner (everything here has internal
+// linkage).
 #pragma once
 
 #include <algorithm>
@@ -229,37 +230,46 @@ inline void fill_zero(void* p, size_t bytes, hipStream_t s) {
 
 // Work items per compute unit: items of one receiver chunk run one after the
 // other on a CU's single workgroup slot, so ~16 rounds keep the last,
-// partly filled round short.
+// partly filled round short. The autotuner (smh_op_autotune) may regroup.
 constexpr int kItemsPerCu = 16;
 
-// Host: bin entries (at most kEntryPairs pairs each) of every non-empty
-// (chunk, bin) key, grouped into work items of similar cost, then uploaded
-// with the per-item pair statistics. Cost of an entry: single-vector kernels
-// one unit per tile of 16 pairs plus 1/4 for loading the bin's taps; the
-// kBatch-RHS kernels (batch set) one unit per pair plus one for the taps.
-// cnt / off: pair count and first pair of each key (chunk-major,
-// bins_per_chunk keys per chunk); row_of(bin) gives the bin's first tap row.
-template <typename Rec, typename RowOf>
-int build_work_items(smh_op* op, smh_op::Binning& bn, const std::vector<int>& cnt, const std::vector<int>& off,
-                     int64_t bins_per_chunk, RowOf row_of, bool batch = false) {
+// Launch the item statistics that depend on the grouping: the most pairs per
+// receiver (always) and the largest tap-row sum (once the table is set).
+inline void item_stats(smh_op::Binning& bn, hipStream_t s) {
+  if (bn.n_items <= 0) return;
+  bn.count_items(bn, s);
+  if (bn.lsum_rows)
+    hipLaunchKernelGGL(item_lsum_kernel, dim3(bn.n_items), dim3(256), 0, s, bn.d_entries, bn.d_items, bn.n_items,
+                       bn.n_entries, bn.lsum_T, bn.lsum_vmode, bn.lsum_len_z, bn.lsum_rows, bn.d_item_lsum);
+}
+
+// Record the tap-row sums the item bounds come from and compute them.
+inline void set_item_lsum(smh_op::Binning& bn, const float* rowsum, int T, int vmode, int len_z, hipStream_t s) {
+  bn.lsum_rows = rowsum;
+  bn.lsum_T = T;
+  bn.lsum_vmode = vmode;
+  bn.lsum_len_z = len_z;
+  if (bn.n_items > 0)
+    hipLaunchKernelGGL(item_lsum_kernel, dim3(bn.n_items), dim3(256), 0, s, bn.d_entries, bn.d_items, bn.n_items,
+                       bn.n_entries, T, vmode, len_z, rowsum, bn.d_item_lsum);
+}
+
+// Host: group the binning's entries (h_entries, in (chunk, bin) order) into
+// work items of similar cost, about items_per_cu per compute unit and never
+// spanning two receiver chunks, and upload them with their launch order and
+// statistics (replacing an earlier grouping). Cost of an entry: single-vector
+// kernels one unit per tile of 16 pairs plus 1/4 for loading the bin's taps;
+// the kBatch-RHS kernels one unit per pair plus one for the taps.
+inline int group_work_items(smh_op* op, smh_op::Binning& bn, int items_per_cu, bool xcd_order) {
+  const bool batch = bn.batch;
   auto cost = [batch](int np) { return batch ? np + 1.0 : (np + 15) / 16 + 0.25; };
   smh_ctx* ctx = op->ctx;
   hipStream_t s = ctx->stream;
-  std::vector<int4> entries;
-  std::vector<int> ent_chunk;
+  const std::vector<int4>& entries = bn.h_entries;
+  const std::vector<int>& ent_chunk = bn.h_ent_chunk;
   double total = 0;
-  for (int c = 0; c < bn.n_chunks; ++c)
-    for (int64_t r = 0; r < bins_per_chunk; ++r) {
-      const int64_t key = (int64_t)c * bins_per_chunk + r;
-      const int n = cnt[key];
-      for (int p0 = 0; p0 < n; p0 += kEntryPairs) {
-        const int np = std::min(kEntryPairs, n - p0);
-        entries.push_back(make_int4(row_of(r), off[key] + p0, np, 0));
-        ent_chunk.push_back(c);
-        total += cost(np);
-      }
-    }
-  const int64_t target_items = std::max<int64_t>(kItemsPerCu * (int64_t)ctx->num_cu, bn.n_chunks);
+  for (const int4& e : entries) total += cost(e.z);
+  const int64_t target_items = std::max<int64_t>((int64_t)items_per_cu * ctx->num_cu, bn.n_chunks);
   const double budget = std::max(1.0, total / (double)target_items);
   std::vector<int2> items;
   std::vector<int> chunk_items(bn.n_chunks + 1, 0);
@@ -275,40 +285,85 @@ int build_work_items(smh_op* op, smh_op::Binning& bn, const std::vector<int>& cn
     run += cost(entries[e].z);
   }
   while (cur_chunk < bn.n_chunks) chunk_items[++cur_chunk] = (int)items.size();
-  bn.n_entries = (int)entries.size();
-  bn.n_items = (int)items.size();
+  const int ni = (int)items.size();
   // Launch order: items sorted by their first table row; blocks are dealt to
   // the XCDs round-robin (XCD = block mod kXcds), so XCD x gets the x-th
   // contiguous share of the sorted list and its CUs run items of neighbouring
-  // bins (from different receiver chunks) at the same time.
+  // bins (from different receiver chunks) at the same time. Without it,
+  // blocks run the items in (chunk, bin) order.
   constexpr int kXcds = 8;
-  const int ni = bn.n_items;
-  std::vector<int> sorted(ni), order(ni);
-  for (int k = 0; k < ni; ++k) sorted[k] = k;
-  std::stable_sort(sorted.begin(), sorted.end(),
-                   [&](int p, int q) { return entries[items[p].y].x < entries[items[q].y].x; });
-  std::vector<int> xoff(kXcds + 1, 0);
-  for (int x = 0; x < kXcds; ++x) xoff[x + 1] = xoff[x] + (ni - x + kXcds - 1) / kXcds;
-  for (int b = 0; b < ni; ++b) order[b] = sorted[xoff[b % kXcds] + b / kXcds];
-  if (hipMalloc(&bn.d_entries, std::max<size_t>(entries.size(), 1) * sizeof(int4)) != hipSuccess ||
-      hipMalloc(&bn.d_items, std::max<size_t>(items.size(), 1) * sizeof(int2)) != hipSuccess ||
+  std::vector<int> order(ni);
+  for (int k = 0; k < ni; ++k) order[k] = k;
+  if (xcd_order) {
+    std::vector<int> sorted(order);
+    std::stable_sort(sorted.begin(), sorted.end(),
+                     [&](int p, int q) { return entries[items[p].y].x < entries[items[q].y].x; });
+    std::vector<int> xoff(kXcds + 1, 0);
+    for (int x = 0; x < kXcds; ++x) xoff[x + 1] = xoff[x] + (ni - x + kXcds - 1) / kXcds;
+    for (int b = 0; b < ni; ++b) order[b] = sorted[xoff[b % kXcds] + b / kXcds];
+  }
+  // the previous grouping's arrays may still be read by queued work
+  if (hipStreamSynchronize(s) != hipSuccess) return smh_fail(ctx, SMH_ERR_HIP, "stream failed before regrouping");
+  void* old[] = {bn.d_items, bn.d_chunk_items, bn.d_item_lsum, bn.d_item_en, bn.d_order};
+  for (void* p : old)
+    if (p) hipFree(p);
+  bn.d_items = nullptr;
+  bn.d_chunk_items = nullptr;
+  bn.d_item_lsum = nullptr;
+  bn.d_item_en = nullptr;
+  bn.d_order = nullptr;
+  bn.n_items = ni;
+  bn.items_per_cu = items_per_cu;
+  bn.xcd_order = xcd_order;
+  if (hipMalloc(&bn.d_items, std::max<size_t>(items.size(), 1) * sizeof(int2)) != hipSuccess ||
       hipMalloc(&bn.d_chunk_items, chunk_items.size() * sizeof(int)) != hipSuccess ||
-      hipMalloc(&bn.d_item_lsum, std::max(bn.n_items, 1) * sizeof(float)) != hipSuccess ||
-      hipMalloc(&bn.d_item_en, std::max(bn.n_items, 1) * sizeof(int)) != hipSuccess ||
-      hipMalloc(&bn.d_order, std::max(bn.n_items, 1) * sizeof(int)) != hipSuccess ||
+      hipMalloc(&bn.d_item_lsum, std::max(ni, 1) * sizeof(float)) != hipSuccess ||
+      hipMalloc(&bn.d_item_en, std::max(ni, 1) * sizeof(int)) != hipSuccess ||
+      hipMalloc(&bn.d_order, std::max(ni, 1) * sizeof(int)) != hipSuccess ||
       hipMemcpyAsync(bn.d_order, order.data(), order.size() * sizeof(int), hipMemcpyHostToDevice, s) != hipSuccess ||
-      hipMemcpyAsync(bn.d_entries, entries.data(), entries.size() * sizeof(int4), hipMemcpyHostToDevice, s) !=
-          hipSuccess ||
       hipMemcpyAsync(bn.d_items, items.data(), items.size() * sizeof(int2), hipMemcpyHostToDevice, s) != hipSuccess ||
       hipMemcpyAsync(bn.d_chunk_items, chunk_items.data(), chunk_items.size() * sizeof(int), hipMemcpyHostToDevice,
                      s) != hipSuccess) {
     return smh_fail(ctx, SMH_ERR_HIP, "device allocation failed (work items)");
   }
-  if (bn.n_items > 0)
-    hipLaunchKernelGGL(item_count_kernel<Rec>, dim3(bn.n_items), dim3(256), 0, s, (const Rec*)bn.d_rec,
-                       bn.d_entries, bn.d_items, bn.n_items, bn.n_entries, bn.d_item_en);
+  item_stats(bn, s);
   if (hipStreamSynchronize(s) != hipSuccess) return smh_fail(ctx, SMH_ERR_HIP, "work item statistics failed");
   return SMH_OK;
+}
+
+// Host: bin entries (at most kEntryPairs pairs each) of every non-empty
+// (chunk, bin) key, uploaded once, then grouped into work items
+// (group_work_items). cnt / off: pair count and first pair of each key
+// (chunk-major, bins_per_chunk keys per chunk); row_of(bin) gives the bin's
+// first tap row.
+template <typename Rec, typename RowOf>
+int build_work_items(smh_op* op, smh_op::Binning& bn, const std::vector<int>& cnt, const std::vector<int>& off,
+                     int64_t bins_per_chunk, RowOf row_of, bool batch = false) {
+  smh_ctx* ctx = op->ctx;
+  hipStream_t s = ctx->stream;
+  bn.h_entries.clear();
+  bn.h_ent_chunk.clear();
+  for (int c = 0; c < bn.n_chunks; ++c)
+    for (int64_t r = 0; r < bins_per_chunk; ++r) {
+      const int64_t key = (int64_t)c * bins_per_chunk + r;
+      const int n = cnt[key];
+      for (int p0 = 0; p0 < n; p0 += kEntryPairs) {
+        const int np = std::min(kEntryPairs, n - p0);
+        bn.h_entries.push_back(make_int4(row_of(r), off[key] + p0, np, 0));
+        bn.h_ent_chunk.push_back(c);
+      }
+    }
+  bn.n_entries = (int)bn.h_entries.size();
+  bn.batch = batch;
+  bn.count_items = [](smh_op::Binning& b, hipStream_t st) {
+    hipLaunchKernelGGL(item_count_kernel<Rec>, dim3(b.n_items), dim3(256), 0, st, (const Rec*)b.d_rec, b.d_entries,
+                       b.d_items, b.n_items, b.n_entries, b.d_item_en);
+  };
+  if (hipMalloc(&bn.d_entries, std::max<size_t>(bn.h_entries.size(), 1) * sizeof(int4)) != hipSuccess ||
+      hipMemcpyAsync(bn.d_entries, bn.h_entries.data(), bn.h_entries.size() * sizeof(int4), hipMemcpyHostToDevice,
+                     s) != hipSuccess)
+    return smh_fail(ctx, SMH_ERR_HIP, "device allocation failed (bin entries)");
+  return group_work_items(op, bn, kItemsPerCu, true);
 }
 
 }  // namespace

@@ -59,6 +59,20 @@ struct smh_op {
     int* d_item_en = nullptr;        // per item: ceil(log2) of the most pairs of one receiver
     int* d_order = nullptr;          // block -> item launch order (XCD-aware)
     size_t lds = 0;
+    // grouping of the entries into work items (regrouped by the autotuner):
+    // host copies of the entries and their chunks, the item cost model
+    // (batch: per pair, else per 16-pair tile), the target items per CU and
+    // whether the launch order deals sorted items to the XCDs
+    std::vector<int4> h_entries;
+    std::vector<int> h_ent_chunk;
+    bool batch = false;
+    int items_per_cu = 0;
+    bool xcd_order = true;
+    void (*count_items)(Binning&, hipStream_t) = nullptr;  // item_count_kernel for the record type
+    // item_lsum_kernel arguments once the table is set (rows == nullptr: not yet)
+    const float* lsum_rows = nullptr;
+    int lsum_T = 0, lsum_vmode = 0, lsum_len_z = 0;
+    size_t part_bytes = 0;  // partial-result bytes one work item writes (d_ypart holds n_items of them)
   } bin[2];
   Binning bbin[2];  // the same pairs binned for the kBatch-RHS kernel (smaller receiver chunks)
   int64_t n_pairs = 0;
@@ -93,9 +107,18 @@ struct smh_op {
   int64_t* d_seg_off = nullptr;  // device copy
   int* d_seg_of = nullptr;       // device: ensemble of every particle (null: one ensemble)
 
-  // right block preconditioner P = blockdiag(C_i), complex128 [n_part][B][B]
-  // on the padded index (null: none); GMRES then solves M P y = b, x = P y
+  // right block-sparse preconditioner: (P v)_i = sum_k P[i][k] v_{nbr[i][k]},
+  // complex128 blocks [n_part][P_k1][B][B] on the padded index and emitters
+  // d_Pnbr [n_part][P_k1] (-1: empty); null: none. GMRES then solves
+  // M P y = b and returns x = P y.
   double2* d_P = nullptr;
+  int* d_Pnbr = nullptr;
+  int P_k1 = 0;
+  // near-field blocks A_ij = diag(t_i) W_ij of each particle and its
+  // near_k1 - 1 nearest neighbours (smh_learn.hip)
+  double2* d_near = nullptr;
+  int* d_near_nbr = nullptr;
+  int near_k1 = 0;
 
   // solver workspace (smh_solver.hip), grown on demand, freed with the operator
   void* solver_ws = nullptr;
@@ -160,8 +183,10 @@ void solver_release(smh_op* op);
 void solver_drop_graphs(smh_op* op);
 // raise a kernel's dynamic LDS limit once per operator (outside stream capture)
 int op_set_lds(smh_op* op, void* fn, size_t lds);
-// y[p] = P_p x[p] (+ y[p] when accumulate) for kBatch padded vectors [n_part][B][kBatch]
+// y = P x (+ y when accumulate) for kBatch padded vectors [n_part][B][kBatch]
 int op_apply_precond_batch(smh_op* op, const double2* x, double2* y, bool accumulate);
+// preconditioner storage for k1 blocks per particle (kept when the size matches)
+int op_reserve_precond(smh_op* op, int k1);
 // explicit operator: LU solve on the device (smh_lu.hip) and its release
 int lu_solve(smh_op* op, const double* t_diag, const double* rhs, double* x, int64_t nrhs, double* stats);
 void lu_release(smh_op* op);

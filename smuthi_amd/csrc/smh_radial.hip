@@ -596,10 +596,7 @@ int op_make_taps(smh_op* op) {
   }
   hipLaunchKernelGGL(tap_rowsum_kernel<double2>, dim3(blocks_for(op->n_tt)), dim3(256), 0, s, op->d_table, d_mh,
                      op->n_tt, op->B, op->interp == SMH_INTERP_SPLINE ? 1 : 0, op->d_rowsum);
-  for (smh_op::Binning* bn : {&op->bin[0], &op->bbin[0]})
-    if (bn->n_items > 0)
-      hipLaunchKernelGGL(item_lsum_kernel, dim3(bn->n_items), dim3(256), 0, s, bn->d_entries, bn->d_items,
-                         bn->n_items, bn->n_entries, op->T, 0, 0, op->d_rowsum, bn->d_item_lsum);
+  for (smh_op::Binning* bn : {&op->bin[0], &op->bbin[0]}) set_item_lsum(*bn, op->d_rowsum, op->T, 0, 0, s);
   SMH_HIP(ctx, hipGetLastError());
   SMH_HIP(ctx, hipStreamSynchronize(s));
   if (d_mh) hipFree(d_mh);
@@ -928,7 +925,7 @@ int op_index_map(smh_op* op, const int* lmax, const int* mmax) {
 void smh_op_free(smh_op* op) {
   smh::lu_release(op);
   smh::solver_release(op);
-  void* sp[] = {op->d_seg_off, op->d_seg_of, op->d_P};
+  void* sp[] = {op->d_seg_off, op->d_seg_of, op->d_P, op->d_Pnbr, op->d_near, op->d_near_nbr};
   for (void* p : sp)
     if (p) hipFree(p);
   void* ptrs[] = {op->d_xy, op->d_xyz, op->d_pk2pad, op->d_table, op->d_table_minus, op->d_tt64, op->d_tt32,
@@ -1127,7 +1124,8 @@ static int radial_create(smh_ctx* ctx, int n_ens, const int64_t* ens_off, int64_
     const int code =
         pi == 0 ? bin_pairs<float>(op, op->bin[0], Rc, false) : bin_pairs<double>(op, op->bin[1], Rc, false);
     if (code) return fail(code);
-    ypart = std::max(ypart, (size_t)op->bin[pi].n_items * Rc * 2 * op->B * rs);
+    op->bin[pi].part_bytes = (size_t)Rc * 2 * op->B * rs;
+    ypart = std::max(ypart, (size_t)op->bin[pi].n_items * op->bin[pi].part_bytes);
     // the same for the kBatch-RHS kernel, whose accumulator holds kBatch vectors per receiver
     const KernelInfo kb = radial_batch_kernel(pi == 0 ? SMH_FP32 : SMH_FP64, op->T, l_max, m_max);
     size_t ldsb = 0;
@@ -1150,7 +1148,8 @@ static int radial_create(smh_ctx* ctx, int n_ens, const int64_t* ens_off, int64_
         }
       }
     }
-    ypart = std::max(ypart, (size_t)op->bbin[pi].n_items * Rb * 2 * kBatch * Sb * rs);
+    op->bbin[pi].part_bytes = (size_t)Rb * 2 * kBatch * Sb * rs;
+    ypart = std::max(ypart, (size_t)op->bbin[pi].n_items * op->bbin[pi].part_bytes);
   }
   op->ypart_bytes = std::max<size_t>(ypart, 16);
   const int64_t n_pad = n_part * op->B;

@@ -141,6 +141,9 @@ struct RadialKArgs {
   // dealt so that the blocks one XCD runs (blockIdx mod 8) walk neighbouring
   // bins together and share the tap rows in that XCD's L2.
   const int* order = nullptr;
+  // single-vector kernel without a launch order: block b runs item item0 + b
+  // and writes its partial at ypart[b] (launches over a range of the items)
+  int item0 = 0;
 };
 
 // FP32 receiver accumulation in LDS is fixed point: a contribution v is split
@@ -285,7 +288,7 @@ __global__ void __launch_bounds__((RadialShape<Real, SvwfIdx<LMAX, MMAX>::B, T>:
   AccT* yacc = reinterpret_cast<AccT*>(smem_raw);
   int* next_entry = reinterpret_cast<int*>(yacc + (size_t)a.Rc * SY);
   // work item of this block (launch order maps blocks to items, see RadialKArgs::order)
-  const int item = a.order ? a.order[blockIdx.x] : (int)blockIdx.x;
+  const int item = a.order ? a.order[blockIdx.x] : a.item0 + (int)blockIdx.x;
   float sh = 0.f, ish = 0.f, sl = 0.f, isl = 0.f;
   // headroom for the additions: 2^eN pairs per receiver, KSPLIT parts each
   const int eN = a.item_en[item] + (KSPLIT > 1 ? 32 - __builtin_clz(KSPLIT - 1) : 0);
@@ -463,7 +466,7 @@ __global__ void __launch_bounds__((RadialShape<Real, SvwfIdx<LMAX, MMAX>::B, T>:
     }
   }
   __syncthreads();
-  Real* yp = reinterpret_cast<Real*>(a.ypart) + (size_t)item * a.Rc * 2 * B;
+  Real* yp = reinterpret_cast<Real*>(a.ypart) + (size_t)(item - a.item0) * a.Rc * 2 * B;
   for (int j = tid; j < a.Rc * 2 * B; j += blockDim.x) {
     const int il = j / (2 * B), k = j - il * 2 * B;
     if constexpr (kFixed)

@@ -404,25 +404,41 @@ __global__ void backsub_kernel(int nsys, int restart, const double2* __restrict_
   if (m > 0) res[sys] = sqrt(gs[m].x * gs[m].x + gs[m].y * gs[m].y);
 }
 
-// y[p] = P_p x[p] (+ y[p]) for kB padded vectors: one workgroup per particle,
-// thread (n1, r)
-__global__ void precond_kernel(const double2* __restrict__ P, const double2* __restrict__ x, double2* y, int B,
-                               int accumulate) {
+// y_i = sum_k P[i][k] x_{nbr[i][k]} (+ y_i) for kB padded vectors: one
+// workgroup per particle, thread (n1, r), the emitter's vector staged in LDS
+__global__ void __launch_bounds__(256) precond_kernel(const double2* __restrict__ P, const int* __restrict__ nbr,
+                                                      int k1, const double2* __restrict__ x, double2* y, int B,
+                                                      int accumulate) {
   extern __shared__ double2 xs[];  // [B][kB]
   const int64_t p = blockIdx.x;
-  const double2* xp = x + (size_t)p * B * kB;
-  for (int j = threadIdx.x; j < B * kB; j += blockDim.x) xs[j] = xp[j];
-  __syncthreads();
-  const double2* Pp = P + (size_t)p * B * B;
-  for (int t = threadIdx.x; t < B * kB; t += blockDim.x) {
-    const int n1 = t / kB, r = t - n1 * kB;
-    double re = 0, im = 0;
-    for (int n2 = 0; n2 < B; ++n2) {
-      const double2 a = Pp[(size_t)n1 * B + n2], b = xs[n2 * kB + r];
-      re += a.x * b.x - a.y * b.y;
-      im += a.x * b.y + a.y * b.x;
+  const int nel = B * kB;
+  double2 acc[2] = {make_double2(0.0, 0.0), make_double2(0.0, 0.0)};
+  for (int k = 0; k < k1; ++k) {
+    const int j = nbr[p * k1 + k];
+    if (j < 0) continue;
+    __syncthreads();
+    const double2* xj = x + (size_t)j * nel;
+    for (int e = threadIdx.x; e < nel; e += blockDim.x) xs[e] = xj[e];
+    __syncthreads();
+    const double2* Pk = P + ((size_t)p * k1 + k) * B * B;
+    for (int u = 0; u < 2; ++u) {
+      const int t = threadIdx.x + u * blockDim.x;
+      if (t >= nel) break;
+      const int n1 = t / kB, r = t - n1 * kB;
+      double re = acc[u].x, im = acc[u].y;
+      for (int n2 = 0; n2 < B; ++n2) {
+        const double2 a = Pk[(size_t)n1 * B + n2], b = xs[n2 * kB + r];
+        re += a.x * b.x - a.y * b.y;
+        im += a.x * b.y + a.y * b.x;
+      }
+      acc[u] = make_double2(re, im);
     }
-    double2* out = y + (size_t)p * B * kB + t;
+  }
+  for (int u = 0; u < 2; ++u) {
+    const int t = threadIdx.x + u * blockDim.x;
+    if (t >= nel) break;
+    double2* out = y + (size_t)p * nel + t;
+    double re = acc[u].x, im = acc[u].y;
     if (accumulate) {
       re += out->x;
       im += out->y;
@@ -892,11 +908,28 @@ void solver_drop_graphs(smh_op* op) {
 
 int op_apply_precond_batch(smh_op* op, const double2* x, double2* y, bool accumulate) {
   smh_ctx* ctx = op->ctx;
-  SMH_CHECK(ctx, op->d_P != nullptr, SMH_ERR_STATE, "no preconditioner set");
+  SMH_CHECK(ctx, op->d_P != nullptr && op->d_Pnbr != nullptr, SMH_ERR_STATE, "no preconditioner set");
+  SMH_CHECK(ctx, op->B * kB <= 512, SMH_ERR_UNSUPPORTED, "blocksize too large for the preconditioner kernel");
   const size_t lds = (size_t)op->B * kB * sizeof(double2);
   hipLaunchKernelGGL(precond_kernel, dim3((unsigned)op->n_part), dim3(256), lds, ctx->stream, (const double2*)op->d_P,
-                     x, y, op->B, accumulate ? 1 : 0);
+                     (const int*)op->d_Pnbr, op->P_k1, x, y, op->B, accumulate ? 1 : 0);
   SMH_HIP(ctx, hipGetLastError());
+  return SMH_OK;
+}
+
+int op_reserve_precond(smh_op* op, int k1) {
+  smh_ctx* ctx = op->ctx;
+  if (op->d_P && op->P_k1 == k1) return SMH_OK;
+  SMH_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  solver_drop_graphs(op);  // graphs hold the buffers' addresses
+  if (op->d_P) (void)hipFree(op->d_P);
+  if (op->d_Pnbr) (void)hipFree(op->d_Pnbr);
+  op->d_P = nullptr;
+  op->d_Pnbr = nullptr;
+  op->P_k1 = 0;
+  SMH_HIP(ctx, hipMalloc(&op->d_P, (size_t)op->n_part * k1 * op->B * op->B * sizeof(double2)));
+  SMH_HIP(ctx, hipMalloc(&op->d_Pnbr, (size_t)op->n_part * k1 * sizeof(int)));
+  op->P_k1 = k1;
   return SMH_OK;
 }
 
@@ -1162,24 +1195,32 @@ extern "C" int smh_solve(smh_op* op, const double* t_diag, const double* rhs, do
   return solve_single(op, t_diag, rhs, x, nrhs, tol, restart, maxiter, mode, stats, t0);
 }
 
-extern "C" int smh_op_set_preconditioner(smh_op* op, const double* blocks) {
+extern "C" int smh_op_set_preconditioner(smh_op* op, int k1, const int* nbr, const double* blocks) {
   if (!op) return SMH_ERR_ARG;
   smh_ctx* ctx = op->ctx;
   SMH_HIP(ctx, hipSetDevice(ctx->device));
   if (!blocks) {
     if (op->d_P) {
       SMH_HIP(ctx, hipStreamSynchronize(ctx->stream));
-      smh::solver_drop_graphs(op);  // graphs hold the buffer's address
-      SMH_HIP(ctx, hipFree(op->d_P));
+      smh::solver_drop_graphs(op);  // graphs hold the buffers' addresses
+      (void)hipFree(op->d_P);
+      if (op->d_Pnbr) (void)hipFree(op->d_Pnbr);
       op->d_P = nullptr;
+      op->d_Pnbr = nullptr;
+      op->P_k1 = 0;
     }
     return SMH_OK;
   }
+  SMH_CHECK(ctx, k1 >= 1 && k1 <= 64 && nbr, SMH_ERR_ARG, "invalid preconditioner pattern");
   SMH_CHECK(ctx, smh::op_has_batch(op, SMH_FP32) && smh::op_has_batch(op, SMH_FP64), SMH_ERR_UNSUPPORTED,
             "preconditioned solves need an operator with the batched kernels");
-  const size_t bytes = (size_t)op->n_part * op->B * op->B * sizeof(double2);
-  if (!op->d_P) SMH_HIP(ctx, hipMalloc(&op->d_P, bytes));
+  for (int64_t i = 0; i < op->n_part * k1; ++i)
+    SMH_CHECK(ctx, nbr[i] >= -1 && nbr[i] < op->n_part, SMH_ERR_ARG, "emitter index %d out of range", nbr[i]);
+  if (int rc = smh::op_reserve_precond(op, k1)) return rc;
+  const size_t bytes = (size_t)op->n_part * k1 * op->B * op->B * sizeof(double2);
   SMH_HIP(ctx, hipMemcpyAsync(op->d_P, blocks, bytes, hipMemcpyHostToDevice, ctx->stream));
+  SMH_HIP(ctx, hipMemcpyAsync(op->d_Pnbr, nbr, (size_t)op->n_part * k1 * sizeof(int), hipMemcpyHostToDevice,
+                              ctx->stream));
   SMH_HIP(ctx, hipStreamSynchronize(ctx->stream));
   return SMH_OK;
 }

@@ -555,10 +555,7 @@ int vol_make_taps(smh_op* op) {
     hipLaunchKernelGGL(frag_rowsum_kernel, dim3((unsigned)std::min<int64_t>(rows, 65536)), dim3(256), 0, s,
                        op->d_vtt[t], rows, op->B, op->Bh, g.NG, g.FR, op->d_vrowsum[t]);
     for (smh_op::Binning* bn : {&op->vbin[t][0], &op->vbbin[t][0]})
-      if (bn->n_items > 0)
-        hipLaunchKernelGGL(item_lsum_kernel, dim3(bn->n_items), dim3(256), 0, s, bn->d_entries, bn->d_items,
-                           bn->n_items, bn->n_entries, 4, op->vmode, (int)op->len_z, op->d_vrowsum[t],
-                           bn->d_item_lsum);
+      set_item_lsum(*bn, op->d_vrowsum[t], 4, op->vmode, (int)op->len_z, s);
     // the moments of this table must be consumed before the next one overwrites them
     if (hipStreamSynchronize(s) != hipSuccess) {
       cleanup();
@@ -801,7 +798,8 @@ static int volume_create(smh_ctx* ctx, int n_ens, const int64_t* ens_off, int64_
       op->vbin[t][pi].lds = lds;
       const int code = pi == 0 ? vol_bin_pairs<float>(op, t, 0, Rc, false) : vol_bin_pairs<double>(op, t, 1, Rc, false);
       if (code) return fail(code);
-      ypart = std::max(ypart, (size_t)op->vbin[t][pi].n_items * Rc * 2 * op->B * rs);
+      op->vbin[t][pi].part_bytes = (size_t)Rc * 2 * op->B * rs;
+      ypart = std::max(ypart, (size_t)op->vbin[t][pi].n_items * op->vbin[t][pi].part_bytes);
     }
     // the same pairs for the kBatch-RHS kernel (its accumulator holds kBatch vectors per receiver)
     const KernelInfo kb = volume_batch_kernel(pi == 0 ? SMH_FP32 : SMH_FP64, l_max, m_max);
@@ -812,7 +810,8 @@ static int volume_create(smh_ctx* ctx, int n_ens, const int64_t* ens_off, int64_
       op->vbbin[t][pi].lds = ldsb;
       const int code = pi == 0 ? vol_bin_pairs<float>(op, t, 0, Rb, true) : vol_bin_pairs<double>(op, t, 1, Rb, true);
       if (code) return fail(code);
-      ypart = std::max(ypart, (size_t)op->vbbin[t][pi].n_items * Rb * 2 * kBatchRhs * op->B * rs);
+      op->vbbin[t][pi].part_bytes = (size_t)Rb * 2 * kBatchRhs * op->B * rs;
+      ypart = std::max(ypart, (size_t)op->vbbin[t][pi].n_items * op->vbbin[t][pi].part_bytes);
     }
   }
   op->ypart_bytes = std::max<size_t>(ypart, 16);

@@ -60,3 +60,62 @@ def small_substrate_ensemble(seed=1):
     xy = sequential_addition(50, 4000.0, 4000.0, radii, 20.0, rng)
     return [Sphere(position=[x, y, 200.0], refractive_index=1.5 + 0.01j, radius=100.0, l_max=3, m_max=3)
             for x, y in xy]
+
+
+def film_ensemble(n, seed, z_range, radius_range, n_real, n_imag=(0.01, 0.01), l_max=3, density=2.78e-6, gap=20.0,
+                  max_tries=400):
+    """Spheres at heights inside a film: radii r ~ U(radius_range), Re n ~ U(n_real), Im n ~ U(n_imag), centre
+    heights z ~ U(z_range(r)) (z_range: radius -> (low, high)), in-plane density `density` per nm^2 in a
+    square centred on the origin, centres at least r_i + r_j + gap apart in 3-D (random sequential
+    addition with a grid hash)."""
+    rng = np.random.default_rng(seed)
+    side = float(np.sqrt(n / density))
+    radii = rng.uniform(*radius_range, n)
+    nre = rng.uniform(*n_real, n)
+    nim = rng.uniform(*n_imag, n)
+    cell = 2 * radii.max() + gap
+    grid = {}
+    pos = np.zeros((n, 3))
+    for i in range(n):
+        lo, hi = z_range(radii[i])
+        for _ in range(max_tries * (n if i == 0 else 1)):
+            c = np.array([rng.uniform(-side / 2, side / 2), rng.uniform(-side / 2, side / 2), rng.uniform(lo, hi)])
+            gx, gy = int((c[0] + side / 2) // cell), int((c[1] + side / 2) // cell)
+            ok = True
+            for dx in (-1, 0, 1):
+                for dy in (-1, 0, 1):
+                    for j in grid.get((gx + dx, gy + dy), ()):
+                        if np.linalg.norm(c - pos[j]) < radii[i] + radii[j] + gap:
+                            ok = False
+                            break
+                    if not ok:
+                        break
+                if not ok:
+                    break
+            if ok:
+                pos[i] = c
+                grid.setdefault((gx, gy), []).append(i)
+                break
+        else:
+            raise RuntimeError('could not place particle %d of %d' % (i, n))
+    return [Sphere(position=list(pos[i]), refractive_index=complex(nre[i], nim[i]), radius=float(radii[i]),
+                   l_max=l_max, m_max=l_max) for i in range(n)]
+
+
+def batching_ensemble(n, seed):
+    """Batching configuration: r ~ U(80, 120) nm, n ~ U(1.5, 2.0) + 0.01i, heights ~ U(150, 600) nm inside
+    a 1 um film (n = 1.8) on glass, 2.78 spheres per um^2, l_max = 3."""
+    return film_ensemble(n, seed, lambda r: (150.0, 600.0), (80.0, 120.0), (1.5, 2.0))
+
+
+def batching_sizes(count=32, seed=4, low=1000, high=20000):
+    """Ensemble sizes round(exp(U(ln low, ln high)))."""
+    rng = np.random.default_rng(seed)
+    return [int(round(np.exp(v))) for v in rng.uniform(np.log(low), np.log(high), count)]
+
+
+def learned_ensemble(n, seed):
+    """Learned-preconditioner configuration: r ~ U(60, 180) nm, Re n ~ U(1.4, 2.4), Im n ~ U(0, 0.05),
+    centres inside the 500 nm film (n = 2.0) of air / film / glass at z ~ U(r + 10, 490 - r) nm,
+    2.78 spheres per um^2, l_max = 3."""
+    return film_ensemble(n, seed, lambda r: (r + 10.0, 490.0 - r), (60.0, 180.0), (1.4, 2.4), (0.0, 0.05))

@@ -215,38 +215,53 @@ class GaussianBeam:
             return flds.default_azimuthal_angles
         return self.azimuthal_angles_array
 
-    def plane_wave_expansion(self, layer_system, i, k_parallel_array=None, azimuthal_angles_array=None):
-        """(upgoing, downgoing) PWEs of the beam in layer i (initial_field.py:167-233)."""
-        kp = np.asarray(self.get_k_parallel_array() if k_parallel_array is None else k_parallel_array)
-        al = np.asarray(self.get_azimuthal_angles_array() if azimuthal_angles_array is None
-                        else azimuthal_angles_array)
+    def _source_side(self, layer_system):
+        """(layer the beam enters from, +1 travelling up / -1 down): the bottom
+        half-space for cos(polar angle) > 0, else the top one."""
         if np.cos(self.polar_angle) > 0:
-            iG, kind = 0, 'upgoing'
-        else:
-            iG, kind = layer_system.number_of_layers() - 1, 'downgoing'
-        niG = layer_system.refractive_indices[iG]
-        if np.imag(niG):
+            return 0, 1
+        return layer_system.number_of_layers() - 1, -1
+
+    def _spectrum(self, k, kappa, alpha, kz, z_ref):
+        """Gaussian angular spectrum on the (kappa, alpha) grid of the source
+        layer with wavenumber k and anchor height z_ref: the waist-w Gaussian
+        around the beam's in-plane wavevector times the shift to the beam's
+        reference point."""
+        centre = k * np.sin(self.polar_angle) * np.array([np.cos(self.azimuthal_angle),
+                                                          np.sin(self.azimuthal_angle)])
+        qx, qy = kappa * np.cos(alpha), kappa * np.sin(alpha)
+        w2 = self.beam_waist ** 2
+        shift = np.asarray(self.reference_point, dtype=float) - np.array([0.0, 0.0, z_ref])
+        envelope = np.exp(-0.25 * w2 * ((qx - centre[0]) ** 2 + (qy - centre[1]) ** 2))
+        return (self.amplitude * w2 / (4 * np.pi)) * envelope * np.exp(-1j * (qx * shift[0] + qy * shift[1]
+                                                                               + kz * shift[2]))
+
+    def plane_wave_expansion(self, layer_system, i, k_parallel_array=None, azimuthal_angles_array=None):
+        """(upgoing, downgoing) PWEs of the beam in layer i (initial_field.py:167-233): the beam's
+        spectrum in its source half-space, polarised TE / TM by its angle to the beam's azimuth (TM
+        negated for a beam travelling down), carried to layer i by the layer response."""
+        kappa = np.asarray(self.get_k_parallel_array() if k_parallel_array is None else k_parallel_array)
+        alpha = np.asarray(self.get_azimuthal_angles_array() if azimuthal_angles_array is None
+                           else azimuthal_angles_array)
+        src, direction = self._source_side(layer_system)
+        n_src = layer_system.refractive_indices[src]
+        if np.imag(n_src):
             warnings.warn('beam coming from absorbing medium')
-        k_iG = niG * self.angular_frequency()
-        z_iG = layer_system.reference_z(iG)
-        pwe = PlaneWaveExpansion(k_iG, kp, al, kind, [0, 0, z_iG], layer_system.lower_zlimit(iG),
-                                 layer_system.upper_zlimit(iG))
-        kGx = k_iG * np.sin(self.polar_angle) * np.cos(self.azimuthal_angle)
-        kGy = k_iG * np.sin(self.polar_angle) * np.sin(self.azimuthal_angle)
-        kpg, alg = np.meshgrid(pwe.k_parallel, pwe.azimuthal_angles, indexing='ij')
-        kx, ky, kz = kpg * np.cos(alg), kpg * np.sin(alg), pwe.k_z()[:, None]
-        w, r = self.beam_waist, self.reference_point
-        g = (self.amplitude * w ** 2 / (4 * np.pi) * np.exp(-w ** 2 / 4 * ((kx - kGx) ** 2 + (ky - kGy) ** 2))
-             * np.exp(-1j * (kx * r[0] + ky * r[1] + kz * (r[2] - z_iG))))
-        ang = alg - self.azimuthal_angle + self.polarization * np.pi / 2
-        pwe.coefficients[0] = g * np.cos(ang)
-        pwe.coefficients[1] = (1 if np.cos(self.polar_angle) > 0 else -1) * g * np.sin(ang)
-        up, dn = layer_response(layer_system, pwe, iG, i)
-        if iG == i:
-            if kind == 'upgoing':
-                up = up + pwe
+        z_src = layer_system.reference_z(src)
+        beam = PlaneWaveExpansion(n_src * self.angular_frequency(), kappa, alpha,
+                                  'upgoing' if direction > 0 else 'downgoing', [0, 0, z_src],
+                                  layer_system.lower_zlimit(src), layer_system.upper_zlimit(src))
+        kap, alp = np.meshgrid(beam.k_parallel, beam.azimuthal_angles, indexing='ij')
+        g = self._spectrum(beam.k, kap, alp, beam.k_z()[:, None], z_src)
+        rel = alp - self.azimuthal_angle + 0.5 * np.pi * self.polarization
+        beam.coefficients[0] = g * np.cos(rel)
+        beam.coefficients[1] = direction * g * np.sin(rel)
+        up, dn = layer_response(layer_system, beam, src, i)
+        if src == i:
+            if direction > 0:
+                up = up + beam
             else:
-                dn = dn + pwe
+                dn = dn + beam
         return up, dn
 
     def spherical_wave_expansion(self, particle, layer_system):

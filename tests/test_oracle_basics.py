@@ -57,6 +57,28 @@ def test_legendre_against_prototype():
     np.testing.assert_almost_equal(taulm[3, 2, 5], -39.706934218093430 + 42.588889121019569j)
 
 
+
+def test_legendre_high_orders_against_closed_forms():
+    """Orders up to 16 (direct coupling of l_max = 8 blocks): the P_m^m prefactor against the exact
+    rational (2m+1)!!/(2 (2m)!!), and P_l^m against scipy's associated Legendre functions with the
+    normalisation sqrt((2l+1)/2 (l-m)!/(l+m)!) and without the Condon-Shortley phase."""
+    import math
+    from fractions import Fraction
+    import scipy.special
+    for m in range(0, 17):
+        exact = Fraction(1)
+        for t in range(1, m + 1):
+            exact *= Fraction(2 * t + 1, 2 * t)
+        assert abs(fb._prefactor(m) - math.sqrt(exact / 2)) <= 4e-16 * math.sqrt(exact / 2)
+    th = np.array([0.3, 1.1, 2.0, 2.9])
+    plm = fb.legendre_normalized(np.cos(th), np.sin(th), 16)[0]
+    for l in range(17):
+        for m in range(l + 1):
+            norm = math.sqrt((2 * l + 1) / 2 * math.factorial(l - m) / math.factorial(l + m))
+            want = (-1) ** m * norm * scipy.special.lpmv(m, l, np.cos(th))
+            np.testing.assert_allclose(plm[l, m].real, want, rtol=1e-11, atol=1e-13)
+            assert np.all(plm[l, m].imag == 0)
+
 ZS = np.array([0.01, 2, 5, 2 + 0.1j, 3 - 0.2j, 20 + 20j])
 
 

@@ -45,12 +45,21 @@ def test_wr_against_prototype(parts):
     assert abs((w01[1, 0] - want) / want) < 1e-5
     # The reference asserts `abs(x - want) / want < 1e-5` for the next two
     # literals (:37-41); with a complex `want` of negative real part numpy's
-    # ordering makes that true for any x. What the literals do pin is the real
-    # part, which agrees to 1e-10; the imaginary parts differ by 8e-5 / 7e-6.
-    for got, want in ((w00[0, 0], -0.058321374924359 - 0.030731607595288j),
-                      (w01[0, 0], -0.065332285111172 - 0.007633555190358j)):
+    # ordering makes that true for any x. On the test's contour (truncated at
+    # neff 3) the real parts agree to 1e-10 and the imaginary parts differ by
+    # 4e-5 / 8e-7 of |want|: the literals hold the integral over a longer
+    # contour. The evanescent tail beyond neff 3 adds (almost) only an
+    # imaginary part here; with the same waypoints continued to neff 6 the
+    # oracle gives both complex literals to 1e-8.
+    wg_literals = ((-0.058321374924359 - 0.030731607595288j), (-0.065332285111172 - 0.007633555190358j))
+    for got, want in zip((w00[0, 0], w01[0, 0]), wg_literals):
         assert abs(got.real - want.real) / abs(want.real) < 1e-10
         assert np.complex128(abs(got - want) / want) < 1e-5
+    kp6 = _contour([0, 0.8, 0.8 - 0.1j, 2.1 - 0.1j, 2.1, 6])
+    w00 = cp.layer_mediated_coupling_block(WL, p1, p1, wg, kp6)
+    w01 = cp.layer_mediated_coupling_block(WL, p1, p2, wg, kp6)
+    for got, want in zip((w00[0, 0], w01[0, 0]), wg_literals):
+        assert abs(got - want) < 1e-8 * abs(want)
 
 
 def test_w_against_prototype(parts):

@@ -231,6 +231,22 @@ int smh_op_download_table(smh_op *op, double *table, double *table_minus);
  * Radial operators apply nrhs > 1 in blocks of 8 with one batched launch each. */
 int smh_op_matvec(smh_op *op, const double *x, double *y, int64_t nrhs, int precision);
 
+/* y = W x like smh_op_matvec for one vector, allocating nothing: no
+ * counterpart in smuthi; it is for callers that cannot allocate device memory
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * allocation). x, y: DEVICE pointers to complex128 [n_unk] in smuthi's
+ * system-vector layout, owned by the caller. Radial lookup operators, FP32 or
+ * FP64. Its staging (padded vectors, input bound, per-work-item partials)
+ * lives in a fixed 64 MiB device array per GPU, held until the process ends;
+ * operators whose padded vectors and one work item's partial do not fit it
+ * are rejected with SMH_ERR_ARG (about 1 million padded unknowns in FP64).
+ * Every call on a GPU shares that array, so calls that overlap on one GPU
+ * (other threads, other contexts of the device) race; the call is
+ * synchronous on the context's stream. Same result as smh_op_matvec (the
+ * same kernels; the work items run in ranges that fit the array and are
+ * summed in the same order). */
+int smh_op_matvec_noalloc(smh_op *op, const double *x, double *y, int precision);
+
 /* Time `iters` matvecs on device-resident synthetic vectors; returns the
  * average time of the coupling kernel (HIP events on the operator's stream)
  * and of the whole matvec in milliseconds. */

@@ -75,6 +75,7 @@ SIGNATURES = {
     'smh_radial_build_table': (_c.c_int, [_p, _c.c_double, _c.c_double, _c.c_double, _dp, _i64, _dp, _dp]),
     'smh_op_download_table': (_c.c_int, [_p, _dp, _dp]),
     'smh_op_matvec': (_c.c_int, [_p, _dp, _dp, _i64, _c.c_int]),
+    'smh_op_matvec_noalloc': (_c.c_int, [_p, _c.c_void_p, _c.c_void_p, _c.c_int]),
     'smh_op_bench_matvec': (_c.c_int, [_p, _c.c_int, _c.c_int, _dp, _dp]),
     'smh_op_bench_matvec_batch': (_c.c_int, [_p, _c.c_int, _c.c_int, _dp, _dp]),
     'smh_op_half_interp': (_c.c_int, [_p, _ip, _dp]),

@@ -21,7 +21,13 @@ struct smh_ctx {
   char arch[64] = {0};
   int live_ops = 0;               // operators created on this context and not yet destroyed
   bool destroy_requested = false; // smh_destroy called while operators were alive
+  // device address of this device's fixed scratch array (smh_runtime.hip),
+  // resolved once in smh_init; used by the entry points that allocate nothing
+  char* scratch = nullptr;
 };
+
+// Size of the fixed per-device scratch array of the no-allocation entry points.
+constexpr size_t kScratchBytes = (size_t)64 << 20;
 
 // Release a context whose destruction was requested once its last operator is gone.
 void smh_ctx_release_if_idle(smh_ctx* ctx);

@@ -1265,6 +1265,28 @@ extern "C" int smh_op_matvec(smh_op* op, const double* x, double* y, int64_t nrh
 }
 
 namespace {
+// Sum the partials of items [k0, k1) of every receiver chunk onto y, in item
+// order (partial of item w at ypart[w - k0]); with y zero before the first
+// range, a sequence of ranges adds exactly what reduce_kernel adds at once.
+template <typename Real>
+__global__ void reduce_range_kernel(const Real* __restrict__ ypart, const int* __restrict__ chunk_items, int k0, int k1,
+                                    int Rc, int B, int64_t n_part, typename Vec2<Real>::t* y) {
+  const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n_part * B) return;
+  const int64_t i1 = j / B;
+  const int n1 = (int)(j - i1 * B);
+  const int c = (int)(i1 / Rc), il = (int)(i1 - (int64_t)c * Rc);
+  const int w0 = max(chunk_items[c], k0), w1 = min(chunk_items[c + 1], k1);
+  if (w0 >= w1) return;
+  Real re = y[j].x, im = y[j].y;
+  for (int w = w0; w < w1; ++w) {
+    const Real* p = ypart + ((size_t)(w - k0) * Rc + il) * 2 * B;
+    re += p[n1];
+    im += p[B + n1];
+  }
+  y[j] = typename Vec2<Real>::t{re, im};
+}
+
 // Synthetic input: n entries into whichever of x32 / x64 is given.
 __global__ void fill_pattern_kernel(float2* x32, double2* x64, int64_t n) {
   const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -1359,5 +1381,86 @@ extern "C" int smh_op_bench_matvec(smh_op* op, int precision, int iters, double*
   hipEventDestroy(t1);
   if (kernel_ms) *kernel_ms = ksum / iters;
   if (total_ms) *total_ms = tot / iters;
+  return SMH_OK;
+}
+
+extern "C" int smh_op_matvec_noalloc(smh_op* op, const double* x, double* y, int precision) {
+  if (!op) return SMH_ERR_ARG;
+  smh_ctx* ctx = op->ctx;
+  SMH_CHECK(ctx, x && y, SMH_ERR_ARG, "invalid matvec arguments");
+  SMH_CHECK(ctx, op->kind == 0, SMH_ERR_UNSUPPORTED, "the no-allocation matvec covers radial lookup operators");
+  SMH_CHECK(ctx, precision == SMH_FP32 || precision == SMH_FP64, SMH_ERR_ARG, "invalid precision %d", precision);
+  SMH_CHECK(ctx, op->has_table, SMH_ERR_STATE, "coupling table not set");
+  SMH_CHECK(ctx, ctx->scratch != nullptr, SMH_ERR_STATE, "no scratch address on this context");
+  const int pi = precision == SMH_FP32 ? 0 : 1;
+  const smh_op::Binning& bn = op->bin[pi];
+  SMH_CHECK(ctx, bn.Rc > 0, SMH_ERR_UNSUPPORTED, "no %s kernel fits LDS for blocksize %d", pi == 0 ? "FP32" : "FP64",
+            op->B);
+  const KernelInfo k = radial_kernel(precision, op->T, op->l_max, op->m_max);
+  SMH_CHECK(ctx, k.fn != nullptr, SMH_ERR_UNSUPPORTED, "no matvec kernel for l_max %d m_max %d", op->l_max,
+            op->m_max);
+  // scratch layout: input bound (xmax), padded input, padded output, the
+  // partials of as many work items as fit (launched in ranges of that size)
+  const size_t rs = pi == 0 ? sizeof(float) : sizeof(double);
+  const int64_t n_pad = op->n_part * op->B;
+  const size_t head = 4096, vec = ((size_t)n_pad * 2 * rs + 255) / 256 * 256;
+  const size_t per_item = (size_t)bn.Rc * 2 * op->B * rs;
+  SMH_CHECK(ctx, head + 2 * vec + per_item <= kScratchBytes, SMH_ERR_ARG,
+            "operator too large for the fixed scratch: %zu bytes needed (%lld padded unknowns, %zu per work item), "
+            "%zu available",
+            head + 2 * vec + per_item, (long long)n_pad, per_item, kScratchBytes);
+  const int range = (int)std::min<size_t>((kScratchBytes - head - 2 * vec) / per_item, (size_t)1 << 20);
+  SMH_HIP(ctx, hipSetDevice(ctx->device));
+  hipStream_t s = ctx->stream;
+  char* base = ctx->scratch;
+  float* xmax = reinterpret_cast<float*>(base);
+  void* xp = base + head;
+  void* yp = base + head + vec;
+  void* part = base + head + 2 * vec;
+  fill_zero(xp, (size_t)n_pad * 2 * rs, s);
+  fill_zero(yp, (size_t)n_pad * 2 * rs, s);
+  if (pi == 0)
+    hipLaunchKernelGGL(scatter_kernel<float>, dim3(blocks_for(op->n_unk)), dim3(256), 0, s, (const double2*)x,
+                       op->d_pk2pad, op->n_unk, (float2*)xp);
+  else
+    hipLaunchKernelGGL(scatter_kernel<double>, dim3(blocks_for(op->n_unk)), dim3(256), 0, s, (const double2*)x,
+                       op->d_pk2pad, op->n_unk, (double2*)xp);
+  if (pi == 0) launch_xmax((const float2*)xp, n_pad, xmax, s);
+  if (int rc_ = op_set_lds(op, k.fn, bn.lds)) return rc_;
+  for (int k0 = 0; k0 < bn.n_items; k0 += range) {
+    const int k1 = std::min(bn.n_items, k0 + range);
+    RadialKArgs a;
+    a.recs = bn.d_rec;
+    a.entries = bn.d_entries;
+    a.items = bn.d_items;
+    a.order = nullptr;
+    a.item0 = k0;
+    a.tt = pi == 0 ? (const void*)op->d_tt32 : (const void*)op->d_tt64;
+    a.x = xp;
+    a.ypart = part;
+    a.xmax = xmax;
+    a.item_lsum = bn.d_item_lsum;
+    a.item_en = bn.d_item_en;
+    a.n_items = bn.n_items;
+    a.n_entries = bn.n_entries;
+    a.interp = op->interp;
+    a.Rc = bn.Rc;
+    void* args[] = {&a};
+    SMH_HIP(ctx, hipLaunchKernel(k.fn, dim3(k1 - k0), dim3(k.threads), args, bn.lds, s));
+    if (pi == 0)
+      hipLaunchKernelGGL(reduce_range_kernel<float>, dim3(blocks_for(n_pad)), dim3(256), 0, s, (const float*)part,
+                         bn.d_chunk_items, k0, k1, bn.Rc, op->B, op->n_part, (float2*)yp);
+    else
+      hipLaunchKernelGGL(reduce_range_kernel<double>, dim3(blocks_for(n_pad)), dim3(256), 0, s, (const double*)part,
+                         bn.d_chunk_items, k0, k1, bn.Rc, op->B, op->n_part, (double2*)yp);
+  }
+  if (pi == 0)
+    hipLaunchKernelGGL(pack_kernel<float>, dim3(blocks_for(op->n_unk)), dim3(256), 0, s, (const float2*)yp,
+                       op->d_pk2pad, op->n_unk, (double2*)y);
+  else
+    hipLaunchKernelGGL(pack_kernel<double>, dim3(blocks_for(op->n_unk)), dim3(256), 0, s, (const double2*)yp,
+                       op->d_pk2pad, op->n_unk, (double2*)y);
+  SMH_HIP(ctx, hipGetLastError());
+  SMH_HIP(ctx, hipStreamSynchronize(s));
   return SMH_OK;
 }

@@ -4,6 +4,11 @@
 #include "smh_common.h"
 #include "smh_special.h"
 
+// Fixed scratch of the entry points that allocate nothing (smh_op_matvec_noalloc):
+// one copy per device, alive until the process ends. Host code uses only the
+// device address that smh_init resolves into ctx->scratch, never this name.
+static __device__ __align__(256) char scratch[kScratchBytes];
+
 int smh_fail(smh_ctx* ctx, int code, const char* fmt, ...) {
   char buf[1024];
   va_list ap;
@@ -32,6 +37,17 @@ extern "C" int smh_init(int device, smh_ctx** out) {
     delete ctx;
     return SMH_ERR_HIP;
   }
+  // this device's copy of the scratch array (the device is current)
+  void* sp = nullptr;
+  if (e == hipSuccess) e = hipGetSymbolAddress(&sp, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || sp == nullptr) {
+    hipEventDestroy(ctx->ev0);
+    hipEventDestroy(ctx->ev1);
+    hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return SMH_ERR_HIP;
+  }
+  ctx->scratch = static_cast<char*>(sp);
   ctx->num_cu = prop.multiProcessorCount;
   ctx->lds_per_block = prop.sharedMemPerBlock;
   snprintf(ctx->arch, sizeof(ctx->arch), "%s", prop.gcnArchName);

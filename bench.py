@@ -6,16 +6,21 @@ U(1.5,2.0) + 0.01i) at z = 200 nm in a 60 x 60 um square with 20 nm minimum
 gaps, on glass (n = 1.52) in air, lambda = 550 nm, l_max = m_max = 3
 (300,000 unknowns), radial lookup with cubic (not-a-knot spline)
 interpolation at 2 nm, Sommerfeld contour with neff_resolution = neff_imag =
-2e-3. A step solves a batch of right-hand sides (plane waves; incidence angle and
-polarisation vary; 8 per step by default) to relative residual 1e-6: GMRES
-for the whole batch in lockstep whose coupling matvecs run with FP16
-operands on MFMA (one launch per 8 RHS; --precision mixed: FP32), wrapped in
-FP64 iterative refinement (FP32 residuals between sweeps, the FP64 residual
-certifies the result). The coefficients of right-hand side 0 are compared with a plain
-FP64 GPU GMRES solve (tol 1e-9) of the same right-hand side; single-RHS
-solve and plain FP64 GMRES throughputs are reported beside.
+2e-3. The right-hand sides are 64 distinct plane waves (32 polar angles from
+0 to 60 degrees, TE and TM); a step solves the next 8 of them to relative
+residual 1e-6: GMRES for the whole batch in lockstep whose coupling matvecs
+run with FP16 operands on MFMA (one launch per 8 RHS; --precision mixed:
+FP32), wrapped in FP64 iterative refinement (FP32 residuals between sweeps,
+the FP64 residual certifies the result). Every timed right-hand side is
+compared with an FP64 GMRES solve (tol 1e-10) of it, and only those within
+1e-5 count. Batched FP64 GMRES, single-RHS FP64 GMRES and the FP32-inner
+refinement are timed on the same incidences beside it, the CPU baseline on
+the host's cores. Before timing, the autotuner picks the matvec kernels'
+launch shapes (setup, reported under "autotune").
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--particles N] [--batch R]
+       python bench.py --config 4   (ensembles of uneven size sharing one volume table, one JSON line)
+       python bench.py --config 5   (learned near-field preconditioner vs plain GMRES, one JSON line)
        python bench.py --near-field    (near-field kernels at the same ensemble, one JSON line)
        python bench.py --explicit N    (explicit coupling matrix of N spheres: build and matvecs, one JSON line)
        python bench.py --rhs           (Gaussian-beam and dipole right-hand sides at the ensemble, one JSON line)
@@ -400,6 +405,25 @@ def cfg1_cpu_vs_gpu(workdir, tol=1e-5):
                          'drop-in), one plane wave' % tol)
 
 
+def autotune(op, shapes, enabled=True):
+    """Launch-shape autotuning of the operator's kernels (setup, outside the timed region): per shape
+    (batch, precision) the kernel time of every candidate grouping and the one kept."""
+    if not enabled:
+        return None
+    out = {}
+    t0 = time.time()
+    for batch, prec in shapes:
+        r = op.autotune(batch=batch, precision=prec, iters=2)
+        out['%s_%s' % ('8rhs' if batch else '1rhs', prec)] = {
+            'best': {'items_per_cu': r['best'][0], 'xcd_order': bool(r['best'][1])},
+            'kernel_ms': {'%d/%s' % (k[0], 'xcd' if k[1] else 'chunk'): round(v, 4) for k, v in r['kernel_ms'].items()}}
+        log('autotune %s %s: best %s, %.3f ms (default 16/xcd %.3f ms)'
+            % ('8-RHS' if batch else '1-RHS', prec, r['best'], r['kernel_ms'][r['best']],
+               r['kernel_ms'].get((16, 1), float('nan'))))
+    out['seconds'] = time.time() - t0
+    return out
+
+
 def _errors(x, ref):
     return [float(np.linalg.norm(a - b) / np.linalg.norm(b)) for a, b in zip(x, ref)]
 
@@ -435,6 +459,7 @@ def main_headline(args):
         raise SystemExit('at most %d distinct right-hand sides (warmup + steps <= %d)' % (len(inc), len(inc) // nb))
     rhs = np.stack([t_diag * PlaneWave(WL, *inc[j]).swe_coefficients(plist, ls) for j in range(n_rhs)])
 
+    tune = autotune(op, ((True, 'half'), (True, 'double'), (False, 'single')), not args.no_autotune)
     # coupling matvec kernels (HIP events on the operator's stream)
     k32, tot32 = op.bench_matvec('single', iters=args.matvec_iters)
     k64, tot64 = op.bench_matvec('double', iters=max(2, args.matvec_iters // 2))
@@ -454,19 +479,32 @@ def main_headline(args):
         x, st, ok = op.solve_master(t_diag, rhs[rows], tol, precision=precision, with_stats=True)
         return x, st, ok
 
+    # the right-hand sides and T in HBM before the timed region (cuda.DeviceArray); the solutions stay there
+    # until the region ends. Every library call returns after its stream's work is done, so the solves need no
+    # further synchronization around the timed region.
+    from smuthi_amd.cuda import DeviceArray
+    rhs_dev = DeviceArray.from_host(rhs)
+    t_dev = DeviceArray.from_host(t_diag)
+
     # timed region: batched solves at the requested inner precision, distinct incidences every step
     prec = args.precision
     for w in range(args.warmup):
-        solve(slice(nb * w, nb * w + nb), prec, args.tol)
+        op.solve_master(t_dev, rhs_dev[nb * w:nb * w + nb], args.tol, precision=prec, with_stats=True)
     t0 = time.time()
     sols, stats = [], []
     for stp in range(args.warmup, args.warmup + args.steps):
-        x, stt, _ = solve(slice(nb * stp, nb * stp + nb), prec, args.tol)
+        x, stt, _ = op.solve_master(t_dev, rhs_dev[nb * stp:nb * stp + nb], args.tol, precision=prec,
+                                    with_stats=True)
         sols.append(x)
         stats.append(stt)
     elapsed = time.time() - t0
     timed = slice(nb * args.warmup, n_rhs)
-    x_timed = np.concatenate(sols)
+    x_timed = np.concatenate([x.to_host() for x in sols])
+    # the same steps from host arrays (PCIe copies of rhs, T and x inside each solve)
+    t0 = time.time()
+    for stp in range(args.warmup, args.warmup + args.steps):
+        solve(slice(nb * stp, nb * stp + nb), prec, args.tol)
+    host_elapsed = time.time() - t0
     # every timed right-hand side against an FP64 solve of it (FP64 GMRES to 1e-10)
     t0 = time.time()
     x_ref, ref_stats, _ = solve(timed, 'double', 1e-10)
@@ -540,6 +578,9 @@ def main_headline(args):
                       'fp32_kernel_8rhs': kb32, 'fp32_total_8rhs': totb32, 'fp64_kernel_8rhs': kb64,
                       'fp64_total_8rhs': totb64},
         'rhs_checked': len(errs), 'rhs_within_1e-5': n_ok,
+        'inputs': 'right-hand sides and T resident in HBM (cuda.DeviceArray) when the timed region starts',
+        'pcie_inclusive': {'rhs_solves_per_s': len(errs) / host_elapsed, 'ms_per_step': host_elapsed / args.steps * 1e3,
+                           'note': 'the same steps from host numpy arrays'},
         'coef_err_vs_fp64_worst': max(errs), 'coef_err_vs_fp64_median': float(np.median(errs)),
         'worst_relative_residual': worst_res,
         'solve_stats': stats,
@@ -552,6 +593,7 @@ def main_headline(args):
         'speedup_vs_fp64_gmres_single_rhs': value * fp64_single_s,
         'prepare_seconds': prep_s,
         'table_seconds': op.table_seconds,
+        'autotune': tune,
         'roofline': {'bound': 'valu-issue (VALU busy ~71 %, MFMA ~9 % in the SQ counters)' if half else 'mfma',
                      'achieved': achieved, 'peak': peak, 'unit': 'TFLOP/s',
                      'frac': achieved / peak,
@@ -608,6 +650,7 @@ def main_batching(args):
     rhs = batch.right_hand_sides([PlaneWave(WL, *inc[j]) for j in range(n_fields)])
     log('right-hand sides %.1f s' % (time.time() - t0))
     op = batch.operator
+    tune = autotune(op, ((True, 'single'), (True, 'double')), not args.no_autotune)
     k32, tot32 = op.bench_matvec_batch('single', iters=2)
     k64, tot64 = op.bench_matvec_batch('double', iters=2)
     for w in range(args.warmup):
@@ -662,7 +705,7 @@ def main_batching(args):
                      'frac': achieved / MFMA_FP32_PEAK_TFLOPS, 'traffic': None,
                      'kernel': 'batch_matvec_kernel<float,float,3,3,4,true> (8 launches: 2 tables x 4 z slices)',
                      'flops_per_matvec': flops},
-        'table_seconds': batch.table_seconds, 'setup_seconds': prep,
+        'table_seconds': batch.table_seconds, 'setup_seconds': prep, 'autotune': tune,
         'device': dev, 'compute_units': n_cu,
     }
     return json.dumps(out)
@@ -720,6 +763,7 @@ def main_learned(args):
                                                                              L1))
     del train
     test = make([args.test_size], 900)
+    tune = autotune(test.operator, ((True, 'single'), (True, 'double')), not args.no_autotune)
     _, feat_t = prepare(lp, test)
     rhs = test.right_hand_sides(fields)
     x_ref, _, _ = test.solve_with_stats(rhs, tol=1e-10, precision='double')
@@ -751,6 +795,7 @@ def main_learned(args):
         'iterations': {k: v['iterations'] for k, v in res.items()},
         'fallback_rate': sum(v['fallback'] for v in res.values() if v is not res['plain']) / 2.0,
         'speedup_vs_plain': res['learned']['rhs_solves_per_s'] / res['plain']['rhs_solves_per_s'],
+        'autotune': tune,
         'results': res, 'training': {'loss_untrained': L0, 'loss_trained': L1, 'seconds': train_s,
                                      'data_seconds': data_s, 'rows': args.train_size * args.train_ensembles,
                                      'solve_stats': stx},
@@ -770,6 +815,7 @@ def main():
     ap.add_argument('--precision', default='mixed16', choices=['mixed16', 'mixed', 'double'],
                     help='inner matvec precision of the timed solves (mixed16: FP16 operands, mixed: FP32)')
     ap.add_argument('--no-cpu-baseline', action='store_true')
+    ap.add_argument('--no-autotune', action='store_true', help='keep the default launch shapes')
     ap.add_argument('--matvec-iters', type=int, default=5)
     ap.add_argument('--config', type=int, default=2, choices=[2, 4, 5],
                     help='2: the 10k-sphere solve (default); 4: ensembles batched; 5: learned preconditioner')

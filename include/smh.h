@@ -52,6 +52,15 @@ int smh_destroy(smh_ctx *ctx);
 const char *smh_last_error(const smh_ctx *ctx);
 int smh_device_info(smh_ctx *ctx, char *name, int name_len, int *num_cu);
 
+/* Device memory on the context's GPU for callers that keep right-hand sides
+ * and solutions in HBM (smh_solve accepts device pointers): allocate, free
+ * (after the context's stream drains) and copy in any direction (host or
+ * device pointers on either side, synchronous on the context's stream). No
+ * counterpart in smuthi, whose vectors live in host memory. */
+int smh_device_alloc(smh_ctx *ctx, int64_t bytes, void **out);
+int smh_device_free(smh_ctx *ctx, void *p);
+int smh_device_copy(smh_ctx *ctx, void *dst, const void *src, int64_t bytes);
+
 /* ---- radial lookup coupling operator -----------------------------------
  * Replaces CouplingMatrixRadialLookupCUDA.__init__ (linearsystem/linear_system.py:826-886):
  * particle coordinates and multipole cut-offs, table grid
@@ -284,8 +293,11 @@ int smh_op_half_interp(smh_op *op, int *terms, double *lin_ratio);
 /* Solve (1 - T W) b = T a (LinearSystem.solve, linear_system.py:254-344)
  * for nrhs right-hand sides.
  * t_diag: complex128 [n_unk] diagonal sphere T-matrix; rhs, x: complex128
- * [nrhs][n_unk]. SMH_SOLVE_FP64 (solver_type 'gmres') starts from x0 = rhs as
- * smuthi does; SMH_SOLVE_MIXED / MIXED16 refine from x0 = 0. They stop at
+ * [nrhs][n_unk]. t_diag, rhs and x may each be host or device memory (HIP's
+ * unified addressing decides the copy direction), so callers that keep the
+ * right-hand sides in HBM solve without PCIe traffic. SMH_SOLVE_FP64
+ * (solver_type 'gmres') starts from x0 = rhs as smuthi does;
+ * SMH_SOLVE_MIXED / MIXED16 refine from x0 = 0. They stop at
  * ||rhs - M x|| <= tol ||rhs|| measured in FP64 and return SMH_ERR_NOCONV
  * when maxiter Arnoldi steps do not get there. SMH_SOLVE_LU (solver_type
  * 'LU', linear_system.py:259-268, explicit operators only) factors M on the

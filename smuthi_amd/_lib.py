@@ -75,6 +75,9 @@ SIGNATURES = {
     'smh_radial_build_table': (_c.c_int, [_p, _c.c_double, _c.c_double, _c.c_double, _dp, _i64, _dp, _dp]),
     'smh_op_download_table': (_c.c_int, [_p, _dp, _dp]),
     'smh_op_matvec': (_c.c_int, [_p, _dp, _dp, _i64, _c.c_int]),
+    'smh_device_alloc': (_c.c_int, [_p, _i64, _c.POINTER(_c.c_void_p)]),
+    'smh_device_free': (_c.c_int, [_p, _c.c_void_p]),
+    'smh_device_copy': (_c.c_int, [_p, _c.c_void_p, _c.c_void_p, _i64]),
     'smh_op_matvec_noalloc': (_c.c_int, [_p, _c.c_void_p, _c.c_void_p, _c.c_int]),
     'smh_op_bench_matvec': (_c.c_int, [_p, _c.c_int, _c.c_int, _dp, _dp]),
     'smh_op_bench_matvec_batch': (_c.c_int, [_p, _c.c_int, _c.c_int, _dp, _dp]),
@@ -125,6 +128,9 @@ def load():
         fn.argtypes = args
     _LIB = lib
     return lib
+
+
+DPTR = _dp  # pointer type of the float64 / complex128 arguments (also for device addresses)
 
 
 def dptr(a):

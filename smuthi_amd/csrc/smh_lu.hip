@@ -130,7 +130,7 @@ int lu_solve(smh_op* op, const double* t_diag, const double* rhs, double* x, int
     int rc__ = (call);       \
     if (rc__) return done(rc__); \
   } while (0)
-  LU_HIP(hipMemcpyAsync(op->d_pk_in, t_diag, op->n_unk * sizeof(double2), hipMemcpyHostToDevice, s));
+  LU_HIP(hipMemcpyAsync(op->d_pk_in, t_diag, op->n_unk * sizeof(double2), hipMemcpyDefault, s));
   LU_TRY(op_unpack(op, SMH_FP64, op->d_pk_in, d_t));
   // the factors of an earlier call serve while T is the same (the reference
   // keeps LU_piv on the master matrix)
@@ -157,7 +157,7 @@ int lu_solve(smh_op* op, const double* t_diag, const double* rhs, double* x, int
     op->lu_t = tkey;
   }
   // right-hand sides in the padded layout, one column each
-  LU_HIP(hipMemcpyAsync(op->d_pk_in, rhs, (size_t)nrhs * op->n_unk * sizeof(double2), hipMemcpyHostToDevice, s));
+  LU_HIP(hipMemcpyAsync(op->d_pk_in, rhs, (size_t)nrhs * op->n_unk * sizeof(double2), hipMemcpyDefault, s));
   for (int64_t r = 0; r < nrhs; ++r) {
     LU_TRY(op_unpack(op, SMH_FP64, op->d_pk_in + r * op->n_unk, d_b + r * n));
   }
@@ -175,7 +175,7 @@ int lu_solve(smh_op* op, const double* t_diag, const double* rhs, double* x, int
   std::vector<double> parts((size_t)nrhs * kParts * 2);
   LU_HIP(hipMemcpyAsync(parts.data(), d_parts, parts.size() * sizeof(double), hipMemcpyDeviceToHost, s));
   for (int64_t r = 0; r < nrhs; ++r) LU_TRY(op_pack(op, SMH_FP64, d_x + r * n, op->d_pk_out + r * op->n_unk));
-  LU_HIP(hipMemcpyAsync(x, op->d_pk_out, (size_t)nrhs * op->n_unk * sizeof(double2), hipMemcpyDeviceToHost, s));
+  LU_HIP(hipMemcpyAsync(x, op->d_pk_out, (size_t)nrhs * op->n_unk * sizeof(double2), hipMemcpyDefault, s));
   LU_HIP(hipStreamSynchronize(s));
   double worst = 0;
   for (int64_t r = 0; r < nrhs; ++r) {

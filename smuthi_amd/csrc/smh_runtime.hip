@@ -97,6 +97,34 @@ extern "C" int smh_device_info(smh_ctx* ctx, char* name, int name_len, int* num_
   return SMH_OK;
 }
 
+extern "C" int smh_device_alloc(smh_ctx* ctx, int64_t bytes, void** out) {
+  if (!ctx || !out) return SMH_ERR_ARG;
+  *out = nullptr;
+  SMH_CHECK(ctx, bytes > 0, SMH_ERR_ARG, "allocation size must be positive");
+  SMH_HIP(ctx, hipSetDevice(ctx->device));
+  SMH_HIP(ctx, hipMalloc(out, (size_t)bytes));
+  return SMH_OK;
+}
+
+extern "C" int smh_device_free(smh_ctx* ctx, void* p) {
+  if (!ctx) return SMH_ERR_ARG;
+  if (!p) return SMH_OK;
+  SMH_HIP(ctx, hipSetDevice(ctx->device));
+  SMH_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  SMH_HIP(ctx, hipFree(p));
+  return SMH_OK;
+}
+
+extern "C" int smh_device_copy(smh_ctx* ctx, void* dst, const void* src, int64_t bytes) {
+  if (!ctx) return SMH_ERR_ARG;
+  SMH_CHECK(ctx, bytes >= 0 && (bytes == 0 || (dst && src)), SMH_ERR_ARG, "invalid copy arguments");
+  if (bytes == 0) return SMH_OK;
+  SMH_HIP(ctx, hipSetDevice(ctx->device));
+  SMH_HIP(ctx, hipMemcpyAsync(dst, src, (size_t)bytes, hipMemcpyDefault, ctx->stream));
+  SMH_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return SMH_OK;
+}
+
 extern "C" int smh_host_bessel_j(int nmax, int64_t count, const double* z, double* out) {
   if (nmax < 0 || nmax > smh::kMaxBesselOrder || count < 0 || (!z && count) || (!out && count)) return SMH_ERR_ARG;
   smh::zc buf[smh::kMaxBesselOrder + 1];

@@ -811,8 +811,13 @@ struct BatchSolver {
     const int key = ((pi * 2 + ui) << 8) | k;
     auto it = ws->graphs.find(key);
     if (it == ws->graphs.end()) {
+      // the matvec counters count launches: the replay below counts this step's one matvec
+      const int64_t c64 = mv64, c32 = mv32, c16 = mv16;
       SMH_HIP(ctx, hipStreamBeginCapture(s, hipStreamCaptureModeRelaxed));
       const int rc = arnoldi_step(k, precision);
+      mv64 = c64;
+      mv32 = c32;
+      mv16 = c16;
       hipGraph_t graph = nullptr;
       const hipError_t e = hipStreamEndCapture(s, &graph);
       if (rc || e != hipSuccess) {
@@ -826,6 +831,12 @@ struct BatchSolver {
       it = ws->graphs.emplace(key, exec).first;
     }
     SMH_HIP(ctx, hipGraphLaunch(it->second, s));
+    if (precision == SMH_FP64)
+      ++mv64;
+    else if (precision == SMH_FP16)
+      ++mv16;
+    else
+      ++mv32;
     return SMH_OK;
   }
 
@@ -958,7 +969,7 @@ static int solve_batched(smh_op* op, const double* t_diag, const double* rhs, do
   const int nsys = S.nsys;
   if ((rc = smh::op_reserve_packed(op, kB))) return rc;
   if ((rc = smh::op_reserve_batch(op))) return rc;
-  SMH_HIP(ctx, hipMemcpyAsync(op->d_pk_in, t_diag, op->n_unk * sizeof(double2), hipMemcpyHostToDevice, s));
+  SMH_HIP(ctx, hipMemcpyAsync(op->d_pk_in, t_diag, op->n_unk * sizeof(double2), hipMemcpyDefault, s));
   if ((rc = smh::op_unpack(op, SMH_FP64, op->d_pk_in, ws->t))) return rc;
   double worst = 0, sweeps_total = 0;
   int64_t steps_total = 0;
@@ -968,7 +979,7 @@ static int solve_batched(smh_op* op, const double* t_diag, const double* rhs, do
   for (int64_t r0 = 0; r0 < nrhs; r0 += kB) {
     const int nb = (int)std::min<int64_t>(kB, nrhs - r0);
     SMH_HIP(ctx, hipMemcpyAsync(op->d_pk_in, rhs + 2 * r0 * op->n_unk, (size_t)nb * op->n_unk * sizeof(double2),
-                                hipMemcpyHostToDevice, s));
+                                hipMemcpyDefault, s));
     if ((rc = smh::op_unpack_batch(op, SMH_FP64, op->d_pk_in, nb, ws->b))) return rc;
     if ((rc = S.norms(ws->b, bnorm.data()))) return rc;
     for (int q = 0; q < nsys; ++q) {
@@ -1056,7 +1067,7 @@ static int solve_batched(smh_op* op, const double* t_diag, const double* rhs, do
     }
     if ((rc = smh::op_pack_batch(op, SMH_FP64, ws->x, nb, op->d_pk_out))) return rc;
     SMH_HIP(ctx, hipMemcpyAsync(x + 2 * r0 * op->n_unk, op->d_pk_out, (size_t)nb * op->n_unk * sizeof(double2),
-                                hipMemcpyDeviceToHost, s));
+                                hipMemcpyDefault, s));
   }
   SMH_HIP(ctx, hipStreamSynchronize(s));
   const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
@@ -1092,14 +1103,14 @@ static int solve_single(smh_op* op, const double* t_diag, const double* rhs, dou
   Ws* ws = S.ws;
   if ((rc = smh::op_reserve_packed(op, 1))) return rc;
   // T diagonal in padded layout
-  SMH_HIP(ctx, hipMemcpyAsync(op->d_pk_in, t_diag, op->n_unk * sizeof(double2), hipMemcpyHostToDevice, s));
+  SMH_HIP(ctx, hipMemcpyAsync(op->d_pk_in, t_diag, op->n_unk * sizeof(double2), hipMemcpyDefault, s));
   if ((rc = smh::op_unpack(op, SMH_FP64, op->d_pk_in, ws->t1))) return rc;
   double worst = 0, sweeps_total = 0;
   int64_t steps_total = 0;
   bool converged_all = true;
   for (int64_t r = 0; r < nrhs; ++r) {
     SMH_HIP(ctx, hipMemcpyAsync(op->d_pk_in, rhs + 2 * r * op->n_unk, op->n_unk * sizeof(double2),
-                                hipMemcpyHostToDevice, s));
+                                hipMemcpyDefault, s));
     if ((rc = smh::op_unpack(op, SMH_FP64, op->d_pk_in, ws->b1))) return rc;
     double bnorm;
     if ((rc = S.norm(ws->b1, bnorm))) return rc;
@@ -1149,7 +1160,7 @@ static int solve_single(smh_op* op, const double* t_diag, const double* rhs, dou
     if (rel > tol) converged_all = false;
     if ((rc = smh::op_pack(op, SMH_FP64, ws->x1, op->d_pk_out))) return rc;
     SMH_HIP(ctx, hipMemcpyAsync(x + 2 * r * op->n_unk, op->d_pk_out, op->n_unk * sizeof(double2),
-                                hipMemcpyDeviceToHost, s));
+                                hipMemcpyDefault, s));
   }
   SMH_HIP(ctx, hipStreamSynchronize(s));
   const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();

@@ -31,6 +31,27 @@ def _precision_code(precision):
     return _lib.FP64
 
 
+def _device_or_host(a):
+    """(array, pointer) of a complex128 vector or block: a cuda.DeviceArray as it is, anything else as a
+    contiguous numpy array."""
+    if isinstance(a, cu.DeviceArray):
+        return a, ctypes.cast(a.ptr, _lib.DPTR)
+    a = np.ascontiguousarray(a, dtype=np.complex128)
+    return a, _lib.dptr(a)
+
+
+def _solve_buffers(rhs, n):
+    """Right-hand sides and a solution buffer of the same kind (DeviceArray or numpy array)."""
+    b, b_ptr = _device_or_host(rhs)
+    if b.shape[-1] != n or b.ndim not in (1, 2):
+        raise ValueError('rhs must be [n] or [nrhs, n] with n = %d' % n)
+    if isinstance(b, cu.DeviceArray):
+        x = cu.DeviceArray.empty(b.shape)
+        return b, b_ptr, x, ctypes.cast(x.ptr, _lib.DPTR)
+    x = np.empty_like(b)
+    return b, b_ptr, x, _lib.dptr(x)
+
+
 def _interp_code(interpolator_kind, cubic_interpolant):
     if interpolator_kind == 'linear':
         return _lib.INTERP_LINEAR
@@ -132,21 +153,23 @@ class _GpuCouplingOperator(SystemMatrix):
     def solve_master(self, t_diagonal, rhs, tol, precision='mixed', restart=20, maxiter=None, with_stats=False):
         """Solve (1 - diag(t) W) x = rhs on the GPU (rhs [n] or [nrhs, n]).
 
+        rhs and t_diagonal may be numpy arrays or cuda.DeviceArrays (HBM); for a
+        device rhs, x is returned as a DeviceArray and nothing crosses PCIe but
+        the solver's scalars.
         Raises SmhError when GMRES misses the tolerance unless with_stats is set,
         in which case (x, stats, converged) is returned."""
-        b = np.ascontiguousarray(rhs, dtype=np.complex128)
-        t = np.ascontiguousarray(t_diagonal, dtype=np.complex128)
-        if t.shape != (self.shape[0],) or b.shape[-1] != self.shape[0]:
+        b, b_ptr, x, x_ptr = _solve_buffers(rhs, self.shape[0])
+        t, t_ptr = _device_or_host(t_diagonal)
+        if tuple(t.shape) != (self.shape[0],):
             raise ValueError('t_diagonal and rhs must have the operator dimension %d' % self.shape[0])
         nrhs = 1 if b.ndim == 1 else b.shape[0]
-        x = np.empty_like(b)
         stats = np.zeros(8)
         mode = {'mixed': _lib.SOLVE_MIXED, 'mixed16': _lib.SOLVE_MIXED16, 'lu': _lib.SOLVE_LU}.get(
             precision, _lib.SOLVE_FP64)
         maxiter = 10 * self.shape[0] if maxiter is None else maxiter
         maxiter = int(min(maxiter, 2 ** 31 - 1))
-        code = _lib.load().smh_solve(self._op, _lib.dptr(t), _lib.dptr(b), _lib.dptr(x), nrhs, float(tol),
-                                     int(restart), maxiter, mode, _lib.dptr(stats))
+        code = _lib.load().smh_solve(self._op, t_ptr, b_ptr, x_ptr, nrhs, float(tol), int(restart), maxiter, mode,
+                                     _lib.dptr(stats))
         converged = code != _lib.SMH_ERR_NOCONV
         if not with_stats or converged:
             _lib.check(code, cu.context(), 'smh_solve')

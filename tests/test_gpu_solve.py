@@ -131,6 +131,11 @@ def test_batched_solve_matches_oracle_lu(gpu, precision):
     X_ref = np.linalg.solve(M, R.T).T
     X, stats, ok = op.solve_master(t, R, tol=1e-9, precision=precision, with_stats=True)
     assert ok and stats['relative_residual'] <= 1e-9
+    # a second call replays the captured Arnoldi-step graphs: same steps, same matvec counts
+    X2, stats2, ok2 = op.solve_master(t, R, tol=1e-9, precision=precision, with_stats=True)
+    assert ok2 and np.abs(X2 - X).max() <= 1e-12 * np.abs(X).max()
+    for key in ('iterations', 'matvecs_fp64', 'matvecs_fp32', 'matvecs_fp16', 'sweeps'):
+        assert stats2[key] == stats[key], (key, stats, stats2)
     if precision == 'mixed16':
         # FP16 inner matvecs; FP32 residuals between sweeps; FP64 certificates
         assert stats['matvecs_fp16'] > 0 and stats['matvecs_fp64'] >= 1
@@ -138,6 +143,40 @@ def test_batched_solve_matches_oracle_lu(gpu, precision):
         assert np.linalg.norm(X[j] - X_ref[j]) / np.linalg.norm(X_ref[j]) < 1e-7, j
         assert np.linalg.norm(R[j] - M @ X[j]) / np.linalg.norm(R[j]) < 2e-9, j
 
+
+
+@pytest.mark.parametrize('precision', ['mixed16', 'double'])
+def test_device_resident_solve_matches_host(gpu, precision):
+    """Right-hand sides, T and the solution in HBM (cuda.DeviceArray; smh_solve copies device to device): the
+    same solutions as from host arrays, against the oracle's LU; a single device row too."""
+    from smuthi_amd import ensembles
+    from smuthi_amd.cuda import DeviceArray
+    from smuthi_amd.layers import LayerSystem
+    from smuthi_amd.linear_system import CouplingMatrixRadialLookupCUDA
+    pp = ensembles.small_substrate_ensemble(seed=3)[:24]
+    plist = [osys.Sphere(p.position, p.refractive_index, p.radius, 3, 3) for p in pp]
+    kp = ob.reasonable_Sommerfeld_kpar_contour(WL, layer_refractive_indices=[1.52, 1], neff_resolution=1e-2)
+    op = CouplingMatrixRadialLookupCUDA(WL, plist, LayerSystem([0, 0], [1.52, 1]), k_parallel=kp, resolution=5.0,
+                                        interpolator_kind='cubic')
+    W = oc.RadialLookupCPU(op.lookup_table, op.radial_distance_array, plist, 'cubic').dense()
+    rng = np.random.default_rng(9)
+    n = W.shape[0]
+    t = 0.3 + 0.1j + 0.05 * (rng.standard_normal(n) + 1j * rng.standard_normal(n))
+    R = rng.standard_normal((9, n)) + 1j * rng.standard_normal((9, n))
+    X_ref = np.linalg.solve(np.eye(n) - t[:, None] * W, R.T).T
+    X_host = op.solve_master(t, R, tol=1e-9, precision=precision)
+    R_dev = DeviceArray.from_host(R)
+    t_dev = DeviceArray.from_host(t)
+    np.testing.assert_array_equal(R_dev.to_host(), R)
+    np.testing.assert_array_equal(R_dev[2:5].to_host(), R[2:5])
+    X_dev, st, ok = op.solve_master(t_dev, R_dev, tol=1e-9, precision=precision, with_stats=True)
+    assert ok and isinstance(X_dev, DeviceArray) and X_dev.shape == R.shape
+    X_dev = X_dev.to_host()
+    x1 = op.solve_master(t_dev, R_dev[4], tol=1e-9, precision=precision).to_host()
+    for j in range(9):
+        assert np.linalg.norm(X_dev[j] - X_ref[j]) / np.linalg.norm(X_ref[j]) < 1e-7, j
+        assert np.linalg.norm(X_dev[j] - X_host[j]) / np.linalg.norm(X_ref[j]) < 1e-7, j
+    assert np.linalg.norm(x1 - X_ref[4]) / np.linalg.norm(X_ref[4]) < 1e-7
 
 def test_single_rhs_mixed16_solve(gpu):
     """One right-hand side with FP16-operand inner matvecs (padded into one batched launch)."""

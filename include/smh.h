@@ -151,11 +151,14 @@ int smh_mlp_backward(smh_mlp *mlp, int64_t rows, const float *x, const float *dy
  * [n_part][n_in] (float), right-hand sides rhs and the solver's solutions sol
  * = M^{-1} rhs, complex128 [nrhs][n_unk]; the operator's near blocks must be
  * computed. The loss is L = sum_{i,r} |(P rhs_r)_i - sol_{i,r}|^2 /
- * sum |sol|^2 with P from the network's scales. */
+ * sum |sol|^2 + (w / rows) sum |y|^2 with P from the network's scales
+ * s = 1 + y; the second term (weight w, smh_mlp_set_prior_weight, default 0)
+ * pulls the scales towards the first-order structure s = 1. */
 int smh_mlp_add_data(smh_mlp *mlp, smh_op *op, const float *feat, int64_t nrhs, const double *rhs, const double *sol);
 /* L and dL/dy over the training set at outputs y [rows][n_out] (NULL: the
  * network's outputs); dy may be NULL. */
 int smh_mlp_data_loss(smh_mlp *mlp, const float *y, float *dy, double *loss);
+int smh_mlp_set_prior_weight(smh_mlp *mlp, double w);
 /* steps of full-batch Adam (beta 0.9 / 0.999) at learning rate lr on the
  * training set; losses (may be NULL) receives L before each step. */
 int smh_mlp_train(smh_mlp *mlp, int steps, double lr, double *losses);

@@ -17,7 +17,8 @@ the products are accumulated in FP64, as the device accumulates in FP32.
 
 Loss of the preconditioner scales s = 1 + y (y [rows][k1][n_ch][2]):
   E[i, r, n] = E0[i, r, n] + sum_k s[i, k, ch(n)] U[i, k, r, n]
-  L = sum |E|^2 / xnorm2,  dL/dy[i, k, c] = 2 (Re, Im) sum_{r, ch(n) = c} conj(U) E / xnorm2.
+  L = sum |E|^2 / xnorm2 + (w / rows) sum y^2,
+  dL/dy[i, k, c] = 2 (Re, Im) sum_{r, ch(n) = c} conj(U) E / xnorm2 + 2 (w / rows) y.
 """
 import numpy as np
 
@@ -86,8 +87,9 @@ def channels(l_max, m_max):
     return np.array(out)
 
 
-def loss(y, U, E0, ch, xnorm2):
-    """(L, dL/dy) for y [rows][k1 n_ch 2], U [rows][k1][nrhs][B], E0 [rows][nrhs][B]."""
+def loss(y, U, E0, ch, xnorm2, prior_weight=0.0):
+    """(L, dL/dy) for y [rows][k1 n_ch 2], U [rows][k1][nrhs][B], E0 [rows][nrhs][B]; the pull towards
+    y = 0 adds (prior_weight / rows) sum y^2."""
     rows, k1, nrhs, B = U.shape
     n_ch = int(ch.max()) + 1
     yc = np.asarray(y, dtype=np.float64).reshape(rows, k1, -1, 2)
@@ -100,7 +102,9 @@ def loss(y, U, E0, ch, xnorm2):
     for c in range(n_ch):
         gc[:, :, c] = g[:, :, ch == c].sum(-1)
     dy = np.stack([2 * gc.real, 2 * gc.imag], -1) / xnorm2
-    return L, dy.reshape(rows, -1)
+    yf = np.asarray(y, dtype=np.float64).reshape(rows, -1)
+    L += float(prior_weight / rows * np.sum(yf ** 2))
+    return L, dy.reshape(rows, -1) + 2 * prior_weight / rows * yf
 
 
 def near_products(A, nbr, R):

@@ -67,6 +67,7 @@ SIGNATURES = {
     'smh_mlp_backward': (_c.c_int, [_p, _i64, _fp, _fp, _fp]),
     'smh_mlp_add_data': (_c.c_int, [_p, _p, _fp, _i64, _dp, _dp]),
     'smh_mlp_data_loss': (_c.c_int, [_p, _fp, _fp, _dp]),
+    'smh_mlp_set_prior_weight': (_c.c_int, [_p, _c.c_double]),
     'smh_mlp_train': (_c.c_int, [_p, _c.c_int, _c.c_double, _dp]),
     'smh_volume_build_table': (_c.c_int, [_p, _c.c_double, _c.c_double, _c.c_double, _c.c_double, _dp, _i64, _dp,
                                           _dp, _dp]),

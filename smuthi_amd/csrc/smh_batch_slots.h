@@ -140,7 +140,12 @@ __device__ inline float fma_xor1(float v, float c, float other) {
   return fmaf(c, __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(other), 0xB1, 0xF, 0xF, false)), v);
 }
 
-template <int LMAX, int MMAX, int T>
+// VOL: volume lookups. The table holds tap rows in the same slot fragment
+// order ([rows][MT][NS/4][64 lanes][4], one row per tap row of the volume
+// lookup: node * vq + moment kind); an entry's T taps for z slice a.vslice sit
+// at rows en.x + vol_tap_offset(a.vmode, a.vslice, t, a.len_z), and a pair
+// weights them with vol_tap_weight(w, w2) instead of Horner in w.
+template <int LMAX, int MMAX, int T, bool VOL = false>
 __global__ void __launch_bounds__(512) batch_slots_kernel(RadialKArgs a) {
   using Sh = SlotShape<LMAX, MMAX, T>;
   constexpr int S = Sh::S, NS = Sh::NS, MT = Sh::MT, MH = Sh::MH, SC = Sh::SC, LO = Sh::LO;
@@ -164,7 +169,8 @@ __global__ void __launch_bounds__(512) batch_slots_kernel(RadialKArgs a) {
   const int tid = threadIdx.x, lane = tid & 63;
   const int2 it = a.items[item];
   const int e_end = item + 1 < a.n_items ? a.items[item + 1].y : a.n_entries;
-  const PairRec<float>* recs = reinterpret_cast<const PairRec<float>*>(a.recs);
+  using Rec = typename std::conditional<VOL, PairRec2<float>, PairRec<float>>::type;
+  const Rec* recs = reinterpret_cast<const Rec*>(a.recs);
   const float2* xg = reinterpret_cast<const float2*>(a.x);
   const float4* tt4 = reinterpret_cast<const float4*>(a.tt);
   for (int j = tid; j < a.Rc * LO; j += blockDim.x) yacc[j] = 0ull;
@@ -208,10 +214,12 @@ __global__ void __launch_bounds__(512) batch_slots_kernel(RadialKArgs a) {
     const int4 en = a.entries[e];
     const int bin = (en.x - a.row_o) / a.row_s;
     // polynomial coefficients tv[t][i][s] = c_t(tile i row rr, slot 4 s + q) of the entry's bin
+    // (VOL: the values of the entry's tap t)
     float tv[T][MT][NS];
 #pragma unroll
     for (int t = 0; t < T; ++t) {
-      const float4* row = tt4 + ((size_t)bin * T + t) * (Sh::FRB / 4);
+      const size_t trow = VOL ? (size_t)(en.x + vol_tap_offset(a.vmode, a.vslice, t, a.len_z)) : (size_t)bin * T + t;
+      const float4* row = tt4 + trow * (Sh::FRB / 4);
 #pragma unroll
       for (int i = 0; i < MT; ++i)
 #pragma unroll
@@ -224,19 +232,24 @@ __global__ void __launch_bounds__(512) batch_slots_kernel(RadialKArgs a) {
         }
     }
     const int p_end = en.z;
-    auto load_x = [&](const PairRec<float>& r, float4 (&xs)[NS / 2]) {
+    auto load_x = [&](const Rec& r, float4 (&xs)[NS / 2]) {
       const float4* xp = reinterpret_cast<const float4*>(xg + (int64_t)(r.ij & kPairMask) * S * KB + xlane);
 #pragma unroll
       for (int k = 0; k < NS / 2; ++k) xs[k] = xp[k];
     };
-    PairRec<float> rec_n = recs[en.y];
+    Rec rec_n = recs[en.y];
     float4 xc[NS / 2];
     load_x(rec_n, xc);
     for (int p = 0; p < p_end; ++p) {
-      const PairRec<float> rec = rec_n;
+      const Rec rec = rec_n;
       rec_n = recs[en.y + (p + 1 < p_end ? p + 1 : p)];
 
       const float w = rec.w;
+      float wt[T];  // VOL: the pair's tap weights
+      if constexpr (VOL) {
+#pragma unroll
+        for (int t = 0; t < T; ++t) wt[t] = vol_tap_weight<float>(a.vmode, a.vslice, t, rec.w, rec.w2);
+      }
       // cos / sin of k phi, k = 1..3
       const float c1 = rec.c, s1 = rec.s;
       const float c2 = c1 * c1 - s1 * s1, s2 = 2.f * c1 * s1;
@@ -273,9 +286,16 @@ __global__ void __launch_bounds__(512) batch_slots_kernel(RadialKArgs a) {
       for (int s = 0; s < NS; ++s) {
 #pragma unroll
         for (int i = 0; i < MT; ++i) {
-          float v = tv[T - 1][i][s];
+          float v;
+          if constexpr (VOL) {
+            v = wt[0] * tv[0][i][s];
 #pragma unroll
-          for (int t = T - 2; t >= 0; --t) v = fmaf(v, w, tv[t][i][s]);
+            for (int t = 1; t < T; ++t) v = fmaf(wt[t], tv[t][i][s], v);
+          } else {
+            v = tv[T - 1][i][s];
+#pragma unroll
+            for (int t = T - 2; t >= 0; --t) v = fmaf(v, w, tv[t][i][s]);
+          }
           acc[i] = Mfma<float>::op(v, z[s], acc[i]);
         }
       }

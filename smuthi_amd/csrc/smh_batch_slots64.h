@@ -51,7 +51,11 @@ __device__ inline double lane_xor1_64(double v) {
 constexpr int kSlots64Parts = 2;
 constexpr int kSlots64Threads = 512;
 
-template <int LMAX, int MMAX, int T>
+// VOL: volume lookups, as batch_slots_kernel's VOL form: the table holds FP64
+// tap rows in this kernel's fragment order ([rows][MT][NS/2][64 lanes][2]),
+// an entry's T taps of z slice a.vslice at rows en.x + vol_tap_offset, and a
+// pair weights them with vol_tap_weight(w, w2).
+template <int LMAX, int MMAX, int T, bool VOL = false>
 __global__ void __launch_bounds__(kSlots64Threads) batch_slots64_kernel(RadialKArgs a) {
   using Sh = SlotShape<LMAX, MMAX, T>;
   constexpr int S = Sh::S, NS = Sh::NS, MT = Sh::MT, MH = Sh::MH, SC = Sh::SC, LO = Sh::LO;
@@ -73,7 +77,8 @@ __global__ void __launch_bounds__(kSlots64Threads) batch_slots64_kernel(RadialKA
   const int tid = threadIdx.x, lane = tid & 63;
   const int2 it = a.items[item];
   const int e_end = item + 1 < a.n_items ? a.items[item + 1].y : a.n_entries;
-  const PairRec<double>* recs = reinterpret_cast<const PairRec<double>*>(a.recs);
+  using Rec = typename std::conditional<VOL, PairRec2<double>, PairRec<double>>::type;
+  const Rec* recs = reinterpret_cast<const Rec*>(a.recs);
   const double* xg = reinterpret_cast<const double*>(a.x);
   const double2* tt2 = reinterpret_cast<const double2*>(a.tt);
   for (int j = tid; j < a.Rc * LO; j += blockDim.x) yacc[j] = 0.0;
@@ -109,7 +114,9 @@ __global__ void __launch_bounds__(kSlots64Threads) batch_slots64_kernel(RadialKA
       for (int t = 0; t < T; ++t)
 #pragma unroll
         for (int i = 0; i < MT; ++i) {
-          const double2* row = tt2 + (((size_t)bin * T + t) * MT + i) * (NS / 2) * 64;
+          const size_t trow =
+              VOL ? (size_t)(en.x + vol_tap_offset(a.vmode, a.vslice, t, a.len_z)) * MT + i : ((size_t)bin * T + t) * MT + i;
+          const double2* row = tt2 + trow * (NS / 2) * 64;
 #pragma unroll
           for (int s2 = 0; s2 < NSH / 2; ++s2) {
             const double2 v = row[(h * (NSH / 2) + s2) * 64 + lane];
@@ -117,7 +124,7 @@ __global__ void __launch_bounds__(kSlots64Threads) batch_slots64_kernel(RadialKA
             tv[t][i][2 * s2 + 1] = v.y;
           }
         }
-      auto load_x = [&](const PairRec<double>& r, double (&xs)[2 * NSH]) {
+      auto load_x = [&](const Rec& r, double (&xs)[2 * NSH]) {
         const double* xp = xg + (int64_t)(r.ij & kPairMask) * S * KB * 2 + xlane + 2 * h * NSH;
 #pragma unroll
         for (int s = 0; s < NSH; ++s) {
@@ -125,14 +132,19 @@ __global__ void __launch_bounds__(kSlots64Threads) batch_slots64_kernel(RadialKA
           xs[2 * s + 1] = xp[2 * s + xother];
         }
       };
-      PairRec<double> rec_n = recs[en.y];
+      Rec rec_n = recs[en.y];
       double xc[2 * NSH];
       load_x(rec_n, xc);
       for (int p = 0; p < p_end; ++p) {
-        const PairRec<double> rec = rec_n;
+        const Rec rec = rec_n;
         rec_n = recs[en.y + (p + 1 < p_end ? p + 1 : p)];
 
         const double w = rec.w;
+        double wt[T];  // VOL: the pair's tap weights
+        if constexpr (VOL) {
+#pragma unroll
+          for (int t = 0; t < T; ++t) wt[t] = vol_tap_weight<double>(a.vmode, a.vslice, t, rec.w, rec.w2);
+        }
         const double c1 = rec.c, s1 = rec.s;
         const double c2 = c1 * c1 - s1 * s1, s2 = 2.0 * c1 * s1;
         const double c3 = c2 * c1 - s2 * s1, s3 = s2 * c1 + c2 * s1;
@@ -164,9 +176,16 @@ __global__ void __launch_bounds__(kSlots64Threads) batch_slots64_kernel(RadialKA
         for (int sl = 0; sl < NSH; ++sl) {
 #pragma unroll
           for (int i = 0; i < MT; ++i) {
-            double v = tv[T - 1][i][sl];
+            double v;
+            if constexpr (VOL) {
+              v = wt[0] * tv[0][i][sl];
 #pragma unroll
-            for (int t = T - 2; t >= 0; --t) v = fma(v, w, tv[t][i][sl]);
+              for (int t = 1; t < T; ++t) v = fma(wt[t], tv[t][i][sl], v);
+            } else {
+              v = tv[T - 1][i][sl];
+#pragma unroll
+              for (int t = T - 2; t >= 0; --t) v = fma(v, w, tv[t][i][sl]);
+            }
             acc[i] = Mfma<double>::op(v, z[sl], acc[i]);
           }
         }

@@ -338,7 +338,7 @@ __global__ void residual0_kernel(const double2* __restrict__ R, const double2* _
 __global__ void __launch_bounds__(256) loss_grad_kernel(const float* __restrict__ y, const float2* __restrict__ U,
                                                         const float2* __restrict__ E0, const int* __restrict__ ch,
                                                         int64_t rows, int k1, int B, int nrhs, int n_ch,
-                                                        float inv_norm, float* dy, float* loss_row) {
+                                                        float inv_norm, float prior, float* dy, float* loss_row) {
   extern __shared__ float2 Es[];  // [nrhs][B]
   const int64_t i = blockIdx.x;
   const int ldy = k1 * n_ch * 2;
@@ -366,7 +366,19 @@ __global__ void __launch_bounds__(256) loss_grad_kernel(const float* __restrict_
     if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
     __syncthreads();
   }
-  if (threadIdx.x == 0) loss_row[i] = red[0] * inv_norm;
+  // the pull towards the first-order structure (y = 0): prior * |y_i|^2
+  float lp = 0.f;
+  for (int j = threadIdx.x; j < ldy; j += blockDim.x) lp += yi[j] * yi[j];
+  __syncthreads();
+  const float ld = red[0];
+  __syncthreads();
+  red[threadIdx.x] = lp;
+  __syncthreads();
+  for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) loss_row[i] = ld * inv_norm + prior * red[0];
   for (int sc = threadIdx.x; sc < k1 * n_ch; sc += blockDim.x) {
     const int slot = sc / n_ch, c = sc - slot * n_ch;
     float gr = 0.f, gi = 0.f;
@@ -376,8 +388,8 @@ __global__ void __launch_bounds__(256) loss_grad_kernel(const float* __restrict_
       gr += u.x * ee.x + u.y * ee.y;  // Re(conj(u) e)
       gi += u.x * ee.y - u.y * ee.x;  // Im(conj(u) e)
     }
-    dy[(size_t)i * ldy + sc * 2] = 2.f * gr * inv_norm;
-    dy[(size_t)i * ldy + sc * 2 + 1] = 2.f * gi * inv_norm;
+    dy[(size_t)i * ldy + sc * 2] = 2.f * gr * inv_norm + 2.f * prior * yi[sc * 2];
+    dy[(size_t)i * ldy + sc * 2 + 1] = 2.f * gi * inv_norm + 2.f * prior * yi[sc * 2 + 1];
   }
 }
 
@@ -402,6 +414,7 @@ struct smh_mlp {
   int* d_ch = nullptr;
   float* d_loss = nullptr;
   double xnorm2 = 0.0;
+  double prior_weight = 0.0;  // w of the pull towards y = 0, w / rows per row
 
   // parameter offsets: W1 [n_in][n_h], b1, W2 [n_h][n_h], b2, W3 [n_h][n_out], b3
   int64_t oW1() const { return 0; }
@@ -864,7 +877,7 @@ int mlp_loss_dev(smh_mlp* m, double* loss) {
   const float inv = (float)(1.0 / std::max(m->xnorm2, 1e-300));
   hipLaunchKernelGGL(loss_grad_kernel, dim3((unsigned)m->rows), dim3(256), lds, s, (const float*)m->d_y,
                      (const float2*)m->d_U, (const float2*)m->d_E0, (const int*)m->d_ch, m->rows, m->k1, m->B, m->nrhs,
-                     m->n_ch, inv, m->d_dy, m->d_loss);
+                     m->n_ch, inv, (float)(m->prior_weight / (double)m->rows), m->d_dy, m->d_loss);
   SMH_HIP(ctx, hipGetLastError());
   if (loss) {
     std::vector<float> l(m->rows);
@@ -877,6 +890,13 @@ int mlp_loss_dev(smh_mlp* m, double* loss) {
   return SMH_OK;
 }
 }  // namespace
+
+extern "C" int smh_mlp_set_prior_weight(smh_mlp* m, double w) {
+  if (!m) return SMH_ERR_ARG;
+  SMH_CHECK(m->ctx, w >= 0 && w < 1e30, SMH_ERR_ARG, "invalid prior weight %g", w);
+  m->prior_weight = w;
+  return SMH_OK;
+}
 
 extern "C" int smh_mlp_data_loss(smh_mlp* m, const float* y, float* dy, double* loss) {
   if (!m) return SMH_ERR_ARG;

@@ -216,6 +216,17 @@ inline void launch_xmax(const double2* x, int64_t n, float* xmax, hipStream_t s)
   hipLaunchKernelGGL(xmax_final_kernel, dim3(1), dim3(kXmaxBlocks), 0, s, xmax, 1.f + 1.2e-7f);
 }
 
+// FP64 padded [n_part][B][kBatchRhs] -> FP64 input of the FP64 slot-order
+// kernel (b32_index layout)
+__global__ void to_slots64_kernel(const double2* __restrict__ v, int64_t n8, int B, int S,
+                                  const int* __restrict__ slot, double2* out) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n8) return;
+  const int r = (int)(e % kBatchRhs);
+  const int64_t ip = e / kBatchRhs, p = ip / B;
+  out[b32_index(p, (int)(ip - p * B), r, S, slot)] = v[e];
+}
+
 // Zero-fill n 32-bit words with a kernel (callers may capture the launch in a
 // graph, where a memset node is avoided).
 __global__ void zero_words_kernel(uint32_t* __restrict__ p, int64_t n) {

@@ -82,6 +82,8 @@ struct smh_op {
   // tables in fragment order, row sums, and a binning per table and precision
   float2* d_vtab[2] = {nullptr, nullptr};
   float* d_vtt[2] = {nullptr, nullptr};
+  float* d_vtts[2] = {nullptr, nullptr};    // the FP32 tap rows in slot fragment order (8-RHS slot kernel)
+  double* d_vtts64[2] = {nullptr, nullptr};  // and in FP64 for the FP64 8-RHS slot kernel
   double* d_vtt64[2] = {nullptr, nullptr};  // FP64 tap tables (spline: FP64 moments)
   int vmode = 1, vq = 1;  // interpolation mode (vol_tap_offset) and tap rows per (rho, z) node
   float* d_vrowsum[2] = {nullptr, nullptr};

@@ -679,16 +679,6 @@ __global__ void scatter_batch_kernel(const double2* __restrict__ pk, const int* 
   pad[b32_index(p, (int)(ip - p * B), (int)r, S, slot)] = typename Vec2<Real>::t{Real(v.x), Real(v.y)};
 }
 
-// FP64 padded [n_part][B][kBatch] -> FP64 input of the FP64 slot-order kernel
-// (b32_index layout)
-__global__ void to_slots64_kernel(const double2* __restrict__ v, int64_t n8, int B, int S,
-                                  const int* __restrict__ slot, double2* out) {
-  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= n8) return;
-  const int r = (int)(e % kBatch);
-  const int64_t ip = e / kBatch, p = ip / B;
-  out[b32_index(p, (int)(ip - p * B), r, S, slot)] = v[e];
-}
 
 // FP64 padded [n_part][B][kBatch] -> FP16 input of the FP16-operand kernel
 // (b16_index layout, values divided by *xmax)
@@ -945,7 +935,7 @@ void smh_op_free(smh_op* op) {
   for (auto& row : op->vbbin)
     for (auto& b : row) free_bin(b);
   for (int t = 0; t < 2; ++t) {
-    void* vp[] = {op->d_vtab[t], op->d_vtt[t], op->d_vtt64[t], op->d_vrowsum[t]};
+    void* vp[] = {op->d_vtab[t], op->d_vtt[t], op->d_vtts[t], op->d_vtts64[t], op->d_vtt64[t], op->d_vrowsum[t]};
     for (void* p : vp)
       if (p) hipFree(p);
   }

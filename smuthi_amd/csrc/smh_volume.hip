@@ -18,6 +18,8 @@
 #include <cmath>
 
 #include "smh_batch_kernel.h"
+#include "smh_batch_slots.h"
+#include "smh_batch_slots64.h"
 #include "smh_lookup.h"
 #include "smh_special.h"
 
@@ -248,6 +250,82 @@ __global__ void vol_taps4_kernel(const float2* __restrict__ table, const double2
   }
 }
 
+// The tap rows in the slot fragment order of batch_slots_kernel<..., VOL>:
+// [rows][MT][NS/4][64 lanes][4] FP32, lane (q, rr) of tile i holding
+// A(slot 16 (i mod MH) + rr, slot 4 (4 s4 + e) + q) of the row (tiles i < MH
+// from Re, the others from Im); row = node * vq + kind, kind 0 the table
+// value, 1..3 (spline) the moments along rho, along z and mixed.
+__global__ void vol_slot_taps_kernel(const float2* __restrict__ table, const double2* __restrict__ mr,
+                                     const double2* __restrict__ mz, const double2* __restrict__ mrz, int64_t rows,
+                                     int vq, int B, int S, const int* __restrict__ n_of, float* tt) {
+  const int NS = S / 4, MH = S / 16, FRB = 2 * MH * NS * 64;
+  const int64_t total = rows * FRB;
+  for (int64_t f = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; f < total; f += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t row = f / FRB;
+    int rem = (int)(f - row * FRB);
+    const int e = rem & 3;
+    const int ln = (rem >> 2) & 63;
+    rem >>= 8;
+    const int s4 = rem % (NS / 4);
+    const int i = rem / (NS / 4);
+    const int q = ln >> 4, rr = ln & 15;
+    const int n1 = n_of[16 * (i % MH) + rr];
+    const int n2 = n_of[4 * (4 * s4 + e) + q];
+    float v = 0.f;
+    if (n1 >= 0 && n2 >= 0) {
+      const int64_t node = row / vq;
+      const int kind = (int)(row - node * vq);
+      const int64_t o = (node * B + n1) * B + n2;
+      double2 c;
+      if (kind == 0) {
+        const float2 t = table[o];
+        c = make_double2((double)t.x, (double)t.y);
+      } else {
+        c = kind == 1 ? mr[o] : (kind == 2 ? mz[o] : mrz[o]);
+      }
+      v = (float)(i < MH ? c.x : c.y);
+    }
+    tt[f] = v;
+  }
+}
+
+// The same rows in FP64 in the fragment order of batch_slots64_kernel<..., VOL>:
+// [rows][MT][NS/2][64 lanes][2], lane (q, rr) of tile i holding
+// A(slot 16 (i mod MH) + rr, slot 4 (2 s2 + e) + q).
+__global__ void vol_slot_taps64_kernel(const float2* __restrict__ table, const double2* __restrict__ mr,
+                                       const double2* __restrict__ mz, const double2* __restrict__ mrz, int64_t rows,
+                                       int vq, int B, int S, const int* __restrict__ n_of, double* tt) {
+  const int NS = S / 4, MH = S / 16, FRB = 2 * MH * NS * 64;
+  const int64_t total = rows * FRB;
+  for (int64_t f = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; f < total; f += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t row = f / FRB;
+    int rem = (int)(f - row * FRB);
+    const int e = rem & 1;
+    const int ln = (rem >> 1) & 63;
+    rem >>= 7;
+    const int s2 = rem % (NS / 2);
+    const int i = rem / (NS / 2);
+    const int q = ln >> 4, rr = ln & 15;
+    const int n1 = n_of[16 * (i % MH) + rr];
+    const int n2 = n_of[4 * (2 * s2 + e) + q];
+    double v = 0.0;
+    if (n1 >= 0 && n2 >= 0) {
+      const int64_t node = row / vq;
+      const int kind = (int)(row - node * vq);
+      const int64_t o = (node * B + n1) * B + n2;
+      double2 c;
+      if (kind == 0) {
+        const float2 t = table[o];
+        c = make_double2((double)t.x, (double)t.y);
+      } else {
+        c = kind == 1 ? mr[o] : (kind == 2 ? mz[o] : mrz[o]);
+      }
+      v = i < MH ? c.x : c.y;
+    }
+    tt[f] = v;
+  }
+}
+
 // Row sums of a fragment-order tap table: per row max over n1 of
 // sum_n2 |Re A(n1, n2)| + |Im A(n1, n2)| (slightly enlarged), one block per row.
 __global__ void frag_rowsum_kernel(const float* __restrict__ tt, int64_t rows, int B, int Bh, int NG, int FR,
@@ -389,12 +467,40 @@ KernelInfo vol_batch_kernel_info() {
   k.sy = Sh::SY;
   return k;
 }
+// FP32: the slot-order kernel where its tap fragments fit the registers
+template <int L, int Mx>
+KernelInfo vol_batch32_kernel_info() {
+  if constexpr (SlotShape<L, Mx, 4>::fits && Mx <= 3) {
+    KernelInfo k;
+    k.fn = reinterpret_cast<void*>(&batch_slots_kernel<L, Mx, 4, true>);
+    k.threads = 512;
+    k.sy = SlotShape<L, Mx, 4>::SY;
+    k.slots = SlotShape<L, Mx, 4>::S;
+    k.slot_tables = &slot_tables<L, Mx>;
+    return k;
+  } else {
+    return vol_batch_kernel_info<L, Mx, float, float>();
+  }
+}
+// FP64: the slot-order kernel where its coefficient parts fit the registers
+template <int L, int Mx>
+KernelInfo vol_batch64_kernel_info(bool tap64) {
+  if constexpr (SlotShape<L, Mx, 4>::fits && Mx <= 3 && SlotShape<L, Mx, 4>::NS % (2 * kSlots64Parts) == 0) {
+    KernelInfo k;
+    k.fn = reinterpret_cast<void*>(&batch_slots64_kernel<L, Mx, 4, true>);
+    k.threads = kSlots64Threads;
+    k.sy = SlotShape<L, Mx, 4>::LO;  // doubles
+    k.slots = SlotShape<L, Mx, 4>::S;
+    k.slot_tables = &slot_tables<L, Mx>;
+    return k;
+  } else {
+    return tap64 ? vol_batch_kernel_info<L, Mx, double, double>() : vol_batch_kernel_info<L, Mx, double, float>();
+  }
+}
 KernelInfo volume_batch_kernel(int precision, int l, int m, bool tap64 = false) {
 #define SMH_V(L, Mx)                                                                                    \
   if (l == L && m == Mx)                                                                                \
-    return precision == SMH_FP32 ? vol_batch_kernel_info<L, Mx, float, float>()                         \
-                                 : (tap64 ? vol_batch_kernel_info<L, Mx, double, double>()               \
-                                          : vol_batch_kernel_info<L, Mx, double, float>());
+    return precision == SMH_FP32 ? vol_batch32_kernel_info<L, Mx>() : vol_batch64_kernel_info<L, Mx>(tap64);
   SMH_V(1, 1)
   SMH_V(2, 1)
   SMH_V(2, 2)
@@ -552,6 +658,24 @@ int vol_make_taps(smh_op* op) {
       hipLaunchKernelGGL(vol_taps4_kernel, dim3(4096), dim3(256), 0, s, op->d_vtab[t], mr, mz, mrz, nodes, op->B,
                          op->Bh, g.NG, g.FR, op->d_vtt[t], op->d_vtt64[t]);
     }
+    if (op->d_b32_tab) {
+      const int S = op->b32_S;
+      if (!op->d_vtts[t] && hipMalloc(&op->d_vtts[t], (size_t)rows * 2 * S * S * sizeof(float)) != hipSuccess) {
+        cleanup();
+        return smh_fail(ctx, SMH_ERR_HIP, "device allocation failed (slot-order tap rows)");
+      }
+      hipLaunchKernelGGL(vol_slot_taps_kernel, dim3(4096), dim3(256), 0, s, op->d_vtab[t], mr, mz, mrz, rows, op->vq,
+                         op->B, S, (const int*)op->d_b32_tab + op->B, op->d_vtts[t]);
+      if (volume_batch_kernel(SMH_FP64, op->l_max, op->m_max, op->vq == 4).slots == S) {
+        if (!op->d_vtts64[t] &&
+            hipMalloc(&op->d_vtts64[t], (size_t)rows * 2 * S * S * sizeof(double)) != hipSuccess) {
+          cleanup();
+          return smh_fail(ctx, SMH_ERR_HIP, "device allocation failed (FP64 slot-order tap rows)");
+        }
+        hipLaunchKernelGGL(vol_slot_taps64_kernel, dim3(4096), dim3(256), 0, s, op->d_vtab[t], mr, mz, mrz, rows,
+                           op->vq, op->B, S, (const int*)op->d_b32_tab + op->B, op->d_vtts64[t]);
+      }
+    }
     hipLaunchKernelGGL(frag_rowsum_kernel, dim3((unsigned)std::min<int64_t>(rows, 65536)), dim3(256), 0, s,
                        op->d_vtt[t], rows, op->B, op->Bh, g.NG, g.FR, op->d_vrowsum[t]);
     for (smh_op::Binning* bn : {&op->vbin[t][0], &op->vbbin[t][0]})
@@ -644,7 +768,27 @@ int vol_apply_batch(smh_op* op, int precision, const void* x_pad, void* y_pad, b
   SMH_CHECK(ctx, k.fn != nullptr && op->vbbin[0][pi].Rc > 0, SMH_ERR_UNSUPPORTED,
             "no batched volume matvec kernel for l_max %d m_max %d", op->l_max, op->m_max);
   hipStream_t s = ctx->stream;
-  if (pi == 0) launch_xmax((const float2*)x_pad, op->n_part * op->B * kBatchRhs, op->d_xmax, s);
+  const bool slots = k.slots > 0;
+  SMH_CHECK(ctx,
+            !slots || (op->d_b32_tab && op->b32_S == k.slots &&
+                       (pi == 0 ? op->d_vtts[0] && op->d_vtts[1] : op->d_vtts64[0] && op->d_vtts64[1])),
+            SMH_ERR_STATE, "slot-order tap rows missing");
+  const int S = slots ? k.slots : op->B;
+  const int* slot = slots ? (const int*)op->d_b32_tab : nullptr;
+  if (pi == 0) launch_xmax((const float2*)x_pad, op->n_part * S * kBatchRhs, op->d_xmax, s);
+  const void* xin = x_pad;
+  if (pi == 1 && slots) {
+    // FP64 padded [n_part][B][kBatch] -> the slot layout (padding slots stay zero)
+    const int64_t n8 = op->n_part * op->B * kBatchRhs;
+    if (!op->d_x64s) {
+      const size_t bytes = (size_t)op->n_part * S * kBatchRhs * sizeof(double2);
+      SMH_HIP(ctx, hipMalloc(&op->d_x64s, bytes));
+      fill_zero(op->d_x64s, bytes, s);
+    }
+    hipLaunchKernelGGL(to_slots64_kernel, dim3(blocks_for(n8)), dim3(256), 0, s, (const double2*)x_pad, n8, op->B, S,
+                       slot, op->d_x64s);
+    xin = op->d_x64s;
+  }
   double ms_total = 0;
   const int n_slices = op->vmode >= 2 ? 4 : 1;
   const int64_t nx = op->n_part * op->B * kBatchRhs;
@@ -656,8 +800,9 @@ int vol_apply_batch(smh_op* op, int precision, const void* x_pad, void* y_pad, b
       a.entries = bn.d_entries;
       a.items = bn.d_items;
       a.order = bn.d_order;
-      a.tt = tap64 ? (const void*)op->d_vtt64[t] : (const void*)op->d_vtt[t];
-      a.x = x_pad;
+      a.tt = slots ? (pi == 0 ? (const void*)op->d_vtts[t] : (const void*)op->d_vtts64[t])
+                   : (tap64 ? (const void*)op->d_vtt64[t] : (const void*)op->d_vtt[t]);
+      a.x = xin;
       a.ypart = op->d_ypart;
       a.xmax = op->d_xmax;
       a.item_lsum = bn.d_item_lsum;
@@ -669,6 +814,8 @@ int vol_apply_batch(smh_op* op, int precision, const void* x_pad, void* y_pad, b
       a.len_z = (int)op->len_z;
       a.vmode = op->vmode;
       a.vslice = sl;
+      a.row_s = 1;
+      a.row_o = 0;
       if (int rc_ = op_set_lds(op, k.fn, bn.lds)) return rc_;
       if (bn.n_items > 0) {
         if (time_kernel) SMH_HIP(ctx, hipEventRecord(ctx->ev0, s));
@@ -685,12 +832,12 @@ int vol_apply_batch(smh_op* op, int precision, const void* x_pad, void* y_pad, b
       const int accumulate = (t > 0 || sl > 0) ? 1 : 0;
       if (pi == 0)
         hipLaunchKernelGGL(reduce_batch_kernel<float>, dim3(blocks_for(nx)), dim3(256), 0, s,
-                           (const float*)op->d_ypart, bn.d_chunk_items, bn.Rc, op->B, op->B, (const int*)nullptr,
-                           op->n_part, (float2*)y_pad, accumulate);
+                           (const float*)op->d_ypart, bn.d_chunk_items, bn.Rc, op->B, S, slot, op->n_part,
+                           (float2*)y_pad, accumulate);
       else
         hipLaunchKernelGGL(reduce_batch_kernel<double>, dim3(blocks_for(nx)), dim3(256), 0, s,
-                           (const double*)op->d_ypart, bn.d_chunk_items, bn.Rc, op->B, op->B, (const int*)nullptr,
-                           op->n_part, (double2*)y_pad, accumulate);
+                           (const double*)op->d_ypart, bn.d_chunk_items, bn.Rc, op->B, S, slot, op->n_part,
+                           (double2*)y_pad, accumulate);
       SMH_HIP(ctx, hipGetLastError());
     }
   }
@@ -779,7 +926,22 @@ static int volume_create(smh_ctx* ctx, int n_ens, const int64_t* ens_off, int64_
   if (int code = op_index_map(op, lmax, mmax)) return fail(code);
   if (ens_off)
     if (int code = op_set_segments(op, n_ens, ens_off)) return fail(code);
-  op->b32_S = op->B;  // the batched kernel's input: plain layout [n_part][B][kBatch]
+  // the FP32 batched kernel's input: slot layout (b32_index) for the slot-order
+  // kernel, else the plain layout [n_part][B][kBatch]
+  op->b32_S = op->B;
+  {
+    const KernelInfo kb32 = volume_batch_kernel(SMH_FP32, l_max, m_max);
+    if (kb32.fn && kb32.slots > 0) {
+      std::vector<int> tab(op->B + kb32.slots);
+      kb32.slot_tables(tab.data());
+      if (hipMalloc(&op->d_b32_tab, tab.size() * sizeof(int)) != hipSuccess ||
+          hipMemcpy(op->d_b32_tab, tab.data(), tab.size() * sizeof(int), hipMemcpyHostToDevice) != hipSuccess) {
+        smh_fail(ctx, SMH_ERR_HIP, "device allocation failed (slot tables)");
+        return fail(SMH_ERR_HIP);
+      }
+      op->b32_S = kb32.slots;
+    }
+  }
   hipStream_t s = ctx->stream;
   if (hipMalloc(&op->d_xyz, n_part * 3 * sizeof(double)) != hipSuccess ||
       hipMemcpyAsync(op->d_xyz, xyz, n_part * 3 * sizeof(double), hipMemcpyHostToDevice, s) != hipSuccess ||
@@ -810,7 +972,7 @@ static int volume_create(smh_ctx* ctx, int n_ens, const int64_t* ens_off, int64_
       op->vbbin[t][pi].lds = ldsb;
       const int code = pi == 0 ? vol_bin_pairs<float>(op, t, 0, Rb, true) : vol_bin_pairs<double>(op, t, 1, Rb, true);
       if (code) return fail(code);
-      op->vbbin[t][pi].part_bytes = (size_t)Rb * 2 * kBatchRhs * op->B * rs;
+      op->vbbin[t][pi].part_bytes = (size_t)Rb * 2 * kBatchRhs * (kb.slots > 0 ? kb.slots : op->B) * rs;
       ypart = std::max(ypart, (size_t)op->vbbin[t][pi].n_items * op->vbbin[t][pi].part_bytes);
     }
   }

@@ -16,7 +16,9 @@ each particle's neighbourhood -- positions of its K = 16 nearest neighbours
 relative to it, their radii and refractive indices, and its place in the
 layer -- to s - 1. It is trained on solutions the solver generates (random
 and plane-wave right-hand sides, FP64-certified) by fitting P to the
-solution operator: L = sum |P R - X|^2 / sum |X|^2.
+solution operator, L = sum |P R - X|^2 / sum |X|^2, with an optional pull
+of the scales towards s = 1. A validation solve (select) keeps the network
+only where it beats the first-order structure.
 
 Plain GMRES stays both judge and fallback: every solve is certified by the
 FP64 residual ||b - M x|| <= tol ||b|| of the reference's stopping rule, and
@@ -103,7 +105,7 @@ class LearnedPreconditioner:
     scales. The output layer starts at zero: the untrained network gives the
     first-order near-field inverse (s = 1)."""
 
-    def __init__(self, K=16, hidden=256, l_max=3, seed=0):
+    def __init__(self, K=16, hidden=256, l_max=3, seed=0, prior_weight=0.0):
         self.K, self.hidden, self.l_max = K, hidden, l_max
         self.n_in = feature_count(K)
         self.n_ch = 2 * l_max
@@ -113,6 +115,15 @@ class LearnedPreconditioner:
                    cu.context(), 'smh_mlp_create')
         self._mlp = handle
         self.losses = []
+        self.choice = 'network'
+        if prior_weight:
+            self.set_prior_weight(prior_weight)
+
+    def set_prior_weight(self, w):
+        """Weight w of the pull of the scales towards the first-order structure: the training loss adds
+        (w / rows) sum |y|^2 (s = 1 + y)."""
+        _lib.check(_lib.load().smh_mlp_set_prior_weight(self._mlp, float(w)), cu.context(),
+                   'smh_mlp_set_prior_weight')
 
     def __del__(self):
         if getattr(self, '_mlp', None) is not None and _lib._LIB is not None:
@@ -209,6 +220,29 @@ class LearnedPreconditioner:
         s[..., 0] = 1.0
         _lib.check(_lib.load().smh_op_set_near_scaling(operator._op, _lib.fptr(s)), cu.context(),
                    'smh_op_set_near_scaling')
+
+    def select(self, operator, feat, t_diagonal, rhs, tol, precision='mixed'):
+        """Validation on held-out systems: solve rhs with the network's scales and with the first-order
+        structure (s = 1) and keep the one that needs fewer GMRES iterations (the structure on a tie or when
+        the network's solve falls back). Returns the iterations of both and the choice."""
+        self.apply(operator, feat)
+        _, st_n, fb_n = solve_with_fallback(operator, t_diagonal, rhs, tol, precision=precision)
+        self.apply_prior(operator)
+        _, st_p, fb_p = solve_with_fallback(operator, t_diagonal, rhs, tol, precision=precision)
+        self.remove(operator)
+        net = st_n['iterations'] if not fb_n else np.inf
+        pri = st_p['iterations'] if not fb_p else np.inf
+        self.choice = 'network' if net < pri else 'prior'
+        return {'network': st_n['iterations'], 'prior': st_p['iterations'], 'network_fell_back': fb_n,
+                'prior_fell_back': fb_p, 'choice': self.choice}
+
+    def apply_selected(self, operator, feat):
+        """Set the preconditioner that won the last validation (select): the network's or the first-order
+        structure."""
+        if self.choice == 'network':
+            self.apply(operator, feat)
+        else:
+            self.apply_prior(operator)
 
     @staticmethod
     def remove(operator):

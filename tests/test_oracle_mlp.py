@@ -51,14 +51,21 @@ def test_loss_gradient_matches_finite_differences():
     U = rng.standard_normal((rows, k1, nrhs, B)) + 1j * rng.standard_normal((rows, k1, nrhs, B))
     E0 = rng.standard_normal((rows, nrhs, B)) + 1j * rng.standard_normal((rows, nrhs, B))
     y = 0.1 * rng.standard_normal((rows, k1 * n_ch * 2))
-    L, dy = mlp.loss(y, U, E0, ch, 7.0)
-    eps = 1e-6
-    for i in range(rows):
-        for k in range(0, dy.shape[1], 3):
-            e = np.zeros_like(y)
-            e[i, k] = eps
-            fd = (mlp.loss(y + e, U, E0, ch, 7.0)[0] - mlp.loss(y - e, U, E0, ch, 7.0)[0]) / (2 * eps)
-            assert abs(fd - dy[i, k]) <= 1e-6 * (1 + abs(dy[i, k])), (i, k, fd, dy[i, k])
+    for w in (0.0, 0.3):
+        L, dy = mlp.loss(y, U, E0, ch, 7.0, w)
+        eps = 1e-6
+        for i in range(rows):
+            for k in range(0, dy.shape[1], 3):
+                e = np.zeros_like(y)
+                e[i, k] = eps
+                fd = (mlp.loss(y + e, U, E0, ch, 7.0, w)[0] - mlp.loss(y - e, U, E0, ch, 7.0, w)[0]) / (2 * eps)
+                assert abs(fd - dy[i, k]) <= 1e-6 * (1 + abs(dy[i, k])), (w, i, k, fd, dy[i, k])
+    # the pull's own value: y = 0 except one entry
+    y0 = np.zeros_like(y)
+    y0[1, 4] = 0.5
+    L0 = mlp.loss(np.zeros_like(y), U, E0, ch, 7.0, 0.3)[0]
+    L1 = mlp.loss(y0, U, E0, ch, 7.0, 0.3)[0] - mlp.loss(y0, U, E0, ch, 7.0, 0.0)[0]
+    assert abs(L1 - 0.3 / rows * 0.25) < 1e-12 and L0 == mlp.loss(np.zeros_like(y), U, E0, ch, 7.0)[0]
 
 
 def test_first_order_preconditioner_inverts_weak_coupling():

@@ -35,11 +35,11 @@ extern "C" {
 
 #define SMH_FP32 0 /* FP32 complex arithmetic, FP32 accumulation */
 #define SMH_FP64 1 /* FP64 complex arithmetic */
-#define SMH_FP16 2 /* FP16 operands, FP32 accumulation (8-RHS radial matvec only; a few 1e-4 relative error) */
+#define SMH_FP16 2 /* FP16 operands, FP32 accumulation (8-RHS matvecs of lookup operators; a few 1e-4 relative error) */
 
 #define SMH_SOLVE_FP64 0  /* restarted GMRES, FP64 matvec, FP64 Arnoldi */
 #define SMH_SOLVE_MIXED 1 /* FP64 iterative refinement around FP32-matvec GMRES */
-#define SMH_SOLVE_MIXED16 2 /* FP64 iterative refinement around FP16-matvec GMRES (batched radial solves) */
+#define SMH_SOLVE_MIXED16 2 /* FP64 iterative refinement around FP16-matvec GMRES (batched lookup solves) */
 #define SMH_SOLVE_LU 3      /* LU factorization of the explicit master matrix (solver_type 'LU') */
 
 typedef struct smh_ctx smh_ctx;

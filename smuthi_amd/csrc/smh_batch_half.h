@@ -50,7 +50,13 @@ typedef unsigned u32x2_t __attribute__((ext_vector_type(2)));
 // workgroup is the CU's only one, and 4 waves per SIMD hide the gather latency
 constexpr int kHalfThreads = 1024;
 
-template <int LMAX, int MMAX, int T>
+// VOL: volume lookups. The table holds the tap rows in this kernel's
+// fragment order ([rows][MT][64 lanes][NC][4] FP16, one row per tap row of the
+// volume lookup), each row divided by its own power of two hscale[row]; an
+// entry's T taps of z slice a.vslice sit at rows en.x + vol_tap_offset, and a
+// pair weights them with vol_tap_weight(w, w2) hscale[row] / (the largest of
+// the entry's four scales), which then scales the result.
+template <int LMAX, int MMAX, int T, bool VOL = false>
 __global__ void __launch_bounds__(kHalfThreads) batch_half_kernel(RadialKArgs a, const float* __restrict__ hscale) {
   using Sh = SlotShape<LMAX, MMAX, T>;
   constexpr int S = Sh::S, MT = Sh::MT, MH = Sh::MH, SC = Sh::SC, LO = Sh::LO;
@@ -76,7 +82,8 @@ __global__ void __launch_bounds__(kHalfThreads) batch_half_kernel(RadialKArgs a,
   const int tid = threadIdx.x, lane = tid & 63;
   const int2 it = a.items[item];
   const int e_end = item + 1 < a.n_items ? a.items[item + 1].y : a.n_entries;
-  const PairRec<float>* recs = reinterpret_cast<const PairRec<float>*>(a.recs);
+  using Rec = typename std::conditional<VOL, PairRec2<float>, PairRec<float>>::type;
+  const Rec* recs = reinterpret_cast<const Rec*>(a.recs);
   const _Float16* xg = reinterpret_cast<const _Float16*>(a.x);
   const uint4* tth = reinterpret_cast<const uint4*>(a.tt);
   for (int j = tid; j < a.Rc * LO; j += blockDim.x) yacc[j] = 0ull;
@@ -129,13 +136,15 @@ __global__ void __launch_bounds__(kHalfThreads) batch_half_kernel(RadialKArgs a,
 
   typedef half2_t Coef[T][MT][NC][2];
   // polynomial coefficients of a bin: tv[t][i][c] = 4 FP16 values (2 packed pairs)
-  auto load_coef = [&](int bin, Coef& tv) {
+  // (VOL: the values of the entry's tap rows rows[t])
+  auto load_coef = [&](int bin, const int* rows, Coef& tv) {
 #pragma unroll
     for (int t = 0; t < T; ++t)
 #pragma unroll
       for (int i = 0; i < MT; ++i) {
+        const size_t trow = VOL ? (size_t)rows[t] : (size_t)bin * T + t;
         if constexpr (NC % 2 == 0) {
-          const uint4* row = tth + (((size_t)bin * T + t) * MT + i) * 64 * (NC / 2);
+          const uint4* row = tth + (trow * MT + i) * 64 * (NC / 2);
 #pragma unroll
           for (int c2 = 0; c2 < NC / 2; ++c2) {
             const uint4 v = row[lane * (NC / 2) + c2];
@@ -145,7 +154,7 @@ __global__ void __launch_bounds__(kHalfThreads) batch_half_kernel(RadialKArgs a,
             tv[t][i][2 * c2 + 1][1] = __builtin_bit_cast(half2_t, v.w);
           }
         } else {
-          const uint2* row2 = reinterpret_cast<const uint2*>(tth) + (((size_t)bin * T + t) * MT + i) * 64 * NC;
+          const uint2* row2 = reinterpret_cast<const uint2*>(tth) + (trow * MT + i) * 64 * NC;
 #pragma unroll
           for (int c = 0; c < NC; ++c) {
             const uint2 v = row2[lane * NC + c];
@@ -155,7 +164,7 @@ __global__ void __launch_bounds__(kHalfThreads) batch_half_kernel(RadialKArgs a,
         }
       }
   };
-  auto load_x = [&](const PairRec<float>& r, uint2 (&xs)[2 * NC]) {
+  auto load_x = [&](const Rec& r, uint2 (&xs)[2 * NC]) {
     const _Float16* xp = xg + (int64_t)(r.ij & kPairMask) * S * KB * 2 + xlane;
 #pragma unroll
     for (int c = 0; c < NC; ++c) {
@@ -172,11 +181,21 @@ __global__ void __launch_bounds__(kHalfThreads) batch_half_kernel(RadialKArgs a,
   uint2 xc[2 * NC];  // input of the next pair to run
   // the pairs of one entry (one bin) with its coefficients tv; rec_n is its
   // first pair, whose input is in xc
-  auto run_entry = [&](const int4& en, const Coef& tv, PairRec<float> rec_n) {
-    const float bscale = hscale[bin_of(en)] * xmax;
+  auto run_entry = [&](const int4& en, const Coef& tv, Rec rec_n, const int* rows) {
+    float bscale, rel[T];
+    if constexpr (VOL) {
+      float hmax = 0.f;
+#pragma unroll
+      for (int t = 0; t < T; ++t) hmax = fmaxf(hmax, hscale[rows[t]]);
+#pragma unroll
+      for (int t = 0; t < T; ++t) rel[t] = hscale[rows[t]] / hmax;  // powers of two: exact
+      bscale = hmax * xmax;
+    } else {
+      bscale = hscale[bin_of(en)] * xmax;
+    }
     const int p_end = en.z;
     for (int p = 0; p < p_end; ++p) {
-      const PairRec<float> rec = rec_n;
+      const Rec rec = rec_n;
       rec_n = recs[en.y + (p + 1 < p_end ? p + 1 : p)];
 
       const float c1 = rec.c, s1 = rec.s;
@@ -214,16 +233,36 @@ __global__ void __launch_bounds__(kHalfThreads) batch_half_kernel(RadialKArgs a,
 
       const _Float16 wh = (_Float16)rec.w;
       const half2_t w2 = {wh, wh};
+      half2_t wt[T];  // VOL: the pair's tap weights relative to the entry's scale
+      if constexpr (VOL) {
+#pragma unroll
+        for (int t = 0; t < T; ++t) {
+          const _Float16 h = (_Float16)(vol_tap_weight<float>(a.vmode, a.vslice, t, rec.w, rec.w2) * rel[t]);
+          wt[t] = half2_t{h, h};
+        }
+      }
       typename Mfma<float>::v4 acc[MH];
 #pragma unroll
       for (int i = 0; i < MH; ++i) acc[i] = typename Mfma<float>::v4{0, 0, 0, 0};
-      // interpolated coefficients of tile i, chunk c (Horner in packed FP16)
+      // interpolated coefficients of tile i, chunk c (Horner in packed FP16; VOL: the weighted taps)
       auto coef = [&](int i, int c) {
-        half2_t v0 = tv[T - 1][i][c][0], v1 = tv[T - 1][i][c][1];
+        half2_t v0, v1;
+        if constexpr (VOL) {
+          v0 = wt[0] * tv[0][i][c][0];
+          v1 = wt[0] * tv[0][i][c][1];
 #pragma unroll
-        for (int t = T - 2; t >= 0; --t) {
-          v0 = v0 * w2 + tv[t][i][c][0];
-          v1 = v1 * w2 + tv[t][i][c][1];
+          for (int t = 1; t < T; ++t) {
+            v0 = wt[t] * tv[t][i][c][0] + v0;
+            v1 = wt[t] * tv[t][i][c][1] + v1;
+          }
+        } else {
+          v0 = tv[T - 1][i][c][0];
+          v1 = tv[T - 1][i][c][1];
+#pragma unroll
+          for (int t = T - 2; t >= 0; --t) {
+            v0 = v0 * w2 + tv[t][i][c][0];
+            v1 = v1 * w2 + tv[t][i][c][1];
+          }
         }
         return half4_t{v0[0], v0[1], v1[0], v1[1]};
       };
@@ -275,10 +314,13 @@ __global__ void __launch_bounds__(kHalfThreads) batch_half_kernel(RadialKArgs a,
     const int e = grab();
     if (e >= e_end) break;
     const int4 en = a.entries[e];
-    load_coef(bin_of(en), tv);
-    const PairRec<float> rec0 = recs[en.y];
+    int rows[T];
+#pragma unroll
+    for (int t = 0; t < T; ++t) rows[t] = VOL ? en.x + vol_tap_offset(a.vmode, a.vslice, t, a.len_z) : 0;
+    load_coef(bin_of(en), rows, tv);
+    const Rec rec0 = recs[en.y];
     load_x(rec0, xc);
-    run_entry(en, tv, rec0);
+    run_entry(en, tv, rec0, rows);
   }
   __syncthreads();
   float* yp = reinterpret_cast<float*>(a.ypart) + (size_t)item * a.Rc * 2 * KB * S;

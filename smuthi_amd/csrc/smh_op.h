@@ -84,6 +84,8 @@ struct smh_op {
   float* d_vtt[2] = {nullptr, nullptr};
   float* d_vtts[2] = {nullptr, nullptr};    // the FP32 tap rows in slot fragment order (8-RHS slot kernel)
   double* d_vtts64[2] = {nullptr, nullptr};  // and in FP64 for the FP64 8-RHS slot kernel
+  _Float16* d_vtth[2] = {nullptr, nullptr};  // and in FP16 for the FP16-operand 8-RHS kernel,
+  float* d_vhs[2] = {nullptr, nullptr};      // each row divided by its power of two d_vhs[row]
   double* d_vtt64[2] = {nullptr, nullptr};  // FP64 tap tables (spline: FP64 moments)
   int vmode = 1, vq = 1;  // interpolation mode (vol_tap_offset) and tap rows per (rho, z) node
   float* d_vrowsum[2] = {nullptr, nullptr};

@@ -769,7 +769,9 @@ int op_reserve_batch(smh_op* op) {
 }
 
 bool op_has_batch(const smh_op* op, int precision) {
-  if (precision == SMH_FP16) return op->kind == 0 && op->bbin[0].Rc > 0 && op->d_tth != nullptr;
+  if (precision == SMH_FP16)
+    return (op->kind == 0 && op->bbin[0].Rc > 0 && op->d_tth != nullptr) ||
+           (op->kind == 1 && op->vbbin[0][0].Rc > 0 && op->d_vtth[0] != nullptr && op->d_vtth[1] != nullptr);
   if (op->kind == 1) return op->vbbin[0][precision == SMH_FP32 ? 0 : 1].Rc > 0;
   return op->kind == 0 && op->bbin[precision == SMH_FP32 ? 0 : 1].Rc > 0;
 }
@@ -798,7 +800,8 @@ int op_apply_batch(smh_op* op, int precision, const void* x_pad, void* y_pad, bo
   smh_ctx* ctx = op->ctx;
   SMH_CHECK(ctx, op->has_table, SMH_ERR_STATE, "coupling table not set");
   if (op->kind == 1) {
-    SMH_CHECK(ctx, precision != SMH_FP16, SMH_ERR_UNSUPPORTED, "no FP16 volume matvec");
+    SMH_CHECK(ctx, precision != SMH_FP16 || op_has_batch(op, SMH_FP16), SMH_ERR_UNSUPPORTED,
+              "no FP16 matvec for this volume operator");
     return vol_apply_batch(op, precision, x_pad, y_pad, time_kernel);
   }
   const bool half = precision == SMH_FP16;
@@ -935,7 +938,8 @@ void smh_op_free(smh_op* op) {
   for (auto& row : op->vbbin)
     for (auto& b : row) free_bin(b);
   for (int t = 0; t < 2; ++t) {
-    void* vp[] = {op->d_vtab[t], op->d_vtt[t], op->d_vtts[t], op->d_vtts64[t], op->d_vtt64[t], op->d_vrowsum[t]};
+    void* vp[] = {op->d_vtab[t],   op->d_vtt[t], op->d_vtts[t], op->d_vtts64[t],
+                  op->d_vtth[t],   op->d_vhs[t], op->d_vtt64[t], op->d_vrowsum[t]};
     for (void* p : vp)
       if (p) hipFree(p);
   }

@@ -1193,7 +1193,7 @@ extern "C" int smh_solve(smh_op* op, const double* t_diag, const double* rhs, do
   SMH_CHECK(ctx, tol > 0 && restart >= 1 && restart < kMaxBasis && maxiter >= 1, SMH_ERR_ARG,
             "invalid tolerance / restart / maxiter");
   SMH_CHECK(ctx, mode != SMH_SOLVE_MIXED16 || (smh::op_has_batch(op, SMH_FP16) && smh::op_has_batch(op, SMH_FP64)),
-            SMH_ERR_UNSUPPORTED, "FP16-operand solves need a radial operator with the batched FP16 kernel");
+            SMH_ERR_UNSUPPORTED, "FP16-operand solves need a lookup operator with the batched FP16 kernel");
   SMH_CHECK(ctx, op->has_table, SMH_ERR_STATE, "coupling table not set");
   SMH_HIP(ctx, hipSetDevice(ctx->device));
   const auto t0 = std::chrono::steady_clock::now();

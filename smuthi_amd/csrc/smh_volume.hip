@@ -20,6 +20,7 @@
 #include "smh_batch_kernel.h"
 #include "smh_batch_slots.h"
 #include "smh_batch_slots64.h"
+#include "smh_batch_half.h"
 #include "smh_lookup.h"
 #include "smh_special.h"
 
@@ -326,6 +327,57 @@ __global__ void vol_slot_taps64_kernel(const float2* __restrict__ table, const d
   }
 }
 
+// The same rows in FP16 in the fragment order of batch_half_kernel<..., VOL>:
+// [rows][MT][64 lanes][NC][4], lane (q, rr) of tile i holding
+// A(slot 16 (i mod MH) + rr, slot 16 c + 4 q + j) / hs[row], hs[row] the
+// smallest power of two with max |A| / hs <= 2^12 over the row; one block
+// per row.
+__global__ void vol_half_taps_kernel(const float2* __restrict__ table, const double2* __restrict__ mr,
+                                     const double2* __restrict__ mz, const double2* __restrict__ mrz, int vq, int B,
+                                     int S, const int* __restrict__ n_of, _Float16* th, float* hs) {
+  const int MH = S / 16, MT = 2 * MH, NC = S / 16;
+  const int per_row = MT * 64 * NC * 4;
+  const int64_t row = blockIdx.x;
+  const int64_t node = row / vq;
+  const int kind = (int)(row - node * vq);
+  __shared__ double red[256];
+  auto value = [&](int f) {
+    int rem = f;
+    const int j = rem & 3;
+    rem >>= 2;
+    const int c = rem % NC;
+    rem /= NC;
+    const int ln = rem & 63;
+    const int i = rem >> 6;
+    const int q = ln >> 4, rr = ln & 15;
+    const int n1 = n_of[16 * (i % MH) + rr];
+    const int n2 = n_of[16 * c + 4 * q + j];
+    if (n1 < 0 || n2 < 0) return 0.0;
+    const int64_t o = (node * B + n1) * B + n2;
+    double2 v;
+    if (kind == 0) {
+      const float2 t = table[o];
+      v = make_double2((double)t.x, (double)t.y);
+    } else {
+      v = kind == 1 ? mr[o] : (kind == 2 ? mz[o] : mrz[o]);
+    }
+    return i < MH ? v.x : v.y;
+  };
+  double m = 0.0;
+  for (int f = threadIdx.x; f < per_row; f += blockDim.x) m = fmax(m, fabs(value(f)));
+  red[threadIdx.x] = m;
+  __syncthreads();
+  for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + w]);
+    __syncthreads();
+  }
+  const double mx = red[0];
+  const int e = mx > 0.0 ? (int)ceil(log2(mx)) - 12 : 0;
+  const double inv = ldexp(1.0, -e);
+  if (threadIdx.x == 0) hs[row] = ldexpf(1.0f, e);
+  for (int f = threadIdx.x; f < per_row; f += blockDim.x) th[row * per_row + f] = (_Float16)(value(f) * inv);
+}
+
 // Row sums of a fragment-order tap table: per row max over n1 of
 // sum_n2 |Re A(n1, n2)| + |Im A(n1, n2)| (slightly enlarged), one block per row.
 __global__ void frag_rowsum_kernel(const float* __restrict__ tt, int64_t rows, int B, int Bh, int NG, int FR,
@@ -467,6 +519,32 @@ KernelInfo vol_batch_kernel_info() {
   k.sy = Sh::SY;
   return k;
 }
+// FP16 operands: where the FP32 slot-order kernel is used (same binning and
+// input slot order)
+template <int L, int Mx>
+void* vol_half_fn() {
+  if constexpr (SlotShape<L, Mx, 4>::fits && Mx <= 3 && SlotShape<L, Mx, 4>::S % 16 == 0)
+    return reinterpret_cast<void*>(&batch_half_kernel<L, Mx, 4, true>);
+  else
+    return nullptr;
+}
+void* vol_half_kernel(int l, int m) {
+#define SMH_V(L, Mx) \
+  if (l == L && m == Mx) return vol_half_fn<L, Mx>();
+  SMH_V(1, 1)
+  SMH_V(2, 1)
+  SMH_V(2, 2)
+  SMH_V(3, 1)
+  SMH_V(3, 2)
+  SMH_V(3, 3)
+  SMH_V(4, 1)
+  SMH_V(4, 2)
+  SMH_V(4, 3)
+  SMH_V(4, 4)
+#undef SMH_V
+  return nullptr;
+}
+
 // FP32: the slot-order kernel where its tap fragments fit the registers
 template <int L, int Mx>
 KernelInfo vol_batch32_kernel_info() {
@@ -666,6 +744,15 @@ int vol_make_taps(smh_op* op) {
       }
       hipLaunchKernelGGL(vol_slot_taps_kernel, dim3(4096), dim3(256), 0, s, op->d_vtab[t], mr, mz, mrz, rows, op->vq,
                          op->B, S, (const int*)op->d_b32_tab + op->B, op->d_vtts[t]);
+      if (vol_half_kernel(op->l_max, op->m_max)) {
+        if (!op->d_vtth[t] && (hipMalloc(&op->d_vtth[t], (size_t)rows * 2 * S * S * sizeof(_Float16)) != hipSuccess ||
+                               hipMalloc(&op->d_vhs[t], (size_t)rows * sizeof(float)) != hipSuccess)) {
+          cleanup();
+          return smh_fail(ctx, SMH_ERR_HIP, "device allocation failed (FP16 tap rows)");
+        }
+        hipLaunchKernelGGL(vol_half_taps_kernel, dim3((unsigned)rows), dim3(256), 0, s, op->d_vtab[t], mr, mz, mrz,
+                           op->vq, op->B, S, (const int*)op->d_b32_tab + op->B, op->d_vtth[t], op->d_vhs[t]);
+      }
       if (volume_batch_kernel(SMH_FP64, op->l_max, op->m_max, op->vq == 4).slots == S) {
         if (!op->d_vtts64[t] &&
             hipMalloc(&op->d_vtts64[t], (size_t)rows * 2 * S * S * sizeof(double)) != hipSuccess) {
@@ -762,20 +849,26 @@ int vol_apply(smh_op* op, int precision, const void* x_pad, void* y_pad, bool ti
 // partials summed into y in launch order.
 int vol_apply_batch(smh_op* op, int precision, const void* x_pad, void* y_pad, bool time_kernel) {
   smh_ctx* ctx = op->ctx;
-  const int pi = precision == SMH_FP32 ? 0 : 1;
+  // the FP16-operand kernel shares the FP32 kernel's binning, slot order and partial-sum layout
+  const bool half = precision == SMH_FP16;
+  const int pi = (precision == SMH_FP32 || half) ? 0 : 1;
   const bool tap64 = pi == 1 && op->vq == 4;
-  const KernelInfo k = volume_batch_kernel(precision, op->l_max, op->m_max, tap64);
-  SMH_CHECK(ctx, k.fn != nullptr && op->vbbin[0][pi].Rc > 0, SMH_ERR_UNSUPPORTED,
+  const KernelInfo k = volume_batch_kernel(half ? SMH_FP32 : precision, op->l_max, op->m_max, tap64);
+  void* fn = half ? vol_half_kernel(op->l_max, op->m_max) : k.fn;
+  SMH_CHECK(ctx, fn != nullptr && op->vbbin[0][pi].Rc > 0, SMH_ERR_UNSUPPORTED,
             "no batched volume matvec kernel for l_max %d m_max %d", op->l_max, op->m_max);
   hipStream_t s = ctx->stream;
   const bool slots = k.slots > 0;
   SMH_CHECK(ctx,
             !slots || (op->d_b32_tab && op->b32_S == k.slots &&
-                       (pi == 0 ? op->d_vtts[0] && op->d_vtts[1] : op->d_vtts64[0] && op->d_vtts64[1])),
+                       (half ? op->d_vtth[0] && op->d_vtth[1]
+                             : (pi == 0 ? op->d_vtts[0] && op->d_vtts[1] : op->d_vtts64[0] && op->d_vtts64[1]))),
             SMH_ERR_STATE, "slot-order tap rows missing");
+  SMH_CHECK(ctx, !half || slots, SMH_ERR_STATE, "the FP16 kernel needs the slot order");
   const int S = slots ? k.slots : op->B;
   const int* slot = slots ? (const int*)op->d_b32_tab : nullptr;
-  if (pi == 0) launch_xmax((const float2*)x_pad, op->n_part * S * kBatchRhs, op->d_xmax, s);
+  // (the FP16 input was scaled by the xmax its conversion wrote)
+  if (pi == 0 && !half) launch_xmax((const float2*)x_pad, op->n_part * S * kBatchRhs, op->d_xmax, s);
   const void* xin = x_pad;
   if (pi == 1 && slots) {
     // FP64 padded [n_part][B][kBatch] -> the slot layout (padding slots stay zero)
@@ -800,8 +893,9 @@ int vol_apply_batch(smh_op* op, int precision, const void* x_pad, void* y_pad, b
       a.entries = bn.d_entries;
       a.items = bn.d_items;
       a.order = bn.d_order;
-      a.tt = slots ? (pi == 0 ? (const void*)op->d_vtts[t] : (const void*)op->d_vtts64[t])
-                   : (tap64 ? (const void*)op->d_vtt64[t] : (const void*)op->d_vtt[t]);
+      a.tt = half ? (const void*)op->d_vtth[t]
+                  : (slots ? (pi == 0 ? (const void*)op->d_vtts[t] : (const void*)op->d_vtts64[t])
+                           : (tap64 ? (const void*)op->d_vtt64[t] : (const void*)op->d_vtt[t]));
       a.x = xin;
       a.ypart = op->d_ypart;
       a.xmax = op->d_xmax;
@@ -816,11 +910,14 @@ int vol_apply_batch(smh_op* op, int precision, const void* x_pad, void* y_pad, b
       a.vslice = sl;
       a.row_s = 1;
       a.row_o = 0;
-      if (int rc_ = op_set_lds(op, k.fn, bn.lds)) return rc_;
+      if (int rc_ = op_set_lds(op, fn, bn.lds)) return rc_;
       if (bn.n_items > 0) {
         if (time_kernel) SMH_HIP(ctx, hipEventRecord(ctx->ev0, s));
-        void* args[] = {&a};
-        SMH_HIP(ctx, hipLaunchKernel(k.fn, dim3(bn.n_items), dim3(k.threads), args, bn.lds, s));
+        const float* hs = op->d_vhs[t];
+        void* args1[] = {&a};
+        void* args2[] = {&a, &hs};
+        SMH_HIP(ctx, hipLaunchKernel(fn, dim3(bn.n_items), dim3(half ? kHalfThreads : k.threads), half ? args2 : args1,
+                                     bn.lds, s));
         if (time_kernel) {
           SMH_HIP(ctx, hipEventRecord(ctx->ev1, s));
           SMH_HIP(ctx, hipEventSynchronize(ctx->ev1));

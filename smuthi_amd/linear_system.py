@@ -96,7 +96,7 @@ class _GpuCouplingOperator(SystemMatrix):
     def matvec(self, vector, precision='single'):
         """W x for one vector or a [nrhs, n] block (complex128 in and out).
         precision: 'single' (FP32), 'double' (FP64) or 'half' (FP16 operands,
-        FP32 accumulation; radial operators, a few 1e-4 relative error)."""
+        FP32 accumulation; radial and volume operators, a few 1e-4 relative error)."""
         v = np.ascontiguousarray(vector, dtype=np.complex128)
         nrhs = 1 if v.ndim == 1 else v.shape[0]
         out = np.empty_like(v)

@@ -80,11 +80,14 @@ def test_volume_batched_matvec_matches_oracle(gpu, kind, cubic, okind):
     want = V @ W.T
     got64 = op.matvec(V, 'double')
     got32 = op.matvec(V, 'single')
+    got16 = op.matvec(V, 'half')  # FP16 operands, FP32 accumulation
     for r in range(11):
         e64 = np.linalg.norm(got64[r] - want[r]) / np.linalg.norm(want[r])
         e32 = np.linalg.norm(got32[r] - want[r]) / np.linalg.norm(want[r])
+        e16 = np.linalg.norm(got16[r] - want[r]) / np.linalg.norm(want[r])
         assert e64 < 1e-11, (r, e64)
         assert e32 < 5e-6, (r, e32)
+        assert e16 < 3e-3, (r, e16)
 
 
 def _fields(n_angles):
@@ -93,7 +96,7 @@ def _fields(n_angles):
     return [(WL, np.pi - t, 0.3, pol) for t in th for pol in (0, 1)], PlaneWave
 
 
-@pytest.mark.parametrize('precision', ['double', 'mixed'])
+@pytest.mark.parametrize('precision', ['double', 'mixed', 'mixed16'])
 def test_ensembles_batched_solve_matches_oracle_lu(gpu, precision):
     """Four ensembles of 20-60 spheres inside a 1 um film share one volume table (Catmull-Rom, the
     reference's CUDA 'cubic'); 16 distinct plane waves (8 polar angles x TE/TM) are solved for every

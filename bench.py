@@ -739,6 +739,16 @@ def main_batching(args):
     k16 = op.bench_matvec_batch('half', iters=2)[0] if args.precision == 'mixed16' else None
     k32, tot32 = op.bench_matvec_batch('single', iters=2)
     k64, tot64 = op.bench_matvec_batch('double', iters=2)
+    if args.matvecs_only:
+        # the volume matvec kernels alone (no solves): a short run for kernel traces and counters
+        log('8-RHS volume matvecs over all ensembles: FP16 %s ms, FP32 %.3f ms, FP64 %.3f ms'
+            % ('%.3f' % k16 if k16 is not None else '-', k32, k64))
+        return json.dumps({'metric': '8-RHS volume matvec time, all ensembles', 'value': k16 if k16 else k32,
+                           'unit': 'ms', 'higher_is_better': False, 'n_gpus': 1,
+                           'matvec_ms': {'fp16_8rhs_all_ensembles': k16, 'fp32_8rhs_all_ensembles': k32,
+                                         'fp64_8rhs_all_ensembles': k64},
+                           'particles': sum(sizes), 'ensembles': len(sizes), 'autotune': tune,
+                           'device': dev, 'compute_units': n_cu})
     for w in range(args.warmup):
         batch.solve_with_stats(rhs[nb * w:nb * w + nb], tol=args.tol, precision=args.precision)
     t0 = time.time()
@@ -947,6 +957,8 @@ def main():
     ap.add_argument('--val-size', type=int, default=800)
     ap.add_argument('--prior-weight', type=float, default=1e-3,
                     help='pull of the learned scales towards the first-order structure')
+    ap.add_argument('--matvecs-only', action='store_true',
+                    help='config 4: time the volume matvec kernels and stop before the solves')
     ap.add_argument('--traffic-probe', action='store_true',
                     help='only run the copy calibration and three FP32 matvecs (for counter passes)')
     ap.add_argument('--explicit', type=int, default=0, metavar='N',

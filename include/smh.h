@@ -329,7 +329,7 @@ int smh_pwe_field(smh_ctx *ctx, int kind, int64_t n_k, const double *kp, const d
  *   g_dir(pol, kappa_k, alpha_a) = sum_p e^{-i kappa (x_p cos a + y_p sin a)} e^{+-i kz (z_ref - z_p)}
  *                                  sum_m e^{i m a} sum_{tau,l} coef[p][n(tau,l,m)] btab[k][dir][pol][n]
  * xyz: [n_part][3]; coef: complex128 [n_part][B] in the (l_max, m_max) layout
- * (m_max <= 4); kp, kz: complex128 [n_k] (kz upgoing); btab: complex128
+ * (l_max <= 64; more than 9 orders m run as several launches); kp, kz: complex128 [n_k] (kz upgoing); btab: complex128
  * [n_k][2 dir (up, down)][2 pol][B], the transformation coefficients
  * B_{n,pol}(+-kz/k, kappa/k) / (2 pi kz k); alpha: [n_a];
  * g: complex128 [2 dir][2 pol][n_k][n_a]. */
@@ -348,7 +348,9 @@ int smh_pwe_assemble(smh_ctx *ctx, int64_t n_part, const double *xyz, int l_max,
  * order (zeros for multipoles a particle does not have). A particle's SWE
  * counts at points with zlim[0] <= z < zlim[1] strictly outside its sphere.
  * field 0: E = sum b N; 1: H = -i k / omega sum b_(1-tau) N_tau. out:
- * [3][n_pts] complex. kernel_ms (may be NULL): device time. */
+ * [3][n_pts] complex. 1 <= l_max <= 20, 0 <= m_max <= l_max (unrolled
+ * kernels up to l_max 4, a runtime-order kernel above). kernel_ms (may be
+ * NULL): device time. */
 int smh_swe_field(smh_ctx *ctx, int64_t n_part, const double *pos, const double *radius, const double *zlim,
                   int l_max, int m_max, const double *coef, double k_re, double k_im, int64_t n_pts,
                   const double *pts, int field, double omega, int precision, double *out, double *kernel_ms);

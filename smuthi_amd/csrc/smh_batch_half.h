@@ -302,7 +302,8 @@ __global__ void __launch_bounds__(kHalfThreads) batch_half_kernel(RadialKArgs a,
         for (int v = 0; v < 4; ++v) {
           const float eps = v < 2 ? elo : ehi;
           // y' = e^{-i m1 phi} y: er y_own + eps y_other, y_other from lane ^ 1
-          const float val = fma_xor1(er * acc[i][v], eps, acc[i][v]);
+          // (the exchange folds into the multiply: v_mul_f32_dpp)
+          const float val = fmaf(er, acc[i][v], eps * lane_xor1(acc[i][v]));
           atomicAdd(yrow + 16 * i + v, fixed_bits64(val));
         }
       }

@@ -42,17 +42,6 @@ struct BatchShape {
   static constexpr int SP = (S8 + SPL - 1) / SPL;  // K-steps per part
 };
 
-// Value of lane l ^ 1 (DPP quad permute [1, 0, 3, 2]: a VALU move, no LDS).
-__device__ inline float lane_xor1(float v) {
-  return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0xB1, 0xF, 0xF, false));
-}
-__device__ inline double lane_xor1(double v) {
-  const long long b = __double_as_longlong(v);
-  const int lo = __builtin_amdgcn_mov_dpp((int)(b & 0xffffffffLL), 0xB1, 0xF, 0xF, false);
-  const int hi = __builtin_amdgcn_mov_dpp((int)(b >> 32), 0xB1, 0xF, 0xF, false);
-  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
-}
-
 template <typename Real, typename TapT, int LMAX, int MMAX, int T, bool VOL>
 __global__ void __launch_bounds__((BatchShape<Real, SvwfIdx<LMAX, MMAX>::B, T>::kThreads))
     batch_matvec_kernel(RadialKArgs a) {

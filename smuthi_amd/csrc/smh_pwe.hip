@@ -329,9 +329,13 @@ extern "C" int smh_pwe_field(smh_ctx* ctx, int kind, int64_t n_k, const double* 
 // particles of one layer as scattered_field_pwe, postprocessing/scattered_field.py:80-150).
 // One workgroup per kappa, one thread per alpha; particles are taken leaf by
 // leaf, their c staged in LDS; the leaf centre's phase is applied in FP64.
+// A launch covers NM consecutive orders m (its accumulators live in
+// registers); more than kAsmOrders orders run as several launches, each adding
+// its orders' share into g.
 namespace {
 
 constexpr int kAsmLeaf = 128;
+constexpr int kAsmOrders = 9;  // orders per launch (m_max <= 4 in one launch)
 
 struct AsmArgs {
   const double* pts;    // [n_part][3], leaf order
@@ -341,11 +345,13 @@ struct AsmArgs {
   const double2* kp;    // [n_k]
   const double2* kz;    // [n_k] upgoing kz
   const double2* btab;  // [n_k][2 dir][2 pol][B], 1 / (2 pi kz k) included
-  const int* nlist;     // [NM][LM] multipole indices n of order m = j - m_max (-1: none)
+  const int* nlist;     // [NM][LM] multipole indices n of order m = j + m_first (-1: none)
   const double* alpha;  // [n_a]
   double2* g;           // [2 dir][2 pol][n_k][n_a]
   double z_ref;
-  int B, NM, LM, n_leaves, n_k, n_a;
+  int B, LM, n_leaves, n_k, n_a;
+  int m_first;     // order of the launch's first row of nlist
+  int accumulate;  // add into g (later launches of a chunked order range)
 };
 
 template <typename Real, int NM>
@@ -445,17 +451,22 @@ __global__ void __launch_bounds__(256) pwe_assemble_kernel(AsmArgs a) {
       }
   }
   if (!active) return;
-  const int mmax = (NM - 1) / 2;
   for (int q = 0; q < Q; ++q) {
     double gr = 0, gi = 0;
 #pragma unroll
     for (int m = 0; m < NM; ++m) {
       double em_s, em_c;
-      sincos((double)(m - mmax) * al, &em_s, &em_c);
+      sincos((double)(m + a.m_first) * al, &em_s, &em_c);
       gr += (double)U[m][q][0] * em_c - (double)U[m][q][1] * em_s;
       gi += (double)U[m][q][0] * em_s + (double)U[m][q][1] * em_c;
     }
-    a.g[((size_t)q * a.n_k + k) * a.n_a + ia] = make_double2(gr, gi);
+    double2* dst = a.g + ((size_t)q * a.n_k + k) * a.n_a + ia;
+    if (a.accumulate) {
+      const double2 o = *dst;
+      gr += o.x;
+      gi += o.y;
+    }
+    *dst = make_double2(gr, gi);
   }
 }
 
@@ -479,16 +490,20 @@ extern "C" int smh_pwe_assemble(smh_ctx* ctx, int64_t n_part, const double* xyz,
   if (!ctx) return SMH_ERR_ARG;
   SMH_CHECK(ctx, n_part >= 1 && xyz && coef && kp && kz && btab && alpha && g, SMH_ERR_ARG,
             "invalid PWE assembly arguments");
-  SMH_CHECK(ctx, l_max >= 1 && m_max >= 1 && m_max <= l_max && m_max <= 4, SMH_ERR_UNSUPPORTED,
-            "l_max %d / m_max %d not supported (m_max <= 4)", l_max, m_max);
+  SMH_CHECK(ctx, l_max >= 1 && m_max >= 1 && m_max <= l_max && l_max <= 64, SMH_ERR_ARG,
+            "invalid l_max %d / m_max %d", l_max, m_max);
   SMH_CHECK(ctx, n_k >= 1 && n_a >= 1 && n_k < (1 << 30) && n_a < (1 << 30) && n_part < (1 << 30), SMH_ERR_ARG,
             "invalid sizes");
   SMH_CHECK(ctx, precision == SMH_FP32 || precision == SMH_FP64, SMH_ERR_ARG, "invalid precision %d", precision);
   if (kernel_ms) *kernel_ms = 0.0;
   SMH_HIP(ctx, hipSetDevice(ctx->device));
-  const int B = smh_blocksize(l_max, m_max), NM = 2 * m_max + 1, LM = 2 * l_max;
-  // multipole indices of each order m
-  std::vector<int> nlist((size_t)NM * LM, -1), fill(NM, 0);
+  const int B = smh_blocksize(l_max, m_max), n_orders = 2 * m_max + 1, LM = 2 * l_max;
+  // orders per launch: all of them when they fit, else chunks of kAsmOrders
+  // (the last one padded with empty orders, whose rows of nlist hold -1)
+  const int NM = n_orders <= kAsmOrders ? n_orders : kAsmOrders;
+  const int n_chunks = (n_orders + NM - 1) / NM;
+  // multipole indices of each order m (row j: order j - m_max)
+  std::vector<int> nlist((size_t)n_chunks * NM * LM, -1), fill(n_chunks * NM, 0);
   for (int tau = 0; tau < 2; ++tau)
     for (int l = 1; l <= l_max; ++l) {
       const int ml = std::min(l, m_max);
@@ -554,7 +569,6 @@ extern "C" int smh_pwe_assemble(smh_ctx* ctx, int64_t n_part, const double* xyz,
   a.g = (double2*)buf[9];
   a.z_ref = z_ref;
   a.B = B;
-  a.NM = NM;
   a.LM = LM;
   a.n_leaves = n_leaves;
   a.n_k = (int)n_k;
@@ -565,7 +579,13 @@ extern "C" int smh_pwe_assemble(smh_ctx* ctx, int64_t n_part, const double* xyz,
   hipError_t e = hipFuncSetAttribute(kfn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
   if (e == hipSuccess) e = hipEventRecord(ctx->ev0, s);
   void* args[] = {&a};
-  if (e == hipSuccess) e = hipLaunchKernel(kfn, dim3((unsigned)n_k, a_blocks), dim3(threads), args, lds, s);
+  const int* nlist_dev = a.nlist;
+  for (int c = 0; c < n_chunks && e == hipSuccess; ++c) {
+    a.nlist = nlist_dev + (size_t)c * NM * LM;
+    a.m_first = c * NM - m_max;
+    a.accumulate = c > 0;
+    e = hipLaunchKernel(kfn, dim3((unsigned)n_k, a_blocks), dim3(threads), args, lds, s);
+  }
   if (e == hipSuccess) e = hipEventRecord(ctx->ev1, s);
   if (e == hipSuccess) e = hipMemcpyAsync(g, buf[9], sizes[9], hipMemcpyDeviceToHost, s);
   if (e == hipSuccess) e = hipStreamSynchronize(s);

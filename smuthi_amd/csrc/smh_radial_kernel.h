@@ -30,6 +30,19 @@
 
 namespace smh {
 
+// Value of lane l ^ 1 (DPP quad permute [1, 0, 3, 2]: a VALU move, no LDS).
+// The permutation never reads an absent lane; bound_ctrl lets the compiler
+// fold the move into a following multiply (v_mul_f32_dpp).
+__device__ inline float lane_xor1(float v) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0xB1, 0xF, 0xF, true));
+}
+__device__ inline double lane_xor1(double v) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_mov_dpp((int)(b & 0xffffffffLL), 0xB1, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_mov_dpp((int)(b >> 32), 0xB1, 0xF, 0xF, false);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
 template <int LMAX, int MMAX>
 struct SvwfIdx {
   static constexpr int B = 2 * (MMAX * (MMAX + 2) + (LMAX - MMAX) * (2 * MMAX + 1));

@@ -17,6 +17,7 @@
 // multipole sum run in Real (FP32 or FP64). The sum is
 // formed in spherical components per m, rotated by e^{i m phi} once per m and
 // added in Cartesian components.
+#include <algorithm>
 #include <cmath>
 #include <complex>
 #include <vector>
@@ -213,6 +214,172 @@ __global__ void __launch_bounds__(kSwePoints) swe_field_kernel(SweArgs a) {
   }
 }
 
+// Any (l_max, m_max): the same sum with runtime orders. Per (point,
+// particle) the orders |m| run one after the other; for each the Hankel and
+// Legendre recurrences run up in l from their start (no per-l arrays), and
+// m = +|m| and -|m| share them. The particle chunk is sized from B so that
+// its coefficients fit the workgroup's LDS. Costs (2 m_max + 1) l_max
+// recurrence steps per pair more than the unrolled kernels, which take
+// l_max <= 4.
+constexpr int kSweMaxL = 20;
+
+template <typename Real>
+__global__ void __launch_bounds__(kSwePoints) swe_field_any_kernel(SweArgs a, int lmax, int mmax, int B, int chunk) {
+  using C = cplx<Real>;
+  extern __shared__ __align__(16) unsigned char swe_smem[];
+  double* sp = reinterpret_cast<double*>(swe_smem);  // [chunk][6]
+  C* sc = reinterpret_cast<C*>(sp + (size_t)chunk * 6);  // [chunk][B]
+
+  const int64_t ip = (int64_t)blockIdx.x * kSwePoints + threadIdx.x;
+  const bool live = ip < a.n_pts;
+  double px = 0, py = 0, pz = 0;
+  if (live) {
+    px = a.pts[3 * ip];
+    py = a.pts[3 * ip + 1];
+    pz = a.pts[3 * ip + 2];
+  }
+  C ex{0, 0}, ey{0, 0}, ez{0, 0};
+  for (int64_t p0 = 0; p0 < a.n_part; p0 += chunk) {
+    const int np = (int)(a.n_part - p0 < chunk ? a.n_part - p0 : chunk);
+    __syncthreads();
+    for (int j = threadIdx.x; j < np * 6; j += kSwePoints) sp[j] = a.part[p0 * 6 + j];
+    for (int j = threadIdx.x; j < np * B; j += kSwePoints) {
+      const double2 v = a.coef[p0 * B + j];
+      sc[j] = C{(Real)v.x, (Real)v.y};
+    }
+    __syncthreads();
+    if (!live) continue;
+    for (int q = 0; q < np; ++q) {
+      const double* pq = sp + 6 * q;
+      const double dx = px - pq[0], dy = py - pq[1], dz = pz - pq[2];
+      const double r2 = dx * dx + dy * dy + dz * dz;
+      if (!(r2 > pq[3]) || pz < pq[4] || pz >= pq[5]) continue;
+      const double r = sqrt(r2), rho = sqrt(dx * dx + dy * dy);
+      const double ir = 1.0 / r;
+      const Real ct = (Real)(dz * ir), st = (Real)(rho * ir);
+      Real cp = 1, sn = 0;
+      if (rho > 0) {
+        const double irho = 1.0 / rho;
+        cp = (Real)(dx * irho);
+        sn = (Real)(dy * irho);
+      }
+      const C x{(Real)(a.k_re * r), (Real)(a.k_im * r)};
+      const C ix = cdiv(C{Real(1), Real(0)}, x);
+      C eix;
+      if constexpr (sizeof(Real) == 8) {
+        const double dmp = exp(-x.im);
+        eix = C{dmp * cos(x.re), dmp * sin(x.re)};
+      } else {
+        const double ph = a.k_re * r;
+        const float red = (float)(ph - 6.283185307179586477 * rint(ph * 0.15915494309189533577));
+        const float dmp = expf((float)(-a.k_im * r));
+        float sr, cr;
+        sincosf(red, &sr, &cr);
+        eix = C{dmp * cr, dmp * sr};
+      }
+      const C h0 = cmul(C{eix.im, -eix.re}, ix);  // -i e^{ix} / x
+      const C h1 = cscale(cmul(cmul(eix, C{x.re, x.im + Real(1)}), cmul(ix, ix)), Real(-1));
+      const Real erx = st * cp, ery = st * sn, erz = ct;
+      const Real etx = ct * cp, ety = ct * sn, etz = -st;
+      const Real epx = -sn, epy = cp;
+      const C* b = sc + (size_t)q * B;
+      C eim{1, 0};  // e^{i am phi}
+      for (int am = 0; am <= mmax; ++am) {
+        if (am > 0) eim = cmul(eim, C{cp, sn});
+        // spherical components at m = +am (s = 0) and m = -am (s = 1), before e^{i m phi}
+        C cr[2] = {{0, 0}, {0, 0}}, cth[2] = {{0, 0}, {0, 0}}, cph[2] = {{0, 0}, {0, 0}};
+        // Legendre state at l (P, pi, tau) and l - 1; Hankel h_l and h_(l-1)
+        Real Pl, Pm1 = 0, PIl, PIm1 = 0, TAl, ppl = 0;
+        int l0;
+        if (am == 0) {
+          // l = 1 (l = 0 carries no multipole)
+          const Real P0 = (Real)0.70710678118654752440;
+          Pm1 = P0;
+          Pl = (Real)sqrt(1.5) * ct;
+          ppl = (Real)sqrt(3.0) * P0;
+          PIl = 0;
+          TAl = -st * ppl;
+          l0 = 1;
+        } else {
+          const Real pref = (Real)legendre_mm_prefactor(am);
+          Real stm1 = 1;
+          for (int j = 1; j < am; ++j) stm1 *= st;
+          Pl = pref * stm1 * st;
+          PIl = pref * stm1;
+          TAl = Real(am) * ct * PIl;
+          l0 = am;
+        }
+        C hm1 = h0, hl = h1;  // h_(l-1), h_l at l = 1
+        for (int l = 1; l < l0; ++l) {
+          const C hn = csub(cscale(cmul(ix, hl), Real(2 * l + 1)), hm1);
+          hm1 = hl;
+          hl = hn;
+        }
+        for (int l = l0; l <= lmax; ++l) {
+          const Real pf = (Real)(1.0 / sqrt(2.0 * l * (l + 1)));
+          const C hx = cmul(hl, ix);
+          const C A = cscale(hl, pf);
+          const C Bv = cscale(hx, pf * Real(l * (l + 1)));
+          const C Cv = cscale(csub(hm1, cscale(hx, Real(l))), pf);
+          for (int s = 0; s < (am > 0 ? 2 : 1); ++s) {
+            const int m = s ? -am : am;
+            const C b0 = b[swe_index(0, l, m, lmax, mmax)], b1 = b[swe_index(1, l, m, lmax, mmax)];
+            const C u = cmul(b0, A), wb = cmul(b1, Bv), wc = cmul(b1, Cv);
+            const Real mpi = Real(m) * PIl, ta = TAl;
+            cr[s].re += wb.re * Pl;
+            cr[s].im += wb.im * Pl;
+            cth[s].re += -mpi * u.im + ta * wc.re;
+            cth[s].im += mpi * u.re + ta * wc.im;
+            cph[s].re += -ta * u.re - mpi * wc.im;
+            cph[s].im += -ta * u.im + mpi * wc.re;
+          }
+          if (l == lmax) break;
+          // step l -> l + 1
+          const C hn = csub(cscale(cmul(ix, hl), Real(2 * l + 1)), hm1);
+          hm1 = hl;
+          hl = hn;
+          if (am == 0) {
+            const Real Pn = (Real)(sqrt((2.0 * l + 1) * (2.0 * l + 3)) / (l + 1)) * ct * Pl -
+                            (Real)(l / (l + 1.0) * sqrt((2.0 * l + 3) / (2.0 * l - 1))) * Pm1;
+            ppl = (Real)((l + 1) * sqrt((2.0 * l + 3) / (2.0 * l + 1))) * Pl +
+                  (Real)sqrt((2.0 * l + 3) / (2.0 * l + 1)) * ct * ppl;
+            Pm1 = Pl;
+            Pl = Pn;
+            TAl = -st * ppl;
+          } else {
+            const Real c1 = (Real)sqrt((2.0 * l + 1) * (2.0 * l + 3) / ((l + 1.0 - am) * (l + 1.0 + am)));
+            const Real c2 =
+                (Real)sqrt((2.0 * l + 3) * (l - am) * (l + am) / ((2.0 * l - 1) * (l + 1.0 - am) * (l + 1.0 + am)));
+            const Real c3 = (Real)((l + 1.0 + am) * sqrt((2.0 * l + 3) * (l + 1.0 - am) / ((2.0 * l + 1) * (l + 1.0 + am))));
+            const Real Pn = c1 * ct * Pl - c2 * Pm1;
+            const Real PIn = c1 * ct * PIl - c2 * PIm1;
+            TAl = Real(l + 1) * ct * PIn - c3 * PIl;
+            Pm1 = Pl;
+            Pl = Pn;
+            PIm1 = PIl;
+            PIl = PIn;
+          }
+        }
+        for (int s = 0; s < (am > 0 ? 2 : 1); ++s) {
+          const C e = s ? C{eim.re, -eim.im} : eim;
+          const C r_ = cmul(cr[s], e), t_ = cmul(cth[s], e), f_ = cmul(cph[s], e);
+          ex.re += r_.re * erx + t_.re * etx + f_.re * epx;
+          ex.im += r_.im * erx + t_.im * etx + f_.im * epx;
+          ey.re += r_.re * ery + t_.re * ety + f_.re * epy;
+          ey.im += r_.im * ery + t_.im * ety + f_.im * epy;
+          ez.re += r_.re * erz + t_.re * etz;
+          ez.im += r_.im * erz + t_.im * etz;
+        }
+      }
+    }
+  }
+  if (live) {
+    a.out[ip] = make_double2((double)ex.re, (double)ex.im);
+    a.out[a.n_pts + ip] = make_double2((double)ey.re, (double)ey.im);
+    a.out[2 * a.n_pts + ip] = make_double2((double)ez.re, (double)ez.im);
+  }
+}
+
 template <int L, int M>
 void* swe_fn(int precision) {
   return precision == SMH_FP32 ? reinterpret_cast<void*>(&swe_field_kernel<float, L, M>)
@@ -247,8 +414,10 @@ extern "C" int smh_swe_field(smh_ctx* ctx, int64_t n_part, const double* pos, co
   SMH_CHECK(ctx, precision == SMH_FP32 || precision == SMH_FP64, SMH_ERR_ARG, "invalid precision %d", precision);
   SMH_CHECK(ctx, n_part >= 0 && n_pts >= 0 && n_pts < (1 << 30), SMH_ERR_ARG, "invalid sizes");
   SMH_CHECK(ctx, field == 0 || omega != 0.0, SMH_ERR_ARG, "omega is zero");
+  SMH_CHECK(ctx, l_max >= 1 && m_max >= 0 && m_max <= l_max && l_max <= kSweMaxL, SMH_ERR_ARG,
+            "l_max %d / m_max %d out of range (l_max <= %d)", l_max, m_max, kSweMaxL);
+  // unrolled kernels for l_max <= 4, the runtime-order kernel otherwise
   void* fn = swe_kernel(precision, l_max, m_max);
-  SMH_CHECK(ctx, fn != nullptr, SMH_ERR_UNSUPPORTED, "no SWE field kernel for l_max %d m_max %d", l_max, m_max);
   if (kernel_ms) *kernel_ms = 0.0;
   if (n_pts == 0) return SMH_OK;
   SMH_CHECK(ctx, pts && out, SMH_ERR_ARG, "points / output are NULL");
@@ -304,9 +473,18 @@ extern "C" int smh_swe_field(smh_ctx* ctx, int64_t n_part, const double* pos, co
   a.k_re = k_re;
   a.k_im = k_im;
   const unsigned nb = (unsigned)((n_pts + kSwePoints - 1) / kSwePoints);
+  const size_t csize = precision == SMH_FP32 ? sizeof(float2) : sizeof(double2);
+  int lmax_ = l_max, mmax_ = m_max, B_ = B;
+  int chunk = (int)std::min<size_t>(32, 65536 / (6 * sizeof(double) + (size_t)B * csize));
+  const size_t lds = fn ? 0 : (size_t)chunk * (6 * sizeof(double) + (size_t)B * csize);
+  if (!fn)
+    fn = precision == SMH_FP32 ? reinterpret_cast<void*>(&swe_field_any_kernel<float>)
+                               : reinterpret_cast<void*>(&swe_field_any_kernel<double>);
   hipError_t e = hipEventRecord(ctx->ev0, s);
-  void* args[] = {&a};
-  if (e == hipSuccess) e = hipLaunchKernel(fn, dim3(nb), dim3(kSwePoints), args, 0, s);
+  void* args_fixed[] = {&a};
+  void* args_any[] = {&a, &lmax_, &mmax_, &B_, &chunk};
+  if (e == hipSuccess)
+    e = hipLaunchKernel(fn, dim3(nb), dim3(kSwePoints), lds ? args_any : args_fixed, lds, s);
   if (e == hipSuccess) e = hipEventRecord(ctx->ev1, s);
   if (e == hipSuccess) e = hipMemcpyAsync(out, buf[3], sizes[3], hipMemcpyDeviceToHost, s);
   if (e == hipSuccess) e = hipStreamSynchronize(s);

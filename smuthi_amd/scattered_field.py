@@ -29,8 +29,6 @@ def _assemble(vacuum_wavelength, particles, layer_system, i_swe, k_parallel, azi
     k = complex(om * layer_system.refractive_indices[i_swe])
     l_max = max(p.l_max for p in particles)
     m_max = max(p.m_max for p in particles)
-    if m_max > 4:
-        raise NotImplementedError('PWE assembly supports m_max <= 4')
     B = flds.blocksize(l_max, m_max)
     coef = np.zeros((len(particles), B), complex)
     # particles grouped by their own (l_max, m_max): one index map and one array copy per group

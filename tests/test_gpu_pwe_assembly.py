@@ -62,6 +62,30 @@ def test_scattered_field_pwe_matches_oracle(gpu, layer_number, direct, mediated)
         assert np.abs(pd.coefficients - od.coefficients).max() < tol * scale, prec
 
 
+def test_scattered_field_pwe_high_orders(gpu):
+    # m_max 5..7: more than 9 orders m, assembled in several launches that add into one expansion;
+    # mixed cut-offs (particles padded to the largest l_max, m_max)
+    from smuthi_amd.layers import LayerSystem
+    from smuthi_amd.scattered_field import scattered_field_pwe
+    rng = np.random.default_rng(21)
+    plist = []
+    for i, (l, m) in enumerate([(7, 7), (6, 5), (5, 5), (3, 2), (7, 4), (6, 6)] * 25):
+        p = osys.Sphere([rng.uniform(-1500, 1500), rng.uniform(-1500, 1500), rng.uniform(80, 320)], 1.6, 60, l, m)
+        nb = 2 * (m * (m + 2) + (l - m) * (2 * m + 1))
+        p.scattered_field = (rng.standard_normal(nb) + 1j * rng.standard_normal(nb)) * rng.uniform(0.1, 1)
+        plist.append(p)
+    thick, n = [0, 400, 0], [1.5, 2.0 + 0.01j, 1.0]
+    kp = ob.reasonable_Sommerfeld_kpar_contour(WL, layer_refractive_indices=n, neff_resolution=5e-2,
+                                               neff_imag=1e-2)
+    alpha = np.linspace(0, 2 * np.pi, 20)
+    ou, od = osys.scattered_field_pwe(WL, plist, ol.LayerSystem(thick, n), 1, kp, alpha)
+    scale = max(np.abs(ou.coefficients).max(), np.abs(od.coefficients).max())
+    for prec, tol in (('double', 1e-10), ('single', 2e-5)):
+        pu, pd = scattered_field_pwe(WL, plist, LayerSystem(thick, n), 1, kp, alpha, precision=prec)
+        assert np.abs(pu.coefficients - ou.coefficients).max() < tol * scale, prec
+        assert np.abs(pd.coefficients - od.coefficients).max() < tol * scale, prec
+
+
 def test_extinction_cross_section_matches_oracle(gpu):
     from smuthi_amd.far_field import extinction_cross_section
     from smuthi_amd.initial_field import PlaneWave

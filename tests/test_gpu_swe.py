@@ -68,6 +68,36 @@ def test_swe_sum_matches_oracle(gpu, field):
         assert all(np.isfinite(g[700]) for g in got)
 
 
+@pytest.mark.parametrize('field', ['E', 'H'])
+def test_swe_sum_high_orders(gpu, field):
+    # l_max 5..7 and m_max 0: the runtime-order kernel (the unrolled ones take l_max <= 4, m_max >= 1)
+    from smuthi_amd.expansions import SphericalWaveSum
+    rng = np.random.default_rng(11)
+    plist = []
+    for i, (l, m) in enumerate([(7, 7), (6, 4), (5, 5), (2, 0), (3, 3), (7, 2)] * 5):
+        p = osys.Sphere([rng.uniform(-1500, 1500), rng.uniform(-1500, 1500), rng.uniform(80, 320)], 1.6, 60, l, m)
+        p.scattered_field = (rng.standard_normal(ob.blocksize(l, m)) + 1j * rng.standard_normal(ob.blocksize(l, m)))
+        plist.append(p)
+    k = 2 * np.pi / WL * 1.5 + 0.003j
+    x, y, z = _points(plist, 500, 12)
+    want = osw.direct_field([(p.scattered_field, p.l_max, p.m_max, p.position, p.radius) for p in plist], k, x, y, z,
+                            field, ob.angular_frequency(WL), z_range=(0.0, 400.0))
+    s = SphericalWaveSum(k, plist, 0.0, 400.0)
+    for prec, tol in (('double', 1e-11), ('single', 5e-5)):
+        got = s.electric_field(x, y, z, precision=prec) if field == 'E' else \
+            s.magnetic_field(x, y, z, WL, precision=prec)
+        assert _rel(got, want) < tol, (prec, _rel(got, want))
+    # m_max = 0 alone (a single order group)
+    q = [osys.Sphere([0.0, 0.0, 200.0], 1.6, 60, 3, 0)]
+    q[0].scattered_field = rng.standard_normal(ob.blocksize(3, 0)) + 1j * rng.standard_normal(ob.blocksize(3, 0))
+    want0 = osw.direct_field([(q[0].scattered_field, 3, 0, q[0].position, q[0].radius)], k, x, y, z, field,
+                             ob.angular_frequency(WL), z_range=(0.0, 400.0))
+    s0 = SphericalWaveSum(k, q, 0.0, 400.0)
+    got0 = s0.electric_field(x, y, z, precision='double') if field == 'E' else \
+        s0.magnetic_field(x, y, z, WL, precision='double')
+    assert _rel(got0, want0) < 1e-11
+
+
 def test_single_swe_drop_in(gpu):
     from smuthi_amd.expansions import SphericalWaveExpansion
     rng = np.random.default_rng(6)

@@ -598,7 +598,8 @@ def main_headline(args):
         'prepare_seconds': prep_s,
         'table_seconds': op.table_seconds,
         'autotune': tune,
-        'roofline': {'bound': 'valu-issue (VALU busy ~71 %, MFMA ~9 % in the SQ counters)' if half else 'mfma',
+        'roofline': {'bound': ('instruction issue: VALU ~67 % and MFMA ~10 % of the SIMD cycles in the SQ counters '
+                               '(they do not co-issue on gfx950)') if half else 'mfma',
                      'achieved': achieved, 'peak': peak, 'unit': 'TFLOP/s',
                      'frac': achieved / peak,
                      'traffic': measured_traffic('batch16' if half else 'batch'),

@@ -11,8 +11,9 @@
 //     and rotated by e^{i m2 phi} in packed FP16;
 //   * a chunk covers 16 slots (4 slot groups): lane (q, rr) holds the 4 slots
 //     of group 4 c + q of chunk c as its K values, so its azimuthal phase
-//     is chosen per lane from the |m| values of the chunk's groups (one-hot
-//     weights, as the receiver rotation of batch_slots_kernel);
+//     cos / sin(|m| phi) is a per-lane polynomial in cos phi (Chebyshev
+//     coefficients of the lane's |m|), shared with the receiver rotation of
+//     result tile c;
 //   * the complex product runs inside the MFMA: one 16x16x32 MFMA per row
 //     tile and chunk takes K = [Re L | Im L] of the chunk's slots against
 //     [Re z; -Im z] (re columns) or [Im z; Re z] (im columns), so each lane
@@ -46,6 +47,27 @@ typedef _Float16 half4_t __attribute__((ext_vector_type(4)));
 typedef _Float16 half8_t __attribute__((ext_vector_type(8)));
 typedef unsigned u32x2_t __attribute__((ext_vector_type(2)));
 
+// Coefficients of c^k in the Chebyshev polynomials T_m(c) (cos(m phi) =
+// T_m(cos phi)) and U_{m-1}(c) (sin(m phi) = sin phi U_{m-1}(cos phi); zero
+// for m = 0), m, k <= kChebMax, from T_{n+1} = 2 c T_n - T_{n-1} and the same
+// recurrence for U.
+constexpr int kChebMax = 4;
+struct ChebTable {
+  float t[kChebMax + 1][kChebMax + 1];
+  float u[kChebMax + 1][kChebMax + 1];
+  constexpr ChebTable() : t(), u() {
+    t[0][0] = 1.f;
+    t[1][1] = 1.f;
+    u[1][0] = 1.f;  // U_0 = 1
+    for (int m = 1; m < kChebMax; ++m)
+      for (int k = 0; k <= kChebMax; ++k) {
+        t[m + 1][k] = (k > 0 ? 2.f * t[m][k - 1] : 0.f) - t[m - 1][k];
+        u[m + 1][k] = (k > 0 ? 2.f * u[m][k - 1] : 0.f) - u[m - 1][k];
+      }
+  }
+};
+__device__ constexpr ChebTable kCheb{};
+
 // 16 waves per workgroup: the receiver accumulator takes the CU's LDS, so the
 // workgroup is the CU's only one, and 4 waves per SIMD hide the gather latency
 constexpr int kHalfThreads = 1024;
@@ -64,7 +86,8 @@ __global__ void __launch_bounds__(kHalfThreads) batch_half_kernel(RadialKArgs a,
   constexpr int KB = kBatchRhs;
   static_assert(KB * 2 == 16, "16 result columns = kBatch right-hand sides x (re, im)");
   static_assert(S % 16 == 0, "slots per particle must be a multiple of 16");
-  static_assert(MMAX <= 3, "phase selection covers |m| <= 3");
+  static_assert(MMAX <= kChebMax, "Chebyshev phase coefficients cover |m| <= kChebMax");
+  static_assert(MH == NC, "result tile c and input chunk c share the lane's slot group");
   constexpr SlotOrder<LMAX, MMAX> so{};
   static_assert(so.ok, "slot order");
 
@@ -99,31 +122,27 @@ __global__ void __launch_bounds__(kHalfThreads) batch_half_kernel(RadialKArgs a,
   const int rr = lane & 15, q = lane >> 4;
   const int rhs = rr >> 1, zp = rr & 1;
   const unsigned zmask = zp ? 0u : 0x80000000u;  // z: sign of the b term for the re column
-  // receiver rotation, as batch_slots_kernel: result rows of tile i are slot group 4 i + q
-  float hot[MH][4];
-  unsigned smlo[MH], smhi[MH];
-#pragma unroll
-  for (int i = 0; i < MH; ++i) {
-    const int g0 = 4 * i;
-    const int ra = q == 0 ? so.absm[g0] : (q == 1 ? so.absm[g0 + 1] : (q == 2 ? so.absm[g0 + 2] : so.absm[g0 + 3]));
-    const int sg = q == 0 ? so.sgn[g0] : (q == 1 ? so.sgn[g0 + 1] : (q == 2 ? so.sgn[g0 + 2] : so.sgn[g0 + 3]));
-#pragma unroll
-    for (int am = 0; am < 4; ++am) hot[i][am] = ra == am ? 1.f : 0.f;
-    // (the im column takes -sin: y'_im = cos y_im - sin y_re)
-    smlo[i] = (sg < 0 ? 0x80000000u : 0u) ^ (zp ? 0x80000000u : 0u);
-    smhi[i] = (sg > 0 ? 0u : 0x80000000u) ^ (zp ? 0x80000000u : 0u);
-  }
-  // input rotation: the K values of chunk c are the 4 slots of group 4 c + q
-  // (the same group structure as the result rows of tile c)
-  float zhot[NC][4];
-  unsigned zlo[NC], zhi[NC];
+  // The rows of result tile i are slot group 4 i + q (receiver rotation, as
+  // batch_slots_kernel) and the K values of chunk c are the 4 slots of group
+  // 4 c + q (input rotation): tile c and chunk c share the lane's |m|. Per
+  // chunk the lane evaluates cos(|m| phi) = T_|m|(cos phi) and sin(|m| phi) =
+  // sin phi U_{|m|-1}(cos phi) as polynomials in cos phi whose coefficients
+  // (Chebyshev T and U) it holds for its |m|.
+  float tcf[NC][MMAX + 1], ucf[NC][MMAX + 1];
+  unsigned smlo[MH], smhi[MH], zlo[NC], zhi[NC];
 #pragma unroll
   for (int c = 0; c < NC; ++c) {
     const int g0 = 4 * c;
     const int ra = q == 0 ? so.absm[g0] : (q == 1 ? so.absm[g0 + 1] : (q == 2 ? so.absm[g0 + 2] : so.absm[g0 + 3]));
     const int sg = q == 0 ? so.sgn[g0] : (q == 1 ? so.sgn[g0 + 1] : (q == 2 ? so.sgn[g0 + 2] : so.sgn[g0 + 3]));
 #pragma unroll
-    for (int am = 0; am < 4; ++am) zhot[c][am] = ra == am ? 1.f : 0.f;
+    for (int k = 0; k <= MMAX; ++k) {
+      tcf[c][k] = kCheb.t[ra][k];
+      ucf[c][k] = kCheb.u[ra][k];
+    }
+    // (the im column takes -sin: y'_im = cos y_im - sin y_re)
+    smlo[c] = (sg < 0 ? 0x80000000u : 0u) ^ (zp ? 0x80000000u : 0u);
+    smhi[c] = (sg > 0 ? 0u : 0x80000000u) ^ (zp ? 0x80000000u : 0u);
     zlo[c] = (sg < 0 ? 0x80000000u : 0u) ^ zmask;  // slots j < 2
     zhi[c] = (sg > 0 ? 0u : 0x80000000u) ^ zmask;  // slots j >= 2 (m < 0 in mixed groups)
   }
@@ -198,11 +217,24 @@ __global__ void __launch_bounds__(kHalfThreads) batch_half_kernel(RadialKArgs a,
       const Rec rec = rec_n;
       rec_n = recs[en.y + (p + 1 < p_end ? p + 1 : p)];
 
+      // cos / sin of |m| phi for the lane's group of each chunk (Horner in cos phi up to the
+      // largest |m| of the chunk's four groups)
       const float c1 = rec.c, s1 = rec.s;
-      const float c2 = c1 * c1 - s1 * s1, s2 = 2.f * c1 * s1;
-      const float c3 = c2 * c1 - s2 * s1, s3 = s2 * c1 + c2 * s1;
-      const float cs[4] = {1.f, c1, c2, c3};
-      const float sn[4] = {0.f, s1, s2, s3};
+      float cph[NC], sph[NC];
+#pragma unroll
+      for (int c = 0; c < NC; ++c) {
+        const int dg = max(max(so.absm[4 * c], so.absm[4 * c + 1]), max(so.absm[4 * c + 2], so.absm[4 * c + 3]));
+        float v = tcf[c][dg];
+#pragma unroll
+        for (int k = MMAX - 1; k >= 0; --k)
+          if (k < dg) v = fmaf(v, c1, tcf[c][k]);
+        cph[c] = v;
+        float u = dg > 0 ? ucf[c][dg - 1] : 0.f;
+#pragma unroll
+        for (int k = MMAX - 2; k >= 0; --k)
+          if (k < dg - 1) u = fmaf(u, c1, ucf[c][k]);
+        sph[c] = u * s1;
+      }
 
       // B operands per chunk, FP16: zb = the lane's column of z = e^{i m2 phi} x
       // (cz a + e b), zb2 = the column of the other part with the sign of the
@@ -210,15 +242,7 @@ __global__ void __launch_bounds__(kHalfThreads) batch_half_kernel(RadialKArgs a,
       half4_t zb[NC], zb2[NC];
 #pragma unroll
       for (int c = 0; c < NC; ++c) {
-        float cz = 0.f, sz = 0.f;
-#pragma unroll
-        for (int am = 0; am <= MMAX; ++am) {
-          if (so.absm[4 * c] != am && so.absm[4 * c + 1] != am && so.absm[4 * c + 2] != am &&
-              so.absm[4 * c + 3] != am)
-            continue;
-          cz = fmaf(zhot[c][am], cs[am], cz);
-          if (am > 0) sz = fmaf(zhot[c][am], sn[am], sz);
-        }
+        const float cz = cph[c], sz = sph[c];
         const _Float16 czh = (_Float16)cz, eloh = (_Float16)xor_sign(sz, zlo[c]), ehih = (_Float16)xor_sign(sz, zhi[c]);
         const half2_t cz2 = {czh, czh}, elo2 = {eloh, eloh}, ehi2 = {ehih, ehih};
         const half2_t a01 = __builtin_bit_cast(half2_t, xc[2 * c].x), a23 = __builtin_bit_cast(half2_t, xc[2 * c].y);
@@ -286,17 +310,7 @@ __global__ void __launch_bounds__(kHalfThreads) batch_half_kernel(RadialKArgs a,
       const float scl = sh * bscale;
 #pragma unroll
       for (int i = 0; i < MH; ++i) {
-        float er = 0.f, ei = 0.f;
-#pragma unroll
-        for (int am = 0; am <= MMAX; ++am) {
-          if (so.absm[4 * i] != am && so.absm[4 * i + 1] != am && so.absm[4 * i + 2] != am &&
-              so.absm[4 * i + 3] != am)
-            continue;
-          er = fmaf(hot[i][am], cs[am], er);
-          if (am > 0) ei = fmaf(hot[i][am], sn[am], ei);
-        }
-        er *= scl;
-        ei *= scl;
+        const float er = cph[i] * scl, ei = sph[i] * scl;
         const float elo = xor_sign(ei, smlo[i]), ehi = xor_sign(ei, smhi[i]);
 #pragma unroll
         for (int v = 0; v < 4; ++v) {

@@ -672,10 +672,16 @@ def main_precision(args):
         t0 = time.time()
         x32, st32, ok32 = op.solve_master(t, rhs, args.tol, precision='mixed', with_stats=True)
         s32 = time.time() - t0
+        # FP16-operand inner solves (the first call captures the solver's graphs: timed on the second)
+        op.solve_master(t, rhs, args.tol, precision='mixed16')
+        t0 = time.time()
+        x16, st16, ok16 = op.solve_master(t, rhs, args.tol, precision='mixed16', with_stats=True)
+        s16 = time.time() - t0
         t0 = time.time()
         x64, st64, ok64 = op.solve_master(t, rhs, 1e-10, precision='double', with_stats=True)
         s64 = time.time() - t0
         coef = _errors(x32, x64)
+        coef16 = _errors(x16, x64)
         ext = []
         for r, w in enumerate(waves):
             e32 = ffm.extinction_cross_section(w, ffm._per_particle(plist, x32[r]), ls, precision='single')
@@ -690,15 +696,25 @@ def main_precision(args):
             fields[name] = np.sqrt(np.abs(ex) ** 2 + np.abs(ey) ** 2 + np.abs(ez) ** 2)
         e_err = float(np.max(np.abs(fields['fp32'] - fields['fp64'])) / np.max(fields['fp64']))
         e_rms = float(np.linalg.norm(fields['fp32'] - fields['fp64']) / np.linalg.norm(fields['fp64']))
-        log('l_max %d: mixed %.2f s (%d its), FP64 %.2f s; coef err %.2e, ext err %.2e, |E| err %.2e (rms %.2e)'
-            % (l_max, s32, st32['iterations'], s64, max(coef), max(ext), e_err, e_rms))
+        log('l_max %d: mixed %.2f s (%d its), mixed16 %.2f s (%d its), FP64 %.2f s; coef err %.2e (mixed16 '
+            '%.2e), ext err %.2e, |E| err %.2e (rms %.2e)'
+            % (l_max, s32, st32['iterations'], s16, st16['iterations'], s64, max(coef), max(coef16), max(ext), e_err,
+               e_rms))
         out['variants']['l_max_%d' % l_max] = {
             'unknowns': op.shape[0], 'mixed_seconds_per_8_rhs': s32, 'mixed_rhs_solves_per_s': nb / s32,
             'fp64_seconds_per_8_rhs_tol_1e-10': s64, 'mixed_stats': st32, 'fp64_stats': st64,
             'coef_err_vs_fp64_worst': max(coef), 'coef_err_vs_fp64_median': float(np.median(coef)),
             'extinction_rel_err_worst': max(ext), 'near_field_abs_E_max_err_rel_to_max': e_err,
             'near_field_abs_E_rms_rel_err': e_rms, 'autotune': tune,
-            'batch_kernel': 'batch_slots_kernel<3,3,4>' if l_max == 3 else 'batch_matvec_kernel<float,float,4,4,4>'}
+            'mixed16_seconds_per_8_rhs': s16, 'mixed16_rhs_solves_per_s': nb / s16, 'mixed16_stats': st16,
+            'mixed16_coef_err_vs_fp64_worst': max(coef16), 'mixed16_certified': bool(ok16),
+            'matvec_ms_8rhs': dict(zip(('fp16', 'fp32', 'fp64'),
+                                       (op.bench_matvec_batch(p_, iters=2)[0] for p_ in ('half', 'single', 'double')))),
+            'batch_kernels': ('batch_half_kernel<%d,%d,%d> (%s), batch_slots_kernel<%d,%d,4> (%s), '
+                              'batch_slots64_kernel<%d,%d,4> (%s)'
+                              % (l_max, l_max, op.half_interpolation()[0], '1 part' if l_max == 3 else '3 tile parts',
+                                 l_max, l_max, '1 K-part' if l_max == 3 else '3 K-parts', l_max, l_max,
+                                 '2 K-parts' if l_max == 3 else '6 K-parts'))}
         del op, base
     out['value'] = out['variants']['l_max_3']['coef_err_vs_fp64_worst']
     out['unit'] = 'relative coefficient error'

@@ -70,7 +70,17 @@ __device__ constexpr ChebTable kCheb{};
 
 // 16 waves per workgroup: the receiver accumulator takes the CU's LDS, so the
 // workgroup is the CU's only one, and 4 waves per SIMD hide the gather latency
+// (128 registers per lane).
 constexpr int kHalfThreads = 1024;
+// Parts of the result tiles (a divisor of the S / 16 tiles) such that one
+// part's FP16 coefficients, T terms x its Re and Im tiles x S / 16 chunks x 2
+// packed registers, take at most 64 registers per lane (0: none).
+__host__ __device__ constexpr int half_tile_parts(int T, int B) {
+  const int MH = (B + 15) / 16;
+  for (int p = 1; p <= MH; ++p)
+    if (MH % p == 0 && T * 2 * (MH / p) * MH * 2 <= 64) return p;
+  return 0;
+}
 
 // VOL: volume lookups. The table holds the tap rows in this kernel's
 // fragment order ([rows][MT][64 lanes][NC][4] FP16, one row per tap row of the
@@ -88,6 +98,11 @@ __global__ void __launch_bounds__(kHalfThreads) batch_half_kernel(RadialKArgs a,
   static_assert(S % 16 == 0, "slots per particle must be a multiple of 16");
   static_assert(MMAX <= kChebMax, "Chebyshev phase coefficients cover |m| <= kChebMax");
   static_assert(MH == NC, "result tile c and input chunk c share the lane's slot group");
+  // result tiles in TP parts of MHP (Re and Im rows of each), each a pass over
+  // an entry's pairs, so that one part's FP16 coefficients take at most 64
+  // registers per lane (one part at l_max <= 3)
+  constexpr int TP = half_tile_parts(T, SvwfIdx<LMAX, MMAX>::B), MHP = MH / TP;
+  static_assert(TP > 0 && MH % TP == 0, "tile parts");
   constexpr SlotOrder<LMAX, MMAX> so{};
   static_assert(so.ok, "slot order");
 
@@ -153,32 +168,34 @@ __global__ void __launch_bounds__(kHalfThreads) batch_half_kernel(RadialKArgs a,
   // first / second operand plane of the lane: (re, im) for the re column, (im, re) for the im column
   const int64_t xlane = (int64_t)(q * KB + rhs) * NC * 8 + 4 * zp, xother = 4 - 8 * zp;
 
-  typedef half2_t Coef[T][MT][NC][2];
-  // polynomial coefficients of a bin: tv[t][i][c] = 4 FP16 values (2 packed pairs)
-  // (VOL: the values of the entry's tap rows rows[t])
-  auto load_coef = [&](int bin, const int* rows, Coef& tv) {
+  typedef half2_t Coef[T][2 * MHP][NC][2];
+  // polynomial coefficients of a bin for the tiles of part h: tv[t][ti][c] = 4
+  // FP16 values (2 packed pairs) of tile h MHP + ti (ti < MHP, Re L) or
+  // MH + h MHP + ti - MHP (Im L) (VOL: the values of the entry's tap rows rows[t])
+  auto load_coef = [&](int bin, const int* rows, Coef& tv, int h) {
 #pragma unroll
     for (int t = 0; t < T; ++t)
 #pragma unroll
-      for (int i = 0; i < MT; ++i) {
+      for (int ti = 0; ti < 2 * MHP; ++ti) {
+        const int i = ti < MHP ? h * MHP + ti : MH + h * MHP + ti - MHP;
         const size_t trow = VOL ? (size_t)rows[t] : (size_t)bin * T + t;
         if constexpr (NC % 2 == 0) {
           const uint4* row = tth + (trow * MT + i) * 64 * (NC / 2);
 #pragma unroll
           for (int c2 = 0; c2 < NC / 2; ++c2) {
             const uint4 v = row[lane * (NC / 2) + c2];
-            tv[t][i][2 * c2][0] = __builtin_bit_cast(half2_t, v.x);
-            tv[t][i][2 * c2][1] = __builtin_bit_cast(half2_t, v.y);
-            tv[t][i][2 * c2 + 1][0] = __builtin_bit_cast(half2_t, v.z);
-            tv[t][i][2 * c2 + 1][1] = __builtin_bit_cast(half2_t, v.w);
+            tv[t][ti][2 * c2][0] = __builtin_bit_cast(half2_t, v.x);
+            tv[t][ti][2 * c2][1] = __builtin_bit_cast(half2_t, v.y);
+            tv[t][ti][2 * c2 + 1][0] = __builtin_bit_cast(half2_t, v.z);
+            tv[t][ti][2 * c2 + 1][1] = __builtin_bit_cast(half2_t, v.w);
           }
         } else {
           const uint2* row2 = reinterpret_cast<const uint2*>(tth) + (trow * MT + i) * 64 * NC;
 #pragma unroll
           for (int c = 0; c < NC; ++c) {
             const uint2 v = row2[lane * NC + c];
-            tv[t][i][c][0] = __builtin_bit_cast(half2_t, v.x);
-            tv[t][i][c][1] = __builtin_bit_cast(half2_t, v.y);
+            tv[t][ti][c][0] = __builtin_bit_cast(half2_t, v.x);
+            tv[t][ti][c][1] = __builtin_bit_cast(half2_t, v.y);
           }
         }
       }
@@ -200,7 +217,7 @@ __global__ void __launch_bounds__(kHalfThreads) batch_half_kernel(RadialKArgs a,
   uint2 xc[2 * NC];  // input of the next pair to run
   // the pairs of one entry (one bin) with its coefficients tv; rec_n is its
   // first pair, whose input is in xc
-  auto run_entry = [&](const int4& en, const Coef& tv, Rec rec_n, const int* rows) {
+  auto run_entry = [&](const int4& en, const Coef& tv, Rec rec_n, const int* rows, int h) {
     float bscale, rel[T];
     if constexpr (VOL) {
       float hmax = 0.f;
@@ -265,10 +282,10 @@ __global__ void __launch_bounds__(kHalfThreads) batch_half_kernel(RadialKArgs a,
           wt[t] = half2_t{h, h};
         }
       }
-      typename Mfma<float>::v4 acc[MH];
+      typename Mfma<float>::v4 acc[MHP];
 #pragma unroll
-      for (int i = 0; i < MH; ++i) acc[i] = typename Mfma<float>::v4{0, 0, 0, 0};
-      // interpolated coefficients of tile i, chunk c (Horner in packed FP16; VOL: the weighted taps)
+      for (int ti = 0; ti < MHP; ++ti) acc[ti] = typename Mfma<float>::v4{0, 0, 0, 0};
+      // interpolated coefficients of part tile i, chunk c (Horner in packed FP16; VOL: the weighted taps)
       auto coef = [&](int i, int c) {
         half2_t v0, v1;
         if constexpr (VOL) {
@@ -298,9 +315,9 @@ __global__ void __launch_bounds__(kHalfThreads) batch_half_kernel(RadialKArgs a,
       for (int c = 0; c < NC; ++c) {
         const half8_t bv = __builtin_shufflevector(zb[c], zb2[c], 0, 1, 2, 3, 4, 5, 6, 7);
 #pragma unroll
-        for (int i = 0; i < MH; ++i) {
-          const half8_t av = __builtin_shufflevector(coef(i, c), coef(i + MH, c), 0, 1, 2, 3, 4, 5, 6, 7);
-          acc[i] = __builtin_amdgcn_mfma_f32_16x16x32_f16(av, bv, acc[i], 0, 0, 0);
+        for (int ti = 0; ti < MHP; ++ti) {
+          const half8_t av = __builtin_shufflevector(coef(ti, c), coef(ti + MHP, c), 0, 1, 2, 3, 4, 5, 6, 7);
+          acc[ti] = __builtin_amdgcn_mfma_f32_16x16x32_f16(av, bv, acc[ti], 0, 0, 0);
         }
       }
 
@@ -309,7 +326,8 @@ __global__ void __launch_bounds__(kHalfThreads) batch_half_kernel(RadialKArgs a,
       unsigned long long* yrow = yacc + (size_t)il * LO + colbase;
       const float scl = sh * bscale;
 #pragma unroll
-      for (int i = 0; i < MH; ++i) {
+      for (int ti = 0; ti < MHP; ++ti) {
+        const int i = h * MHP + ti;
         const float er = cph[i] * scl, ei = sph[i] * scl;
         const float elo = xor_sign(ei, smlo[i]), ehi = xor_sign(ei, smhi[i]);
 #pragma unroll
@@ -317,7 +335,7 @@ __global__ void __launch_bounds__(kHalfThreads) batch_half_kernel(RadialKArgs a,
           const float eps = v < 2 ? elo : ehi;
           // y' = e^{-i m1 phi} y: er y_own + eps y_other, y_other from lane ^ 1
           // (the exchange folds into the multiply: v_mul_f32_dpp)
-          const float val = fmaf(er, acc[i][v], eps * lane_xor1(acc[i][v]));
+          const float val = fmaf(er, acc[ti][v], eps * lane_xor1(acc[ti][v]));
           atomicAdd(yrow + 16 * i + v, fixed_bits64(val));
         }
       }
@@ -332,10 +350,13 @@ __global__ void __launch_bounds__(kHalfThreads) batch_half_kernel(RadialKArgs a,
     int rows[T];
 #pragma unroll
     for (int t = 0; t < T; ++t) rows[t] = VOL ? en.x + vol_tap_offset(a.vmode, a.vslice, t, a.len_z) : 0;
-    load_coef(bin_of(en), rows, tv);
-    const Rec rec0 = recs[en.y];
-    load_x(rec0, xc);
-    run_entry(en, tv, rec0, rows);
+#pragma unroll
+    for (int h = 0; h < TP; ++h) {
+      load_coef(bin_of(en), rows, tv, h);
+      const Rec rec0 = recs[en.y];
+      load_x(rec0, xc);
+      run_entry(en, tv, rec0, rows, h);
+    }
   }
   __syncthreads();
   float* yp = reinterpret_cast<float*>(a.ypart) + (size_t)item * a.Rc * 2 * KB * S;

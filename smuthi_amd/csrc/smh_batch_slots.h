@@ -99,8 +99,12 @@ void slot_tables(int* out) {
   for (int k = 0; k < so.S; ++k) out[so.B + k] = so.n_of[k];
 }
 
-// Shape of the slot kernel (FP32, 512 threads, one tap fragment set of at
-// most 128 registers per lane).
+// Shape of the slot kernels. fits: the FP32 kernel's tap fragments of all
+// K-steps fit one set of at most 128 registers per lane. Larger blocks split
+// the K-steps into parts, each a pass over an entry's pairs with its own
+// epilogue: P32 parts of whole float4 groups (4 K-steps) within 128 FP32
+// registers, P64 parts of an even number of K-steps within 64 FP64 values
+// (0: no such split).
 template <int L, int M, int T>
 struct SlotShape {
   static constexpr int S = SvwfIdx<L, M>::Bh, NS = S / 4, MT = 2 * S / 16, MH = MT / 2;
@@ -109,7 +113,33 @@ struct SlotShape {
   static constexpr int SY = 2 * LO;         // LDS int32 words per receiver
   static constexpr int FRB = MT * NS * 64;  // floats per tap row
   static constexpr bool fits = T * MT * NS <= 128;
+  static constexpr int parts(int budget, int step) {
+    for (int p = 1; p <= NS; ++p)
+      if (NS % p == 0 && (NS / p) % step == 0 && T * MT * (NS / p) <= budget) return p;
+    return 0;
+  }
+  static constexpr int P32 = parts(128, 4), P64 = parts(64, 2);
 };
+
+// cos(k phi), sin(k phi) for k <= K from cos phi, sin phi (angle addition)
+template <int K>
+__device__ inline void phase_powers(float c1, float s1, float (&cs)[K + 1], float (&sn)[K + 1]) {
+  cs[0] = 1.f;
+  sn[0] = 0.f;
+  if constexpr (K >= 1) {
+    cs[1] = c1;
+    sn[1] = s1;
+  }
+  if constexpr (K >= 2) {
+    cs[2] = c1 * c1 - s1 * s1;
+    sn[2] = 2.f * c1 * s1;
+  }
+#pragma unroll
+  for (int k = 2; k < K; ++k) {
+    cs[k + 1] = cs[k] * c1 - sn[k] * s1;
+    sn[k + 1] = sn[k] * c1 + cs[k] * s1;
+  }
+}
 
 // Receiver sums of the slot kernels: one int64 fixed-point word per result.
 // A value v of a work item (|v| <= V = 8 lsum xmax) is added as
@@ -151,8 +181,10 @@ __global__ void __launch_bounds__(512) batch_slots_kernel(RadialKArgs a) {
   constexpr int S = Sh::S, NS = Sh::NS, MT = Sh::MT, MH = Sh::MH, SC = Sh::SC, LO = Sh::LO;
   constexpr int KB = kBatchRhs;
   static_assert(KB * 2 == 16, "16 result columns = kBatch right-hand sides x (re, im)");
-  static_assert(Sh::fits, "tap fragments exceed the register budget");
-  static_assert(MMAX <= 3, "phase selection covers |m| <= 3");
+  // K-steps in P parts (one register set of tap fragments per part)
+  constexpr int P = Sh::P32, NSP = NS / (P > 0 ? P : 1);
+  static_assert(P > 0, "tap fragments exceed the register budget");
+  static_assert(MMAX <= 4, "phase selection covers |m| <= 4");
   constexpr SlotOrder<LMAX, MMAX> so{};
   static_assert(so.ok, "slot order");
 
@@ -164,7 +196,9 @@ __global__ void __launch_bounds__(512) batch_slots_kernel(RadialKArgs a) {
   const int item = a.order ? a.order[blockIdx.x] : (int)blockIdx.x;
   float sh = 0.f;
   double ish = 0.0;
-  fixed_scale64(a.item_lsum[item], *a.xmax, a.item_en[item], sh, ish);
+  // each pair adds once per part to each word of its receiver
+  constexpr int PB = P <= 1 ? 0 : (P <= 2 ? 1 : (P <= 4 ? 2 : (P <= 8 ? 3 : 4)));  // ceil(log2 P)
+  fixed_scale64(a.item_lsum[item], *a.xmax, a.item_en[item] + PB, sh, ish);
 
   const int tid = threadIdx.x, lane = tid & 63;
   const int2 it = a.items[item];
@@ -191,7 +225,7 @@ __global__ void __launch_bounds__(512) batch_slots_kernel(RadialKArgs a) {
   // result rows of tile i are slot group 4 i + q: its |m| and the sign masks of rows v < 2 / v >= 2
   // (as one-hot weights hot[i][a] = [|m| == a], so the per-pair phase of a lane is
   // a short FMA chain over the |m| values that occur in the tile, no branches)
-  float hot[MH][4];
+  float hot[MH][MMAX + 1];
   unsigned smlo[MH], smhi[MH];
 #pragma unroll
   for (int i = 0; i < MH; ++i) {
@@ -199,7 +233,7 @@ __global__ void __launch_bounds__(512) batch_slots_kernel(RadialKArgs a) {
     const int ra = q == 0 ? so.absm[g0] : (q == 1 ? so.absm[g0 + 1] : (q == 2 ? so.absm[g0 + 2] : so.absm[g0 + 3]));
     const int sg = q == 0 ? so.sgn[g0] : (q == 1 ? so.sgn[g0 + 1] : (q == 2 ? so.sgn[g0 + 2] : so.sgn[g0 + 3]));
 #pragma unroll
-    for (int am = 0; am < 4; ++am) hot[i][am] = ra == am ? 1.f : 0.f;
+    for (int am = 0; am <= MMAX; ++am) hot[i][am] = ra == am ? 1.f : 0.f;
     smlo[i] = sg < 0 ? 0x80000000u : 0u;
     smhi[i] = sg > 0 ? 0u : 0x80000000u;
   }
@@ -213,118 +247,123 @@ __global__ void __launch_bounds__(512) batch_slots_kernel(RadialKArgs a) {
     if (e >= e_end) break;
     const int4 en = a.entries[e];
     const int bin = (en.x - a.row_o) / a.row_s;
-    // polynomial coefficients tv[t][i][s] = c_t(tile i row rr, slot 4 s + q) of the entry's bin
-    // (VOL: the values of the entry's tap t)
-    float tv[T][MT][NS];
-#pragma unroll
-    for (int t = 0; t < T; ++t) {
-      const size_t trow = VOL ? (size_t)(en.x + vol_tap_offset(a.vmode, a.vslice, t, a.len_z)) : (size_t)bin * T + t;
-      const float4* row = tt4 + trow * (Sh::FRB / 4);
-#pragma unroll
-      for (int i = 0; i < MT; ++i)
-#pragma unroll
-        for (int s4 = 0; s4 < NS / 4; ++s4) {
-          const float4 v = row[(i * (NS / 4) + s4) * 64 + lane];
-          tv[t][i][4 * s4 + 0] = v.x;
-          tv[t][i][4 * s4 + 1] = v.y;
-          tv[t][i][4 * s4 + 2] = v.z;
-          tv[t][i][4 * s4 + 3] = v.w;
-        }
-    }
     const int p_end = en.z;
-    auto load_x = [&](const Rec& r, float4 (&xs)[NS / 2]) {
-      const float4* xp = reinterpret_cast<const float4*>(xg + (int64_t)(r.ij & kPairMask) * S * KB + xlane);
 #pragma unroll
-      for (int k = 0; k < NS / 2; ++k) xs[k] = xp[k];
-    };
-    Rec rec_n = recs[en.y];
-    float4 xc[NS / 2];
-    load_x(rec_n, xc);
-    for (int p = 0; p < p_end; ++p) {
-      const Rec rec = rec_n;
-      rec_n = recs[en.y + (p + 1 < p_end ? p + 1 : p)];
-
-      const float w = rec.w;
-      float wt[T];  // VOL: the pair's tap weights
-      if constexpr (VOL) {
+    for (int h = 0; h < P; ++h) {
+      // K-steps s0 .. s0 + NSP - 1 of this part
+      const int s0 = h * NSP;
+      // polynomial coefficients tv[t][i][s] = c_t(tile i row rr, slot 4 (s0 + s) + q) of the entry's bin
+      // (VOL: the values of the entry's tap t)
+      float tv[T][MT][NSP];
 #pragma unroll
-        for (int t = 0; t < T; ++t) wt[t] = vol_tap_weight<float>(a.vmode, a.vslice, t, rec.w, rec.w2);
-      }
-      // cos / sin of k phi, k = 1..3
-      const float c1 = rec.c, s1 = rec.s;
-      const float c2 = c1 * c1 - s1 * s1, s2 = 2.f * c1 * s1;
-      const float c3 = c2 * c1 - s2 * s1, s3 = s2 * c1 + c2 * s1;
-      const float cs[4] = {1.f, c1, c2, c3};
-      const float sn[4] = {0.f, s1, s2, s3};
-      float ez[4] = {0.f, 0.f, 0.f, 0.f};  // sin(k phi) with the re-column sign
+      for (int t = 0; t < T; ++t) {
+        const size_t trow =
+            VOL ? (size_t)(en.x + vol_tap_offset(a.vmode, a.vslice, t, a.len_z)) : (size_t)bin * T + t;
+        const float4* row = tt4 + trow * (Sh::FRB / 4);
 #pragma unroll
-      for (int k = 1; k <= MMAX; ++k) ez[k] = xor_sign(sn[k], zmask);
-
-      // B operand per K-step: (e^{i m2 phi} x[slot 4 s + q][rhs]).re or .im
-      float z[NS];
+        for (int i = 0; i < MT; ++i)
 #pragma unroll
-      for (int s = 0; s < NS; ++s) {
-        const float4 x4 = xc[s >> 1];
-        const float xr = (s & 1) ? x4.z : x4.x, xi = (s & 1) ? x4.w : x4.y;
-        const float av0 = zp ? xi : xr, bv0 = zp ? xr : xi;
-        const int am = so.absm[s];
-        if (am == 0) {
-          z[s] = av0;
-        } else {
-          const int sg = so.sgn[s];
-          const float e = sg > 0 ? ez[am] : (sg < 0 ? -ez[am] : xor_sign(ez[am], qmask));
-          z[s] = fmaf(cs[am], av0, e * bv0);
-        }
-      }
-      // the input of the next pair goes into the same registers
-      load_x(rec_n, xc);
-
-      typename Mfma<float>::v4 acc[MT];
-#pragma unroll
-      for (int i = 0; i < MT; ++i) acc[i] = typename Mfma<float>::v4{0, 0, 0, 0};
-#pragma unroll
-      for (int s = 0; s < NS; ++s) {
-#pragma unroll
-        for (int i = 0; i < MT; ++i) {
-          float v;
-          if constexpr (VOL) {
-            v = wt[0] * tv[0][i][s];
-#pragma unroll
-            for (int t = 1; t < T; ++t) v = fmaf(wt[t], tv[t][i][s], v);
-          } else {
-            v = tv[T - 1][i][s];
-#pragma unroll
-            for (int t = T - 2; t >= 0; --t) v = fmaf(v, w, tv[t][i][s]);
+          for (int s4 = 0; s4 < NSP / 4; ++s4) {
+            const float4 v = row[(i * (NS / 4) + s0 / 4 + s4) * 64 + lane];
+            tv[t][i][4 * s4 + 0] = v.x;
+            tv[t][i][4 * s4 + 1] = v.y;
+            tv[t][i][4 * s4 + 2] = v.z;
+            tv[t][i][4 * s4 + 3] = v.w;
           }
-          acc[i] = Mfma<float>::op(v, z[s], acc[i]);
-        }
       }
+      auto load_x = [&](const Rec& r, float4 (&xs)[NSP / 2]) {
+        const float4* xp =
+            reinterpret_cast<const float4*>(xg + (int64_t)(r.ij & kPairMask) * S * KB + xlane) + s0 / 2;
+#pragma unroll
+        for (int k = 0; k < NSP / 2; ++k) xs[k] = xp[k];
+      };
+      Rec rec_n = recs[en.y];
+      float4 xc[NSP / 2];
+      load_x(rec_n, xc);
+      for (int p = 0; p < p_end; ++p) {
+        const Rec rec = rec_n;
+        rec_n = recs[en.y + (p + 1 < p_end ? p + 1 : p)];
 
-      // y (rhs, re|im) of rows 16 i + 4 q + v, rotated by e^{-i m1 phi}, scaled by S_hi
-      unsigned long long* yrow = yacc + (size_t)(rec.ij >> kPairShift) * LO + colbase;
+        const float w = rec.w;
+        float wt[T];  // VOL: the pair's tap weights
+        if constexpr (VOL) {
 #pragma unroll
-      for (int i = 0; i < MH; ++i) {
-        float er = 0.f, ei = 0.f;
-#pragma unroll
-        for (int am = 0; am <= MMAX; ++am) {
-          // |m| values outside this tile's four row groups drop out at compile time
-          if (so.absm[4 * i] != am && so.absm[4 * i + 1] != am && so.absm[4 * i + 2] != am &&
-              so.absm[4 * i + 3] != am)
-            continue;
-          er = fmaf(hot[i][am], cs[am], er);
-          if (am > 0) ei = fmaf(hot[i][am], sn[am], ei);
+          for (int t = 0; t < T; ++t) wt[t] = vol_tap_weight<float>(a.vmode, a.vslice, t, rec.w, rec.w2);
         }
-        er *= sh;
-        ei *= sh;
-        const float elo = xor_sign(ei, smlo[i]), ehi = xor_sign(ei, smhi[i]);
+        // cos / sin of k phi, k = 1..MMAX
+        float cs[MMAX + 1], sn[MMAX + 1];
+        phase_powers<MMAX>(rec.c, rec.s, cs, sn);
+        float ez[MMAX + 1];  // sin(k phi) with the re-column sign
+        ez[0] = 0.f;
 #pragma unroll
-        for (int v = 0; v < 4; ++v) {
-          const float eps = v < 2 ? elo : ehi;
-          const float al = acc[i][v], be = acc[i + MH][v];
-          const float own = fmaf(al, er, be * eps);
-          const float snd = fmaf(be, er, -(al * eps));
-          const float val = fma_xor1(own, sgnz, snd);
-          atomicAdd(yrow + 16 * i + v, fixed_bits64(val));
+        for (int k = 1; k <= MMAX; ++k) ez[k] = xor_sign(sn[k], zmask);
+
+        // B operand per K-step: (e^{i m2 phi} x[slot 4 s + q][rhs]).re or .im
+        float z[NSP];
+#pragma unroll
+        for (int s = 0; s < NSP; ++s) {
+          const float4 x4 = xc[s >> 1];
+          const float xr = (s & 1) ? x4.z : x4.x, xi = (s & 1) ? x4.w : x4.y;
+          const float av0 = zp ? xi : xr, bv0 = zp ? xr : xi;
+          const int am = so.absm[s0 + s];
+          if (am == 0) {
+            z[s] = av0;
+          } else {
+            const int sg = so.sgn[s0 + s];
+            const float e = sg > 0 ? ez[am] : (sg < 0 ? -ez[am] : xor_sign(ez[am], qmask));
+            z[s] = fmaf(cs[am], av0, e * bv0);
+          }
+        }
+        // the input of the next pair goes into the same registers
+        load_x(rec_n, xc);
+
+        typename Mfma<float>::v4 acc[MT];
+#pragma unroll
+        for (int i = 0; i < MT; ++i) acc[i] = typename Mfma<float>::v4{0, 0, 0, 0};
+#pragma unroll
+        for (int s = 0; s < NSP; ++s) {
+#pragma unroll
+          for (int i = 0; i < MT; ++i) {
+            float v;
+            if constexpr (VOL) {
+              v = wt[0] * tv[0][i][s];
+#pragma unroll
+              for (int t = 1; t < T; ++t) v = fmaf(wt[t], tv[t][i][s], v);
+            } else {
+              v = tv[T - 1][i][s];
+#pragma unroll
+              for (int t = T - 2; t >= 0; --t) v = fmaf(v, w, tv[t][i][s]);
+            }
+            acc[i] = Mfma<float>::op(v, z[s], acc[i]);
+          }
+        }
+
+        // y (rhs, re|im) of rows 16 i + 4 q + v, rotated by e^{-i m1 phi}, scaled by S_hi
+        unsigned long long* yrow = yacc + (size_t)(rec.ij >> kPairShift) * LO + colbase;
+#pragma unroll
+        for (int i = 0; i < MH; ++i) {
+          float er = 0.f, ei = 0.f;
+#pragma unroll
+          for (int am = 0; am <= MMAX; ++am) {
+            // |m| values outside this tile's four row groups drop out at compile time
+            if (so.absm[4 * i] != am && so.absm[4 * i + 1] != am && so.absm[4 * i + 2] != am &&
+                so.absm[4 * i + 3] != am)
+              continue;
+            er = fmaf(hot[i][am], cs[am], er);
+            if (am > 0) ei = fmaf(hot[i][am], sn[am], ei);
+          }
+          er *= sh;
+          ei *= sh;
+          const float elo = xor_sign(ei, smlo[i]), ehi = xor_sign(ei, smhi[i]);
+#pragma unroll
+          for (int v = 0; v < 4; ++v) {
+            const float eps = v < 2 ? elo : ehi;
+            const float al = acc[i][v], be = acc[i + MH][v];
+            const float own = fmaf(al, er, be * eps);
+            const float snd = fmaf(be, er, -(al * eps));
+            const float val = fma_xor1(own, sgnz, snd);
+            atomicAdd(yrow + 16 * i + v, fixed_bits64(val));
+          }
         }
       }
     }
@@ -336,7 +375,7 @@ __global__ void __launch_bounds__(512) batch_slots_kernel(RadialKArgs a) {
     const int il = j / (2 * KB * S), k = j - il * 2 * KB * S;
     const int c = k / S, sl2 = k - c * S;
     const int w0 = il * LO + c * SC + sl2;
-    const long long sum = (long long)(yacc[w0] - (unsigned long long)n_adds[il] * kMagic64Bits);  // modulo 2^64
+    const long long sum = (long long)(yacc[w0] - (unsigned long long)n_adds[il] * P * kMagic64Bits);  // modulo 2^64
     yp[j] = (float)((double)sum * ish);
   }
 }

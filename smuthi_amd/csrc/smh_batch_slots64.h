@@ -18,9 +18,10 @@
 //   * the re/im columns of a right-hand side are combined with one DPP
 //     exchange (lane ^ 1) and each value is added to the receiver's LDS
 //     accumulator with one FP64 LDS atomic;
-//   * the K-steps run in two parts, each a pass over the entry's pairs with
-//     its own epilogue, so one part's coefficients (128 registers) leave room
-//     for two waves per SIMD.
+//   * the K-steps run in SlotShape::P64 parts (two at l_max = m_max = 3, six
+//     at 4), each a pass over the entry's pairs with its own epilogue, so one
+//     part's coefficients (at most 128 registers) leave room for two waves
+//     per SIMD.
 //
 // Layouts (S = slots per particle, NS = S / 4 K-steps):
 //   coefficients [bins][T][MT][NS/2][64 lanes][2] FP64: lane (q, rr) of tile i
@@ -47,9 +48,29 @@ __device__ inline double lane_xor1_64(double v) {
   return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
 }
 
-// two parts of the K-steps per pair, two waves per SIMD
-constexpr int kSlots64Parts = 2;
+// two waves per SIMD; the K-steps in SlotShape::P64 parts per pair (two at
+// l_max = m_max = 3)
 constexpr int kSlots64Threads = 512;
+
+// cos(k phi), sin(k phi) for k <= K (angle addition)
+template <int K>
+__device__ inline void phase_powers64(double c1, double s1, double (&cs)[K + 1], double (&sn)[K + 1]) {
+  cs[0] = 1.0;
+  sn[0] = 0.0;
+  if constexpr (K >= 1) {
+    cs[1] = c1;
+    sn[1] = s1;
+  }
+  if constexpr (K >= 2) {
+    cs[2] = c1 * c1 - s1 * s1;
+    sn[2] = 2.0 * c1 * s1;
+  }
+#pragma unroll
+  for (int k = 2; k < K; ++k) {
+    cs[k + 1] = cs[k] * c1 - sn[k] * s1;
+    sn[k + 1] = sn[k] * c1 + cs[k] * s1;
+  }
+}
 
 // VOL: volume lookups, as batch_slots_kernel's VOL form: the table holds FP64
 // tap rows in this kernel's fragment order ([rows][MT][NS/2][64 lanes][2]),
@@ -61,10 +82,10 @@ __global__ void __launch_bounds__(kSlots64Threads) batch_slots64_kernel(RadialKA
   constexpr int S = Sh::S, NS = Sh::NS, MT = Sh::MT, MH = Sh::MH, SC = Sh::SC, LO = Sh::LO;
   constexpr int KB = kBatchRhs;
   static_assert(KB * 2 == 16, "16 result columns = kBatch right-hand sides x (re, im)");
-  static_assert(Sh::fits, "coefficients exceed the register budget");
-  constexpr int NSH = NS / kSlots64Parts;  // K-steps per part
-  static_assert(NS % (2 * kSlots64Parts) == 0, "coefficient pairs per lane and part");
-  static_assert(MMAX <= 3, "phase selection covers |m| <= 3");
+    constexpr int NP = Sh::P64;  // parts of the K-steps
+  static_assert(NP > 0, "coefficients exceed the register budget");
+  constexpr int NSH = NS / NP;  // K-steps per part (even: coefficient pairs per lane)
+  static_assert(MMAX <= 4, "phase selection covers |m| <= 4");
   constexpr SlotOrder<LMAX, MMAX> so{};
   static_assert(so.ok, "slot order");
   using V4 = typename Mfma<double>::v4;
@@ -104,11 +125,11 @@ __global__ void __launch_bounds__(kSlots64Threads) batch_slots64_kernel(RadialKA
     const int4 en = a.entries[e];
     const int bin = (en.x - a.row_o) / a.row_s;
     const int p_end = en.z;
-    // the K-steps in kSlots64Parts parts of NSH, each with its own pass over
-    // the entry's pairs and its own epilogue (coefficients of one part in
+    // the K-steps in NP parts of NSH, each with its own pass over the
+    // entry's pairs and its own epilogue (coefficients of one part in
     // registers: two waves per SIMD)
 #pragma unroll
-    for (int h = 0; h < kSlots64Parts; ++h) {
+    for (int h = 0; h < NP; ++h) {
       double tv[T][MT][NSH];
 #pragma unroll
       for (int t = 0; t < T; ++t)
@@ -145,11 +166,8 @@ __global__ void __launch_bounds__(kSlots64Threads) batch_slots64_kernel(RadialKA
 #pragma unroll
           for (int t = 0; t < T; ++t) wt[t] = vol_tap_weight<double>(a.vmode, a.vslice, t, rec.w, rec.w2);
         }
-        const double c1 = rec.c, s1 = rec.s;
-        const double c2 = c1 * c1 - s1 * s1, s2 = 2.0 * c1 * s1;
-        const double c3 = c2 * c1 - s2 * s1, s3 = s2 * c1 + c2 * s1;
-        const double cs[4] = {1.0, c1, c2, c3};
-        const double sn[4] = {0.0, s1, s2, s3};
+        double cs[MMAX + 1], sn[MMAX + 1];
+        phase_powers64<MMAX>(rec.c, rec.s, cs, sn);
 
         // B operand per K-step: (e^{i m2 phi} x[slot 4 s + q][rhs]).re or .im
         double z[NSH];

@@ -382,19 +382,19 @@ KernelInfo pick(int l, int m) {
   return KernelInfo{};
 }
 
-// Operators whose coefficient fragments fit one register set use the
-// slot-order kernels (FP32 and FP64); the others the plain kBatch-RHS kernel.
+// Operators whose coefficient fragments fit the registers in whole parts of
+// the K-steps use the slot-order kernels (FP32 and FP64); the others the
+// plain kBatch-RHS kernel.
 template <int L, int Mx, int T, typename Real>
 KernelInfo batch_kernel_info() {
   KernelInfo k;
-  if constexpr (std::is_same<Real, float>::value && SlotShape<L, Mx, T>::fits && Mx <= 3) {
+  if constexpr (std::is_same<Real, float>::value && SlotShape<L, Mx, T>::P32 > 0 && Mx <= 4) {
     k.fn = reinterpret_cast<void*>(&batch_slots_kernel<L, Mx, T>);
     k.threads = 512;
     k.sy = SlotShape<L, Mx, T>::SY;  // int32 words (one int64 word per result)
     k.slots = SlotShape<L, Mx, T>::S;
     k.slot_tables = &slot_tables<L, Mx>;
-  } else if constexpr (std::is_same<Real, double>::value && SlotShape<L, Mx, T>::fits && Mx <= 3 &&
-                       SlotShape<L, Mx, T>::NS % (2 * kSlots64Parts) == 0) {
+  } else if constexpr (std::is_same<Real, double>::value && SlotShape<L, Mx, T>::P64 > 0 && Mx <= 4) {
     k.fn = reinterpret_cast<void*>(&batch_slots64_kernel<L, Mx, T>);
     k.threads = kSlots64Threads;
     k.sy = SlotShape<L, Mx, T>::LO;  // doubles
@@ -428,9 +428,10 @@ KernelInfo pick_batch(int l, int m) {
 }
 
 // FP16-operand kBatch-RHS kernel: where the FP32 slot-order kernel is used
+// (its binning and slot order)
 template <int L, int Mx, int T>
 void* half_fn() {
-  if constexpr (SlotShape<L, Mx, T>::fits && Mx <= 3)
+  if constexpr (SlotShape<L, Mx, 4>::P32 > 0 && Mx <= 4 && half_tile_parts(T, SvwfIdx<L, Mx>::B) > 0)
     return reinterpret_cast<void*>(&batch_half_kernel<L, Mx, T>);
   else
     return nullptr;
@@ -859,7 +860,8 @@ int op_apply_batch(smh_op* op, int precision, const void* x_pad, void* y_pad, bo
     const float* hs = op->d_hscale;
     void* args1[] = {&a};
     void* args2[] = {&a, &hs};
-    SMH_HIP(ctx, hipLaunchKernel(fn, dim3(bn.n_items), dim3(half ? kHalfThreads : k.threads), half ? args2 : args1,
+    const int threads = half ? kHalfThreads : k.threads;
+    SMH_HIP(ctx, hipLaunchKernel(fn, dim3(bn.n_items), dim3(threads), half ? args2 : args1,
                                  bn.lds, s));
     if (time_kernel) SMH_HIP(ctx, hipEventRecord(ctx->ev1, s));
   }

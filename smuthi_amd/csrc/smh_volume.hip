@@ -563,7 +563,7 @@ KernelInfo vol_batch32_kernel_info() {
 // FP64: the slot-order kernel where its coefficient parts fit the registers
 template <int L, int Mx>
 KernelInfo vol_batch64_kernel_info(bool tap64) {
-  if constexpr (SlotShape<L, Mx, 4>::fits && Mx <= 3 && SlotShape<L, Mx, 4>::NS % (2 * kSlots64Parts) == 0) {
+  if constexpr (SlotShape<L, Mx, 4>::fits && Mx <= 3 && SlotShape<L, Mx, 4>::P64 > 0) {
     KernelInfo k;
     k.fn = reinterpret_cast<void*>(&batch_slots64_kernel<L, Mx, 4, true>);
     k.threads = kSlots64Threads;
@@ -916,7 +916,8 @@ int vol_apply_batch(smh_op* op, int precision, const void* x_pad, void* y_pad, b
         const float* hs = op->d_vhs[t];
         void* args1[] = {&a};
         void* args2[] = {&a, &hs};
-        SMH_HIP(ctx, hipLaunchKernel(fn, dim3(bn.n_items), dim3(half ? kHalfThreads : k.threads), half ? args2 : args1,
+        const int threads = half ? kHalfThreads : k.threads;
+        SMH_HIP(ctx, hipLaunchKernel(fn, dim3(bn.n_items), dim3(threads), half ? args2 : args1,
                                      bn.lds, s));
         if (time_kernel) {
           SMH_HIP(ctx, hipEventRecord(ctx->ev1, s));

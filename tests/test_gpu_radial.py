@@ -64,10 +64,11 @@ def test_gpu_table_matches_oracle(gpu):
     assert (err <= 2e-7 * np.abs(want) + 1e-12 * scale).all(), err.max() / scale
 
 
-@pytest.mark.parametrize('order', [(1, 1), (2, 1), (2, 2), (3, 1), (3, 2), (4, 1), (4, 4)])
-def test_gpu_batched_matvec_orders(gpu, order):
-    # every multipole cut-off of the 8-RHS kernels: the slot-order FP32 kernel (blocks up to 32 rows)
-    # and the plain one (l_max = m_max = 4), 9 right-hand sides = one full and one padded block
+@pytest.mark.parametrize('order,res', [((1, 1), 5.0), ((2, 1), 5.0), ((2, 2), 5.0), ((3, 1), 5.0), ((3, 2), 5.0),
+                                       ((4, 1), 5.0), ((4, 2), 2.0), ((4, 3), 2.0), ((4, 4), 2.0), ((4, 4), 5.0)])
+def test_gpu_batched_matvec_orders(gpu, order, res):
+    # every multipole cut-off of the 8-RHS slot-order kernels (l_max = 4 with m_max >= 2: 48-slot blocks,
+    # the FP32 kernel in 3 K-parts, the FP64 kernel in 6), 9 right-hand sides = one full and one padded block
     rng = np.random.default_rng(31)
     pts = []
     while len(pts) < 16:
@@ -76,20 +77,20 @@ def test_gpu_batched_matvec_orders(gpu, order):
             pts.append(p)
     plist = [OSphere([p[0], p[1], 200.0], 1.5 + 0.01j, 100, *order) for p in pts]
     kp = ob.reasonable_Sommerfeld_kpar_contour(WL, layer_refractive_indices=[1.52, 1], neff_resolution=1e-2)
-    op = _product_op(plist, ([0, 0], [1.52, 1]), kp, 5.0, 'cubic')
+    op = _product_op(plist, ([0, 0], [1.52, 1]), kp, res, 'cubic')
     ref_op = oc.RadialLookupCPU(op.lookup_table, op.radial_distance_array, plist, 'cubic')
     V = rng.standard_normal((9, ref_op.n_unk)) + 1j * rng.standard_normal((9, ref_op.n_unk))
     W64 = op.matvec(V, precision='double')
     W32 = op.matvec(V, precision='single')
-    # FP16 operands (FP16 interpolant coefficients scaled per bin, FP16 inputs scaled by max |x|):
-    # the slot-order orders only
-    W16 = op.matvec(V, precision='half') if order != (4, 4) else None
+    # FP16 operands (FP16 interpolant coefficients scaled per bin, FP16 inputs scaled by max |x|; the
+    # 48-slot blocks in 3 parts of the result tiles), linear or cubic inner operator
+    W16 = op.matvec(V, precision='half')
+    print(order, res, 'FP16 Horner terms', op.half_interpolation()[0], flush=True)
     for r in range(9):
         want = ref_op.matvec(V[r])
         assert np.linalg.norm(W64[r] - want) / np.linalg.norm(want) < 1e-12, r
         assert np.linalg.norm(W32[r] - want) / np.linalg.norm(want) < 3e-6, r
-        if W16 is not None:
-            assert np.linalg.norm(W16[r] - want) / np.linalg.norm(want) < 2e-3, r
+        assert np.linalg.norm(W16[r] - want) / np.linalg.norm(want) < 2e-3, r
 
 
 @pytest.mark.parametrize('kind,cubic,okind', [('linear', 'spline', 'linear'), ('cubic', 'spline', 'cubic'),

@@ -179,17 +179,18 @@ def test_device_resident_solve_matches_host(gpu, precision):
     assert np.linalg.norm(x1 - X_ref[4]) / np.linalg.norm(X_ref[4]) < 1e-7
 
 
-@pytest.mark.parametrize('precision', ['double', 'mixed'])
-def test_lmax4_batched_solve_matches_oracle_lu(gpu, precision):
-    """l_max = m_max = 4 (48 coefficients per particle, the batched plain-layout kernel): ten right-hand
-    sides against the oracle's LU of the same lookup operator; single-RHS solve too."""
+@pytest.mark.parametrize('precision,res', [('double', 5.0), ('mixed', 5.0), ('mixed16', 2.0), ('mixed16', 5.0)])
+def test_lmax4_batched_solve_matches_oracle_lu(gpu, precision, res):
+    """l_max = m_max = 4 (48 coefficients per particle, the slot-order 8-RHS kernels in K-parts, the FP16
+    kernel in parts of the result tiles): ten right-hand sides against the oracle's LU of the same lookup
+    operator; single-RHS solve too."""
     from smuthi_amd import ensembles
     from smuthi_amd.layers import LayerSystem
     from smuthi_amd.linear_system import CouplingMatrixRadialLookupCUDA
     pp = ensembles.small_substrate_ensemble(seed=4)[:20]
     plist = [osys.Sphere(p.position, p.refractive_index, p.radius, 4, 4) for p in pp]
     kp = ob.reasonable_Sommerfeld_kpar_contour(WL, layer_refractive_indices=[1.52, 1], neff_resolution=1e-2)
-    op = CouplingMatrixRadialLookupCUDA(WL, plist, LayerSystem([0, 0], [1.52, 1]), k_parallel=kp, resolution=5.0,
+    op = CouplingMatrixRadialLookupCUDA(WL, plist, LayerSystem([0, 0], [1.52, 1]), k_parallel=kp, resolution=res,
                                         interpolator_kind='cubic')
     assert op.blocksize == 48
     W = oc.RadialLookupCPU(op.lookup_table, op.radial_distance_array, plist, 'cubic').dense()
@@ -200,7 +201,10 @@ def test_lmax4_batched_solve_matches_oracle_lu(gpu, precision):
     M = np.eye(n) - t[:, None] * W
     X_ref = np.linalg.solve(M, R.T).T
     X, st, ok = op.solve_master(t, R, tol=1e-9, precision=precision, with_stats=True)
+    print(precision, st, flush=True)
     assert ok and st['relative_residual'] <= 1e-9
+    if precision == 'mixed16':
+        assert st['matvecs_fp16'] > 0
     x1 = op.solve_master(t, R[3], tol=1e-9, precision=precision)
     for j in range(10):
         assert np.linalg.norm(X[j] - X_ref[j]) / np.linalg.norm(X_ref[j]) < 1e-7, j

@@ -523,7 +523,7 @@ KernelInfo vol_batch_kernel_info() {
 // input slot order)
 template <int L, int Mx>
 void* vol_half_fn() {
-  if constexpr (SlotShape<L, Mx, 4>::fits && Mx <= 3 && SlotShape<L, Mx, 4>::S % 16 == 0)
+  if constexpr (SlotShape<L, Mx, 4>::P32 > 0 && Mx <= 4 && half_tile_parts(4, SvwfIdx<L, Mx>::B) > 0)
     return reinterpret_cast<void*>(&batch_half_kernel<L, Mx, 4, true>);
   else
     return nullptr;
@@ -545,10 +545,11 @@ void* vol_half_kernel(int l, int m) {
   return nullptr;
 }
 
-// FP32: the slot-order kernel where its tap fragments fit the registers
+// FP32: the slot-order kernel where its tap fragments fit the registers in
+// whole parts of the K-steps
 template <int L, int Mx>
 KernelInfo vol_batch32_kernel_info() {
-  if constexpr (SlotShape<L, Mx, 4>::fits && Mx <= 3) {
+  if constexpr (SlotShape<L, Mx, 4>::P32 > 0 && Mx <= 4) {
     KernelInfo k;
     k.fn = reinterpret_cast<void*>(&batch_slots_kernel<L, Mx, 4, true>);
     k.threads = 512;
@@ -563,7 +564,7 @@ KernelInfo vol_batch32_kernel_info() {
 // FP64: the slot-order kernel where its coefficient parts fit the registers
 template <int L, int Mx>
 KernelInfo vol_batch64_kernel_info(bool tap64) {
-  if constexpr (SlotShape<L, Mx, 4>::fits && Mx <= 3 && SlotShape<L, Mx, 4>::P64 > 0) {
+  if constexpr (SlotShape<L, Mx, 4>::P64 > 0 && Mx <= 4) {
     KernelInfo k;
     k.fn = reinterpret_cast<void*>(&batch_slots64_kernel<L, Mx, 4, true>);
     k.threads = kSlots64Threads;

@@ -90,6 +90,41 @@ def test_volume_batched_matvec_matches_oracle(gpu, kind, cubic, okind):
         assert e16 < 3e-3, (r, e16)
 
 
+def test_volume_lmax4_batched_matvec_and_solve(gpu):
+    """l_max = m_max = 4 (48-slot blocks: the FP32 volume kernel in 3 K-parts, FP64 in 6, FP16 in 3 parts of
+    the result tiles) and one (3, 2) sphere: 8-RHS matvecs against the oracle's Catmull-Rom volume operator
+    on the operator's own tables (built on the GPU), and a mixed16 solve against LU."""
+    from smuthi_amd.layers import LayerSystem
+    from smuthi_amd.particles import Sphere
+    from smuthi_amd.volume import CouplingMatrixVolumeLookupCUDA
+    ens = _film_ensemble(12, 17, l_max=4)
+    ens[2] = osys.Sphere(ens[2].position, 1.7, 90, 3, 2)
+    plist = [Sphere(p.position, p.refractive_index, p.radius, p.l_max, p.m_max) for p in ens]
+    op = CouplingMatrixVolumeLookupCUDA(WL, plist, LayerSystem(*FILM), k_parallel=_contour(), resolution=10.0,
+                                        interpolator_kind='cubic', cubic_interpolant='catmull-rom')
+    assert op.blocksize == 48
+    W = oc.VolumeLookupCPU(op.lookup_table_plus, op.lookup_table_minus, op.rho_array, op.sum_z_array,
+                           op.diff_z_array, ens, 'catmull-rom').dense()
+    rng = np.random.default_rng(21)
+    V = rng.standard_normal((9, W.shape[0])) + 1j * rng.standard_normal((9, W.shape[0]))
+    want = V @ W.T
+    got = {p: op.matvec(V, p) for p in ('double', 'single', 'half')}
+    one = op.matvec(V[4], 'double')
+    assert np.linalg.norm(one - want[4]) / np.linalg.norm(want[4]) < 1e-11
+    for r in range(9):
+        nrm = np.linalg.norm(want[r])
+        e = {p: np.linalg.norm(g[r] - want[r]) / nrm for p, g in got.items()}
+        assert e['double'] < 1e-11 and e['single'] < 5e-6 and e['half'] < 3e-3, (r, e)
+    t = np.concatenate([p.t_matrix(WL, FILM[1][1]).diagonal() for p in ens])
+    M = np.eye(W.shape[0]) - t[:, None] * W
+    X, st, ok = op.solve_master(t, V, tol=1e-9, precision='mixed16', with_stats=True)
+    print('l_max 4 volume mixed16', st, flush=True)
+    assert ok and st['matvecs_fp16'] > 0
+    X_ref = np.linalg.solve(M, V.T).T
+    for r in range(9):
+        assert np.linalg.norm(X[r] - X_ref[r]) / np.linalg.norm(X_ref[r]) < 1e-7, r
+
+
 def _fields(n_angles):
     from smuthi_amd.initial_field import PlaneWave
     th = np.arange(n_angles) * np.radians(60) / max(1, n_angles - 1)

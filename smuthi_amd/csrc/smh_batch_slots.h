@@ -1,9 +1,9 @@
 // FP32 lookup matvec for kBatch right-hand sides in "slot" order.
 //
-// Same product as batch_matvec_kernel (per pair: interpolate the lookup
-// block, then one MFMA 16x16x4 tile per K-step applies it to all right-hand
-// sides, columns = (rhs, re|im)), with the multipoles permuted inside the
-// kernel so that the azimuthal phases cost no per-lane selection:
+// Per pair: interpolate the lookup block, then one MFMA 16x16x4 tile per
+// K-step applies it to all right-hand sides (columns = (rhs, re|im)), with
+// the multipoles permuted inside the kernel so that the azimuthal phases
+// cost no per-lane selection:
 //
 //   slot order: multipoles grouped by |m| (m = 0 first); inside a group the
 //   m > 0 ones before the m < 0 ones; every group padded to a multiple of 4

@@ -149,6 +149,7 @@ void smh_op_free(smh_op* op);
 namespace smh {
 // right-hand sides per batched matvec / batched GMRES block
 constexpr int kBatchRhs = 8;
+constexpr int kBatch = kBatchRhs;
 // Index of (particle p, multipole n, rhs r) in the FP32 kBatch-RHS input:
 // slot layout [n_part][4][kBatch][S/4] (slot k = slot[n] sits at (k % 4, r,
 // k / 4)) when a slot table is given, else the plain layout [n_part][S = B][kBatch].

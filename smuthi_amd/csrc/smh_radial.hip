@@ -3,7 +3,6 @@
 #include <cmath>
 
 #include "smh_op.h"
-#include "smh_batch_kernel.h"
 #include "smh_batch_half.h"
 #include "smh_batch_slots.h"
 #include "smh_batch_slots64.h"
@@ -382,9 +381,9 @@ KernelInfo pick(int l, int m) {
   return KernelInfo{};
 }
 
-// Operators whose coefficient fragments fit the registers in whole parts of
-// the K-steps use the slot-order kernels (FP32 and FP64); the others the
-// plain kBatch-RHS kernel.
+// The slot-order kBatch-RHS kernels (FP32 and FP64), their coefficient
+// fragments in whole parts of the K-steps (every cut-off up to l_max = m_max
+// = 4; none otherwise).
 template <int L, int Mx, int T, typename Real>
 KernelInfo batch_kernel_info() {
   KernelInfo k;
@@ -400,11 +399,6 @@ KernelInfo batch_kernel_info() {
     k.sy = SlotShape<L, Mx, T>::LO;  // doubles
     k.slots = SlotShape<L, Mx, T>::S;
     k.slot_tables = &slot_tables<L, Mx>;
-  } else {
-    using Sh = BatchShape<Real, SvwfIdx<L, Mx>::B, T>;
-    k.fn = reinterpret_cast<void*>(&batch_matvec_kernel<Real, Real, L, Mx, T, false>);
-    k.threads = Sh::kThreads;
-    k.sy = Sh::SY;
   }
   return k;
 }

@@ -17,7 +17,6 @@
 // rows. The FP64 kernel reads the same FP32 tap table (complex64 data, exact).
 #include <cmath>
 
-#include "smh_batch_kernel.h"
 #include "smh_batch_slots.h"
 #include "smh_batch_slots64.h"
 #include "smh_batch_half.h"
@@ -509,16 +508,6 @@ KernelInfo volume_kernel(int precision, int l, int m, bool tap64 = false) {
   return KernelInfo{};
 }
 
-// Volume kBatch-RHS kernel (plain layout) of an instance; tap64 as above.
-template <int L, int Mx, typename Real, typename TapT>
-KernelInfo vol_batch_kernel_info() {
-  using Sh = BatchShape<Real, SvwfIdx<L, Mx>::B, 4>;
-  KernelInfo k;
-  k.fn = reinterpret_cast<void*>(&batch_matvec_kernel<Real, TapT, L, Mx, 4, true>);
-  k.threads = Sh::kThreads;
-  k.sy = Sh::SY;
-  return k;
-}
 // FP16 operands: where the FP32 slot-order kernel is used (same binning and
 // input slot order)
 template <int L, int Mx>
@@ -545,8 +534,8 @@ void* vol_half_kernel(int l, int m) {
   return nullptr;
 }
 
-// FP32: the slot-order kernel where its tap fragments fit the registers in
-// whole parts of the K-steps
+// FP32 kBatch-RHS kernel: slot order, its tap fragments in whole parts of the
+// K-steps (every cut-off up to l_max = m_max = 4)
 template <int L, int Mx>
 KernelInfo vol_batch32_kernel_info() {
   if constexpr (SlotShape<L, Mx, 4>::P32 > 0 && Mx <= 4) {
@@ -558,12 +547,13 @@ KernelInfo vol_batch32_kernel_info() {
     k.slot_tables = &slot_tables<L, Mx>;
     return k;
   } else {
-    return vol_batch_kernel_info<L, Mx, float, float>();
+    return KernelInfo{};
   }
 }
-// FP64: the slot-order kernel where its coefficient parts fit the registers
+// FP64 kBatch-RHS kernel: slot order, its coefficients in parts of the
+// K-steps (the tap rows in FP64 whatever the single-vector kernel's taps)
 template <int L, int Mx>
-KernelInfo vol_batch64_kernel_info(bool tap64) {
+KernelInfo vol_batch64_kernel_info() {
   if constexpr (SlotShape<L, Mx, 4>::P64 > 0 && Mx <= 4) {
     KernelInfo k;
     k.fn = reinterpret_cast<void*>(&batch_slots64_kernel<L, Mx, 4, true>);
@@ -573,13 +563,13 @@ KernelInfo vol_batch64_kernel_info(bool tap64) {
     k.slot_tables = &slot_tables<L, Mx>;
     return k;
   } else {
-    return tap64 ? vol_batch_kernel_info<L, Mx, double, double>() : vol_batch_kernel_info<L, Mx, double, float>();
+    return KernelInfo{};
   }
 }
-KernelInfo volume_batch_kernel(int precision, int l, int m, bool tap64 = false) {
+KernelInfo volume_batch_kernel(int precision, int l, int m) {
 #define SMH_V(L, Mx)                                                                                    \
   if (l == L && m == Mx)                                                                                \
-    return precision == SMH_FP32 ? vol_batch32_kernel_info<L, Mx>() : vol_batch64_kernel_info<L, Mx>(tap64);
+    return precision == SMH_FP32 ? vol_batch32_kernel_info<L, Mx>() : vol_batch64_kernel_info<L, Mx>();
   SMH_V(1, 1)
   SMH_V(2, 1)
   SMH_V(2, 2)
@@ -754,7 +744,7 @@ int vol_make_taps(smh_op* op) {
         hipLaunchKernelGGL(vol_half_taps_kernel, dim3((unsigned)rows), dim3(256), 0, s, op->d_vtab[t], mr, mz, mrz,
                            op->vq, op->B, S, (const int*)op->d_b32_tab + op->B, op->d_vtth[t], op->d_vhs[t]);
       }
-      if (volume_batch_kernel(SMH_FP64, op->l_max, op->m_max, op->vq == 4).slots == S) {
+      if (volume_batch_kernel(SMH_FP64, op->l_max, op->m_max).slots == S) {
         if (!op->d_vtts64[t] &&
             hipMalloc(&op->d_vtts64[t], (size_t)rows * 2 * S * S * sizeof(double)) != hipSuccess) {
           cleanup();
@@ -854,7 +844,7 @@ int vol_apply_batch(smh_op* op, int precision, const void* x_pad, void* y_pad, b
   const bool half = precision == SMH_FP16;
   const int pi = (precision == SMH_FP32 || half) ? 0 : 1;
   const bool tap64 = pi == 1 && op->vq == 4;
-  const KernelInfo k = volume_batch_kernel(half ? SMH_FP32 : precision, op->l_max, op->m_max, tap64);
+  const KernelInfo k = volume_batch_kernel(half ? SMH_FP32 : precision, op->l_max, op->m_max);
   void* fn = half ? vol_half_kernel(op->l_max, op->m_max) : k.fn;
   SMH_CHECK(ctx, fn != nullptr && op->vbbin[0][pi].Rc > 0, SMH_ERR_UNSUPPORTED,
             "no batched volume matvec kernel for l_max %d m_max %d", op->l_max, op->m_max);

@@ -923,6 +923,18 @@ def main_learned(args):
                      'fallback': bool(fb), 'worst_relative_residual': st['relative_residual'],
                      'coef_err_vs_fp64_worst': max(errs), 'stats': st}
         log('%s: %.2f s, %d iterations, fallback %s, error %.2e' % (name, secs, st['iterations'], fb, max(errs)))
+    # plain FP64 GMRES on the GPU, no preconditioner: the 16 right-hand sides batched, and one right-hand
+    # side per solve as smuthi runs them
+    LearnedPreconditioner.remove(test.operator)
+    t0 = time.time()
+    _, st64b, _ = test.solve_with_stats(rhs, tol=args.tol, precision='double')
+    s64b = time.time() - t0
+    t0 = time.time()
+    _, st64s, _ = test.solve_with_stats(rhs[:1], tol=args.tol, precision='double')
+    s64s = time.time() - t0
+    log('plain FP64 GMRES: %.2f s for %d right-hand sides batched, %.2f s for one' % (s64b, len(rhs), s64s))
+    fp64 = {'batched': {'seconds': s64b, 'rhs_solves_per_s': len(rhs) / s64b, 'stats': st64b},
+            'single_rhs': {'seconds': s64s, 'rhs_solves_per_s': 1.0 / s64s, 'stats': st64s}}
     value = res['selected']['rhs_solves_per_s'] if res['selected']['coef_err_vs_fp64_worst'] <= 1e-5 else 0.0
     out = {
         'metric': ('RHS-solves/s with the learned near-field preconditioner as selected on a validation ensemble '
@@ -938,6 +950,9 @@ def main_learned(args):
         'iterations': {k: v['iterations'] for k, v in res.items()},
         'fallback_rate': sum(v['fallback'] for v in res.values() if v is not res['plain']) / 3.0,
         'speedup_vs_plain': res['selected']['rhs_solves_per_s'] / res['plain']['rhs_solves_per_s'],
+        'plain_fp64_gmres': fp64,
+        'speedup_vs_plain_fp64_gmres_batched': value / fp64['batched']['rhs_solves_per_s'],
+        'speedup_vs_plain_fp64_gmres_single_rhs': value / fp64['single_rhs']['rhs_solves_per_s'],
         'validation': selection, 'prior_weight': args.prior_weight,
         'autotune': tune,
         'results': res, 'training': {'loss_untrained': L0, 'loss_trained': L1, 'seconds': train_s,

@@ -364,7 +364,9 @@ int smh_swe_field(smh_ctx *ctx, int64_t n_part, const double *pos, const double 
  * reference point), kz signed by the expansion's kind, w_k = 4 kappa_k times
  * the kappa trapezoid weight, w_a the alpha trapezoid weights, bdag the
  * B-dagger coefficients of (l_max, m_max) at (kz / k, kappa / k).
- * kp, kz, w_k, bdag, coef, out: complex128; out [n_pts][blocksize]. */
+ * kp, kz, w_k, bdag, coef, out: complex128; out [n_pts][blocksize].
+ * 1 <= l_max <= 20, 0 <= m_max <= l_max (unrolled kernels up to l_max 4, a
+ * runtime-order kernel above, five orders m per launch). */
 int smh_pwe_to_swe(smh_ctx *ctx, int64_t n_k, const double *kp, const double *kz, const double *w_k, int64_t n_a,
                    const double *alpha, const double *w_a, const double *coef, const double *bdag, int l_max,
                    int m_max, int64_t n_pts, const double *pts, double *out, double *kernel_ms);

@@ -142,8 +142,11 @@ __global__ void __launch_bounds__(kHalfThreads) batch_half_kernel(RadialKArgs a,
   // 4 c + q (input rotation): tile c and chunk c share the lane's |m|. Per
   // chunk the lane evaluates cos(|m| phi) = T_|m|(cos phi) and sin(|m| phi) =
   // sin phi U_{|m|-1}(cos phi) as polynomials in cos phi whose coefficients
-  // (Chebyshev T and U) it holds for its |m|.
-  float tcf[NC][MMAX + 1], ucf[NC][MMAX + 1];
+  // (Chebyshev T and U) it holds for its |m|. The volume form, short of
+  // registers for these coefficients, selects among cos / sin(k phi) instead
+  // (one-hot weights hot[c][k] = [|m| == k]).
+  constexpr bool kChebPhase = !VOL;
+  float tcf[NC][MMAX + 1], ucf[NC][MMAX + 1], hot[NC][MMAX + 1];
   unsigned smlo[MH], smhi[MH], zlo[NC], zhi[NC];
 #pragma unroll
   for (int c = 0; c < NC; ++c) {
@@ -152,8 +155,12 @@ __global__ void __launch_bounds__(kHalfThreads) batch_half_kernel(RadialKArgs a,
     const int sg = q == 0 ? so.sgn[g0] : (q == 1 ? so.sgn[g0 + 1] : (q == 2 ? so.sgn[g0 + 2] : so.sgn[g0 + 3]));
 #pragma unroll
     for (int k = 0; k <= MMAX; ++k) {
-      tcf[c][k] = kCheb.t[ra][k];
-      ucf[c][k] = kCheb.u[ra][k];
+      if constexpr (kChebPhase) {
+        tcf[c][k] = kCheb.t[ra][k];
+        ucf[c][k] = kCheb.u[ra][k];
+      } else {
+        hot[c][k] = ra == k ? 1.f : 0.f;
+      }
     }
     // (the im column takes -sin: y'_im = cos y_im - sin y_re)
     smlo[c] = (sg < 0 ? 0x80000000u : 0u) ^ (zp ? 0x80000000u : 0u);
@@ -238,19 +245,39 @@ __global__ void __launch_bounds__(kHalfThreads) batch_half_kernel(RadialKArgs a,
       // largest |m| of the chunk's four groups)
       const float c1 = rec.c, s1 = rec.s;
       float cph[NC], sph[NC];
+      if constexpr (kChebPhase) {
 #pragma unroll
-      for (int c = 0; c < NC; ++c) {
-        const int dg = max(max(so.absm[4 * c], so.absm[4 * c + 1]), max(so.absm[4 * c + 2], so.absm[4 * c + 3]));
-        float v = tcf[c][dg];
+        for (int c = 0; c < NC; ++c) {
+          const int dg = max(max(so.absm[4 * c], so.absm[4 * c + 1]), max(so.absm[4 * c + 2], so.absm[4 * c + 3]));
+          float v = tcf[c][dg];
 #pragma unroll
-        for (int k = MMAX - 1; k >= 0; --k)
-          if (k < dg) v = fmaf(v, c1, tcf[c][k]);
-        cph[c] = v;
-        float u = dg > 0 ? ucf[c][dg - 1] : 0.f;
+          for (int k = MMAX - 1; k >= 0; --k)
+            if (k < dg) v = fmaf(v, c1, tcf[c][k]);
+          cph[c] = v;
+          float u = dg > 0 ? ucf[c][dg - 1] : 0.f;
 #pragma unroll
-        for (int k = MMAX - 2; k >= 0; --k)
-          if (k < dg - 1) u = fmaf(u, c1, ucf[c][k]);
-        sph[c] = u * s1;
+          for (int k = MMAX - 2; k >= 0; --k)
+            if (k < dg - 1) u = fmaf(u, c1, ucf[c][k]);
+          sph[c] = u * s1;
+        }
+      } else {
+        float cs[MMAX + 1], sn[MMAX + 1];
+        phase_powers<MMAX>(c1, s1, cs, sn);
+#pragma unroll
+        for (int c = 0; c < NC; ++c) {
+          float cz = 0.f, sz = 0.f;
+#pragma unroll
+          for (int am = 0; am <= MMAX; ++am) {
+            // |m| values outside the chunk's four groups drop out at compile time
+            if (so.absm[4 * c] != am && so.absm[4 * c + 1] != am && so.absm[4 * c + 2] != am &&
+                so.absm[4 * c + 3] != am)
+              continue;
+            cz = fmaf(hot[c][am], cs[am], cz);
+            if (am > 0) sz = fmaf(hot[c][am], sn[am], sz);
+          }
+          cph[c] = cz;
+          sph[c] = sz;
+        }
       }
 
       // B operands per chunk, FP16: zb = the lane's column of z = e^{i m2 phi} x

@@ -144,6 +144,101 @@ __global__ void __launch_bounds__(kP2sPoints) pwe_to_swe_kernel(P2sArgs a) {
   }
 }
 
+// Any (l_max, m_max): the orders m in chunks of kP2sOrders, one launch each
+// (m0 .. m0 + nm - 1); per (point, kappa) the chunk's alpha sums as above,
+// then the B-dagger contraction of the chunk's multipoles (nlist: [nm][LM]
+// indices n, -1 padded) added to the point's partial row, which the thread
+// owns (in global memory: B values do not fit registers at large l_max; the
+// rows start zeroed). The unrolled kernels cover l_max <= 4.
+constexpr int kP2sOrders = 5;
+constexpr int kP2sMaxL = 20;
+
+__global__ void __launch_bounds__(kP2sPoints)
+    pwe_to_swe_any_kernel(P2sArgs a, int m0, int nm, const int* __restrict__ nlist, int LM, int B) {
+  using C = cplx<double>;
+  __shared__ C G[2][kP2sOrders][kP2sTile];
+  __shared__ double CS[kP2sTile][2];
+  const int64_t ip = (int64_t)blockIdx.x * kP2sPoints + threadIdx.x;
+  const bool live = ip < a.n_pts;
+  double dx = 0, dy = 0, dz = 0;
+  if (live) {
+    dx = a.pts[3 * ip];
+    dy = a.pts[3 * ip + 1];
+    dz = a.pts[3 * ip + 2];
+  }
+  double2* o = a.part + ((int64_t)blockIdx.y * a.n_pts + (live ? ip : 0)) * B;
+  const int64_t k0 = (int64_t)blockIdx.y * a.k_per;
+  const int64_t k1 = min(a.n_k, k0 + a.k_per);
+  for (int64_t k = k0; k < k1; ++k) {
+    const double2 kp = a.kp[k];
+    C S[2][kP2sOrders];
+#pragma unroll
+    for (int q = 0; q < 2; ++q)
+#pragma unroll
+      for (int m = 0; m < kP2sOrders; ++m) S[q][m] = C{0.0, 0.0};
+    for (int64_t j0 = 0; j0 < a.n_a; j0 += kP2sTile) {
+      const int nj = (int)min<int64_t>(kP2sTile, a.n_a - j0);
+      __syncthreads();
+      for (int t = threadIdx.x; t < 2 * kP2sOrders * kP2sTile; t += kP2sPoints) {
+        const int q = t / (kP2sOrders * kP2sTile), r = t - q * kP2sOrders * kP2sTile, mi = r / kP2sTile,
+                  j = r - mi * kP2sTile;
+        C v{0.0, 0.0};
+        if (j < nj && mi < nm) {
+          const double al = a.alpha[j0 + j];
+          double sn, cn;
+          sincos((double)(m0 + mi) * al, &sn, &cn);
+          const double2 gv = a.g[((int64_t)q * a.n_k + k) * a.n_a + j0 + j];
+          const double w = a.wa[j0 + j];
+          v = C{w * (gv.x * cn + gv.y * sn), w * (gv.y * cn - gv.x * sn)};
+        }
+        G[q][mi][j] = v;
+      }
+      for (int j = threadIdx.x; j < kP2sTile; j += kP2sPoints) {
+        double sn = 0, cn = 0;
+        if (j < nj) sincos(a.alpha[j0 + j], &sn, &cn);
+        CS[j][0] = cn;
+        CS[j][1] = sn;
+      }
+      __syncthreads();
+      for (int j = 0; j < nj; ++j) {
+        const double u = CS[j][0] * dx + CS[j][1] * dy;
+        double sn, cn;
+        sincos(kp.x * u, &sn, &cn);
+        const double mag = kp.y == 0.0 ? 1.0 : exp(-kp.y * u);
+        const C ph{mag * cn, mag * sn};
+#pragma unroll
+        for (int q = 0; q < 2; ++q)
+#pragma unroll
+          for (int m = 0; m < kP2sOrders; ++m) {
+            const C gq = G[q][m][j];
+            S[q][m].re = fma(gq.re, ph.re, fma(-gq.im, ph.im, S[q][m].re));
+            S[q][m].im = fma(gq.re, ph.im, fma(gq.im, ph.re, S[q][m].im));
+          }
+      }
+    }
+    if (!live) continue;
+    const double2 kzv = a.kz[k], wkv = a.wk[k];
+    double sn, cn;
+    sincos(kzv.x * dz, &sn, &cn);
+    const double mag = exp(-kzv.y * dz);
+    const C e = cmul(C{wkv.x, wkv.y}, C{mag * cn, mag * sn});
+    const double2* bd = a.bdag + k * 2 * B;
+#pragma unroll
+    for (int mi = 0; mi < kP2sOrders; ++mi) {
+      if (mi >= nm) break;
+      for (int jj = 0; jj < LM; ++jj) {
+        const int n = nlist[mi * LM + jj];
+        if (n < 0) break;
+        const double2 b0 = bd[n], b1 = bd[B + n];
+        const C t = cadd(cmul(C{b0.x, b0.y}, S[0][mi]), cmul(C{b1.x, b1.y}, S[1][mi]));
+        const C et = cmul(e, t);
+        const double2 prev = o[n];
+        o[n] = make_double2(prev.x + et.re, prev.y + et.im);
+      }
+    }
+  }
+}
+
 __global__ void p2s_reduce_kernel(const double2* __restrict__ part, int n_split, int64_t count,
                                   double2* __restrict__ out) {
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -188,8 +283,10 @@ extern "C" int smh_pwe_to_swe(smh_ctx* ctx, int64_t n_k, const double* kp, const
   if (!ctx) return SMH_ERR_ARG;
   SMH_CHECK(ctx, n_k >= 1 && n_a >= 1 && n_pts >= 0 && n_pts < (1 << 30), SMH_ERR_ARG, "invalid sizes");
   SMH_CHECK(ctx, kp && kz && w_k && alpha && w_a && coef && bdag, SMH_ERR_ARG, "expansion arrays are NULL");
+  SMH_CHECK(ctx, l_max >= 1 && m_max >= 0 && m_max <= l_max && l_max <= kP2sMaxL, SMH_ERR_ARG,
+            "l_max %d / m_max %d out of range (l_max <= %d)", l_max, m_max, kP2sMaxL);
+  // unrolled kernels up to l_max 4, the runtime-order kernel above
   void* fn = p2s_kernel(l_max, m_max);
-  SMH_CHECK(ctx, fn != nullptr, SMH_ERR_UNSUPPORTED, "no PWE-to-SWE kernel for l_max %d m_max %d", l_max, m_max);
   if (kernel_ms) *kernel_ms = 0.0;
   if (n_pts == 0) return SMH_OK;
   SMH_CHECK(ctx, pts && out, SMH_ERR_ARG, "points / output are NULL");
@@ -235,9 +332,42 @@ extern "C" int smh_pwe_to_swe(smh_ctx* ctx, int64_t n_k, const double* kp, const
   a.k_per = k_per;
   a.part = (double2*)buf[8];
   const int64_t count = n_pts * B;
+  // runtime-order kernel: the multipoles of each order m, in chunks of kP2sOrders orders
+  const int n_orders = 2 * m_max + 1, LM = 2 * l_max;
+  const int n_chunks = (n_orders + kP2sOrders - 1) / kP2sOrders;
+  std::vector<int> nlist;
+  void* d_nlist = nullptr;
+  if (!fn) {
+    nlist.assign((size_t)n_chunks * kP2sOrders * LM, -1);
+    std::vector<int> fill((size_t)n_chunks * kP2sOrders, 0);
+    for (int tau = 0; tau < 2; ++tau)
+      for (int l = 1; l <= l_max; ++l) {
+        const int ml = std::min(l, m_max);
+        for (int m = -ml; m <= ml; ++m) {
+          const int r = m + m_max;  // row: chunk r / kP2sOrders, order r % kP2sOrders in it
+          nlist[(size_t)r * LM + fill[r]++] = smh_multi_to_single(tau, l, m, l_max, m_max);
+        }
+      }
+    if (hipMalloc(&d_nlist, nlist.size() * sizeof(int)) != hipSuccess ||
+        hipMemcpyAsync(d_nlist, nlist.data(), nlist.size() * sizeof(int), hipMemcpyHostToDevice, s) != hipSuccess ||
+        hipMemsetAsync(buf[8], 0, sz[8], s) != hipSuccess) {
+      if (d_nlist) (void)hipFree(d_nlist);
+      release();
+      return smh_fail(ctx, SMH_ERR_HIP, "PWE-to-SWE device allocation failed");
+    }
+  }
   hipError_t e = hipEventRecord(ctx->ev0, s);
   void* args[] = {&a};
-  if (e == hipSuccess) e = hipLaunchKernel(fn, dim3((unsigned)pblocks, (unsigned)splits), dim3(kP2sPoints), args, 0, s);
+  if (fn) {
+    if (e == hipSuccess) e = hipLaunchKernel(fn, dim3((unsigned)pblocks, (unsigned)splits), dim3(kP2sPoints), args, 0, s);
+  } else {
+    for (int c = 0; c < n_chunks && e == hipSuccess; ++c) {
+      const int m0 = c * kP2sOrders - m_max, nm = std::min(kP2sOrders, n_orders - c * kP2sOrders);
+      hipLaunchKernelGGL(pwe_to_swe_any_kernel, dim3((unsigned)pblocks, (unsigned)splits), dim3(kP2sPoints), 0, s, a,
+                         m0, nm, (const int*)d_nlist + (size_t)c * kP2sOrders * LM, LM, B);
+      e = hipGetLastError();
+    }
+  }
   if (e == hipSuccess) e = hipEventRecord(ctx->ev1, s);
   if (e == hipSuccess) {
     hipLaunchKernelGGL(p2s_reduce_kernel, dim3((unsigned)((count + 255) / 256)), dim3(256), 0, s,
@@ -248,6 +378,7 @@ extern "C" int smh_pwe_to_swe(smh_ctx* ctx, int64_t n_k, const double* kp, const
   if (e == hipSuccess) e = hipStreamSynchronize(s);
   float ms = 0;
   if (e == hipSuccess) e = hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1);
+  if (d_nlist) (void)hipFree(d_nlist);
   release();
   SMH_CHECK(ctx, e == hipSuccess, SMH_ERR_HIP, "PWE-to-SWE kernel failed: %s", hipGetErrorString(e));
   if (kernel_ms) *kernel_ms = ms;

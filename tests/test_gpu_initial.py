@@ -73,6 +73,27 @@ def test_gaussian_beam_many_particles_layered(gpu):
     assert np.abs(got - want).max() < 1e-10 * np.abs(want).max()
 
 
+def test_gaussian_beam_high_orders(gpu):
+    # l_max 5..7 and m_max 0: the runtime-order PWE-to-SWE kernel (orders m in chunks of 5 per launch)
+    from smuthi_amd.initial_field import GaussianBeam
+    from smuthi_amd.layers import LayerSystem
+    wl = 550.0
+    thick, nidx = [0, 250, 0], [1.5, 1.9 + 0.02j, 1.0]
+    kp = ob.reasonable_Sommerfeld_kpar_contour(wl, layer_refractive_indices=[1.5, 1.9, 1.0], neff_resolution=5e-3)
+    kp = kp[np.real(kp) < 0.95 * 2 * np.pi / wl]
+    al = np.linspace(0, 2 * np.pi, 181)
+    rng = np.random.default_rng(5)
+    parts = [_P([rng.uniform(-400, 400), rng.uniform(-400, 400), z], l, m)
+             for z, l, m in ((-90, 6, 6), (100, 5, 3), (200, 7, 2), (420, 5, 5), (460, 3, 0))]
+    beam = GaussianBeam(wl, np.pi - 0.35, 1.1, 1, 1200.0, k_parallel_array=kp, azimuthal_angles_array=al,
+                        amplitude=0.8, reference_point=[50, -20, 300])
+    got = beam.swe_coefficients(parts, LayerSystem(thick, nidx))
+    beam_o = oin.GaussianBeam(wl, np.pi - 0.35, 1.1, 1, 1200.0, kp, al, amplitude=0.8, reference_point=[50, -20, 300])
+    want = np.concatenate([osys.initial_field_coefficients(beam_o, osys.Sphere(p.position, 1.5, 50, p.l_max, p.m_max),
+                                                           ol.LayerSystem(thick, nidx)) for p in parts])
+    assert np.abs(got - want).max() < 1e-10 * np.abs(want).max()
+
+
 def test_dipole_golden_vector(gpu):
     # tests/unit_tests/initial_field_tests/test_initial_field_dipole.py:8-52
     from smuthi_amd.initial_field import DipoleSource

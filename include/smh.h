@@ -265,7 +265,7 @@ int smh_op_matvec_noalloc(smh_op *op, const double *x, double *y, int precision)
 int smh_op_bench_matvec(smh_op *op, int precision, int iters, double *kernel_ms, double *total_ms);
 
 /* The same for the batched kernel: one launch applies W to 8 right-hand
- * sides (smh_op_matvec uses it for nrhs > 1 on radial operators). No
+ * sides (smh_op_matvec uses it for nrhs > 1 on lookup operators). No
  * counterpart in smuthi, which has no multi-RHS path. */
 int smh_op_bench_matvec_batch(smh_op *op, int precision, int iters, double *kernel_ms, double *total_ms);
 

@@ -75,8 +75,11 @@ __device__ inline void phase_powers64(double c1, double s1, double (&cs)[K + 1],
 // VOL: volume lookups, as batch_slots_kernel's VOL form: the table holds FP64
 // tap rows in this kernel's fragment order ([rows][MT][NS/2][64 lanes][2]),
 // an entry's T taps of z slice a.vslice at rows en.x + vol_tap_offset, and a
-// pair weights them with vol_tap_weight(w, w2).
-template <int LMAX, int MMAX, int T, bool VOL = false>
+// pair weights them with vol_tap_weight(w, w2). TAP32 (VOL, tables of plain
+// FP32 values: no spline moments): the taps come from the FP32 kernel's rows
+// ([rows][MT][NS/4][64 lanes][4], the same values exactly) and are widened in
+// registers, half the tap-row bytes of the FP64 rows.
+template <int LMAX, int MMAX, int T, bool VOL = false, bool TAP32 = false>
 __global__ void __launch_bounds__(kSlots64Threads) batch_slots64_kernel(RadialKArgs a) {
   using Sh = SlotShape<LMAX, MMAX, T>;
   constexpr int S = Sh::S, NS = Sh::NS, MT = Sh::MT, MH = Sh::MH, SC = Sh::SC, LO = Sh::LO;
@@ -86,6 +89,7 @@ __global__ void __launch_bounds__(kSlots64Threads) batch_slots64_kernel(RadialKA
   static_assert(NP > 0, "coefficients exceed the register budget");
   constexpr int NSH = NS / NP;  // K-steps per part (even: coefficient pairs per lane)
   static_assert(MMAX <= 4, "phase selection covers |m| <= 4");
+  static_assert(VOL || !TAP32, "FP32 tap rows are a volume form");
   constexpr SlotOrder<LMAX, MMAX> so{};
   static_assert(so.ok, "slot order");
   using V4 = typename Mfma<double>::v4;
@@ -137,12 +141,22 @@ __global__ void __launch_bounds__(kSlots64Threads) batch_slots64_kernel(RadialKA
         for (int i = 0; i < MT; ++i) {
           const size_t trow =
               VOL ? (size_t)(en.x + vol_tap_offset(a.vmode, a.vslice, t, a.len_z)) * MT + i : ((size_t)bin * T + t) * MT + i;
-          const double2* row = tt2 + trow * (NS / 2) * 64;
+          if constexpr (TAP32) {
+            // K-step k of the lane: float k % 4 of float4 (k / 4) * 64 + lane of the row
+            const float* rowf = reinterpret_cast<const float*>(a.tt) + trow * (NS / 4) * 64 * 4;
 #pragma unroll
-          for (int s2 = 0; s2 < NSH / 2; ++s2) {
-            const double2 v = row[(h * (NSH / 2) + s2) * 64 + lane];
-            tv[t][i][2 * s2] = v.x;
-            tv[t][i][2 * s2 + 1] = v.y;
+            for (int sl = 0; sl < NSH; ++sl) {
+              const int k = h * NSH + sl;
+              tv[t][i][sl] = (double)rowf[((k >> 2) * 64 + lane) * 4 + (k & 3)];
+            }
+          } else {
+            const double2* row = tt2 + trow * (NS / 2) * 64;
+#pragma unroll
+            for (int s2 = 0; s2 < NSH / 2; ++s2) {
+              const double2 v = row[(h * (NSH / 2) + s2) * 64 + lane];
+              tv[t][i][2 * s2] = v.x;
+              tv[t][i][2 * s2 + 1] = v.y;
+            }
           }
         }
       auto load_x = [&](const Rec& r, double (&xs)[2 * NSH]) {

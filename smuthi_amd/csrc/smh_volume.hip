@@ -566,6 +566,32 @@ KernelInfo vol_batch64_kernel_info() {
     return KernelInfo{};
   }
 }
+// FP64 kBatch-RHS kernel over the FP32 slot-order tap rows (tables of plain
+// values, vq = 1: the rows hold the FP32 table values exactly)
+template <int L, int Mx>
+void* vol_batch64_tap32_fn() {
+  if constexpr (SlotShape<L, Mx, 4>::P64 > 0 && SlotShape<L, Mx, 4>::P32 > 0 && Mx <= 4)
+    return reinterpret_cast<void*>(&batch_slots64_kernel<L, Mx, 4, true, true>);
+  else
+    return nullptr;
+}
+void* vol_batch64_tap32_kernel(int l, int m) {
+#define SMH_V(L, Mx) \
+  if (l == L && m == Mx) return vol_batch64_tap32_fn<L, Mx>();
+  SMH_V(1, 1)
+  SMH_V(2, 1)
+  SMH_V(2, 2)
+  SMH_V(3, 1)
+  SMH_V(3, 2)
+  SMH_V(3, 3)
+  SMH_V(4, 1)
+  SMH_V(4, 2)
+  SMH_V(4, 3)
+  SMH_V(4, 4)
+#undef SMH_V
+  return nullptr;
+}
+
 KernelInfo volume_batch_kernel(int precision, int l, int m) {
 #define SMH_V(L, Mx)                                                                                    \
   if (l == L && m == Mx)                                                                                \
@@ -744,7 +770,9 @@ int vol_make_taps(smh_op* op) {
         hipLaunchKernelGGL(vol_half_taps_kernel, dim3((unsigned)rows), dim3(256), 0, s, op->d_vtab[t], mr, mz, mrz,
                            op->vq, op->B, S, (const int*)op->d_b32_tab + op->B, op->d_vtth[t], op->d_vhs[t]);
       }
-      if (volume_batch_kernel(SMH_FP64, op->l_max, op->m_max).slots == S) {
+      // FP64 slot rows only for tables with spline moments (vq = 4); plain
+      // values (vq = 1) reach the FP64 kernel through the FP32 rows above
+      if (op->vq == 4 && volume_batch_kernel(SMH_FP64, op->l_max, op->m_max).slots == S) {
         if (!op->d_vtts64[t] &&
             hipMalloc(&op->d_vtts64[t], (size_t)rows * 2 * S * S * sizeof(double)) != hipSuccess) {
           cleanup();
@@ -845,7 +873,10 @@ int vol_apply_batch(smh_op* op, int precision, const void* x_pad, void* y_pad, b
   const int pi = (precision == SMH_FP32 || half) ? 0 : 1;
   const bool tap64 = pi == 1 && op->vq == 4;
   const KernelInfo k = volume_batch_kernel(half ? SMH_FP32 : precision, op->l_max, op->m_max);
-  void* fn = half ? vol_half_kernel(op->l_max, op->m_max) : k.fn;
+  // FP64 over tables of plain values: the FP32 slot rows, widened in the kernel
+  const bool rows32 = pi == 1 && op->vq == 1 && k.slots > 0;
+  void* fn = half ? vol_half_kernel(op->l_max, op->m_max)
+                  : (rows32 ? vol_batch64_tap32_kernel(op->l_max, op->m_max) : k.fn);
   SMH_CHECK(ctx, fn != nullptr && op->vbbin[0][pi].Rc > 0, SMH_ERR_UNSUPPORTED,
             "no batched volume matvec kernel for l_max %d m_max %d", op->l_max, op->m_max);
   hipStream_t s = ctx->stream;
@@ -853,7 +884,8 @@ int vol_apply_batch(smh_op* op, int precision, const void* x_pad, void* y_pad, b
   SMH_CHECK(ctx,
             !slots || (op->d_b32_tab && op->b32_S == k.slots &&
                        (half ? op->d_vtth[0] && op->d_vtth[1]
-                             : (pi == 0 ? op->d_vtts[0] && op->d_vtts[1] : op->d_vtts64[0] && op->d_vtts64[1]))),
+                             : (pi == 0 || rows32 ? op->d_vtts[0] && op->d_vtts[1]
+                                                  : op->d_vtts64[0] && op->d_vtts64[1]))),
             SMH_ERR_STATE, "slot-order tap rows missing");
   SMH_CHECK(ctx, !half || slots, SMH_ERR_STATE, "the FP16 kernel needs the slot order");
   const int S = slots ? k.slots : op->B;
@@ -885,7 +917,7 @@ int vol_apply_batch(smh_op* op, int precision, const void* x_pad, void* y_pad, b
       a.items = bn.d_items;
       a.order = bn.d_order;
       a.tt = half ? (const void*)op->d_vtth[t]
-                  : (slots ? (pi == 0 ? (const void*)op->d_vtts[t] : (const void*)op->d_vtts64[t])
+                  : (slots ? (pi == 0 || rows32 ? (const void*)op->d_vtts[t] : (const void*)op->d_vtts64[t])
                            : (tap64 ? (const void*)op->d_vtt64[t] : (const void*)op->d_vtt[t]));
       a.x = xin;
       a.ypart = op->d_ypart;

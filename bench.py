@@ -409,6 +409,10 @@ def cfg1_cpu_vs_gpu(workdir, tol=1e-5):
                          'drop-in), one plane wave' % tol)
 
 
+def _order_name(order):
+    return {0: 'chunk', 1: 'xcd'}.get(order, 'bands%d' % order)
+
+
 def autotune(op, shapes, enabled=True):
     """Launch-shape autotuning of the operator's kernels (setup, outside the timed region): per shape
     (batch, precision) the kernel time of every candidate grouping and the one kept."""
@@ -419,8 +423,8 @@ def autotune(op, shapes, enabled=True):
     for batch, prec in shapes:
         r = op.autotune(batch=batch, precision=prec, iters=2)
         out['%s_%s' % ('8rhs' if batch else '1rhs', prec)] = {
-            'best': {'items_per_cu': r['best'][0], 'xcd_order': bool(r['best'][1])},
-            'kernel_ms': {'%d/%s' % (k[0], 'xcd' if k[1] else 'chunk'): round(v, 4) for k, v in r['kernel_ms'].items()}}
+            'best': {'items_per_cu': r['best'][0], 'launch_order': _order_name(r['best'][1])},
+            'kernel_ms': {'%d/%s' % (k[0], _order_name(k[1])): round(v, 4) for k, v in r['kernel_ms'].items()}}
         log('autotune %s %s: best %s, %.3f ms (default 16/xcd %.3f ms)'
             % ('8-RHS' if batch else '1-RHS', prec, r['best'], r['kernel_ms'][r['best']],
                r['kernel_ms'].get((16, 1), float('nan'))))

@@ -271,8 +271,10 @@ int smh_op_bench_matvec_batch(smh_op *op, int precision, int iters, double *kern
 
 /* Launch-shape autotuner of a lookup operator's matvec (radial or volume,
  * table set). For each of n_cand candidates c it regroups the pair bins into
- * work items (about items_per_cu[c] workgroups per compute unit; launch order
- * dealt to the XCDs by table row when xcd_order[c] != 0, else chunk order),
+ * work items (about items_per_cu[c] workgroups per compute unit) launched in
+ * the order launch_order[c]: 0 chunk order, 1 sorted by table row and dealt
+ * to the XCDs in contiguous shares, n in [2, 4096] items also cut at the
+ * boundaries of n equal bands of table rows and launched band after band;
  * times `iters` matvecs of the given shape (batch != 0: the 8-RHS kernel in
  * SMH_FP16 / FP32 / FP64; else the single-vector kernel in FP32 / FP64) into
  * kernel_ms[c], and keeps the fastest grouping (*best = its index). The FP16
@@ -282,7 +284,7 @@ int smh_op_bench_matvec_batch(smh_op *op, int precision, int iters, double *kern
  * smuthi, whose CUDA kernels launch one thread per matrix row
  * (linear_system_cuda.py:26-405, cuda_blocksize). */
 int smh_op_autotune(smh_op *op, int batch, int precision, int iters, int n_cand, const int *items_per_cu,
-                    const int *xcd_order, double *kernel_ms, int *best);
+                    const int *launch_order, double *kernel_ms, int *best);
 
 /* Interpolation of the FP16 operator used by the mixed16 inner solves (and
  * smh_op_matvec with SMH_FP16): *terms = 2 when it interpolates linearly

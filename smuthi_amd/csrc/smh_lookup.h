@@ -271,7 +271,11 @@ inline void set_item_lsum(smh_op::Binning& bn, const float* rowsum, int T, int v
 // statistics (replacing an earlier grouping). Cost of an entry: single-vector
 // kernels one unit per tile of 16 pairs plus 1/4 for loading the bin's taps;
 // the kBatch-RHS kernels one unit per pair plus one for the taps.
-inline int group_work_items(smh_op* op, smh_op::Binning& bn, int items_per_cu, bool xcd_order) {
+// order: 0 chunk order, 1 sorted by first table row and dealt to the XCDs in
+// contiguous shares, n >= 2 cut at the boundaries of n equal bands of table
+// rows and launched band after band (kRowBandOrder).
+constexpr int kRowBandOrder = 2;
+inline int group_work_items(smh_op* op, smh_op::Binning& bn, int items_per_cu, int order_mode) {
   const bool batch = bn.batch;
   auto cost = [batch](int np) { return batch ? np + 1.0 : (np + 15) / 16 + 0.25; };
   smh_ctx* ctx = op->ctx;
@@ -282,13 +286,19 @@ inline int group_work_items(smh_op* op, smh_op::Binning& bn, int items_per_cu, b
   for (const int4& e : entries) total += cost(e.z);
   const int64_t target_items = std::max<int64_t>((int64_t)items_per_cu * ctx->num_cu, bn.n_chunks);
   const double budget = std::max(1.0, total / (double)target_items);
+  // row bands: n_bands equal ranges of the entries' first tap rows
+  const int n_bands = order_mode >= kRowBandOrder ? order_mode : 1;
+  int max_row = 0;
+  for (const int4& e : entries) max_row = std::max(max_row, e.x);
+  const int band_rows = (int)(((int64_t)max_row + n_bands) / n_bands);
+  auto band_of = [&](size_t e) { return entries[e].x / band_rows; };
   std::vector<int2> items;
   std::vector<int> chunk_items(bn.n_chunks + 1, 0);
   int cur_chunk = -1;
   double run = 0;
   for (size_t e = 0; e < entries.size(); ++e) {
     const int c = ent_chunk[e];
-    if (c != cur_chunk || run >= budget) {
+    if (c != cur_chunk || run >= budget || (n_bands > 1 && band_of(e) != band_of(e - 1))) {
       while (cur_chunk < c) chunk_items[++cur_chunk] = (int)items.size();
       items.push_back(make_int2(c, (int)e));
       run = 0;
@@ -297,15 +307,20 @@ inline int group_work_items(smh_op* op, smh_op::Binning& bn, int items_per_cu, b
   }
   while (cur_chunk < bn.n_chunks) chunk_items[++cur_chunk] = (int)items.size();
   const int ni = (int)items.size();
-  // Launch order: items sorted by their first table row; blocks are dealt to
-  // the XCDs round-robin (XCD = block mod kXcds), so XCD x gets the x-th
-  // contiguous share of the sorted list and its CUs run items of neighbouring
-  // bins (from different receiver chunks) at the same time. Without it,
-  // blocks run the items in (chunk, bin) order.
+  // Launch order (order_mode 1): items sorted by their first table row;
+  // blocks are dealt to the XCDs round-robin (XCD = block mod kXcds), so XCD x
+  // gets the x-th contiguous share of the sorted list and its CUs run items of
+  // neighbouring bins (from different receiver chunks) at the same time.
+  // Row bands: the items of band 0 of every chunk first, then band 1, ..., so
+  // all XCDs read the same band's tap rows at a time and share them through
+  // the Infinity Cache. Order 0: blocks run the items in (chunk, bin) order.
   constexpr int kXcds = 8;
   std::vector<int> order(ni);
   for (int k = 0; k < ni; ++k) order[k] = k;
-  if (xcd_order) {
+  if (n_bands > 1) {
+    std::stable_sort(order.begin(), order.end(),
+                     [&](int p, int q) { return band_of(items[p].y) < band_of(items[q].y); });
+  } else if (order_mode == 1) {
     std::vector<int> sorted(order);
     std::stable_sort(sorted.begin(), sorted.end(),
                      [&](int p, int q) { return entries[items[p].y].x < entries[items[q].y].x; });
@@ -325,7 +340,7 @@ inline int group_work_items(smh_op* op, smh_op::Binning& bn, int items_per_cu, b
   bn.d_order = nullptr;
   bn.n_items = ni;
   bn.items_per_cu = items_per_cu;
-  bn.xcd_order = xcd_order;
+  bn.order_mode = order_mode;
   if (hipMalloc(&bn.d_items, std::max<size_t>(items.size(), 1) * sizeof(int2)) != hipSuccess ||
       hipMalloc(&bn.d_chunk_items, chunk_items.size() * sizeof(int)) != hipSuccess ||
       hipMalloc(&bn.d_item_lsum, std::max(ni, 1) * sizeof(float)) != hipSuccess ||
@@ -374,7 +389,7 @@ int build_work_items(smh_op* op, smh_op::Binning& bn, const std::vector<int>& cn
       hipMemcpyAsync(bn.d_entries, bn.h_entries.data(), bn.h_entries.size() * sizeof(int4), hipMemcpyHostToDevice,
                      s) != hipSuccess)
     return smh_fail(ctx, SMH_ERR_HIP, "device allocation failed (bin entries)");
-  return group_work_items(op, bn, kItemsPerCu, true);
+  return group_work_items(op, bn, kItemsPerCu, 1);
 }
 
 }  // namespace

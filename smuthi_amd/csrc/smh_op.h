@@ -62,12 +62,13 @@ struct smh_op {
     // grouping of the entries into work items (regrouped by the autotuner):
     // host copies of the entries and their chunks, the item cost model
     // (batch: per pair, else per 16-pair tile), the target items per CU and
-    // whether the launch order deals sorted items to the XCDs
+    // the launch order (0 chunk order, 1 XCD-dealt by table row, n >= 2 row
+    // bands)
     std::vector<int4> h_entries;
     std::vector<int> h_ent_chunk;
     bool batch = false;
     int items_per_cu = 0;
-    bool xcd_order = true;
+    int order_mode = 1;
     void (*count_items)(Binning&, hipStream_t) = nullptr;  // item_count_kernel for the record type
     // item_lsum_kernel arguments once the table is set (rows == nullptr: not yet)
     const float* lsum_rows = nullptr;

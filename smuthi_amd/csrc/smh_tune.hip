@@ -6,12 +6,16 @@
 // balance between the tail of the launch (few large items leave CUs idle in
 // the last round) and per-item overhead (every item loads its entries' taps
 // and writes its own partial sums, which the reduction reads back). Where the
-// items go in the launch order sets L2 reuse: sorted by table row and dealt
-// to the XCDs in contiguous shares, the CUs of one XCD run neighbouring bins
-// of different chunks together and share tap rows in that XCD's L2.
+// items go in the launch order sets cache reuse: sorted by table row and
+// dealt to the XCDs in contiguous shares, the CUs of one XCD run neighbouring
+// bins of different chunks together and share tap rows in that XCD's L2; cut
+// into bands of table rows and launched band after band, every XCD reads the
+// same band's rows at a time and they share it through the Infinity Cache
+// (volume tables, whose rows far exceed the L2s, and whose receiver chunks
+// each hold about one pair per bin).
 //
 // smh_op_autotune regroups the operator's entries for each candidate (items
-// per CU, XCD-dealt order or not), times the matvec kernel of the requested
+// per CU, launch order), times the matvec kernel of the requested
 // shape on a synthetic input and keeps the fastest grouping. The best choice
 // depends on the pair distribution (ensemble size and density, table
 // resolution) and on the kernel (its LDS footprint decides how many
@@ -36,11 +40,11 @@ std::vector<smh_op::Binning*> tuned_binnings(smh_op* op, bool batch, int pi) {
 
 // Regroup every binning and grow the partial-result buffer when the new
 // grouping needs more than it holds.
-int regroup(smh_op* op, const std::vector<smh_op::Binning*>& bins, int per_cu, bool xcd) {
+int regroup(smh_op* op, const std::vector<smh_op::Binning*>& bins, int per_cu, int order_mode) {
   smh_ctx* ctx = op->ctx;
   size_t need = 0;
   for (smh_op::Binning* bn : bins) {
-    if (int rc = group_work_items(op, *bn, per_cu, xcd)) return rc;
+    if (int rc = group_work_items(op, *bn, per_cu, order_mode)) return rc;
     need = std::max(need, (size_t)bn->n_items * bn->part_bytes);
   }
   if (need > op->ypart_bytes) {
@@ -57,18 +61,21 @@ int regroup(smh_op* op, const std::vector<smh_op::Binning*>& bins, int per_cu, b
 }  // namespace
 
 extern "C" int smh_op_autotune(smh_op* op, int batch, int precision, int iters, int n_cand, const int* items_per_cu,
-                               const int* xcd_order, double* kernel_ms, int* best) {
+                               const int* launch_order, double* kernel_ms, int* best) {
   if (!op) return SMH_ERR_ARG;
   smh_ctx* ctx = op->ctx;
   SMH_CHECK(ctx, op->kind == 0 || op->kind == 1, SMH_ERR_UNSUPPORTED, "the autotuner covers lookup operators");
   SMH_CHECK(ctx, op->has_table, SMH_ERR_STATE, "coupling table not set");
-  SMH_CHECK(ctx, iters >= 1 && n_cand >= 1 && n_cand <= 64 && items_per_cu && xcd_order && kernel_ms, SMH_ERR_ARG,
+  SMH_CHECK(ctx, iters >= 1 && n_cand >= 1 && n_cand <= 64 && items_per_cu && launch_order && kernel_ms, SMH_ERR_ARG,
             "invalid autotune arguments");
   SMH_CHECK(ctx, precision == SMH_FP32 || precision == SMH_FP64 || (batch && precision == SMH_FP16), SMH_ERR_ARG,
             "invalid precision %d for %s matvecs", precision, batch ? "batched" : "single-vector");
-  for (int c = 0; c < n_cand; ++c)
+  for (int c = 0; c < n_cand; ++c) {
     SMH_CHECK(ctx, items_per_cu[c] >= 1 && items_per_cu[c] <= 1024, SMH_ERR_ARG, "items per CU %d out of [1, 1024]",
               items_per_cu[c]);
+    SMH_CHECK(ctx, launch_order[c] >= 0 && launch_order[c] <= 4096, SMH_ERR_ARG, "launch order %d out of [0, 4096]",
+              launch_order[c]);
+  }
   const int pi = precision == SMH_FP64 ? 1 : 0;
   if (batch)
     SMH_CHECK(ctx, op_has_batch(op, precision), SMH_ERR_UNSUPPORTED, "no batched matvec for this operator");
@@ -81,7 +88,7 @@ extern "C" int smh_op_autotune(smh_op* op, int batch, int precision, int iters, 
   smh::solver_drop_graphs(op);
   int pick = -1;
   for (int c = 0; c < n_cand; ++c) {
-    if (int rc = regroup(op, bins, items_per_cu[c], xcd_order[c] != 0)) return rc;
+    if (int rc = regroup(op, bins, items_per_cu[c], launch_order[c])) return rc;
     double kms = 0;
     const int rc = batch ? smh_op_bench_matvec_batch(op, precision, iters, &kms, nullptr)
                          : smh_op_bench_matvec(op, precision, iters, &kms, nullptr);
@@ -89,7 +96,7 @@ extern "C" int smh_op_autotune(smh_op* op, int batch, int precision, int iters, 
     kernel_ms[c] = kms;
     if (pick < 0 || kms < kernel_ms[pick]) pick = c;
   }
-  if (int rc = regroup(op, bins, items_per_cu[pick], xcd_order[pick] != 0)) return rc;
+  if (int rc = regroup(op, bins, items_per_cu[pick], launch_order[pick])) return rc;
   if (best) *best = pick;
   return SMH_OK;
 }

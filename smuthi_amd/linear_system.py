@@ -123,17 +123,20 @@ class _GpuCouplingOperator(SystemMatrix):
                    cu.context(), 'smh_op_bench_matvec_batch')
         return k.value, t.value
 
-    # launch shapes the autotuner tries: work items per CU x XCD-dealt launch order
-    AUTOTUNE_CANDIDATES = tuple((n, x) for n in (4, 8, 16, 32, 64) for x in (1, 0))
+    # launch shapes the autotuner tries: (work items per CU, launch order), the order 0 (chunk order),
+    # 1 (XCD-dealt by table row) or n >= 2 (n bands of table rows, launched band after band)
+    AUTOTUNE_CANDIDATES = (tuple((n, x) for n in (4, 8, 16, 32, 64) for x in (1, 0))
+                           + tuple((n, b) for n in (16, 64) for b in (32, 128, 512)))
 
     def autotune(self, batch=True, precision='half', iters=3, candidates=None):
         """Time the matvec kernel of one shape (batch: the 8-RHS kernel in 'half' / 'single' / 'double';
         else the single-vector kernel in 'single' / 'double') over launch shapes (work items per CU,
-        XCD-dealt order) and keep the fastest (smh_op_autotune). Returns {'best': (items_per_cu,
-        xcd_order), 'kernel_ms': {(items_per_cu, xcd_order): ms}}."""
+        launch order: 0 chunk order, 1 XCD-dealt by table row, n >= 2 row bands) and keep the fastest
+        (smh_op_autotune). Returns {'best': (items_per_cu, order), 'kernel_ms': {(items_per_cu, order):
+        ms}}."""
         cand = list(self.AUTOTUNE_CANDIDATES if candidates is None else candidates)
         per_cu = np.ascontiguousarray([c[0] for c in cand], dtype=np.int32)
-        xcd = np.ascontiguousarray([int(bool(c[1])) for c in cand], dtype=np.int32)
+        xcd = np.ascontiguousarray([int(c[1]) for c in cand], dtype=np.int32)
         ms = np.zeros(len(cand))
         best = ctypes.c_int(-1)
         _lib.check(_lib.load().smh_op_autotune(self._op, int(bool(batch)), _precision_code(precision), int(iters),

@@ -24,7 +24,7 @@ def gpu():
     return cuda
 
 
-CANDIDATES = [(2, 1), (16, 1), (16, 0), (64, 1)]
+CANDIDATES = [(2, 1), (16, 1), (16, 0), (64, 1), (16, 8), (4, 64)]
 
 
 def _check_tune(res):
@@ -99,11 +99,17 @@ def test_volume_autotune_keeps_results(gpu):
     V = rng.standard_normal((8, ref.n_unk)) + 1j * rng.standard_normal((8, ref.n_unk))
     want = np.stack([ref.matvec(v) for v in V])
     scale = np.linalg.norm(want, axis=1)
-    for shape in ((True, 'single'), (True, 'double'), (False, 'single')):
+    for shape in ((True, 'single'), (True, 'double'), (False, 'single'), (True, 'half')):
         res = op.autotune(batch=shape[0], precision=shape[1], iters=2, candidates=CANDIDATES)
         print('volume autotune', shape, res['best'], flush=True)
         _check_tune(res)
         b64 = np.max(np.linalg.norm(op.matvec(V, 'double') - want, axis=1) / scale)
         b32 = np.max(np.linalg.norm(op.matvec(V, 'single') - want, axis=1) / scale)
+        b16 = np.max(np.linalg.norm(op.matvec(V, 'half') - want, axis=1) / scale)
         e32 = np.linalg.norm(op.matvec(V[0], 'single') - want[0]) / scale[0]
-        assert b64 < 1e-12 and b32 < 3e-6 and e32 < 3e-6, (b64, b32, e32)
+        assert b64 < 1e-12 and b32 < 3e-6 and e32 < 3e-6 and b16 < 3e-3, (b64, b32, e32, b16)
+    # every candidate in turn for the FP64 8-RHS kernel: results stay put
+    for c in CANDIDATES:
+        op.autotune(batch=True, precision='double', iters=1, candidates=[c])
+        b64 = np.max(np.linalg.norm(op.matvec(V, 'double') - want, axis=1) / scale)
+        assert b64 < 1e-12, c

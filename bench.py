@@ -424,10 +424,10 @@ def autotune(op, shapes, enabled=True):
         r = op.autotune(batch=batch, precision=prec, iters=2)
         out['%s_%s' % ('8rhs' if batch else '1rhs', prec)] = {
             'best': {'items_per_cu': r['best'][0], 'launch_order': _order_name(r['best'][1])},
-            'kernel_ms': {'%d/%s' % (k[0], _order_name(k[1])): round(v, 4) for k, v in r['kernel_ms'].items()}}
+            'matvec_ms': {'%d/%s' % (k[0], _order_name(k[1])): round(v, 4) for k, v in r['matvec_ms'].items()}}
         log('autotune %s %s: best %s, %.3f ms (default 16/xcd %.3f ms)'
-            % ('8-RHS' if batch else '1-RHS', prec, r['best'], r['kernel_ms'][r['best']],
-               r['kernel_ms'].get((16, 1), float('nan'))))
+            % ('8-RHS' if batch else '1-RHS', prec, r['best'], r['matvec_ms'][r['best']],
+               r['matvec_ms'].get((16, 1), float('nan'))))
     out['seconds'] = time.time() - t0
     return out
 
@@ -756,7 +756,7 @@ def main_batching(args):
     log('right-hand sides %.1f s' % (time.time() - t0))
     op = batch.operator
     inner = 'half' if args.precision == 'mixed16' else 'single'
-    tune = autotune(op, ((True, inner),), not args.no_autotune)
+    tune = autotune(op, ((True, inner), (True, 'double')), not args.no_autotune)
     k16 = op.bench_matvec_batch('half', iters=2)[0] if args.precision == 'mixed16' else None
     k32, tot32 = op.bench_matvec_batch('single', iters=2)
     k64, tot64 = op.bench_matvec_batch('double', iters=2)
@@ -903,7 +903,7 @@ def main_learned(args):
     log('validation (%d spheres, 4 plane waves): %s in %.1f s' % (args.val_size, selection, time.time() - t0))
     del val
     test = make([args.test_size], 900)
-    tune = autotune(test.operator, ((True, 'half' if args.precision == 'mixed16' else 'single'),),
+    tune = autotune(test.operator, ((True, 'half' if args.precision == 'mixed16' else 'single'), (True, 'double')),
                     not args.no_autotune)
     _, feat_t = prepare(lp, test)
     rhs = test.right_hand_sides(fields)

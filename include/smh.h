@@ -276,15 +276,16 @@ int smh_op_bench_matvec_batch(smh_op *op, int precision, int iters, double *kern
  * to the XCDs in contiguous shares, n in [2, 4096] items also cut at the
  * boundaries of n equal bands of table rows and launched band after band;
  * times `iters` matvecs of the given shape (batch != 0: the 8-RHS kernel in
- * SMH_FP16 / FP32 / FP64; else the single-vector kernel in FP32 / FP64) into
- * kernel_ms[c], and keeps the fastest grouping (*best = its index). The FP16
+ * SMH_FP16 / FP32 / FP64; else the single-vector kernel in FP32 / FP64; the
+ * kernels and the reduction of their partial sums) into matvec_ms[c] (ms per
+ * matvec), and keeps the fastest grouping (*best = its index). The FP16
  * and FP32 8-RHS kernels share one grouping. Results stay within each
  * kernel's accuracy (the grouping changes summation order and fixed-point
  * scales only). Drops the solver's captured graphs. No counterpart in
  * smuthi, whose CUDA kernels launch one thread per matrix row
  * (linear_system_cuda.py:26-405, cuda_blocksize). */
 int smh_op_autotune(smh_op *op, int batch, int precision, int iters, int n_cand, const int *items_per_cu,
-                    const int *launch_order, double *kernel_ms, int *best);
+                    const int *launch_order, double *matvec_ms, int *best);
 
 /* Interpolation of the FP16 operator used by the mixed16 inner solves (and
  * smh_op_matvec with SMH_FP16): *terms = 2 when it interpolates linearly

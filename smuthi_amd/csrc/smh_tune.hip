@@ -15,8 +15,9 @@
 // each hold about one pair per bin).
 //
 // smh_op_autotune regroups the operator's entries for each candidate (items
-// per CU, launch order), times the matvec kernel of the requested
-// shape on a synthetic input and keeps the fastest grouping. The best choice
+// per CU, launch order), times the matvec of the requested shape (its kernels
+// and the reduction of their partial sums, whose size grows with the number
+// of items) on a synthetic input and keeps the fastest grouping. The best choice
 // depends on the pair distribution (ensemble size and density, table
 // resolution) and on the kernel (its LDS footprint decides how many
 // workgroups share a CU).
@@ -61,12 +62,12 @@ int regroup(smh_op* op, const std::vector<smh_op::Binning*>& bins, int per_cu, i
 }  // namespace
 
 extern "C" int smh_op_autotune(smh_op* op, int batch, int precision, int iters, int n_cand, const int* items_per_cu,
-                               const int* launch_order, double* kernel_ms, int* best) {
+                               const int* launch_order, double* matvec_ms, int* best) {
   if (!op) return SMH_ERR_ARG;
   smh_ctx* ctx = op->ctx;
   SMH_CHECK(ctx, op->kind == 0 || op->kind == 1, SMH_ERR_UNSUPPORTED, "the autotuner covers lookup operators");
   SMH_CHECK(ctx, op->has_table, SMH_ERR_STATE, "coupling table not set");
-  SMH_CHECK(ctx, iters >= 1 && n_cand >= 1 && n_cand <= 64 && items_per_cu && launch_order && kernel_ms, SMH_ERR_ARG,
+  SMH_CHECK(ctx, iters >= 1 && n_cand >= 1 && n_cand <= 64 && items_per_cu && launch_order && matvec_ms, SMH_ERR_ARG,
             "invalid autotune arguments");
   SMH_CHECK(ctx, precision == SMH_FP32 || precision == SMH_FP64 || (batch && precision == SMH_FP16), SMH_ERR_ARG,
             "invalid precision %d for %s matvecs", precision, batch ? "batched" : "single-vector");
@@ -89,12 +90,12 @@ extern "C" int smh_op_autotune(smh_op* op, int batch, int precision, int iters, 
   int pick = -1;
   for (int c = 0; c < n_cand; ++c) {
     if (int rc = regroup(op, bins, items_per_cu[c], launch_order[c])) return rc;
-    double kms = 0;
-    const int rc = batch ? smh_op_bench_matvec_batch(op, precision, iters, &kms, nullptr)
-                         : smh_op_bench_matvec(op, precision, iters, &kms, nullptr);
+    double kms = 0, tms = 0;
+    const int rc = batch ? smh_op_bench_matvec_batch(op, precision, iters, &kms, &tms)
+                         : smh_op_bench_matvec(op, precision, iters, &kms, &tms);
     if (rc) return rc;
-    kernel_ms[c] = kms;
-    if (pick < 0 || kms < kernel_ms[pick]) pick = c;
+    matvec_ms[c] = tms;
+    if (pick < 0 || tms < matvec_ms[pick]) pick = c;
   }
   if (int rc = regroup(op, bins, items_per_cu[pick], launch_order[pick])) return rc;
   if (best) *best = pick;

@@ -129,11 +129,11 @@ class _GpuCouplingOperator(SystemMatrix):
                            + tuple((n, b) for n in (16, 64) for b in (32, 128, 512)))
 
     def autotune(self, batch=True, precision='half', iters=3, candidates=None):
-        """Time the matvec kernel of one shape (batch: the 8-RHS kernel in 'half' / 'single' / 'double';
-        else the single-vector kernel in 'single' / 'double') over launch shapes (work items per CU,
-        launch order: 0 chunk order, 1 XCD-dealt by table row, n >= 2 row bands) and keep the fastest
-        (smh_op_autotune). Returns {'best': (items_per_cu, order), 'kernel_ms': {(items_per_cu, order):
-        ms}}."""
+        """Time the matvec of one shape (batch: the 8-RHS kernel in 'half' / 'single' / 'double'; else
+        the single-vector kernel in 'single' / 'double'; kernels and the reduction of their partial sums)
+        over launch shapes (work items per CU, launch order: 0 chunk order, 1 XCD-dealt by table row,
+        n >= 2 row bands) and keep the fastest (smh_op_autotune). Returns {'best': (items_per_cu, order),
+        'matvec_ms': {(items_per_cu, order): ms per matvec}}."""
         cand = list(self.AUTOTUNE_CANDIDATES if candidates is None else candidates)
         per_cu = np.ascontiguousarray([c[0] for c in cand], dtype=np.int32)
         xcd = np.ascontiguousarray([int(c[1]) for c in cand], dtype=np.int32)
@@ -142,7 +142,7 @@ class _GpuCouplingOperator(SystemMatrix):
         _lib.check(_lib.load().smh_op_autotune(self._op, int(bool(batch)), _precision_code(precision), int(iters),
                                                len(cand), _lib.iptr(per_cu), _lib.iptr(xcd), _lib.dptr(ms),
                                                ctypes.byref(best)), cu.context(), 'smh_op_autotune')
-        return dict(best=tuple(cand[best.value]), kernel_ms={tuple(c): float(m) for c, m in zip(cand, ms)})
+        return dict(best=tuple(cand[best.value]), matvec_ms={tuple(c): float(m) for c, m in zip(cand, ms)})
 
     def half_interpolation(self):
         """(Horner terms, measured linear-interpolation error / bin scale) of the FP16 operator of the

@@ -28,9 +28,9 @@ CANDIDATES = [(2, 1), (16, 1), (16, 0), (64, 1), (16, 8), (4, 64)]
 
 
 def _check_tune(res):
-    assert set(res['kernel_ms']) == set(CANDIDATES)
-    assert all(v > 0 for v in res['kernel_ms'].values())
-    assert res['kernel_ms'][res['best']] == min(res['kernel_ms'].values())
+    assert set(res['matvec_ms']) == set(CANDIDATES)
+    assert all(v > 0 for v in res['matvec_ms'].values())
+    assert res['matvec_ms'][res['best']] == min(res['matvec_ms'].values())
 
 
 def test_radial_autotune_keeps_results(gpu):
@@ -53,7 +53,7 @@ def test_radial_autotune_keeps_results(gpu):
 
     for shape in ((False, 'double'), (False, 'single'), (True, 'half'), (True, 'double')):
         res = op.autotune(batch=shape[0], precision=shape[1], iters=2, candidates=CANDIDATES)
-        print('autotune', shape, {k: round(v, 4) for k, v in res['kernel_ms'].items()}, 'best', res['best'],
+        print('autotune', shape, {k: round(v, 4) for k, v in res['matvec_ms'].items()}, 'best', res['best'],
               flush=True)
         _check_tune(res)
         e64, e32, b64, b32, b16 = errors()

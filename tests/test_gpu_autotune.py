@@ -24,7 +24,9 @@ def gpu():
     return cuda
 
 
-CANDIDATES = [(2, 1), (16, 1), (16, 0), (64, 1), (16, 8), (4, 64)]
+# (items per CU, launch order): 1 XCD-dealt, 0 chunk order, n >= 2 row bands (4096: about one
+# band per table row, every entry its own item)
+CANDIDATES = [(2, 1), (16, 1), (16, 0), (64, 1), (16, 8), (4, 64), (2, 4096)]
 
 
 def _check_tune(res):
@@ -65,6 +67,12 @@ def test_radial_autotune_keeps_results(gpu):
     for c in CANDIDATES:
         op.autotune(batch=False, precision='double', iters=1, candidates=[c])
         assert np.linalg.norm(op.matvec(V[1], 'double') - want[1]) / scale[1] < 1e-12, c
+    # launch orders outside [0, 4096] and item counts outside [1, 1024] are rejected
+    from smuthi_amd._lib import SmhError
+    for bad in ((16, 4097), (16, -1), (0, 1), (1025, 1)):
+        with pytest.raises(SmhError):
+            op.autotune(batch=True, precision='double', iters=1, candidates=[bad])
+    assert np.linalg.norm(op.matvec(V[1], 'double') - want[1]) / scale[1] < 1e-12
     # the solver's graphs were dropped with the old item arrays: a solve still matches LU
     t = 0.05 + 0.02j + 0.01 * rng.standard_normal(ref.n_unk)  # a well-conditioned M (GMRES(20) converges)
     # the dense W from FP64 matvecs of the unit vectors (checked against the oracle above)

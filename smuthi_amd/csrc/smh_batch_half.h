@@ -87,7 +87,8 @@ __host__ __device__ constexpr int half_tile_parts(int T, int B) {
 // volume lookup), each row divided by its own power of two hscale[row]; an
 // entry's T taps of z slice a.vslice sit at rows en.x + vol_tap_offset, and a
 // pair weights them with vol_tap_weight(w, w2) hscale[row] / (the largest of
-// the entry's four scales), which then scales the result.
+// the entry's four scales), which then scales the result. The weights go to
+// the input side (one MFMA per tap), so the taps are summed in FP32.
 template <int LMAX, int MMAX, int T, bool VOL = false>
 __global__ void __launch_bounds__(kHalfThreads) batch_half_kernel(RadialKArgs a, const float* __restrict__ hscale) {
   using Sh = SlotShape<LMAX, MMAX, T>;
@@ -301,50 +302,55 @@ __global__ void __launch_bounds__(kHalfThreads) batch_half_kernel(RadialKArgs a,
 
       const _Float16 wh = (_Float16)rec.w;
       const half2_t w2 = {wh, wh};
-      half2_t wt[T];  // VOL: the pair's tap weights relative to the entry's scale
+      half4_t wt[T];  // VOL: the pair's tap weights relative to the entry's scale
       if constexpr (VOL) {
 #pragma unroll
         for (int t = 0; t < T; ++t) {
           const _Float16 h = (_Float16)(vol_tap_weight<float>(a.vmode, a.vslice, t, rec.w, rec.w2) * rel[t]);
-          wt[t] = half2_t{h, h};
+          wt[t] = half4_t{h, h, h, h};
         }
       }
       typename Mfma<float>::v4 acc[MHP];
 #pragma unroll
       for (int ti = 0; ti < MHP; ++ti) acc[ti] = typename Mfma<float>::v4{0, 0, 0, 0};
-      // interpolated coefficients of part tile i, chunk c (Horner in packed FP16; VOL: the weighted taps)
+      // interpolated coefficients of part tile i, chunk c (Horner in packed FP16)
       auto coef = [&](int i, int c) {
-        half2_t v0, v1;
-        if constexpr (VOL) {
-          v0 = wt[0] * tv[0][i][c][0];
-          v1 = wt[0] * tv[0][i][c][1];
+        half2_t v0 = tv[T - 1][i][c][0], v1 = tv[T - 1][i][c][1];
 #pragma unroll
-          for (int t = 1; t < T; ++t) {
-            v0 = wt[t] * tv[t][i][c][0] + v0;
-            v1 = wt[t] * tv[t][i][c][1] + v1;
-          }
-        } else {
-          v0 = tv[T - 1][i][c][0];
-          v1 = tv[T - 1][i][c][1];
-#pragma unroll
-          for (int t = T - 2; t >= 0; --t) {
-            v0 = v0 * w2 + tv[t][i][c][0];
-            v1 = v1 * w2 + tv[t][i][c][1];
-          }
+        for (int t = T - 2; t >= 0; --t) {
+          v0 = v0 * w2 + tv[t][i][c][0];
+          v1 = v1 * w2 + tv[t][i][c][1];
         }
         return half4_t{v0[0], v0[1], v1[0], v1[1]};
       };
       // one 16x16x32 MFMA (the full-rate FP16 form) per row tile and chunk:
       // the lane's 8 K values are its 4 slots of chunk c from Re L (times zb)
       // and the same 4 from Im L (times zb2), so the column sums are the
-      // complex product's re or im part directly
+      // complex product's re or im part directly. VOL: one MFMA per tap, the
+      // tap's weight applied to the input side (zb, zb2) and the taps summed
+      // by the MFMA in FP32, so the weighted sum of the taps (Catmull-Rom
+      // weights of both signs) is not rounded to FP16
 #pragma unroll
       for (int c = 0; c < NC; ++c) {
-        const half8_t bv = __builtin_shufflevector(zb[c], zb2[c], 0, 1, 2, 3, 4, 5, 6, 7);
+        if constexpr (VOL) {
 #pragma unroll
-        for (int ti = 0; ti < MHP; ++ti) {
-          const half8_t av = __builtin_shufflevector(coef(ti, c), coef(ti + MHP, c), 0, 1, 2, 3, 4, 5, 6, 7);
-          acc[ti] = __builtin_amdgcn_mfma_f32_16x16x32_f16(av, bv, acc[ti], 0, 0, 0);
+          for (int t = 0; t < T; ++t) {
+            const half8_t bv = __builtin_shufflevector(zb[c] * wt[t], zb2[c] * wt[t], 0, 1, 2, 3, 4, 5, 6, 7);
+#pragma unroll
+            for (int ti = 0; ti < MHP; ++ti) {
+              const half4_t re = __builtin_shufflevector(tv[t][ti][c][0], tv[t][ti][c][1], 0, 1, 2, 3);
+              const half4_t im = __builtin_shufflevector(tv[t][ti + MHP][c][0], tv[t][ti + MHP][c][1], 0, 1, 2, 3);
+              const half8_t av = __builtin_shufflevector(re, im, 0, 1, 2, 3, 4, 5, 6, 7);
+              acc[ti] = __builtin_amdgcn_mfma_f32_16x16x32_f16(av, bv, acc[ti], 0, 0, 0);
+            }
+          }
+        } else {
+          const half8_t bv = __builtin_shufflevector(zb[c], zb2[c], 0, 1, 2, 3, 4, 5, 6, 7);
+#pragma unroll
+          for (int ti = 0; ti < MHP; ++ti) {
+            const half8_t av = __builtin_shufflevector(coef(ti, c), coef(ti + MHP, c), 0, 1, 2, 3, 4, 5, 6, 7);
+            acc[ti] = __builtin_amdgcn_mfma_f32_16x16x32_f16(av, bv, acc[ti], 0, 0, 0);
+          }
         }
       }
 
